@@ -1,0 +1,310 @@
+"""geogram_amd — MI355X-native meshless Voronoi / Laguerre cells.
+
+Python mirror of geogram's entry points for the meshless cell path, over the
+C ABI of ``libvgc.so`` (include/vgc.h):
+
+* ``NearestNeighborSearch`` — points/nn_search.h (set_points,
+  get_nearest_neighbors), batched over all points;
+* ``RestrictedVoronoiDiagram.compute_centroids`` — voronoi/RVD.h:284-290
+  (caller-zeroed mg / m accumulated, RVD.cpp:500-528), volumetric, box domain;
+* ``CentroidalVoronoiTesselation`` — voronoi/CVT.h (set_points,
+  Lloyd_iterations, embedding), security radius on;
+* ``cells`` — per-cell volume, centroid, neighbour CSR and facet areas, the
+  outputs of VBW::ConvexCell (voronoi/convex_cell.h);
+* ``OptimalTransportMap3d`` — the semi-discrete optimal-transport solver
+  called by examples/exploragram/compute_OTM/main.cpp:283-297.
+
+Arrays may be numpy (host) or torch CUDA tensors (device-resident, no PCIe).
+Everything runs on the GPU; without ``libvgc.so`` or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Stats, VgcError, VGC_DEVICE_PTRS, VGC_NO_NEIGHBORS, VGC_OK, VGC_PARTIAL
+
+UNIT_BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+
+__all__ = [
+    "Context", "CellResult", "NearestNeighborSearch", "RestrictedVoronoiDiagram",
+    "CentroidalVoronoiTesselation", "default_context", "cells", "knn", "lloyd",
+    "random_float64", "sample_points", "Stats", "VgcError", "UNIT_BOX",
+]
+
+
+def _is_torch_cuda(a):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_torch_cuda(a):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class CellResult:
+    """Per-seed outputs in the caller's numbering.
+
+    vol[n]; mg[n,3] (integral of x, so centroid = mg / vol); nbr_ptr[n+1];
+    nbr[nnz] (neighbour seed ids); farea[nnz] (facet areas); stats (dict)."""
+
+    def __init__(self, vol, mg, nbr_ptr, nbr, farea, stats, status):
+        self.vol, self.mg, self.nbr_ptr, self.nbr, self.farea = vol, mg, nbr_ptr, nbr, farea
+        self.stats = stats
+        self.status = status
+
+    @property
+    def centroid(self):
+        return self.mg / self.vol[:, None]
+
+    def neighbors(self, i):
+        return self.nbr[self.nbr_ptr[i]:self.nbr_ptr[i + 1]]
+
+    def facet_areas(self, i):
+        return self.farea[self.nbr_ptr[i]:self.nbr_ptr[i + 1]]
+
+
+class Context:
+    """A device context (workspace reused across calls)."""
+
+    def __init__(self, device: int = 0, lib=None):
+        self.lib = lib if lib is not None else _lib.gpu()
+        self.device = device
+        self.ctx = self.lib.create(device)
+
+    @classmethod
+    def emulated(cls):
+        """CPU emulation of the kernels (test-suite only)."""
+        return cls(0, _lib.emu())
+
+    def close(self):
+        if self.ctx:
+            self.lib.destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow_partial=True):
+        if rc == VGC_OK or (rc == VGC_PARTIAL and allow_partial):
+            return rc
+        raise VgcError(rc, self.lib.last_error(self.ctx))
+
+    # ---------------------------------------------------------------- cells
+    def cells(self, points, weights=None, box=UNIT_BOX, k0=64, neighbors=True,
+              inst_n=None, boxes=None):
+        """Voronoi (weights None) or Laguerre cells clipped to box.
+
+        For a batch of instances give inst_n (seed counts, consecutive in
+        points) and boxes (6 per instance)."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            w = None if weights is None else weights.contiguous().to(torch.float64)
+            n = pts.shape[0]
+            vol = torch.empty(n, dtype=torch.float64, device=pts.device)
+            mg = torch.empty((n, 3), dtype=torch.float64, device=pts.device)
+            ptr = torch.empty(n + 1, dtype=torch.int64, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+            n = pts.shape[0]
+            vol = np.empty(n)
+            mg = np.empty((n, 3))
+            ptr = np.empty(n + 1, dtype=np.uint64)
+        flags = VGC_DEVICE_PTRS if dev else 0
+        st = Stats()
+        if inst_n is None:
+            boxa = np.ascontiguousarray(box, dtype=np.float64)
+            call = lambda nb, fa, cap: self.lib.fn("cells")(
+                self.ctx, _ptr(pts), _ptr(w), n, boxa.ctypes.data_as(ctypes.c_void_p), int(k0),
+                flags | (0 if neighbors else VGC_NO_NEIGHBORS), _ptr(vol), _ptr(mg), _ptr(ptr),
+                nb, fa, cap, ctypes.byref(st))
+        else:
+            inst = np.ascontiguousarray(inst_n, dtype=np.uint32)
+            boxa = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1)
+            call = lambda nb, fa, cap: self.lib.fn("cells_batch")(
+                self.ctx, _ptr(pts), _ptr(w), inst.ctypes.data_as(ctypes.c_void_p), len(inst),
+                boxa.ctypes.data_as(ctypes.c_void_p), int(k0),
+                flags | (0 if neighbors else VGC_NO_NEIGHBORS), _ptr(vol), _ptr(mg), _ptr(ptr),
+                nb, fa, cap, ctypes.byref(st))
+        if not neighbors:
+            rc = self._check(call(None, None, 0))
+            return CellResult(vol, mg, ptr, None, None, st.as_dict(), rc)
+        # capacity guess (about 16 neighbours per cell), retry on VGC_ERR_CAPACITY
+        cap = 24 * n + 64
+        for _ in range(2):
+            if dev:
+                import torch
+                nbr = torch.empty(cap, dtype=torch.int32, device=pts.device)
+                fa = torch.empty(cap, dtype=torch.float64, device=pts.device)
+            else:
+                nbr = np.empty(cap, dtype=np.int32)
+                fa = np.empty(cap)
+            rc = call(_ptr(nbr), _ptr(fa), cap)
+            if rc == _lib.VGC_ERR_CAPACITY:
+                cap = int(st.nb_neighbors)
+                continue
+            self._check(rc)
+            nnz = int(st.nb_neighbors)
+            return CellResult(vol, mg, ptr, nbr[:nnz], fa[:nnz], st.as_dict(), rc)
+        raise VgcError(rc, self.lib.last_error(self.ctx))
+
+    # ------------------------------------------------------------------ kNN
+    def knn(self, points, k):
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            n = pts.shape[0]
+            idx = torch.empty((n, k), dtype=torch.int32, device=pts.device)
+            d2 = torch.empty((n, k), dtype=torch.float32, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            n = pts.shape[0]
+            idx = np.empty((n, k), dtype=np.int32)
+            d2 = np.empty((n, k), dtype=np.float32)
+        rc = self.lib.fn("knn")(self.ctx, _ptr(pts), n, int(k), VGC_DEVICE_PTRS if dev else 0,
+                                _ptr(idx), _ptr(d2))
+        self._check(rc, allow_partial=False)
+        return idx, d2
+
+    # ---------------------------------------------------------------- Lloyd
+    def lloyd(self, points, iters, box=UNIT_BOX, k0=64):
+        """iters Lloyd iterations; returns the new points (same kind of array)."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            x = points.detach().clone().to(torch.float64).contiguous().reshape(-1, 3)
+        else:
+            x = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        st = Stats()
+        rc = self.lib.fn("lloyd")(self.ctx, _ptr(x), x.shape[0], boxa.ctypes.data_as(ctypes.c_void_p),
+                                  int(iters), int(k0), VGC_DEVICE_PTRS if dev else 0,
+                                  ctypes.byref(st))
+        self._check(rc)
+        return x, st.as_dict()
+
+
+_default = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default:
+        _default[device] = Context(device)
+    return _default[device]
+
+
+def cells(points, weights=None, box=UNIT_BOX, k0=64, neighbors=True, device=0):
+    return default_context(device).cells(points, weights, box, k0, neighbors)
+
+
+def knn(points, k, device=0):
+    return default_context(device).knn(points, k)
+
+
+def lloyd(points, iters, box=UNIT_BOX, k0=64, device=0):
+    return default_context(device).lloyd(points, iters, box, k0)
+
+
+def random_float64(seed: int, count: int) -> np.ndarray:
+    """std::mt19937_64(seed) + uniform_real_distribution<double>(0,1) — the
+    stream of Numeric::random_reset / random_float64 (basic/numeric.cpp:76-103)."""
+    out = np.empty(count)
+    rc = _lib.gpu().cdll.vgc_random_float64(seed, out.ctypes.data_as(ctypes.c_void_p), count)
+    if rc != VGC_OK:
+        raise VgcError(rc, "vgc_random_float64")
+    return out
+
+
+def sample_points(kind: int, seed: int, n: int) -> np.ndarray:
+    """Synthetic seeds (0: uniform cube, 1: half uniform + 8-Gaussian mixture,
+    2: 16-Gaussian mixture), see include/vgc.h."""
+    out = np.empty((n, 3))
+    rc = _lib.gpu().cdll.vgc_sample_points(kind, seed, n, out.ctypes.data_as(ctypes.c_void_p))
+    if rc != VGC_OK:
+        raise VgcError(rc, "vgc_sample_points")
+    return out
+
+
+# ------------------------------------------------------- reference mirrors
+
+class NearestNeighborSearch:
+    """points/nn_search.h: set_points then nearest neighbours of every point
+    (self excluded, ascending distance, FP32 ranking), k in {16,24,32,48,64}."""
+
+    def __init__(self, dimension: int = 3, ctx: Context | None = None):
+        if dimension != 3:
+            raise ValueError("only dimension 3 is supported")
+        self.ctx = ctx
+        self.points = None
+
+    def set_points(self, points):
+        self.points = points
+
+    def get_nearest_neighbors(self, k: int):
+        ctx = self.ctx or default_context()
+        return ctx.knn(self.points, k)
+
+
+class RestrictedVoronoiDiagram:
+    """Volumetric restricted Voronoi diagram of seeds in a box domain
+    (voronoi/RVD.h).  compute_centroids accumulates into caller-zeroed arrays
+    like RVD_Nd_Impl::compute_centroids_in_volume (RVD.cpp:500-528)."""
+
+    def __init__(self, points, box=UNIT_BOX, ctx: Context | None = None, k0: int = 64):
+        self.points = points
+        self.box = box
+        self.ctx = ctx
+        self.k0 = k0
+        self.check_SR = True
+
+    def set_check_SR(self, x: bool):
+        if not x:
+            raise ValueError("the GPU path always applies the security radius")
+
+    def compute_centroids(self, mg, m):
+        ctx = self.ctx or default_context()
+        r = ctx.cells(self.points, None, self.box, self.k0, neighbors=False)
+        mg += r.mg.reshape(mg.shape)
+        m += r.vol
+        return r
+
+
+class CentroidalVoronoiTesselation:
+    """voronoi/CVT.h: set_points, Lloyd_iterations, embedding (box domain)."""
+
+    def __init__(self, box=UNIT_BOX, ctx: Context | None = None, k0: int = 64):
+        self.box = box
+        self.ctx = ctx
+        self.k0 = k0
+        self.points_ = None
+        self.last_stats = None
+
+    def set_points(self, points):
+        self.points_ = np.array(points, dtype=np.float64).reshape(-1, 3) \
+            if not _is_torch_cuda(points) else points.clone()
+
+    def nb_points(self):
+        return self.points_.shape[0]
+
+    def embedding(self, i):
+        return self.points_[i]
+
+    def Lloyd_iterations(self, nb_iter: int):
+        ctx = self.ctx or default_context()
+        self.points_, self.last_stats = ctx.lloyd(self.points_, nb_iter, self.box, self.k0)

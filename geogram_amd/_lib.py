@@ -1,0 +1,132 @@
+"""Loader of the C ABI (include/vgc.h).
+
+``libvgc.so`` is the gfx950 build and the only library the package uses.  If it
+is missing, or no GPU is visible, calls raise: there is no CPU fallback.
+``emu()`` loads ``libvgc_emu.so``, the CPU emulation of the same kernels, which
+only the test-suite uses.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvgc.so")
+EMU_PATH = os.path.join(_HERE, "libvgc_emu.so")
+
+VGC_OK = 0
+VGC_PARTIAL = 1
+VGC_ERR_ARG = -1
+VGC_ERR_DEVICE = -2
+VGC_ERR_CAPACITY = -3
+VGC_ERR_INTERNAL = -4
+
+VGC_DEVICE_PTRS = 1
+VGC_NO_NEIGHBORS = 2
+
+_STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_uncertain nb_failed nb_empty "
+             "nb_disconnected nb_clips nb_tests radius_entries").split()
+_STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in _STAT_U64] + [(k, ctypes.c_double) for k in _STAT_F64]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VgcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vgc error {code}: {msg}")
+        self.code = code
+
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+
+
+def _declare(L, prefix):
+    ctx_t = _vp
+    L[prefix + "last_error"].restype = ctypes.c_char_p
+    L[prefix + "last_error"].argtypes = [ctx_t]
+    f = L[prefix + "knn"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _u32, _u32, _vp, _vp]
+    f = L[prefix + "cells"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u64,
+                  ctypes.POINTER(Stats)]
+    f = L[prefix + "cells_batch"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u64,
+                  ctypes.POINTER(Stats)]
+    f = L[prefix + "lloyd"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
+
+
+class _Lib:
+    """Uniform view of libvgc (prefix vgc_) or libvgc_emu (prefix vgc_emu_)."""
+
+    def __init__(self, path, prefix):
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{os.path.basename(path)} is not built; run "
+                "`python -c 'import __graft_entry__; __graft_entry__.build()'`")
+        self.path = path
+        self.cdll = ctypes.CDLL(path)
+        self.prefix = prefix
+        self._f = {}
+        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "create", "destroy"]
+        for nm in names:
+            self._f[prefix + nm] = getattr(self.cdll, prefix + nm)
+        _declare(self._f, prefix)
+        if prefix == "vgc_":
+            self._f["vgc_create"].argtypes = [ctypes.POINTER(_vp), ctypes.c_int]
+            self.cdll.vgc_random_float64.argtypes = [_u64, _vp, _u64]
+            self.cdll.vgc_sample_points.argtypes = [_u32, _u64, _u32, _vp]
+        else:
+            self._f["vgc_emu_create"].argtypes = [ctypes.POINTER(_vp)]
+        self._f[prefix + "destroy"].argtypes = [_vp]
+        self._f[prefix + "destroy"].restype = None
+
+    def fn(self, name):
+        return self._f[self.prefix + name]
+
+    def create(self, device=0):
+        ctx = _vp()
+        if self.prefix == "vgc_":
+            rc = self.fn("create")(ctypes.byref(ctx), int(device))
+        else:
+            rc = self.fn("create")(ctypes.byref(ctx))
+        if rc != VGC_OK:
+            raise VgcError(rc, f"cannot create a context on device {device} "
+                               "(no GPU visible, or HIP runtime failure)")
+        return ctx
+
+    def destroy(self, ctx):
+        self.fn("destroy")(ctx)
+
+    def last_error(self, ctx):
+        s = self.fn("last_error")(ctx)
+        return s.decode() if s else ""
+
+
+_gpu = None
+_emu = None
+
+
+def gpu() -> _Lib:
+    global _gpu
+    if _gpu is None:
+        _gpu = _Lib(LIB_PATH, "vgc_")
+    return _gpu
+
+
+def emu() -> _Lib:
+    global _emu
+    if _emu is None:
+        _emu = _Lib(EMU_PATH, "vgc_emu_")
+    return _emu

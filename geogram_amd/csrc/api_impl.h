@@ -1,0 +1,332 @@
+// C-ABI implementation, written once against a backend: the gfx950 library
+// (libvgc.so) instantiates it with the HIP backend; the CPU emulation library
+// used by the tests instantiates it with the emulation backend.
+#pragma once
+
+#include <exception>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vgc.h"
+#include "pipeline.h"
+
+namespace vgc {
+
+template <class B>
+struct Ctx {
+    explicit Ctx(int dev) : be(dev), eng(be), device(dev) {}
+    ~Ctx() {
+        for (auto& s : stage)
+            if (s.p) be.dfree(s.p);
+    }
+    B be;
+    Engine<B> eng;
+    int device;
+    std::string err;
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<Buf> stage;
+    void* staging(int id, size_t bytes) {
+        if ((int)stage.size() <= id) stage.resize(id + 1);
+        Buf& b = stage[id];
+        if (b.bytes < bytes) {
+            if (b.p) be.dfree(b.p);
+            b.p = be.dalloc(bytes);
+            b.bytes = bytes;
+        }
+        return b.p;
+    }
+};
+
+inline void fill_stats(const EngineStats& e, vgc_stats* s) {
+    if (!s) return;
+    s->nb_cells = e.nb_cells;
+    s->nb_neighbors = e.nb_neighbors;
+    s->nb_pass_b = e.nb_pass_b;
+    s->nb_pass_c = e.nb_pass_c;
+    s->nb_uncertain = e.nb_uncertain;
+    s->nb_failed = e.nb_failed;
+    s->nb_empty = e.nb_empty;
+    s->nb_disconnected = e.nb_disconnected;
+    s->nb_clips = e.nb_clips;
+    s->nb_tests = e.nb_tests;
+    s->radius_entries = e.radius_entries;
+    s->t_build_ms = e.t_build_ms;
+    s->t_knn_ms = e.t_knn_ms;
+    s->t_clip_ms = e.t_clip_ms;
+    s->t_pass_bc_ms = e.t_pass_bc_ms;
+    s->t_compact_ms = e.t_compact_ms;
+    s->t_total_ms = e.t_total_ms;
+}
+
+inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
+
+template <class B>
+int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* inst_n,
+              uint32_t n_inst, const double* boxes, uint32_t k0, uint32_t flags, double* vol,
+              double* mg, uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
+              vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (n_inst == 0 || !inst_n || !boxes || !xyz) {
+            ctx->err = "vgc_cells: null argument";
+            return VGC_ERR_ARG;
+        }
+        if (k0 == 0) k0 = 32;
+        if (!valid_k0(k0)) {
+            ctx->err = "vgc_cells: k0 must be 16, 24, 32, 48 or 64";
+            return VGC_ERR_ARG;
+        }
+        long long n = 0;
+        Problem pb;
+        pb.n_inst = (int)n_inst;
+        pb.inst_n.resize(n_inst);
+        pb.box.assign(boxes, boxes + 6 * (size_t)n_inst);
+        for (uint32_t i = 0; i < n_inst; ++i) {
+            if (inst_n[i] == 0) {
+                ctx->err = "vgc_cells: empty instance";
+                return VGC_ERR_ARG;
+            }
+            pb.inst_n[i] = (int)inst_n[i];
+            n += inst_n[i];
+            const double* b = boxes + 6 * (size_t)i;
+            if (!(b[3] > b[0] && b[4] > b[1] && b[5] > b[2])) {
+                ctx->err = "vgc_cells: degenerate box";
+                return VGC_ERR_ARG;
+            }
+        }
+        if (n > (1LL << 30)) {
+            ctx->err = "vgc_cells: too many seeds";
+            return VGC_ERR_ARG;
+        }
+        pb.n = (int)n;
+        pb.k0 = (int)k0;
+        pb.want_neighbors = !(flags & VGC_NO_NEIGHBORS) && (nbr || farea);
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        B& be = ctx->be;
+        if (dev) {
+            pb.d_xyz = xyz;
+            pb.d_w = w;
+        } else {
+            double* dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+            pb.d_xyz = dx;
+            if (w) {
+                double* dw = (double*)ctx->staging(1, sizeof(double) * (size_t)n);
+                be.h2d(dw, w, sizeof(double) * (size_t)n);
+                pb.d_w = dw;
+            }
+        }
+        EngineStats es;
+        if (ctx->eng.cells(pb, es)) {
+            ctx->err = "vgc_cells: " + ctx->eng.error;
+            return VGC_ERR_INTERNAL;
+        }
+        Engine<B>& E = ctx->eng;
+        auto out = [&](void* dst, const void* src, size_t bytes) {
+            if (!dst) return;
+            if (dev)
+                be.d2d(dst, src, bytes);
+            else
+                be.d2h(dst, src, bytes);
+        };
+        out(vol, E.d_vol, sizeof(double) * (size_t)n);
+        out(mg, E.d_mg, sizeof(double) * 3 * (size_t)n);
+        out(nbr_ptr, E.d_ptr, sizeof(long long) * ((size_t)n + 1));
+        fill_stats(es, stats);
+        if (nbr || farea) {
+            if ((uint64_t)E.total_neighbors > cap) {
+                ctx->err = "vgc_cells: neighbour capacity too small";
+                return VGC_ERR_CAPACITY;
+            }
+            out(nbr, E.d_nbr, sizeof(int) * (size_t)E.total_neighbors);
+            out(farea, E.d_area, sizeof(double) * (size_t)E.total_neighbors);
+        }
+        be.sync();
+        if (es.nb_failed || es.nb_uncertain) {
+            ctx->err = "vgc_cells: " + std::to_string(es.nb_failed) + " cells failed, " +
+                       std::to_string(es.nb_uncertain) + " uncertain";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
+            float* d2) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !nbr || n < 2) {
+            ctx->err = "vgc_knn: bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (!valid_k0(k)) {
+            ctx->err = "vgc_knn: k must be 16, 24, 32, 48 or 64";
+            return VGC_ERR_ARG;
+        }
+        if (k > n - 1) {
+            ctx->err = "vgc_knn: k exceeds n-1";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        // bounding box of the points for the Morton quantisation
+        std::vector<double> hx;
+        const double* hp = xyz;
+        if (dev) {
+            hx.resize(3 * (size_t)n);
+            be.d2h(hx.data(), xyz, sizeof(double) * 3 * (size_t)n);
+            hp = hx.data();
+            pb.d_xyz = xyz;
+        } else {
+            double* dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+            pb.d_xyz = dx;
+        }
+        double lo[3] = {hp[0], hp[1], hp[2]}, hi[3] = {hp[0], hp[1], hp[2]};
+        for (uint32_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], hp[3 * (size_t)i + c]);
+                hi[c] = std::max(hi[c], hp[3 * (size_t)i + c]);
+            }
+        pb.box = {lo[0], lo[1], lo[2], hi[0] + 1e-300 + (hi[0] - lo[0]) * 1e-12,
+                  hi[1] + 1e-300 + (hi[1] - lo[1]) * 1e-12, hi[2] + 1e-300 + (hi[2] - lo[2]) * 1e-12};
+        for (int c = 0; c < 3; ++c)
+            if (!(pb.box[3 + c] > pb.box[c])) pb.box[3 + c] = pb.box[c] + 1.0;
+        EngineStats es;
+        Engine<B>& E = ctx->eng;
+        if (E.build(pb, es)) {
+            ctx->err = "vgc_knn: " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        int* d_list = nullptr;
+        float* d_d2 = (float*)ctx->staging(2, sizeof(float) * (size_t)n * k);
+        E.knn_lists((int)k, d_list, d_d2);
+        int* d_out = (int*)ctx->staging(3, sizeof(int) * (size_t)n * k);
+        float* d_outd = (float*)ctx->staging(4, sizeof(float) * (size_t)n * k);
+        be.knn_to_caller(E.d_orig, d_list, d_d2, (int)n, (int)k, d_out, d_outd);
+        if (dev) {
+            be.d2d(nbr, d_out, sizeof(int) * (size_t)n * k);
+            if (d2) be.d2d(d2, d_outd, sizeof(float) * (size_t)n * k);
+        } else {
+            be.d2h(nbr, d_out, sizeof(int) * (size_t)n * k);
+            if (d2) be.d2h(d2, d_outd, sizeof(float) * (size_t)n * k);
+        }
+        be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+              uint32_t k0, uint32_t flags, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !box || n < 1) {
+            ctx->err = "vgc_lloyd: bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (k0 == 0) k0 = 32;
+        if (!valid_k0(k0)) {
+            ctx->err = "vgc_lloyd: k0 must be 16, 24, 32, 48 or 64";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        pb.box.assign(box, box + 6);
+        pb.k0 = (int)k0;
+        double* dx = xyz;
+        if (!dev) {
+            dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+        }
+        EngineStats es;
+        if (ctx->eng.lloyd(pb, dx, (int)iters, es)) {
+            ctx->err = "vgc_lloyd: " + ctx->eng.error;
+            return VGC_ERR_INTERNAL;
+        }
+        if (!dev) be.d2h(xyz, dx, sizeof(double) * 3 * (size_t)n);
+        be.sync();
+        fill_stats(es, stats);
+        if (es.nb_failed || es.nb_uncertain) {
+            ctx->err = "vgc_lloyd: flagged cells";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+// ----------------------------------------------------- synthetic inputs
+inline int random_float64(uint64_t seed, double* out, uint64_t count) {
+    if (!out && count) return VGC_ERR_ARG;
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (uint64_t i = 0; i < count; ++i) out[i] = U(eng);
+    return VGC_OK;
+}
+
+inline int sample_points(uint32_t kind, uint64_t seed, uint32_t n, double* xyz) {
+    if (!xyz || kind > 2) return VGC_ERR_ARG;
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    uint32_t nu = kind == 0 ? n : (kind == 1 ? n / 2 : 0);
+    for (uint32_t i = 0; i < nu; ++i) {
+        xyz[3 * (size_t)i + 0] = U(eng);
+        xyz[3 * (size_t)i + 1] = U(eng);
+        xyz[3 * (size_t)i + 2] = U(eng);
+    }
+    if (nu == n) return VGC_OK;
+    const int nc = kind == 1 ? 8 : 16;
+    const double smin = kind == 1 ? 0.01 : 0.02, smax = kind == 1 ? 0.1 : 0.15;
+    std::vector<double> mean(3 * nc), sig(nc);
+    for (int c = 0; c < nc; ++c) {
+        for (int a = 0; a < 3; ++a) mean[3 * c + a] = 0.2 + 0.6 * U(eng);
+        sig[c] = smin + (smax - smin) * U(eng);
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint32_t i = nu; i < n; ++i) {
+        int c = (int)(U(eng) * nc);
+        if (c >= nc) c = nc - 1;
+        while (true) {
+            // Box-Muller: two pairs, the fourth normal is discarded
+            double u1 = 1.0 - U(eng), u2 = U(eng), u3 = 1.0 - U(eng), u4 = U(eng);
+            double r1 = std::sqrt(-2.0 * std::log(u1)), r2 = std::sqrt(-2.0 * std::log(u3));
+            double z0 = r1 * std::cos(two_pi * u2), z1 = r1 * std::sin(two_pi * u2);
+            double z2 = r2 * std::cos(two_pi * u4);
+            double x = mean[3 * c + 0] + sig[c] * z0;
+            double y = mean[3 * c + 1] + sig[c] * z1;
+            double z = mean[3 * c + 2] + sig[c] * z2;
+            if (x > 0.0 && x < 1.0 && y > 0.0 && y < 1.0 && z > 0.0 && z < 1.0) {
+                xyz[3 * (size_t)i + 0] = x;
+                xyz[3 * (size_t)i + 1] = y;
+                xyz[3 * (size_t)i + 2] = z;
+                break;
+            }
+        }
+    }
+    return VGC_OK;
+}
+
+}  // namespace vgc

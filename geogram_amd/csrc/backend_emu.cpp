@@ -1,0 +1,266 @@
+// CPU emulation of the gfx950 pipeline, for tests on machines without a GPU.
+// It runs the very kernels' code (clip_cell.h through the emulated wave layer,
+// the scalar kNN / tree / radius functions of common.h) and the same host
+// driver (pipeline.h).  It is a test library (libvgc_emu.so); the product
+// library libvgc.so is the HIP build and has no CPU path.
+#define VGC_EMU 1
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "api_impl.h"
+
+namespace vgc {
+
+struct EmuBackend {
+    explicit EmuBackend(int) {}
+
+    void* dalloc(size_t b) {
+        void* p = std::malloc(b ? b : 1);
+        if (!p) throw std::bad_alloc();
+        return p;
+    }
+    void dfree(void* p) { std::free(p); }
+    void h2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+    void d2h(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+    void d2d(void* d, const void* s, size_t b) { std::memmove(d, s, b); }
+    void dzero(void* d, size_t b) { std::memset(d, 0, b); }
+    void sync() {}
+
+    void fill_inst(int* inst_orig, int n, const int* p0, int ni) {
+        for (int i = 0; i < n; ++i) {
+            int lo = 0, hi = ni - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) / 2;
+                if (p0[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            inst_orig[i] = lo;
+        }
+    }
+
+    void morton(const double* xyz, const int* inst_orig, const double* box, int n,
+                uint64_t* keys, int* vals) {
+        for (int i = 0; i < n; ++i) {
+            int inst = inst_orig ? inst_orig[i] : 0;
+            keys[i] = morton_key(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2],
+                                 box + 6 * inst, (uint64_t)inst);
+            vals[i] = i;
+        }
+    }
+
+    // stable sort by key (as a radix sort is)
+    void sort_pairs(const uint64_t* keys, const int* vals, uint64_t* keys2, int* vals2, int n,
+                    int) {
+        std::vector<int> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+        for (int i = 0; i < n; ++i) {
+            keys2[i] = keys[idx[i]];
+            vals2[i] = vals[idx[i]];
+        }
+    }
+
+    void gather(const double* xyz, const double* w, const int* inst_orig, const int* perm, int n,
+                pt4* pts, pf4* pf, int* orig, int* inst_of) {
+        for (int s = 0; s < n; ++s) {
+            int i = perm[s];
+            pt4 p;
+            p.x = xyz[3 * (size_t)i];
+            p.y = xyz[3 * (size_t)i + 1];
+            p.z = xyz[3 * (size_t)i + 2];
+            p.w = w ? w[i] : 0.0;
+            pts[s] = p;
+            pf4 f;
+            f.x = float(p.x);
+            f.y = float(p.y);
+            f.z = float(p.z);
+            f.w = float(p.w);
+            pf[s] = f;
+            orig[s] = i;
+            if (inst_of) inst_of[s] = inst_orig ? inst_orig[i] : 0;
+        }
+    }
+
+    void bvh_leaves(const pt4* pts, const InstLayout& IL, bnode* nodes, const long long* leaf_off,
+                    int ni, long long) {
+        for (int inst = 0; inst < ni; ++inst)
+            for (long long l = 0; l < leaf_off[inst + 1] - leaf_off[inst]; ++l)
+                bvh_leaf(pts, IL, nodes, inst, (int)l);
+    }
+
+    void bvh_level(const InstLayout& IL, bnode* nodes, const long long*, int ni, long long, int h) {
+        for (int inst = 0; inst < ni; ++inst) {
+            int L = IL.L[inst];
+            for (int k = L >> h; k < (2 * L) >> h; ++k) bvh_merge(nodes, IL.t0[inst], k);
+        }
+    }
+
+    void wmax(const pt4* pts, const InstLayout& IL, int ni, double* out) {
+        for (int inst = 0; inst < ni; ++inst) {
+            double m = -INFINITY;
+            for (int s = IL.p0[inst]; s < IL.p0[inst] + IL.n[inst]; ++s) m = std::fmax(m, pts[s].w);
+            out[inst] = m;
+        }
+    }
+
+    template <int K>
+    void knn_k(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* queries, int nq,
+               int* out, float* out_d2) {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int t = 0; t < nq; ++t) {
+            int q = queries ? queries[t] : t;
+            knn_query<K>(pf, nodes, IL, q, out + (size_t)t * K, out_d2 ? out_d2 + (size_t)t * K : nullptr);
+        }
+    }
+
+    void knn(int K, const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* queries,
+             int nq, int* out, float* out_d2) {
+        switch (K) {
+            case 16: knn_k<16>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 24: knn_k<24>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 32: knn_k<32>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 48: knn_k<48>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 64: knn_k<64>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            default: throw std::runtime_error("unsupported K");
+        }
+    }
+
+    template <int S>
+    void clip_s(const ClipParams& P) {
+#pragma omp parallel
+        {
+            std::unique_ptr<CellLds<S>> L(new CellLds<S>());
+            std::memset(L.get(), 0, sizeof(CellLds<S>));
+#pragma omp for schedule(dynamic, 16)
+            for (int it = 0; it < P.n_items; ++it) process_cell<S>(P, it, *L);
+        }
+    }
+
+    void clip(int S, const ClipParams& P) {
+        if (S == 1)
+            clip_s<1>(P);
+        else
+            clip_s<2>(P);
+    }
+
+    void radius_r2(const int* items, int ni, const pt4* pts, const double* r2, const int* inst_of,
+                   const double* wmaxv, int weighted, float* rho2) {
+        for (int t = 0; t < ni; ++t) {
+            int c = items[t];
+            int inst = inst_of ? inst_of[c] : 0;
+            rho2[t] = radius_for(r2[c], pts[c].w, weighted ? wmaxv[inst] : 0.0, weighted);
+        }
+    }
+
+    void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
+                      int ni, const float* rho2, int* cnt) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int t = 0; t < ni; ++t) cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], nullptr, nullptr);
+    }
+
+    void radius_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
+                     int ni, const float* rho2, const long long* off, int* list, float* d2) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int t = 0; t < ni; ++t)
+            radius_query(pf, nodes, IL, items[t], rho2[t], list + off[t], d2 + off[t]);
+    }
+
+    void scan_i32(const int* in, long long* out, int n) {
+        out[0] = 0;
+        for (int i = 0; i < n; ++i) out[i + 1] = out[i] + in[i];
+    }
+
+    void seg_sort(const float* kin, const int* vin, float* kout, int* vout, long long, int nseg,
+                  const long long* off) {
+        for (int s = 0; s < nseg; ++s) {
+            long long b = off[s], e = off[s + 1];
+            std::vector<long long> idx(e - b);
+            std::iota(idx.begin(), idx.end(), b);
+            std::stable_sort(idx.begin(), idx.end(), [&](long long x, long long y) { return kin[x] < kin[y]; });
+            for (long long i = 0; i < e - b; ++i) {
+                kout[b + i] = kin[idx[i]];
+                vout[b + i] = vin[idx[i]];
+            }
+        }
+    }
+
+    void compact(int n, const int* cnt, const long long* ptr, const int* nbr_tmp,
+                 const double* area_tmp, int stride, int* nbr, double* area) {
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < cnt[i]; ++k) {
+                nbr[ptr[i] + k] = nbr_tmp[(size_t)i * stride + k];
+                area[ptr[i] + k] = area_tmp[(size_t)i * stride + k];
+            }
+    }
+
+    void lloyd_update(double* xyz, const double* vol, const double* mg, int n) {
+        for (int i = 0; i < n; ++i) {
+            if (vol[i] > 1e-30) {
+                double s = 1.0 / vol[i];
+                xyz[3 * (size_t)i + 0] = s * mg[3 * (size_t)i + 0];
+                xyz[3 * (size_t)i + 1] = s * mg[3 * (size_t)i + 1];
+                xyz[3 * (size_t)i + 2] = s * mg[3 * (size_t)i + 2];
+            }
+        }
+    }
+
+    void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
+                       float* outd) {
+        for (int q = 0; q < n; ++q) {
+            int i = orig[q];
+            for (int k = 0; k < K; ++k) {
+                int j = list[(size_t)q * K + k];
+                out[(size_t)i * K + k] = j >= 0 ? orig[j] : -1;
+                outd[(size_t)i * K + k] = d2[(size_t)q * K + k];
+            }
+        }
+    }
+};
+
+typedef Ctx<EmuBackend> EmuCtx;
+
+}  // namespace vgc
+
+using vgc::EmuCtx;
+
+extern "C" {
+
+int vgc_emu_create(void** ctx) {
+    if (!ctx) return VGC_ERR_ARG;
+    *ctx = new EmuCtx(0);
+    return VGC_OK;
+}
+void vgc_emu_destroy(void* ctx) { delete (EmuCtx*)ctx; }
+const char* vgc_emu_last_error(const void* ctx) {
+    return ctx ? ((const EmuCtx*)ctx)->err.c_str() : "null context";
+}
+int vgc_emu_knn(void* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
+                float* d2) {
+    return vgc::api_knn((EmuCtx*)ctx, xyz, n, k, flags, nbr, d2);
+}
+int vgc_emu_cells(void* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+                  uint32_t k0, uint32_t flags, double* vol, double* mg, uint64_t* nbr_ptr,
+                  int32_t* nbr, double* farea, uint64_t cap, vgc_stats* st) {
+    return vgc::api_cells((EmuCtx*)ctx, xyz, w, &n, 1u, box, k0, flags, vol, mg, nbr_ptr, nbr,
+                          farea, cap, st);
+}
+int vgc_emu_cells_batch(void* ctx, const double* xyz, const double* w, const uint32_t* inst_n,
+                        uint32_t n_inst, const double* boxes, uint32_t k0, uint32_t flags,
+                        double* vol, double* mg, uint64_t* nbr_ptr, int32_t* nbr, double* farea,
+                        uint64_t cap, vgc_stats* st) {
+    return vgc::api_cells((EmuCtx*)ctx, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr,
+                          nbr, farea, cap, st);
+}
+int vgc_emu_lloyd(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+                  uint32_t k0, uint32_t flags, vgc_stats* st) {
+    return vgc::api_lloyd((EmuCtx*)ctx, xyz, n, box, iters, k0, flags, st);
+}
+}

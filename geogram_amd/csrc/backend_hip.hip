@@ -1,0 +1,455 @@
+// gfx950 backend of the cell pipeline and the exported C ABI (libvgc.so).
+//
+// Kernels:
+//   k_morton / rocPRIM radix sort / k_gather   Morton order of the seeds
+//   k_bvh_leaves, k_bvh_level                  implicit tree over buckets
+//   k_knn<K>                                   lane-per-query kNN, FP32 ranking
+//   k_clip<S>                                  wavefront-per-cell clipping and
+//                                              FP64 integration (clip_cell.h)
+//   k_radius_*                                 complete lists for pass B / C
+//   k_compact                                  neighbour CSR
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include <stdexcept>
+#include <string>
+
+#include "api_impl.h"
+
+#define VGC_HIP_CHECK(x)                                                              \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                     " at " #x);                                      \
+    } while (0)
+
+namespace vgc {
+
+// ----------------------------------------------------------------- kernels
+
+__device__ __forceinline__ int find_inst(const long long* off, int ni, long long t) {
+    int lo = 0, hi = ni - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_fill_inst(int* inst_orig, int n, const int* p0, int ni) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = ni - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (p0[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    inst_orig[i] = lo;
+}
+
+__global__ void k_morton(const double* __restrict__ xyz, const int* __restrict__ inst_orig,
+                         const double* __restrict__ box, int n, uint64_t* keys, int* vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int inst = inst_orig ? inst_orig[i] : 0;
+    keys[i] = morton_key(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2],
+                         box + 6 * inst, (uint64_t)inst);
+    vals[i] = i;
+}
+
+__global__ void k_gather(const double* __restrict__ xyz, const double* __restrict__ w,
+                         const int* __restrict__ inst_orig, const int* __restrict__ perm, int n,
+                         pt4* pts, pf4* pf, int* orig, int* inst_of) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int i = perm[s];
+    pt4 p;
+    p.x = xyz[3 * (size_t)i];
+    p.y = xyz[3 * (size_t)i + 1];
+    p.z = xyz[3 * (size_t)i + 2];
+    p.w = w ? w[i] : 0.0;
+    pts[s] = p;
+    pf4 f;
+    f.x = float(p.x);
+    f.y = float(p.y);
+    f.z = float(p.z);
+    f.w = float(p.w);
+    pf[s] = f;
+    orig[s] = i;
+    if (inst_of) inst_of[s] = inst_orig ? inst_orig[i] : 0;
+}
+
+__global__ void k_bvh_leaves(const pt4* __restrict__ pts, InstLayout IL, bnode* nodes,
+                             const long long* __restrict__ leaf_off, int ni, long long total) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int inst = find_inst(leaf_off, ni, t);
+    bvh_leaf(pts, IL, nodes, inst, (int)(t - leaf_off[inst]));
+}
+
+__global__ void k_bvh_level(InstLayout IL, bnode* nodes, const long long* __restrict__ lvl_off,
+                            int ni, long long total, int h) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int inst = find_inst(lvl_off, ni, t);
+    int k = (IL.L[inst] >> h) + (int)(t - lvl_off[inst]);
+    bvh_merge(nodes, IL.t0[inst], k);
+}
+
+__global__ void k_wmax(const pt4* __restrict__ pts, InstLayout IL, double* out) {
+    __shared__ double red[256];
+    int inst = blockIdx.x;
+    int p0 = IL.p0[inst], n = IL.n[inst];
+    double m = -INFINITY;
+    for (int s = p0 + threadIdx.x; s < p0 + n; s += blockDim.x) m = fmax(m, pts[s].w);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[inst] = red[0];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_knn(const pf4* __restrict__ pf,
+                                             const bnode* __restrict__ nodes, InstLayout IL,
+                                             const int* __restrict__ queries, int nq, int* out,
+                                             float* out_d2) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    int q = queries ? queries[t] : t;
+    knn_query<K>(pf, nodes, IL, q, out + (size_t)t * K, out_d2 ? out_d2 + (size_t)t * K : nullptr);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_clip(ClipParams P) {
+    __shared__ CellLds<S> lds[4];
+    const int wave = threadIdx.x >> 6;
+    const int item = blockIdx.x * 4 + wave;
+    if (item >= P.n_items) return;  // whole wavefront
+    process_cell<S>(P, item, lds[wave]);
+}
+
+__global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
+                            const double* __restrict__ r2, const int* __restrict__ inst_of,
+                            const double* __restrict__ wmaxv, int weighted, float* rho2) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni) return;
+    int c = items[t];
+    int inst = inst_of ? inst_of[c] : 0;
+    rho2[t] = radius_for(r2[c], pts[c].w, weighted ? wmaxv[inst] : 0.0, weighted);
+}
+
+__global__ void k_radius_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
+                               InstLayout IL, const int* __restrict__ items, int ni,
+                               const float* __restrict__ rho2, int* cnt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni) return;
+    cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], nullptr, nullptr);
+}
+
+__global__ void k_radius_fill(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
+                              InstLayout IL, const int* __restrict__ items, int ni,
+                              const float* __restrict__ rho2, const long long* __restrict__ off,
+                              int* list, float* d2) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni) return;
+    radius_query(pf, nodes, IL, items[t], rho2[t], list + off[t], d2 + off[t]);
+}
+
+__global__ void k_compact(int n, const int* __restrict__ cnt, const long long* __restrict__ ptr,
+                          const int* __restrict__ nbr_tmp, const double* __restrict__ area_tmp,
+                          int stride, int* nbr, double* area) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long o = ptr[i];
+    for (int k = 0; k < cnt[i]; ++k) {
+        nbr[o + k] = nbr_tmp[(size_t)i * stride + k];
+        area[o + k] = area_tmp[(size_t)i * stride + k];
+    }
+}
+
+__global__ void k_lloyd_update(double* xyz, const double* __restrict__ vol,
+                               const double* __restrict__ mg, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double m = vol[i];
+    if (m > 1e-30) {
+        double s = 1.0 / m;
+        xyz[3 * (size_t)i + 0] = s * mg[3 * (size_t)i + 0];
+        xyz[3 * (size_t)i + 1] = s * mg[3 * (size_t)i + 1];
+        xyz[3 * (size_t)i + 2] = s * mg[3 * (size_t)i + 2];
+    }
+}
+
+__global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restrict__ list,
+                                const float* __restrict__ d2, int n, int K, int* out, float* outd) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int i = orig[q];
+    for (int k = 0; k < K; ++k) {
+        int j = list[(size_t)q * K + k];
+        out[(size_t)i * K + k] = j >= 0 ? orig[j] : -1;
+        outd[(size_t)i * K + k] = d2[(size_t)q * K + k];
+    }
+}
+
+__global__ void k_set0_i64(long long* p) { *p = 0; }
+
+// ----------------------------------------------------------------- backend
+
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct HipBackend {
+    explicit HipBackend(int device) : dev_(device) {
+        VGC_HIP_CHECK(hipSetDevice(dev_));
+        VGC_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    }
+    ~HipBackend() {
+        if (temp_) (void)hipFree(temp_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    hipStream_t stream() const { return stream_; }
+
+    void* dalloc(size_t b) {
+        void* p = nullptr;
+        VGC_HIP_CHECK(hipMalloc(&p, b ? b : 1));
+        return p;
+    }
+    void dfree(void* p) { (void)hipFree(p); }
+    void h2d(void* d, const void* s, size_t b) {
+        VGC_HIP_CHECK(hipMemcpyAsync(d, s, b, hipMemcpyHostToDevice, stream_));
+        VGC_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    void d2h(void* d, const void* s, size_t b) {
+        VGC_HIP_CHECK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToHost, stream_));
+        VGC_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    void d2d(void* d, const void* s, size_t b) {
+        VGC_HIP_CHECK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream_));
+    }
+    void dzero(void* d, size_t b) { VGC_HIP_CHECK(hipMemsetAsync(d, 0, b, stream_)); }
+    void sync() { VGC_HIP_CHECK(hipStreamSynchronize(stream_)); }
+    void check() { VGC_HIP_CHECK(hipGetLastError()); }
+
+    void* temp(size_t b) {
+        if (b > temp_bytes_) {
+            if (temp_) VGC_HIP_CHECK(hipFree(temp_));
+            temp_bytes_ = b + b / 4;
+            VGC_HIP_CHECK(hipMalloc(&temp_, temp_bytes_));
+        }
+        return temp_;
+    }
+
+    void fill_inst(int* inst_orig, int n, const int* p0, int ni) {
+        k_fill_inst<<<blocks_for(n, 256), 256, 0, stream_>>>(inst_orig, n, p0, ni);
+        check();
+    }
+    void morton(const double* xyz, const int* inst_orig, const double* box, int n, uint64_t* keys,
+                int* vals) {
+        k_morton<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, inst_orig, box, n, keys, vals);
+        check();
+    }
+    void sort_pairs(uint64_t* keys, int* vals, uint64_t* keys2, int* vals2, int n, int end_bit) {
+        size_t tb = 0;
+        VGC_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, (unsigned)n,
+                                                0, end_bit, stream_));
+        void* t = temp(tb);
+        VGC_HIP_CHECK(rocprim::radix_sort_pairs(t, tb, keys, keys2, vals, vals2, (unsigned)n, 0,
+                                                end_bit, stream_));
+    }
+    void gather(const double* xyz, const double* w, const int* inst_orig, const int* perm, int n,
+                pt4* pts, pf4* pf, int* orig, int* inst_of) {
+        k_gather<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, w, inst_orig, perm, n, pts, pf, orig,
+                                                          inst_of);
+        check();
+    }
+    void bvh_leaves(const pt4* pts, const InstLayout& IL, bnode* nodes, const long long* leaf_off,
+                    int ni, long long total) {
+        k_bvh_leaves<<<blocks_for(total, 256), 256, 0, stream_>>>(pts, IL, nodes, leaf_off, ni, total);
+        check();
+    }
+    void bvh_level(const InstLayout& IL, bnode* nodes, const long long* lvl_off, int ni,
+                   long long total, int h) {
+        k_bvh_level<<<blocks_for(total, 256), 256, 0, stream_>>>(IL, nodes, lvl_off, ni, total, h);
+        check();
+    }
+    void wmax(const pt4* pts, const InstLayout& IL, int ni, double* out) {
+        k_wmax<<<ni, 256, 0, stream_>>>(pts, IL, out);
+        check();
+    }
+    void knn(int K, const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* queries,
+             int nq, int* out, float* out_d2) {
+        unsigned g = blocks_for(nq, 256);
+        switch (K) {
+            case 16: k_knn<16><<<g, 256, 0, stream_>>>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 24: k_knn<24><<<g, 256, 0, stream_>>>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 32: k_knn<32><<<g, 256, 0, stream_>>>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 48: k_knn<48><<<g, 256, 0, stream_>>>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            case 64: k_knn<64><<<g, 256, 0, stream_>>>(pf, nodes, IL, queries, nq, out, out_d2); break;
+            default: throw std::runtime_error("unsupported K");
+        }
+        check();
+    }
+    void clip(int S, const ClipParams& P) {
+        if (P.n_items <= 0) return;
+        unsigned g = blocks_for(P.n_items, 4);
+        if (S == 1)
+            k_clip<1><<<g, 256, 0, stream_>>>(P);
+        else
+            k_clip<2><<<g, 256, 0, stream_>>>(P);
+        check();
+    }
+    void radius_r2(const int* items, int ni, const pt4* pts, const double* r2, const int* inst_of,
+                   const double* wmaxv, int weighted, float* rho2) {
+        k_radius_r2<<<blocks_for(ni, 256), 256, 0, stream_>>>(items, ni, pts, r2, inst_of, wmaxv,
+                                                              weighted, rho2);
+        check();
+    }
+    void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
+                      int ni, const float* rho2, int* cnt) {
+        k_radius_count<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, cnt);
+        check();
+    }
+    void radius_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
+                     int ni, const float* rho2, const long long* off, int* list, float* d2) {
+        k_radius_fill<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, off,
+                                                              list, d2);
+        check();
+    }
+    void scan_i32(const int* in, long long* out, int n) {
+        size_t tb = 0;
+        VGC_HIP_CHECK(rocprim::inclusive_scan(nullptr, tb, in, out + 1, (size_t)n,
+                                              rocprim::plus<long long>(), stream_));
+        void* t = temp(tb);
+        VGC_HIP_CHECK(rocprim::inclusive_scan(t, tb, in, out + 1, (size_t)n,
+                                              rocprim::plus<long long>(), stream_));
+        k_set0_i64<<<1, 1, 0, stream_>>>(out);
+        check();
+    }
+    void seg_sort(const float* kin, const int* vin, float* kout, int* vout, long long total,
+                  int nseg, const long long* off) {
+        if (total <= 0) return;
+        size_t tb = 0;
+        VGC_HIP_CHECK(rocprim::segmented_radix_sort_pairs(nullptr, tb, kin, kout, vin, vout,
+                                                          (unsigned)total, (unsigned)nseg, off,
+                                                          off + 1, 0, 32, stream_));
+        void* t = temp(tb);
+        VGC_HIP_CHECK(rocprim::segmented_radix_sort_pairs(t, tb, kin, kout, vin, vout,
+                                                          (unsigned)total, (unsigned)nseg, off,
+                                                          off + 1, 0, 32, stream_));
+    }
+    void compact(int n, const int* cnt, const long long* ptr, const int* nbr_tmp,
+                 const double* area_tmp, int stride, int* nbr, double* area) {
+        k_compact<<<blocks_for(n, 256), 256, 0, stream_>>>(n, cnt, ptr, nbr_tmp, area_tmp, stride,
+                                                           nbr, area);
+        check();
+    }
+    void lloyd_update(double* xyz, const double* vol, const double* mg, int n) {
+        k_lloyd_update<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, vol, mg, n);
+        check();
+    }
+    void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
+                       float* outd) {
+        k_knn_to_caller<<<blocks_for(n, 256), 256, 0, stream_>>>(orig, list, d2, n, K, out, outd);
+        check();
+    }
+
+private:
+    int dev_;
+    hipStream_t stream_ = nullptr;
+    void* temp_ = nullptr;
+    size_t temp_bytes_ = 0;
+};
+
+typedef Ctx<HipBackend> HipCtx;
+
+}  // namespace vgc
+
+struct vgc_ctx {
+    vgc::HipCtx* impl;
+};
+
+extern "C" {
+
+int vgc_create(vgc_ctx** ctx, int device) {
+    if (!ctx) return VGC_ERR_ARG;
+    *ctx = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return VGC_ERR_DEVICE;
+    try {
+        vgc_ctx* c = new vgc_ctx;
+        c->impl = new vgc::HipCtx(device);
+        *ctx = c;
+        return VGC_OK;
+    } catch (const std::exception&) {
+        return VGC_ERR_DEVICE;
+    }
+}
+
+void vgc_destroy(vgc_ctx* ctx) {
+    if (!ctx) return;
+    delete ctx->impl;
+    delete ctx;
+}
+
+const char* vgc_last_error(const vgc_ctx* ctx) {
+    if (!ctx || !ctx->impl) return "null context";
+    return ctx->impl->err.c_str();
+}
+
+int vgc_knn(vgc_ctx* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
+            float* d2) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_knn(ctx->impl, xyz, n, k, flags, nbr, d2);
+}
+
+int vgc_cells(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+              uint32_t k0, uint32_t flags, double* vol, double* mg, uint64_t* nbr_ptr,
+              int32_t* nbr, double* farea, uint64_t cap, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_cells(ctx->impl, xyz, w, &n, 1u, box, k0, flags, vol, mg, nbr_ptr, nbr, farea,
+                          cap, stats);
+}
+
+int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w, const uint32_t* inst_n,
+                    uint32_t n_inst, const double* boxes, uint32_t k0, uint32_t flags, double* vol,
+                    double* mg, uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
+                    vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_cells(ctx->impl, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr, nbr,
+                          farea, cap, stats);
+}
+
+int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+              uint32_t k0, uint32_t flags, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_lloyd(ctx->impl, xyz, n, box, iters, k0, flags, stats);
+}
+
+int vgc_random_float64(uint64_t seed, double* out, uint64_t count) {
+    return vgc::random_float64(seed, out, count);
+}
+
+int vgc_sample_points(uint32_t kind, uint64_t seed, uint32_t n, double* xyz) {
+    return vgc::sample_points(kind, seed, n, xyz);
+}
+}

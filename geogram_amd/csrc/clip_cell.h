@@ -1,0 +1,642 @@
+// One Voronoi / Laguerre cell per 64-lane wavefront: clip the domain box by
+// bisector (power) planes in nearest-first order until the security radius is
+// reached, then integrate volume, first moment and facet areas.
+//
+// Semantics follow geogram's VBW::ConvexCell (voronoi/convex_cell.cpp) driven
+// by the security-radius loop of RestrictedVoronoiDiagram::clip_by_cell_SR
+// (voronoi/generic_RVD.h:2278-2347):
+//   * the cell is kept in dual form: planes are the dual vertices, corners are
+//     the dual triangles (three plane ids, counter-clockwise seen from
+//     outside), with the three corners across their edges;
+//   * a corner conflicts with a new plane when det4(Pa,Pb,Pc,P) >= 0
+//     (convex_cell.cpp:905-911): ties remove the corner;
+//   * the hole left by the conflicting corners is re-triangulated by one new
+//     corner per border edge (convex_cell.cpp:761-865);
+//   * clipping stops at the first neighbour j with |pi-pj|^2 > 4.1 R^2,
+//     R = largest corner distance to pi (generic_RVD.h:2303-2323);
+//   * a coincident neighbour with a smaller original index leaves the cell
+//     unclipped, a coincident one with a larger index is skipped
+//     (delaunay/delaunay_nn.cpp:125-133);
+//   * volume and first moment are fans of tetrahedra from one corner, facet
+//     areas fans of triangles around each face (convex_cell.cpp:1160-1307).
+//
+// gfx950 layout.  Coordinates are relative to the seed, so a plane is
+// e.y <= h with e = pj - pi and h = (|e|^2 + wi - wj) / 2.  Lane l of the
+// wave owns corners l, 64+l, ... (S slots): homogeneous point (N, D) with
+// D = det(ea, eb, ec) > 0 and N / D the corner, so the conflict
+// determinant is the four-FMA dot product t = e.N - h D.  Each corner also
+// keeps the magnitudes that bound the rounding error of t: a sign is
+// certified when |t| > gamma_9 * (|e|_1 max|N| + |h| |D|); an uncertified one
+// follows the tie rule above and flags the cell.  Plane equations, new-corner
+// jobs and the two border-walk tables live in the wave's slice of LDS.
+#pragma once
+
+#include "wave.h"
+
+namespace vgc {
+
+struct alignas(32) pt4 {
+    double x, y, z, w;
+};
+
+enum CellFlag : uint32_t {
+    CELL_DONE = 1u,          // outputs written
+    CELL_NEED_K = 2u,        // neighbour list exhausted before the security radius
+    CELL_OVERFLOW = 4u,      // corner / plane / face capacity exceeded
+    CELL_UNCERTAIN = 8u,     // a sign fell inside its error bound (tie rule applied)
+    CELL_EMPTY = 16u,        // empty Laguerre cell
+    CELL_DISCONNECTED = 32u, // coincident seed with a smaller index
+    CELL_BAD_TOPO = 64u,     // conflict zone was not a disk (only after uncertain signs)
+};
+
+constexpr int kPMax = 128;         // plane slots per cell (8-bit ids)
+constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
+constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
+
+struct ClipParams {
+    const pt4* pts;        // seeds in Morton order (x, y, z, weight)
+    const int* orig;       // Morton position -> caller's index
+    const int* inst_of;    // Morton position -> instance (nullptr: one instance)
+    const double* box;     // 6 per instance: xmin ymin zmin xmax ymax zmax
+    const double* wmax;    // per instance max weight (weighted only)
+    const int* inst_n;     // per instance seed count
+    const int* items;      // work items (Morton positions); nullptr = 0..n_items-1
+    int n_items;
+    const int* list;        // neighbour lists (Morton positions), -1 terminated
+    const long long* list_ptr;  // per item start (nullptr: item * list_stride)
+    const int* list_len;        // per item length (nullptr: list_stride)
+    int list_stride;
+    int list_complete;      // lists hold every seed the cell can need
+    int weighted;
+    // outputs, indexed by the caller's seed index
+    double* vol;
+    double* mg;             // 3 per seed: integral of x over the cell
+    int* nbr_cnt;
+    int* nbr_tmp;           // face_stride per seed: caller's index of neighbour
+    double* area_tmp;       // face_stride per seed
+    int face_stride;
+    // per item (Morton position) bookkeeping
+    uint32_t* flags;        // indexed by Morton position
+    double* r2;             // squared radius of the (partial) cell, by Morton position
+    unsigned long long* counters;  // [0] clips, [1] planes tested
+};
+
+template <int S>
+struct CellLds {
+    double pe[kPMax][4];   // e.x e.y e.z h
+    int pg[kPMax];         // Morton position of the neighbour, -1 for the box
+    int v2t[kPMax];        // first corner of each face (integration)
+    uint32_t job[64 * S];  // new corner jobs: a | b << 8 | across << 16
+    uint32_t cpl[64 * S];  // corner planes (integration walk)
+    uint32_t cadj[64 * S]; // corner adjacency (integration walk)
+    double q[64 * S][3];   // corner points (integration)
+    uint8_t st_tab[kPMax]; // new corner whose first plane is p
+    uint8_t en_tab[kPMax]; // new corner whose second plane is p
+};
+
+// ---------------------------------------------------------------- helpers
+
+template <int S>
+VGC_HD LB conf_bit(const uint64_t (&m)[S], const LI& cid) {
+    LU64 mm = LU64(m[0]);
+#pragma unroll
+    for (int s = 1; s < S; ++s) mm = select(cid >= 64 * s, LU64(m[s]), mm);
+    LU64 sh = cvt<uint64_t>(cid & 63);
+    return ((mm >> sh) & LU64(1)) != LU64(0);
+}
+
+VGC_HD LU byte_of(const LU& w, int k) { return (w >> LU(8 * k)) & LU(0xFF); }
+
+struct Corner {
+    LD nx, ny, nz, d, nmax, dabs, r2;
+    LB degenerate;
+};
+
+// Corner of planes a, b, c (outward normals e, offsets h), oriented so that
+// D = det(ea, eb, ec) > 0.  N = ha (eb x ec) + hb (ec x ea) + hc (ea x eb).
+VGC_HD Corner make_corner(const LD& eax, const LD& eay, const LD& eaz, const LD& ha,
+                          const LD& ebx, const LD& eby, const LD& ebz, const LD& hb,
+                          const LD& ecx, const LD& ecy, const LD& ecz, const LD& hc) {
+    Corner C;
+    // cross products and their magnitude sums
+    LD bcx = vfma(eby, ecz, -(ebz * ecy)), bcy = vfma(ebz, ecx, -(ebx * ecz)),
+       bcz = vfma(ebx, ecy, -(eby * ecx));
+    LD cax = vfma(ecy, eaz, -(ecz * eay)), cay = vfma(ecz, eax, -(ecx * eaz)),
+       caz = vfma(ecx, eay, -(ecy * eax));
+    LD abx = vfma(eay, ebz, -(eaz * eby)), aby = vfma(eaz, ebx, -(eax * ebz)),
+       abz = vfma(eax, eby, -(eay * ebx));
+    LD Aeax = vabs(eax), Aeay = vabs(eay), Aeaz = vabs(eaz);
+    LD Aebx = vabs(ebx), Aeby = vabs(eby), Aebz = vabs(ebz);
+    LD Aecx = vabs(ecx), Aecy = vabs(ecy), Aecz = vabs(ecz);
+    LD abcx = vfma(Aeby, Aecz, Aebz * Aecy), abcy = vfma(Aebz, Aecx, Aebx * Aecz),
+       abcz = vfma(Aebx, Aecy, Aeby * Aecx);
+    LD acax = vfma(Aecy, Aeaz, Aecz * Aeay), acay = vfma(Aecz, Aeax, Aecx * Aeaz),
+       acaz = vfma(Aecx, Aeay, Aecy * Aeax);
+    LD aabx = vfma(Aeay, Aebz, Aeaz * Aeby), aaby = vfma(Aeaz, Aebx, Aeax * Aebz),
+       aabz = vfma(Aeax, Aeby, Aeay * Aebx);
+    LD Aha = vabs(ha), Ahb = vabs(hb), Ahc = vabs(hc);
+    C.nx = vfma(ha, bcx, vfma(hb, cax, hc * abx));
+    C.ny = vfma(ha, bcy, vfma(hb, cay, hc * aby));
+    C.nz = vfma(ha, bcz, vfma(hb, caz, hc * abz));
+    C.d = vfma(eax, bcx, vfma(eay, bcy, eaz * bcz));
+    LD nax = vfma(Aha, abcx, vfma(Ahb, acax, Ahc * aabx));
+    LD nay = vfma(Aha, abcy, vfma(Ahb, acay, Ahc * aaby));
+    LD naz = vfma(Aha, abcz, vfma(Ahb, acaz, Ahc * aabz));
+    C.nmax = vmax(nax, vmax(nay, naz));
+    C.dabs = vfma(Aeax, abcx, vfma(Aeay, abcy, Aeaz * abcz));
+    C.degenerate = C.d <= kGamma5 * C.dabs;
+    LD dd = select(C.degenerate, LD(1.0), C.d);
+    LD inv = 1.0 / dd;
+    LD qx = C.nx * inv, qy = C.ny * inv, qz = C.nz * inv;
+    C.r2 = vfma(qx, qx, vfma(qy, qy, qz * qz));
+    return C;
+}
+
+// ------------------------------------------------------------- the cell
+
+template <int S>
+struct CellState {
+    LD nx[S], ny[S], nz[S], d[S], nmax[S], dabs[S], r2[S];
+    LU pl[S], adj[S];
+    uint64_t valid[S];
+};
+
+// Box planes 0..5 = +x -x +y -y +z -z (outward), corners in lanes 0..7.
+template <int S>
+VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const pt4& pc) {
+    const LI lane = lane_id();
+    {
+        LB m = lane < 6;
+        LI axis = lane >> 1;
+        LB neg = (lane & 1) != 0;
+        LD sg = select(neg, LD(-1.0), LD(1.0));
+        LD pcoord = select(axis == 0, LD(pc.x), select(axis == 1, LD(pc.y), LD(pc.z)));
+        LD bmin = select(axis == 0, LD(box[0]), select(axis == 1, LD(box[1]), LD(box[2])));
+        LD bmax = select(axis == 0, LD(box[3]), select(axis == 1, LD(box[4]), LD(box[5])));
+        LD h = select(neg, pcoord - bmin, bmax - pcoord);
+        LI li = select(m, lane, LI(0));
+        lds_st(&L.pe[0][0], li * 4 + 0, select(axis == 0, sg, LD(0.0)), m);
+        lds_st(&L.pe[0][0], li * 4 + 1, select(axis == 1, sg, LD(0.0)), m);
+        lds_st(&L.pe[0][0], li * 4 + 2, select(axis == 2, sg, LD(0.0)), m);
+        lds_st(&L.pe[0][0], li * 4 + 3, h, m);
+        lds_st(L.pg, li, LI(-1), m);
+    }
+    wave_sync();
+    {
+        LB m = lane < 8;
+        LI sx = lane & 1, sy = (lane >> 1) & 1, sz = (lane >> 2) & 1;
+        // max faces are planes 0,2,4 (normal +), min faces 1,3,5 (normal -)
+        LI X = select(sx != 0, LI(0), LI(1));
+        LI Y = select(sy != 0, LI(2), LI(3));
+        LI Z = select(sz != 0, LI(4), LI(5));
+        // product of the normal signs is +1 when the number of min faces is even
+        LI nmin = (1 - sx) + (1 - sy) + (1 - sz);
+        LB pos = (nmin & 1) == 0;
+        LI a = select(pos, X, Y), b = select(pos, Y, X), c = Z;
+        LI l8 = lane & 7;
+        LI a0 = l8 ^ 4;
+        LI a1 = select(pos, l8 ^ 1, l8 ^ 2);
+        LI a2 = select(pos, l8 ^ 2, l8 ^ 1);
+        LD eax = lds_ld(&L.pe[0][0], a * 4 + 0), eay = lds_ld(&L.pe[0][0], a * 4 + 1),
+           eaz = lds_ld(&L.pe[0][0], a * 4 + 2), ha = lds_ld(&L.pe[0][0], a * 4 + 3);
+        LD ebx = lds_ld(&L.pe[0][0], b * 4 + 0), eby = lds_ld(&L.pe[0][0], b * 4 + 1),
+           ebz = lds_ld(&L.pe[0][0], b * 4 + 2), hb = lds_ld(&L.pe[0][0], b * 4 + 3);
+        LD ecx = lds_ld(&L.pe[0][0], c * 4 + 0), ecy = lds_ld(&L.pe[0][0], c * 4 + 1),
+           ecz = lds_ld(&L.pe[0][0], c * 4 + 2), hc = lds_ld(&L.pe[0][0], c * 4 + 3);
+        Corner C = make_corner(eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc);
+        st.nx[0] = C.nx;
+        st.ny[0] = C.ny;
+        st.nz[0] = C.nz;
+        st.d[0] = C.d;
+        st.nmax[0] = C.nmax;
+        st.dabs[0] = C.dabs;
+        st.r2[0] = select(m, C.r2, LD(0.0));
+        st.pl[0] = cvt<uint32_t>(a) | (cvt<uint32_t>(b) << LU(8)) | (cvt<uint32_t>(c) << LU(16));
+        st.adj[0] = cvt<uint32_t>(a0) | (cvt<uint32_t>(a1) << LU(8)) | (cvt<uint32_t>(a2) << LU(16));
+        st.valid[0] = 0xFFull;
+#pragma unroll
+        for (int s = 1; s < S; ++s) {
+            st.valid[s] = 0;
+            st.nx[s] = st.ny[s] = st.nz[s] = 0.0;
+            st.d[s] = 1.0;
+            st.nmax[s] = st.dabs[s] = st.r2[s] = 0.0;
+            st.pl[s] = 0;
+            st.adj[s] = 0;
+        }
+    }
+}
+
+// Clip by plane (e, h) stored at slot np with neighbour id j.  Returns the
+// updated flags; *cut tells whether the plane removed a corner.
+template <int S>
+VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, double ey,
+                           double ez, double h, int j, uint32_t flags, bool& cut) {
+    const LI lane = lane_id();
+    const double e1 = vabs(ex) + vabs(ey) + vabs(ez);
+    const double ha = vabs(h);
+    uint64_t cm[S];
+    bool anyc = false, allc = true, anyu = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LD t = vfma(LD(ex), st.nx[s], vfma(LD(ey), st.ny[s], vfma(LD(ez), st.nz[s], -(h * st.d[s]))));
+        LD bnd = kGamma9 * vfma(LD(e1), st.nmax[s], ha * st.dabs[s]);
+        LB vs = mask_bit(st.valid[s]);
+        cm[s] = ballot(vs & (t >= 0.0));
+        uint64_t um = ballot(vs & (vabs(t) <= bnd));
+        anyc = anyc || (cm[s] != 0);
+        allc = allc && (cm[s] == st.valid[s]);
+        anyu = anyu || (um != 0);
+    }
+    cut = false;
+    if (!anyc) {
+        if (anyu) flags |= CELL_UNCERTAIN;
+        return flags;
+    }
+    cut = true;
+    if (anyu) flags |= CELL_UNCERTAIN;
+    if (allc) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) st.valid[s] = 0;
+        return flags | CELL_EMPTY;
+    }
+    if (np >= kPMax) return flags | CELL_OVERFLOW;
+    // new plane
+    {
+        LB m = lane < 4;
+        LD v = select(lane == 0, LD(ex), select(lane == 1, LD(ey), select(lane == 2, LD(ez), LD(h))));
+        lds_st(&L.pe[0][0], np * 4 + (lane & 3), v, m);
+        lds_st(L.pg, LI(np), LI(j), lane == 0);
+    }
+    // border edges: conflict corner, neighbour across edge k kept
+    uint64_t bk[3][S];
+    int total = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LB cs = mask_bit(cm[s]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            LI nb = cvt<int>(byte_of(st.adj[s], k));
+            LB nbc = conf_bit<S>(cm, nb);
+            bk[k][s] = ballot(cs & !nbc);
+            total += popc64(bk[k][s]);
+        }
+    }
+    uint64_t fr[S];
+    int nfree = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        fr[s] = ~(st.valid[s] & ~cm[s]);
+        nfree += popc64(fr[s]);
+    }
+    if (total > nfree) return flags | CELL_OVERFLOW;
+    // jobs in (edge k, slot, lane) order
+    {
+        int base = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                LB m = mask_bit(bk[k][s]);
+                LI r = base + mbcnt(bk[k][s]);
+                LU pa = byte_of(st.pl[s], k), pb = byte_of(st.pl[s], (k + 1) % 3);
+                LU y = byte_of(st.adj[s], k);
+                lds_st(L.job, select(m, r, LI(0)), pa | (pb << LU(8)) | (y << LU(16)), m);
+                base += popc64(bk[k][s]);
+            }
+        }
+    }
+    wave_sync();
+    // targets: the first `total` free corner slots compute the new corners
+    uint64_t tm[S];
+    {
+        int fbase = 0;
+        const double hn = h;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            LI fr_rank = fbase + mbcnt(fr[s]);
+            LB is_t = mask_bit(fr[s]) & (fr_rank < total);
+            tm[s] = ballot(is_t);
+            fbase += popc64(fr[s]);
+            if (tm[s] != 0) {
+                LU jb = lds_ld(L.job, select(is_t, fr_rank, LI(0)));
+                LI a = cvt<int>(jb & LU(0xFF)) & (kPMax - 1);
+                LI b = cvt<int>((jb >> LU(8)) & LU(0xFF)) & (kPMax - 1);
+                LU y = (jb >> LU(16)) & LU(0xFF);
+                LD eax = lds_ld(&L.pe[0][0], a * 4 + 0), eay = lds_ld(&L.pe[0][0], a * 4 + 1),
+                   eaz = lds_ld(&L.pe[0][0], a * 4 + 2), hav = lds_ld(&L.pe[0][0], a * 4 + 3);
+                LD ebx = lds_ld(&L.pe[0][0], b * 4 + 0), eby = lds_ld(&L.pe[0][0], b * 4 + 1),
+                   ebz = lds_ld(&L.pe[0][0], b * 4 + 2), hbv = lds_ld(&L.pe[0][0], b * 4 + 3);
+                Corner C = make_corner(eax, eay, eaz, hav, ebx, eby, ebz, hbv, LD(ex), LD(ey),
+                                       LD(ez), LD(hn));
+                if (any(is_t & C.degenerate)) flags |= CELL_UNCERTAIN;
+                st.nx[s] = select(is_t, C.nx, st.nx[s]);
+                st.ny[s] = select(is_t, C.ny, st.ny[s]);
+                st.nz[s] = select(is_t, C.nz, st.nz[s]);
+                st.d[s] = select(is_t, C.d, st.d[s]);
+                st.nmax[s] = select(is_t, C.nmax, st.nmax[s]);
+                st.dabs[s] = select(is_t, C.dabs, st.dabs[s]);
+                st.r2[s] = select(is_t, C.r2, st.r2[s]);
+                st.pl[s] = select(is_t, cvt<uint32_t>(a) | (cvt<uint32_t>(b) << LU(8)) |
+                                            (LU(uint32_t(np)) << LU(16)),
+                                  st.pl[s]);
+                st.adj[s] = select(is_t, y, st.adj[s]);
+                LI cid = lane + 64 * s;
+                lds_st(L.st_tab, a, cvt<uint8_t>(cid), is_t);
+                lds_st(L.en_tab, b, cvt<uint8_t>(cid), is_t);
+            }
+        }
+    }
+    wave_sync();
+    // link the new corners around the new face; kept corners re-point to them
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LI cid = lane + 64 * s;
+        if (tm[s] != 0) {
+            LB is_t = mask_bit(tm[s]);
+            LI a = cvt<int>(byte_of(st.pl[s], 0)) & (kPMax - 1);
+            LI b = cvt<int>(byte_of(st.pl[s], 1)) & (kPMax - 1);
+            LI n1 = cvt<int>(lds_ld(L.st_tab, b)) & (64 * S - 1);
+            LI n2 = cvt<int>(lds_ld(L.en_tab, a)) & (64 * S - 1);
+            LI own_a = cvt<int>(lds_ld(L.st_tab, a));
+            LI own_b = cvt<int>(lds_ld(L.en_tab, b));
+            LB wrong = is_t & ((own_a != cid) | (own_b != cid) | !conf_bit<S>(tm, n1) |
+                               !conf_bit<S>(tm, n2));
+            bad = bad || any(wrong);
+            st.adj[s] = select(is_t, (st.adj[s] & LU(0xFF)) | (cvt<uint32_t>(n1) << LU(8)) |
+                                         (cvt<uint32_t>(n2) << LU(16)),
+                               st.adj[s]);
+        }
+        LB keep = mask_bit(st.valid[s] & ~cm[s]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            LI nb = cvt<int>(byte_of(st.adj[s], k));
+            LB hit = keep & conf_bit<S>(cm, nb);
+            if (any(hit)) {
+                LI v = cvt<int>(byte_of(st.pl[s], (k + 1) % 3)) & (kPMax - 1);
+                LI z = cvt<int>(lds_ld(L.st_tab, v)) & (64 * S - 1);
+                bad = bad || any(hit & !conf_bit<S>(tm, z));
+                LU cleared = st.adj[s] & ~LU(0xFFu << (8 * k));
+                st.adj[s] = select(hit, cleared | (cvt<uint32_t>(z) << LU(8 * k)), st.adj[s]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) st.valid[s] = (st.valid[s] & ~cm[s]) | tm[s];
+    ++np;
+    wave_sync();
+    if (bad) flags |= CELL_BAD_TOPO;
+    return flags;
+}
+
+// Security-radius loop + integration for work item `item`.
+template <int S>
+VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
+    const LI lane = lane_id();
+    wave_sync();  // the previous cell of this wave is done with the LDS
+    const int c = P.items ? P.items[item] : item;
+    const pt4 pc = P.pts[c];
+    const int inst = P.inst_of ? P.inst_of[c] : 0;
+    const double* box = P.box + 6 * inst;
+    const double wi = pc.w;
+    const double cw = P.weighted ? (wi - P.wmax[inst]) : 0.0;
+    const int orig_c = P.orig[c];
+    long long lbeg;
+    int llen;
+    if (P.list_ptr) {
+        lbeg = P.list_ptr[item];
+        llen = P.list_len[item];
+    } else {
+        lbeg = (long long)item * P.list_stride;
+        llen = P.list_stride;
+    }
+    bool complete = P.list_complete != 0 || llen >= P.inst_n[inst] - 1;
+
+    CellState<S> st;
+    init_box<S>(st, L, box, pc);
+    int np = 6;
+    uint32_t flags = 0;
+    bool stopped = false;
+    unsigned long long nclip = 0, ntest = 0;
+    for (int jj = 0; jj < llen; ++jj) {
+        const int j = P.list[lbeg + jj];
+        if (j < 0) {
+            complete = true;  // short list: the instance has fewer seeds
+            break;
+        }
+        if (j == c) continue;
+        const pt4 pj = P.pts[j];
+        const double ex = pj.x - pc.x, ey = pj.y - pc.y, ez = pj.z - pc.z;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        if (d2 == 0.0) {
+            if (P.orig[j] < orig_c) {
+                wave_sync();
+                init_box<S>(st, L, box, pc);
+                np = 6;
+                flags = CELL_DISCONNECTED;
+                stopped = true;
+                break;
+            }
+            continue;
+        }
+        // security radius
+        bool reach;
+        if (!P.weighted) {
+            bool anyin = false;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                anyin = anyin || any(mask_bit(st.valid[s]) & (4.1 * st.r2[s] >= d2));
+            reach = !anyin;
+        } else {
+            const double sd = d2 + cw;
+            bool anyin = false;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                anyin = anyin || any(mask_bit(st.valid[s]) & ((4.1 * d2) * st.r2[s] >= sd * sd));
+            reach = (sd > 0.0) && !anyin;
+        }
+        if (reach) {
+            stopped = true;
+            break;
+        }
+        const double h = P.weighted ? 0.5 * (d2 + (wi - pj.w)) : 0.5 * d2;
+        bool cut = false;
+        ++ntest;
+        flags = clip_plane<S>(st, L, np, ex, ey, ez, h, j, flags, cut);
+        if (cut) ++nclip;
+        if (flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
+            stopped = (flags & CELL_EMPTY) != 0;
+            break;
+        }
+    }
+    if (!stopped && !complete && !(flags & (CELL_OVERFLOW | CELL_BAD_TOPO))) flags |= CELL_NEED_K;
+
+    // squared radius of the (possibly partial) cell
+    LD r2m = LD(0.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) r2m = vmax(r2m, select(mask_bit(st.valid[s]), st.r2[s], LD(0.0)));
+    const double r2max = wave_max(r2m);
+
+    if (P.counters && lane0()) {
+        atomic_add_u64(P.counters + 0, nclip);
+        atomic_add_u64(P.counters + 1, ntest);
+    }
+    if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
+        if (lane0()) {
+            P.flags[c] = flags;
+            P.r2[c] = r2max;
+        }
+        return;
+    }
+
+    // ------------------------------------------------------ integration
+    if (flags & CELL_EMPTY) {
+        if (lane0()) {
+            P.vol[orig_c] = 0.0;
+            P.mg[3 * (long long)orig_c + 0] = 0.0;
+            P.mg[3 * (long long)orig_c + 1] = 0.0;
+            P.mg[3 * (long long)orig_c + 2] = 0.0;
+            P.nbr_cnt[orig_c] = 0;
+            P.flags[c] = flags | CELL_DONE;
+            P.r2[c] = 0.0;
+        }
+        return;
+    }
+    // corner points and topology to LDS
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LB vs = mask_bit(st.valid[s]);
+        LI cid = lane + 64 * s;
+        LD inv = 1.0 / select(vs, st.d[s], LD(1.0));
+        lds_st(&L.q[0][0], cid * 3 + 0, st.nx[s] * inv, vs);
+        lds_st(&L.q[0][0], cid * 3 + 1, st.ny[s] * inv, vs);
+        lds_st(&L.q[0][0], cid * 3 + 2, st.nz[s] * inv, vs);
+        lds_st(L.cpl, cid, st.pl[s], vs);
+        lds_st(L.cadj, cid, st.adj[s], vs);
+    }
+    for (int f0 = 0; f0 < np; f0 += 64) {
+        LI f = f0 + lane;
+        lds_st(L.v2t, select(f < np, f, LI(0)), LI(0x7FFFFFFF), f < np);
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LB vs = mask_bit(st.valid[s]);
+        LI cid = lane + 64 * s;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            LI p = cvt<int>(byte_of(st.pl[s], k)) & (kPMax - 1);
+            lds_min(L.v2t, p, cid, vs);
+        }
+    }
+    wave_sync();
+    int cid0 = 0;
+#pragma unroll
+    for (int s = S - 1; s >= 0; --s)
+        if (st.valid[s]) cid0 = 64 * s + ffs64(st.valid[s]);
+    const double ox = L.q[cid0][0], oy = L.q[cid0][1], oz = L.q[cid0][2];
+
+    LD vacc = LD(0.0), mxa = LD(0.0), mya = LD(0.0), mza = LD(0.0);
+    int out_base = 0;
+    bool face_overflow = false;
+    const long long obase = (long long)orig_c * P.face_stride;
+    for (int f0 = 0; f0 < np; f0 += 64) {
+        LI f = f0 + lane;
+        LB inr = f < np;
+        LI fs = select(inr, f, LI(0));
+        LI t0 = lds_ld(L.v2t, fs);
+        LB act = inr & (t0 != LI(0x7FFFFFFF));
+        t0 = select(act, t0, LI(0)) & (64 * S - 1);
+        // successor around face f: across edge (p[lv+2] -> p[lv])
+        auto nxt = [&](const LI& t) -> LI {
+            LU plt = lds_ld(L.cpl, t);
+            LU adt = lds_ld(L.cadj, t);
+            LU fu = cvt<uint32_t>(fs);
+            LI sh = select(byte_of(plt, 0) == fu, LI(16), select(byte_of(plt, 1) == fu, LI(0), LI(8)));
+            return cvt<int>((adt >> cvt<uint32_t>(sh)) & LU(0xFF)) & (64 * S - 1);
+        };
+        LD q0x = lds_ld(&L.q[0][0], t0 * 3 + 0), q0y = lds_ld(&L.q[0][0], t0 * 3 + 1),
+           q0z = lds_ld(&L.q[0][0], t0 * 3 + 2);
+        LI t1 = nxt(t0);
+        LI t = nxt(t1);
+        LD q1x = lds_ld(&L.q[0][0], t1 * 3 + 0), q1y = lds_ld(&L.q[0][0], t1 * 3 + 1),
+           q1z = lds_ld(&L.q[0][0], t1 * 3 + 2);
+        LD area = LD(0.0), fv = LD(0.0), fmx = LD(0.0), fmy = LD(0.0), fmz = LD(0.0);
+        LB run = act & (t != t0);
+        for (int guard = 0; guard < 64 * S && any(run); ++guard) {
+            LD qx = lds_ld(&L.q[0][0], t * 3 + 0), qy = lds_ld(&L.q[0][0], t * 3 + 1),
+               qz = lds_ld(&L.q[0][0], t * 3 + 2);
+            // tetrahedron (apex, q0, q1, q) as in tet_volume (convex_cell.cpp:1193-1213)
+            LD ux = q0x - ox, uy = q0y - oy, uz = q0z - oz;
+            LD vx = q1x - ox, vy = q1y - oy, vz = q1z - oz;
+            LD wx = qx - ox, wy = qy - oy, wz = qz - oz;
+            LD cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            LD tv = vabs(cx * wx + cy * wy + cz * wz) / 6.0;
+            // triangle (q0, q1, q) as in triangle_area (convex_cell.cpp:1147-1158)
+            LD ax = q1x - q0x, ay = q1y - q0y, az = q1z - q0z;
+            LD bx = qx - q0x, by = qy - q0y, bz = qz - q0z;
+            LD wx2 = ay * bz - az * by, wy2 = az * bx - ax * bz, wz2 = ax * by - ay * bx;
+            LD ta = 0.5 * vsqrt(wx2 * wx2 + wy2 * wy2 + wz2 * wz2);
+            fv = select(run, fv + tv, fv);
+            fmx = select(run, fmx + tv * (ox + q0x + q1x + qx) / 4.0, fmx);
+            fmy = select(run, fmy + tv * (oy + q0y + q1y + qy) / 4.0, fmy);
+            fmz = select(run, fmz + tv * (oz + q0z + q1z + qz) / 4.0, fmz);
+            area = select(run, area + ta, area);
+            q1x = select(run, qx, q1x);
+            q1y = select(run, qy, q1y);
+            q1z = select(run, qz, q1z);
+            LI tn = nxt(t);
+            t = select(run, tn, t);
+            run = run & (t != t0);
+        }
+        vacc = vacc + fv;
+        mxa = mxa + fmx;
+        mya = mya + fmy;
+        mza = mza + fmz;
+        LI pgf = lds_ld(L.pg, fs);
+        LB outm = act & (pgf >= 0);
+        uint64_t om = ballot(outm);
+        LI rank = out_base + mbcnt(om);
+        if (out_base + popc64(om) > P.face_stride) {
+            face_overflow = true;
+        } else {
+            // neighbour ids and areas straight to HBM (caller's numbering)
+            LI gj = select(outm, pgf, LI(0));
+            LI oj = LI(0);
+#if defined(VGC_EMU)
+            for (int l = 0; l < kWave; ++l)
+                if (outm[l]) oj[l] = P.orig[gj[l]];
+            for (int l = 0; l < kWave; ++l)
+                if (outm[l]) {
+                    P.nbr_tmp[obase + rank[l]] = oj[l];
+                    P.area_tmp[obase + rank[l]] = area[l];
+                }
+#else
+            if (outm) {
+                oj = P.orig[gj];
+                P.nbr_tmp[obase + rank] = oj;
+                P.area_tmp[obase + rank] = area;
+            }
+#endif
+        }
+        out_base += popc64(om);
+    }
+    const double V = wave_sum(vacc);
+    const double Mx = wave_sum(mxa), My = wave_sum(mya), Mz = wave_sum(mza);
+    if (face_overflow) flags |= CELL_OVERFLOW;
+    if (lane0()) {
+        if (!face_overflow) {
+            P.vol[orig_c] = V;
+            P.mg[3 * (long long)orig_c + 0] = Mx + V * pc.x;
+            P.mg[3 * (long long)orig_c + 1] = My + V * pc.y;
+            P.mg[3 * (long long)orig_c + 2] = Mz + V * pc.z;
+            P.nbr_cnt[orig_c] = out_base;
+            P.flags[c] = flags | CELL_DONE;
+        } else {
+            P.flags[c] = flags;
+        }
+        P.r2[c] = r2max;
+    }
+}
+
+}  // namespace vgc

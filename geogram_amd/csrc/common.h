@@ -1,0 +1,283 @@
+// Types and scalar helpers shared by the gfx950 kernels and their CPU
+// emulation.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(VGC_EMU)
+#include <cmath>
+#define VGC_SD inline
+#else
+#include <hip/hip_runtime.h>
+#define VGC_SD __host__ __device__ __forceinline__
+#endif
+
+#include "clip_cell.h"
+
+namespace vgc {
+
+// Morton-ordered seed in FP32, for the neighbour search.
+struct alignas(16) pf4 {
+    float x, y, z, w;
+};
+
+// Node of the implicit bounding-volume tree over buckets of kBucket
+// consecutive Morton-ordered seeds (float box, rounded outwards).
+struct alignas(32) bnode {
+    float lo[3];
+    float hi[3];
+    int pad[2];
+};
+
+constexpr int kBucket = 16;
+
+// 16-bit quantisation per axis, bits interleaved: 48-bit Morton code; the
+// instance id sits above it so instances sort into contiguous ranges.
+VGC_SD uint64_t spread16(uint64_t v) {
+    v &= 0xFFFFull;
+    v = (v | (v << 16)) & 0x0000FF0000FFull;
+    v = (v | (v << 8)) & 0x00F00F00F00Full;
+    v = (v | (v << 4)) & 0x0C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x249249249249ull;
+    return v;
+}
+
+VGC_SD uint64_t quant16(double x, double lo, double hi) {
+    double t = (x - lo) / (hi - lo);
+    if (!(t > 0.0)) t = 0.0;
+    if (t >= 1.0) t = 0.99999999;
+    return uint64_t(t * 65536.0);
+}
+
+VGC_SD uint64_t morton_key(double x, double y, double z, const double* box, uint64_t inst) {
+    uint64_t qx = quant16(x, box[0], box[3]);
+    uint64_t qy = quant16(y, box[1], box[4]);
+    uint64_t qz = quant16(z, box[2], box[5]);
+    return (inst << 48) | spread16(qx) | (spread16(qy) << 1) | (spread16(qz) << 2);
+}
+
+// float conversions rounded down / up (exact bounds for the tree)
+VGC_SD float f_down(double x) {
+    float f = float(x);
+    if (double(f) > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+VGC_SD float f_up(double x) {
+    float f = float(x);
+    if (double(f) < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+VGC_SD float box_dist2(const bnode& b, float px, float py, float pz) {
+    float dx = fmaxf(fmaxf(b.lo[0] - px, px - b.hi[0]), 0.0f);
+    float dy = fmaxf(fmaxf(b.lo[1] - py, py - b.hi[1]), 0.0f);
+    float dz = fmaxf(fmaxf(b.lo[2] - pz, pz - b.hi[2]), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+VGC_SD float dist2f(const pf4& a, float px, float py, float pz) {
+    float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Per-instance layout of the Morton-ordered seeds, buckets and trees.
+struct InstLayout {
+    const int* inst_of;       // Morton position -> instance (nullptr: one)
+    const int* p0;            // first Morton position
+    const int* n;             // seed count
+    const int* b0;            // first bucket
+    const int* nb;            // bucket count
+    const long long* t0;      // node offset: node k (1 <= k < 2L) at t0 + k
+    const int* L;             // leaves (power of two >= nb)
+};
+
+// --------------------------------------------------------------- kNN
+// K nearest other seeds of each query (FP32 ranking), sorted by (d2, id);
+// unfilled entries are -1.  Traversal: own bucket and its two Morton
+// neighbours first (tight radius), then the implicit tree from the root,
+// left to right, pruning boxes beyond the current K-th distance.
+
+template <int K>
+VGC_SD bool knn_less(float d, int id, float bd, int bi) {
+    return d < bd || (d == bd && id < bi);
+}
+
+template <int K>
+VGC_SD void knn_insert(float (&bd)[K], int (&bi)[K], float d, int id) {
+#pragma unroll
+    for (int k = K - 1; k > 0; --k) {
+        bool lp = knn_less<K>(d, id, bd[k - 1], bi[k - 1]);
+        bool lc = knn_less<K>(d, id, bd[k], bi[k]);
+        float nd = lp ? bd[k - 1] : (lc ? d : bd[k]);
+        int ni = lp ? bi[k - 1] : (lc ? id : bi[k]);
+        bd[k] = nd;
+        bi[k] = ni;
+    }
+    bool l0 = knn_less<K>(d, id, bd[0], bi[0]);
+    bd[0] = l0 ? d : bd[0];
+    bi[0] = l0 ? id : bi[0];
+}
+
+template <int K>
+VGC_SD void knn_scan_bucket(const pf4* pf, int p0, int n, int b, int self, float px,
+                            float py, float pz, float (&bd)[K], int (&bi)[K]) {
+    int s = p0 + b * kBucket;
+    int e = s + kBucket;
+    if (e > p0 + n) e = p0 + n;
+    for (int m = s; m < e; ++m) {
+        if (m == self) continue;
+        float d = dist2f(pf[m], px, py, pz);
+        if (knn_less<K>(d, m, bd[K - 1], bi[K - 1])) knn_insert<K>(bd, bi, d, m);
+    }
+}
+
+template <int K>
+VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
+                      int* out, float* out_d2) {
+    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+    const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
+    const bnode* T = nodes + IL.t0[inst];
+    const pf4 pq = pf[q];
+    const float px = pq.x, py = pq.y, pz = pq.z;
+    float bd[K];
+    int bi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        bd[k] = INFINITY;
+        bi[k] = -1;
+    }
+    const int bq = (q - p0) / kBucket;
+    for (int b = bq - 1; b <= bq + 1; ++b)
+        if (b >= 0 && b < nb) knn_scan_bucket<K>(pf, p0, n, b, q, px, py, pz, bd, bi);
+    int node = 1;
+    while (true) {
+        float lim = bd[K - 1] * 1.00001f;
+        bool inside = box_dist2(T[node], px, py, pz) <= lim;
+        if (inside && node < L) {
+            node = 2 * node;
+            continue;
+        }
+        if (inside) {
+            int b = node - L;
+            if (b < nb && (b < bq - 1 || b > bq + 1))
+                knn_scan_bucket<K>(pf, p0, n, b, q, px, py, pz, bd, bi);
+        }
+        while (node & 1) node >>= 1;
+        if (node == 0) break;
+        node += 1;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        out[k] = bi[k];
+        if (out_d2) out_d2[k] = bd[k];
+    }
+}
+
+// ------------------------------------------------------ radius queries
+// Seeds (other than q) with FP32 squared distance <= r2, in traversal order.
+// count mode when out == nullptr.
+VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
+                        float r2, int* out, float* out_d2) {
+    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+    const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
+    const bnode* T = nodes + IL.t0[inst];
+    const pf4 pq = pf[q];
+    const float px = pq.x, py = pq.y, pz = pq.z;
+    int cnt = 0;
+    int node = 1;
+    while (true) {
+        bool inside = box_dist2(T[node], px, py, pz) <= r2;
+        if (inside && node < L) {
+            node = 2 * node;
+            continue;
+        }
+        if (inside) {
+            int b = node - L;
+            if (b < nb) {
+                int s = p0 + b * kBucket;
+                int e = s + kBucket;
+                if (e > p0 + n) e = p0 + n;
+                for (int m = s; m < e; ++m) {
+                    if (m == q) continue;
+                    float d = dist2f(pf[m], px, py, pz);
+                    if (d <= r2) {
+                        if (out) {
+                            out[cnt] = m;
+                            out_d2[cnt] = d;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+        while (node & 1) node >>= 1;
+        if (node == 0) break;
+        node += 1;
+    }
+    return cnt;
+}
+
+// Squared radius (FP32, rounded up) within which a seed can still cut a
+// partial cell of squared radius r2 around a seed of weight wi; wmax is the
+// largest weight of the instance (Voronoi: weighted = false).  A plane at
+// distance d can cut only if (d^2 + wi - wj) / (2 d) < sqrt(4.1)/2 R, the
+// security-radius margin of generic_RVD.h:2319-2322.
+VGC_SD float radius_for(double r2, double wi, double wmax, int weighted) {
+    double rho2;
+    if (!weighted) {
+        rho2 = 4.1 * r2;
+    } else {
+        double a = 1.0124228365658292 * sqrt(r2);  // sqrt(4.1) / 2
+        double cw = wi - wmax;                     // <= 0
+        double rho = a + sqrt(a * a - cw);
+        rho2 = rho * rho;
+    }
+    return f_up(rho2 * 1.0001);
+}
+
+// ---------------------------------------------------------- tree build
+VGC_SD void bvh_leaf(const pt4* pts, const InstLayout& IL, bnode* nodes, int inst, int leaf) {
+    const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
+    bnode B;
+    B.lo[0] = B.lo[1] = B.lo[2] = INFINITY;
+    B.hi[0] = B.hi[1] = B.hi[2] = -INFINITY;
+    B.pad[0] = B.pad[1] = 0;
+    if (leaf < nb) {
+        int s = p0 + leaf * kBucket;
+        int e = s + kBucket;
+        if (e > p0 + n) e = p0 + n;
+        double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
+        double hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+        for (int m = s; m < e; ++m) {
+            const pt4 p = pts[m];
+            lo0 = fmin(lo0, p.x);
+            lo1 = fmin(lo1, p.y);
+            lo2 = fmin(lo2, p.z);
+            hi0 = fmax(hi0, p.x);
+            hi1 = fmax(hi1, p.y);
+            hi2 = fmax(hi2, p.z);
+        }
+        // the float coordinates used by the search are round-to-nearest of
+        // the doubles, so the outward-rounded box contains them
+        B.lo[0] = fminf(f_down(lo0), float(lo0));
+        B.lo[1] = fminf(f_down(lo1), float(lo1));
+        B.lo[2] = fminf(f_down(lo2), float(lo2));
+        B.hi[0] = fmaxf(f_up(hi0), float(hi0));
+        B.hi[1] = fmaxf(f_up(hi1), float(hi1));
+        B.hi[2] = fmaxf(f_up(hi2), float(hi2));
+    }
+    nodes[IL.t0[inst] + L + leaf] = B;
+}
+
+VGC_SD void bvh_merge(bnode* nodes, long long t0, int k) {
+    const bnode a = nodes[t0 + 2 * k], b = nodes[t0 + 2 * k + 1];
+    bnode B;
+    for (int c = 0; c < 3; ++c) {
+        B.lo[c] = fminf(a.lo[c], b.lo[c]);
+        B.hi[c] = fmaxf(a.hi[c], b.hi[c]);
+    }
+    B.pad[0] = B.pad[1] = 0;
+    nodes[t0 + k] = B;
+}
+
+}  // namespace vgc

@@ -1,0 +1,396 @@
+// Host driver of the cell pipeline, written once against a backend (the HIP
+// backend on gfx950, or the CPU emulation used by the tests).
+//
+//   1. Morton keys (instance id above a 48-bit Morton code) and a radix sort;
+//      seeds are gathered into Morton order (FP64 x,y,z,w and an FP32 copy).
+//   2. An implicit bounding-volume tree per instance over buckets of 16
+//      consecutive seeds.
+//   3. kNN lists of K0 neighbours (FP32 ranking).  Replaces the kd-tree
+//      neighbourhoods of Delaunay_NearestNeighbors (delaunay/delaunay_nn.cpp).
+//   4. Pass A: every cell, one wavefront each, 64 corner slots.
+//   5. Pass B: cells whose list ran out before the security radius.  Their
+//      partial cell bounds the final one, so every seed within
+//      sqrt(4.1) R_partial is gathered (radius query), sorted by distance and
+//      the cell is clipped again: the list is complete by construction.  This
+//      replaces geogram's repeated k += k/8 enlargements
+//      (voronoi/generic_RVD.h:2335-2345) with one exact step.
+//   6. Pass C: cells that overflowed 64 corners, with 128 corner slots.
+//   7. Neighbour lists and facet areas compacted into CSR (caller's order).
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace vgc {
+
+struct EngineStats {
+    unsigned long long nb_cells = 0, nb_neighbors = 0, nb_pass_b = 0, nb_pass_c = 0;
+    unsigned long long nb_uncertain = 0, nb_failed = 0, nb_empty = 0, nb_disconnected = 0;
+    unsigned long long nb_clips = 0, nb_tests = 0, radius_entries = 0;
+    double t_build_ms = 0, t_knn_ms = 0, t_clip_ms = 0, t_pass_bc_ms = 0, t_compact_ms = 0,
+           t_total_ms = 0;
+};
+
+struct Problem {
+    int n = 0;                      // total seeds (all instances)
+    int n_inst = 1;
+    std::vector<int> inst_n;        // seeds per instance (caller order = instance order)
+    std::vector<double> box;        // 6 per instance
+    const double* d_xyz = nullptr;  // device (backend) memory, 3n, caller order
+    const double* d_w = nullptr;    // device, n, or nullptr (Voronoi)
+    int k0 = 32;                    // neighbours of pass A (16, 24, 32, 48 or 64)
+    bool want_neighbors = true;
+};
+
+template <class B>
+class Engine {
+public:
+    explicit Engine(B& be) : be_(be) {}
+    ~Engine() {
+        for (auto& s : slots_)
+            if (s.p) be_.dfree(s.p);
+    }
+
+    std::string error;
+
+    // outputs of the last cells() call (backend memory)
+    double* d_vol = nullptr;
+    double* d_mg = nullptr;
+    long long* d_ptr = nullptr;  // n + 1
+    int* d_nbr = nullptr;
+    double* d_area = nullptr;
+    long long total_neighbors = 0;
+    std::vector<uint32_t> h_flags;  // by Morton position
+
+    // layout of the last build
+    InstLayout IL{};
+    pt4* d_pts = nullptr;
+    pf4* d_pf = nullptr;
+    int* d_orig = nullptr;
+    int* d_inst_of = nullptr;
+    bnode* d_nodes = nullptr;
+
+    static constexpr int kFaceStride = 72;
+
+    // ------------------------------------------------------------ build
+    int build(const Problem& pb, EngineStats& st) {
+        auto t0 = now();
+        n_ = pb.n;
+        ni_ = pb.n_inst;
+        if ((int)pb.inst_n.size() != ni_ || (int)pb.box.size() != 6 * ni_) {
+            error = "instance description does not match n_inst";
+            return -1;
+        }
+        h_p0_.assign(ni_, 0);
+        h_n_ = pb.inst_n;
+        h_b0_.assign(ni_, 0);
+        h_nb_.assign(ni_, 0);
+        h_L_.assign(ni_, 1);
+        h_t0_.assign(ni_, 0);
+        h_leaf_off_.assign(ni_ + 1, 0);
+        long long pp = 0, bb = 0, tt = 0;
+        max_h_ = 0;
+        for (int i = 0; i < ni_; ++i) {
+            h_p0_[i] = (int)pp;
+            pp += h_n_[i];
+            int nb = (h_n_[i] + kBucket - 1) / kBucket;
+            h_nb_[i] = nb;
+            h_b0_[i] = (int)bb;
+            bb += nb;
+            int L = 1, h = 0;
+            while (L < nb) {
+                L *= 2;
+                ++h;
+            }
+            h_L_[i] = L;
+            max_h_ = std::max(max_h_, h);
+            h_t0_[i] = tt;
+            tt += 2LL * L;
+            h_leaf_off_[i + 1] = h_leaf_off_[i] + L;
+        }
+        if (pp != n_) {
+            error = "instance sizes do not add up to n";
+            return -1;
+        }
+        n_nodes_ = tt;
+        int* d_p0 = (int*)slot(S_P0, sizeof(int) * ni_);
+        int* d_n = (int*)slot(S_N, sizeof(int) * ni_);
+        int* d_b0 = (int*)slot(S_B0, sizeof(int) * ni_);
+        int* d_nb = (int*)slot(S_NB, sizeof(int) * ni_);
+        int* d_L = (int*)slot(S_L, sizeof(int) * ni_);
+        long long* d_t0 = (long long*)slot(S_T0, sizeof(long long) * ni_);
+        long long* d_leaf_off = (long long*)slot(S_LEAFOFF, sizeof(long long) * (ni_ + 1));
+        double* d_box = (double*)slot(S_BOX, sizeof(double) * 6 * ni_);
+        be_.h2d(d_p0, h_p0_.data(), sizeof(int) * ni_);
+        be_.h2d(d_n, h_n_.data(), sizeof(int) * ni_);
+        be_.h2d(d_b0, h_b0_.data(), sizeof(int) * ni_);
+        be_.h2d(d_nb, h_nb_.data(), sizeof(int) * ni_);
+        be_.h2d(d_L, h_L_.data(), sizeof(int) * ni_);
+        be_.h2d(d_t0, h_t0_.data(), sizeof(long long) * ni_);
+        be_.h2d(d_leaf_off, h_leaf_off_.data(), sizeof(long long) * (ni_ + 1));
+        be_.h2d(d_box, pb.box.data(), sizeof(double) * 6 * ni_);
+        d_box_ = d_box;
+
+        // Morton sort
+        int* d_inst_orig = nullptr;
+        if (ni_ > 1) {
+            d_inst_orig = (int*)slot(S_INSTORIG, sizeof(int) * (size_t)n_);
+            be_.fill_inst(d_inst_orig, n_, d_p0, ni_);
+        }
+        uint64_t* keys = (uint64_t*)slot(S_KEYS, sizeof(uint64_t) * (size_t)n_);
+        uint64_t* keys2 = (uint64_t*)slot(S_KEYS2, sizeof(uint64_t) * (size_t)n_);
+        int* vals = (int*)slot(S_VALS, sizeof(int) * (size_t)n_);
+        int* vals2 = (int*)slot(S_VALS2, sizeof(int) * (size_t)n_);
+        be_.morton(pb.d_xyz, d_inst_orig, d_box, n_, keys, vals);
+        int end_bit = 48;
+        while ((1LL << (end_bit - 48)) < ni_) ++end_bit;
+        be_.sort_pairs(keys, vals, keys2, vals2, n_, end_bit);  // result in keys2 / vals2
+        d_pts = (pt4*)slot(S_PTS, sizeof(pt4) * (size_t)n_);
+        d_pf = (pf4*)slot(S_PF, sizeof(pf4) * (size_t)n_);
+        d_orig = (int*)slot(S_ORIG, sizeof(int) * (size_t)n_);
+        d_inst_of = ni_ > 1 ? (int*)slot(S_INSTOF, sizeof(int) * (size_t)n_) : nullptr;
+        be_.gather(pb.d_xyz, pb.d_w, d_inst_orig, vals2, n_, d_pts, d_pf, d_orig, d_inst_of);
+
+        IL.inst_of = d_inst_of;
+        IL.p0 = d_p0;
+        IL.n = d_n;
+        IL.b0 = d_b0;
+        IL.nb = d_nb;
+        IL.t0 = d_t0;
+        IL.L = d_L;
+        d_nodes = (bnode*)slot(S_NODES, sizeof(bnode) * (size_t)n_nodes_);
+        be_.bvh_leaves(d_pts, IL, d_nodes, d_leaf_off, ni_, h_leaf_off_[ni_]);
+        std::vector<long long> off(ni_ + 1);
+        long long* d_lvl = (long long*)slot(S_LVLOFF, sizeof(long long) * (ni_ + 1));
+        for (int h = 1; h <= max_h_; ++h) {
+            off[0] = 0;
+            for (int i = 0; i < ni_; ++i) off[i + 1] = off[i] + (h_L_[i] >> h);
+            if (off[ni_] == 0) break;
+            be_.h2d(d_lvl, off.data(), sizeof(long long) * (ni_ + 1));
+            be_.bvh_level(IL, d_nodes, d_lvl, ni_, off[ni_], h);
+        }
+        // weights: max per instance (security radius of Laguerre cells)
+        weighted_ = pb.d_w != nullptr;
+        d_wmax_ = (double*)slot(S_WMAX, sizeof(double) * ni_);
+        if (weighted_) be_.wmax(d_pts, IL, ni_, d_wmax_);
+        be_.sync();
+        st.t_build_ms += ms_since(t0);
+        return 0;
+    }
+
+    // ------------------------------------------------------------ kNN
+    // K nearest of every seed (Morton positions), n x K.
+    int knn_lists(int K, int*& d_list, float* d_d2 = nullptr) {
+        d_list = (int*)slot(S_LIST, sizeof(int) * (size_t)n_ * K);
+        be_.knn(K, d_pf, d_nodes, IL, nullptr, n_, d_list, d_d2);
+        return 0;
+    }
+
+    // ------------------------------------------------------------ cells
+    int cells(const Problem& pb, EngineStats& st) {
+        auto tstart = now();
+        if (build(pb, st)) return -1;
+        const int K0 = pb.k0;
+        auto t1 = now();
+        int* d_list = nullptr;
+        knn_lists(K0, d_list);
+        be_.sync();
+        st.t_knn_ms += ms_since(t1);
+
+        d_vol = (double*)slot(S_VOL, sizeof(double) * (size_t)n_);
+        d_mg = (double*)slot(S_MG, sizeof(double) * 3 * (size_t)n_);
+        int* d_cnt = (int*)slot(S_CNT, sizeof(int) * (size_t)n_);
+        int* d_nbr_tmp = (int*)slot(S_NBRTMP, sizeof(int) * (size_t)n_ * kFaceStride);
+        double* d_area_tmp = (double*)slot(S_AREATMP, sizeof(double) * (size_t)n_ * kFaceStride);
+        uint32_t* d_flags = (uint32_t*)slot(S_FLAGS, sizeof(uint32_t) * (size_t)n_);
+        double* d_r2 = (double*)slot(S_R2, sizeof(double) * (size_t)n_);
+        unsigned long long* d_ctr = (unsigned long long*)slot(S_CTR, sizeof(unsigned long long) * 4);
+        be_.dzero(d_ctr, sizeof(unsigned long long) * 4);
+        be_.dzero(d_flags, sizeof(uint32_t) * (size_t)n_);
+        be_.dzero(d_cnt, sizeof(int) * (size_t)n_);
+
+        ClipParams P{};
+        P.pts = d_pts;
+        P.orig = d_orig;
+        P.inst_of = d_inst_of;
+        P.box = d_box_;
+        P.wmax = d_wmax_;
+        P.inst_n = IL.n;
+        P.weighted = weighted_ ? 1 : 0;
+        P.vol = d_vol;
+        P.mg = d_mg;
+        P.nbr_cnt = d_cnt;
+        P.nbr_tmp = d_nbr_tmp;
+        P.area_tmp = d_area_tmp;
+        P.face_stride = kFaceStride;
+        P.flags = d_flags;
+        P.r2 = d_r2;
+        P.counters = d_ctr;
+
+        // pass A
+        auto t2 = now();
+        P.items = nullptr;
+        P.n_items = n_;
+        P.list = d_list;
+        P.list_ptr = nullptr;
+        P.list_len = nullptr;
+        P.list_stride = K0;
+        P.list_complete = 0;
+        be_.clip(1, P);
+        be_.sync();
+        st.t_clip_ms += ms_since(t2);
+
+        // passes B and C
+        auto t3 = now();
+        h_flags.resize(n_);
+        be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+        std::vector<int> need, over;
+        for (int c = 0; c < n_; ++c) {
+            uint32_t f = h_flags[c];
+            if (f & CELL_OVERFLOW)
+                over.push_back(c);
+            else if (f & CELL_NEED_K)
+                need.push_back(c);
+        }
+        st.nb_pass_b += need.size();
+        if (!need.empty()) {
+            std::vector<int> over_b;
+            if (radius_pass(1, need, P, st)) return -1;
+            be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+            for (int c : need)
+                if (h_flags[c] & CELL_OVERFLOW) over.push_back(c);
+        }
+        st.nb_pass_c += over.size();
+        if (!over.empty()) {
+            if (radius_pass(2, over, P, st)) return -1;
+            be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+        }
+        st.t_pass_bc_ms += ms_since(t3);
+
+        // statistics
+        unsigned long long ctr[4];
+        be_.d2h(ctr, d_ctr, sizeof(ctr));
+        st.nb_clips += ctr[0];
+        st.nb_tests += ctr[1];
+        st.nb_cells += n_;
+        for (int c = 0; c < n_; ++c) {
+            uint32_t f = h_flags[c];
+            if (!(f & CELL_DONE)) st.nb_failed++;
+            if (f & CELL_UNCERTAIN) st.nb_uncertain++;
+            if (f & CELL_EMPTY) st.nb_empty++;
+            if (f & CELL_DISCONNECTED) st.nb_disconnected++;
+        }
+
+        // CSR
+        auto t4 = now();
+        d_ptr = (long long*)slot(S_PTR, sizeof(long long) * ((size_t)n_ + 1));
+        be_.scan_i32(d_cnt, d_ptr, n_);
+        be_.d2h(&total_neighbors, d_ptr + n_, sizeof(long long));
+        st.nb_neighbors += total_neighbors;
+        if (pb.want_neighbors) {
+            d_nbr = (int*)slot(S_NBR, sizeof(int) * std::max<long long>(1, total_neighbors));
+            d_area = (double*)slot(S_AREA, sizeof(double) * std::max<long long>(1, total_neighbors));
+            be_.compact(n_, d_cnt, d_ptr, d_nbr_tmp, d_area_tmp, kFaceStride, d_nbr, d_area);
+        }
+        be_.sync();
+        st.t_compact_ms += ms_since(t4);
+        st.t_total_ms += ms_since(tstart);
+        return 0;
+    }
+
+    // Lloyd relaxation in place on d_xyz (caller order), security radius on.
+    int lloyd(Problem pb, double* d_xyz, int iters, EngineStats& st) {
+        pb.d_xyz = d_xyz;
+        pb.want_neighbors = false;
+        for (int it = 0; it < iters; ++it) {
+            if (cells(pb, st)) return -1;
+            be_.lloyd_update(d_xyz, d_vol, d_mg, n_);
+        }
+        be_.sync();
+        return 0;
+    }
+
+    int n() const { return n_; }
+
+private:
+    enum Slot {
+        S_P0, S_N, S_B0, S_NB, S_L, S_T0, S_LEAFOFF, S_BOX, S_INSTORIG, S_KEYS, S_KEYS2,
+        S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
+        S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
+        S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN,
+        S_COUNT
+    };
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+
+    void* slot(int id, size_t bytes) {
+        if (slots_.size() < S_COUNT) slots_.resize(S_COUNT);
+        Buf& b = slots_[id];
+        if (b.bytes < bytes) {
+            if (b.p) be_.dfree(b.p);
+            size_t nb = std::max(bytes, b.bytes + b.bytes / 4);
+            b.p = be_.dalloc(nb);
+            b.bytes = nb;
+        }
+        return b.p;
+    }
+
+    // Complete lists for `items`: every seed within the distance that can
+    // still cut the partial cell of the previous pass.
+    int radius_pass(int S, const std::vector<int>& items, ClipParams P, EngineStats& st) {
+        const int ni = (int)items.size();
+        int* d_items = (int*)slot(S_ITEMS, sizeof(int) * ni);
+        be_.h2d(d_items, items.data(), sizeof(int) * ni);
+        float* d_rho2 = (float*)slot(S_RHO2, sizeof(float) * ni);
+        be_.radius_r2(d_items, ni, d_pts, P.r2, d_inst_of, d_wmax_, weighted_ ? 1 : 0, d_rho2);
+        int* d_rcnt = (int*)slot(S_RCNT, sizeof(int) * ni);
+        be_.radius_count(d_pf, d_nodes, IL, d_items, ni, d_rho2, d_rcnt);
+        long long* d_roff = (long long*)slot(S_ROFF, sizeof(long long) * (ni + 1));
+        be_.scan_i32(d_rcnt, d_roff, ni);
+        long long total = 0;
+        be_.d2h(&total, d_roff + ni, sizeof(long long));
+        st.radius_entries += total;
+        size_t cap = (size_t)std::max<long long>(total, 1);
+        int* d_rl = (int*)slot(S_RLIST, sizeof(int) * cap);
+        float* d_rd = (float*)slot(S_RD2, sizeof(float) * cap);
+        int* d_rl2 = (int*)slot(S_RLIST2, sizeof(int) * cap);
+        float* d_rd2 = (float*)slot(S_RD22, sizeof(float) * cap);
+        be_.radius_fill(d_pf, d_nodes, IL, d_items, ni, d_rho2, d_roff, d_rl, d_rd);
+        be_.seg_sort(d_rd, d_rl, d_rd2, d_rl2, total, ni, d_roff);  // sorted into rd2 / rl2
+        P.items = d_items;
+        P.n_items = ni;
+        P.list = d_rl2;
+        P.list_ptr = d_roff;
+        P.list_len = d_rcnt;
+        P.list_stride = 0;
+        P.list_complete = 1;
+        be_.clip(S, P);
+        be_.sync();
+        return 0;
+    }
+
+    static std::chrono::steady_clock::time_point now() { return std::chrono::steady_clock::now(); }
+    static double ms_since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(now() - t).count();
+    }
+
+    B& be_;
+    std::vector<Buf> slots_;
+    int n_ = 0, ni_ = 1, max_h_ = 0;
+    long long n_nodes_ = 0;
+    bool weighted_ = false;
+    double* d_box_ = nullptr;
+    double* d_wmax_ = nullptr;
+    std::vector<int> h_p0_, h_n_, h_b0_, h_nb_, h_L_;
+    std::vector<long long> h_t0_, h_leaf_off_;
+};
+
+}  // namespace vgc

@@ -1,0 +1,414 @@
+// Wavefront programming layer shared by the gfx950 kernels and their CPU
+// emulation.
+//
+// The clipping kernel runs one 64-lane wavefront per Voronoi / Laguerre cell:
+// every lane owns cell corners in registers and the lanes cooperate through
+// ballots, lane-rank prefix counts and the wave's slice of LDS.  The same
+// source is compiled twice:
+//   * for gfx950 (hipcc), where Lane<T> is T, a ballot is __ballot and an LDS
+//     gather is an ordinary indexed load;
+//   * for the host with VGC_EMU defined (g++), where Lane<T> holds the 64 lane
+//     values and every operation loops over the lanes in lane order.
+// The emulation reproduces the kernel bit for bit (same operations, same fused
+// multiply-adds, same reduction trees), so the kernel's logic can be tested on
+// a machine without a GPU.
+//
+// Rules for code written on top of this layer: control flow that encloses a
+// wave_sync() or a ballot must be wave-uniform; per-lane choices are made with
+// select() and masked stores, never with if/else on a Lane<bool>.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(VGC_EMU)
+
+#include <cmath>
+#include <cstring>
+
+#define VGC_HD inline
+#define VGC_DEVICE_ONLY inline
+
+namespace vgc {
+
+constexpr int kWave = 64;
+
+template <class T>
+struct ident {
+    typedef T type;
+};
+
+template <class T>
+struct Lane {
+    T v[kWave];
+    Lane() {}
+    Lane(T s) {  // broadcast
+        for (int i = 0; i < kWave; ++i) v[i] = s;
+    }
+    T& operator[](int i) { return v[i]; }
+    const T& operator[](int i) const { return v[i]; }
+};
+
+#define VGC_LANE_BINOP(OP)                                                   \
+    template <class T>                                                       \
+    inline Lane<T> operator OP(const Lane<T>& a, const Lane<T>& b) {         \
+        Lane<T> r;                                                           \
+        for (int i = 0; i < kWave; ++i) r.v[i] = T(a.v[i] OP b.v[i]);        \
+        return r;                                                            \
+    }                                                                        \
+    template <class T>                                                       \
+    inline Lane<T> operator OP(const Lane<T>& a, typename ident<T>::type b) { \
+        Lane<T> r;                                                           \
+        for (int i = 0; i < kWave; ++i) r.v[i] = T(a.v[i] OP b);             \
+        return r;                                                            \
+    }                                                                        \
+    template <class T>                                                       \
+    inline Lane<T> operator OP(typename ident<T>::type a, const Lane<T>& b) { \
+        Lane<T> r;                                                           \
+        for (int i = 0; i < kWave; ++i) r.v[i] = T(a OP b.v[i]);             \
+        return r;                                                            \
+    }                                                                        \
+    template <class T>                                                       \
+    inline Lane<T>& operator OP##=(Lane<T>& a, const Lane<T>& b) {           \
+        a = a OP b;                                                          \
+        return a;                                                            \
+    }                                                                        \
+    template <class T>                                                       \
+    inline Lane<T>& operator OP##=(Lane<T>& a, typename ident<T>::type b) {  \
+        a = a OP b;                                                          \
+        return a;                                                            \
+    }
+
+VGC_LANE_BINOP(+)
+VGC_LANE_BINOP(-)
+VGC_LANE_BINOP(*)
+VGC_LANE_BINOP(/)
+VGC_LANE_BINOP(&)
+VGC_LANE_BINOP(|)
+VGC_LANE_BINOP(^)
+VGC_LANE_BINOP(<<)
+VGC_LANE_BINOP(>>)
+#undef VGC_LANE_BINOP
+
+#define VGC_LANE_CMP(OP)                                                         \
+    template <class T>                                                           \
+    inline Lane<bool> operator OP(const Lane<T>& a, const Lane<T>& b) {          \
+        Lane<bool> r;                                                            \
+        for (int i = 0; i < kWave; ++i) r.v[i] = (a.v[i] OP b.v[i]);             \
+        return r;                                                                \
+    }                                                                            \
+    template <class T>                                                           \
+    inline Lane<bool> operator OP(const Lane<T>& a, typename ident<T>::type b) { \
+        Lane<bool> r;                                                            \
+        for (int i = 0; i < kWave; ++i) r.v[i] = (a.v[i] OP b);                  \
+        return r;                                                                \
+    }                                                                            \
+    template <class T>                                                           \
+    inline Lane<bool> operator OP(typename ident<T>::type a, const Lane<T>& b) { \
+        Lane<bool> r;                                                            \
+        for (int i = 0; i < kWave; ++i) r.v[i] = (a OP b.v[i]);                  \
+        return r;                                                                \
+    }
+
+VGC_LANE_CMP(<)
+VGC_LANE_CMP(<=)
+VGC_LANE_CMP(>)
+VGC_LANE_CMP(>=)
+VGC_LANE_CMP(==)
+VGC_LANE_CMP(!=)
+#undef VGC_LANE_CMP
+
+template <class T>
+inline Lane<T> operator-(const Lane<T>& a) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = -a.v[i];
+    return r;
+}
+inline Lane<bool> operator!(const Lane<bool>& a) {
+    Lane<bool> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = !a.v[i];
+    return r;
+}
+template <class T>
+inline Lane<T> operator~(const Lane<T>& a) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = ~a.v[i];
+    return r;
+}
+
+typedef Lane<double> LD;
+typedef Lane<float> LF;
+typedef Lane<int> LI;
+typedef Lane<uint32_t> LU;
+typedef Lane<uint64_t> LU64;
+typedef Lane<bool> LB;
+
+template <class T, class U>
+inline Lane<T> cvt(const Lane<U>& a) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = T(a.v[i]);
+    return r;
+}
+template <class T, class U>
+inline T cvt(U a) {
+    return T(a);
+}
+
+template <class T>
+inline Lane<T> select(const LB& m, const Lane<T>& a, const Lane<T>& b) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = m.v[i] ? a.v[i] : b.v[i];
+    return r;
+}
+template <class T>
+inline Lane<T> select(const LB& m, const Lane<T>& a, typename ident<T>::type b) {
+    return select(m, a, Lane<T>(b));
+}
+template <class T>
+inline Lane<T> select(const LB& m, typename ident<T>::type a, const Lane<T>& b) {
+    return select(m, Lane<T>(a), b);
+}
+template <class T>
+inline Lane<T> select_s(const LB& m, T a, T b) {
+    return select(m, Lane<T>(a), Lane<T>(b));
+}
+
+inline LD vfma(const LD& a, const LD& b, const LD& c) {
+    LD r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fma(a.v[i], b.v[i], c.v[i]);
+    return r;
+}
+inline LD vabs(const LD& a) {
+    LD r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fabs(a.v[i]);
+    return r;
+}
+inline LD vsqrt(const LD& a) {
+    LD r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::sqrt(a.v[i]);
+    return r;
+}
+inline LD vmax(const LD& a, const LD& b) {
+    LD r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmax(a.v[i], b.v[i]);
+    return r;
+}
+inline LD vmin(const LD& a, const LD& b) {
+    LD r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmin(a.v[i], b.v[i]);
+    return r;
+}
+inline double vfma(double a, double b, double c) { return std::fma(a, b, c); }
+inline double vabs(double a) { return std::fabs(a); }
+inline double vsqrt(double a) { return std::sqrt(a); }
+inline double vmax(double a, double b) { return std::fmax(a, b); }
+inline double vmin(double a, double b) { return std::fmin(a, b); }
+
+inline LI lane_id() {
+    LI r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = i;
+    return r;
+}
+
+inline uint64_t ballot(const LB& b) {
+    uint64_t m = 0;
+    for (int i = 0; i < kWave; ++i)
+        if (b.v[i]) m |= (uint64_t(1) << i);
+    return m;
+}
+
+inline int popc64(uint64_t m) { return __builtin_popcountll(m); }
+inline int ffs64(uint64_t m) { return m ? __builtin_ctzll(m) : -1; }
+
+// number of set bits of m below each lane
+inline LI mbcnt(uint64_t m) {
+    LI r;
+    for (int i = 0; i < kWave; ++i)
+        r.v[i] = __builtin_popcountll(m & ((uint64_t(1) << i) - 1));
+    return r;
+}
+
+// bit (lane-th) of a uniform mask, per lane
+inline LB mask_bit(uint64_t m) {
+    LB r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = (m >> i) & 1;
+    return r;
+}
+
+// test bit idx of a uniform mask at a per-lane index (0..63)
+inline LB mask_test(uint64_t m, const LI& idx) {
+    LB r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = (m >> (idx.v[i] & 63)) & 1;
+    return r;
+}
+
+template <class T>
+inline T readlane(const Lane<T>& a, int lane) {
+    return a.v[lane];
+}
+template <class T>
+inline Lane<T> shfl(const Lane<T>& a, const LI& src) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = a.v[src.v[i] & 63];
+    return r;
+}
+template <class T>
+inline Lane<T> shfl_xor(const Lane<T>& a, int m) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = a.v[i ^ m];
+    return r;
+}
+
+// Deterministic butterfly reductions: every lane ends with the same value.
+inline double wave_sum(const LD& a) {
+    LD x = a;
+    for (int m = 32; m >= 1; m >>= 1) x = x + shfl_xor(x, m);
+    return x.v[0];
+}
+inline double wave_max(const LD& a) {
+    LD x = a;
+    for (int m = 32; m >= 1; m >>= 1) x = vmax(x, shfl_xor(x, m));
+    return x.v[0];
+}
+
+// LDS access: gather, masked scatter (lane order: higher lane wins on a
+// collision, like the hardware's unspecified order; callers never rely on it)
+template <class T>
+inline Lane<T> lds_ld(const T* p, const LI& idx) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = p[idx.v[i]];
+    return r;
+}
+template <class T>
+inline void lds_st(T* p, const LI& idx, const Lane<T>& val, const LB& m) {
+    for (int i = 0; i < kWave; ++i)
+        if (m.v[i]) p[idx.v[i]] = val.v[i];
+}
+inline void lds_min(int* p, const LI& idx, const LI& val, const LB& m) {
+    for (int i = 0; i < kWave; ++i)
+        if (m.v[i] && val.v[i] < p[idx.v[i]]) p[idx.v[i]] = val.v[i];
+}
+inline void wave_sync() {}
+
+inline bool any(const LB& b) { return ballot(b) != 0; }
+
+// a store of a wave-uniform value, done once
+inline bool lane0() { return true; }
+
+inline void atomic_add_u64(unsigned long long* p, unsigned long long v) {
+    __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+}
+
+}  // namespace vgc
+
+#else  // ---------------------------------------------------------- gfx950
+
+#include <hip/hip_runtime.h>
+
+#define VGC_HD __device__ __forceinline__
+#define VGC_DEVICE_ONLY __device__ __forceinline__
+
+namespace vgc {
+
+constexpr int kWave = 64;
+
+template <class T>
+using Lane = T;
+typedef double LD;
+typedef float LF;
+typedef int LI;
+typedef uint32_t LU;
+typedef uint64_t LU64;
+typedef bool LB;
+
+template <class T, class U>
+VGC_HD T cvt(U a) {
+    return T(a);
+}
+
+template <class T>
+VGC_HD T select(bool m, T a, T b) {
+    return m ? a : b;
+}
+template <class T>
+VGC_HD T select_s(bool m, T a, T b) {
+    return m ? a : b;
+}
+
+VGC_HD double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+VGC_HD double vabs(double a) { return __builtin_fabs(a); }
+VGC_HD double vsqrt(double a) { return __builtin_sqrt(a); }
+VGC_HD double vmax(double a, double b) { return __builtin_fmax(a, b); }
+VGC_HD double vmin(double a, double b) { return __builtin_fmin(a, b); }
+
+VGC_HD int lane_id() { return __lane_id(); }
+
+VGC_HD uint64_t ballot(bool b) { return __ballot(b); }
+
+VGC_HD int popc64(uint64_t m) { return __popcll(m); }
+VGC_HD int ffs64(uint64_t m) { return m ? __builtin_ctzll(m) : -1; }
+
+VGC_HD int mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+VGC_HD bool mask_bit(uint64_t m) { return (m >> __lane_id()) & 1; }
+VGC_HD bool mask_test(uint64_t m, int idx) { return (m >> (idx & 63)) & 1; }
+
+template <class T>
+VGC_HD T readlane(T a, int lane) {
+    return __shfl(a, lane);
+}
+template <class T>
+VGC_HD T shfl(T a, int src) {
+    return __shfl(a, src & 63);
+}
+template <class T>
+VGC_HD T shfl_xor(T a, int m) {
+    return __shfl_xor(a, m);
+}
+
+VGC_HD double wave_sum(double a) {
+    double x = a;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = x + __shfl_xor(x, m);
+    return x;
+}
+VGC_HD double wave_max(double a) {
+    double x = a;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = __builtin_fmax(x, __shfl_xor(x, m));
+    return x;
+}
+
+template <class T>
+VGC_HD T lds_ld(const T* p, int idx) {
+    return p[idx];
+}
+template <class T>
+VGC_HD void lds_st(T* p, int idx, T val, bool m) {
+    if (m) p[idx] = val;
+}
+VGC_HD void lds_min(int* p, int idx, int val, bool m) {
+    if (m) atomicMin(p + idx, val);
+}
+
+// Orders the wave's LDS traffic across lanes: the compiler may not move
+// memory operations across it, and every lane's earlier LDS accesses complete
+// before its later ones.
+VGC_HD void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+VGC_HD bool any(bool b) { return __ballot(b) != 0; }
+
+// a store of a wave-uniform value, done by lane 0 only
+VGC_HD bool lane0() { return __lane_id() == 0; }
+
+VGC_HD void atomic_add_u64(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
+
+}  // namespace vgc
+
+#endif

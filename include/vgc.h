@@ -1,0 +1,115 @@
+/*
+ * vgc — MI355X-native meshless Voronoi / Laguerre cells (C ABI).
+ *
+ * Drop-in boundary for geogram's meshless cell path.  Every entry point takes
+ * plain pointers and sizes, never throws, returns VGC_OK (0), a positive
+ * warning (VGC_PARTIAL) or a negative error, and leaves a message readable
+ * with vgc_last_error().  Output buffers are owned and sized by the caller, as
+ * in geogram (CentroidalVoronoiTesselation::Lloyd_iterations hands
+ * RestrictedVoronoiDiagram::compute_centroids caller-allocated mg / m arrays,
+ * voronoi/CVT.cpp:146-152).  With VGC_DEVICE_PTRS the pointers are device
+ * memory of the context's GPU and nothing crosses PCIe.
+ *
+ * Numbering: seeds are indexed in the caller's order; neighbour ids too.
+ * Coordinates are FP64 x,y,z triples (the layout of
+ * CentroidalVoronoiTesselation::embedding, voronoi/CVT.h:333).
+ */
+#ifndef VGC_H
+#define VGC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGC_OK 0
+#define VGC_PARTIAL 1        /* some cells are flagged, see vgc_stats */
+#define VGC_ERR_ARG (-1)
+#define VGC_ERR_DEVICE (-2)
+#define VGC_ERR_CAPACITY (-3)
+#define VGC_ERR_INTERNAL (-4)
+
+#define VGC_DEVICE_PTRS 1u   /* all array arguments are device pointers */
+#define VGC_NO_NEIGHBORS 2u  /* skip the neighbour CSR (volumes/centroids only) */
+
+typedef struct vgc_ctx vgc_ctx;
+
+typedef struct vgc_stats {
+    uint64_t nb_cells;        /* cells computed                                   */
+    uint64_t nb_neighbors;    /* total CSR entries                                */
+    uint64_t nb_pass_b;       /* cells re-clipped with a complete radius list     */
+    uint64_t nb_pass_c;       /* cells re-clipped with 128 corner slots           */
+    uint64_t nb_uncertain;    /* cells with a sign inside its error bound         */
+    uint64_t nb_failed;       /* cells without valid output                       */
+    uint64_t nb_empty;        /* empty Laguerre cells                             */
+    uint64_t nb_disconnected; /* cells left unclipped by the coincident-seed rule */
+    uint64_t nb_clips;        /* planes that cut a cell                           */
+    uint64_t nb_tests;        /* planes tested against a cell                     */
+    uint64_t radius_entries;  /* seeds gathered by the radius pass                */
+    double t_build_ms, t_knn_ms, t_clip_ms, t_pass_bc_ms, t_compact_ms, t_total_ms;
+} vgc_stats;
+
+/* Context on one GPU (device index as seen by HIP).  Replaces nothing in
+   geogram: it owns the device workspace reused across calls. */
+int vgc_create(vgc_ctx** ctx, int device);
+void vgc_destroy(vgc_ctx* ctx);
+const char* vgc_last_error(const vgc_ctx* ctx);
+
+/* k nearest neighbours of every seed, self excluded, ascending distance
+   (FP32 ranking), caller's indices; nbr and d2 are n*k (d2 may be NULL).
+   k <= 64.  Replaces NearestNeighborSearch::get_nearest_neighbors
+   (points/nn_search.h, points/kd_tree.cpp:156-241) called for every point,
+   as Delaunay_NearestNeighbors::store_neighbors_CB does
+   (delaunay/delaunay_nn.cpp:73-84, 105-143). */
+int vgc_knn(vgc_ctx* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags,
+            int32_t* nbr, float* d2);
+
+/* Voronoi (w == NULL) or Laguerre cells (power distance |x-p|^2 - w) of n
+   seeds clipped to box = {xmin,ymin,zmin,xmax,ymax,zmax}.
+   vol[n]: cell volumes; mg[3n]: integral of x over each cell (so mg/vol is
+   the centroid, the quantities RestrictedVoronoiDiagram::compute_centroids
+   accumulates, voronoi/RVD.cpp:429-528, and VBW::ConvexCell::volume /
+   compute_mg return, voronoi/convex_cell.cpp:1215-1307);
+   nbr_ptr[n+1], nbr[], farea[]: CSR of contributing neighbours (the
+   v_global_index of contributing planes, convex_cell.h:728-741,830) and their
+   facet areas (ConvexCell::facet_area, convex_cell.cpp:1160-1191).
+   nbr / farea may be NULL; if cap is too small the call returns
+   VGC_ERR_CAPACITY with the required size in stats->nb_neighbors.
+   k0: first neighbourhood size (16, 24, 32, 48 or 64; 0 = 32). */
+int vgc_cells(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n,
+              const double* box, uint32_t k0, uint32_t flags, double* vol, double* mg,
+              uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
+              vgc_stats* stats);
+
+/* Same for n_inst independent instances in one launch: seeds of instance i
+   are the inst_n[i] consecutive triples after those of instance i-1, boxes
+   holds 6 doubles per instance.  Neighbour ids are global (caller's) indices. */
+int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w,
+                    const uint32_t* inst_n, uint32_t n_inst, const double* boxes,
+                    uint32_t k0, uint32_t flags, double* vol, double* mg,
+                    uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
+                    vgc_stats* stats);
+
+/* iters Lloyd iterations in place with the security radius on: each seed
+   moves to the centroid of its cell when the cell mass exceeds 1e-30.
+   Replaces CentroidalVoronoiTesselation::Lloyd_iterations
+   (voronoi/CVT.cpp:133-167), which would run with the security radius off. */
+int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+              uint32_t k0, uint32_t flags, vgc_stats* stats);
+
+/* count doubles of the stream std::mt19937_64(seed) +
+   uniform_real_distribution<double>(0,1), the generator behind
+   Numeric::random_reset(seed) / random_float64() (basic/numeric.cpp:76-103). */
+int vgc_random_float64(uint64_t seed, double* out, uint64_t count);
+
+/* Synthetic seeds: kind 0 = n uniform in [0,1]^3; kind 1 = n/2 uniform then
+   n - n/2 from an 8-component isotropic Gaussian mixture (means U[0.2,0.8]^3,
+   sigma U[0.01,0.1], rejection to the cube); kind 2 = 16 components,
+   sigma U[0.02,0.15].  All draws from vgc_random_float64(seed). */
+int vgc_sample_points(uint32_t kind, uint64_t seed, uint32_t n, double* xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings to ``liboracle.so``, the plain-C restatement of geogram's
+meshless Voronoi / Laguerre cell path (see ``oracle.h`` for the reference
+files and lines each part follows).  Only ``tests/``, ``__graft_entry__.smoke``
+and the CPU-baseline leg of ``bench.py`` may import this module; the product
+package never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("nb_clip", ctypes.c_longlong),
+        ("nb_knn_grow", ctypes.c_longlong),
+        ("nb_disconnected", ctypes.c_longlong),
+        ("nb_empty", ctypes.c_longlong),
+        ("nb_bad_border", ctypes.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_knn.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.orc_knn_query.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.orc_cells.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+                                dp, dp, ip, ip, dp, ctypes.c_int, ctypes.POINTER(Stats)]
+        L.orc_cells_subset.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+                                       ip, ctypes.c_int, dp, dp, ip, ip, dp, ctypes.c_int,
+                                       ctypes.POINTER(Stats)]
+        L.orc_lloyd.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(Stats)]
+        L.orc_det4_sign.argtypes = [dp, dp, dp, dp]
+        L.orc_det4_sign_exact.argtypes = [dp, dp, dp, dp]
+        L.orc_det3_sign.argtypes = [dp, dp, dp]
+        L.orc_exact_fallbacks.restype = ctypes.c_longlong
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+UNIT_BOX = np.array([0.0, 0.0, 0.0, 1.0, 1.0, 1.0])
+
+MODE_NO_GROWTH = 0
+MODE_SR = 1
+MODE_ALL = 2
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
+
+def knn(xyz, k):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    idx = np.zeros((n, k), dtype=np.int32)
+    d2 = np.zeros((n, k), dtype=np.float64)
+    err = lib().orc_knn(_dp(xyz), n, k, _ip(idx), _dp(d2))
+    if err:
+        raise RuntimeError(f"orc_knn failed ({err})")
+    return idx, d2
+
+
+def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None):
+    """Per-cell volume, mg (= integral of x), neighbour lists and facet areas.
+
+    Returns dict(vol, mg, cnt, nbr, farea, stats); nbr/farea are (n, stride)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    wp = None
+    if w is not None:
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        wp = _dp(w)
+    if subset is None:
+        nc = n
+    else:
+        subset = np.ascontiguousarray(subset, dtype=np.int32)
+        nc = subset.shape[0]
+    vol = np.zeros(nc)
+    mg = np.zeros((nc, 3))
+    cnt = np.zeros(nc, dtype=np.int32)
+    nbr = np.full((nc, stride), -1, dtype=np.int32)
+    fa = np.zeros((nc, stride))
+    st = Stats()
+    if subset is None:
+        err = lib().orc_cells(_dp(xyz), wp, n, _dp(box), int(k0), int(mode), _dp(vol),
+                              _dp(mg), _ip(cnt), _ip(nbr), _dp(fa), stride, ctypes.byref(st))
+    else:
+        err = lib().orc_cells_subset(_dp(xyz), wp, n, _dp(box), int(k0), int(mode),
+                                     _ip(subset), nc, _dp(vol), _dp(mg), _ip(cnt), _ip(nbr),
+                                     _dp(fa), stride, ctypes.byref(st))
+    if err:
+        raise RuntimeError(f"orc_cells failed ({err})")
+    return dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, stats=st.as_dict())
+
+
+def lloyd(xyz, iters, box=UNIT_BOX, k0=20):
+    x = np.array(xyz, dtype=np.float64, copy=True).reshape(-1, 3)
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    st = Stats()
+    err = lib().orc_lloyd(_dp(x), x.shape[0], _dp(box), int(iters), int(k0), ctypes.byref(st))
+    if err:
+        raise RuntimeError(f"orc_lloyd failed ({err})")
+    return x, st.as_dict()
+
+
+def det4_sign(r0, r1, r2, r3, exact_only=False):
+    rows = [np.ascontiguousarray(r, dtype=np.float64) for r in (r0, r1, r2, r3)]
+    f = lib().orc_det4_sign_exact if exact_only else lib().orc_det4_sign
+    return int(f(*[_dp(r) for r in rows]))
+
+
+def det3_sign(r0, r1, r2):
+    rows = [np.ascontiguousarray(r, dtype=np.float64) for r in (r0, r1, r2)]
+    return int(lib().orc_det3_sign(*[_dp(r) for r in rows]))
+
+
+def exact_fallbacks() -> int:
+    return int(lib().orc_exact_fallbacks())
+
+
+def neighbour_sets(res):
+    """List of frozensets of neighbour ids per cell."""
+    return [frozenset(res["nbr"][i, : res["cnt"][i]].tolist()) for i in range(len(res["cnt"]))]

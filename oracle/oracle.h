@@ -1,0 +1,81 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A CPU restatement, in plain C, of geogram's meshless Voronoi / Laguerre cell
+ * path, used to check the HIP implementation.  Only tests, the smoke check and
+ * the CPU-baseline leg of the benchmark may load it; the product never does.
+ *
+ * What it restates (reference files, relative to the geogram source tree):
+ *  - kNN with self removal and the duplicate-point rule of
+ *    Delaunay_NearestNeighbors::get_neighbors_internal
+ *    (delaunay/delaunay_nn.cpp:105-143), over an exact FP64 kd-tree
+ *    (points/kd_tree.cpp:156-387; ties ordered by index here);
+ *  - VBW::ConvexCell in dual form (voronoi/convex_cell.h:419-1436):
+ *    init_with_box (convex_cell.cpp:175-215), clip_by_plane with the exact
+ *    conflict test det_4d >= 0 (convex_cell.cpp:461-516, 870-911),
+ *    triangulate_conflict_zone (convex_cell.cpp:761-865), Cramer corners
+ *    (convex_cell.cpp:971-1082), volume / compute_mg / facet_area fans
+ *    (convex_cell.cpp:1160-1307), squared_radius (convex_cell.cpp:1312-1330);
+ *  - the security-radius loop of RestrictedVoronoiDiagram::clip_by_cell_SR
+ *    (voronoi/generic_RVD.h:2278-2347): stop when d(pi,pj)^2 > 4.1 R^2, grow
+ *    k by k/8 (k > 8) or by 1;
+ *  - the Laguerre plane of PeriodicDelaunay3d::copy_Laguerre_cell_facet_from_
+ *    Delaunay (delaunay/periodic_delaunay_3d.cpp:3719-3726);
+ *  - the Lloyd update of CentroidalVoronoiTesselation::Lloyd_iterations
+ *    (voronoi/CVT.cpp:148-162), with the security radius on.
+ */
+#ifndef VGC_ORACLE_H
+#define VGC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    long long nb_clip;          /* clip_by_plane calls                      */
+    long long nb_knn_grow;      /* neighbourhood enlargements               */
+    long long nb_disconnected;  /* cells left unclipped (duplicate rule)    */
+    long long nb_empty;         /* empty (Laguerre) cells                   */
+    long long nb_bad_border;    /* conflict zones that were not a disk      */
+} orc_stats;
+
+/* Exact FP64 k nearest neighbours of every point, self excluded, sorted by
+   (squared distance, index).  idx / d2 are n*k arrays.  Returns 0 or <0. */
+int orc_knn(const double* xyz, int n, int k, int* idx, double* d2);
+
+/* Same, for arbitrary query points (self is not excluded). */
+int orc_knn_query(const double* xyz, int n, const double* q, int nq, int k,
+                  int* idx, double* d2);
+
+/* Voronoi (w == NULL) or Laguerre (power distance |x-p|^2 - w) cells of every
+   seed, clipped to box = {xmin,ymin,zmin,xmax,ymax,zmax}.
+   mode 1: security radius with k0 initial neighbours, growing as geogram;
+   mode 0: exactly the k0 nearest neighbours (no security radius);
+   mode 2: all n-1 other seeds (brute force).
+   Outputs: vol[n]; mg[3n] = sum over the cell of x dV (so mg/vol is the
+   centroid); nbr_cnt[n]; nbr / farea [n*stride], contributing neighbours in
+   clipping order and their facet areas.  Returns 0, or -1 if a cell has more
+   than stride neighbours, -2 on allocation failure. */
+int orc_cells(const double* xyz, const double* w, int n, const double* box,
+              int k0, int mode, double* vol, double* mg, int* nbr_cnt,
+              int* nbr, double* farea, int stride, orc_stats* st);
+
+/* Cells of a subset of seeds (cell ids in `cells`, count nc): same outputs,
+   indexed 0..nc-1.  Used for bounded CPU samples. */
+int orc_cells_subset(const double* xyz, const double* w, int n,
+                     const double* box, int k0, int mode, const int* cells,
+                     int nc, double* vol, double* mg, int* nbr_cnt, int* nbr,
+                     double* farea, int stride, orc_stats* st);
+
+/* iters Lloyd iterations in place, security radius on (mode 1, k0). */
+int orc_lloyd(double* xyz, int n, const double* box, int iters, int k0,
+              orc_stats* st);
+
+/* Threads used by the OpenMP loops (0 = runtime default). */
+void orc_set_threads(int nt);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
