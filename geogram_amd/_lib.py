@@ -24,7 +24,7 @@ VGC_ERR_INTERNAL = -4
 VGC_DEVICE_PTRS = 1
 VGC_NO_NEIGHBORS = 2
 
-_STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_uncertain nb_failed nb_empty "
+_STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain nb_failed nb_empty "
              "nb_disconnected nb_clips nb_tests radius_entries").split()
 _STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
 

@@ -48,6 +48,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->nb_neighbors = e.nb_neighbors;
     s->nb_pass_b = e.nb_pass_b;
     s->nb_pass_c = e.nb_pass_c;
+    s->nb_pass_e = e.nb_pass_e;
     s->nb_uncertain = e.nb_uncertain;
     s->nb_failed = e.nb_failed;
     s->nb_empty = e.nb_empty;

@@ -133,23 +133,36 @@ struct EmuBackend {
         }
     }
 
-    template <int S>
+    template <int S, bool E>
     void clip_s(const ClipParams& P) {
 #pragma omp parallel
         {
             std::unique_ptr<CellLds<S>> L(new CellLds<S>());
             std::memset(L.get(), 0, sizeof(CellLds<S>));
 #pragma omp for schedule(dynamic, 16)
-            for (int it = 0; it < P.n_items; ++it) process_cell<S>(P, it, *L);
+            for (int it = 0; it < P.n_items; ++it) process_cell<S, E>(P, it, *L);
         }
     }
 
-    void clip(int S, const ClipParams& P) {
-        if (S == 1)
-            clip_s<1>(P);
-        else
-            clip_s<2>(P);
+    void clip(int S, bool exact, const ClipParams& P) {
+        if (exact) {
+            if (S == 1)
+                clip_s<1, true>(P);
+            else if (S == 2)
+                clip_s<2, true>(P);
+            else
+                clip_s<4, true>(P);
+        } else {
+            if (S == 1)
+                clip_s<1, false>(P);
+            else if (S == 2)
+                clip_s<2, false>(P);
+            else
+                clip_s<4, false>(P);
+        }
     }
+
+    int det4_sign(const double* r) { return det4_sign_exact(r, r + 4, r + 8, r + 12); }
 
     void radius_r2(const int* items, int ni, const pt4* pts, const double* r2, const int* inst_of,
                    const double* wmaxv, int weighted, float* rho2) {
@@ -258,6 +271,9 @@ int vgc_emu_cells_batch(void* ctx, const double* xyz, const double* w, const uin
                         uint64_t cap, vgc_stats* st) {
     return vgc::api_cells((EmuCtx*)ctx, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr,
                           nbr, farea, cap, st);
+}
+int vgc_emu_det4_sign(const double* rows) {
+    return vgc::det4_sign_exact(rows, rows + 4, rows + 8, rows + 12);
 }
 int vgc_emu_lloyd(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
                   uint32_t k0, uint32_t flags, vgc_stats* st) {

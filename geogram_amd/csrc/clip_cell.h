@@ -32,6 +32,7 @@
 #pragma once
 
 #include "wave.h"
+#include "exact_det.h"
 
 namespace vgc {
 
@@ -49,7 +50,12 @@ enum CellFlag : uint32_t {
     CELL_BAD_TOPO = 64u,     // conflict zone was not a disk (only after uncertain signs)
 };
 
-constexpr int kPMax = 128;         // plane slots per cell (8-bit ids)
+// plane slots per cell (8-bit ids): 128 with 64 or 128 corner slots, 256 with
+// 256 corner slots
+template <int S>
+struct PMax {
+    static constexpr int value = S >= 4 ? 256 : 128;
+};
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -63,7 +69,7 @@ struct ClipParams {
     const int* items;      // work items (Morton positions); nullptr = 0..n_items-1
     int n_items;
     const int* list;        // neighbour lists (Morton positions), -1 terminated
-    const long long* list_ptr;  // per item start (nullptr: item * list_stride)
+    const long long* list_ptr;  // per item start (nullptr: cell * list_stride)
     const int* list_len;        // per item length (nullptr: list_stride)
     int list_stride;
     int list_complete;      // lists hold every seed the cell can need
@@ -83,6 +89,7 @@ struct ClipParams {
 
 template <int S>
 struct CellLds {
+    static constexpr int kPMax = PMax<S>::value;
     double pe[kPMax][4];   // e.x e.y e.z h
     int pg[kPMax];         // Morton position of the neighbour, -1 for the box
     int v2t[kPMax];        // first corner of each face (integration)
@@ -226,11 +233,53 @@ VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const p
     }
 }
 
+// Exact sign of det4(Pa, Pb, Pc, Pn) for the lanes in m, with the rows
+// (-e, h) of the corner planes pl and of the new plane: >= 0 means conflict
+// (voronoi/convex_cell.cpp:905-911).  2 when not decidable.
+template <int S>
+VGC_HD LI exact_sign_lanes(const CellLds<S>& L, const LU& pl, const LB& m, double ex, double ey,
+                           double ez, double h) {
+    constexpr int kPMax = PMax<S>::value;
+    LI a = cvt<int>(byte_of(pl, 0)) & (kPMax - 1);
+    LI b = cvt<int>(byte_of(pl, 1)) & (kPMax - 1);
+    LI c = cvt<int>(byte_of(pl, 2)) & (kPMax - 1);
+    const double* pe = &L.pe[0][0];
+    LD a0 = lds_ld(pe, a * 4 + 0), a1 = lds_ld(pe, a * 4 + 1), a2 = lds_ld(pe, a * 4 + 2),
+       a3 = lds_ld(pe, a * 4 + 3);
+    LD b0 = lds_ld(pe, b * 4 + 0), b1 = lds_ld(pe, b * 4 + 1), b2 = lds_ld(pe, b * 4 + 2),
+       b3 = lds_ld(pe, b * 4 + 3);
+    LD c0 = lds_ld(pe, c * 4 + 0), c1 = lds_ld(pe, c * 4 + 1), c2 = lds_ld(pe, c * 4 + 2),
+       c3 = lds_ld(pe, c * 4 + 3);
+#if defined(VGC_EMU)
+    LI r = LI(0);
+    for (int l = 0; l < kWave; ++l) {
+        if (!m[l]) continue;
+        double ra[4] = {-a0[l], -a1[l], -a2[l], a3[l]};
+        double rb[4] = {-b0[l], -b1[l], -b2[l], b3[l]};
+        double rc[4] = {-c0[l], -c1[l], -c2[l], c3[l]};
+        double rn[4] = {-ex, -ey, -ez, h};
+        r[l] = det4_sign_exact(ra, rb, rc, rn);
+    }
+    return r;
+#else
+    int r = 0;
+    if (m) {
+        double ra[4] = {-a0, -a1, -a2, a3};
+        double rb[4] = {-b0, -b1, -b2, b3};
+        double rc[4] = {-c0, -c1, -c2, c3};
+        double rn[4] = {-ex, -ey, -ez, h};
+        r = det4_sign_exact(ra, rb, rc, rn);
+    }
+    return r;
+#endif
+}
+
 // Clip by plane (e, h) stored at slot np with neighbour id j.  Returns the
 // updated flags; *cut tells whether the plane removed a corner.
-template <int S>
+template <int S, bool EXACT>
 VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, double ey,
                            double ez, double h, int j, uint32_t flags, bool& cut) {
+    constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
     const double e1 = vabs(ex) + vabs(ey) + vabs(ez);
     const double ha = vabs(h);
@@ -241,8 +290,18 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
         LD t = vfma(LD(ex), st.nx[s], vfma(LD(ey), st.ny[s], vfma(LD(ez), st.nz[s], -(h * st.d[s]))));
         LD bnd = kGamma9 * vfma(LD(e1), st.nmax[s], ha * st.dabs[s]);
         LB vs = mask_bit(st.valid[s]);
-        cm[s] = ballot(vs & (t >= 0.0));
-        uint64_t um = ballot(vs & (vabs(t) <= bnd));
+        LB conf = t >= 0.0;
+        LB unc = vs & (vabs(t) <= bnd);
+        if (EXACT) {
+            if (any(unc)) {
+                LI sg = exact_sign_lanes<S>(L, st.pl[s], unc, ex, ey, ez, h);
+                LB known = unc & (sg != 2);
+                conf = select(known, sg >= 0, conf);
+                unc = unc & (sg == 2);
+            }
+        }
+        cm[s] = ballot(vs & conf);
+        uint64_t um = ballot(unc);
         anyc = anyc || (cm[s] != 0);
         allc = allc && (cm[s] == st.valid[s]);
         anyu = anyu || (um != 0);
@@ -390,8 +449,9 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
 }
 
 // Security-radius loop + integration for work item `item`.
-template <int S>
+template <int S, bool EXACT>
 VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
+    constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
     wave_sync();  // the previous cell of this wave is done with the LDS
     const int c = P.items ? P.items[item] : item;
@@ -407,7 +467,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         lbeg = P.list_ptr[item];
         llen = P.list_len[item];
     } else {
-        lbeg = (long long)item * P.list_stride;
+        lbeg = (long long)c * P.list_stride;  // n x K lists by Morton position
         llen = P.list_stride;
     }
     bool complete = P.list_complete != 0 || llen >= P.inst_n[inst] - 1;
@@ -418,6 +478,11 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     uint32_t flags = 0;
     bool stopped = false;
     unsigned long long nclip = 0, ntest = 0;
+    // previous candidate: a coincident copy of a seed gives the same plane;
+    // under the exact tie rule the copy takes over the face of the first one
+    pt4 prev;
+    prev.x = prev.y = prev.z = prev.w = NAN;
+    int prev_slot = -1;
     for (int jj = 0; jj < llen; ++jj) {
         const int j = P.list[lbeg + jj];
         if (j < 0) {
@@ -459,10 +524,19 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             stopped = true;
             break;
         }
+        if (pj.x == prev.x && pj.y == prev.y && pj.z == prev.z && pj.w == prev.w) {
+            if (prev_slot >= 0) {
+                lds_st(L.pg, LI(prev_slot), LI(j), lane == 0);
+                wave_sync();
+            }
+            continue;
+        }
+        prev = pj;
         const double h = P.weighted ? 0.5 * (d2 + (wi - pj.w)) : 0.5 * d2;
         bool cut = false;
         ++ntest;
-        flags = clip_plane<S>(st, L, np, ex, ey, ez, h, j, flags, cut);
+        flags = clip_plane<S, EXACT>(st, L, np, ex, ey, ez, h, j, flags, cut);
+        prev_slot = (cut && !(flags & (CELL_EMPTY | CELL_OVERFLOW))) ? np - 1 : -1;
         if (cut) ++nclip;
         if (flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
             stopped = (flags & CELL_EMPTY) != 0;

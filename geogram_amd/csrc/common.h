@@ -6,15 +6,15 @@
 
 #if defined(VGC_EMU)
 #include <cmath>
-#define VGC_SD inline
 #else
 #include <hip/hip_runtime.h>
-#define VGC_SD __host__ __device__ __forceinline__
 #endif
 
 #include "clip_cell.h"
 
 namespace vgc {
+
+VGC_SD int imin(int a, int b) { return a < b ? a : b; }
 
 // Morton-ordered seed in FP32, for the neighbour search.
 struct alignas(16) pf4 {
@@ -93,9 +93,9 @@ struct InstLayout {
 
 // --------------------------------------------------------------- kNN
 // K nearest other seeds of each query (FP32 ranking), sorted by (d2, id);
-// unfilled entries are -1.  Traversal: own bucket and its two Morton
-// neighbours first (tight radius), then the implicit tree from the root,
-// left to right, pruning boxes beyond the current K-th distance.
+// unfilled entries are -1.  Traversal: own bucket and enough Morton-adjacent
+// buckets to hold K seeds first (tight radius), then the implicit tree from
+// the root, left to right, pruning boxes beyond the current K-th distance.
 
 template <int K>
 VGC_SD bool knn_less(float d, int id, float bd, int bi) {
@@ -146,9 +146,24 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
         bd[k] = INFINITY;
         bi[k] = -1;
     }
+    // Morton neighbours first: buckets bq, bq-1, bq+1, bq-2, ... until K
+    // other seeds have been seen, so the tree walk starts with a finite radius
     const int bq = (q - p0) / kBucket;
-    for (int b = bq - 1; b <= bq + 1; ++b)
-        if (b >= 0 && b < nb) knn_scan_bucket<K>(pf, p0, n, b, q, px, py, pz, bd, bi);
+    int blo = bq, bhi = bq;
+    knn_scan_bucket<K>(pf, p0, n, bq, q, px, py, pz, bd, bi);
+    int seen = imin(p0 + n, p0 + (bq + 1) * kBucket) - (p0 + bq * kBucket) - 1;
+    while (seen < K && (blo > 0 || bhi < nb - 1)) {
+        if (blo > 0) {
+            --blo;
+            knn_scan_bucket<K>(pf, p0, n, blo, q, px, py, pz, bd, bi);
+            seen += kBucket;
+        }
+        if (bhi < nb - 1) {
+            ++bhi;
+            knn_scan_bucket<K>(pf, p0, n, bhi, q, px, py, pz, bd, bi);
+            seen += imin(p0 + n, p0 + (bhi + 1) * kBucket) - (p0 + bhi * kBucket);
+        }
+    }
     int node = 1;
     while (true) {
         float lim = bd[K - 1] * 1.00001f;
@@ -159,7 +174,7 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
         }
         if (inside) {
             int b = node - L;
-            if (b < nb && (b < bq - 1 || b > bq + 1))
+            if (b < nb && (b < blo || b > bhi))
                 knn_scan_bucket<K>(pf, p0, n, b, q, px, py, pz, bd, bi);
         }
         while (node & 1) node >>= 1;

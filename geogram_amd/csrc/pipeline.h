@@ -14,7 +14,8 @@
 //      the cell is clipped again: the list is complete by construction.  This
 //      replaces geogram's repeated k += k/8 enlargements
 //      (voronoi/generic_RVD.h:2335-2345) with one exact step.
-//   6. Pass C: cells that overflowed 64 corners, with 128 corner slots.
+//   6. Pass C: cells that overflowed 64 corners, with 128 corner slots, then
+//      256 for the few that overflow again (huge Laguerre cells).
 //   7. Neighbour lists and facet areas compacted into CSR (caller's order).
 #pragma once
 
@@ -30,7 +31,7 @@
 namespace vgc {
 
 struct EngineStats {
-    unsigned long long nb_cells = 0, nb_neighbors = 0, nb_pass_b = 0, nb_pass_c = 0;
+    unsigned long long nb_cells = 0, nb_neighbors = 0, nb_pass_b = 0, nb_pass_c = 0, nb_pass_e = 0;
     unsigned long long nb_uncertain = 0, nb_failed = 0, nb_empty = 0, nb_disconnected = 0;
     unsigned long long nb_clips = 0, nb_tests = 0, radius_entries = 0;
     double t_build_ms = 0, t_knn_ms = 0, t_clip_ms = 0, t_pass_bc_ms = 0, t_compact_ms = 0,
@@ -76,7 +77,7 @@ public:
     int* d_inst_of = nullptr;
     bnode* d_nodes = nullptr;
 
-    static constexpr int kFaceStride = 72;
+    static constexpr int kFaceStride = 136;  // >= 2 + 256/2 faces
 
     // ------------------------------------------------------------ build
     int build(const Problem& pb, EngineStats& st) {
@@ -233,7 +234,7 @@ public:
         P.r2 = d_r2;
         P.counters = d_ctr;
 
-        // pass A
+        // pass A over every cell, then the tiers for the flagged ones
         auto t2 = now();
         P.items = nullptr;
         P.n_items = n_;
@@ -242,11 +243,9 @@ public:
         P.list_len = nullptr;
         P.list_stride = K0;
         P.list_complete = 0;
-        be_.clip(1, P);
+        be_.clip(1, false, P);
         be_.sync();
         st.t_clip_ms += ms_since(t2);
-
-        // passes B and C
         auto t3 = now();
         h_flags.resize(n_);
         be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
@@ -258,18 +257,30 @@ public:
             else if (f & CELL_NEED_K)
                 need.push_back(c);
         }
-        st.nb_pass_b += need.size();
-        if (!need.empty()) {
-            std::vector<int> over_b;
-            if (radius_pass(1, need, P, st)) return -1;
+        if (tiers(need, over, false, P, st)) return -1;
+        // cells with signs inside their error bound: exact re-clip
+        std::vector<int> unsure;
+        for (int c = 0; c < n_; ++c)
+            if (h_flags[c] & (CELL_UNCERTAIN | CELL_BAD_TOPO)) unsure.push_back(c);
+        st.nb_pass_e += unsure.size();
+        if (!unsure.empty()) {
+            int ni = (int)unsure.size();
+            int* d_items = (int*)slot(S_ITEMS, sizeof(int) * ni);
+            be_.h2d(d_items, unsure.data(), sizeof(int) * ni);
+            ClipParams Q = P;
+            Q.items = d_items;
+            Q.n_items = ni;
+            be_.clip(1, true, Q);
             be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
-            for (int c : need)
-                if (h_flags[c] & CELL_OVERFLOW) over.push_back(c);
-        }
-        st.nb_pass_c += over.size();
-        if (!over.empty()) {
-            if (radius_pass(2, over, P, st)) return -1;
-            be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+            std::vector<int> need2, over2;
+            for (int c : unsure) {
+                uint32_t f = h_flags[c];
+                if (f & CELL_OVERFLOW)
+                    over2.push_back(c);
+                else if (f & CELL_NEED_K)
+                    need2.push_back(c);
+            }
+            if (tiers(need2, over2, true, P, st)) return -1;
         }
         st.t_pass_bc_ms += ms_since(t3);
 
@@ -343,9 +354,37 @@ private:
         return b.p;
     }
 
+    // Pass B for `need` (complete radius lists, 64 corner slots), then pass C
+    // with 128 and 256 corner slots for the cells that overflow.  h_flags is
+    // refreshed.
+    int tiers(const std::vector<int>& need, std::vector<int> over, bool exact, const ClipParams& P,
+              EngineStats& st) {
+        st.nb_pass_b += need.size();
+        if (!need.empty()) {
+            if (radius_pass(1, exact, need, P, st)) return -1;
+            be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+            for (int c : need)
+                if (h_flags[c] & CELL_OVERFLOW) over.push_back(c);
+        }
+        st.nb_pass_c += over.size();
+        if (!over.empty()) {
+            if (radius_pass(2, exact, over, P, st)) return -1;
+            be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+            std::vector<int> over4;
+            for (int c : over)
+                if (h_flags[c] & CELL_OVERFLOW) over4.push_back(c);
+            if (!over4.empty()) {
+                if (radius_pass(4, exact, over4, P, st)) return -1;
+                be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+            }
+        }
+        return 0;
+    }
+
     // Complete lists for `items`: every seed within the distance that can
     // still cut the partial cell of the previous pass.
-    int radius_pass(int S, const std::vector<int>& items, ClipParams P, EngineStats& st) {
+    int radius_pass(int S, bool exact, const std::vector<int>& items, ClipParams P,
+                    EngineStats& st) {
         const int ni = (int)items.size();
         int* d_items = (int*)slot(S_ITEMS, sizeof(int) * ni);
         be_.h2d(d_items, items.data(), sizeof(int) * ni);
@@ -372,7 +411,7 @@ private:
         P.list_len = d_rcnt;
         P.list_stride = 0;
         P.list_complete = 1;
-        be_.clip(S, P);
+        be_.clip(S, exact, P);
         be_.sync();
         return 0;
     }

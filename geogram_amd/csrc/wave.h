@@ -26,6 +26,7 @@
 #include <cstring>
 
 #define VGC_HD inline
+#define VGC_SD inline
 #define VGC_DEVICE_ONLY inline
 
 namespace vgc {
@@ -305,6 +306,7 @@ inline void atomic_add_u64(unsigned long long* p, unsigned long long v) {
 #include <hip/hip_runtime.h>
 
 #define VGC_HD __device__ __forceinline__
+#define VGC_SD __host__ __device__ __forceinline__
 #define VGC_DEVICE_ONLY __device__ __forceinline__
 
 namespace vgc {

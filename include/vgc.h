@@ -39,8 +39,9 @@ typedef struct vgc_stats {
     uint64_t nb_cells;        /* cells computed                                   */
     uint64_t nb_neighbors;    /* total CSR entries                                */
     uint64_t nb_pass_b;       /* cells re-clipped with a complete radius list     */
-    uint64_t nb_pass_c;       /* cells re-clipped with 128 corner slots           */
-    uint64_t nb_uncertain;    /* cells with a sign inside its error bound         */
+    uint64_t nb_pass_c;       /* cells re-clipped with 128 / 256 corner slots     */
+    uint64_t nb_pass_e;       /* cells re-clipped with exact signs (ties)         */
+    uint64_t nb_uncertain;    /* cells left with a sign not decided exactly       */
     uint64_t nb_failed;       /* cells without valid output                       */
     uint64_t nb_empty;        /* empty Laguerre cells                             */
     uint64_t nb_disconnected; /* cells left unclipped by the coincident-seed rule */

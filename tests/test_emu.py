@@ -1,0 +1,179 @@
+"""The gfx950 kernels' logic, run through their CPU emulation (libvgc_emu.so:
+the same clip_cell.h / common.h / pipeline.h sources with the wave layer
+emulated lane by lane), against the oracle.  The GPU tests then check that the
+device reproduces the emulation bit for bit."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+import geogram_amd as ga
+import oracle
+
+REL = 1e-10
+
+
+def compare(r, o, n, rel=REL):
+    vrel = np.abs(r.vol - o["vol"]) / np.maximum(o["vol"], 1e-300)
+    assert vrel.max() < rel, f"volume rel err {vrel.max()} at {vrel.argmax()}"
+    cen = r.mg / r.vol[:, None]
+    ocen = o["mg"] / o["vol"][:, None]
+    assert (np.abs(cen - ocen) / np.cbrt(o["vol"])[:, None]).max() < rel
+    bad = []
+    for i in range(n):
+        a = r.neighbors(i)
+        b = o["nbr"][i, : o["cnt"][i]]
+        if set(a.tolist()) != set(b.tolist()):
+            bad.append(i)
+            continue
+        da = dict(zip(a.tolist(), r.facet_areas(i).tolist()))
+        for j, ar in zip(b.tolist(), o["farea"][i, : o["cnt"][i]].tolist()):
+            # facet areas: 1e-10 relative, or 2e-11 of the cell's face scale
+            # (the reference's global plane offset |pj|^2-|pi|^2 carries
+            # ~3e-16 absolute rounding, amplified at ill-conditioned corners)
+            assert abs(da[j] - ar) <= rel * ar + 2e-11 * o["vol"][i] ** (2 / 3)
+    assert not bad, f"{len(bad)} neighbour sets differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("kind,n,k0", [(0, 2000, 32), (0, 4000, 64), (1, 2500, 48), (2, 2000, 16)])
+def test_cells_vs_oracle(emu_ctx, kind, n, k0):
+    x = ga.sample_points(kind, 100 + kind, n)
+    r = emu_ctx.cells(x, k0=k0)
+    assert r.stats["nb_failed"] == 0 and r.stats["nb_uncertain"] == 0
+    o = oracle.cells(x, k0=20)
+    compare(r, o, n)
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+def test_laguerre_vs_oracle(emu_ctx):
+    rng = np.random.default_rng(3)
+    n = 2500
+    x = rng.random((n, 3))
+    w = rng.random(n) * 4e-4
+    r = emu_ctx.cells(x, weights=w, k0=32)
+    o = oracle.cells(x, w=w, k0=20)
+    compare(r, o, n)
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+def test_laguerre_empty_cells(emu_ctx):
+    rng = np.random.default_rng(4)
+    n = 800
+    x = rng.random((n, 3))
+    w = np.zeros(n)
+    w[:10] = 0.05   # ten heavy seeds swallow their neighbours
+    r = emu_ctx.cells(x, weights=w, k0=32)
+    o = oracle.cells(x, w=w, k0=20, stride=512)
+    assert r.stats["nb_empty"] == o["stats"]["nb_empty"] > 0
+    nz = o["vol"] > 0
+    assert np.array_equal(r.vol == 0, ~nz)
+    assert (np.abs(r.vol[nz] - o["vol"][nz]) / o["vol"][nz]).max() < REL
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+def test_batch_of_instances(emu_ctx):
+    sizes = [300, 1700, 50, 1000]
+    xs = [ga.sample_points(i % 3, 40 + i, s) for i, s in enumerate(sizes)]
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    boxes[2] = [-1, -1, -1, 2, 2, 2]
+    x = np.vstack(xs)
+    r = emu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes)
+    off = np.r_[0, np.cumsum(sizes)]
+    for i, s in enumerate(sizes):
+        o = oracle.cells(xs[i], box=boxes[i], k0=20)
+        sl = slice(off[i], off[i + 1])
+        assert (np.abs(r.vol[sl] - o["vol"]) / o["vol"]).max() < REL
+        assert abs(r.vol[sl].sum() - np.prod(boxes[i, 3:] - boxes[i, :3])) < 1e-11
+        for c in range(s):
+            got = set((r.neighbors(off[i] + c) - off[i]).tolist())
+            assert got == set(o["nbr"][c, : o["cnt"][c]].tolist())
+
+
+def test_small_and_coincident(emu_ctx):
+    # n smaller than k0, and a coincident pair (delaunay_nn.cpp:125-133)
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 5, 17):
+        x = rng.random((n, 3))
+        r = emu_ctx.cells(x, k0=32)
+        o = oracle.cells(x, k0=20)
+        assert np.allclose(r.vol, o["vol"], rtol=1e-12)
+        assert abs(r.vol.sum() - 1) < 1e-13
+    x = rng.random((300, 3))
+    x[200] = x[17]
+    r = emu_ctx.cells(x, k0=32)
+    o = oracle.cells(x, k0=20)
+    assert r.stats["nb_disconnected"] == 1
+    assert np.allclose(r.vol, o["vol"], rtol=1e-12)
+    assert oracle.neighbour_sets(o) == [frozenset(r.neighbors(i).tolist()) for i in range(300)]
+
+
+def test_lattice_ties_exact_pass(emu_ctx):
+    # exact lattices: every corner is shared by 8 cells, so many signs are ties
+    # (det4 = 0); the cells are re-clipped with exact signs and the tie rule
+    # (ties conflict, convex_cell.cpp:905-911)
+    for m in (5, 8):
+        g = (np.arange(m) + 0.5) / m
+        x = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+        r = emu_ctx.cells(x, k0=32)
+        o = oracle.cells(x, k0=20)
+        assert r.stats["nb_pass_e"] > 0
+        assert r.stats["nb_uncertain"] == 0 and r.stats["nb_failed"] == 0
+        assert np.allclose(r.vol, 1.0 / m**3, rtol=1e-14)
+        if m == 8:
+            # dyadic coordinates: seed-relative and absolute planes are both
+            # exact, so the tie resolution, hence the neighbour sets (zero-area
+            # facets included), match the oracle's exactly
+            assert oracle.neighbour_sets(o) == [frozenset(r.neighbors(i).tolist())
+                                                for i in range(x.shape[0])]
+
+
+def test_det4_sign_exact_integer():
+    # the device's fixed-point evaluator against rational arithmetic
+    from tests.test_oracle import det4_exact
+    lib = ctypes.CDLL(ga._lib.EMU_PATH)
+    lib.vgc_emu_det4_sign.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(21)
+    for t in range(400):
+        R = rng.standard_normal((4, 4)) * (10.0 ** rng.integers(-8, 3, (4, 4)))
+        if t % 2:
+            # exactly singular with dyadic combination
+            R[3] = 2 * R[0] - 0.5 * R[1] + R[2]
+            R[3] = R[3]  # rounded combination: the sign is decided exactly
+        R = np.ascontiguousarray(R)
+        assert lib.vgc_emu_det4_sign(R.ctypes.data) == det4_exact(R)
+    # exact zeros
+    for _ in range(100):
+        R = np.round(rng.standard_normal((3, 4)) * 2**10) / 2**10
+        M = np.ascontiguousarray(np.vstack([R, R[0] + R[1] - R[2]]))
+        assert det4_exact(M) == 0 and lib.vgc_emu_det4_sign(M.ctypes.data) == 0
+
+
+def test_lloyd_vs_oracle(emu_ctx):
+    x = ga.sample_points(0, 77, 3000)
+    xg, st = emu_ctx.lloyd(x, 4, k0=32)
+    xo, _ = oracle.lloyd(x, 4)
+    assert np.abs(xg - xo).max() < 1e-12
+
+
+def test_knn_vs_kdtree(emu_ctx):
+    x = ga.sample_points(1, 9, 5000)
+    for k in (16, 64):
+        idx, d2 = emu_ctx.knn(x, k)
+        dd, ii = cKDTree(x).query(x, k + 1)
+        # FP32 ranking: sets agree except at FP64 near-ties of the k-th distance
+        for i in range(x.shape[0]):
+            if set(idx[i]) != set(ii[i, 1:]):
+                kth, nxt = cKDTree(x).query(x[i], k + 2)[0][k:k + 2]
+                assert (nxt - kth) <= 1e-5 * kth
+        assert np.allclose(d2, dd[:, 1:] ** 2, rtol=2e-5)
+
+
+def test_random_float64_is_mt19937_64():
+    # C++ [rand.predef]: the 10000th output of a default-constructed
+    # mt19937_64 (seed 5489) is 9981545732273789042
+    out = ga.random_float64(5489, 10000)
+    assert out[9999] == np.float64(9981545732273789042) / 2.0**64
+    assert np.all((out >= 0) & (out < 1))
+    assert np.array_equal(ga.random_float64(1, 5), ga.random_float64(1, 5))

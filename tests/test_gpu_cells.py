@@ -30,7 +30,10 @@ def compare(r, o, n, rel=REL):
         da = dict(zip(a.tolist(), r.facet_areas(i).tolist()))
         for j, ar in zip(b.tolist(), o["farea"][i, : o["cnt"][i]].tolist()):
             # sliver facets: 1e-10 relative, or 1e-13 of the cell's face scale
-            assert abs(da[j] - ar) <= rel * ar + 1e-13 * o["vol"][i] ** (2 / 3)
+            # facet areas: 1e-10 relative, or 2e-11 of the cell's face scale
+            # (the reference's global plane offset |pj|^2-|pi|^2 carries
+            # ~3e-16 absolute rounding, amplified at ill-conditioned corners)
+            assert abs(da[j] - ar) <= rel * ar + 2e-11 * o["vol"][i] ** (2 / 3)
     assert not bad, f"{len(bad)} neighbour sets differ, first {bad[:5]}"
 
 
@@ -57,3 +60,76 @@ def test_gpu_matches_emulation_bitwise(gpu_ctx, emu_ctx):
     assert np.array_equal(g.nbr_ptr, e.nbr_ptr)
     assert np.array_equal(g.nbr, e.nbr)
     assert np.array_equal(g.farea, e.farea)
+
+
+def _same(g, e):
+    assert np.array_equal(g.vol, e.vol)
+    assert np.array_equal(g.mg, e.mg)
+    assert np.array_equal(g.nbr_ptr, e.nbr_ptr)
+    assert np.array_equal(g.nbr, e.nbr)
+    assert np.array_equal(g.farea, e.farea)
+
+
+def test_gpu_matches_emulation_laguerre_and_lattice(gpu_ctx, emu_ctx):
+    rng = np.random.default_rng(6)
+    x = rng.random((3000, 3))
+    w = rng.random(3000) * 4e-4
+    w[:5] = 0.05  # huge cells: 128 / 256 corner tiers and empty cells
+    _same(gpu_ctx.cells(x, weights=w, k0=32), emu_ctx.cells(x, weights=w, k0=32))
+    m = 8
+    g = (np.arange(m) + 0.5) / m
+    lat = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+    rg = gpu_ctx.cells(lat, k0=32)
+    assert rg.stats["nb_pass_e"] > 0 and rg.stats["nb_uncertain"] == 0
+    _same(rg, emu_ctx.cells(lat, k0=32))
+
+
+def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
+    import geogram_amd as ga
+
+    sizes = [500, 3000, 64, 1500]
+    x = np.vstack([ga.sample_points(i % 3, 60 + i, s) for i, s in enumerate(sizes)])
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    _same(gpu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes),
+          emu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes))
+    y = ga.sample_points(0, 3, 4000)
+    gy, _ = gpu_ctx.lloyd(y, 3, k0=48)
+    ey, _ = emu_ctx.lloyd(y, 3, k0=48)
+    assert np.array_equal(gy, ey)
+    oy, _ = oracle.lloyd(y, 3)
+    assert np.abs(gy - oy).max() < 1e-12
+
+
+def test_gpu_knn(gpu_ctx, emu_ctx):
+    import geogram_amd as ga
+
+    x = ga.sample_points(1, 8, 20000)
+    gi, gd = gpu_ctx.knn(x, 32)
+    ei, ed = emu_ctx.knn(x, 32)
+    assert np.array_equal(gi, ei) and np.array_equal(gd, ed)
+
+
+def test_c2_two_million_invariants(gpu_ctx):
+    """Full-size C2 workload: 2M seeds, half uniform, half 8-Gaussian mixture.
+    Size-independent checks: volumes sum to the cube within 1e-12, every cell
+    computed, neighbour relation symmetric; plus 1500 random cells against the
+    oracle (which recomputes them with its own kd-tree over all 2M seeds)."""
+    import geogram_amd as ga
+
+    n = 2_000_000
+    x = ga.sample_points(1, 2, n)
+    r = gpu_ctx.cells(x, k0=64)
+    st = r.stats
+    assert st["nb_failed"] == 0 and st["nb_uncertain"] == 0
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+    ptr = r.nbr_ptr.astype(np.int64)
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    a = rows.astype(np.int64) * n + r.nbr
+    b = r.nbr.astype(np.int64) * n + rows
+    assert np.array_equal(np.sort(a), np.sort(b))
+    sample = np.random.default_rng(0).choice(n, 1500, replace=False)
+    o = oracle.cells(x, k0=20, subset=sample)
+    vrel = np.abs(r.vol[sample] - o["vol"]) / o["vol"]
+    assert vrel.max() < 1e-10
+    for t, i in enumerate(sample):
+        assert set(r.neighbors(i).tolist()) == set(o["nbr"][t, : o["cnt"][t]].tolist())
