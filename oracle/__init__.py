@@ -62,6 +62,7 @@ def lib():
         L.orc_det3_sign.argtypes = [dp, dp, dp]
         L.orc_exact_fallbacks.restype = ctypes.c_longlong
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_local_planes.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -101,10 +102,13 @@ def knn(xyz, k):
     return idx, d2
 
 
-def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None):
+def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None,
+          centred=False):
     """Per-cell volume, mg (= integral of x), neighbour lists and facet areas.
 
+    centred=True forms the planes in the seed-centred frame (same algorithm).
     Returns dict(vol, mg, cnt, nbr, farea, stats); nbr/farea are (n, stride)."""
+    lib().orc_set_local_planes(1 if centred else 0)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     n = xyz.shape[0]
     box = np.ascontiguousarray(box, dtype=np.float64)

@@ -16,6 +16,8 @@
 #endif
 
 static int g_threads = 0;
+static int g_local = 0;
+void orc_set_local_planes(int on) { g_local = on; }
 void orc_set_threads(int nt) { g_threads = nt; }
 int orc_get_threads(void) {
 #ifdef _OPENMP
@@ -696,11 +698,24 @@ typedef struct {
 static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
                        double* vol, double* mg, int* cnt, int* nbr,
                        double* farea, int stride, orc_stats* st) {
-    const double* pi = J->xyz + 3 * (size_t)i;
+    const double* pi_abs = J->xyz + 3 * (size_t)i;
+    /* seed-centred frame (optional): the same planes and box translated by
+       -pi before they are formed, so the offsets carry no |pi|^2 rounding */
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    const double* pi = g_local ? zero3 : pi_abs;
+    double box_l[6];
+    const double* box = J->box;
+    if (g_local) {
+        for (int c = 0; c < 3; ++c) {
+            box_l[c] = J->box[c] - pi_abs[c];
+            box_l[3 + c] = J->box[3 + c] - pi_abs[c];
+        }
+        box = box_l;
+    }
     double wi = J->w ? J->w[i] : 0.0;
     double c = J->w ? (wi - J->wmax) : 0.0;
     int weighted = J->w != NULL;
-    cell_init_box(C, J->box);
+    cell_init_box(C, box);
     int n = J->n;
     int k = J->k0;
     if (J->mode == 2) k = n - 1;
@@ -716,7 +731,13 @@ static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
         if (prev == nb) break;
         for (; jj < nb; ++jj) {
             int j = S->nb[jj];
-            const double* pj = J->xyz + 3 * (size_t)j;
+            const double* pj_abs = J->xyz + 3 * (size_t)j;
+            double pj_l[3];
+            const double* pj = pj_abs;
+            if (g_local) {
+                for (int c = 0; c < 3; ++c) pj_l[c] = pj_abs[c] - pi_abs[c];
+                pj = pj_l;
+            }
             double R2 = cell_sq_radius(C, pi);
             double dij = dist2(pi, pj);
             if (J->mode != 2 && sr_reached(dij, R2, c, weighted)) {
@@ -752,6 +773,8 @@ static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
     }
     cell_geometry(C);
     cell_volume_mg(C, vol, mg);
+    if (g_local)
+        for (int c = 0; c < 3; ++c) mg[c] += *vol * pi_abs[c];
     for (int v = 1; v < C->nv; ++v) {
         if (C->v2t[v] < 0 || C->vg[v] < 0) continue;
         if (*cnt >= stride) return -1;

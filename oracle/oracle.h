@@ -71,6 +71,12 @@ int orc_cells_subset(const double* xyz, const double* w, int n,
 int orc_lloyd(double* xyz, int n, const double* box, int iters, int k0,
               orc_stats* st);
 
+/* 1: form every cell's planes in the seed-centred frame (coordinates
+   translated by -pi first), 0 (default): absolute coordinates, exactly as
+   copy_Laguerre_cell_facet_from_Delaunay does.  Same algorithm; the centred
+   frame avoids the |pi|^2 rounding of the plane offsets on tiny cells. */
+void orc_set_local_planes(int on);
+
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int nt);
 int orc_get_threads(void);

@@ -128,8 +128,16 @@ def test_c2_two_million_invariants(gpu_ctx):
     b = r.nbr.astype(np.int64) * n + rows
     assert np.array_equal(np.sort(a), np.sort(b))
     sample = np.random.default_rng(0).choice(n, 1500, replace=False)
-    o = oracle.cells(x, k0=20, subset=sample)
-    vrel = np.abs(r.vol[sample] - o["vol"]) / o["vol"]
+    # seed-centred oracle frame: 1e-10 relative
+    oc = oracle.cells(x, k0=20, subset=sample, centred=True)
+    vrel = np.abs(r.vol[sample] - oc["vol"]) / oc["vol"]
     assert vrel.max() < 1e-10
+    # reference-faithful absolute frame: its plane offset |pj|^2 - |pi|^2
+    # rounds to ~1.3e-15 absolute, i.e. ~4e-15 / vol^(2/3) relative in volume
+    # on the tiny cells of the sigma = 0.01 clusters
+    o = oracle.cells(x, k0=20, subset=sample)
+    vg = r.vol[sample]
+    assert np.all(np.abs(vg - o["vol"]) <= o["vol"] * (1e-10 + 4e-15 / o["vol"] ** (2 / 3)))
     for t, i in enumerate(sample):
         assert set(r.neighbors(i).tolist()) == set(o["nbr"][t, : o["cnt"][t]].tolist())
+        assert set(r.neighbors(i).tolist()) == set(oc["nbr"][t, : oc["cnt"][t]].tolist())
