@@ -1,0 +1,147 @@
+"""Benchmark: Voronoi cells per second on one MI355X.
+
+Workload (N=1): 2,000,000 seeds in [0,1]^3, 1M uniform + 1M from an
+8-component isotropic Gaussian mixture (means U[0.2,0.8]^3, sigma U[0.01,0.1],
+rejection to the cube), seed 2.  One step = one full pass of the cell path on
+seeds already resident in HBM: Morton sort, tree, kNN, clipping with the
+security radius (all tiers), FP64 volume / centroid / facet-area integration
+and the neighbour CSR.  Steps are timed with torch.cuda.synchronize() on both
+sides.
+
+The JSON line also carries:
+  roofline      the first clipping pass (one launch over every cell, the
+                dominant kernel): algorithmic FLOPs of that launch
+                (sum over cells of 7 V + 66 C + 34 T + 20 F: 7 flops per
+                corner-plane test, 66 per corner from three planes, 34 per
+                integration tetrahedron, 20 per facet), divided by its
+                duration from HIP events recorded on the library's launch
+                stream, against the FP64 vector peak of MI355X (78.6 TFLOP/s);
+  cpu_baseline  the oracle (a plain-C restatement of geogram's VBW cell +
+                security-radius loop with exact predicates) on the host cores,
+                on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--k0", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=400_000,
+                    help="cells of the workload timed on the host with the oracle (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by Morton ranges with a seed halo, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import geogram_amd as ga
+
+    n = args.n
+    x = ga.sample_points(1, 2, n)
+    xd = torch.from_numpy(x).cuda()
+    ctx = ga.Context(0)
+
+    for _ in range(args.warmup):
+        r = ctx.cells(xd, k0=args.k0)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.cells(xd, k0=args.k0)
+        stats.append(r.stats)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = n * args.steps / dt
+
+    sumvol_err = abs(float(r.vol.sum().item()) - 1.0)
+    s = stats[-1]
+    flops_a = 7 * s["a_live"] + 66 * s["a_new"] + 34 * s["a_tets"] + 20 * s["a_facets"]
+    clip_ms = float(np.mean([t["t_clip_a_gpu_ms"] for t in stats]))
+    achieved = flops_a / (clip_ms * 1e-3) / 1e12
+    roofline = {
+        "kernel": "k_clip<1,4,false> (first clipping pass, every cell)",
+        "bound": "valu",
+        "achieved": round(achieved, 4),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+        "traffic": None,
+        "alg_flops_per_launch": int(flops_a),
+        "flops_per_cell": round(flops_a / n, 1),
+        "launch_ms_hip_events": round(clip_ms, 3),
+    }
+
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle
+
+        sample = np.random.default_rng(0).choice(n, min(args.cpu_sample, n), replace=False)
+        sample.sort()
+        oracle.cells(x[:1000], k0=20)  # load
+        t1 = time.perf_counter()
+        oracle.cells(x, k0=20, subset=sample)
+        tc = time.perf_counter() - t1
+        cpu = {
+            "value": round(len(sample) / tc, 1),
+            "unit": "cells/s",
+            "cores": oracle.get_threads(),
+            "kind": "port",
+            "sample": f"{len(sample)} random cells of the same 2M workload, kd-tree over all "
+                      f"2M seeds included ({tc:.2f} s)",
+        }
+
+    out = {
+        "metric": "Voronoi cells/sec; 1M-seed OT time to 1e-4 mass err (learned warm start vs plain)",
+        "value": round(value, 1),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "2M seeds in [0,1]^3 (1M uniform + 1M 8-Gaussian mixture, seed 2), "
+                        "one pass of Voronoi cells: volume, centroid, neighbours, facet areas",
+            "n": n,
+            "k0": args.k0,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "checks": {"sum_vol_minus_1": sumvol_err, "failed": int(s["nb_failed"]),
+                   "uncertain": int(s["nb_uncertain"])},
+        "phases_ms": {k: round(s[k], 3) for k in ("t_build_ms", "t_knn_ms", "t_clip_ms",
+                                                  "t_pass_bc_ms", "t_compact_ms", "t_total_ms",
+                                                  "t_clip_a_gpu_ms", "t_knn_gpu_ms")},
+        "tiers": {k: int(s[k]) for k in ("nb_pass_b", "nb_pass_c", "nb_pass_e", "radius_entries",
+                                         "nb_tests", "nb_clips")},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -27,10 +27,13 @@ VGC_NO_NEIGHBORS = 2
 _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain nb_failed nb_empty "
              "nb_disconnected nb_clips nb_tests radius_entries").split()
 _STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
+_STAT_F64B = "t_clip_a_gpu_ms t_knn_gpu_ms".split()
+_STAT_U64B = "a_cells a_tests a_live a_new a_tets a_facets".split()
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_uint64) for k in _STAT_U64] + [(k, ctypes.c_double) for k in _STAT_F64]
+    _fields_ = ([(k, ctypes.c_uint64) for k in _STAT_U64] + [(k, ctypes.c_double) for k in _STAT_F64]
+                + [(k, ctypes.c_double) for k in _STAT_F64B] + [(k, ctypes.c_uint64) for k in _STAT_U64B])
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

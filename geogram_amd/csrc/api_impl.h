@@ -62,6 +62,14 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->t_pass_bc_ms = e.t_pass_bc_ms;
     s->t_compact_ms = e.t_compact_ms;
     s->t_total_ms = e.t_total_ms;
+    s->t_clip_a_gpu_ms = e.t_clip_a_gpu_ms;
+    s->t_knn_gpu_ms = e.t_knn_gpu_ms;
+    s->a_cells = e.a_cells;
+    s->a_tests = e.a_tests;
+    s->a_live = e.a_live;
+    s->a_new = e.a_new;
+    s->a_tets = e.a_tets;
+    s->a_facets = e.a_facets;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }

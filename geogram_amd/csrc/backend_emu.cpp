@@ -31,6 +31,9 @@ struct EmuBackend {
     void d2d(void* d, const void* s, size_t b) { std::memmove(d, s, b); }
     void dzero(void* d, size_t b) { std::memset(d, 0, b); }
     void sync() {}
+    void event_begin(int) {}
+    void event_end(int) {}
+    double event_ms(int) { return 0.0; }
 
     void fill_inst(int* inst_orig, int n, const int* p0, int ni) {
         for (int i = 0; i < n; ++i) {

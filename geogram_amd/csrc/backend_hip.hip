@@ -219,6 +219,9 @@ struct HipBackend {
         VGC_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     }
     ~HipBackend() {
+        for (auto& e : ev_)
+            for (auto& x : e)
+                if (x) (void)hipEventDestroy(x);
         if (temp_) (void)hipFree(temp_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -243,6 +246,21 @@ struct HipBackend {
     }
     void dzero(void* d, size_t b) { VGC_HIP_CHECK(hipMemsetAsync(d, 0, b, stream_)); }
     void sync() { VGC_HIP_CHECK(hipStreamSynchronize(stream_)); }
+    // HIP events on the launch stream around a phase (slot 0: kNN, 1: pass A)
+    void event_begin(int k) {
+        if (!ev_[k][0]) {
+            VGC_HIP_CHECK(hipEventCreate(&ev_[k][0]));
+            VGC_HIP_CHECK(hipEventCreate(&ev_[k][1]));
+        }
+        VGC_HIP_CHECK(hipEventRecord(ev_[k][0], stream_));
+    }
+    void event_end(int k) { VGC_HIP_CHECK(hipEventRecord(ev_[k][1], stream_)); }
+    double event_ms(int k) {
+        float ms = 0.f;
+        VGC_HIP_CHECK(hipEventSynchronize(ev_[k][1]));
+        VGC_HIP_CHECK(hipEventElapsedTime(&ms, ev_[k][0], ev_[k][1]));
+        return ms;
+    }
     void check() { VGC_HIP_CHECK(hipGetLastError()); }
 
     void* temp(size_t b) {
@@ -382,6 +400,7 @@ struct HipBackend {
 private:
     int dev_;
     hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     void* temp_ = nullptr;
     size_t temp_bytes_ = 0;
 };

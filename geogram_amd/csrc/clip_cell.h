@@ -84,7 +84,9 @@ struct ClipParams {
     // per item (Morton position) bookkeeping
     uint32_t* flags;        // indexed by Morton position
     double* r2;             // squared radius of the (partial) cell, by Morton position
-    unsigned long long* counters;  // [0] clips, [1] planes tested
+    // [0] planes that cut, [1] planes tested, [2] live corners at each test,
+    // [3] corners created, [4] integration tetrahedra, [5] facets
+    unsigned long long* counters;
 };
 
 template <int S>
@@ -278,13 +280,16 @@ VGC_HD LI exact_sign_lanes(const CellLds<S>& L, const LU& pl, const LB& m, doubl
 // updated flags; *cut tells whether the plane removed a corner.
 template <int S, bool EXACT>
 VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, double ey,
-                           double ez, double h, int j, uint32_t flags, bool& cut) {
+                           double ez, double h, int j, uint32_t flags, bool& cut,
+                           unsigned long long& nlive, unsigned long long& nnew) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
     const double e1 = vabs(ex) + vabs(ey) + vabs(ez);
     const double ha = vabs(h);
     uint64_t cm[S];
     bool anyc = false, allc = true, anyu = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) nlive += (unsigned long long)popc64(st.valid[s]);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LD t = vfma(LD(ex), st.nx[s], vfma(LD(ey), st.ny[s], vfma(LD(ez), st.nz[s], -(h * st.d[s]))));
@@ -348,6 +353,7 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
         nfree += popc64(fr[s]);
     }
     if (total > nfree) return flags | CELL_OVERFLOW;
+    nnew += (unsigned long long)total;
     // jobs in (edge k, slot, lane) order
     {
         int base = 0;
@@ -477,7 +483,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     int np = 6;
     uint32_t flags = 0;
     bool stopped = false;
-    unsigned long long nclip = 0, ntest = 0;
+    unsigned long long nclip = 0, ntest = 0, nlive = 0, nnew = 0;
     // previous candidate: a coincident copy of a seed gives the same plane;
     // under the exact tie rule the copy takes over the face of the first one
     pt4 prev;
@@ -535,7 +541,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         const double h = P.weighted ? 0.5 * (d2 + (wi - pj.w)) : 0.5 * d2;
         bool cut = false;
         ++ntest;
-        flags = clip_plane<S, EXACT>(st, L, np, ex, ey, ez, h, j, flags, cut);
+        flags = clip_plane<S, EXACT>(st, L, np, ex, ey, ez, h, j, flags, cut, nlive, nnew);
         prev_slot = (cut && !(flags & (CELL_EMPTY | CELL_OVERFLOW))) ? np - 1 : -1;
         if (cut) ++nclip;
         if (flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
@@ -554,6 +560,8 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (P.counters && lane0()) {
         atomic_add_u64(P.counters + 0, nclip);
         atomic_add_u64(P.counters + 1, ntest);
+        atomic_add_u64(P.counters + 2, nlive);
+        atomic_add_u64(P.counters + 3, nnew);
     }
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         if (lane0()) {
@@ -611,6 +619,8 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     const double ox = L.q[cid0][0], oy = L.q[cid0][1], oz = L.q[cid0][2];
 
     LD vacc = LD(0.0), mxa = LD(0.0), mya = LD(0.0), mza = LD(0.0);
+    LD ntet = LD(0.0);
+    int nfacets = 0;
     int out_base = 0;
     bool face_overflow = false;
     const long long obase = (long long)orig_c * P.face_stride;
@@ -651,6 +661,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LD bx = qx - q0x, by = qy - q0y, bz = qz - q0z;
             LD wx2 = ay * bz - az * by, wy2 = az * bx - ax * bz, wz2 = ax * by - ay * bx;
             LD ta = 0.5 * vsqrt(wx2 * wx2 + wy2 * wy2 + wz2 * wz2);
+            ntet = select(run, ntet + 1.0, ntet);
             fv = select(run, fv + tv, fv);
             fmx = select(run, fmx + tv * (ox + q0x + q1x + qx) / 4.0, fmx);
             fmy = select(run, fmy + tv * (oy + q0y + q1y + qy) / 4.0, fmy);
@@ -667,6 +678,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         mxa = mxa + fmx;
         mya = mya + fmy;
         mza = mza + fmz;
+        nfacets += popc64(ballot(act));
         LI pgf = lds_ld(L.pg, fs);
         LB outm = act & (pgf >= 0);
         uint64_t om = ballot(outm);
@@ -696,6 +708,11 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         out_base += popc64(om);
     }
     const double V = wave_sum(vacc);
+    const double ntets = wave_sum(ntet);
+    if (P.counters && lane0()) {
+        atomic_add_u64(P.counters + 4, (unsigned long long)ntets);
+        atomic_add_u64(P.counters + 5, (unsigned long long)nfacets);
+    }
     const double Mx = wave_sum(mxa), My = wave_sum(mya), Mz = wave_sum(mza);
     if (face_overflow) flags |= CELL_OVERFLOW;
     if (lane0()) {

@@ -36,6 +36,11 @@ struct EngineStats {
     unsigned long long nb_clips = 0, nb_tests = 0, radius_entries = 0;
     double t_build_ms = 0, t_knn_ms = 0, t_clip_ms = 0, t_pass_bc_ms = 0, t_compact_ms = 0,
            t_total_ms = 0;
+    // pass A clip launch: GPU time (events on the launch stream) and the
+    // algorithmic work it did (live corners per test, corners created,
+    // integration tetrahedra, facets)
+    double t_clip_a_gpu_ms = 0, t_knn_gpu_ms = 0;
+    unsigned long long a_live = 0, a_new = 0, a_tets = 0, a_facets = 0, a_tests = 0, a_cells = 0;
 };
 
 struct Problem {
@@ -200,9 +205,12 @@ public:
         const int K0 = pb.k0;
         auto t1 = now();
         int* d_list = nullptr;
+        be_.event_begin(0);
         knn_lists(K0, d_list);
+        be_.event_end(0);
         be_.sync();
         st.t_knn_ms += ms_since(t1);
+        st.t_knn_gpu_ms += be_.event_ms(0);
 
         d_vol = (double*)slot(S_VOL, sizeof(double) * (size_t)n_);
         d_mg = (double*)slot(S_MG, sizeof(double) * 3 * (size_t)n_);
@@ -211,8 +219,8 @@ public:
         double* d_area_tmp = (double*)slot(S_AREATMP, sizeof(double) * (size_t)n_ * kFaceStride);
         uint32_t* d_flags = (uint32_t*)slot(S_FLAGS, sizeof(uint32_t) * (size_t)n_);
         double* d_r2 = (double*)slot(S_R2, sizeof(double) * (size_t)n_);
-        unsigned long long* d_ctr = (unsigned long long*)slot(S_CTR, sizeof(unsigned long long) * 4);
-        be_.dzero(d_ctr, sizeof(unsigned long long) * 4);
+        unsigned long long* d_ctr = (unsigned long long*)slot(S_CTR, sizeof(unsigned long long) * 8);
+        be_.dzero(d_ctr, sizeof(unsigned long long) * 8);
         be_.dzero(d_flags, sizeof(uint32_t) * (size_t)n_);
         be_.dzero(d_cnt, sizeof(int) * (size_t)n_);
 
@@ -243,9 +251,22 @@ public:
         P.list_len = nullptr;
         P.list_stride = K0;
         P.list_complete = 0;
+        be_.event_begin(1);
         be_.clip(1, false, P);
+        be_.event_end(1);
         be_.sync();
         st.t_clip_ms += ms_since(t2);
+        st.t_clip_a_gpu_ms += be_.event_ms(1);
+        {
+            unsigned long long c[8];
+            be_.d2h(c, d_ctr, sizeof(c));
+            st.a_tests += c[1];
+            st.a_live += c[2];
+            st.a_new += c[3];
+            st.a_tets += c[4];
+            st.a_facets += c[5];
+            st.a_cells += n_;
+        }
         auto t3 = now();
         h_flags.resize(n_);
         be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
@@ -285,7 +306,7 @@ public:
         st.t_pass_bc_ms += ms_since(t3);
 
         // statistics
-        unsigned long long ctr[4];
+        unsigned long long ctr[8];
         be_.d2h(ctr, d_ctr, sizeof(ctr));
         st.nb_clips += ctr[0];
         st.nb_tests += ctr[1];

@@ -49,6 +49,12 @@ typedef struct vgc_stats {
     uint64_t nb_tests;        /* planes tested against a cell                     */
     uint64_t radius_entries;  /* seeds gathered by the radius pass                */
     double t_build_ms, t_knn_ms, t_clip_ms, t_pass_bc_ms, t_compact_ms, t_total_ms;
+    /* first clipping pass (one launch over every cell): GPU time from events
+       on the launch stream, and its algorithmic work: planes tested, live
+       corners summed over the tests, corners created, integration
+       tetrahedra and facets */
+    double t_clip_a_gpu_ms, t_knn_gpu_ms;
+    uint64_t a_cells, a_tests, a_live, a_new, a_tets, a_facets;
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in
