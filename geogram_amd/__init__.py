@@ -24,7 +24,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import Stats, VgcError, VGC_DEVICE_PTRS, VGC_NO_NEIGHBORS, VGC_OK, VGC_PARTIAL
+from ._lib import (Stats, VgcError, VGC_COUNT_WORK, VGC_DEVICE_PTRS, VGC_NO_NEIGHBORS, VGC_OK,
+                   VGC_PARTIAL)
 
 UNIT_BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
 
@@ -103,8 +104,8 @@ class Context:
         raise VgcError(rc, self.lib.last_error(self.ctx))
 
     # ---------------------------------------------------------------- cells
-    def cells(self, points, weights=None, box=UNIT_BOX, k0=64, neighbors=True,
-              inst_n=None, boxes=None):
+    def cells(self, points, weights=None, box=UNIT_BOX, k0=0, neighbors=True,
+              inst_n=None, boxes=None, count_work=False):
         """Voronoi (weights None) or Laguerre cells clipped to box.
 
         For a batch of instances give inst_n (seed counts, consecutive in
@@ -125,7 +126,7 @@ class Context:
             vol = np.empty(n)
             mg = np.empty((n, 3))
             ptr = np.empty(n + 1, dtype=np.uint64)
-        flags = VGC_DEVICE_PTRS if dev else 0
+        flags = (VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
         st = Stats()
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)
@@ -183,7 +184,7 @@ class Context:
         return idx, d2
 
     # ---------------------------------------------------------------- Lloyd
-    def lloyd(self, points, iters, box=UNIT_BOX, k0=64):
+    def lloyd(self, points, iters, box=UNIT_BOX, k0=0):
         """iters Lloyd iterations; returns the new points (same kind of array)."""
         dev = _is_torch_cuda(points)
         if dev:
@@ -209,7 +210,7 @@ def default_context(device: int = 0) -> Context:
     return _default[device]
 
 
-def cells(points, weights=None, box=UNIT_BOX, k0=64, neighbors=True, device=0):
+def cells(points, weights=None, box=UNIT_BOX, k0=0, neighbors=True, device=0):
     return default_context(device).cells(points, weights, box, k0, neighbors)
 
 
@@ -217,7 +218,7 @@ def knn(points, k, device=0):
     return default_context(device).knn(points, k)
 
 
-def lloyd(points, iters, box=UNIT_BOX, k0=64, device=0):
+def lloyd(points, iters, box=UNIT_BOX, k0=0, device=0):
     return default_context(device).lloyd(points, iters, box, k0)
 
 

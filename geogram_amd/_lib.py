@@ -23,6 +23,7 @@ VGC_ERR_INTERNAL = -4
 
 VGC_DEVICE_PTRS = 1
 VGC_NO_NEIGHBORS = 2
+VGC_COUNT_WORK = 4
 
 _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain nb_failed nb_empty "
              "nb_disconnected nb_clips nb_tests radius_entries").split()

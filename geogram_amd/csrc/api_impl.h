@@ -85,9 +85,8 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
             ctx->err = "vgc_cells: null argument";
             return VGC_ERR_ARG;
         }
-        if (k0 == 0) k0 = 32;
-        if (!valid_k0(k0)) {
-            ctx->err = "vgc_cells: k0 must be 16, 24, 32, 48 or 64";
+        if (k0 != 0 && !valid_k0(k0)) {
+            ctx->err = "vgc_cells: k0 must be 0, 16, 24, 32, 48 or 64";
             return VGC_ERR_ARG;
         }
         long long n = 0;
@@ -115,6 +114,7 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
         pb.n = (int)n;
         pb.k0 = (int)k0;
         pb.want_neighbors = !(flags & VGC_NO_NEIGHBORS) && (nbr || farea);
+        pb.count_work = (flags & VGC_COUNT_WORK) != 0;
         const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
         B& be = ctx->be;
         if (dev) {
@@ -250,9 +250,8 @@ int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t 
             ctx->err = "vgc_lloyd: bad argument";
             return VGC_ERR_ARG;
         }
-        if (k0 == 0) k0 = 32;
-        if (!valid_k0(k0)) {
-            ctx->err = "vgc_lloyd: k0 must be 16, 24, 32, 48 or 64";
+        if (k0 != 0 && !valid_k0(k0)) {
+            ctx->err = "vgc_lloyd: k0 must be 0, 16, 24, 32, 48 or 64";
             return VGC_ERR_ARG;
         }
         B& be = ctx->be;

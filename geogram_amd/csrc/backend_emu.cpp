@@ -136,32 +136,38 @@ struct EmuBackend {
         }
     }
 
-    template <int S, bool E>
+    template <int S, bool E, bool G>
     void clip_s(const ClipParams& P) {
 #pragma omp parallel
         {
             std::unique_ptr<CellLds<S>> L(new CellLds<S>());
             std::memset(L.get(), 0, sizeof(CellLds<S>));
 #pragma omp for schedule(dynamic, 16)
-            for (int it = 0; it < P.n_items; ++it) process_cell<S, E>(P, it, *L);
+            for (int it = 0; it < P.n_items; ++it) process_cell<S, E, G>(P, it, *L);
         }
     }
 
-    void clip(int S, bool exact, const ClipParams& P) {
+    template <bool E, bool G>
+    void clip_eg(int S, const ClipParams& P) {
+        if (S == 1)
+            clip_s<1, E, G>(P);
+        else if (S == 2)
+            clip_s<2, E, G>(P);
+        else
+            clip_s<4, E, G>(P);
+    }
+
+    void clip(int S, bool exact, bool gather, const ClipParams& P) {
         if (exact) {
-            if (S == 1)
-                clip_s<1, true>(P);
-            else if (S == 2)
-                clip_s<2, true>(P);
+            if (gather)
+                clip_eg<true, true>(S, P);
             else
-                clip_s<4, true>(P);
+                clip_eg<true, false>(S, P);
         } else {
-            if (S == 1)
-                clip_s<1, false>(P);
-            else if (S == 2)
-                clip_s<2, false>(P);
+            if (gather)
+                clip_eg<false, true>(S, P);
             else
-                clip_s<4, false>(P);
+                clip_eg<false, false>(S, P);
         }
     }
 

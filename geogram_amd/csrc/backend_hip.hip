@@ -134,13 +134,13 @@ __global__ __launch_bounds__(256) void k_knn(const pf4* __restrict__ pf,
     knn_query<K>(pf, nodes, IL, q, out + (size_t)t * K, out_d2 ? out_d2 + (size_t)t * K : nullptr);
 }
 
-template <int S, int WAVES, bool EXACT>
+template <int S, int WAVES, bool EXACT, bool GATHER>
 __global__ __launch_bounds__(64 * WAVES) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = blockIdx.x * WAVES + wave;
     if (item >= P.n_items) return;  // whole wavefront
-    process_cell<S, EXACT>(P, item, lds[wave]);
+    process_cell<S, EXACT, GATHER>(P, item, lds[wave]);
 }
 
 __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
@@ -322,23 +322,28 @@ struct HipBackend {
         }
         check();
     }
-    void clip(int S, bool exact, const ClipParams& P) {
-        if (P.n_items <= 0) return;
+    template <bool E, bool G>
+    void clip_eg(int S, const ClipParams& P) {
         const unsigned g4 = blocks_for(P.n_items, 4), g1 = blocks_for(P.n_items, 1);
-        if (!exact) {
-            if (S == 1)
-                k_clip<1, 4, false><<<g4, 256, 0, stream_>>>(P);
-            else if (S == 2)
-                k_clip<2, 4, false><<<g4, 256, 0, stream_>>>(P);
+        if (S == 1)
+            k_clip<1, 4, E, G><<<g4, 256, 0, stream_>>>(P);
+        else if (S == 2)
+            k_clip<2, 4, E, G><<<g4, 256, 0, stream_>>>(P);
+        else
+            k_clip<4, 1, E, G><<<g1, 64, 0, stream_>>>(P);
+    }
+    void clip(int S, bool exact, bool gather, const ClipParams& P) {
+        if (P.n_items <= 0) return;
+        if (exact) {
+            if (gather)
+                clip_eg<true, true>(S, P);
             else
-                k_clip<4, 1, false><<<g1, 64, 0, stream_>>>(P);
+                clip_eg<true, false>(S, P);
         } else {
-            if (S == 1)
-                k_clip<1, 4, true><<<g4, 256, 0, stream_>>>(P);
-            else if (S == 2)
-                k_clip<2, 4, true><<<g4, 256, 0, stream_>>>(P);
+            if (gather)
+                clip_eg<false, true>(S, P);
             else
-                k_clip<4, 1, true><<<g1, 64, 0, stream_>>>(P);
+                clip_eg<false, false>(S, P);
         }
         check();
     }

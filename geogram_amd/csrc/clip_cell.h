@@ -32,13 +32,10 @@
 #pragma once
 
 #include "wave.h"
+#include "types.h"
 #include "exact_det.h"
 
 namespace vgc {
-
-struct alignas(32) pt4 {
-    double x, y, z, w;
-};
 
 enum CellFlag : uint32_t {
     CELL_DONE = 1u,          // outputs written
@@ -56,6 +53,10 @@ template <int S>
 struct PMax {
     static constexpr int value = S >= 4 ? 256 : 128;
 };
+
+constexpr int kCandCap = 128;   // candidates per distance ring
+constexpr int kFrontCap = 128;  // tree nodes per level of the walk
+constexpr int kBucketCap = 128; // buckets per ring
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -68,6 +69,10 @@ struct ClipParams {
     const int* inst_n;     // per instance seed count
     const int* items;      // work items (Morton positions); nullptr = 0..n_items-1
     int n_items;
+    // neighbour search inside the kernel (gather mode)
+    const pf4* pf;          // seeds in Morton order, FP32
+    const bnode* nodes;     // implicit trees
+    InstLayout IL;
     const int* list;        // neighbour lists (Morton positions), -1 terminated
     const long long* list_ptr;  // per item start (nullptr: cell * list_stride)
     const int* list_len;        // per item length (nullptr: list_stride)
@@ -101,6 +106,11 @@ struct CellLds {
     double q[64 * S][3];   // corner points (integration)
     uint8_t st_tab[kPMax]; // new corner whose first plane is p
     uint8_t en_tab[kPMax]; // new corner whose second plane is p
+    // gather mode: candidates of the current distance ring, sorted by
+    // (FP32 squared distance, Morton position), and the tree walk buffers
+    uint64_t cand[kCandCap];
+    int front[2][kFrontCap];
+    int bucket[kBucketCap];
 };
 
 // ---------------------------------------------------------------- helpers
@@ -455,7 +465,227 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
 }
 
 // Security-radius loop + integration for work item `item`.
+template <int S>
+struct CellRun {
+    CellState<S> st;
+    int np;
+    uint32_t flags;
+    bool stopped;
+    pt4 pc, prev;
+    double wi, cw;
+    int c, orig_c, prev_slot;
+    const double* box;
+    unsigned long long nclip, ntest, nlive, nnew;
+};
+
+// One candidate neighbour j (Morton position) in nearest-first order: the
+// coincident-seed rule, the security-radius stop, then the clip.  Returns
+// true when the cell is finished (or failed) and the candidate loop must end.
 template <int S, bool EXACT>
+VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
+    if (j == R.c) return false;
+    const pt4 pj = P.pts[j];
+    const double ex = pj.x - R.pc.x, ey = pj.y - R.pc.y, ez = pj.z - R.pc.z;
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    if (d2 == 0.0) {
+        if (P.orig[j] < R.orig_c) {
+            wave_sync();
+            init_box<S>(R.st, L, R.box, R.pc);
+            R.np = 6;
+            R.flags = CELL_DISCONNECTED;
+            R.stopped = true;
+            return true;
+        }
+        return false;
+    }
+    // security radius (generic_RVD.h:2303-2323)
+    bool reach;
+    if (!P.weighted) {
+        bool anyin = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            anyin = anyin || any(mask_bit(R.st.valid[s]) & (4.1 * R.st.r2[s] >= d2));
+        reach = !anyin;
+    } else {
+        const double sd = d2 + R.cw;
+        bool anyin = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            anyin = anyin || any(mask_bit(R.st.valid[s]) & ((4.1 * d2) * R.st.r2[s] >= sd * sd));
+        reach = (sd > 0.0) && !anyin;
+    }
+    if (reach) {
+        R.stopped = true;
+        return true;
+    }
+    // previous candidate: a coincident copy of a seed gives the same plane;
+    // under the exact tie rule the copy takes over the face of the first one
+    if (pj.x == R.prev.x && pj.y == R.prev.y && pj.z == R.prev.z && pj.w == R.prev.w) {
+        if (R.prev_slot >= 0) {
+            lds_st(L.pg, LI(R.prev_slot), LI(j), lane_id() == 0);
+            wave_sync();
+        }
+        return false;
+    }
+    R.prev = pj;
+    const double h = P.weighted ? 0.5 * (d2 + (R.wi - pj.w)) : 0.5 * d2;
+    bool cut = false;
+    ++R.ntest;
+    R.flags = clip_plane<S, EXACT>(R.st, L, R.np, ex, ey, ez, h, j, R.flags, cut, R.nlive, R.nnew);
+    R.prev_slot = (cut && !(R.flags & (CELL_EMPTY | CELL_OVERFLOW))) ? R.np - 1 : -1;
+    if (cut) ++R.nclip;
+    if (R.flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
+        R.stopped = (R.flags & CELL_EMPTY) != 0;
+        return true;
+    }
+    return false;
+}
+
+// Tree node / seed loads per lane (masked lanes get zeros).
+struct BoxLanes {
+    LF lx, ly, lz, hx, hy, hz;
+};
+VGC_HD BoxLanes load_box(const bnode* T, const LI& node, const LB& m) {
+    BoxLanes B;
+#if defined(VGC_EMU)
+    for (int l = 0; l < kWave; ++l) {
+        bnode b{};
+        if (m[l]) b = T[node[l]];
+        B.lx[l] = b.lo[0];
+        B.ly[l] = b.lo[1];
+        B.lz[l] = b.lo[2];
+        B.hx[l] = b.hi[0];
+        B.hy[l] = b.hi[1];
+        B.hz[l] = b.hi[2];
+    }
+#else
+    bnode b{};
+    if (m) b = T[node];
+    B.lx = b.lo[0];
+    B.ly = b.lo[1];
+    B.lz = b.lo[2];
+    B.hx = b.hi[0];
+    B.hy = b.hi[1];
+    B.hz = b.hi[2];
+#endif
+    return B;
+}
+
+VGC_HD LF lane_dist2(const pf4* pf, const LI& idx, const LB& m, float px, float py, float pz) {
+#if defined(VGC_EMU)
+    LF r = LF(0.0f);
+    for (int l = 0; l < kWave; ++l)
+        if (m[l]) r[l] = dist2f(pf[idx[l]], px, py, pz);
+    return r;
+#else
+    return m ? dist2f(pf[idx], px, py, pz) : 0.0f;
+#endif
+}
+
+// Seeds q != c of the instance with rlo2 < d2 <= rhi2 (FP32 squared
+// distance) into L.cand as (d2 bits << 32 | q), unsorted.  Returns their
+// number (only the first kCandCap are stored), or -1 when the tree walk
+// overflows its buffers.  Breadth-first walk of the implicit tree, one node
+// per lane; boxes entirely inside the inner ball are skipped.
+template <int S>
+VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, float px, float py,
+                       float pz, float rlo2, float rhi2) {
+    const LI lane = lane_id();
+    const int p0 = P.IL.p0[inst], n = P.IL.n[inst], nb = P.IL.nb[inst], Lv = P.IL.L[inst];
+    const bnode* T = P.nodes + P.IL.t0[inst];
+    lds_st(&L.front[0][0], LI(0), LI(1), lane == 0);
+    wave_sync();
+    int nf = 1, cur = 0, nbk = 0;
+    bool overflow = false;
+    while (nf > 0) {
+        int nn = 0;
+        for (int base = 0; base < nf; base += kWave) {
+            LI idx = base + lane;
+            LB in = idx < nf;
+            LI node = lds_ld(&L.front[cur][0], select(in, idx, LI(0)));
+            BoxLanes B = load_box(T, node, in);
+            LF dx = vmaxf(vmaxf(B.lx - px, px - B.hx), 0.0f);
+            LF dy = vmaxf(vmaxf(B.ly - py, py - B.hy), 0.0f);
+            LF dz = vmaxf(vmaxf(B.lz - pz, pz - B.hz), 0.0f);
+            LF dmin = dx * dx + dy * dy + dz * dz;
+            LF fx = vmaxf(px - B.lx, B.hx - px), fy = vmaxf(py - B.ly, B.hy - py),
+               fz = vmaxf(pz - B.lz, B.hz - pz);
+            LF dmax = fx * fx + fy * fy + fz * fz;
+            LB hit = in & (dmin <= rhi2) & (dmax * 1.00001f > rlo2);
+            LB internal = node < Lv;
+            LB push = hit & internal;
+            LB leaf = hit & !internal;
+            uint64_t mp = ballot(push), ml = ballot(leaf);
+            LI rp = nn + 2 * mbcnt(mp);
+            LB okp = push & (rp + 1 < kFrontCap);
+            lds_st(&L.front[cur ^ 1][0], select(okp, rp, LI(0)), 2 * node, okp);
+            lds_st(&L.front[cur ^ 1][0], select(okp, rp + 1, LI(0)), 2 * node + 1, okp);
+            LI rl = nbk + mbcnt(ml);
+            LB okl = leaf & (rl < kBucketCap);
+            lds_st(L.bucket, select(okl, rl, LI(0)), node - Lv, okl);
+            nn += 2 * popc64(mp);
+            nbk += popc64(ml);
+        }
+        wave_sync();
+        if (nn > kFrontCap || nbk > kBucketCap) overflow = true;
+        if (overflow) break;
+        nf = nn;
+        cur ^= 1;
+    }
+    if (overflow) return -1;
+    int cnt = 0;
+    for (int base = 0; base < nbk; base += kWave / kBucket) {
+        LI bi = base + (lane >> 4);
+        LI k = lane & (kBucket - 1);
+        LB inb = bi < nbk;
+        LI b = lds_ld(L.bucket, select(inb, bi, LI(0)));
+        LI m = p0 + b * kBucket + k;
+        LB ok = inb & (b < nb) & (m < p0 + n) & (m != c);
+        LF d2 = lane_dist2(P.pf, select(ok, m, LI(0)), ok, px, py, pz);
+        LB take = ok & (d2 > rlo2) & (d2 <= rhi2);
+        uint64_t mt = ballot(take);
+        LI r = cnt + mbcnt(mt);
+        LB st = take & (r < kCandCap);
+        LU64 key = (cvt<uint64_t>(fbits(d2)) << LU64(32)) | cvt<uint64_t>(cvt<uint32_t>(m));
+        lds_st(L.cand, select(st, r, LI(0)), key, st);
+        cnt += popc64(mt);
+    }
+    wave_sync();
+    return cnt;
+}
+
+// Bitonic sort of L.cand[0..cnt) (cnt <= kCandCap), padded to 64 or 128.
+template <int S>
+VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
+    const LI lane = lane_id();
+    const int P2 = cnt <= 64 ? 64 : 128;
+    for (int i0 = 0; i0 < P2; i0 += kWave) {
+        LI i = i0 + lane;
+        lds_st(L.cand, i, LU64(~0ull), i >= cnt);
+    }
+    wave_sync();
+    for (int k = 2; k <= P2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            LI p = lane;  // pair index, P2 / 2 <= 64 pairs
+            LB act = p < P2 / 2;
+            LI i = (p / j) * (2 * j) + (p % j);
+            LI ip = i + j;
+            LI is = select(act, i, LI(0)), ips = select(act, ip, LI(0));
+            LU64 a = lds_ld(L.cand, is), b = lds_ld(L.cand, ips);
+            LB asc = (i & k) == 0;
+            LB sw = act & ((a > b) == asc);
+            wave_sync();
+            lds_st(L.cand, is, b, sw);
+            lds_st(L.cand, ips, a, sw);
+            wave_sync();
+        }
+    }
+}
+
+// Security-radius loop + integration for work item `item`.  GATHER: the
+// wave finds the neighbours itself, ring after ring of distance; otherwise
+// it reads them from P.list.
+template <int S, bool EXACT, bool GATHER>
 VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
@@ -464,91 +694,123 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     const pt4 pc = P.pts[c];
     const int inst = P.inst_of ? P.inst_of[c] : 0;
     const double* box = P.box + 6 * inst;
-    const double wi = pc.w;
-    const double cw = P.weighted ? (wi - P.wmax[inst]) : 0.0;
-    const int orig_c = P.orig[c];
-    long long lbeg;
-    int llen;
-    if (P.list_ptr) {
-        lbeg = P.list_ptr[item];
-        llen = P.list_len[item];
-    } else {
-        lbeg = (long long)c * P.list_stride;  // n x K lists by Morton position
-        llen = P.list_stride;
-    }
-    bool complete = P.list_complete != 0 || llen >= P.inst_n[inst] - 1;
 
-    CellState<S> st;
-    init_box<S>(st, L, box, pc);
-    int np = 6;
-    uint32_t flags = 0;
-    bool stopped = false;
-    unsigned long long nclip = 0, ntest = 0, nlive = 0, nnew = 0;
-    // previous candidate: a coincident copy of a seed gives the same plane;
-    // under the exact tie rule the copy takes over the face of the first one
-    pt4 prev;
-    prev.x = prev.y = prev.z = prev.w = NAN;
-    int prev_slot = -1;
-    for (int jj = 0; jj < llen; ++jj) {
-        const int j = P.list[lbeg + jj];
-        if (j < 0) {
-            complete = true;  // short list: the instance has fewer seeds
-            break;
+    CellRun<S> R;
+    R.pc = pc;
+    R.c = c;
+    R.orig_c = P.orig[c];
+    R.wi = pc.w;
+    R.cw = P.weighted ? (pc.w - P.wmax[inst]) : 0.0;
+    R.box = box;
+    R.np = 6;
+    R.flags = 0;
+    R.stopped = false;
+    R.prev.x = R.prev.y = R.prev.z = R.prev.w = NAN;
+    R.prev_slot = -1;
+    R.nclip = R.ntest = R.nlive = R.nnew = 0;
+    init_box<S>(R.st, L, box, pc);
+    bool complete = false;
+
+    if (!GATHER) {
+        long long lbeg;
+        int llen;
+        if (P.list_ptr) {
+            lbeg = P.list_ptr[item];
+            llen = P.list_len[item];
+        } else {
+            lbeg = (long long)c * P.list_stride;  // n x K lists by Morton position
+            llen = P.list_stride;
         }
-        if (j == c) continue;
-        const pt4 pj = P.pts[j];
-        const double ex = pj.x - pc.x, ey = pj.y - pc.y, ez = pj.z - pc.z;
-        const double d2 = ex * ex + ey * ey + ez * ez;
-        if (d2 == 0.0) {
-            if (P.orig[j] < orig_c) {
-                wave_sync();
-                init_box<S>(st, L, box, pc);
-                np = 6;
-                flags = CELL_DISCONNECTED;
-                stopped = true;
+        complete = P.list_complete != 0 || llen >= P.inst_n[inst] - 1;
+        for (int jj = 0; jj < llen; ++jj) {
+            const int j = P.list[lbeg + jj];
+            if (j < 0) {
+                complete = true;  // short list: the instance has fewer seeds
                 break;
             }
-            continue;
+            if (consider<S, EXACT>(P, L, R, j)) break;
         }
-        // security radius
-        bool reach;
-        if (!P.weighted) {
-            bool anyin = false;
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                anyin = anyin || any(mask_bit(st.valid[s]) & (4.1 * st.r2[s] >= d2));
-            reach = !anyin;
-        } else {
-            const double sd = d2 + cw;
-            bool anyin = false;
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                anyin = anyin || any(mask_bit(st.valid[s]) & ((4.1 * d2) * st.r2[s] >= sd * sd));
-            reach = (sd > 0.0) && !anyin;
-        }
-        if (reach) {
-            stopped = true;
-            break;
-        }
-        if (pj.x == prev.x && pj.y == prev.y && pj.z == prev.z && pj.w == prev.w) {
-            if (prev_slot >= 0) {
-                lds_st(L.pg, LI(prev_slot), LI(j), lane == 0);
-                wave_sync();
+    } else {
+        const pf4 pq = P.pf[c];
+        const int p0i = P.IL.p0[inst], ni = P.IL.n[inst];
+        // first ring, about 56 seeds: 2.4 times the distance to the 4th
+        // nearest seed of the own bucket (Morton neighbours, an upper bound of
+        // the 4th nearest distance)
+        const int bq = (c - p0i) / kBucket;
+        float rhi2;
+        {
+            LI m = p0i + bq * kBucket + (lane & (kBucket - 1));
+            LB ok = (lane < kBucket) & (m < p0i + ni) & (m != c);
+            LF d2 = lane_dist2(P.pf, select(ok, m, LI(0)), ok, pq.x, pq.y, pq.z);
+            d2 = select(ok, d2, LF(INFINITY));
+            float d4 = INFINITY;
+            for (int k = 0; k < kBucket; ++k) {
+                const float t = readlane(d2, k);
+                if (popc64(ballot(d2 < t)) == 3 && t < d4) d4 = t;
             }
-            continue;
+            rhi2 = 5.8f * d4;
+            if (!(rhi2 > 0.0f) || !(rhi2 < INFINITY)) {
+                const bnode ob = P.nodes[P.IL.t0[inst] + P.IL.L[inst] + bq];
+                const float gx = ob.hi[0] - ob.lo[0], gy = ob.hi[1] - ob.lo[1],
+                            gz = ob.hi[2] - ob.lo[2];
+                rhi2 = 0.3f * (gx * gx + gy * gy + gz * gz);
+            }
         }
-        prev = pj;
-        const double h = P.weighted ? 0.5 * (d2 + (wi - pj.w)) : 0.5 * d2;
-        bool cut = false;
-        ++ntest;
-        flags = clip_plane<S, EXACT>(st, L, np, ex, ey, ez, h, j, flags, cut, nlive, nnew);
-        prev_slot = (cut && !(flags & (CELL_EMPTY | CELL_OVERFLOW))) ? np - 1 : -1;
-        if (cut) ++nclip;
-        if (flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
-            stopped = (flags & CELL_EMPTY) != 0;
-            break;
+        const float bx = float(box[3] - box[0]), by = float(box[4] - box[1]),
+                    bz = float(box[5] - box[2]);
+        const float all2 = f_up(1.0001 * double(bx * bx + by * by + bz * bz));
+        if (!(rhi2 > 0.0f)) rhi2 = all2 * 1e-6f;
+        float rlo2 = -1.0f;
+        while (true) {
+            int cnt = gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2);
+            if (cnt < 0 || cnt > kCandCap) {
+                // too many seeds (or tree nodes) in the ring: narrow it, same
+                // shell-volume fraction in r^3, and gather again
+                float lo = rlo2 > 0.0f ? rlo2 : 0.0f;
+                float r3lo = lo * sqrtf(lo), r3hi = rhi2 * sqrtf(rhi2);
+                float frac = cnt < 0 ? 0.125f : 0.75f * kCandCap / float(cnt);
+                float r3 = r3lo + (r3hi - r3lo) * frac;
+                float nh = cbrtf(r3);
+                nh = nh * nh;
+                if (!(nh > lo * 1.000001f) || !(nh < rhi2)) {
+                    R.flags |= CELL_NEED_K;  // seeds at one distance: list tiers
+                    break;
+                }
+                rhi2 = nh;
+                continue;
+            }
+            sort_cand<S>(L, cnt);
+            bool done = false;
+            for (int jj = 0; jj < cnt; ++jj) {
+                const int j = int(uint32_t(L.cand[jj]));
+                if (consider<S, EXACT>(P, L, R, j)) {
+                    done = true;
+                    break;
+                }
+            }
+            if (done) break;
+            // ring exhausted: can a seed beyond rhi2 still cut the cell?
+            LD r2m = LD(0.0);
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                r2m = vmax(r2m, select(mask_bit(R.st.valid[s]), R.st.r2[s], LD(0.0)));
+            const double r2now = wave_max(r2m);
+            const float need2 = radius_for(r2now, R.wi, P.weighted ? P.wmax[inst] : 0.0, P.weighted);
+            if (need2 <= rhi2 || rhi2 >= all2) {
+                complete = true;
+                break;
+            }
+            rlo2 = rhi2;
+            rhi2 = need2 < all2 ? need2 : all2;
+            wave_sync();
         }
     }
+    const bool stopped = R.stopped;
+    uint32_t flags = R.flags;
+    CellState<S>& st = R.st;
+    int np = R.np;
+    const unsigned long long nclip = R.nclip, ntest = R.ntest, nlive = R.nlive, nnew = R.nnew;
+    const int orig_c = R.orig_c;
     if (!stopped && !complete && !(flags & (CELL_OVERFLOW | CELL_BAD_TOPO))) flags |= CELL_NEED_K;
 
     // squared radius of the (possibly partial) cell

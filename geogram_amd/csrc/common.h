@@ -14,23 +14,6 @@
 
 namespace vgc {
 
-VGC_SD int imin(int a, int b) { return a < b ? a : b; }
-
-// Morton-ordered seed in FP32, for the neighbour search.
-struct alignas(16) pf4 {
-    float x, y, z, w;
-};
-
-// Node of the implicit bounding-volume tree over buckets of kBucket
-// consecutive Morton-ordered seeds (float box, rounded outwards).
-struct alignas(32) bnode {
-    float lo[3];
-    float hi[3];
-    int pad[2];
-};
-
-constexpr int kBucket = 16;
-
 // 16-bit quantisation per axis, bits interleaved: 48-bit Morton code; the
 // instance id sits above it so instances sort into contiguous ranges.
 VGC_SD uint64_t spread16(uint64_t v) {
@@ -55,41 +38,6 @@ VGC_SD uint64_t morton_key(double x, double y, double z, const double* box, uint
     uint64_t qz = quant16(z, box[2], box[5]);
     return (inst << 48) | spread16(qx) | (spread16(qy) << 1) | (spread16(qz) << 2);
 }
-
-// float conversions rounded down / up (exact bounds for the tree)
-VGC_SD float f_down(double x) {
-    float f = float(x);
-    if (double(f) > x) f = nextafterf(f, -INFINITY);
-    return f;
-}
-VGC_SD float f_up(double x) {
-    float f = float(x);
-    if (double(f) < x) f = nextafterf(f, INFINITY);
-    return f;
-}
-
-VGC_SD float box_dist2(const bnode& b, float px, float py, float pz) {
-    float dx = fmaxf(fmaxf(b.lo[0] - px, px - b.hi[0]), 0.0f);
-    float dy = fmaxf(fmaxf(b.lo[1] - py, py - b.hi[1]), 0.0f);
-    float dz = fmaxf(fmaxf(b.lo[2] - pz, pz - b.hi[2]), 0.0f);
-    return dx * dx + dy * dy + dz * dz;
-}
-
-VGC_SD float dist2f(const pf4& a, float px, float py, float pz) {
-    float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
-    return dx * dx + dy * dy + dz * dz;
-}
-
-// Per-instance layout of the Morton-ordered seeds, buckets and trees.
-struct InstLayout {
-    const int* inst_of;       // Morton position -> instance (nullptr: one)
-    const int* p0;            // first Morton position
-    const int* n;             // seed count
-    const int* b0;            // first bucket
-    const int* nb;            // bucket count
-    const long long* t0;      // node offset: node k (1 <= k < 2L) at t0 + k
-    const int* L;             // leaves (power of two >= nb)
-};
 
 // --------------------------------------------------------------- kNN
 // K nearest other seeds of each query (FP32 ranking), sorted by (d2, id);
@@ -230,24 +178,6 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
         node += 1;
     }
     return cnt;
-}
-
-// Squared radius (FP32, rounded up) within which a seed can still cut a
-// partial cell of squared radius r2 around a seed of weight wi; wmax is the
-// largest weight of the instance (Voronoi: weighted = false).  A plane at
-// distance d can cut only if (d^2 + wi - wj) / (2 d) < sqrt(4.1)/2 R, the
-// security-radius margin of generic_RVD.h:2319-2322.
-VGC_SD float radius_for(double r2, double wi, double wmax, int weighted) {
-    double rho2;
-    if (!weighted) {
-        rho2 = 4.1 * r2;
-    } else {
-        double a = 1.0124228365658292 * sqrt(r2);  // sqrt(4.1) / 2
-        double cw = wi - wmax;                     // <= 0
-        double rho = a + sqrt(a * a - cw);
-        rho2 = rho * rho;
-    }
-    return f_up(rho2 * 1.0001);
 }
 
 // ---------------------------------------------------------- tree build

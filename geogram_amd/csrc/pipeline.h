@@ -5,17 +5,22 @@
 //      seeds are gathered into Morton order (FP64 x,y,z,w and an FP32 copy).
 //   2. An implicit bounding-volume tree per instance over buckets of 16
 //      consecutive seeds.
-//   3. kNN lists of K0 neighbours (FP32 ranking).  Replaces the kd-tree
-//      neighbourhoods of Delaunay_NearestNeighbors (delaunay/delaunay_nn.cpp).
-//   4. Pass A: every cell, one wavefront each, 64 corner slots.
-//   5. Pass B: cells whose list ran out before the security radius.  Their
+//   3. Pass A: every cell, one wavefront each, 64 corner slots.  The wave
+//      finds its neighbours itself (breadth-first walk of the tree into LDS,
+//      one ring of distance at a time, sorted, nearest first) until the
+//      security radius is reached; this replaces the kd-tree neighbourhoods
+//      of Delaunay_NearestNeighbors (delaunay/delaunay_nn.cpp) and their
+//      k += k/8 enlargements (voronoi/generic_RVD.h:2335-2345).  With k0 > 0
+//      the first pass reads kNN lists of k0 instead (FP32 ranking).
+//   4. Pass B: cells whose list ran out before the security radius (list
+//      mode), or whose ring did not fit the wave's buffers.  Their
 //      partial cell bounds the final one, so every seed within
 //      sqrt(4.1) R_partial is gathered (radius query), sorted by distance and
-//      the cell is clipped again: the list is complete by construction.  This
-//      replaces geogram's repeated k += k/8 enlargements
-//      (voronoi/generic_RVD.h:2335-2345) with one exact step.
-//   6. Pass C: cells that overflowed 64 corners, with 128 corner slots, then
+//      the cell is clipped again: the list is complete by construction.
+//   5. Pass C: cells that overflowed 64 corners, with 128 corner slots, then
 //      256 for the few that overflow again (huge Laguerre cells).
+//   6. Pass E: cells with a sign inside its error bound, again with exact
+//      signs (ties resolved as geogram's exact predicates do).
 //   7. Neighbour lists and facet areas compacted into CSR (caller's order).
 #pragma once
 
@@ -50,8 +55,10 @@ struct Problem {
     std::vector<double> box;        // 6 per instance
     const double* d_xyz = nullptr;  // device (backend) memory, 3n, caller order
     const double* d_w = nullptr;    // device, n, or nullptr (Voronoi)
-    int k0 = 32;                    // neighbours of pass A (16, 24, 32, 48 or 64)
+    int k0 = 0;                     // 0: neighbours gathered inside the clipping kernel;
+                                    // 16..64: kNN lists of k0 for the first pass
     bool want_neighbors = true;
+    bool count_work = false;        // algorithmic work counters (global atomics)
 };
 
 template <class B>
@@ -203,14 +210,17 @@ public:
         auto tstart = now();
         if (build(pb, st)) return -1;
         const int K0 = pb.k0;
+        const bool gather = K0 == 0;
         auto t1 = now();
         int* d_list = nullptr;
-        be_.event_begin(0);
-        knn_lists(K0, d_list);
-        be_.event_end(0);
-        be_.sync();
-        st.t_knn_ms += ms_since(t1);
-        st.t_knn_gpu_ms += be_.event_ms(0);
+        if (!gather) {
+            be_.event_begin(0);
+            knn_lists(K0, d_list);
+            be_.event_end(0);
+            be_.sync();
+            st.t_knn_ms += ms_since(t1);
+            st.t_knn_gpu_ms += be_.event_ms(0);
+        }
 
         d_vol = (double*)slot(S_VOL, sizeof(double) * (size_t)n_);
         d_mg = (double*)slot(S_MG, sizeof(double) * 3 * (size_t)n_);
@@ -240,7 +250,10 @@ public:
         P.face_stride = kFaceStride;
         P.flags = d_flags;
         P.r2 = d_r2;
-        P.counters = d_ctr;
+        P.counters = pb.count_work ? d_ctr : nullptr;
+        P.pf = d_pf;
+        P.nodes = d_nodes;
+        P.IL = IL;
 
         // pass A over every cell, then the tiers for the flagged ones
         auto t2 = now();
@@ -252,7 +265,7 @@ public:
         P.list_stride = K0;
         P.list_complete = 0;
         be_.event_begin(1);
-        be_.clip(1, false, P);
+        be_.clip(1, false, gather, P);
         be_.event_end(1);
         be_.sync();
         st.t_clip_ms += ms_since(t2);
@@ -291,7 +304,7 @@ public:
             ClipParams Q = P;
             Q.items = d_items;
             Q.n_items = ni;
-            be_.clip(1, true, Q);
+            be_.clip(1, true, gather, Q);
             be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
             std::vector<int> need2, over2;
             for (int c : unsure) {
@@ -432,7 +445,7 @@ private:
         P.list_len = d_rcnt;
         P.list_stride = 0;
         P.list_complete = 1;
-        be_.clip(S, exact, P);
+        be_.clip(S, exact, false, P);
         be_.sync();
         return 0;
     }

@@ -83,6 +83,7 @@ VGC_LANE_BINOP(+)
 VGC_LANE_BINOP(-)
 VGC_LANE_BINOP(*)
 VGC_LANE_BINOP(/)
+VGC_LANE_BINOP(%)
 VGC_LANE_BINOP(&)
 VGC_LANE_BINOP(|)
 VGC_LANE_BINOP(^)
@@ -203,6 +204,25 @@ inline double vabs(double a) { return std::fabs(a); }
 inline double vsqrt(double a) { return std::sqrt(a); }
 inline double vmax(double a, double b) { return std::fmax(a, b); }
 inline double vmin(double a, double b) { return std::fmin(a, b); }
+
+inline LF vmaxf(const LF& a, const LF& b) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmax(a.v[i], b.v[i]);
+    return r;
+}
+inline LF vmaxf(const LF& a, float b) { return vmaxf(a, LF(b)); }
+inline LU fbits(const LF& a) {
+    LU r;
+    for (int i = 0; i < kWave; ++i) std::memcpy(&r.v[i], &a.v[i], 4);
+    return r;
+}
+// global-memory gather: base[idx] on the lanes of m, T() elsewhere
+template <class T>
+inline Lane<T> gld(const T* base, const LI& idx, const LB& m) {
+    Lane<T> r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = m.v[i] ? base[idx.v[i]] : T();
+    return r;
+}
 
 inline LI lane_id() {
     LI r;
@@ -341,6 +361,13 @@ VGC_HD double vabs(double a) { return __builtin_fabs(a); }
 VGC_HD double vsqrt(double a) { return __builtin_sqrt(a); }
 VGC_HD double vmax(double a, double b) { return __builtin_fmax(a, b); }
 VGC_HD double vmin(double a, double b) { return __builtin_fmin(a, b); }
+
+VGC_HD float vmaxf(float a, float b) { return __builtin_fmaxf(a, b); }
+VGC_HD uint32_t fbits(float a) { return __float_as_uint(a); }
+template <class T>
+VGC_HD T gld(const T* base, int idx, bool m) {
+    return m ? base[idx] : T();
+}
 
 VGC_HD int lane_id() { return __lane_id(); }
 

@@ -32,6 +32,8 @@ extern "C" {
 
 #define VGC_DEVICE_PTRS 1u   /* all array arguments are device pointers */
 #define VGC_NO_NEIGHBORS 2u  /* skip the neighbour CSR (volumes/centroids only) */
+#define VGC_COUNT_WORK 4u    /* fill the algorithmic work counters of vgc_stats
+                                (global atomics: slows the first pass down) */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -45,14 +47,14 @@ typedef struct vgc_stats {
     uint64_t nb_failed;       /* cells without valid output                       */
     uint64_t nb_empty;        /* empty Laguerre cells                             */
     uint64_t nb_disconnected; /* cells left unclipped by the coincident-seed rule */
-    uint64_t nb_clips;        /* planes that cut a cell                           */
-    uint64_t nb_tests;        /* planes tested against a cell                     */
+    uint64_t nb_clips;        /* planes that cut a cell (VGC_COUNT_WORK)          */
+    uint64_t nb_tests;        /* planes tested against a cell (VGC_COUNT_WORK)    */
     uint64_t radius_entries;  /* seeds gathered by the radius pass                */
     double t_build_ms, t_knn_ms, t_clip_ms, t_pass_bc_ms, t_compact_ms, t_total_ms;
     /* first clipping pass (one launch over every cell): GPU time from events
-       on the launch stream, and its algorithmic work: planes tested, live
-       corners summed over the tests, corners created, integration
-       tetrahedra and facets */
+       on the launch stream, and (with VGC_COUNT_WORK) its algorithmic work:
+       planes tested, live corners summed over the tests, corners created,
+       integration tetrahedra and facets */
     double t_clip_a_gpu_ms, t_knn_gpu_ms;
     uint64_t a_cells, a_tests, a_live, a_new, a_tets, a_facets;
 } vgc_stats;
@@ -83,7 +85,9 @@ int vgc_knn(vgc_ctx* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t fl
    facet areas (ConvexCell::facet_area, convex_cell.cpp:1160-1191).
    nbr / farea may be NULL; if cap is too small the call returns
    VGC_ERR_CAPACITY with the required size in stats->nb_neighbors.
-   k0: first neighbourhood size (16, 24, 32, 48 or 64; 0 = 32). */
+   k0: 0 (default) lets each cell's wavefront gather its own neighbours,
+   nearest first, ring after ring of distance, until the security radius is
+   reached; 16, 24, 32, 48 or 64 first computes kNN lists of that size. */
 int vgc_cells(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n,
               const double* box, uint32_t k0, uint32_t flags, double* vol, double* mg,
               uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,

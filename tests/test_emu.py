@@ -37,7 +37,8 @@ def compare(r, o, n, rel=REL):
     assert not bad, f"{len(bad)} neighbour sets differ, first {bad[:5]}"
 
 
-@pytest.mark.parametrize("kind,n,k0", [(0, 2000, 32), (0, 4000, 64), (1, 2500, 48), (2, 2000, 16)])
+@pytest.mark.parametrize("kind,n,k0", [(0, 2000, 32), (0, 4000, 64), (1, 2500, 48), (2, 2000, 16),
+                                       (0, 4000, 0), (1, 5000, 0), (2, 5000, 0)])
 def test_cells_vs_oracle(emu_ctx, kind, n, k0):
     x = ga.sample_points(kind, 100 + kind, n)
     r = emu_ctx.cells(x, k0=k0)
@@ -52,9 +53,10 @@ def test_laguerre_vs_oracle(emu_ctx):
     n = 2500
     x = rng.random((n, 3))
     w = rng.random(n) * 4e-4
-    r = emu_ctx.cells(x, weights=w, k0=32)
-    o = oracle.cells(x, w=w, k0=20)
-    compare(r, o, n)
+    for k0 in (32, 0):
+        r = emu_ctx.cells(x, weights=w, k0=k0)
+        o = oracle.cells(x, w=w, k0=20)
+        compare(r, o, n)
     assert abs(r.vol.sum() - 1.0) < 1e-12
 
 

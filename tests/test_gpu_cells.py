@@ -37,7 +37,8 @@ def compare(r, o, n, rel=REL):
     assert not bad, f"{len(bad)} neighbour sets differ, first {bad[:5]}"
 
 
-@pytest.mark.parametrize("kind,n,k0", [(0, 3000, 32), (1, 5000, 64), (0, 20000, 48)])
+@pytest.mark.parametrize("kind,n,k0", [(0, 3000, 32), (1, 5000, 64), (0, 20000, 48), (1, 20000, 0),
+                                       (2, 20000, 0)])
 def test_voronoi_vs_oracle(gpu_ctx, kind, n, k0):
     import geogram_amd as ga
 
@@ -53,8 +54,8 @@ def test_gpu_matches_emulation_bitwise(gpu_ctx, emu_ctx):
     import geogram_amd as ga
 
     x = ga.sample_points(1, 5, 3000)
-    g = gpu_ctx.cells(x, k0=32)
-    e = emu_ctx.cells(x, k0=32)
+    g = gpu_ctx.cells(x, k0=0)
+    e = emu_ctx.cells(x, k0=0)
     assert np.array_equal(g.vol, e.vol)
     assert np.array_equal(g.mg, e.mg)
     assert np.array_equal(g.nbr_ptr, e.nbr_ptr)
@@ -76,12 +77,13 @@ def test_gpu_matches_emulation_laguerre_and_lattice(gpu_ctx, emu_ctx):
     w = rng.random(3000) * 4e-4
     w[:5] = 0.05  # huge cells: 128 / 256 corner tiers and empty cells
     _same(gpu_ctx.cells(x, weights=w, k0=32), emu_ctx.cells(x, weights=w, k0=32))
+    _same(gpu_ctx.cells(x, weights=w, k0=0), emu_ctx.cells(x, weights=w, k0=0))
     m = 8
     g = (np.arange(m) + 0.5) / m
     lat = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
-    rg = gpu_ctx.cells(lat, k0=32)
+    rg = gpu_ctx.cells(lat, k0=0)
     assert rg.stats["nb_pass_e"] > 0 and rg.stats["nb_uncertain"] == 0
-    _same(rg, emu_ctx.cells(lat, k0=32))
+    _same(rg, emu_ctx.cells(lat, k0=0))
 
 
 def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
@@ -92,9 +94,11 @@ def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
     boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
     _same(gpu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes),
           emu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes))
+    _same(gpu_ctx.cells(x, k0=0, inst_n=sizes, boxes=boxes),
+          emu_ctx.cells(x, k0=0, inst_n=sizes, boxes=boxes))
     y = ga.sample_points(0, 3, 4000)
-    gy, _ = gpu_ctx.lloyd(y, 3, k0=48)
-    ey, _ = emu_ctx.lloyd(y, 3, k0=48)
+    gy, _ = gpu_ctx.lloyd(y, 3, k0=0)
+    ey, _ = emu_ctx.lloyd(y, 3, k0=0)
     assert np.array_equal(gy, ey)
     oy, _ = oracle.lloyd(y, 3)
     assert np.abs(gy - oy).max() < 1e-12
@@ -118,7 +122,7 @@ def test_c2_two_million_invariants(gpu_ctx):
 
     n = 2_000_000
     x = ga.sample_points(1, 2, n)
-    r = gpu_ctx.cells(x, k0=64)
+    r = gpu_ctx.cells(x, k0=0)
     st = r.stats
     assert st["nb_failed"] == 0 and st["nb_uncertain"] == 0
     assert abs(r.vol.sum() - 1.0) < 1e-12
