@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--n", type=int, default=2_000_000)
     ap.add_argument("--k0", type=int, default=0,
                     help="0: neighbours gathered in the clipping kernel; 16..64: kNN lists")
+    ap.add_argument("--ot-n", type=int, default=1_000_000,
+                    help="seeds of the semi-discrete OT leg (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=400_000,
                     help="cells of the workload timed on the host with the oracle (0: skip)")
     args = ap.parse_args()
@@ -97,6 +99,24 @@ def main():
         "launch_ms_hip_events": round(clip_ms, 3),
     }
 
+    # semi-discrete OT leg: 1M uniform seeds, equal masses, to 1e-4 max
+    # relative mass error, plain start (w = 0)
+    ot = None
+    if args.ot_n > 0:
+        m = args.ot_n
+        xo = torch.from_numpy(ga.sample_points(0, 3, m)).cuda()
+        nu = torch.full((m,), 1.0 / m, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        w, ost = ctx.ot_solve(xo, nu, tol=1e-4)
+        torch.cuda.synchronize()
+        tot = time.perf_counter() - t2
+        ot = {"n": m, "start": "plain (w = 0)", "time_s": round(tot, 3),
+              "newton_iters": int(ost["newton_iters"]), "cg_iters": int(ost["cg_iters"]),
+              "cell_passes": int(ost["cell_passes"]), "err0": ost["err0"],
+              "final_err": ost["final_err"], "converged": bool(ost["converged"]),
+              "t_cells_ms": round(ost["t_cells_ms"], 1), "t_cg_ms": round(ost["t_cg_ms"], 1)}
+
     cpu = None
     if args.cpu_sample > 0:
         import oracle
@@ -137,6 +157,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ot": ot,
         "checks": {"sum_vol_minus_1": sumvol_err, "failed": int(s["nb_failed"]),
                    "uncertain": int(s["nb_uncertain"])},
         "phases_ms": {k: round(s[k], 3) for k in ("t_build_ms", "t_knn_ms", "t_clip_ms",

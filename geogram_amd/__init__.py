@@ -24,14 +24,15 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (Stats, VgcError, VGC_COUNT_WORK, VGC_DEVICE_PTRS, VGC_NO_NEIGHBORS, VGC_OK,
+from ._lib import (OtStats, Stats, VgcError, VGC_COUNT_WORK, VGC_DEVICE_PTRS, VGC_NO_NEIGHBORS, VGC_OK,
                    VGC_PARTIAL)
 
 UNIT_BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
 
 __all__ = [
     "Context", "CellResult", "NearestNeighborSearch", "RestrictedVoronoiDiagram",
-    "CentroidalVoronoiTesselation", "default_context", "cells", "knn", "lloyd",
+    "CentroidalVoronoiTesselation", "OptimalTransportMap3d", "default_context", "cells", "knn",
+    "lloyd", "ot_solve",
     "random_float64", "sample_points", "Stats", "VgcError", "UNIT_BOX",
 ]
 
@@ -183,6 +184,30 @@ class Context:
         self._check(rc, allow_partial=False)
         return idx, d2
 
+    # ------------------------------------------------------------------- OT
+    def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100):
+        """Laguerre weights w with cell masses nu (uniform density on box).
+        Returns (w, stats); w0 is the start (None: zeros)."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            nu_ = nu.contiguous().to(torch.float64)
+            w = (torch.zeros(pts.shape[0], dtype=torch.float64, device=pts.device) if w0 is None
+                 else w0.detach().clone().to(torch.float64).contiguous())
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            nu_ = np.ascontiguousarray(nu, dtype=np.float64)
+            w = np.zeros(pts.shape[0]) if w0 is None else np.array(w0, dtype=np.float64, copy=True)
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        st = OtStats()
+        rc = self.lib.fn("ot_solve")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w), pts.shape[0],
+                                     boxa.ctypes.data_as(ctypes.c_void_p), float(tol),
+                                     int(max_newton), VGC_DEVICE_PTRS if dev else 0,
+                                     ctypes.byref(st))
+        self._check(rc)
+        return w, st.as_dict()
+
     # ---------------------------------------------------------------- Lloyd
     def lloyd(self, points, iters, box=UNIT_BOX, k0=0):
         """iters Lloyd iterations; returns the new points (same kind of array)."""
@@ -220,6 +245,10 @@ def knn(points, k, device=0):
 
 def lloyd(points, iters, box=UNIT_BOX, k0=0, device=0):
     return default_context(device).lloyd(points, iters, box, k0)
+
+
+def ot_solve(points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100, device=0):
+    return default_context(device).ot_solve(points, nu, w0, box, tol, max_newton)
 
 
 def random_float64(seed: int, count: int) -> np.ndarray:
@@ -309,3 +338,47 @@ class CentroidalVoronoiTesselation:
     def Lloyd_iterations(self, nb_iter: int):
         ctx = self.ctx or default_context()
         self.points_, self.last_stats = ctx.lloyd(self.points_, nb_iter, self.box, self.k0)
+
+
+class OptimalTransportMap3d:
+    """Semi-discrete optimal transport from the uniform density on a box to
+    Dirac masses at the seeds — the calls compute_OTM makes on Exploragram's
+    OptimalTransportMap3d (examples/exploragram/compute_OTM/main.cpp:283-297):
+    set_points, set_epsilon, optimize.  The source domain is a box here
+    (the reference takes a tetrahedral mesh)."""
+
+    def __init__(self, box=UNIT_BOX, ctx: Context | None = None):
+        self.box = box
+        self.ctx = ctx
+        self.epsilon = 1e-4
+        self.points_ = None
+        self.nu = None
+        self.weights = None
+        self.last_stats = None
+
+    def set_points(self, points, masses=None):
+        """Seeds and their target masses (default: equal, summing to the box volume)."""
+        self.points_ = points
+        n = points.shape[0]
+        vol = float(np.prod(np.asarray(self.box[3:]) - np.asarray(self.box[:3])))
+        if masses is None:
+            if _is_torch_cuda(points):
+                import torch
+                masses = torch.full((n,), vol / n, dtype=torch.float64, device=points.device)
+            else:
+                masses = np.full(n, vol / n)
+        self.nu = masses
+        self.weights = None
+
+    def set_epsilon(self, eps: float):
+        """Target max relative mass error of a cell."""
+        self.epsilon = eps
+
+    def set_initial_weights(self, w0):
+        self.weights = w0
+
+    def optimize(self, max_iter: int = 100):
+        ctx = self.ctx or default_context()
+        self.weights, self.last_stats = ctx.ot_solve(self.points_, self.nu, self.weights, self.box,
+                                                     self.epsilon, max_iter)
+        return self.weights

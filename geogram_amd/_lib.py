@@ -40,6 +40,16 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OtStats(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_uint32) for k in ("newton_iters", "cg_iters", "cell_passes", "halvings",
+                                                 "used_fallback_start", "converged")]
+                + [(k, ctypes.c_double) for k in ("err0", "final_err", "t_total_ms", "t_cells_ms",
+                                                   "t_cg_ms")])
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VgcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"vgc error {code}: {msg}")
@@ -69,6 +79,10 @@ def _declare(L, prefix):
     f = L[prefix + "lloyd"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
+    f = L[prefix + "ot_solve"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
+                  ctypes.POINTER(OtStats)]
 
 
 class _Lib:
@@ -83,7 +97,8 @@ class _Lib:
         self.cdll = ctypes.CDLL(path)
         self.prefix = prefix
         self._f = {}
-        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "create", "destroy"]
+        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "ot_solve", "create",
+                 "destroy"]
         for nm in names:
             self._f[prefix + nm] = getattr(self.cdll, prefix + nm)
         _declare(self._f, prefix)

@@ -10,19 +10,21 @@
 #include <vector>
 
 #include "../../include/vgc.h"
+#include "ot_solver.h"
 #include "pipeline.h"
 
 namespace vgc {
 
 template <class B>
 struct Ctx {
-    explicit Ctx(int dev) : be(dev), eng(be), device(dev) {}
+    explicit Ctx(int dev) : be(dev), eng(be), ot(be, eng), device(dev) {}
     ~Ctx() {
         for (auto& s : stage)
             if (s.p) be.dfree(s.p);
     }
     B be;
     Engine<B> eng;
+    OtSolver<B> ot;
     int device;
     std::string err;
     struct Buf {
@@ -277,6 +279,87 @@ int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t 
         fill_stats(es, stats);
         if (es.nb_failed || es.nb_uncertain) {
             ctx->err = "vgc_lloyd: flagged cells";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                 const double* box, double tol, uint32_t max_newton, uint32_t flags,
+                 vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !nu || !w || !box || n < 1 || !(tol > 0.0)) {
+            ctx->err = "vgc_ot_solve: bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+            ctx->err = "vgc_ot_solve: degenerate box";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        pb.box.assign(box, box + 6);
+        pb.k0 = 0;
+        const double *dx = xyz, *dnu = nu;
+        double* dw = w;
+        if (!dev) {
+            double* sx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            double* sn = (double*)ctx->staging(5, sizeof(double) * (size_t)n);
+            double* sw = (double*)ctx->staging(6, sizeof(double) * (size_t)n);
+            be.h2d(sx, xyz, sizeof(double) * 3 * (size_t)n);
+            be.h2d(sn, nu, sizeof(double) * (size_t)n);
+            be.h2d(sw, w, sizeof(double) * (size_t)n);
+            dx = sx;
+            dnu = sn;
+            dw = sw;
+        }
+        // the target masses must add up to the domain volume
+        std::vector<double> hnu(n);
+        be.d2h(hnu.data(), dnu, sizeof(double) * (size_t)n);
+        double tot = 0.0, mn = 1e300;
+        for (double v : hnu) {
+            tot += v;
+            mn = std::min(mn, v);
+        }
+        const double V = (box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
+        if (!(mn > 0.0) || std::fabs(tot - V) > 1e-9 * V) {
+            ctx->err = "vgc_ot_solve: target masses must be positive and sum to the box volume";
+            return VGC_ERR_ARG;
+        }
+        pb.d_xyz = dx;
+        OtStats os;
+        const uint32_t maxn = max_newton ? max_newton : 100;
+        if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os)) {
+            ctx->err = "vgc_ot_solve: " + ctx->eng.error;
+            return VGC_ERR_INTERNAL;
+        }
+        if (!dev) be.d2h(w, dw, sizeof(double) * (size_t)n);
+        be.sync();
+        if (stats) {
+            stats->newton_iters = (uint32_t)os.newton_iters;
+            stats->cg_iters = (uint32_t)os.cg_iters;
+            stats->cell_passes = (uint32_t)os.cell_passes;
+            stats->halvings = (uint32_t)os.halvings;
+            stats->used_fallback_start = (uint32_t)os.used_fallback_start;
+            stats->converged = (uint32_t)os.converged;
+            stats->err0 = os.err0;
+            stats->final_err = os.final_err;
+            stats->t_total_ms = os.t_total_ms;
+            stats->t_cells_ms = os.t_cells_ms;
+            stats->t_cg_ms = os.t_cg_ms;
+        }
+        if (!os.converged) {
+            ctx->err = "vgc_ot_solve: not converged";
             return VGC_PARTIAL;
         }
         return VGC_OK;

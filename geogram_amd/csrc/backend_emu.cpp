@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "api_impl.h"
+#include "ot_kernels.h"
 
 namespace vgc {
 
@@ -234,6 +235,93 @@ struct EmuBackend {
         }
     }
 
+    // ----------------------------------------------------------------- OT
+    // reductions in the order of the device kernels (512 blocks x 256
+    // threads, grid-stride, halving-stride block trees)
+    static constexpr int kRedBlocks = 512;
+    static double tree_sum(double* v) {
+        for (int off = 128; off > 0; off >>= 1)
+            for (int t = 0; t < off; ++t) v[t] += v[t + off];
+        return v[0];
+    }
+    template <class F>
+    static double red_sum(int n, F f) {
+        std::vector<double> part(kRedBlocks);
+        double sh[256];
+        for (int b = 0; b < kRedBlocks; ++b) {
+            for (int t = 0; t < 256; ++t) {
+                double s = 0.0;
+                for (long long i = (long long)b * 256 + t; i < n; i += (long long)kRedBlocks * 256)
+                    s += f((int)i);
+                sh[t] = s;
+            }
+            part[b] = tree_sum(sh);
+        }
+        for (int t = 0; t < 256; ++t) {
+            double s = 0.0;
+            for (int i = t; i < kRedBlocks; i += 256) s += part[i];
+            sh[t] = s;
+        }
+        return tree_sum(sh);
+    }
+    void ot_dot(int n, const double* a, const double* b, double* out) {
+        *out = red_sum(n, [&](int i) { return a[i] * b[i]; });
+    }
+    void ot_mass_stats(int n, const double* m, const double* nu, double* out) {
+        double mx = 0.0, mn = INFINITY;
+        for (int i = 0; i < n; ++i) {
+            mx = std::fmax(mx, std::fabs(m[i] - nu[i]) / nu[i]);
+            mn = std::fmin(mn, m[i]);
+        }
+        out[0] = mx;
+        out[1] = red_sum(n, [&](int i) {
+            double d = m[i] - nu[i];
+            return d * d;
+        });
+        out[2] = mn;
+    }
+    void ot_min(int n, const double* a, double* out) {
+        double mn = INFINITY;
+        for (int i = 0; i < n; ++i) mn = std::fmin(mn, a[i]);
+        *out = mn;
+    }
+    void ot_hessian(int n, const double* xyz, const long long* ptr, const int* col,
+                    const double* area, double* val, double* diag) {
+        for (int i = 0; i < n; ++i) ot_hessian_row(i, xyz, ptr, col, area, val, diag);
+    }
+    void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,
+                 const double* x, double* y) {
+        for (int i = 0; i < n; ++i) y[i] = ot_spmv_row(i, ptr, col, val, diag, x);
+    }
+    void ot_rhs(int n, const double* nu, const double* m, double* b) {
+        for (int i = 0; i < n; ++i) b[i] = nu[i] - m[i];
+    }
+    void ot_axpy(int n, const double* w, double t, const double* d, double* out) {
+        for (int i = 0; i < n; ++i) out[i] = w[i] + t * d[i];
+    }
+    void ot_pcg_init(int n, const double* b, const double* diag, double* x, double* r, double* z,
+                     double* p) {
+        for (int i = 0; i < n; ++i) {
+            double ri = b[i], zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+            x[i] = 0.0;
+            r[i] = ri;
+            z[i] = zi;
+            p[i] = zi;
+        }
+    }
+    void ot_pcg_update(int n, double alpha, const double* p, const double* q, const double* diag,
+                       double* x, double* r, double* z) {
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            double ri = r[i] - alpha * q[i];
+            r[i] = ri;
+            z[i] = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+        }
+    }
+    void ot_pcg_dir(int n, double beta, const double* z, double* p) {
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         for (int q = 0; q < n; ++q) {
@@ -280,6 +368,11 @@ int vgc_emu_cells_batch(void* ctx, const double* xyz, const double* w, const uin
                         uint64_t cap, vgc_stats* st) {
     return vgc::api_cells((EmuCtx*)ctx, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr,
                           nbr, farea, cap, st);
+}
+int vgc_emu_ot_solve(void* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                     const double* box, double tol, uint32_t max_newton, uint32_t flags,
+                     vgc_ot_stats* stats) {
+    return vgc::api_ot_solve((EmuCtx*)ctx, xyz, nu, w, n, box, tol, max_newton, flags, stats);
 }
 int vgc_emu_det4_sign(const double* rows) {
     return vgc::det4_sign_exact(rows, rows + 4, rows + 8, rows + 12);

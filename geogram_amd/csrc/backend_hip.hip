@@ -20,6 +20,7 @@
 #include <string>
 
 #include "api_impl.h"
+#include "ot_kernels.h"
 
 #define VGC_HIP_CHECK(x)                                                              \
     do {                                                                              \
@@ -207,6 +208,184 @@ __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restr
     }
 }
 
+// ------------------------------------------------------------- OT kernels
+// Deterministic reductions: kRedBlocks blocks of 256 threads; thread t of
+// block b accumulates elements b*256+t + k*kRedBlocks*256 in order, the block
+// reduces by halving strides, and one block reduces the partials the same way.
+constexpr int kRedBlocks = 512;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += a[i] * b[i];
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, double* out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) s += part[i];
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// [0] max |m-nu|/nu  [1] sum (m-nu)^2  [2] min m, per block then final
+__global__ __launch_bounds__(256) void k_mass_partial(int n, const double* __restrict__ m,
+                                                      const double* __restrict__ nu, double* part) {
+    __shared__ double sh[256];
+    double mx = 0.0, s2 = 0.0, mn = INFINITY;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
+        double d = m[i] - nu[i];
+        mx = fmax(mx, fabs(d) / nu[i]);
+        s2 += d * d;
+        mn = fmin(mn, m[i]);
+    }
+    double t = block_sum(s2, sh);
+    __syncthreads();
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    double tmx = sh[0];
+    __syncthreads();
+    sh[threadIdx.x] = mn;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tmx;
+        part[kRedBlocks + blockIdx.x] = t;
+        part[2 * kRedBlocks + blockIdx.x] = sh[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mass_final(const double* __restrict__ part, double* out) {
+    __shared__ double sh[256];
+    double mx = 0.0, s2 = 0.0, mn = INFINITY;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
+        mx = fmax(mx, part[i]);
+        s2 += part[kRedBlocks + i];
+        mn = fmin(mn, part[2 * kRedBlocks + i]);
+    }
+    double t = block_sum(s2, sh);
+    __syncthreads();
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    double tmx = sh[0];
+    __syncthreads();
+    sh[threadIdx.x] = mn;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = tmx;
+        out[1] = t;
+        out[2] = sh[0];
+    }
+}
+
+__global__ void k_ot_hessian(int n, const double* __restrict__ xyz, const long long* __restrict__ ptr,
+                             const int* __restrict__ col, const double* __restrict__ area,
+                             double* val, double* diag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ot_hessian_row(i, xyz, ptr, col, area, val, diag);
+}
+
+__global__ void k_ot_spmv(int n, const long long* __restrict__ ptr, const int* __restrict__ col,
+                          const double* __restrict__ val, const double* __restrict__ diag,
+                          const double* __restrict__ x, double* y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ot_spmv_row(i, ptr, col, val, diag, x);
+}
+
+__global__ void k_ot_rhs(int n, const double* __restrict__ nu, const double* __restrict__ m,
+                         double* b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = nu[i] - m[i];
+}
+
+__global__ void k_ot_axpy(int n, const double* __restrict__ w, double t,
+                          const double* __restrict__ d, double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = w[i] + t * d[i];
+}
+
+__global__ void k_ot_pcg_init(int n, const double* __restrict__ b, const double* __restrict__ diag,
+                              double* x, double* r, double* z, double* p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ri = b[i];
+    double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+}
+
+__global__ void k_ot_pcg_update(int n, double alpha, const double* __restrict__ p,
+                                const double* __restrict__ q, const double* __restrict__ diag,
+                                double* x, double* r, double* z) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += alpha * p[i];
+    double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    z[i] = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+}
+
+__global__ void k_ot_pcg_dir(int n, double beta, const double* __restrict__ z, double* p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_min_partial(int n, const double* __restrict__ a, double* part) {
+    __shared__ double sh[256];
+    double mn = INFINITY;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) mn = fmin(mn, a[i]);
+    sh[threadIdx.x] = mn;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_min_final(const double* __restrict__ part, double* out) {
+    __shared__ double sh[256];
+    double mn = INFINITY;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) mn = fmin(mn, part[i]);
+    sh[threadIdx.x] = mn;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
 // ----------------------------------------------------------------- backend
@@ -223,6 +402,7 @@ struct HipBackend {
             for (auto& x : e)
                 if (x) (void)hipEventDestroy(x);
         if (temp_) (void)hipFree(temp_);
+        if (red_) (void)hipFree(red_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     hipStream_t stream() const { return stream_; }
@@ -396,6 +576,59 @@ struct HipBackend {
         k_lloyd_update<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, vol, mg, n);
         check();
     }
+    // ----------------------------------------------------------------- OT
+    double* red_part() {
+        if (!red_) VGC_HIP_CHECK(hipMalloc(&red_, sizeof(double) * 3 * kRedBlocks));
+        return red_;
+    }
+    void ot_dot(int n, const double* a, const double* b, double* out) {
+        k_dot_partial<<<kRedBlocks, 256, 0, stream_>>>(n, a, b, red_part());
+        k_sum_final<<<1, 256, 0, stream_>>>(red_part(), out);
+        check();
+    }
+    void ot_mass_stats(int n, const double* m, const double* nu, double* out) {
+        k_mass_partial<<<kRedBlocks, 256, 0, stream_>>>(n, m, nu, red_part());
+        k_mass_final<<<1, 256, 0, stream_>>>(red_part(), out);
+        check();
+    }
+    void ot_min(int n, const double* a, double* out) {
+        k_min_partial<<<kRedBlocks, 256, 0, stream_>>>(n, a, red_part());
+        k_min_final<<<1, 256, 0, stream_>>>(red_part(), out);
+        check();
+    }
+    void ot_hessian(int n, const double* xyz, const long long* ptr, const int* col,
+                    const double* area, double* val, double* diag) {
+        k_ot_hessian<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, ptr, col, area, val, diag);
+        check();
+    }
+    void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,
+                 const double* x, double* y) {
+        k_ot_spmv<<<blocks_for(n, 256), 256, 0, stream_>>>(n, ptr, col, val, diag, x, y);
+        check();
+    }
+    void ot_rhs(int n, const double* nu, const double* m, double* b) {
+        k_ot_rhs<<<blocks_for(n, 256), 256, 0, stream_>>>(n, nu, m, b);
+        check();
+    }
+    void ot_axpy(int n, const double* w, double t, const double* d, double* out) {
+        k_ot_axpy<<<blocks_for(n, 256), 256, 0, stream_>>>(n, w, t, d, out);
+        check();
+    }
+    void ot_pcg_init(int n, const double* b, const double* diag, double* x, double* r, double* z,
+                     double* p) {
+        k_ot_pcg_init<<<blocks_for(n, 256), 256, 0, stream_>>>(n, b, diag, x, r, z, p);
+        check();
+    }
+    void ot_pcg_update(int n, double alpha, const double* p, const double* q, const double* diag,
+                       double* x, double* r, double* z) {
+        k_ot_pcg_update<<<blocks_for(n, 256), 256, 0, stream_>>>(n, alpha, p, q, diag, x, r, z);
+        check();
+    }
+    void ot_pcg_dir(int n, double beta, const double* z, double* p) {
+        k_ot_pcg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, beta, z, p);
+        check();
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         k_knn_to_caller<<<blocks_for(n, 256), 256, 0, stream_>>>(orig, list, d2, n, K, out, outd);
@@ -408,6 +641,7 @@ private:
     hipEvent_t ev_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     void* temp_ = nullptr;
     size_t temp_bytes_ = 0;
+    double* red_ = nullptr;
 };
 
 typedef Ctx<HipBackend> HipCtx;
@@ -478,6 +712,14 @@ int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t
     if (!ctx) return VGC_ERR_ARG;
     if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
     return vgc::api_lloyd(ctx->impl, xyz, n, box, iters, k0, flags, stats);
+}
+
+int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                 const double* box, double tol, uint32_t max_newton, uint32_t flags,
+                 vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_ot_solve(ctx->impl, xyz, nu, w, n, box, tol, max_newton, flags, stats);
 }
 
 int vgc_random_float64(uint64_t seed, double* out, uint64_t count) {

@@ -1,0 +1,49 @@
+// Semi-discrete optimal transport: per-element kernels shared by the gfx950
+// backend and the CPU emulation.
+//
+// The Laguerre cells of seeds p_i with weights w_i (power distance
+// |x - p_i|^2 - w_i, the convention of PeriodicDelaunay3d::set_weights,
+// delaunay/periodic_delaunay_3d.cpp:3093, 3719-3726) have masses m_i(w) for
+// the uniform density on the box.  Newton's method solves m(w) = nu with the
+// Jacobian
+//     dm_i/dw_j = -A_ij / (2 |p_i - p_j|)   (j a Laguerre neighbour of i),
+//     dm_i/dw_i = sum_j A_ij / (2 |p_i - p_j|),
+// A_ij the area of the facet shared by the two cells: a weighted graph
+// Laplacian, symmetric positive semi-definite, kernel = constants.  It is the
+// Newton system the Exploragram solver called by compute_OTM
+// (examples/exploragram/compute_OTM/main.cpp:283-297) hands to OpenNL.
+#pragma once
+
+#include "types.h"
+
+namespace vgc {
+
+// Row i of the Hessian from the cells' neighbour CSR (caller numbering):
+// off-diagonal values into val[ptr[i]..ptr[i+1]) and the diagonal.
+VGC_SD void ot_hessian_row(int i, const double* __restrict__ xyz, const long long* __restrict__ ptr,
+                           const int* __restrict__ col, const double* __restrict__ area,
+                           double* __restrict__ val, double* __restrict__ diag) {
+    const double px = xyz[3 * (long long)i], py = xyz[3 * (long long)i + 1],
+                 pz = xyz[3 * (long long)i + 2];
+    double d = 0.0;
+    for (long long k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const int j = col[k];
+        const double ex = xyz[3 * (long long)j] - px, ey = xyz[3 * (long long)j + 1] - py,
+                     ez = xyz[3 * (long long)j + 2] - pz;
+        const double h = -area[k] / (2.0 * sqrt(ex * ex + ey * ey + ez * ez));
+        val[k] = h;
+        d -= h;
+    }
+    diag[i] = d;
+}
+
+// y_i = diag_i x_i + sum_k val_k x_col_k
+VGC_SD double ot_spmv_row(int i, const long long* __restrict__ ptr, const int* __restrict__ col,
+                          const double* __restrict__ val, const double* __restrict__ diag,
+                          const double* __restrict__ x) {
+    double s = diag[i] * x[i];
+    for (long long k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
+    return s;
+}
+
+}  // namespace vgc

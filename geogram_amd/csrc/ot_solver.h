@@ -1,0 +1,192 @@
+// Semi-discrete optimal transport solver (host driver, written once against a
+// backend): uniform density on a box to Dirac masses nu_i at the seeds.
+//
+// Damped Newton with the step rule of Kitagawa, Merigot and Thibert
+// ("Convergence of a Newton algorithm for semi-discrete optimal transport",
+// JEMS 2019): from weights w, solve H d = nu - m(w) with the Laguerre
+// Hessian H (ot_kernels.h) by Jacobi-preconditioned conjugate gradients in
+// FP64; accept w + t d for the largest t in {1, 1/2, 1/4, ...} such that
+// every cell keeps mass >= eps0 = min(min nu, min m(w0)) / 2 and
+// |m - nu|_2 decreases by the factor (1 - t/2).  Stops when
+// max_i |m_i - nu_i| / nu_i <= tol.  The Exploragram solver behind
+// OptimalTransportMap3d::optimize (called at
+// examples/exploragram/compute_OTM/main.cpp:283-297) is not part of the
+// geogram tree; this driver follows the published method.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "pipeline.h"
+
+namespace vgc {
+
+struct OtStats {
+    int newton_iters = 0;
+    int cg_iters = 0;
+    int cell_passes = 0;
+    int halvings = 0;
+    int used_fallback_start = 0;
+    double err0 = 0, final_err = 0;
+    double t_total_ms = 0, t_cells_ms = 0, t_cg_ms = 0;
+    int converged = 0;
+};
+
+template <class B>
+class OtSolver {
+public:
+    OtSolver(B& be, Engine<B>& E) : be_(be), E_(E) {}
+    ~OtSolver() {
+        for (auto& s : slots_)
+            if (s.p) be_.dfree(s.p);
+    }
+
+    // d_xyz (3n), d_nu (n), d_w (n, in: start, out: solution): device memory,
+    // caller order.  cg_tol: relative residual of the Newton systems.
+    int solve(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
+              double cg_tol, int cg_max, OtStats& st) {
+        auto t0 = std::chrono::steady_clock::now();
+        const int n = pb.n;
+        pb.want_neighbors = true;
+        pb.d_w = d_w;
+        double* d_b = (double*)slot(0, sizeof(double) * n);
+        double* d_x = (double*)slot(1, sizeof(double) * n);
+        double* d_r = (double*)slot(2, sizeof(double) * n);
+        double* d_z = (double*)slot(3, sizeof(double) * n);
+        double* d_p = (double*)slot(4, sizeof(double) * n);
+        double* d_q = (double*)slot(5, sizeof(double) * n);
+        double* d_diag = (double*)slot(6, sizeof(double) * n);
+        double* d_wt = (double*)slot(7, sizeof(double) * n);
+        double* d_scal = (double*)slot(8, sizeof(double) * 8);
+        EngineStats es;
+
+        auto cells = [&](const double* w) -> int {
+            auto tc = std::chrono::steady_clock::now();
+            pb.d_w = w;
+            int rc = E_.cells(pb, es);
+            st.cell_passes++;
+            st.t_cells_ms += ms_since(tc);
+            return rc;
+        };
+        // mass statistics: [0] max |m-nu|/nu, [1] |m-nu|_2^2, [2] min m
+        auto masses = [&](double out[3]) {
+            be_.ot_mass_stats(n, E_.d_vol, d_nu, d_scal);
+            be_.d2h(out, d_scal, sizeof(double) * 3);
+        };
+
+        if (cells(d_w)) return -1;
+        double ms[3];
+        masses(ms);
+        if (!(ms[2] > 0.0)) {
+            // the warm start leaves an empty cell: start again from w = 0
+            be_.dzero(d_w, sizeof(double) * n);
+            st.used_fallback_start = 1;
+            if (cells(d_w)) return -1;
+            masses(ms);
+        }
+        double numin = 0;
+        be_.ot_min(n, d_nu, d_scal + 3);
+        be_.d2h(&numin, d_scal + 3, sizeof(double));
+        const double eps0 = 0.5 * std::min(numin, ms[2]);
+        st.err0 = ms[0];
+        double err = ms[0], g2 = ms[1];
+        while (err > tol && st.newton_iters < max_newton) {
+            // Newton system H d = nu - m
+            auto tg = std::chrono::steady_clock::now();
+            const long long nnz = E_.total_neighbors;
+            double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
+            be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
+            be_.ot_rhs(n, d_nu, E_.d_vol, d_b);
+            int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_scal,
+                         cg_tol, cg_max);
+            st.cg_iters += it;
+            st.t_cg_ms += ms_since(tg);
+            // damped step
+            double t = 1.0;
+            bool ok = false;
+            for (int h = 0; h < 40; ++h) {
+                be_.ot_axpy(n, d_w, t, d_x, d_wt);
+                if (cells(d_wt)) return -1;
+                double m2[3];
+                masses(m2);
+                if (m2[2] >= eps0 && std::sqrt(m2[1]) <= (1.0 - 0.5 * t) * std::sqrt(g2)) {
+                    err = m2[0];
+                    g2 = m2[1];
+                    ok = true;
+                    break;
+                }
+                t *= 0.5;
+                st.halvings++;
+            }
+            if (!ok) break;
+            be_.d2d(d_w, d_wt, sizeof(double) * n);
+            st.newton_iters++;
+        }
+        be_.sync();
+        st.final_err = err;
+        st.converged = err <= tol;
+        st.t_total_ms = ms_since(t0);
+        return 0;
+    }
+
+private:
+    // Jacobi-preconditioned CG from x = 0; returns the iterations done.
+    int pcg(int n, const long long* ptr, const int* col, const double* val, const double* diag,
+            const double* b, double* x, double* r, double* z, double* p, double* q, double* scal,
+            double tol, int maxit) {
+        be_.ot_pcg_init(n, b, diag, x, r, z, p);
+        double h[2];
+        be_.ot_dot(n, r, z, scal + 0);
+        be_.ot_dot(n, b, b, scal + 1);
+        be_.d2h(h, scal, sizeof(double) * 2);
+        double rz = h[0];
+        const double bb = h[1];
+        if (!(bb > 0.0)) return 0;
+        int it = 0;
+        for (; it < maxit; ++it) {
+            be_.ot_spmv(n, ptr, col, val, diag, p, q);
+            be_.ot_dot(n, p, q, scal + 2);
+            double pq;
+            be_.d2h(&pq, scal + 2, sizeof(double));
+            if (!(pq > 0.0)) break;
+            const double alpha = rz / pq;
+            be_.ot_pcg_update(n, alpha, p, q, diag, x, r, z);
+            be_.ot_dot(n, r, z, scal + 0);
+            be_.ot_dot(n, r, r, scal + 1);
+            be_.d2h(h, scal, sizeof(double) * 2);
+            const double rz_new = h[0], rr = h[1];
+            if (rr <= tol * tol * bb) {
+                ++it;
+                break;
+            }
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            be_.ot_pcg_dir(n, beta, z, p);
+        }
+        return it;
+    }
+
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    void* slot(int id, size_t bytes) {
+        if ((int)slots_.size() <= id) slots_.resize(id + 1);
+        Buf& b = slots_[id];
+        if (b.bytes < bytes) {
+            if (b.p) be_.dfree(b.p);
+            b.bytes = std::max(bytes, b.bytes + b.bytes / 4);
+            b.p = be_.dalloc(b.bytes);
+        }
+        return b.p;
+    }
+    static double ms_since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    }
+
+    B& be_;
+    Engine<B>& E_;
+    std::vector<Buf> slots_;
+};
+
+}  // namespace vgc

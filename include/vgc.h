@@ -109,6 +109,32 @@ int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w,
 int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
               uint32_t k0, uint32_t flags, vgc_stats* stats);
 
+typedef struct vgc_ot_stats {
+    uint32_t newton_iters;        /* accepted Newton steps                       */
+    uint32_t cg_iters;            /* conjugate-gradient iterations, all steps    */
+    uint32_t cell_passes;         /* Laguerre cell evaluations (line search too) */
+    uint32_t halvings;            /* step halvings of the damped Newton rule     */
+    uint32_t used_fallback_start; /* the given start had an empty cell: w = 0    */
+    uint32_t converged;
+    double err0, final_err;       /* max_i |m_i - nu_i| / nu_i, first and last   */
+    double t_total_ms, t_cells_ms, t_cg_ms;
+} vgc_ot_stats;
+
+/* Semi-discrete optimal transport from the uniform density on box to the
+   masses nu[n] at the seeds (nu must sum to the box volume): finds Laguerre
+   weights w with cell masses m_i(w) = nu_i up to
+   max_i |m_i - nu_i| / nu_i <= tol, by damped Newton (Kitagawa-Merigot-
+   Thibert step rule) with a Jacobi-preconditioned CG on the Laguerre
+   Hessian.  w is the start on input (zeros: plain start; a start with an
+   empty cell falls back to zeros) and the solution on output.
+   Replaces OptimalTransportMap3d::set_points / set_epsilon / optimize
+   (examples/exploragram/compute_OTM/main.cpp:283-297; the solver itself is
+   Exploragram's, outside the geogram tree).  max_newton 0 = 100.
+   Returns VGC_PARTIAL if not converged. */
+int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                 const double* box, double tol, uint32_t max_newton, uint32_t flags,
+                 vgc_ot_stats* stats);
+
 /* count doubles of the stream std::mt19937_64(seed) +
    uniform_real_distribution<double>(0,1), the generator behind
    Numeric::random_reset(seed) / random_float64() (basic/numeric.cpp:76-103). */

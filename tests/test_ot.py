@@ -1,0 +1,122 @@
+"""Semi-discrete optimal transport: the Laguerre Hessian against finite
+differences of the oracle's cell masses, the solver against a numpy/scipy
+restatement of the same damped Newton method (direct sparse solves), and the
+solution's masses re-measured by the oracle.  The Exploragram solver that
+compute_OTM calls is not in the geogram tree, so its iteration trajectory is
+not pinned; the method (Kitagawa-Merigot-Thibert damped Newton) is."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import geogram_amd as ga
+import oracle
+
+
+def hessian_from_cells(x, r):
+    """dm/dw from a CellResult: -A_ij / (2 |p_i - p_j|) off the diagonal."""
+    n = x.shape[0]
+    ptr = r.nbr_ptr.astype(np.int64)
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    cols = r.nbr.astype(np.int64)
+    d = np.linalg.norm(x[cols] - x[rows], axis=1)
+    v = -r.farea / (2 * d)
+    H = sp.csr_matrix((v, (rows, cols)), shape=(n, n))
+    H = H - sp.diags(np.asarray(H.sum(axis=1)).ravel())
+    return H
+
+
+def test_hessian_matches_finite_differences(emu_ctx):
+    rng = np.random.default_rng(0)
+    n = 400
+    x = rng.random((n, 3))
+    w = rng.random(n) * 1e-3
+    r = emu_ctx.cells(x, weights=w)
+    H = hessian_from_cells(x, r).toarray()
+    h = 1e-7
+    for j in rng.choice(n, 12, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += h
+        wm[j] -= h
+        mp = oracle.cells(x, w=wp, centred=True)["vol"]
+        mm = oracle.cells(x, w=wm, centred=True)["vol"]
+        fd = (mp - mm) / (2 * h)
+        assert np.allclose(H[:, j], fd, rtol=2e-5, atol=1e-9 * np.abs(H[:, j]).max())
+    # Laplacian structure: symmetric, rows sum to zero
+    assert np.allclose(H, H.T, rtol=1e-9, atol=1e-12 * np.abs(H).max())
+    assert np.abs(H.sum(axis=1)).max() < 1e-10 * np.abs(H).max()
+
+
+def kmt_newton_reference(x, nu, tol, w0=None, max_newton=100):
+    """The damped Newton method with exact (sparse direct) Newton steps and the
+    oracle's Laguerre cells."""
+    n = x.shape[0]
+    w = np.zeros(n) if w0 is None else w0.copy()
+
+    def cells(w):
+        o = oracle.cells(x, w=w, centred=True, stride=256)
+        class R:  # CellResult-like view
+            pass
+        R.vol = o["vol"]
+        cnt = o["cnt"]
+        R.nbr_ptr = np.r_[0, np.cumsum(cnt)]
+        R.nbr = np.concatenate([o["nbr"][i, :cnt[i]] for i in range(n)])
+        R.farea = np.concatenate([o["farea"][i, :cnt[i]] for i in range(n)])
+        return R
+
+    r = cells(w)
+    m = r.vol
+    eps0 = 0.5 * min(nu.min(), m.min())
+    err = np.max(np.abs(m - nu) / nu)
+    g2 = np.sum((m - nu) ** 2)
+    it = 0
+    while err > tol and it < max_newton:
+        H = hessian_from_cells(x, r).tocsc()
+        # fix the additive constant: ground the last weight
+        d = np.zeros(n)
+        d[:-1] = spla.spsolve(H[:-1, :-1], (nu - m)[:-1])
+        d -= d.mean()
+        t = 1.0
+        while True:
+            r2 = cells(w + t * d)
+            m2 = r2.vol
+            if m2.min() >= eps0 and np.sqrt(np.sum((m2 - nu) ** 2)) <= (1 - t / 2) * np.sqrt(g2):
+                break
+            t /= 2
+        w = w + t * d
+        r, m = r2, m2
+        err = np.max(np.abs(m - nu) / nu)
+        g2 = np.sum((m - nu) ** 2)
+        it += 1
+    return w, it, err
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_ot_solver_vs_reference_newton(emu_ctx, kind):
+    n = 1500
+    x = ga.sample_points(kind, 30 + kind, n)
+    rng = np.random.default_rng(kind)
+    nu = np.exp(rng.normal(0.0, 0.5, n)) if kind == 2 else np.ones(n)
+    nu /= nu.sum()
+    w, st = emu_ctx.ot_solve(x, nu, tol=1e-6)
+    assert st["converged"] == 1 and st["final_err"] <= 1e-6
+    wr, itr, errr = kmt_newton_reference(x, nu, 1e-6)
+    assert st["newton_iters"] == itr
+    # weights are defined up to a constant
+    dw = (w - w.mean()) - (wr - wr.mean())
+    assert np.abs(dw).max() <= 1e-6 * np.abs(wr - wr.mean()).max()
+    # masses of the solution, re-measured by the oracle
+    o = oracle.cells(x, w=w, centred=True, stride=256)
+    assert np.max(np.abs(o["vol"] - nu) / nu) <= 1.01e-6
+
+
+def test_ot_fallback_start_and_argument_checks(emu_ctx):
+    n = 800
+    x = ga.sample_points(0, 5, n)
+    nu = np.full(n, 1.0 / n)
+    w0 = np.zeros(n)
+    w0[:5] = 1.0  # five seeds swallow everything: empty cells at the start
+    w, st = emu_ctx.ot_solve(x, nu, w0=w0, tol=1e-5)
+    assert st["used_fallback_start"] == 1 and st["converged"] == 1
+    with pytest.raises(ga.VgcError):
+        emu_ctx.ot_solve(x, nu * 2, tol=1e-5)  # masses do not sum to the volume
