@@ -1,0 +1,186 @@
+// C++ mirror of geogram's entry points for the meshless cell path, header-only
+// over the C ABI of vgc.h (link with libvgc.so).  Class and method names follow
+// geogram so a caller switches by changing the type:
+//   NearestNeighborSearch          points/nn_search.h:69-180
+//   RestrictedVoronoiDiagram       voronoi/RVD.h:267-290 (volumetric, box domain)
+//   CentroidalVoronoiTesselation   voronoi/CVT.h:139-164
+//   OptimalTransportMap3d          examples/exploragram/compute_OTM/main.cpp:283-297
+// Errors are reported as std::runtime_error (geogram's shims would map them to
+// geo_assert or its own exceptions).
+#ifndef VGC_GEOGRAM_HPP
+#define VGC_GEOGRAM_HPP
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vgc.h"
+
+namespace vgc {
+namespace geogram {
+
+typedef unsigned int index_t;
+
+class Context {
+public:
+    explicit Context(int device = 0) {
+        int rc = vgc_create(&ctx_, device);
+        if (rc != VGC_OK) throw std::runtime_error("vgc: no usable GPU (vgc_create failed)");
+    }
+    ~Context() { vgc_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    vgc_ctx* get() const { return ctx_; }
+    void check(int rc) const {
+        if (rc < 0) throw std::runtime_error(std::string("vgc: ") + vgc_last_error(ctx_));
+    }
+
+private:
+    vgc_ctx* ctx_ = nullptr;
+};
+
+// All-points k nearest neighbours (k in 16, 24, 32, 48, 64), self excluded.
+class NearestNeighborSearch {
+public:
+    explicit NearestNeighborSearch(Context& ctx) : ctx_(ctx) {}
+    void set_points(index_t nb_points, const double* points) {
+        n_ = nb_points;
+        pts_ = points;
+    }
+    index_t nb_points() const { return n_; }
+    // neighbours[i*k + m], sq_dist[i*k + m] for every point i
+    void get_nearest_neighbors_all(index_t k, int* neighbours, float* sq_dist) const {
+        ctx_.check(vgc_knn(ctx_.get(), pts_, n_, k, 0, neighbours, sq_dist));
+    }
+
+private:
+    Context& ctx_;
+    index_t n_ = 0;
+    const double* pts_ = nullptr;
+};
+
+// Volumetric restricted Voronoi diagram of seeds in a box.
+class RestrictedVoronoiDiagram {
+public:
+    RestrictedVoronoiDiagram(Context& ctx, const double box[6]) : ctx_(ctx) {
+        for (int i = 0; i < 6; ++i) box_[i] = box[i];
+    }
+    void set_points(index_t nb_points, const double* points) {
+        n_ = nb_points;
+        pts_ = points;
+    }
+    // mg[3n], m[n]: caller-allocated and zeroed; the cell integrals are added
+    // (RVD_Nd_Impl::compute_centroids_in_volume accumulates the same way)
+    void compute_centroids(double* mg, double* m) {
+        std::vector<double> vol(n_), g(3 * (size_t)n_);
+        vgc_stats st;
+        ctx_.check(vgc_cells(ctx_.get(), pts_, nullptr, n_, box_, 0, VGC_NO_NEIGHBORS, vol.data(),
+                             g.data(), nullptr, nullptr, nullptr, 0, &st));
+        for (index_t i = 0; i < n_; ++i) {
+            m[i] += vol[i];
+            for (int c = 0; c < 3; ++c) mg[3 * (size_t)i + c] += g[3 * (size_t)i + c];
+        }
+    }
+    // Laguerre variant (power distance |x - p|^2 - w)
+    void compute_centroids(const double* weights, double* mg, double* m) {
+        std::vector<double> vol(n_), g(3 * (size_t)n_);
+        vgc_stats st;
+        ctx_.check(vgc_cells(ctx_.get(), pts_, weights, n_, box_, 0, VGC_NO_NEIGHBORS, vol.data(),
+                             g.data(), nullptr, nullptr, nullptr, 0, &st));
+        for (index_t i = 0; i < n_; ++i) {
+            m[i] += vol[i];
+            for (int c = 0; c < 3; ++c) mg[3 * (size_t)i + c] += g[3 * (size_t)i + c];
+        }
+    }
+    // per-cell neighbour CSR and facet areas (VBW::ConvexCell outputs)
+    void compute_cells(std::vector<double>& vol, std::vector<double>& mg,
+                       std::vector<uint64_t>& ptr, std::vector<int32_t>& nbr,
+                       std::vector<double>& farea, const double* weights = nullptr) {
+        vol.resize(n_);
+        mg.resize(3 * (size_t)n_);
+        ptr.resize((size_t)n_ + 1);
+        vgc_stats st;
+        uint64_t cap = 24ull * n_ + 64;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            nbr.resize(cap);
+            farea.resize(cap);
+            int rc = vgc_cells(ctx_.get(), pts_, weights, n_, box_, 0, 0, vol.data(), mg.data(),
+                               ptr.data(), nbr.data(), farea.data(), cap, &st);
+            if (rc == VGC_ERR_CAPACITY) {
+                cap = st.nb_neighbors;
+                continue;
+            }
+            ctx_.check(rc);
+            nbr.resize(st.nb_neighbors);
+            farea.resize(st.nb_neighbors);
+            return;
+        }
+        throw std::runtime_error("vgc: neighbour capacity");
+    }
+
+private:
+    Context& ctx_;
+    double box_[6];
+    index_t n_ = 0;
+    const double* pts_ = nullptr;
+};
+
+// Lloyd relaxation of seeds in a box (security radius on).
+class CentroidalVoronoiTesselation {
+public:
+    CentroidalVoronoiTesselation(Context& ctx, const double box[6]) : ctx_(ctx) {
+        for (int i = 0; i < 6; ++i) box_[i] = box[i];
+    }
+    void set_points(index_t nb_points, const double* points) {
+        points_.assign(points, points + 3 * (size_t)nb_points);
+    }
+    index_t nb_points() const { return index_t(points_.size() / 3); }
+    const double* embedding(index_t i) const { return points_.data() + 3 * (size_t)i; }
+    virtual void Lloyd_iterations(index_t nb_iter) {
+        vgc_stats st;
+        ctx_.check(vgc_lloyd(ctx_.get(), points_.data(), nb_points(), box_, nb_iter, 0, 0, &st));
+    }
+    virtual ~CentroidalVoronoiTesselation() {}
+
+private:
+    Context& ctx_;
+    double box_[6];
+    std::vector<double> points_;
+};
+
+// Semi-discrete optimal transport, uniform density on a box -> seeds.
+class OptimalTransportMap3d {
+public:
+    OptimalTransportMap3d(Context& ctx, const double box[6]) : ctx_(ctx) {
+        for (int i = 0; i < 6; ++i) box_[i] = box[i];
+    }
+    void set_points(index_t nb_points, const double* points, const double* masses = nullptr) {
+        points_.assign(points, points + 3 * (size_t)nb_points);
+        const double vol = (box_[3] - box_[0]) * (box_[4] - box_[1]) * (box_[5] - box_[2]);
+        if (masses)
+            nu_.assign(masses, masses + nb_points);
+        else
+            nu_.assign(nb_points, vol / nb_points);
+        w_.assign(nb_points, 0.0);
+    }
+    void set_epsilon(double eps) { eps_ = eps; }
+    void set_initial_weights(const double* w) { w_.assign(w, w + w_.size()); }
+    void optimize(index_t max_iter) {
+        ctx_.check(vgc_ot_solve(ctx_.get(), points_.data(), nu_.data(), w_.data(),
+                                index_t(nu_.size()), box_, eps_, max_iter, 0, &stats_));
+    }
+    const std::vector<double>& weights() const { return w_; }
+    const vgc_ot_stats& stats() const { return stats_; }
+
+private:
+    Context& ctx_;
+    double box_[6];
+    double eps_ = 1e-4;
+    std::vector<double> points_, nu_, w_;
+    vgc_ot_stats stats_{};
+};
+
+}  // namespace geogram
+}  // namespace vgc
+
+#endif

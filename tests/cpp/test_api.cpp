@@ -1,0 +1,42 @@
+// Uses the C++ mirror (include/vgc_geogram.hpp) the way geogram's callers use
+// the reference classes.  Without a GPU it checks that context creation fails
+// cleanly (exit 3); with one it runs CVT Lloyd, cells and OT and checks
+// invariants (exit 0 on success).
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "vgc_geogram.hpp"
+
+int main() {
+    using namespace vgc::geogram;
+    const double box[6] = {0, 0, 0, 1, 1, 1};
+    const unsigned n = 5000;
+    std::vector<double> x(3 * n);
+    vgc_sample_points(0, 42, n, x.data());
+    Context* ctx = nullptr;
+    try {
+        ctx = new Context(0);
+    } catch (const std::exception& e) {
+        std::printf("no GPU: %s\n", e.what());
+        return 3;
+    }
+    RestrictedVoronoiDiagram rvd(*ctx, box);
+    rvd.set_points(n, x.data());
+    std::vector<double> mg(3 * n, 0.0), m(n, 0.0);
+    rvd.compute_centroids(mg.data(), m.data());
+    double s = 0;
+    for (double v : m) s += v;
+    if (std::fabs(s - 1.0) > 1e-12) return 1;
+    CentroidalVoronoiTesselation cvt(*ctx, box);
+    cvt.set_points(n, x.data());
+    cvt.Lloyd_iterations(5);
+    OptimalTransportMap3d otm(*ctx, box);
+    otm.set_points(n, cvt.embedding(0));
+    otm.set_epsilon(1e-4);
+    otm.optimize(50);
+    if (!otm.stats().converged) return 2;
+    std::printf("ok: sum m = %.17g, OT newton %u\n", s, otm.stats().newton_iters);
+    delete ctx;
+    return 0;
+}
