@@ -185,9 +185,11 @@ class Context:
         return idx, d2
 
     # ------------------------------------------------------------------- OT
-    def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100):
+    def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100,
+                 judge_start=False):
         """Laguerre weights w with cell masses nu (uniform density on box).
-        Returns (w, stats); w0 is the start (None: zeros)."""
+        Returns (w, stats); w0 is the start (None: zeros).  judge_start: keep
+        w0 only if its mass residual beats the zero start's."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
@@ -203,10 +205,74 @@ class Context:
         st = OtStats()
         rc = self.lib.fn("ot_solve")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w), pts.shape[0],
                                      boxa.ctypes.data_as(ctypes.c_void_p), float(tol),
-                                     int(max_newton), VGC_DEVICE_PTRS if dev else 0,
+                                     int(max_newton),
+                                     (VGC_DEVICE_PTRS if dev else 0)
+                                     | (_lib.VGC_OT_JUDGE_START if judge_start else 0),
                                      ctypes.byref(st))
         self._check(rc)
         return w, st.as_dict()
+
+    # ------------------------------------------------- learned warm start
+    def ot_features(self, points, nu, box=UNIT_BOX):
+        """Per-seed MLP inputs (n x 6, float32) of the learned OT warm start."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            nu_ = nu.contiguous().to(torch.float64)
+            f = torch.empty((pts.shape[0], _lib.VGC_MLP_FEATURES), dtype=torch.float32,
+                            device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            nu_ = np.ascontiguousarray(nu, dtype=np.float64)
+            f = np.empty((pts.shape[0], _lib.VGC_MLP_FEATURES), dtype=np.float32)
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        rc = self.lib.fn("ot_features")(self.ctx, _ptr(pts), _ptr(nu_), pts.shape[0],
+                                        boxa.ctypes.data_as(ctypes.c_void_p),
+                                        VGC_DEVICE_PTRS if dev else 0, _ptr(f))
+        self._check(rc, allow_partial=False)
+        return f
+
+    def mlp_forward(self, params, feats):
+        dev = _is_torch_cuda(feats)
+        n = feats.shape[0]
+        if dev:
+            import torch
+            out = torch.empty(n, dtype=torch.float32, device=feats.device)
+        else:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            feats = np.ascontiguousarray(feats, dtype=np.float32)
+            out = np.empty(n, dtype=np.float32)
+        rc = self.lib.fn("mlp_forward")(self.ctx, _ptr(params), _ptr(feats), n,
+                                        VGC_DEVICE_PTRS if dev else 0, _ptr(out))
+        self._check(rc, allow_partial=False)
+        return out
+
+    def mlp_grad(self, params, feats, target, grad=None):
+        """Gradient of the mean squared error of the MLP over the rows;
+        returns (grad, loss)."""
+        dev = _is_torch_cuda(feats)
+        n = feats.shape[0]
+        if dev:
+            import torch
+            if grad is None:
+                grad = torch.empty(_lib.VGC_MLP_PARAMS, dtype=torch.float32, device=feats.device)
+        else:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            feats = np.ascontiguousarray(feats, dtype=np.float32)
+            target = np.ascontiguousarray(target, dtype=np.float32)
+            grad = np.empty(_lib.VGC_MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        rc = self.lib.fn("mlp_grad")(self.ctx, _ptr(params), _ptr(feats), _ptr(target), n,
+                                     VGC_DEVICE_PTRS if dev else 0, _ptr(grad), ctypes.byref(loss))
+        self._check(rc, allow_partial=False)
+        return grad, loss.value
+
+    def mlp_adam(self, params, grad, m, v, step, lr):
+        """One in-place Adam step on device tensors (params, m, v)."""
+        rc = self.lib.fn("mlp_adam")(self.ctx, _ptr(params), _ptr(grad), _ptr(m), _ptr(v),
+                                     int(step), float(lr), VGC_DEVICE_PTRS)
+        self._check(rc, allow_partial=False)
 
     # ---------------------------------------------------------------- Lloyd
     def lloyd(self, points, iters, box=UNIT_BOX, k0=0):
@@ -355,6 +421,7 @@ class OptimalTransportMap3d:
         self.nu = None
         self.weights = None
         self.last_stats = None
+        self.learned_params = None
 
     def set_points(self, points, masses=None):
         """Seeds and their target masses (default: equal, summing to the box volume)."""
@@ -377,8 +444,25 @@ class OptimalTransportMap3d:
     def set_initial_weights(self, w0):
         self.weights = w0
 
+    def use_learned_warm_start(self, params=None):
+        """Start Newton from the weights predicted by the trained MLP
+        (geogram_amd.warmstart); the zero start stays the fallback when the
+        prediction leaves a cell empty or has a larger mass residual.
+        params: None = the packaged network, False = off."""
+        if params is False:
+            self.learned_params = None
+            return
+        from . import warmstart
+        self.learned_params = warmstart.load_params() if params is None else params
+
     def optimize(self, max_iter: int = 100):
         ctx = self.ctx or default_context()
-        self.weights, self.last_stats = ctx.ot_solve(self.points_, self.nu, self.weights, self.box,
-                                                     self.epsilon, max_iter)
+        w0, judge = self.weights, False
+        if w0 is None and self.learned_params is not None:
+            from . import warmstart
+            w0 = warmstart.predict_weights(ctx, self.points_, self.nu, self.box,
+                                           self.learned_params)
+            judge = True
+        self.weights, self.last_stats = ctx.ot_solve(self.points_, self.nu, w0, self.box,
+                                                     self.epsilon, max_iter, judge_start=judge)
         return self.weights

@@ -24,6 +24,10 @@ VGC_ERR_INTERNAL = -4
 VGC_DEVICE_PTRS = 1
 VGC_NO_NEIGHBORS = 2
 VGC_COUNT_WORK = 4
+VGC_OT_JUDGE_START = 8
+
+VGC_MLP_PARAMS = 4673
+VGC_MLP_FEATURES = 6
 
 _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain nb_failed nb_empty "
              "nb_disconnected nb_clips nb_tests radius_entries").split()
@@ -83,6 +87,18 @@ def _declare(L, prefix):
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
                   ctypes.POINTER(OtStats)]
+    f = L[prefix + "ot_features"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _vp]
+    f = L[prefix + "mlp_forward"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _u32, _vp]
+    f = L[prefix + "mlp_grad"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_double)]
+    f = L[prefix + "mlp_adam"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _vp, _u32, ctypes.c_float, _u32]
 
 
 class _Lib:
@@ -97,8 +113,8 @@ class _Lib:
         self.cdll = ctypes.CDLL(path)
         self.prefix = prefix
         self._f = {}
-        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "ot_solve", "create",
-                 "destroy"]
+        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "ot_solve", "ot_features",
+                 "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:
             self._f[prefix + nm] = getattr(self.cdll, prefix + nm)
         _declare(self._f, prefix)
@@ -137,9 +153,23 @@ _gpu = None
 _emu = None
 
 
+def _bind_torch_runtime():
+    """torch ships its own HIP runtime (torch/lib/libamdhip64.so).  A process
+    can drive the GPU through one HIP runtime only: if libvgc.so were loaded
+    first it would map /opt/rocm's, and torch's would then find no GPU.
+    Importing torch first makes the dynamic loader bind libvgc.so to the
+    runtime already mapped (same SONAME), so both share one runtime and torch
+    device pointers are valid in the library."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def gpu() -> _Lib:
     global _gpu
     if _gpu is None:
+        _bind_torch_runtime()
         _gpu = _Lib(LIB_PATH, "vgc_")
     return _gpu
 

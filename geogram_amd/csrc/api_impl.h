@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/vgc.h"
+#include "mlp.h"
 #include "ot_solver.h"
 #include "pipeline.h"
 
@@ -339,7 +340,8 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         pb.d_xyz = dx;
         OtStats os;
         const uint32_t maxn = max_newton ? max_newton : 100;
-        if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os)) {
+        if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
+                          (flags & VGC_OT_JUDGE_START) != 0)) {
             ctx->err = "vgc_ot_solve: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
         }
@@ -362,6 +364,161 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
             ctx->err = "vgc_ot_solve: not converged";
             return VGC_PARTIAL;
         }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+// ------------------------------------------------------- OT warm start
+template <class B>
+int api_ot_features(Ctx<B>* ctx, const double* xyz, const double* nu, uint32_t n,
+                    const double* box, uint32_t flags, float* feats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !nu || !box || !feats || n < 17) {
+            ctx->err = "vgc_ot_features: bad argument (n must exceed 16)";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        const double *dx = xyz, *dnu = nu;
+        if (!dev) {
+            double* sx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            double* sn = (double*)ctx->staging(5, sizeof(double) * (size_t)n);
+            be.h2d(sx, xyz, sizeof(double) * 3 * (size_t)n);
+            be.h2d(sn, nu, sizeof(double) * (size_t)n);
+            dx = sx;
+            dnu = sn;
+        }
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        pb.box.assign(box, box + 6);
+        pb.d_xyz = dx;
+        EngineStats es;
+        Engine<B>& E = ctx->eng;
+        if (E.build(pb, es)) {
+            ctx->err = "vgc_ot_features: " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        int* d_list = nullptr;
+        float* d_d2 = (float*)ctx->staging(2, sizeof(float) * (size_t)n * 16);
+        E.knn_lists(16, d_list, d_d2);
+        int* d_out = (int*)ctx->staging(3, sizeof(int) * (size_t)n * 16);
+        float* d_outd = (float*)ctx->staging(4, sizeof(float) * (size_t)n * 16);
+        be.knn_to_caller(E.d_orig, d_list, d_d2, (int)n, 16, d_out, d_outd);
+        double* d_box = (double*)ctx->staging(7, sizeof(double) * 6);
+        be.h2d(d_box, box, sizeof(double) * 6);
+        const double V = (box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
+        const double h = std::cbrt(V / n);
+        float* d_f = dev ? feats : (float*)ctx->staging(8, sizeof(float) * (size_t)n * kMlpIn);
+        be.ot_features((int)n, dx, dnu, d_outd, d_box, n / V, h, d_f);
+        if (!dev) be.d2h(feats, d_f, sizeof(float) * (size_t)n * kMlpIn);
+        be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_mlp_forward(Ctx<B>* ctx, const float* params, const float* feats, uint32_t n,
+                    uint32_t flags, float* out) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!params || !feats || !out || n == 0) {
+            ctx->err = "vgc_mlp_forward: bad argument";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        const float *dp = params, *df = feats;
+        float* dout = out;
+        if (!dev) {
+            float* sp = (float*)ctx->staging(9, sizeof(float) * kMlpParams);
+            float* sf = (float*)ctx->staging(8, sizeof(float) * (size_t)n * kMlpIn);
+            dout = (float*)ctx->staging(10, sizeof(float) * (size_t)n);
+            be.h2d(sp, params, sizeof(float) * kMlpParams);
+            be.h2d(sf, feats, sizeof(float) * (size_t)n * kMlpIn);
+            dp = sp;
+            df = sf;
+        }
+        be.mlp_forward(dp, df, (int)n, dout);
+        if (!dev) be.d2h(out, dout, sizeof(float) * (size_t)n);
+        be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_mlp_grad(Ctx<B>* ctx, const float* params, const float* feats, const float* target,
+                 uint32_t n, uint32_t flags, float* grad, double* loss) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!params || !feats || !target || !grad || n == 0) {
+            ctx->err = "vgc_mlp_grad: bad argument";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        const float *dp = params, *df = feats, *dt = target;
+        float* dg = grad;
+        if (!dev) {
+            float* sp = (float*)ctx->staging(9, sizeof(float) * kMlpParams);
+            float* sf = (float*)ctx->staging(8, sizeof(float) * (size_t)n * kMlpIn);
+            float* st = (float*)ctx->staging(10, sizeof(float) * (size_t)n);
+            dg = (float*)ctx->staging(11, sizeof(float) * kMlpParams);
+            be.h2d(sp, params, sizeof(float) * kMlpParams);
+            be.h2d(sf, feats, sizeof(float) * (size_t)n * kMlpIn);
+            be.h2d(st, target, sizeof(float) * (size_t)n);
+            dp = sp;
+            df = sf;
+            dt = st;
+        }
+        const int waves = be.mlp_grad_waves((int)n);
+        float* part = (float*)ctx->staging(12, sizeof(float) * (size_t)waves * kMlpParams);
+        float* lpart = (float*)ctx->staging(13, sizeof(float) * (size_t)waves);
+        be.mlp_grad(dp, df, dt, (int)n, 1.0f / float(n), part, lpart, dg);
+        if (!dev) be.d2h(grad, dg, sizeof(float) * kMlpParams);
+        if (loss) {
+            std::vector<float> lp(waves);
+            be.d2h(lp.data(), lpart, sizeof(float) * waves);
+            double l = 0.0;
+            for (float v : lp) l += v;
+            *loss = l / n;
+        }
+        be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_mlp_adam(Ctx<B>* ctx, float* params, const float* grad, float* m, float* v, uint32_t step,
+                 float lr, uint32_t flags) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!params || !grad || !m || !v || step == 0) {
+            ctx->err = "vgc_mlp_adam: bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (!(flags & VGC_DEVICE_PTRS)) {
+            ctx->err = "vgc_mlp_adam: device pointers required";
+            return VGC_ERR_ARG;
+        }
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+        ctx->be.mlp_adam(params, grad, m, v, lr, b1, b2, c1, c2, 1e-8f);
+        ctx->be.sync();
         return VGC_OK;
     } catch (const std::exception& e) {
         ctx->err = e.what();

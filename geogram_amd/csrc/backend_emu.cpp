@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "api_impl.h"
+#include "mlp.h"
 #include "ot_kernels.h"
 
 namespace vgc {
@@ -322,6 +323,35 @@ struct EmuBackend {
         for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     }
 
+    // ---------------------------------------------------------------- MLP
+    static constexpr int kRowsFwd = 64, kRowsGrad = 32;
+    void mlp_forward(const float* P, const float* X, int n, float* out) {
+        for (int r0 = 0; r0 < n; r0 += kRowsFwd)
+            mlp_rows<false>(P, X, nullptr, r0, std::min(n, r0 + kRowsFwd), 0.0f, out, nullptr);
+    }
+    int mlp_grad_waves(int n) { return (n + kRowsGrad - 1) / kRowsGrad; }
+    void mlp_grad(const float* P, const float* X, const float* T, int n, float scale, float* part,
+                  float* lpart, float* grad) {
+        int waves = mlp_grad_waves(n);
+#pragma omp parallel for schedule(static)
+        for (int w = 0; w < waves; ++w)
+            lpart[w] = mlp_rows<true>(P, X, T, w * kRowsGrad, std::min(n, (w + 1) * kRowsGrad), scale,
+                                      nullptr, part + (long long)w * kMlpParams);
+        for (int i = 0; i < kMlpParams; ++i) {
+            float s = 0.0f;
+            for (int w = 0; w < waves; ++w) s += part[(long long)w * kMlpParams + i];
+            grad[i] = s;
+        }
+    }
+    void mlp_adam(float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                  float c1, float c2, float eps) {
+        for (int i = 0; i < kMlpParams; ++i) adam_elem(i, p, g, m, v, lr, b1, b2, c1, c2, eps);
+    }
+    void ot_features(int n, const double* xyz, const double* nu, const float* d2,
+                     const double* box, double inv_cell, double h, float* f) {
+        for (int i = 0; i < n; ++i) ot_features_row(i, xyz, nu, d2, box, inv_cell, h, f + (long long)i * kMlpIn);
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         for (int q = 0; q < n; ++q) {
@@ -373,6 +403,22 @@ int vgc_emu_ot_solve(void* ctx, const double* xyz, const double* nu, double* w, 
                      const double* box, double tol, uint32_t max_newton, uint32_t flags,
                      vgc_ot_stats* stats) {
     return vgc::api_ot_solve((EmuCtx*)ctx, xyz, nu, w, n, box, tol, max_newton, flags, stats);
+}
+int vgc_emu_ot_features(void* ctx, const double* xyz, const double* nu, uint32_t n,
+                        const double* box, uint32_t flags, float* feats) {
+    return vgc::api_ot_features((EmuCtx*)ctx, xyz, nu, n, box, flags, feats);
+}
+int vgc_emu_mlp_forward(void* ctx, const float* params, const float* feats, uint32_t n,
+                        uint32_t flags, float* out) {
+    return vgc::api_mlp_forward((EmuCtx*)ctx, params, feats, n, flags, out);
+}
+int vgc_emu_mlp_grad(void* ctx, const float* params, const float* feats, const float* target,
+                     uint32_t n, uint32_t flags, float* grad, double* loss) {
+    return vgc::api_mlp_grad((EmuCtx*)ctx, params, feats, target, n, flags, grad, loss);
+}
+int vgc_emu_mlp_adam(void* ctx, float* params, const float* grad, float* m, float* v,
+                     uint32_t step, float lr, uint32_t flags) {
+    return vgc::api_mlp_adam((EmuCtx*)ctx, params, grad, m, v, step, lr, flags);
 }
 int vgc_emu_det4_sign(const double* rows) {
     return vgc::det4_sign_exact(rows, rows + 4, rows + 8, rows + 12);

@@ -20,6 +20,7 @@
 #include <string>
 
 #include "api_impl.h"
+#include "mlp.h"
 #include "ot_kernels.h"
 
 #define VGC_HIP_CHECK(x)                                                              \
@@ -386,6 +387,52 @@ __global__ __launch_bounds__(256) void k_min_final(const double* __restrict__ pa
     if (threadIdx.x == 0) *out = sh[0];
 }
 
+// ------------------------------------------------------------- MLP kernels
+constexpr int kMlpRowsFwd = 64;   // rows per wavefront, inference
+constexpr int kMlpRowsGrad = 32;  // rows per wavefront, gradient
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P,
+                                                     const float* __restrict__ X, int n, float* out) {
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r0 = wave * kMlpRowsFwd;
+    if (r0 >= n) return;
+    const int r1 = min(n, r0 + kMlpRowsFwd);
+    mlp_rows<false>(P, X, nullptr, r0, r1, 0.0f, out, nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P,
+                                                  const float* __restrict__ X,
+                                                  const float* __restrict__ T, int n, float scale,
+                                                  float* part, float* lpart) {
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r0 = wave * kMlpRowsGrad;
+    if (r0 >= n) return;
+    const int r1 = min(n, r0 + kMlpRowsGrad);
+    float l = mlp_rows<true>(P, X, T, r0, r1, scale, nullptr, part + (long long)wave * kMlpParams);
+    if (__lane_id() == 0) lpart[wave] = l;
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nw, float* grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    float s = 0.0f;
+    for (int w = 0; w < nw; ++w) s += part[(long long)w * kMlpParams + i];
+    grad[i] = s;
+}
+
+__global__ void k_mlp_adam(float* p, const float* __restrict__ g, float* m, float* v, float lr,
+                           float b1, float b2, float c1, float c2, float eps) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kMlpParams) adam_elem(i, p, g, m, v, lr, b1, b2, c1, c2, eps);
+}
+
+__global__ void k_ot_features(int n, const double* __restrict__ xyz, const double* __restrict__ nu,
+                              const float* __restrict__ d2, const double* __restrict__ box,
+                              double inv_cell, double h, float* f) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ot_features_row(i, xyz, nu, d2, box, inv_cell, h, f + (long long)i * kMlpIn);
+}
+
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
 // ----------------------------------------------------------------- backend
@@ -629,6 +676,32 @@ struct HipBackend {
         check();
     }
 
+    // ---------------------------------------------------------------- MLP
+    void mlp_forward(const float* P, const float* X, int n, float* out) {
+        int waves = (n + kMlpRowsFwd - 1) / kMlpRowsFwd;
+        k_mlp_forward<<<blocks_for(waves, 4), 256, 0, stream_>>>(P, X, n, out);
+        check();
+    }
+    int mlp_grad_waves(int n) { return (n + kMlpRowsGrad - 1) / kMlpRowsGrad; }
+    void mlp_grad(const float* P, const float* X, const float* T, int n, float scale, float* part,
+                  float* lpart, float* grad) {
+        int waves = mlp_grad_waves(n);
+        k_mlp_grad<<<blocks_for(waves, 4), 256, 0, stream_>>>(P, X, T, n, scale, part, lpart);
+        k_mlp_reduce<<<blocks_for(kMlpParams, 256), 256, 0, stream_>>>(part, waves, grad);
+        check();
+    }
+    void mlp_adam(float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                  float c1, float c2, float eps) {
+        k_mlp_adam<<<blocks_for(kMlpParams, 256), 256, 0, stream_>>>(p, g, m, v, lr, b1, b2, c1, c2,
+                                                                      eps);
+        check();
+    }
+    void ot_features(int n, const double* xyz, const double* nu, const float* d2,
+                     const double* box, double inv_cell, double h, float* f) {
+        k_ot_features<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, nu, d2, box, inv_cell, h, f);
+        check();
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         k_knn_to_caller<<<blocks_for(n, 256), 256, 0, stream_>>>(orig, list, d2, n, K, out, outd);
@@ -720,6 +793,31 @@ int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, u
     if (!ctx) return VGC_ERR_ARG;
     if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
     return vgc::api_ot_solve(ctx->impl, xyz, nu, w, n, box, tol, max_newton, flags, stats);
+}
+
+int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t n,
+                    const double* box, uint32_t flags, float* feats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_ot_features(ctx->impl, xyz, nu, n, box, flags, feats);
+}
+int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint32_t n,
+                    uint32_t flags, float* out) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_mlp_forward(ctx->impl, params, feats, n, flags, out);
+}
+int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const float* target,
+                 uint32_t n, uint32_t flags, float* grad, double* loss) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_mlp_grad(ctx->impl, params, feats, target, n, flags, grad, loss);
+}
+int vgc_mlp_adam(vgc_ctx* ctx, float* params, const float* grad, float* m, float* v,
+                 uint32_t step, float lr, uint32_t flags) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_mlp_adam(ctx->impl, params, grad, m, v, step, lr, flags);
 }
 
 int vgc_random_float64(uint64_t seed, double* out, uint64_t count) {

@@ -42,9 +42,11 @@ public:
     }
 
     // d_xyz (3n), d_nu (n), d_w (n, in: start, out: solution): device memory,
-    // caller order.  cg_tol: relative residual of the Newton systems.
+    // caller order.  cg_tol: relative residual of the Newton systems.  judge:
+    // also measure the zero start and keep whichever start has the smaller
+    // |m - nu|_2 (one more cell pass).
     int solve(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
-              double cg_tol, int cg_max, OtStats& st) {
+              double cg_tol, int cg_max, OtStats& st, bool judge = false) {
         auto t0 = std::chrono::steady_clock::now();
         const int n = pb.n;
         pb.want_neighbors = true;
@@ -74,11 +76,19 @@ public:
             be_.d2h(out, d_scal, sizeof(double) * 3);
         };
 
+        // zero start first when judging: the usual winner, the given start,
+        // is then the last one measured and its CSR feeds the first step
+        double m0[3] = {0, 0, 0};
+        if (judge) {
+            be_.dzero(d_wt, sizeof(double) * n);
+            if (cells(d_wt)) return -1;
+            masses(m0);
+        }
         if (cells(d_w)) return -1;
         double ms[3];
         masses(ms);
-        if (!(ms[2] > 0.0)) {
-            // the warm start leaves an empty cell: start again from w = 0
+        if (!(ms[2] > 0.0) || (judge && m0[1] < ms[1])) {
+            // the start leaves an empty cell, or is worse than w = 0: use w = 0
             be_.dzero(d_w, sizeof(double) * n);
             st.used_fallback_start = 1;
             if (cells(d_w)) return -1;

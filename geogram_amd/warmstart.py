@@ -1,0 +1,264 @@
+"""Learned warm start of the semi-discrete optimal-transport solver (no
+counterpart in geogram; it only changes the start of the damped Newton
+iterations of ``OptimalTransportMap3d.optimize``).
+
+A small MLP (6 -> 64 -> 64 -> 1, leaky ReLU, FP32; kernels in csrc/mlp.h)
+maps a seed's features -- position in the box, log of its target mass over
+the mean mass, log of the mean and max distance to its 16 nearest seeds over
+the mean spacing h = (V/n)^(1/3) -- to its Laguerre weight w_i - mean(w), in
+units of the squared box diagonal scale.  The weight field of a converged
+transport plan is mostly a smooth potential that does not depend on n; the
+part that follows mass fluctuations of single seeds scales with h^2 and is
+left to Newton.
+
+Training data are (features, converged weight) pairs that the solver itself
+produces on Gaussian-mixture instances; the network is trained on the GPU by
+``vgc_mlp_grad`` / ``vgc_mlp_adam``.  The solver keeps the plain zero start as
+fallback: a predicted start that leaves a cell empty, or whose mass residual
+is worse than the zero start's, is dropped (``vgc_ot_solve`` with
+``VGC_OT_JUDGE_START``).
+
+    python -m geogram_amd.warmstart --instances 200 --out geogram_amd/warmstart_params.npy
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import VGC_MLP_FEATURES, VGC_MLP_PARAMS
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PARAMS_FILE = os.path.join(_HERE, "warmstart_params.npy")
+N_HID = 64
+OFF_B1 = VGC_MLP_FEATURES * N_HID
+OFF_W2 = OFF_B1 + N_HID
+OFF_B2 = OFF_W2 + N_HID * N_HID
+OFF_W3 = OFF_B2 + N_HID
+OFF_B3 = OFF_W3 + N_HID
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-initialised weights, zero biases."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(VGC_MLP_PARAMS, dtype=np.float32)
+    p[:OFF_B1] = rng.normal(0.0, math.sqrt(2.0 / VGC_MLP_FEATURES), OFF_B1)
+    p[OFF_W2:OFF_B2] = rng.normal(0.0, math.sqrt(2.0 / N_HID), N_HID * N_HID)
+    p[OFF_W3:OFF_B3] = rng.normal(0.0, math.sqrt(1.0 / N_HID), N_HID)
+    return p
+
+
+def fold_normalisation(p, mu, sd, t_mu, t_sd) -> np.ndarray:
+    """Parameters of a network trained on (x - mu) / sd -> (t - t_mu) / t_sd,
+    rewritten to take raw features and return raw targets."""
+    q = np.asarray(p, dtype=np.float64).copy()
+    W1 = q[:OFF_B1].reshape(VGC_MLP_FEATURES, N_HID)
+    mu = np.asarray(mu, dtype=np.float64)
+    sd = np.asarray(sd, dtype=np.float64)
+    q[OFF_B1:OFF_W2] -= (mu / sd) @ W1
+    q[:OFF_B1] = (W1 / sd[:, None]).ravel()
+    q[OFF_W3:OFF_B3] *= t_sd
+    q[OFF_B3] = q[OFF_B3] * t_sd + t_mu
+    return q.astype(np.float32)
+
+
+def load_params(path: str | None = None) -> np.ndarray:
+    p = np.load(path or PARAMS_FILE, allow_pickle=False)
+    if p.dtype != np.float32 or p.shape != (VGC_MLP_PARAMS,):
+        raise ValueError(f"{path or PARAMS_FILE}: expected {VGC_MLP_PARAMS} float32 values")
+    return p
+
+
+def predict_weights(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), params=None):
+    """Initial Laguerre weights (FP64, same kind of array as points)."""
+    if params is None:
+        params = load_params()
+    feats = ctx.ot_features(points, nu, box)
+    if _is_torch_cuda(points):
+        import torch
+        p = torch.from_numpy(np.asarray(params, dtype=np.float32)).to(points.device)
+        return ctx.mlp_forward(p, feats).to(torch.float64)
+    return ctx.mlp_forward(params, feats).astype(np.float64)
+
+
+def _is_torch_cuda(a):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+# ------------------------------------------------------------ instances
+def make_instance(seed: int, n: int):
+    """Seeds of a 16-component Gaussian mixture in the unit cube and
+    log-normal (mu 0, sigma 0.5) target masses summing to 1."""
+    from . import sample_points
+    x = sample_points(2, seed, n)
+    rng = np.random.default_rng(seed + 7919)
+    nu = np.exp(rng.normal(0.0, 0.5, n))
+    nu /= nu.sum()
+    return x, nu
+
+
+def generate(ctx, seeds, sizes, tol=1e-5, log=print):
+    """Solve OT on each instance from the zero start; returns lists of
+    features (n x 6 float32) and targets w - mean(w) (float32)."""
+    feats, targets = [], []
+    for s, n in zip(seeds, sizes):
+        t0 = time.time()
+        x, nu = make_instance(int(s), int(n))
+        if _is_gpu_ctx(ctx):
+            import torch
+            xd = torch.from_numpy(x).cuda()
+            nud = torch.from_numpy(nu).cuda()
+            w, st = ctx.ot_solve(xd, nud, tol=tol)
+            f = ctx.ot_features(xd, nud)
+            t = (w - w.mean()).to(torch.float32)
+        else:
+            w, st = ctx.ot_solve(x, nu, tol=tol)
+            f = ctx.ot_features(x, nu)
+            t = (w - w.mean()).astype(np.float32)
+        if st["converged"] != 1:
+            log(f"instance seed={s} n={n}: not converged ({st['final_err']:.2e}), skipped")
+            continue
+        feats.append(f)
+        targets.append(t)
+        log(f"instance seed={s} n={n}: newton={st['newton_iters']} cg={st['cg_iters']} "
+            f"{time.time() - t0:.2f}s")
+    return feats, targets
+
+
+def _is_gpu_ctx(ctx):
+    return ctx.lib.prefix == "vgc_"
+
+
+# -------------------------------------------------------------- training
+def train(ctx, feats, targets, steps=4000, batch=32768, lr=2e-3, seed=0, log=print,
+          val_feats=None, val_targets=None):
+    """Adam on the mean squared error of normalised targets.  Arrays are torch
+    CUDA tensors for a GPU context, numpy for the emulation.  Returns raw-unit
+    parameters (normalisation folded in) and a history of losses."""
+    gpu = _is_gpu_ctx(ctx)
+    if gpu:
+        import torch
+        X = torch.cat(feats)
+        T = torch.cat(targets)
+        mu = X.double().mean(0).cpu().numpy()
+        sd = X.double().std(0).cpu().numpy()
+        t_mu, t_sd = 0.0, float(T.double().std())
+        sd = np.where(sd > 0, sd, 1.0)
+        Xn = ((X.double() - torch.from_numpy(mu).cuda()) / torch.from_numpy(sd).cuda()).float()
+        Tn = (T.double() / t_sd).float()
+        P = torch.from_numpy(init_params(seed)).cuda()
+        M = torch.zeros_like(P)
+        V = torch.zeros_like(P)
+        G = torch.empty_like(P)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        N = X.shape[0]
+    else:
+        X = np.concatenate(feats).astype(np.float64)
+        T = np.concatenate(targets).astype(np.float64)
+        mu, sd = X.mean(0), X.std(0)
+        sd = np.where(sd > 0, sd, 1.0)
+        t_mu, t_sd = 0.0, float(T.std())
+        Xn = ((X - mu) / sd).astype(np.float32)
+        Tn = (T / t_sd).astype(np.float32)
+        P = init_params(seed)
+        M = np.zeros_like(P)
+        V = np.zeros_like(P)
+        rng = np.random.default_rng(seed)
+        N = X.shape[0]
+    hist = []
+    for step in range(1, steps + 1):
+        if gpu:
+            idx = torch.randint(0, N, (min(batch, N),), device="cuda", generator=gen)
+            xb = Xn.index_select(0, idx).contiguous()
+            tb = Tn.index_select(0, idx).contiguous()
+            G, loss = ctx.mlp_grad(P, xb, tb, G)
+        else:
+            idx = rng.integers(0, N, min(batch, N))
+            G, loss = ctx.mlp_grad(P, Xn[idx], Tn[idx])
+        # cosine decay to lr / 20
+        lr_t = lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * (step - 1) / steps)))
+        ctx.mlp_adam(P, G, M, V, step, lr_t)
+        if step % max(1, steps // 20) == 0 or step == steps:
+            hist.append((step, loss))
+            log(f"step {step}: loss {loss:.4e} (normalised)")
+    Pn = P.cpu().numpy() if gpu else P
+    raw = fold_normalisation(Pn, mu, sd, t_mu, t_sd)
+    out = {"loss": hist, "t_sd": t_sd}
+    if val_feats is not None:
+        out["val"] = [validate(ctx, raw, f, t) for f, t in zip(val_feats, val_targets)]
+    return raw, out
+
+
+def validate(ctx, params, f, t):
+    """Relative L2 error of predicted weights w - mean(w) on one instance."""
+    if _is_gpu_ctx(ctx):
+        import torch
+        y = ctx.mlp_forward(torch.from_numpy(params).cuda(), f).double()
+        y = y - y.mean()
+        t = t.double()
+        return float(torch.linalg.norm(y - t) / torch.linalg.norm(t))
+    y = ctx.mlp_forward(params, f).astype(np.float64)
+    y -= y.mean()
+    t = np.asarray(t, dtype=np.float64)
+    return float(np.linalg.norm(y - t) / np.linalg.norm(t))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--instances", type=int, default=200)
+    ap.add_argument("--nmin", type=int, default=50_000)
+    ap.add_argument("--nmax", type=int, default=200_000)
+    ap.add_argument("--val", type=int, default=8, help="held-out instances")
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=32768)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--emulated", action="store_true", help="CPU emulation (small runs only)")
+    ap.add_argument("--out", default=PARAMS_FILE)
+    ap.add_argument("--report", default=None, help="JSON training report")
+    a = ap.parse_args(argv)
+    from . import Context
+
+    def log(msg):
+        print(msg, flush=True)
+
+    ctx = Context.emulated() if a.emulated else Context(0)
+    rng = np.random.default_rng(a.seed)
+    total = a.instances + a.val
+    sizes = np.exp(rng.uniform(math.log(a.nmin), math.log(a.nmax), total)).astype(np.int64)
+    seeds = a.seed + np.arange(total)
+    t0 = time.time()
+    feats, targets = generate(ctx, seeds, sizes, a.tol, log)
+    t_gen = time.time() - t0
+    nv = min(a.val, max(0, len(feats) - 1))
+    tr_f, tr_t = feats[:len(feats) - nv], targets[:len(feats) - nv]
+    va_f, va_t = feats[len(feats) - nv:], targets[len(feats) - nv:]
+    t1 = time.time()
+    params, rep = train(ctx, tr_f, tr_t, a.steps, a.batch, a.lr, a.seed, log, va_f, va_t)
+    t_train = time.time() - t1
+    np.save(a.out, params, allow_pickle=False)
+    rep.update({"instances": len(tr_f), "rows": int(sum(f.shape[0] for f in tr_f)),
+                "val_instances": nv, "t_generate_s": t_gen, "t_train_s": t_train,
+                "steps": a.steps, "batch": a.batch, "lr": a.lr, "tol": a.tol,
+                "sizes": [int(s) for s in sizes], "seed": a.seed})
+    log(json.dumps({k: v for k, v in rep.items() if k != "sizes"}))
+    if a.report:
+        with open(a.report, "w") as fh:
+            json.dump(rep, fh, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,6 +34,8 @@ extern "C" {
 #define VGC_NO_NEIGHBORS 2u  /* skip the neighbour CSR (volumes/centroids only) */
 #define VGC_COUNT_WORK 4u    /* fill the algorithmic work counters of vgc_stats
                                 (global atomics: slows the first pass down) */
+#define VGC_OT_JUDGE_START 8u /* vgc_ot_solve: also measure the zero start and keep
+                                 the start with the smaller mass residual */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -134,6 +136,25 @@ typedef struct vgc_ot_stats {
 int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                  const double* box, double tol, uint32_t max_newton, uint32_t flags,
                  vgc_ot_stats* stats);
+
+/* ---- learned warm start of vgc_ot_solve (no counterpart in geogram) ----
+   MLP 6 -> 64 -> 64 -> 1, leaky ReLU, FP32; VGC_MLP_PARAMS floats laid out
+   W1[6][64], b1[64], W2[64][64], b2[64], W3[64], b3.  Features of a seed:
+   position in the box (3), log(nu_i n / V), log of the mean and of the max
+   distance to its 16 nearest neighbours over h = (V/n)^(1/3).  The network
+   predicts w_i - mean(w) (box units). */
+#define VGC_MLP_PARAMS 4673
+#define VGC_MLP_FEATURES 6
+int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t n,
+                    const double* box, uint32_t flags, float* feats);
+int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint32_t n,
+                    uint32_t flags, float* out);
+/* gradient of the mean squared error over the n rows; loss = that mean */
+int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const float* target,
+                 uint32_t n, uint32_t flags, float* grad, double* loss);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); device pointers only */
+int vgc_mlp_adam(vgc_ctx* ctx, float* params, const float* grad, float* m, float* v,
+                 uint32_t step, float lr, uint32_t flags);
 
 /* count doubles of the stream std::mt19937_64(seed) +
    uniform_real_distribution<double>(0,1), the generator behind
