@@ -176,25 +176,35 @@ struct EmuBackend {
     int det4_sign(const double* r) { return det4_sign_exact(r, r + 4, r + 8, r + 12); }
 
     void radius_r2(const int* items, int ni, const pt4* pts, const double* r2, const int* inst_of,
-                   const double* wmaxv, int weighted, float* rho2) {
+                   const double* wmaxv, int weighted, const fball* ball, const double* box,
+                   float* rho2, PowerFilter* filt) {
         for (int t = 0; t < ni; ++t) {
             int c = items[t];
             int inst = inst_of ? inst_of[c] : 0;
             rho2[t] = radius_for(r2[c], pts[c].w, weighted ? wmaxv[inst] : 0.0, weighted);
+            if (filt) filt[t] = filter_from_ball(ball[c], pts[c], box + 6 * inst);
         }
     }
 
     void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
-                      int ni, const float* rho2, int* cnt) {
+                      int ni, const float* rho2, const PowerFilter* filt, int* cnt) {
 #pragma omp parallel for schedule(dynamic, 16)
-        for (int t = 0; t < ni; ++t) cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], nullptr, nullptr);
+        for (int t = 0; t < ni; ++t)
+            cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
+                                  nullptr, nullptr);
     }
 
     void radius_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
-                     int ni, const float* rho2, const long long* off, int* list, float* d2) {
+                     int ni, const float* rho2, const PowerFilter* filt, const long long* off,
+                     long long base, int* list, float* d2) {
 #pragma omp parallel for schedule(dynamic, 16)
         for (int t = 0; t < ni; ++t)
-            radius_query(pf, nodes, IL, items[t], rho2[t], list + off[t], d2 + off[t]);
+            radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
+                         list + (off[t] - base), d2 + (off[t] - base));
+    }
+
+    void rebase(const long long* off, int n, long long base, long long* out) {
+        for (int t = 0; t < n; ++t) out[t] = off[t] - base;
     }
 
     void scan_i32(const int* in, long long* out, int n) {
@@ -420,6 +430,8 @@ int vgc_emu_mlp_adam(void* ctx, float* params, const float* grad, float* m, floa
                      uint32_t step, float lr, uint32_t flags) {
     return vgc::api_mlp_adam((EmuCtx*)ctx, params, grad, m, v, step, lr, flags);
 }
+// test hook: entries per chunk of the list tiers
+void vgc_emu_set_list_chunk(void* ctx, long long n) { ((EmuCtx*)ctx)->eng.list_chunk = n; }
 int vgc_emu_det4_sign(const double* rows) {
     return vgc::det4_sign_exact(rows, rows + 4, rows + 8, rows + 12);
 }

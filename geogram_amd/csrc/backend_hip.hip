@@ -147,29 +147,41 @@ __global__ __launch_bounds__(64 * WAVES) void k_clip(ClipParams P) {
 
 __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
                             const double* __restrict__ r2, const int* __restrict__ inst_of,
-                            const double* __restrict__ wmaxv, int weighted, float* rho2) {
+                            const double* __restrict__ wmaxv, int weighted,
+                            const fball* __restrict__ ball, const double* __restrict__ box,
+                            float* rho2, PowerFilter* filt) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
     int c = items[t];
     int inst = inst_of ? inst_of[c] : 0;
     rho2[t] = radius_for(r2[c], pts[c].w, weighted ? wmaxv[inst] : 0.0, weighted);
+    if (filt) filt[t] = filter_from_ball(ball[c], pts[c], box + 6 * inst);
 }
 
 __global__ void k_radius_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
                                InstLayout IL, const int* __restrict__ items, int ni,
-                               const float* __restrict__ rho2, int* cnt) {
+                               const float* __restrict__ rho2, const PowerFilter* __restrict__ filt,
+                               int* cnt) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
-    cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], nullptr, nullptr);
+    cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr, nullptr,
+                          nullptr);
 }
 
 __global__ void k_radius_fill(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
                               InstLayout IL, const int* __restrict__ items, int ni,
-                              const float* __restrict__ rho2, const long long* __restrict__ off,
-                              int* list, float* d2) {
+                              const float* __restrict__ rho2, const PowerFilter* __restrict__ filt,
+                              const long long* __restrict__ off, long long base, int* list,
+                              float* d2) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
-    radius_query(pf, nodes, IL, items[t], rho2[t], list + off[t], d2 + off[t]);
+    radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
+                 list + (off[t] - base), d2 + (off[t] - base));
+}
+
+__global__ void k_rebase(const long long* __restrict__ off, int n, long long base, long long* out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = off[t] - base;
 }
 
 __global__ void k_compact(int n, const int* __restrict__ cnt, const long long* __restrict__ ptr,
@@ -575,20 +587,27 @@ struct HipBackend {
         check();
     }
     void radius_r2(const int* items, int ni, const pt4* pts, const double* r2, const int* inst_of,
-                   const double* wmaxv, int weighted, float* rho2) {
+                   const double* wmaxv, int weighted, const fball* ball, const double* box,
+                   float* rho2, PowerFilter* filt) {
         k_radius_r2<<<blocks_for(ni, 256), 256, 0, stream_>>>(items, ni, pts, r2, inst_of, wmaxv,
-                                                              weighted, rho2);
+                                                              weighted, ball, box, rho2, filt);
         check();
     }
     void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
-                      int ni, const float* rho2, int* cnt) {
-        k_radius_count<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, cnt);
+                      int ni, const float* rho2, const PowerFilter* filt, int* cnt) {
+        k_radius_count<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, filt,
+                                                               cnt);
         check();
     }
     void radius_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
-                     int ni, const float* rho2, const long long* off, int* list, float* d2) {
-        k_radius_fill<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, off,
-                                                              list, d2);
+                     int ni, const float* rho2, const PowerFilter* filt, const long long* off,
+                     long long base, int* list, float* d2) {
+        k_radius_fill<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, filt,
+                                                              off, base, list, d2);
+        check();
+    }
+    void rebase(const long long* off, int n, long long base, long long* out) {
+        k_rebase<<<blocks_for(n, 256), 256, 0, stream_>>>(off, n, base, out);
         check();
     }
     void scan_i32(const int* in, long long* out, int n) {
@@ -604,6 +623,9 @@ struct HipBackend {
     void seg_sort(const float* kin, const int* vin, float* kout, int* vout, long long total,
                   int nseg, const long long* off) {
         if (total <= 0) return;
+        // rocPRIM takes 32-bit sizes and sizes its scratch by them
+        if (total > 0x7FFFFFFFll || nseg < 0)
+            throw std::runtime_error("seg_sort: list chunk exceeds 2^31 entries");
         size_t tb = 0;
         VGC_HIP_CHECK(rocprim::segmented_radix_sort_pairs(nullptr, tb, kin, kout, vin, vout,
                                                           (unsigned)total, (unsigned)nseg, off,

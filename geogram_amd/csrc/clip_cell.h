@@ -57,6 +57,7 @@ struct PMax {
 constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 128;  // tree nodes per level of the walk
 constexpr int kBucketCap = 128; // buckets per ring
+constexpr int kMaxFilteredRings = 6;  // power-filtered rings before the list tiers
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -89,6 +90,7 @@ struct ClipParams {
     // per item (Morton position) bookkeeping
     uint32_t* flags;        // indexed by Morton position
     double* r2;             // squared radius of the (partial) cell, by Morton position
+    fball* ball;            // weighted: ball of a flagged partial cell (list tiers)
     // [0] planes that cut, [1] planes tested, [2] live corners at each test,
     // [3] corners created, [4] integration tetrahedra, [5] facets
     unsigned long long* counters;
@@ -464,7 +466,7 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
     return flags;
 }
 
-// Security-radius loop + integration for work item `item`.
+// State of one cell's security-radius loop.
 template <int S>
 struct CellRun {
     CellState<S> st;
@@ -543,7 +545,7 @@ VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
 
 // Tree node / seed loads per lane (masked lanes get zeros).
 struct BoxLanes {
-    LF lx, ly, lz, hx, hy, hz;
+    LF lx, ly, lz, hx, hy, hz, wm;
 };
 VGC_HD BoxLanes load_box(const bnode* T, const LI& node, const LB& m) {
     BoxLanes B;
@@ -557,6 +559,7 @@ VGC_HD BoxLanes load_box(const bnode* T, const LI& node, const LB& m) {
         B.hx[l] = b.hi[0];
         B.hy[l] = b.hi[1];
         B.hz[l] = b.hi[2];
+        B.wm[l] = b.wmax;
     }
 #else
     bnode b{};
@@ -567,8 +570,35 @@ VGC_HD BoxLanes load_box(const bnode* T, const LI& node, const LB& m) {
     B.hx = b.hi[0];
     B.hy = b.hi[1];
     B.hz = b.hi[2];
+    B.wm = b.wmax;
 #endif
     return B;
+}
+
+// Seed (FP32 x, y, z, w) per lane (masked lanes get zeros).
+struct SeedLanes {
+    LF x, y, z, w;
+};
+VGC_HD SeedLanes lane_seed(const pf4* pf, const LI& idx, const LB& m) {
+    SeedLanes q;
+#if defined(VGC_EMU)
+    for (int l = 0; l < kWave; ++l) {
+        pf4 a{};
+        if (m[l]) a = pf[idx[l]];
+        q.x[l] = a.x;
+        q.y[l] = a.y;
+        q.z[l] = a.z;
+        q.w[l] = a.w;
+    }
+#else
+    pf4 a{};
+    if (m) a = pf[idx];
+    q.x = a.x;
+    q.y = a.y;
+    q.z = a.z;
+    q.w = a.w;
+#endif
+    return q;
 }
 
 VGC_HD LF lane_dist2(const pf4* pf, const LI& idx, const LB& m, float px, float py, float pz) {
@@ -586,10 +616,11 @@ VGC_HD LF lane_dist2(const pf4* pf, const LI& idx, const LB& m, float px, float 
 // distance) into L.cand as (d2 bits << 32 | q), unsorted.  Returns their
 // number (only the first kCandCap are stored), or -1 when the tree walk
 // overflows its buffers.  Breadth-first walk of the implicit tree, one node
-// per lane; boxes entirely inside the inner ball are skipped.
+// per lane; boxes entirely inside the inner ball are skipped.  With F.on,
+// only seeds (and subtrees) that pass the power-distance filter are taken.
 template <int S>
 VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, float px, float py,
-                       float pz, float rlo2, float rhi2) {
+                       float pz, float rlo2, float rhi2, const PowerFilter& F) {
     const LI lane = lane_id();
     const int p0 = P.IL.p0[inst], n = P.IL.n[inst], nb = P.IL.nb[inst], Lv = P.IL.L[inst];
     const bnode* T = P.nodes + P.IL.t0[inst];
@@ -612,6 +643,14 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
                fz = vmaxf(pz - B.lz, B.hz - pz);
             LF dmax = fx * fx + fy * fy + fz * fz;
             LB hit = in & (dmin <= rhi2) & (dmax * 1.00001f > rlo2);
+            if (F.on) {
+                // lower bound of the filter's left side over the node
+                LF cx = vmaxf(vmaxf(B.lx - F.cx, F.cx - B.hx), 0.0f);
+                LF cy = vmaxf(vmaxf(B.ly - F.cy, F.cy - B.hy), 0.0f);
+                LF cz = vmaxf(vmaxf(B.lz - F.cz, F.cz - B.hz), 0.0f);
+                LF dc2 = cx * cx + cy * cy + cz * cz;
+                hit = hit & pf_may_cut(F, dc2, B.wm, vsqrtf(dmax));
+            }
             LB internal = node < Lv;
             LB push = hit & internal;
             LB leaf = hit & !internal;
@@ -641,8 +680,19 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
         LI b = lds_ld(L.bucket, select(inb, bi, LI(0)));
         LI m = p0 + b * kBucket + k;
         LB ok = inb & (b < nb) & (m < p0 + n) & (m != c);
-        LF d2 = lane_dist2(P.pf, select(ok, m, LI(0)), ok, px, py, pz);
-        LB take = ok & (d2 > rlo2) & (d2 <= rhi2);
+        LB take;
+        LF d2;
+        if (!F.on) {
+            d2 = lane_dist2(P.pf, select(ok, m, LI(0)), ok, px, py, pz);
+            take = ok & (d2 > rlo2) & (d2 <= rhi2);
+        } else {
+            SeedLanes q = lane_seed(P.pf, select(ok, m, LI(0)), ok);
+            LF ex = q.x - px, ey = q.y - py, ez = q.z - pz;
+            d2 = ex * ex + ey * ey + ez * ez;
+            LF gx = q.x - F.cx, gy = q.y - F.cy, gz = q.z - F.cz;
+            LF dc2 = gx * gx + gy * gy + gz * gz;
+            take = ok & (d2 > rlo2) & (d2 <= rhi2) & pf_may_cut(F, dc2, q.w, vsqrtf(d2));
+        }
         uint64_t mt = ballot(take);
         LI r = cnt + mbcnt(mt);
         LB st = take & (r < kCandCap);
@@ -680,6 +730,68 @@ VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
             wave_sync();
         }
     }
+}
+
+// Ball around the partial cell of a weighted seed for the power-distance
+// filter (types.h): centre = middle of the corners' bounding box, radius =
+// farthest corner plus the corner's certified position error
+// (|dN| <= g nmax, |dD| <= g dabs).  Returns false (no filter) when a corner
+// is too close to degenerate for that bound.
+template <int S>
+VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
+    const CellState<S>& st = R.st;
+    LD lox = LD(INFINITY), loy = LD(INFINITY), loz = LD(INFINITY);
+    LD hix = LD(-INFINITY), hiy = LD(-INFINITY), hiz = LD(-INFINITY);
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LB vs = mask_bit(st.valid[s]);
+        LD den = st.d[s] - kGamma9 * st.dabs[s];
+        bad = bad || any(vs & !(den > 0.0));
+        LD inv = 1.0 / select(vs, st.d[s], LD(1.0));
+        LD qx = st.nx[s] * inv, qy = st.ny[s] * inv, qz = st.nz[s] * inv;
+        lox = select(vs, vmin(lox, qx), lox);
+        loy = select(vs, vmin(loy, qy), loy);
+        loz = select(vs, vmin(loz, qz), loz);
+        hix = select(vs, vmax(hix, qx), hix);
+        hiy = select(vs, vmax(hiy, qy), hiy);
+        hiz = select(vs, vmax(hiz, qz), hiz);
+    }
+    if (bad) return false;
+    const double ox = 0.5 * (-wave_max(-lox) + wave_max(hix));
+    const double oy = 0.5 * (-wave_max(-loy) + wave_max(hiy));
+    const double oz = 0.5 * (-wave_max(-loz) + wave_max(hiz));
+    F.cx = float(R.pc.x + ox);
+    F.cy = float(R.pc.y + oy);
+    F.cz = float(R.pc.z + oz);
+    // the FP32 centre, relative to the seed
+    const double fx = double(F.cx) - R.pc.x, fy = double(F.cy) - R.pc.y,
+                 fz = double(F.cz) - R.pc.z;
+    LD rmax = LD(0.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        LB vs = mask_bit(st.valid[s]);
+        LD dd = select(vs, st.d[s], LD(1.0));
+        LD inv = 1.0 / dd;
+        LD qx = st.nx[s] * inv, qy = st.ny[s] * inv, qz = st.nz[s] * inv;
+        LD gx = qx - fx, gy = qy - fy, gz = qz - fz;
+        LD dist = vsqrt(vfma(gx, gx, vfma(gy, gy, gz * gz)));
+        LD qn = vsqrt(vfma(qx, qx, vfma(qy, qy, qz * qz)));
+        LD err = kGamma9 * (st.nmax[s] + qn * st.dabs[s]) /
+                 select(vs, dd - kGamma9 * st.dabs[s], LD(1.0));
+        rmax = vmax(rmax, select(vs, dist + err, LD(0.0)));
+    }
+    const double rc = wave_max(rmax) * (1.0 + 1e-9);
+    const double off2 = fx * fx + fy * fy + fz * fz;
+    if (!(rc < 1e30) || !(off2 < 1e30)) return false;
+    F.rc = f_up(rc);
+    F.phi = float(off2 - R.wi);
+    F.dcp = f_up(sqrt(off2));
+    double cs = 0.0;
+    for (int k = 0; k < 6; ++k) cs = fmax(cs, fabs(R.box[k]));
+    F.s = float(1e-6 * cs);
+    F.on = 1;
+    return true;
 }
 
 // Security-radius loop + integration for work item `item`.  GATHER: the
@@ -761,8 +873,20 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         const float all2 = f_up(1.0001 * double(bx * bx + by * by + bz * bz));
         if (!(rhi2 > 0.0f)) rhi2 = all2 * 1e-6f;
         float rlo2 = -1.0f;
+        PowerFilter F;
+        F.on = 0;
+        int nfilt = 0;
         while (true) {
-            int cnt = gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2);
+            int cnt = gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
+#ifdef VGC_DEBUG_RINGS
+            fprintf(stderr, "RING c=%d rlo2=%g rhi2=%g F=%d rc=%g dcp=%g cnt=%d np=%d\n", c, rlo2, rhi2, F.on, F.rc, F.dcp, cnt, R.np);
+#endif
+            if (F.on && (cnt < 0 || ++nfilt > kMaxFilteredRings)) {
+                // the filtered walk is wider than the wave's buffers: the list
+                // tiers take the cell, with the same filter and no capacity
+                R.flags |= CELL_NEED_K;
+                break;
+            }
             if (cnt < 0 || cnt > kCandCap) {
                 // too many seeds (or tree nodes) in the ring: narrow it, same
                 // shell-volume fraction in r^3, and gather again
@@ -801,7 +925,14 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 break;
             }
             rlo2 = rhi2;
-            rhi2 = need2 < all2 ? need2 : all2;
+            if (P.weighted && make_filter<S>(R, F)) {
+                // every seed beyond the ring that passes the power filter of
+                // the cell as it is now (narrowed above if too many)
+                rhi2 = all2;
+            } else {
+                F.on = 0;
+                rhi2 = need2 < all2 ? need2 : all2;
+            }
             wave_sync();
         }
     }
@@ -826,9 +957,21 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         atomic_add_u64(P.counters + 3, nnew);
     }
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
+        // the partial cell contains the cell: its ball filters the list tiers
+        PowerFilter G;
+        G.on = 0;
+        const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
         if (lane0()) {
             P.flags[c] = flags;
             P.r2[c] = r2max;
+            if (P.ball) {
+                fball b;
+                b.cx = G.cx;
+                b.cy = G.cy;
+                b.cz = G.cz;
+                b.rc = okf ? G.rc : -1.0f;
+                P.ball[c] = b;
+            }
         }
         return;
     }
@@ -987,6 +1130,12 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             P.flags[c] = flags | CELL_DONE;
         } else {
             P.flags[c] = flags;
+            if (P.ball) {
+                fball b;
+                b.cx = b.cy = b.cz = 0.0f;
+                b.rc = -1.0f;
+                P.ball[c] = b;
+            }
         }
         P.r2[c] = r2max;
     }

@@ -137,19 +137,29 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
 }
 
 // ------------------------------------------------------ radius queries
-// Seeds (other than q) with FP32 squared distance <= r2, in traversal order.
-// count mode when out == nullptr.
+// Seeds (other than q) with FP32 squared distance <= r2 and, with a filter
+// F, that pass the power-distance test (types.h), in traversal order.  Count
+// mode when out == nullptr.
 VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
-                        float r2, int* out, float* out_d2) {
+                        float r2, const PowerFilter* F, int* out, float* out_d2) {
     const int inst = IL.inst_of ? IL.inst_of[q] : 0;
     const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
     const bnode* T = nodes + IL.t0[inst];
     const pf4 pq = pf[q];
     const float px = pq.x, py = pq.y, pz = pq.z;
+    const bool filt = F && F->on;
     int cnt = 0;
     int node = 1;
     while (true) {
-        bool inside = box_dist2(T[node], px, py, pz) <= r2;
+        const bnode B = T[node];
+        bool inside = box_dist2(B, px, py, pz) <= r2;
+        if (inside && filt) {
+            const float fx = fmaxf(px - B.lo[0], B.hi[0] - px);
+            const float fy = fmaxf(py - B.lo[1], B.hi[1] - py);
+            const float fz = fmaxf(pz - B.lo[2], B.hi[2] - pz);
+            const float dfar = sqrtf(fx * fx + fy * fy + fz * fz);
+            inside = pf_may_cut(*F, box_dist2(B, F->cx, F->cy, F->cz), B.wmax, dfar);
+        }
         if (inside && node < L) {
             node = 2 * node;
             continue;
@@ -162,8 +172,12 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
                 if (e > p0 + n) e = p0 + n;
                 for (int m = s; m < e; ++m) {
                     if (m == q) continue;
-                    float d = dist2f(pf[m], px, py, pz);
-                    if (d <= r2) {
+                    const pf4 a = pf[m];
+                    float d = dist2f(a, px, py, pz);
+                    bool take = d <= r2;
+                    if (take && filt)
+                        take = pf_may_cut(*F, dist2f(a, F->cx, F->cy, F->cz), a.w, sqrtf(d));
+                    if (take) {
                         if (out) {
                             out[cnt] = m;
                             out_d2[cnt] = d;
@@ -186,15 +200,17 @@ VGC_SD void bvh_leaf(const pt4* pts, const InstLayout& IL, bnode* nodes, int ins
     bnode B;
     B.lo[0] = B.lo[1] = B.lo[2] = INFINITY;
     B.hi[0] = B.hi[1] = B.hi[2] = -INFINITY;
-    B.pad[0] = B.pad[1] = 0;
+    B.wmax = -INFINITY;
+    B.pad = 0;
     if (leaf < nb) {
         int s = p0 + leaf * kBucket;
         int e = s + kBucket;
         if (e > p0 + n) e = p0 + n;
         double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
-        double hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+        double hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY, wm = -INFINITY;
         for (int m = s; m < e; ++m) {
             const pt4 p = pts[m];
+            wm = fmax(wm, p.w);
             lo0 = fmin(lo0, p.x);
             lo1 = fmin(lo1, p.y);
             lo2 = fmin(lo2, p.z);
@@ -210,6 +226,7 @@ VGC_SD void bvh_leaf(const pt4* pts, const InstLayout& IL, bnode* nodes, int ins
         B.hi[0] = fmaxf(f_up(hi0), float(hi0));
         B.hi[1] = fmaxf(f_up(hi1), float(hi1));
         B.hi[2] = fmaxf(f_up(hi2), float(hi2));
+        B.wmax = fmaxf(f_up(wm), float(wm));
     }
     nodes[IL.t0[inst] + L + leaf] = B;
 }
@@ -221,7 +238,8 @@ VGC_SD void bvh_merge(bnode* nodes, long long t0, int k) {
         B.lo[c] = fminf(a.lo[c], b.lo[c]);
         B.hi[c] = fmaxf(a.hi[c], b.hi[c]);
     }
-    B.pad[0] = B.pad[1] = 0;
+    B.wmax = fmaxf(a.wmax, b.wmax);
+    B.pad = 0;
     nodes[t0 + k] = B;
 }
 

@@ -62,13 +62,18 @@ public:
         double* d_scal = (double*)slot(8, sizeof(double) * 8);
         EngineStats es;
 
-        auto cells = [&](const double* w) -> int {
+        // 0: cells of w computed; 1: some cell empty or failed (the masses
+        // are not all valid); -1: error
+        auto cells = [&](const double* w, bool trial) -> int {
             auto tc = std::chrono::steady_clock::now();
             pb.d_w = w;
+            pb.abort_on_empty = trial;
+            const long long failed0 = es.nb_failed, empty0 = es.nb_empty;
             int rc = E_.cells(pb, es);
             st.cell_passes++;
             st.t_cells_ms += ms_since(tc);
-            return rc;
+            if (rc < 0) return -1;
+            return (rc > 0 || es.nb_failed > failed0 || es.nb_empty > empty0) ? 1 : 0;
         };
         // mass statistics: [0] max |m-nu|/nu, [1] |m-nu|_2^2, [2] min m
         auto masses = [&](double out[3]) {
@@ -81,17 +86,18 @@ public:
         double m0[3] = {0, 0, 0};
         if (judge) {
             be_.dzero(d_wt, sizeof(double) * n);
-            if (cells(d_wt)) return -1;
+            if (cells(d_wt, false)) return fail("the cells of the zero start failed");
             masses(m0);
         }
-        if (cells(d_w)) return -1;
-        double ms[3];
-        masses(ms);
-        if (!(ms[2] > 0.0) || (judge && m0[1] < ms[1])) {
-            // the start leaves an empty cell, or is worse than w = 0: use w = 0
+        int rc0 = cells(d_w, true);
+        if (rc0 < 0) return -1;
+        double ms[3] = {0, 0, 0};
+        if (rc0 == 0) masses(ms);
+        if (rc0 > 0 || !(ms[2] > 0.0) || (judge && m0[1] < ms[1])) {
+            // the start leaves a cell empty, or is worse than w = 0: use w = 0
             be_.dzero(d_w, sizeof(double) * n);
             st.used_fallback_start = 1;
-            if (cells(d_w)) return -1;
+            if (cells(d_w, false)) return fail("the cells of the zero start failed");
             masses(ms);
         }
         double numin = 0;
@@ -116,10 +122,14 @@ public:
             bool ok = false;
             for (int h = 0; h < 40; ++h) {
                 be_.ot_axpy(n, d_w, t, d_x, d_wt);
-                if (cells(d_wt)) return -1;
-                double m2[3];
-                masses(m2);
-                if (m2[2] >= eps0 && std::sqrt(m2[1]) <= (1.0 - 0.5 * t) * std::sqrt(g2)) {
+                // a trial with an empty cell fails the mass test whatever the
+                // other cells: its evaluation stops after the first pass
+                const int rc = cells(d_wt, true);
+                if (rc < 0) return -1;
+                double m2[3] = {0, 0, 0};
+                if (rc == 0) masses(m2);
+                if (rc == 0 && m2[2] >= eps0 &&
+                    std::sqrt(m2[1]) <= (1.0 - 0.5 * t) * std::sqrt(g2)) {
                     err = m2[0];
                     g2 = m2[1];
                     ok = true;
@@ -176,6 +186,10 @@ private:
         return it;
     }
 
+    int fail(const char* msg) {
+        E_.error = msg;
+        return -1;
+    }
     struct Buf {
         void* p = nullptr;
         size_t bytes = 0;

@@ -59,6 +59,8 @@ struct Problem {
                                     // 16..64: kNN lists of k0 for the first pass
     bool want_neighbors = true;
     bool count_work = false;        // algorithmic work counters (global atomics)
+    bool abort_on_empty = false;    // stop after the first pass if a cell is empty
+                                    // (cells() then returns 1; outputs unset)
 };
 
 template <class B>
@@ -79,6 +81,7 @@ public:
     int* d_nbr = nullptr;
     double* d_area = nullptr;
     long long total_neighbors = 0;
+    long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
     std::vector<uint32_t> h_flags;  // by Morton position
 
     // layout of the last build
@@ -250,6 +253,7 @@ public:
         P.face_stride = kFaceStride;
         P.flags = d_flags;
         P.r2 = d_r2;
+        P.ball = weighted_ ? (fball*)slot(S_BALL, sizeof(fball) * (size_t)n_) : nullptr;
         P.counters = pb.count_work ? d_ctr : nullptr;
         P.pf = d_pf;
         P.nodes = d_nodes;
@@ -284,12 +288,19 @@ public:
         h_flags.resize(n_);
         be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
         std::vector<int> need, over;
+        long long nempty = 0;
         for (int c = 0; c < n_; ++c) {
             uint32_t f = h_flags[c];
             if (f & CELL_OVERFLOW)
                 over.push_back(c);
             else if (f & CELL_NEED_K)
                 need.push_back(c);
+            if (f & CELL_EMPTY) ++nempty;
+        }
+        if (pb.abort_on_empty && nempty > 0) {
+            st.nb_empty += nempty;
+            st.t_total_ms += ms_since(tstart);
+            return 1;
         }
         if (tiers(need, over, false, P, st)) return -1;
         // cells with signs inside their error bound: exact re-clip
@@ -368,8 +379,8 @@ private:
         S_P0, S_N, S_B0, S_NB, S_L, S_T0, S_LEAFOFF, S_BOX, S_INSTORIG, S_KEYS, S_KEYS2,
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
-        S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN,
-        S_COUNT
+        S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
+        S_FILT, S_COFF, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
@@ -416,37 +427,53 @@ private:
     }
 
     // Complete lists for `items`: every seed within the distance that can
-    // still cut the partial cell of the previous pass.
+    // still cut the partial cell of the previous pass (weighted: that also
+    // passes the power filter of its ball), in chunks of at most list_chunk
+    // entries (or one item), each sorted by distance and clipped in list mode.
     int radius_pass(int S, bool exact, const std::vector<int>& items, ClipParams P,
                     EngineStats& st) {
         const int ni = (int)items.size();
         int* d_items = (int*)slot(S_ITEMS, sizeof(int) * ni);
         be_.h2d(d_items, items.data(), sizeof(int) * ni);
         float* d_rho2 = (float*)slot(S_RHO2, sizeof(float) * ni);
-        be_.radius_r2(d_items, ni, d_pts, P.r2, d_inst_of, d_wmax_, weighted_ ? 1 : 0, d_rho2);
+        PowerFilter* d_filt =
+            weighted_ ? (PowerFilter*)slot(S_FILT, sizeof(PowerFilter) * ni) : nullptr;
+        be_.radius_r2(d_items, ni, d_pts, P.r2, d_inst_of, d_wmax_, weighted_ ? 1 : 0, P.ball,
+                      d_box_, d_rho2, d_filt);
         int* d_rcnt = (int*)slot(S_RCNT, sizeof(int) * ni);
-        be_.radius_count(d_pf, d_nodes, IL, d_items, ni, d_rho2, d_rcnt);
+        be_.radius_count(d_pf, d_nodes, IL, d_items, ni, d_rho2, d_filt, d_rcnt);
         long long* d_roff = (long long*)slot(S_ROFF, sizeof(long long) * (ni + 1));
         be_.scan_i32(d_rcnt, d_roff, ni);
-        long long total = 0;
-        be_.d2h(&total, d_roff + ni, sizeof(long long));
-        st.radius_entries += total;
-        size_t cap = (size_t)std::max<long long>(total, 1);
-        int* d_rl = (int*)slot(S_RLIST, sizeof(int) * cap);
-        float* d_rd = (float*)slot(S_RD2, sizeof(float) * cap);
-        int* d_rl2 = (int*)slot(S_RLIST2, sizeof(int) * cap);
-        float* d_rd2 = (float*)slot(S_RD22, sizeof(float) * cap);
-        be_.radius_fill(d_pf, d_nodes, IL, d_items, ni, d_rho2, d_roff, d_rl, d_rd);
-        be_.seg_sort(d_rd, d_rl, d_rd2, d_rl2, total, ni, d_roff);  // sorted into rd2 / rl2
-        P.items = d_items;
-        P.n_items = ni;
-        P.list = d_rl2;
-        P.list_ptr = d_roff;
-        P.list_len = d_rcnt;
-        P.list_stride = 0;
-        P.list_complete = 1;
-        be_.clip(S, exact, false, P);
-        be_.sync();
+        std::vector<long long> off(ni + 1);
+        be_.d2h(off.data(), d_roff, sizeof(long long) * (ni + 1));
+        st.radius_entries += off[ni];
+        for (int a = 0; a < ni;) {
+            int b = a + 1;
+            while (b < ni && off[b + 1] - off[a] <= list_chunk) ++b;
+            const long long base = off[a], total = off[b] - base;
+            const int nc = b - a;
+            size_t cap = (size_t)std::max<long long>(total, 1);
+            int* d_rl = (int*)slot(S_RLIST, sizeof(int) * cap);
+            float* d_rd = (float*)slot(S_RD2, sizeof(float) * cap);
+            int* d_rl2 = (int*)slot(S_RLIST2, sizeof(int) * cap);
+            float* d_rd2 = (float*)slot(S_RD22, sizeof(float) * cap);
+            long long* d_coff = (long long*)slot(S_COFF, sizeof(long long) * (nc + 1));
+            be_.radius_fill(d_pf, d_nodes, IL, d_items + a, nc, d_rho2 + a,
+                            d_filt ? d_filt + a : nullptr, d_roff + a, base, d_rl, d_rd);
+            be_.rebase(d_roff + a, nc + 1, base, d_coff);
+            be_.seg_sort(d_rd, d_rl, d_rd2, d_rl2, total, nc, d_coff);  // into rd2 / rl2
+            ClipParams Q = P;
+            Q.items = d_items + a;
+            Q.n_items = nc;
+            Q.list = d_rl2;
+            Q.list_ptr = d_coff;
+            Q.list_len = d_rcnt + a;
+            Q.list_stride = 0;
+            Q.list_complete = 1;
+            be_.clip(S, exact, false, Q);
+            be_.sync();
+            a = b;
+        }
         return 0;
     }
 

@@ -22,11 +22,13 @@ struct alignas(16) pf4 {
 };
 
 // Node of the implicit bounding-volume tree over buckets of kBucket
-// consecutive Morton-ordered seeds (float box, rounded outwards).
+// consecutive Morton-ordered seeds (float box, rounded outwards) and the
+// largest weight below it (rounded up; -inf for an empty node).
 struct alignas(32) bnode {
     float lo[3];
     float hi[3];
-    int pad[2];
+    float wmax;
+    int pad;
 };
 
 constexpr int kBucket = 16;
@@ -82,6 +84,62 @@ VGC_SD float radius_for(double r2, double wi, double wmax, int weighted) {
         rho2 = rho * rho;
     }
     return f_up(rho2 * 1.0001);
+}
+
+// Power-distance filter of the weighted neighbour search.  A partial
+// Laguerre cell of seed p (weight wi) lies in the ball B(ctr, rc).  For x in
+// the ball, pow_j(x) - pow_i(x) (pow_k(x) = |x - q_k|^2 - w_k) differs from its
+// value at ctr by at most 2 rc |q_j - p|, so seed q (weight wj) can cut the
+// cell only if
+//     |ctr - q|^2 - wj - (|ctr - p|^2 - wi) < 2 rc |q - p|.
+// Unlike the ball around p itself, this stays local when the cell sits away
+// from its seed, as Laguerre cells of a transport plan do.  The tests run in
+// FP32; the margin covers the rounding of the FP32 coordinates (relative
+// 2^-24 of the coordinate scale), weights and arithmetic, so a seed that can
+// cut is never dropped.
+struct PowerFilter {
+    int on;
+    float cx, cy, cz;  // ball centre
+    float rc;          // ball radius, rounded up
+    float phi;         // |ctr - p|^2 - wi
+    float dcp;         // |ctr - p|
+    float s;           // 1e-6 x coordinate scale of the instance
+};
+
+// Ball of a partial cell handed from the clipping kernel to the list tiers
+// (rc < 0: none).
+struct fball {
+    float cx, cy, cz, rc;
+};
+
+// dc2: squared distance from the ball centre to q (or a lower bound over a
+// tree node), wj: q's weight (or an upper bound), d: |q - p| (or an upper
+// bound).  True when q may cut.
+template <class TF>
+VGC_HD auto pf_may_cut(const PowerFilter& F, const TF& dc2, const TF& wj, const TF& d) {
+    const TF rd = (2.0f * F.rc) * d;
+    const TF lhs = dc2 - wj - F.phi - rd;
+    const TF mar = 1e-6f * (dc2 + vabsf(wj) + fabsf(F.phi) + rd) + F.s * (F.dcp + 2.0f * d + F.rc);
+    return lhs < mar;
+}
+
+// Filter of seed p (FP64 position, weight) from a ball; the instance box
+// gives the coordinate scale.
+VGC_SD PowerFilter filter_from_ball(const fball& b, const pt4& p, const double* box) {
+    PowerFilter F;
+    F.on = b.rc >= 0.0f;
+    F.cx = b.cx;
+    F.cy = b.cy;
+    F.cz = b.cz;
+    F.rc = b.rc;
+    const double fx = double(b.cx) - p.x, fy = double(b.cy) - p.y, fz = double(b.cz) - p.z;
+    const double off2 = fx * fx + fy * fy + fz * fz;
+    F.phi = float(off2 - p.w);
+    F.dcp = f_up(sqrt(off2));
+    double cs = 0.0;
+    for (int k = 0; k < 6; ++k) cs = fmax(cs, fabs(box[k]));
+    F.s = float(1e-6 * cs);
+    return F;
 }
 
 }  // namespace vgc

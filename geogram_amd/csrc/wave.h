@@ -211,6 +211,17 @@ inline LF vmaxf(const LF& a, const LF& b) {
     return r;
 }
 inline LF vmaxf(const LF& a, float b) { return vmaxf(a, LF(b)); }
+inline LF vsqrtf(const LF& a) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::sqrt(a.v[i]);
+    return r;
+}
+inline LF vabsf(const LF& a) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fabs(a.v[i]);
+    return r;
+}
+inline float vabsf(float a) { return std::fabs(a); }
 inline LU fbits(const LF& a) {
     LU r;
     for (int i = 0; i < kWave; ++i) std::memcpy(&r.v[i], &a.v[i], 4);
@@ -363,6 +374,8 @@ VGC_HD double vmax(double a, double b) { return __builtin_fmax(a, b); }
 VGC_HD double vmin(double a, double b) { return __builtin_fmin(a, b); }
 
 VGC_HD float vmaxf(float a, float b) { return __builtin_fmaxf(a, b); }
+VGC_HD float vsqrtf(float a) { return __builtin_sqrtf(a); }
+VGC_HD float vabsf(float a) { return __builtin_fabsf(a); }
 VGC_HD uint32_t fbits(float a) { return __float_as_uint(a); }
 template <class T>
 VGC_HD T gld(const T* base, int idx, bool m) {

@@ -194,3 +194,39 @@ def test_random_float64_is_mt19937_64():
     assert out[9999] == np.float64(9981545732273789042) / 2.0**64
     assert np.all((out >= 0) & (out < 1))
     assert np.array_equal(ga.random_float64(1, 5), ga.random_float64(1, 5))
+
+
+def _transport_weights(emu_ctx, seed, n, newton):
+    """Laguerre weights of a damped Newton iterate of the semi-discrete
+    transport onto a 16-Gaussian mixture with log-normal masses: a weight
+    field of wide spread whose cells sit away from their seeds."""
+    from geogram_amd import warmstart
+    x, nu = warmstart.make_instance(seed, n)
+    w, _ = emu_ctx.ot_solve(x, nu, tol=1e-9, max_newton=newton)
+    return x, w
+
+
+def test_transport_weights_power_filter(emu_ctx):
+    x, w = _transport_weights(emu_ctx, 77, 2500, 3)
+    assert np.ptp(w) > 1e-2
+    r = emu_ctx.cells(x, weights=w)
+    s = r.stats
+    assert s["nb_failed"] == 0 and s["nb_pass_b"] > 0
+    # the power filter keeps the list tiers local: far fewer entries than
+    # the seeds within the max-weight radius (most of the instance)
+    assert s["radius_entries"] < 0.2 * len(x) * s["nb_pass_b"]
+    o = oracle.cells(x, w=w, centred=True, stride=512)
+    assert np.abs(r.vol - o["vol"]).max() <= 1e-10 * o["vol"].max()
+    nz = o["vol"] > 0
+    assert np.all(np.abs(r.vol[nz] - o["vol"][nz]) <= 1e-10 * o["vol"][nz])
+    bad = [i for i in range(len(x))
+           if set(r.neighbors(i).tolist()) != set(o["nbr"][i, :o["cnt"][i]].tolist())]
+    assert not bad, f"{len(bad)} neighbour sets differ"
+    # list tiers cut into small chunks: same cells bit for bit
+    emu_ctx.lib.cdll.vgc_emu_set_list_chunk(emu_ctx.ctx, ctypes.c_longlong(300))
+    try:
+        r2 = emu_ctx.cells(x, weights=w)
+    finally:
+        emu_ctx.lib.cdll.vgc_emu_set_list_chunk(emu_ctx.ctx, ctypes.c_longlong(1 << 27))
+    assert np.array_equal(r2.vol, r.vol) and np.array_equal(r2.mg, r.mg)
+    assert np.array_equal(r2.nbr, r.nbr) and np.array_equal(r2.farea, r.farea)

@@ -99,6 +99,17 @@ def test_gpu_matches_emulation_laguerre_and_lattice(gpu_ctx, emu_ctx):
     _same(rg, emu_ctx.cells(lat, k0=0))
 
 
+def test_gpu_matches_emulation_transport_weights(gpu_ctx, emu_ctx):
+    # weights of a Newton iterate of transport onto a Gaussian mixture: the
+    # power-filtered rings and list tiers, bit for bit
+    from geogram_amd import warmstart
+    x, nu = warmstart.make_instance(77, 2500)
+    w, _ = emu_ctx.ot_solve(x, nu, tol=1e-9, max_newton=3)
+    rg = gpu_ctx.cells(x, weights=w)
+    assert rg.stats["nb_pass_b"] > 0 and rg.stats["nb_failed"] == 0
+    _same(rg, emu_ctx.cells(x, weights=w))
+
+
 def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
     import geogram_amd as ga
 
