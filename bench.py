@@ -108,7 +108,7 @@ def main():
         nu = torch.full((m,), 1.0 / m, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        w, ost = ctx.ot_solve(xo, nu, tol=1e-4)
+        _, ost = ctx.ot_solve(xo, nu, tol=1e-4)
         torch.cuda.synchronize()
         tot = time.perf_counter() - t2
         ot = {"n": m, "start": "plain (w = 0)", "time_s": round(tot, 3),

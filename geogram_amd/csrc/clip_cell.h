@@ -57,7 +57,7 @@ struct PMax {
 constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 128;  // tree nodes per level of the walk
 constexpr int kBucketCap = 128; // buckets per ring
-constexpr int kMaxFilteredRings = 6;  // power-filtered rings before the list tiers
+constexpr int kMaxFilteredRings = 16;  // power-filtered rings before the list tiers
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -649,7 +649,7 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
                 LF cy = vmaxf(vmaxf(B.ly - F.cy, F.cy - B.hy), 0.0f);
                 LF cz = vmaxf(vmaxf(B.lz - F.cz, F.cz - B.hz), 0.0f);
                 LF dc2 = cx * cx + cy * cy + cz * cz;
-                hit = hit & pf_may_cut(F, dc2, B.wm, vsqrtf(dmax));
+                hit = hit & pf_node_may_cut(F, dc2, B.wm, vsqrtf(dmin), vsqrtf(dmax), fx, fy, fz);
             }
             LB internal = node < Lv;
             LB push = hit & internal;
@@ -691,7 +691,8 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
             d2 = ex * ex + ey * ey + ez * ez;
             LF gx = q.x - F.cx, gy = q.y - F.cy, gz = q.z - F.cz;
             LF dc2 = gx * gx + gy * gy + gz * gz;
-            take = ok & (d2 > rlo2) & (d2 <= rhi2) & pf_may_cut(F, dc2, q.w, vsqrtf(d2));
+            take = ok & (d2 > rlo2) & (d2 <= rhi2) &
+                   pf_may_cut(F, dc2, q.w, vsqrtf(d2), vabsf(ex), vabsf(ey), vabsf(ez));
         }
         uint64_t mt = ballot(take);
         LI r = cnt + mbcnt(mt);
@@ -732,6 +733,102 @@ VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
     }
 }
 
+// Seeds q != c of the instance with rlo2 < d2 <= rhi2 (FP32 squared
+// distance from the seed) that pass the power filter F, into L.cand as
+// (d2 bits << 32 | q), unsorted.  Depth-first walk of the tree: a stack in
+// the wave's LDS, up to 64 nodes popped per step (one per lane), passing
+// leaves scanned at once, so the stack stays short however many nodes pass.
+// When the candidates would overflow, the buffer is sorted and only the
+// seeds strictly nearer than its 64th are kept; rhi2 is lowered to just
+// below that distance, so the ring (rlo2, rhi2] it returns is complete.
+// Returns the count (<= kCandCap), or -1 when the stack overflows or the
+// nearest 64 are all at one distance.
+template <int S>
+VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float px, float py,
+                      float pz, float rlo2, float& rhi2, const PowerFilter& F) {
+    constexpr int kStack = 2 * kFrontCap;
+    const LI lane = lane_id();
+    const int p0 = P.IL.p0[inst], n = P.IL.n[inst], nb = P.IL.nb[inst], Lv = P.IL.L[inst];
+    const bnode* T = P.nodes + P.IL.t0[inst];
+    int* stk = &L.front[0][0];
+    lds_st(stk, LI(0), LI(1), lane == 0);
+    wave_sync();
+    int top = 1, cnt = 0;
+    float lim = rhi2;
+    while (top > 0) {
+        const int k = top < kWave ? top : kWave;
+        top -= k;
+        LB in = lane < k;
+        LI node = lds_ld(stk, select(in, top + lane, LI(0)));
+        wave_sync();  // popped before the pushes below reuse the slots
+        BoxLanes B = load_box(T, node, in);
+        LF dx = vmaxf(vmaxf(B.lx - px, px - B.hx), 0.0f);
+        LF dy = vmaxf(vmaxf(B.ly - py, py - B.hy), 0.0f);
+        LF dz = vmaxf(vmaxf(B.lz - pz, pz - B.hz), 0.0f);
+        LF dmin = dx * dx + dy * dy + dz * dz;
+        LF fx = vmaxf(px - B.lx, B.hx - px), fy = vmaxf(py - B.ly, B.hy - py),
+           fz = vmaxf(pz - B.lz, B.hz - pz);
+        LF dmax = fx * fx + fy * fy + fz * fz;
+        LF cx = vmaxf(vmaxf(B.lx - F.cx, F.cx - B.hx), 0.0f);
+        LF cy = vmaxf(vmaxf(B.ly - F.cy, F.cy - B.hy), 0.0f);
+        LF cz = vmaxf(vmaxf(B.lz - F.cz, F.cz - B.hz), 0.0f);
+        LF dc2 = cx * cx + cy * cy + cz * cz;
+        LB hit = in & (dmin <= lim) & (dmax * 1.00001f > rlo2) &
+                 pf_node_may_cut(F, dc2, B.wm, vsqrtf(dmin), vsqrtf(dmax), fx, fy, fz);
+        LB internal = node < Lv;
+        LB push = hit & internal;
+        LB leaf = hit & !internal;
+        uint64_t mp = ballot(push), ml = ballot(leaf);
+        if (top + 2 * popc64(mp) > kStack) return -1;
+        LI rp = top + 2 * mbcnt(mp);
+        lds_st(stk, select(push, rp, LI(0)), 2 * node, push);
+        lds_st(stk, select(push, rp + 1, LI(0)), 2 * node + 1, push);
+        top += 2 * popc64(mp);
+        lds_st(L.bucket, select(leaf, mbcnt(ml), LI(0)), node - Lv, leaf);
+        const int nl = popc64(ml);
+        wave_sync();
+        for (int base = 0; base < nl; base += kWave / kBucket) {
+            LI bi = base + (lane >> 4);
+            LI kk = lane & (kBucket - 1);
+            LB inb = bi < nl;
+            LI b = lds_ld(L.bucket, select(inb, bi, LI(0)));
+            LI m = p0 + b * kBucket + kk;
+            LB ok = inb & (b < nb) & (m < p0 + n) & (m != c);
+            SeedLanes q = lane_seed(P.pf, select(ok, m, LI(0)), ok);
+            LF ex = q.x - px, ey = q.y - py, ez = q.z - pz;
+            LF d2 = ex * ex + ey * ey + ez * ez;
+            LF gx = q.x - F.cx, gy = q.y - F.cy, gz = q.z - F.cz;
+            LF gc2 = gx * gx + gy * gy + gz * gz;
+            LB take = ok & (d2 > rlo2) & (d2 <= lim) &
+                      pf_may_cut(F, gc2, q.w, vsqrtf(d2), vabsf(ex), vabsf(ey), vabsf(ez));
+            if (cnt + popc64(ballot(take)) > kCandCap) {
+                // keep the seeds strictly nearer than the 64th nearest so far
+                sort_cand<S>(L, cnt);
+                const float d64 = __builtin_bit_cast(float, uint32_t(L.cand[kWave - 1] >> 32));
+                const float nl2 = nextafterf(d64, 0.0f);
+                if (!(nl2 > rlo2)) return -1;
+                const uint64_t kmax = uint64_t(__builtin_bit_cast(uint32_t, nl2)) << 32 |
+                                      0xFFFFFFFFull;
+                const int kept = popc64(ballot(lds_ld(L.cand, lane) <= LU64(kmax)));
+                if (kept == 0) return -1;
+                cnt = kept;
+                lim = nl2;
+                take = take & (d2 <= lim);
+                wave_sync();
+            }
+            uint64_t mt = ballot(take);
+            LI r = cnt + mbcnt(mt);
+            LB st = take & (r < kCandCap);
+            LU64 key = (cvt<uint64_t>(fbits(d2)) << LU64(32)) | cvt<uint64_t>(cvt<uint32_t>(m));
+            lds_st(L.cand, select(st, r, LI(0)), key, st);
+            cnt += popc64(mt);
+        }
+        wave_sync();
+    }
+    rhi2 = lim;
+    return cnt;
+}
+
 // Ball around the partial cell of a weighted seed for the power-distance
 // filter (types.h): centre = middle of the corners' bounding box, radius =
 // farthest corner plus the corner's certified position error
@@ -767,7 +864,7 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
     // the FP32 centre, relative to the seed
     const double fx = double(F.cx) - R.pc.x, fy = double(F.cy) - R.pc.y,
                  fz = double(F.cz) - R.pc.z;
-    LD rmax = LD(0.0);
+    LD rmax = LD(0.0), hmx = LD(0.0), hmy = LD(0.0), hmz = LD(0.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LB vs = mask_bit(st.valid[s]);
@@ -780,8 +877,14 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
         LD err = kGamma9 * (st.nmax[s] + qn * st.dabs[s]) /
                  select(vs, dd - kGamma9 * st.dabs[s], LD(1.0));
         rmax = vmax(rmax, select(vs, dist + err, LD(0.0)));
+        hmx = vmax(hmx, select(vs, vabs(gx) + err, LD(0.0)));
+        hmy = vmax(hmy, select(vs, vabs(gy) + err, LD(0.0)));
+        hmz = vmax(hmz, select(vs, vabs(gz) + err, LD(0.0)));
     }
     const double rc = wave_max(rmax) * (1.0 + 1e-9);
+    F.hx = f_up(wave_max(hmx) * (1.0 + 1e-9));
+    F.hy = f_up(wave_max(hmy) * (1.0 + 1e-9));
+    F.hz = f_up(wave_max(hmz) * (1.0 + 1e-9));
     const double off2 = fx * fx + fy * fy + fz * fz;
     if (!(rc < 1e30) || !(off2 < 1e30)) return false;
     F.rc = f_up(rc);
@@ -873,17 +976,14 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         const float all2 = f_up(1.0001 * double(bx * bx + by * by + bz * bz));
         if (!(rhi2 > 0.0f)) rhi2 = all2 * 1e-6f;
         float rlo2 = -1.0f;
-        PowerFilter F;
-        F.on = 0;
+        PowerFilter F{};
         int nfilt = 0;
         while (true) {
-            int cnt = gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
-#ifdef VGC_DEBUG_RINGS
-            fprintf(stderr, "RING c=%d rlo2=%g rhi2=%g F=%d rc=%g dcp=%g cnt=%d np=%d\n", c, rlo2, rhi2, F.on, F.rc, F.dcp, cnt, R.np);
-#endif
-            if (F.on && (cnt < 0 || ++nfilt > kMaxFilteredRings)) {
-                // the filtered walk is wider than the wave's buffers: the list
-                // tiers take the cell, with the same filter and no capacity
+            int cnt = F.on ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
+                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
+            if (F.on && ++nfilt > kMaxFilteredRings) {
+                // the list tiers take the cell, with the same filter and no
+                // capacity limit
                 R.flags |= CELL_NEED_K;
                 break;
             }
@@ -926,9 +1026,10 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             }
             rlo2 = rhi2;
             if (P.weighted && make_filter<S>(R, F)) {
-                // every seed beyond the ring that passes the power filter of
-                // the cell as it is now (narrowed above if too many)
-                rhi2 = all2;
+                // every seed beyond the ring, within the max-weight radius,
+                // that passes the power filter of the cell as it is now
+                // (narrowed above if too many)
+                rhi2 = need2 < all2 ? need2 : all2;
             } else {
                 F.on = 0;
                 rhi2 = need2 < all2 ? need2 : all2;
@@ -958,8 +1059,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     }
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         // the partial cell contains the cell: its ball filters the list tiers
-        PowerFilter G;
-        G.on = 0;
+        PowerFilter G{};
         const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
         if (lane0()) {
             P.flags[c] = flags;
@@ -970,6 +1070,10 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 b.cy = G.cy;
                 b.cz = G.cz;
                 b.rc = okf ? G.rc : -1.0f;
+                b.hx = G.hx;
+                b.hy = G.hy;
+                b.hz = G.hz;
+                b.pad = 0.0f;
                 P.ball[c] = b;
             }
         }
@@ -1132,7 +1236,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             P.flags[c] = flags;
             if (P.ball) {
                 fball b;
-                b.cx = b.cy = b.cz = 0.0f;
+                b.cx = b.cy = b.cz = b.hx = b.hy = b.hz = b.pad = 0.0f;
                 b.rc = -1.0f;
                 P.ball[c] = b;
             }

@@ -158,7 +158,8 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
             const float fy = fmaxf(py - B.lo[1], B.hi[1] - py);
             const float fz = fmaxf(pz - B.lo[2], B.hi[2] - pz);
             const float dfar = sqrtf(fx * fx + fy * fy + fz * fz);
-            inside = pf_may_cut(*F, box_dist2(B, F->cx, F->cy, F->cz), B.wmax, dfar);
+            inside = pf_node_may_cut(*F, box_dist2(B, F->cx, F->cy, F->cz), B.wmax,
+                                     sqrtf(box_dist2(B, px, py, pz)), dfar, fx, fy, fz);
         }
         if (inside && node < L) {
             node = 2 * node;
@@ -176,7 +177,8 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
                     float d = dist2f(a, px, py, pz);
                     bool take = d <= r2;
                     if (take && filt)
-                        take = pf_may_cut(*F, dist2f(a, F->cx, F->cy, F->cz), a.w, sqrtf(d));
+                        take = pf_may_cut(*F, dist2f(a, F->cx, F->cy, F->cz), a.w, sqrtf(d),
+                                          fabsf(a.x - px), fabsf(a.y - py), fabsf(a.z - pz));
                     if (take) {
                         if (out) {
                             out[cnt] = m;

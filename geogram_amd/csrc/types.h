@@ -99,28 +99,57 @@ VGC_SD float radius_for(double r2, double wi, double wmax, int weighted) {
 // cut is never dropped.
 struct PowerFilter {
     int on;
-    float cx, cy, cz;  // ball centre
+    float cx, cy, cz;  // ball centre = centre of the cell's bounding box
     float rc;          // ball radius, rounded up
+    float hx, hy, hz;  // half-widths of the bounding box, rounded up
     float phi;         // |ctr - p|^2 - wi
     float dcp;         // |ctr - p|
     float s;           // 1e-6 x coordinate scale of the instance
 };
 
-// Ball of a partial cell handed from the clipping kernel to the list tiers
-// (rc < 0: none).
+// Ball and box of a flagged partial cell, handed from the clipping kernel to
+// the list tiers (rc < 0: none).
 struct fball {
-    float cx, cy, cz, rc;
+    float cx, cy, cz, rc, hx, hy, hz, pad;
 };
 
-// dc2: squared distance from the ball centre to q (or a lower bound over a
-// tree node), wj: q's weight (or an upper bound), d: |q - p| (or an upper
-// bound).  True when q may cut.
+// The cell lies in the ball and in the box, so over the cell
+// 2 (x - ctr).(p - q) >= -min(2 rc |q - p|, 2 sum_a h_a |q_a - p_a|).
+// dc2: squared distance from the centre to q (or a lower bound over a tree
+// node), wj: q's weight (or an upper bound), d: |q - p| and ax, ay, az:
+// |q_a - p_a| (or upper bounds).  True when q may cut.
 template <class TF>
-VGC_HD auto pf_may_cut(const PowerFilter& F, const TF& dc2, const TF& wj, const TF& d) {
+VGC_HD auto pf_may_cut(const PowerFilter& F, const TF& dc2, const TF& wj, const TF& d,
+                       const TF& ax, const TF& ay, const TF& az) {
+    const TF rb = (2.0f * F.rc) * d;
+    const TF rx = 2.0f * (F.hx * ax + F.hy * ay + F.hz * az);
+    const TF rd = vminf(rb, rx);
+    const TF lhs = dc2 - wj - F.phi - rd;
+    const TF mar = 1e-6f * (dc2 + vabsf(wj) + fabsf(F.phi) + rd) + F.s * (F.dcp + 2.0f * d + F.rc);
+    return lhs < mar;
+}
+
+// Ball form only (d a lower bound of |q - p| on a monotone branch).
+template <class TF>
+VGC_HD auto pf_may_cut_ball(const PowerFilter& F, const TF& dc2, const TF& wj, const TF& d) {
     const TF rd = (2.0f * F.rc) * d;
     const TF lhs = dc2 - wj - F.phi - rd;
     const TF mar = 1e-6f * (dc2 + vabsf(wj) + fabsf(F.phi) + rd) + F.s * (F.dcp + 2.0f * d + F.rc);
     return lhs < mar;
+}
+
+// Tree node: dc2 = squared distance from the centre to the node's box,
+// wm = the node's max weight, dmin / dmax = nearest / farthest distance from
+// p to the box, fx, fy, fz = farthest per-axis offsets.  Two lower bounds of
+// the left side over the node: with the farthest offsets, and, since
+// |ctr - q| >= |q - p| - |ctr - p| and (d - dcp)^2 - 2 rc d grows for
+// d >= dcp + rc, with d = dmin there.
+template <class TF>
+VGC_HD auto pf_node_may_cut(const PowerFilter& F, const TF& dc2, const TF& wm, const TF& dmin,
+                            const TF& dmax, const TF& fx, const TF& fy, const TF& fz) {
+    auto may = pf_may_cut(F, dc2, wm, dmax, fx, fy, fz);
+    const TF t = dmin - F.dcp;
+    return may & ((dmin < F.dcp + F.rc) | pf_may_cut_ball(F, t * t, wm, dmin));
 }
 
 // Filter of seed p (FP64 position, weight) from a ball; the instance box
@@ -132,6 +161,9 @@ VGC_SD PowerFilter filter_from_ball(const fball& b, const pt4& p, const double* 
     F.cy = b.cy;
     F.cz = b.cz;
     F.rc = b.rc;
+    F.hx = b.hx;
+    F.hy = b.hy;
+    F.hz = b.hz;
     const double fx = double(b.cx) - p.x, fy = double(b.cy) - p.y, fz = double(b.cz) - p.z;
     const double off2 = fx * fx + fy * fy + fz * fz;
     F.phi = float(off2 - p.w);

@@ -222,6 +222,12 @@ inline LF vabsf(const LF& a) {
     return r;
 }
 inline float vabsf(float a) { return std::fabs(a); }
+inline LF vminf(const LF& a, const LF& b) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmin(a.v[i], b.v[i]);
+    return r;
+}
+inline float vminf(float a, float b) { return std::fmin(a, b); }
 inline LU fbits(const LF& a) {
     LU r;
     for (int i = 0; i < kWave; ++i) std::memcpy(&r.v[i], &a.v[i], 4);
@@ -376,6 +382,7 @@ VGC_HD double vmin(double a, double b) { return __builtin_fmin(a, b); }
 VGC_HD float vmaxf(float a, float b) { return __builtin_fmaxf(a, b); }
 VGC_HD float vsqrtf(float a) { return __builtin_sqrtf(a); }
 VGC_HD float vabsf(float a) { return __builtin_fabsf(a); }
+VGC_HD float vminf(float a, float b) { return __builtin_fminf(a, b); }
 VGC_HD uint32_t fbits(float a) { return __float_as_uint(a); }
 template <class T>
 VGC_HD T gld(const T* base, int idx, bool m) {

@@ -36,6 +36,13 @@ int main() {
     otm.set_epsilon(1e-4);
     otm.optimize(50);
     if (!otm.stats().converged) return 2;
+    // learned start with an untrained (zero) network: weights 0, judged, converges
+    std::vector<float> params(VGC_MLP_PARAMS, 0.0f);
+    OptimalTransportMap3d otm2(*ctx, box);
+    otm2.set_points(n, cvt.embedding(0));
+    otm2.set_learned_initial_weights(params.data());
+    otm2.optimize(50);
+    if (!otm2.stats().converged || otm2.stats().newton_iters != otm.stats().newton_iters) return 4;
     std::printf("ok: sum m = %.17g, OT newton %u\n", s, otm.stats().newton_iters);
     delete ctx;
     return 0;

@@ -209,23 +209,31 @@ def _transport_weights(emu_ctx, seed, n, newton):
 def test_transport_weights_power_filter(emu_ctx):
     x, w = _transport_weights(emu_ctx, 77, 2500, 3)
     assert np.ptp(w) > 1e-2
-    r = emu_ctx.cells(x, weights=w)
-    s = r.stats
-    assert s["nb_failed"] == 0 and s["nb_pass_b"] > 0
-    # the power filter keeps the list tiers local: far fewer entries than
-    # the seeds within the max-weight radius (most of the instance)
-    assert s["radius_entries"] < 0.2 * len(x) * s["nb_pass_b"]
     o = oracle.cells(x, w=w, centred=True, stride=512)
-    assert np.abs(r.vol - o["vol"]).max() <= 1e-10 * o["vol"].max()
     nz = o["vol"] > 0
-    assert np.all(np.abs(r.vol[nz] - o["vol"][nz]) <= 1e-10 * o["vol"][nz])
-    bad = [i for i in range(len(x))
-           if set(r.neighbors(i).tolist()) != set(o["nbr"][i, :o["cnt"][i]].tolist())]
-    assert not bad, f"{len(bad)} neighbour sets differ"
+
+    def check(r):
+        assert r.stats["nb_failed"] == 0
+        assert np.abs(r.vol - o["vol"]).max() <= 1e-10 * o["vol"].max()
+        assert np.all(np.abs(r.vol[nz] - o["vol"][nz]) <= 1e-10 * o["vol"][nz])
+        bad = [i for i in range(len(x))
+               if set(r.neighbors(i).tolist()) != set(o["nbr"][i, :o["cnt"][i]].tolist())]
+        assert not bad, f"{len(bad)} neighbour sets differ"
+
+    # in-kernel gathering: power-filtered depth-first rings
+    check(emu_ctx.cells(x, weights=w))
+    # first pass from 16-NN lists: most cells go to the list tiers, whose
+    # radius queries carry the power filter of the partial cell
+    r = emu_ctx.cells(x, weights=w, k0=16)
+    s = r.stats
+    assert s["nb_pass_b"] > len(x) // 4
+    # far fewer entries than the seeds within the max-weight radius
+    assert s["radius_entries"] < 0.2 * len(x) * s["nb_pass_b"]
+    check(r)
     # list tiers cut into small chunks: same cells bit for bit
     emu_ctx.lib.cdll.vgc_emu_set_list_chunk(emu_ctx.ctx, ctypes.c_longlong(300))
     try:
-        r2 = emu_ctx.cells(x, weights=w)
+        r2 = emu_ctx.cells(x, weights=w, k0=16)
     finally:
         emu_ctx.lib.cdll.vgc_emu_set_list_chunk(emu_ctx.ctx, ctypes.c_longlong(1 << 27))
     assert np.array_equal(r2.vol, r.vol) and np.array_equal(r2.mg, r.mg)
