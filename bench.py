@@ -45,6 +45,8 @@ def main():
                     help="0: neighbours gathered in the clipping kernel; 16..64: kNN lists")
     ap.add_argument("--ot-n", type=int, default=1_000_000,
                     help="seeds of the semi-discrete OT leg (0: skip)")
+    ap.add_argument("--ot4-n", type=int, default=1_000_000,
+                    help="seeds of the learned-warm-start OT leg (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=400_000,
                     help="cells of the workload timed on the host with the oracle (0: skip)")
     args = ap.parse_args()
@@ -87,7 +89,7 @@ def main():
     clip_ms = float(np.mean([t["t_clip_a_gpu_ms"] for t in stats]))
     achieved = flops_a / (clip_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "k_clip<1,4,false> (first clipping pass, every cell)",
+        "kernel": "k_clip<1,4,false,true,false> (first clipping pass, every cell)",
         "bound": "valu",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
@@ -108,7 +110,7 @@ def main():
         nu = torch.full((m,), 1.0 / m, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _, ost = ctx.ot_solve(xo, nu, tol=1e-4)
+        _, ost = ctx.ot_solve(xo, nu, tol=1e-4, verbose=True)
         torch.cuda.synchronize()
         tot = time.perf_counter() - t2
         ot = {"n": m, "start": "plain (w = 0)", "time_s": round(tot, 3),
@@ -116,6 +118,41 @@ def main():
               "cell_passes": int(ost["cell_passes"]), "err0": ost["err0"],
               "final_err": ost["final_err"], "converged": bool(ost["converged"]),
               "t_cells_ms": round(ost["t_cells_ms"], 1), "t_cg_ms": round(ost["t_cg_ms"], 1)}
+
+    # learned warm start: 1M seeds of a 16-Gaussian mixture with log-normal
+    # masses (seed 4), plain start against the MLP start (features + forward
+    # included in its time), both to 1e-4 max relative mass error
+    ws = None
+    if args.ot4_n > 0:
+        from geogram_amd import warmstart
+
+        m = args.ot4_n
+        x4, nu4 = warmstart.make_instance(4, m)
+        x4d, nu4d = torch.from_numpy(x4).cuda(), torch.from_numpy(nu4).cuda()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        _, sp = ctx.ot_solve(x4d, nu4d, tol=1e-4, verbose=True)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t2
+        ws = {"n": m, "plain": {"time_s": round(tp, 3), "newton_iters": int(sp["newton_iters"]),
+                                "cell_passes": int(sp["cell_passes"]),
+                                "converged": bool(sp["converged"]), "err0": sp["err0"]}}
+        if os.path.exists(warmstart.PARAMS_FILE):
+            params = torch.from_numpy(warmstart.load_params()).cuda()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            w0 = warmstart.predict_weights(ctx, x4d, nu4d, params=params)
+            _, sl = ctx.ot_solve(x4d, nu4d, w0=w0, tol=1e-4, judge_start=True, verbose=True)
+            torch.cuda.synchronize()
+            tl = time.perf_counter() - t2
+            ws["learned"] = {"time_s": round(tl, 3), "newton_iters": int(sl["newton_iters"]),
+                             "cell_passes": int(sl["cell_passes"]),
+                             "converged": bool(sl["converged"]), "err0": sl["err0"],
+                             "fell_back_to_zero": bool(sl["used_fallback_start"])}
+            ws["newton_cut"] = round(1.0 - sl["newton_iters"] / max(1, sp["newton_iters"]), 3)
+            ws["time_cut"] = round(1.0 - tl / tp, 3)
+        else:
+            ws["learned"] = None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -158,6 +195,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ot": ot,
+        "ot_warm_start": ws,
         "checks": {"sum_vol_minus_1": sumvol_err, "failed": int(s["nb_failed"]),
                    "uncertain": int(s["nb_uncertain"])},
         "phases_ms": {k: round(s[k], 3) for k in ("t_build_ms", "t_knn_ms", "t_clip_ms",

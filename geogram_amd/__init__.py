@@ -186,10 +186,11 @@ class Context:
 
     # ------------------------------------------------------------------- OT
     def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                 judge_start=False):
+                 judge_start=False, verbose=False):
         """Laguerre weights w with cell masses nu (uniform density on box).
         Returns (w, stats); w0 is the start (None: zeros).  judge_start: keep
-        w0 only if its mass residual beats the zero start's."""
+        w0 only if its mass residual beats the zero start's.  verbose: one
+        line per Newton step on stderr."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
@@ -207,7 +208,8 @@ class Context:
                                      boxa.ctypes.data_as(ctypes.c_void_p), float(tol),
                                      int(max_newton),
                                      (VGC_DEVICE_PTRS if dev else 0)
-                                     | (_lib.VGC_OT_JUDGE_START if judge_start else 0),
+                                     | (_lib.VGC_OT_JUDGE_START if judge_start else 0)
+                                     | (_lib.VGC_OT_VERBOSE if verbose else 0),
                                      ctypes.byref(st))
         self._check(rc)
         return w, st.as_dict()

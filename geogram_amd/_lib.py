@@ -25,6 +25,7 @@ VGC_DEVICE_PTRS = 1
 VGC_NO_NEIGHBORS = 2
 VGC_COUNT_WORK = 4
 VGC_OT_JUDGE_START = 8
+VGC_OT_VERBOSE = 16
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6
@@ -33,7 +34,7 @@ _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain n
              "nb_disconnected nb_clips nb_tests radius_entries").split()
 _STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
 _STAT_F64B = "t_clip_a_gpu_ms t_knn_gpu_ms".split()
-_STAT_U64B = "a_cells a_tests a_live a_new a_tets a_facets".split()
+_STAT_U64B = "a_cells a_tests a_live a_new a_tets a_facets nb_fail_capacity".split()
 
 
 class Stats(ctypes.Structure):

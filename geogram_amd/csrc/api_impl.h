@@ -73,6 +73,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->a_new = e.a_new;
     s->a_tets = e.a_tets;
     s->a_facets = e.a_facets;
+    s->nb_fail_capacity = e.nb_fail_capacity;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
@@ -341,7 +342,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         OtStats os;
         const uint32_t maxn = max_newton ? max_newton : 100;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
-                          (flags & VGC_OT_JUDGE_START) != 0)) {
+                          (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0)) {
             ctx->err = "vgc_ot_solve: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
         }

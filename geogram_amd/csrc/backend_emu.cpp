@@ -145,18 +145,29 @@ struct EmuBackend {
             std::unique_ptr<CellLds<S>> L(new CellLds<S>());
             std::memset(L.get(), 0, sizeof(CellLds<S>));
 #pragma omp for schedule(dynamic, 16)
-            for (int it = 0; it < P.n_items; ++it) process_cell<S, E, G>(P, it, *L);
+            for (int it = 0; it < P.n_items; ++it) {
+                if (P.weighted)
+                    process_cell<S, E, G, true>(P, it, *L);
+                else
+                    process_cell<S, E, G, false>(P, it, *L);
+            }
         }
     }
 
     template <bool E, bool G>
     void clip_eg(int S, const ClipParams& P) {
-        if (S == 1)
+        if (S == 1) {
             clip_s<1, E, G>(P);
-        else if (S == 2)
+        } else if (S == 2) {
             clip_s<2, E, G>(P);
-        else
+        } else if (S == 4) {
             clip_s<4, E, G>(P);
+        } else {
+            if constexpr (!G)
+                clip_s<8, E, false>(P);
+            else
+                throw std::runtime_error("clip: 512-corner cells run from lists only");
+        }
     }
 
     void clip(int S, bool exact, bool gather, const ClipParams& P) {

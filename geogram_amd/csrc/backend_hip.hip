@@ -136,13 +136,15 @@ __global__ __launch_bounds__(256) void k_knn(const pf4* __restrict__ pf,
     knn_query<K>(pf, nodes, IL, q, out + (size_t)t * K, out_d2 ? out_d2 + (size_t)t * K : nullptr);
 }
 
-template <int S, int WAVES, bool EXACT, bool GATHER>
-__global__ __launch_bounds__(64 * WAVES) void k_clip(ClipParams P) {
+// S = 1 (every cell's first pass): registers capped for 3 waves per SIMD
+// (latency-bound kernel; occupancy hides the dependent LDS round trips)
+template <int S, int WAVES, bool EXACT, bool GATHER, bool W>
+__global__ __launch_bounds__(64 * WAVES, S == 1 ? 3 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = blockIdx.x * WAVES + wave;
     if (item >= P.n_items) return;  // whole wavefront
-    process_cell<S, EXACT, GATHER>(P, item, lds[wave]);
+    process_cell<S, EXACT, GATHER, W>(P, item, lds[wave]);
 }
 
 __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
@@ -561,15 +563,29 @@ struct HipBackend {
         }
         check();
     }
+    template <bool E, bool G, bool W>
+    void clip_egw(int S, const ClipParams& P) {
+        const unsigned g4 = blocks_for(P.n_items, 4), g1 = blocks_for(P.n_items, 1);
+        if (S == 1) {
+            k_clip<1, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
+        } else if (S == 2) {
+            k_clip<2, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
+        } else if (S == 4) {
+            k_clip<4, 1, E, G, W><<<g1, 64, 0, stream_>>>(P);
+        } else {
+            // 512 corners: list mode only (the last tier)
+            if constexpr (!G)
+                k_clip<8, 1, E, false, W><<<g1, 64, 0, stream_>>>(P);
+            else
+                throw std::runtime_error("clip: 512-corner cells run from lists only");
+        }
+    }
     template <bool E, bool G>
     void clip_eg(int S, const ClipParams& P) {
-        const unsigned g4 = blocks_for(P.n_items, 4), g1 = blocks_for(P.n_items, 1);
-        if (S == 1)
-            k_clip<1, 4, E, G><<<g4, 256, 0, stream_>>>(P);
-        else if (S == 2)
-            k_clip<2, 4, E, G><<<g4, 256, 0, stream_>>>(P);
+        if (P.weighted)
+            clip_egw<E, G, true>(S, P);
         else
-            k_clip<4, 1, E, G><<<g1, 64, 0, stream_>>>(P);
+            clip_egw<E, G, false>(S, P);
     }
     void clip(int S, bool exact, bool gather, const ClipParams& P) {
         if (P.n_items <= 0) return;

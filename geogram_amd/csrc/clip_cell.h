@@ -31,6 +31,8 @@
 // jobs and the two border-walk tables live in the wave's slice of LDS.
 #pragma once
 
+#include <type_traits>
+
 #include "wave.h"
 #include "types.h"
 #include "exact_det.h"
@@ -48,10 +50,19 @@ enum CellFlag : uint32_t {
 };
 
 // plane slots per cell (8-bit ids): 128 with 64 or 128 corner slots, 256 with
-// 256 corner slots
+// 256 or 512 corner slots
 template <int S>
 struct PMax {
     static constexpr int value = S >= 4 ? 256 : 128;
+};
+
+// Corner ids in the adjacency words: three fields of 8 bits up to 256
+// corners, of 10 bits for 512; the per-plane corner tables widen with them.
+template <int S>
+struct CornerBits {
+    static constexpr int value = S <= 4 ? 8 : 10;
+    static constexpr uint32_t mask = (1u << value) - 1u;
+    using id_t = typename std::conditional<(S <= 4), uint8_t, uint16_t>::type;
 };
 
 constexpr int kCandCap = 128;   // candidates per distance ring
@@ -106,8 +117,8 @@ struct CellLds {
     uint32_t cpl[64 * S];  // corner planes (integration walk)
     uint32_t cadj[64 * S]; // corner adjacency (integration walk)
     double q[64 * S][3];   // corner points (integration)
-    uint8_t st_tab[kPMax]; // new corner whose first plane is p
-    uint8_t en_tab[kPMax]; // new corner whose second plane is p
+    typename CornerBits<S>::id_t st_tab[kPMax];  // new corner whose first plane is p
+    typename CornerBits<S>::id_t en_tab[kPMax];  // new corner whose second plane is p
     // gather mode: candidates of the current distance ring, sorted by
     // (FP32 squared distance, Morton position), and the tree walk buffers
     uint64_t cand[kCandCap];
@@ -127,6 +138,12 @@ VGC_HD LB conf_bit(const uint64_t (&m)[S], const LI& cid) {
 }
 
 VGC_HD LU byte_of(const LU& w, int k) { return (w >> LU(8 * k)) & LU(0xFF); }
+
+// field k of an adjacency word
+template <int S>
+VGC_HD LU adj_of(const LU& w, int k) {
+    return (w >> LU(CornerBits<S>::value * k)) & LU(CornerBits<S>::mask);
+}
 
 struct Corner {
     LD nx, ny, nz, d, nmax, dabs, r2;
@@ -233,7 +250,8 @@ VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const p
         st.dabs[0] = C.dabs;
         st.r2[0] = select(m, C.r2, LD(0.0));
         st.pl[0] = cvt<uint32_t>(a) | (cvt<uint32_t>(b) << LU(8)) | (cvt<uint32_t>(c) << LU(16));
-        st.adj[0] = cvt<uint32_t>(a0) | (cvt<uint32_t>(a1) << LU(8)) | (cvt<uint32_t>(a2) << LU(16));
+        st.adj[0] = cvt<uint32_t>(a0) | (cvt<uint32_t>(a1) << LU(CornerBits<S>::value)) |
+                    (cvt<uint32_t>(a2) << LU(2 * CornerBits<S>::value));
         st.valid[0] = 0xFFull;
 #pragma unroll
         for (int s = 1; s < S; ++s) {
@@ -351,7 +369,7 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
         LB cs = mask_bit(cm[s]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            LI nb = cvt<int>(byte_of(st.adj[s], k));
+            LI nb = cvt<int>(adj_of<S>(st.adj[s], k));
             LB nbc = conf_bit<S>(cm, nb);
             bk[k][s] = ballot(cs & !nbc);
             total += popc64(bk[k][s]);
@@ -376,7 +394,7 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
                 LB m = mask_bit(bk[k][s]);
                 LI r = base + mbcnt(bk[k][s]);
                 LU pa = byte_of(st.pl[s], k), pb = byte_of(st.pl[s], (k + 1) % 3);
-                LU y = byte_of(st.adj[s], k);
+                LU y = adj_of<S>(st.adj[s], k);
                 lds_st(L.job, select(m, r, LI(0)), pa | (pb << LU(8)) | (y << LU(16)), m);
                 base += popc64(bk[k][s]);
             }
@@ -398,7 +416,7 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
                 LU jb = lds_ld(L.job, select(is_t, fr_rank, LI(0)));
                 LI a = cvt<int>(jb & LU(0xFF)) & (kPMax - 1);
                 LI b = cvt<int>((jb >> LU(8)) & LU(0xFF)) & (kPMax - 1);
-                LU y = (jb >> LU(16)) & LU(0xFF);
+                LU y = (jb >> LU(16)) & LU(CornerBits<S>::mask);
                 LD eax = lds_ld(&L.pe[0][0], a * 4 + 0), eay = lds_ld(&L.pe[0][0], a * 4 + 1),
                    eaz = lds_ld(&L.pe[0][0], a * 4 + 2), hav = lds_ld(&L.pe[0][0], a * 4 + 3);
                 LD ebx = lds_ld(&L.pe[0][0], b * 4 + 0), eby = lds_ld(&L.pe[0][0], b * 4 + 1),
@@ -418,8 +436,8 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
                                   st.pl[s]);
                 st.adj[s] = select(is_t, y, st.adj[s]);
                 LI cid = lane + 64 * s;
-                lds_st(L.st_tab, a, cvt<uint8_t>(cid), is_t);
-                lds_st(L.en_tab, b, cvt<uint8_t>(cid), is_t);
+                lds_st(L.st_tab, a, cvt<typename CornerBits<S>::id_t>(cid), is_t);
+                lds_st(L.en_tab, b, cvt<typename CornerBits<S>::id_t>(cid), is_t);
             }
         }
     }
@@ -440,21 +458,23 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
             LB wrong = is_t & ((own_a != cid) | (own_b != cid) | !conf_bit<S>(tm, n1) |
                                !conf_bit<S>(tm, n2));
             bad = bad || any(wrong);
-            st.adj[s] = select(is_t, (st.adj[s] & LU(0xFF)) | (cvt<uint32_t>(n1) << LU(8)) |
-                                         (cvt<uint32_t>(n2) << LU(16)),
+            st.adj[s] = select(is_t, (st.adj[s] & LU(CornerBits<S>::mask)) |
+                                         (cvt<uint32_t>(n1) << LU(CornerBits<S>::value)) |
+                                         (cvt<uint32_t>(n2) << LU(2 * CornerBits<S>::value)),
                                st.adj[s]);
         }
         LB keep = mask_bit(st.valid[s] & ~cm[s]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            LI nb = cvt<int>(byte_of(st.adj[s], k));
+            LI nb = cvt<int>(adj_of<S>(st.adj[s], k));
             LB hit = keep & conf_bit<S>(cm, nb);
             if (any(hit)) {
                 LI v = cvt<int>(byte_of(st.pl[s], (k + 1) % 3)) & (kPMax - 1);
                 LI z = cvt<int>(lds_ld(L.st_tab, v)) & (64 * S - 1);
                 bad = bad || any(hit & !conf_bit<S>(tm, z));
-                LU cleared = st.adj[s] & ~LU(0xFFu << (8 * k));
-                st.adj[s] = select(hit, cleared | (cvt<uint32_t>(z) << LU(8 * k)), st.adj[s]);
+                LU cleared = st.adj[s] & ~LU(CornerBits<S>::mask << (CornerBits<S>::value * k));
+                st.adj[s] = select(hit, cleared | (cvt<uint32_t>(z) << LU(CornerBits<S>::value * k)),
+                                   st.adj[s]);
             }
         }
     }
@@ -483,7 +503,7 @@ struct CellRun {
 // One candidate neighbour j (Morton position) in nearest-first order: the
 // coincident-seed rule, the security-radius stop, then the clip.  Returns
 // true when the cell is finished (or failed) and the candidate loop must end.
-template <int S, bool EXACT>
+template <int S, bool EXACT, bool W>
 VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
     if (j == R.c) return false;
     const pt4 pj = P.pts[j];
@@ -502,7 +522,7 @@ VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
     }
     // security radius (generic_RVD.h:2303-2323)
     bool reach;
-    if (!P.weighted) {
+    if (!W) {
         bool anyin = false;
 #pragma unroll
         for (int s = 0; s < S; ++s)
@@ -530,7 +550,7 @@ VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
         return false;
     }
     R.prev = pj;
-    const double h = P.weighted ? 0.5 * (d2 + (R.wi - pj.w)) : 0.5 * d2;
+    const double h = W ? 0.5 * (d2 + (R.wi - pj.w)) : 0.5 * d2;
     bool cut = false;
     ++R.ntest;
     R.flags = clip_plane<S, EXACT>(R.st, L, R.np, ex, ey, ez, h, j, R.flags, cut, R.nlive, R.nnew);
@@ -900,7 +920,7 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
 // Security-radius loop + integration for work item `item`.  GATHER: the
 // wave finds the neighbours itself, ring after ring of distance; otherwise
 // it reads them from P.list.
-template <int S, bool EXACT, bool GATHER>
+template <int S, bool EXACT, bool GATHER, bool W>
 VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
@@ -915,7 +935,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     R.c = c;
     R.orig_c = P.orig[c];
     R.wi = pc.w;
-    R.cw = P.weighted ? (pc.w - P.wmax[inst]) : 0.0;
+    R.cw = W ? (pc.w - P.wmax[inst]) : 0.0;
     R.box = box;
     R.np = 6;
     R.flags = 0;
@@ -943,7 +963,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 complete = true;  // short list: the instance has fewer seeds
                 break;
             }
-            if (consider<S, EXACT>(P, L, R, j)) break;
+            if (consider<S, EXACT, W>(P, L, R, j)) break;
         }
     } else {
         const pf4 pq = P.pf[c];
@@ -979,9 +999,10 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         PowerFilter F{};
         int nfilt = 0;
         while (true) {
-            int cnt = F.on ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
-                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
-            if (F.on && ++nfilt > kMaxFilteredRings) {
+            int cnt = (W && F.on)
+                          ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
+                          : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
+            if (W && F.on && ++nfilt > kMaxFilteredRings) {
                 // the list tiers take the cell, with the same filter and no
                 // capacity limit
                 R.flags |= CELL_NEED_K;
@@ -1007,7 +1028,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             bool done = false;
             for (int jj = 0; jj < cnt; ++jj) {
                 const int j = int(uint32_t(L.cand[jj]));
-                if (consider<S, EXACT>(P, L, R, j)) {
+                if (consider<S, EXACT, W>(P, L, R, j)) {
                     done = true;
                     break;
                 }
@@ -1019,13 +1040,13 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             for (int s = 0; s < S; ++s)
                 r2m = vmax(r2m, select(mask_bit(R.st.valid[s]), R.st.r2[s], LD(0.0)));
             const double r2now = wave_max(r2m);
-            const float need2 = radius_for(r2now, R.wi, P.weighted ? P.wmax[inst] : 0.0, P.weighted);
+            const float need2 = radius_for(r2now, R.wi, W ? P.wmax[inst] : 0.0, W);
             if (need2 <= rhi2 || rhi2 >= all2) {
                 complete = true;
                 break;
             }
             rlo2 = rhi2;
-            if (P.weighted && make_filter<S>(R, F)) {
+            if (W && make_filter<S>(R, F)) {
                 // every seed beyond the ring, within the max-weight radius,
                 // that passes the power filter of the cell as it is now
                 // (narrowed above if too many)
@@ -1060,7 +1081,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         // the partial cell contains the cell: its ball filters the list tiers
         PowerFilter G{};
-        const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
+        const bool okf = W && P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
         if (lane0()) {
             P.flags[c] = flags;
             P.r2[c] = r2max;
@@ -1145,8 +1166,9 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LU plt = lds_ld(L.cpl, t);
             LU adt = lds_ld(L.cadj, t);
             LU fu = cvt<uint32_t>(fs);
-            LI sh = select(byte_of(plt, 0) == fu, LI(16), select(byte_of(plt, 1) == fu, LI(0), LI(8)));
-            return cvt<int>((adt >> cvt<uint32_t>(sh)) & LU(0xFF)) & (64 * S - 1);
+            constexpr int B = CornerBits<S>::value;
+            LI sh = select(byte_of(plt, 0) == fu, LI(2 * B), select(byte_of(plt, 1) == fu, LI(0), LI(B)));
+            return cvt<int>((adt >> cvt<uint32_t>(sh)) & LU(CornerBits<S>::mask)) & (64 * S - 1);
         };
         LD q0x = lds_ld(&L.q[0][0], t0 * 3 + 0), q0y = lds_ld(&L.q[0][0], t0 * 3 + 1),
            q0z = lds_ld(&L.q[0][0], t0 * 3 + 2);

@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "pipeline.h"
@@ -46,7 +47,7 @@ public:
     // also measure the zero start and keep whichever start has the smaller
     // |m - nu|_2 (one more cell pass).
     int solve(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
-              double cg_tol, int cg_max, OtStats& st, bool judge = false) {
+              double cg_tol, int cg_max, OtStats& st, bool judge = false, bool verbose = false) {
         auto t0 = std::chrono::steady_clock::now();
         const int n = pb.n;
         pb.want_neighbors = true;
@@ -64,6 +65,7 @@ public:
 
         // 0: cells of w computed; 1: some cell empty or failed (the masses
         // are not all valid); -1: error
+        long long rej_empty = 0, rej_failed = 0, rej_test = 0;
         auto cells = [&](const double* w, bool trial) -> int {
             auto tc = std::chrono::steady_clock::now();
             pb.d_w = w;
@@ -73,7 +75,15 @@ public:
             st.cell_passes++;
             st.t_cells_ms += ms_since(tc);
             if (rc < 0) return -1;
-            return (rc > 0 || es.nb_failed > failed0 || es.nb_empty > empty0) ? 1 : 0;
+            if (es.nb_failed > failed0) {
+                rej_failed++;
+                return 1;
+            }
+            if (rc > 0 || es.nb_empty > empty0) {
+                rej_empty++;
+                return 1;
+            }
+            return 0;
         };
         // mass statistics: [0] max |m-nu|/nu, [1] |m-nu|_2^2, [2] min m
         auto masses = [&](double out[3]) {
@@ -128,6 +138,9 @@ public:
                 if (rc < 0) return -1;
                 double m2[3] = {0, 0, 0};
                 if (rc == 0) masses(m2);
+                if (rc == 0 && !(m2[2] >= eps0 &&
+                                 std::sqrt(m2[1]) <= (1.0 - 0.5 * t) * std::sqrt(g2)))
+                    rej_test++;
                 if (rc == 0 && m2[2] >= eps0 &&
                     std::sqrt(m2[1]) <= (1.0 - 0.5 * t) * std::sqrt(g2)) {
                     err = m2[0];
@@ -137,6 +150,15 @@ public:
                 }
                 t *= 0.5;
                 st.halvings++;
+            }
+            if (verbose) {
+                std::fprintf(stderr,
+                             "[vgc ot] newton %d: err %.3e step %.4g cg %d cells %.0f ms cg %.0f ms"
+                             " rejected trials: %lld failed cells (%llu over capacity), %lld empty,"
+                             " %lld mass test\n",
+                             st.newton_iters + 1, ok ? err : -1.0, ok ? t : 0.0, it, st.t_cells_ms,
+                             st.t_cg_ms, rej_failed, es.nb_fail_capacity, rej_empty, rej_test);
+                std::fflush(stderr);
             }
             if (!ok) break;
             be_.d2d(d_w, d_wt, sizeof(double) * n);

@@ -46,6 +46,7 @@ struct EngineStats {
     // integration tetrahedra, facets)
     double t_clip_a_gpu_ms = 0, t_knn_gpu_ms = 0;
     unsigned long long a_live = 0, a_new = 0, a_tets = 0, a_facets = 0, a_tests = 0, a_cells = 0;
+    unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
 };
 
 struct Problem {
@@ -92,7 +93,7 @@ public:
     int* d_inst_of = nullptr;
     bnode* d_nodes = nullptr;
 
-    static constexpr int kFaceStride = 136;  // >= 2 + 256/2 faces
+    static constexpr int kFaceStride = 256;  // faces of a cell <= 256 planes
 
     // ------------------------------------------------------------ build
     int build(const Problem& pb, EngineStats& st) {
@@ -338,6 +339,7 @@ public:
         for (int c = 0; c < n_; ++c) {
             uint32_t f = h_flags[c];
             if (!(f & CELL_DONE)) st.nb_failed++;
+            if (!(f & CELL_DONE) && (f & CELL_OVERFLOW)) st.nb_fail_capacity++;
             if (f & CELL_UNCERTAIN) st.nb_uncertain++;
             if (f & CELL_EMPTY) st.nb_empty++;
             if (f & CELL_DISCONNECTED) st.nb_disconnected++;
@@ -421,6 +423,13 @@ private:
             if (!over4.empty()) {
                 if (radius_pass(4, exact, over4, P, st)) return -1;
                 be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+                std::vector<int> over8;
+                for (int c : over4)
+                    if (h_flags[c] & CELL_OVERFLOW) over8.push_back(c);
+                if (!over8.empty()) {
+                    if (radius_pass(8, exact, over8, P, st)) return -1;
+                    be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+                }
             }
         }
         return 0;

@@ -118,7 +118,7 @@ def generate(ctx, seeds, sizes, tol=1e-5, log=print):
             import torch
             xd = torch.from_numpy(x).cuda()
             nud = torch.from_numpy(nu).cuda()
-            w, st = ctx.ot_solve(xd, nud, tol=tol)
+            w, st = ctx.ot_solve(xd, nud, tol=tol, verbose=True)
             f = ctx.ot_features(xd, nud)
             t = (w - w.mean()).to(torch.float32)
         else:

@@ -36,6 +36,8 @@ extern "C" {
                                 (global atomics: slows the first pass down) */
 #define VGC_OT_JUDGE_START 8u /* vgc_ot_solve: also measure the zero start and keep
                                  the start with the smaller mass residual */
+#define VGC_OT_VERBOSE 16u    /* vgc_ot_solve: one progress line per Newton step
+                                 on stderr */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -59,6 +61,8 @@ typedef struct vgc_stats {
        integration tetrahedra and facets */
     double t_clip_a_gpu_ms, t_knn_gpu_ms;
     uint64_t a_cells, a_tests, a_live, a_new, a_tets, a_facets;
+    uint64_t nb_fail_capacity; /* failed cells that exceed 256 corners / planes or
+                                  the facet staging stride                          */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in

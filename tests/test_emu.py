@@ -238,3 +238,25 @@ def test_transport_weights_power_filter(emu_ctx):
         emu_ctx.lib.cdll.vgc_emu_set_list_chunk(emu_ctx.ctx, ctypes.c_longlong(1 << 27))
     assert np.array_equal(r2.vol, r.vol) and np.array_equal(r2.mg, r.mg)
     assert np.array_equal(r2.nbr, r.nbr) and np.array_equal(r2.farea, r.farea)
+
+
+def test_cells_beyond_256_corners(emu_ctx):
+    # a heavy seed in a uniform cloud: a ball-like cell with 185 facets and
+    # ~370 corners runs in the 512-corner tier; 295 facets exceed the 256
+    # plane slots and are reported as a capacity failure
+    x = ga.sample_points(0, 41, 20000)
+    x[0] = [0.5, 0.5, 0.5]
+    w = np.zeros(len(x))
+    w[0] = 0.012
+    r = emu_ctx.cells(x, weights=w)
+    assert r.stats["nb_failed"] == 0 and len(r.neighbors(0)) == 185
+    o = oracle.cells(x, w=w, centred=True, stride=512)
+    assert np.all(np.abs(r.vol - o["vol"]) <= 1e-10 * o["vol"])
+    assert set(r.neighbors(0).tolist()) == set(o["nbr"][0, :o["cnt"][0]].tolist())
+    da = dict(zip(r.neighbors(0).tolist(), r.facet_areas(0).tolist()))
+    for j, a in zip(o["nbr"][0, :o["cnt"][0]].tolist(), o["farea"][0, :o["cnt"][0]].tolist()):
+        assert abs(da[j] - a) <= 1e-10 * a + 1e-14
+    w[0] = 0.02
+    r = emu_ctx.cells(x, weights=w)
+    assert r.status == ga.VGC_PARTIAL
+    assert r.stats["nb_failed"] == 1 and r.stats["nb_fail_capacity"] == 1

@@ -113,6 +113,18 @@ def test_gpu_matches_emulation_transport_weights(gpu_ctx, emu_ctx):
     _same(rg, emu_ctx.cells(x, weights=w, k0=16))
 
 
+def test_gpu_matches_emulation_512_corner_tier(gpu_ctx, emu_ctx):
+    import geogram_amd as ga
+
+    x = ga.sample_points(0, 41, 20000)
+    x[0] = [0.5, 0.5, 0.5]
+    w = np.zeros(len(x))
+    w[0] = 0.012  # 185 facets: beyond 256 corners
+    rg = gpu_ctx.cells(x, weights=w)
+    assert rg.stats["nb_failed"] == 0 and len(rg.neighbors(0)) == 185
+    _same(rg, emu_ctx.cells(x, weights=w))
+
+
 def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
     import geogram_amd as ga
 
