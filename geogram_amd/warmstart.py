@@ -5,8 +5,8 @@ iterations of ``OptimalTransportMap3d.optimize``).
 A small MLP (6 -> 64 -> 64 -> 1, leaky ReLU, FP32; kernels in csrc/mlp.h)
 maps a seed's features -- position in the box, log of its target mass over
 the mean mass, log of the mean and max distance to its 16 nearest seeds over
-the mean spacing h = (V/n)^(1/3) -- to its Laguerre weight w_i - mean(w), in
-units of the squared box diagonal scale.  The weight field of a converged
+the mean spacing h = (V/n)^(1/3) -- to its Laguerre weight w_i - mean(w)
+(box units).  The weight field of a converged
 transport plan is mostly a smooth potential that does not depend on n; the
 part that follows mass fluctuations of single seeds scales with h^2 and is
 left to Newton.
@@ -18,7 +18,9 @@ fallback: a predicted start that leaves a cell empty, or whose mass residual
 is worse than the zero start's, is dropped (``vgc_ot_solve`` with
 ``VGC_OT_JUDGE_START``).
 
-    python -m geogram_amd.warmstart --instances 200 --out geogram_amd/warmstart_params.npy
+    python -m geogram_amd.warmstart generate --first 0 --count 100 --out part0.npz
+    python -m geogram_amd.warmstart generate --first 100 --count 108 --out part1.npz
+    python -m geogram_amd.warmstart train --data part0.npz part1.npz
 """
 from __future__ import annotations
 
@@ -214,19 +216,55 @@ def validate(ctx, params, f, t):
     return float(np.linalg.norm(y - t) / np.linalg.norm(t))
 
 
+def instance_list(count, seed=1000, nmin=50_000, nmax=200_000):
+    """The fixed list of training instances: seeds seed + i, sizes log-uniform
+    in [nmin, nmax] drawn from rng(seed)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), count)).astype(np.int64)
+    return seed + np.arange(count), sizes
+
+
+def generate_rows(ctx, seeds, sizes, rows, tol=1e-4, log=print):
+    """Solve each instance, keep `rows` random (features, target) rows of it
+    (all rows if rows <= 0).  Returns numpy arrays and per-instance stats."""
+    F, T, I, stats = [], [], [], []
+    for k, (s, n) in enumerate(zip(seeds, sizes)):
+        f, t = generate(ctx, [s], [n], tol, log)
+        if not f:
+            continue
+        if _is_gpu_ctx(ctx):
+            f, t = f[0].cpu().numpy(), t[0].cpu().numpy()
+        else:
+            f, t = f[0], t[0]
+        if 0 < rows < len(t):
+            pick = np.sort(np.random.default_rng(int(s)).choice(len(t), rows, replace=False))
+            f, t = f[pick], t[pick]
+        F.append(f)
+        T.append(t)
+        I.append(np.full(len(t), int(s), dtype=np.int64))
+        stats.append((int(s), int(n)))
+    return (np.concatenate(F).astype(np.float32), np.concatenate(T).astype(np.float32),
+            np.concatenate(I), np.array(stats, dtype=np.int64).reshape(-1, 2))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--instances", type=int, default=200)
+    ap.add_argument("mode", choices=["generate", "train", "all"])
+    ap.add_argument("--instances", type=int, default=200, help="training instances")
+    ap.add_argument("--val", type=int, default=8, help="held-out instances after them")
+    ap.add_argument("--first", type=int, default=0, help="generate: first instance index")
+    ap.add_argument("--count", type=int, default=None, help="generate: number of instances")
+    ap.add_argument("--rows", type=int, default=8000, help="rows kept per instance (0: all)")
     ap.add_argument("--nmin", type=int, default=50_000)
     ap.add_argument("--nmax", type=int, default=200_000)
-    ap.add_argument("--val", type=int, default=8, help="held-out instances")
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--data", nargs="*", default=[], help="train: .npz files from generate")
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--tol", type=float, default=1e-5)
-    ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--emulated", action="store_true", help="CPU emulation (small runs only)")
-    ap.add_argument("--out", default=PARAMS_FILE)
+    ap.add_argument("--out", default=PARAMS_FILE, help="generate: .npz; train/all: params .npy")
     ap.add_argument("--report", default=None, help="JSON training report")
     a = ap.parse_args(argv)
     from . import Context
@@ -235,25 +273,49 @@ def main(argv=None):
         print(msg, flush=True)
 
     ctx = Context.emulated() if a.emulated else Context(0)
-    rng = np.random.default_rng(a.seed)
-    total = a.instances + a.val
-    sizes = np.exp(rng.uniform(math.log(a.nmin), math.log(a.nmax), total)).astype(np.int64)
-    seeds = a.seed + np.arange(total)
-    t0 = time.time()
-    feats, targets = generate(ctx, seeds, sizes, a.tol, log)
-    t_gen = time.time() - t0
-    nv = min(a.val, max(0, len(feats) - 1))
-    tr_f, tr_t = feats[:len(feats) - nv], targets[:len(feats) - nv]
-    va_f, va_t = feats[len(feats) - nv:], targets[len(feats) - nv:]
+    seeds, sizes = instance_list(a.instances + a.val, a.seed, a.nmin, a.nmax)
+    if a.mode in ("generate", "all"):
+        lo = a.first if a.mode == "generate" else 0
+        hi = len(seeds) if a.count is None or a.mode == "all" else min(len(seeds), lo + a.count)
+        t0 = time.time()
+        parts = []
+        for b in range(lo, hi, 10):  # saved every 10 instances
+            e = min(hi, b + 10)
+            parts.append(generate_rows(ctx, seeds[b:e], sizes[b:e], a.rows, a.tol, log))
+            F, T, I, st = (np.concatenate([p[k] for p in parts]) for k in range(4))
+            if a.mode == "generate":
+                np.savez(a.out, feats=F, targets=T, inst=I, stats=st)
+            log(f"generated {len(st)} instances, {len(T)} rows in {time.time() - t0:.1f}s")
+        if a.mode == "generate":
+            return 0
+        datasets = [(F, T, I)]
+    else:
+        datasets = []
+        for path in a.data:
+            z = np.load(path, allow_pickle=False)
+            datasets.append((z["feats"], z["targets"], z["inst"]))
+    F = np.concatenate([d[0] for d in datasets])
+    T = np.concatenate([d[1] for d in datasets])
+    I = np.concatenate([d[2] for d in datasets])
+    val_seeds = set(int(s) for s in seeds[a.instances:])
+    is_val = np.isin(I, list(val_seeds))
+    tr_f, tr_t = F[~is_val], T[~is_val]
+    va = [(F[I == s], T[I == s]) for s in sorted(val_seeds) if np.any(I == s)]
+    if not a.emulated:
+        import torch
+        tr_f, tr_t = [torch.from_numpy(tr_f).cuda()], [torch.from_numpy(tr_t).cuda()]
+        va = [(torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda()) for f, t in va]
+    else:
+        tr_f, tr_t = [tr_f], [tr_t]
     t1 = time.time()
-    params, rep = train(ctx, tr_f, tr_t, a.steps, a.batch, a.lr, a.seed, log, va_f, va_t)
-    t_train = time.time() - t1
+    params, rep = train(ctx, tr_f, tr_t, a.steps, a.batch, a.lr, a.seed, log,
+                        [f for f, _ in va], [t for _, t in va])
+    rep.update({"train_instances": int(len(set(I[~is_val].tolist()))),
+                "train_rows": int((~is_val).sum()), "val_instances": len(va),
+                "t_train_s": time.time() - t1, "steps": a.steps, "batch": a.batch,
+                "lr": a.lr, "tol": a.tol, "seed": a.seed, "rows_per_instance": a.rows})
     np.save(a.out, params, allow_pickle=False)
-    rep.update({"instances": len(tr_f), "rows": int(sum(f.shape[0] for f in tr_f)),
-                "val_instances": nv, "t_generate_s": t_gen, "t_train_s": t_train,
-                "steps": a.steps, "batch": a.batch, "lr": a.lr, "tol": a.tol,
-                "sizes": [int(s) for s in sizes], "seed": a.seed})
-    log(json.dumps({k: v for k, v in rep.items() if k != "sizes"}))
+    log(json.dumps(rep))
     if a.report:
         with open(a.report, "w") as fh:
             json.dump(rep, fh, indent=1)
