@@ -343,6 +343,45 @@ struct EmuBackend {
     void ot_pcg_dir(int n, double beta, const double* z, double* p) {
         for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     }
+    // device-resident PCG of the gfx950 backend, same operations and orders
+    void cg_start(int n, const double* b, const double* diag, double* x, double* r, double* z,
+                  double* p, double* cg, double tol2) {
+        ot_pcg_init(n, b, diag, x, r, z, p);
+        const double rz = red_sum(n, [&](int i) { return r[i] * z[i]; });
+        const double bb = red_sum(n, [&](int i) { return r[i] * r[i]; });
+        cg[0] = rz;
+        cg[1] = bb;
+        cg[5] = tol2 * bb;
+        cg[6] = bb > 0.0 ? 0.0 : 1.0;
+        cg[7] = 0.0;
+    }
+    void cg_iter(int n, const long long* ptr, const int* col, const double* val,
+                 const double* diag, double* x, double* r, double* z, double* p, double* q,
+                 double* cg) {
+        if (cg[6] != 0.0) return;
+        ot_spmv(n, ptr, col, val, diag, p, q);
+        const double pq = red_sum(n, [&](int i) { return p[i] * q[i]; });
+        cg[2] = pq;
+        if (!(pq > 0.0)) {
+            cg[3] = 0.0;
+            cg[6] = 2.0;
+            return;
+        }
+        cg[3] = cg[0] / pq;
+        ot_pcg_update(n, cg[3], p, q, diag, x, r, z);
+        const double rz = red_sum(n, [&](int i) { return r[i] * z[i]; });
+        const double rr = red_sum(n, [&](int i) { return r[i] * r[i]; });
+        cg[1] = rr;
+        cg[7] += 1.0;
+        if (rr <= cg[5]) {
+            cg[4] = 0.0;
+            cg[6] = 1.0;
+            return;
+        }
+        cg[4] = rz / cg[0];
+        cg[0] = rz;
+        ot_pcg_dir(n, cg[4], z, p);
+    }
 
     // ---------------------------------------------------------------- MLP
     static constexpr int kRowsFwd = 64, kRowsGrad = 32;

@@ -447,6 +447,163 @@ __global__ void k_ot_features(int n, const double* __restrict__ xyz, const doubl
     if (i < n) ot_features_row(i, xyz, nu, d2, box, inv_cell, h, f + (long long)i * kMlpIn);
 }
 
+
+// Device-resident Jacobi-PCG: the scalars live in cg[8] = {rz, rr, pq, alpha,
+// beta, threshold, done, iterations}; no host round trip per iteration.  The
+// reductions follow the kRedBlocks x 256 order above, so the iterates equal
+// those of a host-driven loop.  done: 1 converged (or breakdown); every
+// kernel is a no-op afterwards, so extra launched iterations change nothing.
+__device__ __forceinline__ void block_sum2(double a, double b, double* sh, double* sh2, double& ta,
+                                           double& tb) {
+    sh[threadIdx.x] = a;
+    sh2[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+            sh2[threadIdx.x] += sh2[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    ta = sh[0];
+    tb = sh2[0];
+}
+
+__global__ __launch_bounds__(256) void k_cg_start_partial(int n, const double* __restrict__ b,
+                                                          const double* __restrict__ diag,
+                                                          double* x, double* r, double* z,
+                                                          double* p, double* part) {
+    __shared__ double sh[256], sh2[256];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
+        const double ri = b[i], zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        s1 += ri * zi;
+        s2 += ri * ri;
+    }
+    double t1, t2;
+    block_sum2(s1, s2, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t1;
+        part[kRedBlocks + blockIdx.x] = t2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_start_final(const double* __restrict__ part, double* cg,
+                                                        double tol2) {
+    __shared__ double sh[256], sh2[256];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
+        s1 += part[i];
+        s2 += part[kRedBlocks + i];
+    }
+    double t1, t2;
+    block_sum2(s1, s2, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        cg[0] = t1;
+        cg[1] = t2;
+        cg[5] = tol2 * t2;
+        cg[6] = t2 > 0.0 ? 0.0 : 1.0;
+        cg[7] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_spmv_dot(int n, const long long* __restrict__ ptr,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ diag,
+                                                     const double* __restrict__ p, double* q,
+                                                     const double* __restrict__ cg, double* part) {
+    __shared__ double sh[256];
+    if (cg[6] != 0.0) return;  // uniform over the grid
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
+        const double qi = ot_spmv_row(i, ptr, col, val, diag, p);
+        q[i] = qi;
+        s += p[i] * qi;
+    }
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ part, double* cg) {
+    __shared__ double sh[256];
+    if (cg[6] != 0.0) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) s += part[i];
+    double pq = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        cg[2] = pq;
+        if (!(pq > 0.0)) {
+            cg[3] = 0.0;
+            cg[6] = 2.0;  // breakdown: stop, the iteration does not count
+        } else {
+            cg[3] = cg[0] / pq;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_update_dots(int n, const double* __restrict__ p,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ diag, double* x,
+                                                        double* r, double* z,
+                                                        const double* __restrict__ cg,
+                                                        double* part) {
+    __shared__ double sh[256], sh2[256];
+    if (cg[6] != 0.0) return;
+    const double alpha = cg[3];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+        z[i] = zi;
+        s1 += ri * zi;
+        s2 += ri * ri;
+    }
+    double t1, t2;
+    block_sum2(s1, s2, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t1;
+        part[kRedBlocks + blockIdx.x] = t2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ part, double* cg) {
+    __shared__ double sh[256], sh2[256];
+    if (cg[6] != 0.0) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
+        s1 += part[i];
+        s2 += part[kRedBlocks + i];
+    }
+    double rz, rr;
+    block_sum2(s1, s2, sh, sh2, rz, rr);
+    if (threadIdx.x == 0) {
+        cg[1] = rr;
+        cg[7] += 1.0;
+        if (rr <= cg[5]) {
+            cg[4] = 0.0;
+            cg[6] = 1.0;
+        } else {
+            cg[4] = rz / cg[0];
+            cg[0] = rz;
+        }
+    }
+}
+
+__global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
+                         const double* __restrict__ cg) {
+    if (cg[6] != 0.0) return;
+    const double beta = cg[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
 // ----------------------------------------------------------------- backend
@@ -711,6 +868,24 @@ struct HipBackend {
     }
     void ot_pcg_dir(int n, double beta, const double* z, double* p) {
         k_ot_pcg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, beta, z, p);
+        check();
+    }
+    // device-resident PCG (scalars in cg[8])
+    void cg_start(int n, const double* b, const double* diag, double* x, double* r, double* z,
+                  double* p, double* cg, double tol2) {
+        k_cg_start_partial<<<kRedBlocks, 256, 0, stream_>>>(n, b, diag, x, r, z, p, red_part());
+        k_cg_start_final<<<1, 256, 0, stream_>>>(red_part(), cg, tol2);
+        check();
+    }
+    void cg_iter(int n, const long long* ptr, const int* col, const double* val,
+                 const double* diag, double* x, double* r, double* z, double* p, double* q,
+                 double* cg) {
+        double* part = red_part();
+        k_cg_spmv_dot<<<kRedBlocks, 256, 0, stream_>>>(n, ptr, col, val, diag, p, q, cg, part);
+        k_cg_alpha<<<1, 256, 0, stream_>>>(part, cg);
+        k_cg_update_dots<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part);
+        k_cg_beta<<<1, 256, 0, stream_>>>(part, cg);
+        k_cg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, z, p, cg);
         check();
     }
 

@@ -61,6 +61,7 @@ public:
         double* d_diag = (double*)slot(6, sizeof(double) * n);
         double* d_wt = (double*)slot(7, sizeof(double) * n);
         double* d_scal = (double*)slot(8, sizeof(double) * 8);
+        double* d_cg = (double*)slot(10, sizeof(double) * 8);
         EngineStats es;
 
         // 0: cells of w computed; 1: some cell empty or failed (the masses
@@ -123,7 +124,7 @@ public:
             double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
             be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
             be_.ot_rhs(n, d_nu, E_.d_vol, d_b);
-            int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_scal,
+            int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_cg,
                          cg_tol, cg_max);
             st.cg_iters += it;
             st.t_cg_ms += ms_since(tg);
@@ -172,40 +173,24 @@ public:
     }
 
 private:
-    // Jacobi-preconditioned CG from x = 0; returns the iterations done.
+    // Jacobi-preconditioned CG from x = 0; returns the iterations done.  The
+    // iterations run on the device (scalars in device memory); the host reads
+    // the state every kCgCheck launches.
+    static constexpr int kCgCheck = 16;
     int pcg(int n, const long long* ptr, const int* col, const double* val, const double* diag,
             const double* b, double* x, double* r, double* z, double* p, double* q, double* scal,
             double tol, int maxit) {
-        be_.ot_pcg_init(n, b, diag, x, r, z, p);
-        double h[2];
-        be_.ot_dot(n, r, z, scal + 0);
-        be_.ot_dot(n, b, b, scal + 1);
-        be_.d2h(h, scal, sizeof(double) * 2);
-        double rz = h[0];
-        const double bb = h[1];
-        if (!(bb > 0.0)) return 0;
-        int it = 0;
-        for (; it < maxit; ++it) {
-            be_.ot_spmv(n, ptr, col, val, diag, p, q);
-            be_.ot_dot(n, p, q, scal + 2);
-            double pq;
-            be_.d2h(&pq, scal + 2, sizeof(double));
-            if (!(pq > 0.0)) break;
-            const double alpha = rz / pq;
-            be_.ot_pcg_update(n, alpha, p, q, diag, x, r, z);
-            be_.ot_dot(n, r, z, scal + 0);
-            be_.ot_dot(n, r, r, scal + 1);
-            be_.d2h(h, scal, sizeof(double) * 2);
-            const double rz_new = h[0], rr = h[1];
-            if (rr <= tol * tol * bb) {
-                ++it;
-                break;
-            }
-            const double beta = rz_new / rz;
-            rz = rz_new;
-            be_.ot_pcg_dir(n, beta, z, p);
+        be_.cg_start(n, b, diag, x, r, z, p, scal, tol * tol);
+        int issued = 0;
+        double h[8];
+        be_.d2h(h, scal, sizeof(double) * 8);
+        while (h[6] == 0.0 && issued < maxit) {
+            const int k = std::min(kCgCheck, maxit - issued);
+            for (int i = 0; i < k; ++i) be_.cg_iter(n, ptr, col, val, diag, x, r, z, p, q, scal);
+            issued += k;
+            be_.d2h(h, scal, sizeof(double) * 8);
         }
-        return it;
+        return (int)h[7];
     }
 
     int fail(const char* msg) {
