@@ -84,9 +84,14 @@ def predict_weights(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), params=None):
     feats = ctx.ot_features(points, nu, box)
     if _is_torch_cuda(points):
         import torch
-        p = torch.from_numpy(np.asarray(params, dtype=np.float32)).to(points.device)
+        if isinstance(params, torch.Tensor):
+            p = params.to(device=points.device, dtype=torch.float32).contiguous()
+        else:
+            p = torch.from_numpy(np.asarray(params, dtype=np.float32)).to(points.device)
         return ctx.mlp_forward(p, feats).to(torch.float64)
-    return ctx.mlp_forward(params, feats).astype(np.float64)
+    if not isinstance(params, np.ndarray):
+        params = params.cpu().numpy()
+    return ctx.mlp_forward(np.asarray(params, dtype=np.float32), feats).astype(np.float64)
 
 
 def _is_torch_cuda(a):
