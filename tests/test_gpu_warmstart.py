@@ -86,3 +86,33 @@ def test_gpu_training_matches_emulation(gpu_ctx, emu_ctx):
     assert np.allclose(ld, le, rtol=1e-4)
     assert le[-1] < 0.5 * le[0]
     assert np.allclose(pd.cpu().numpy(), pe, rtol=1e-4, atol=1e-6)
+
+
+def test_learned_start_against_plain(gpu_ctx):
+    """What the trained network gains over the plain start on a held-out
+    200k-seed 16-Gaussian instance (log-normal masses) solved to 1e-4.  The
+    judged start may fall back to w = 0; it must then cost at most the one
+    extra cell pass of the judgement, and never more Newton steps."""
+    import time
+    import torch
+
+    x, nu = warmstart.make_instance(4242, 200_000)
+    xd, nud = torch.from_numpy(x).cuda(), torch.from_numpy(nu).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, sp = gpu_ctx.ot_solve(xd, nud, tol=1e-4)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    params = torch.from_numpy(warmstart.load_params()).cuda()
+    t0 = time.perf_counter()
+    w0 = warmstart.predict_weights(gpu_ctx, xd, nud, params=params)
+    _, sl = gpu_ctx.ot_solve(xd, nud, w0=w0, tol=1e-4, judge_start=True)
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    print(f"plain: {sp['newton_iters']} Newton, {tp:.2f} s; learned: {sl['newton_iters']} Newton, "
+          f"{tl:.2f} s, fell back to w = 0: {bool(sl['used_fallback_start'])}")
+    assert sp["converged"] == 1 and sl["converged"] == 1
+    assert sl["newton_iters"] <= sp["newton_iters"]
+    if sl["used_fallback_start"]:
+        assert sl["newton_iters"] == sp["newton_iters"]
+        assert sl["cell_passes"] <= sp["cell_passes"] + 2
