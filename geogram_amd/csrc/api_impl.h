@@ -74,6 +74,9 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->a_tets = e.a_tets;
     s->a_facets = e.a_facets;
     s->nb_fail_capacity = e.nb_fail_capacity;
+    s->why_rings = e.why_rings;
+    s->why_walk = e.why_walk;
+    s->why_narrow = e.why_narrow;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }

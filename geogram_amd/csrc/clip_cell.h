@@ -47,6 +47,10 @@ enum CellFlag : uint32_t {
     CELL_EMPTY = 16u,        // empty Laguerre cell
     CELL_DISCONNECTED = 32u, // coincident seed with a smaller index
     CELL_BAD_TOPO = 64u,     // conflict zone was not a disk (only after uncertain signs)
+    // why a gathering cell went to the list tiers (statistics only)
+    CELL_WHY_RINGS = 128u,   // filtered-ring limit
+    CELL_WHY_WALK = 256u,    // a tree walk outgrew the wave's buffers
+    CELL_WHY_NARROW = 512u,  // a ring could not be narrowed further
 };
 
 // plane slots per cell (8-bit ids): 128 with 64 or 128 corner slots, 256 with
@@ -66,9 +70,9 @@ struct CornerBits {
 };
 
 constexpr int kCandCap = 128;   // candidates per distance ring
-constexpr int kFrontCap = 128;  // tree nodes per level of the walk
+constexpr int kFrontCap = 320;  // tree nodes per level of the walk (two levels: the depth-first stack)
 constexpr int kBucketCap = 128; // buckets per ring
-constexpr int kMaxFilteredRings = 16;  // power-filtered rings before the list tiers
+constexpr int kMaxFilteredRings = 48;  // power-filtered rings before the list tiers
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -754,7 +758,7 @@ VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
 }
 
 // Seeds q != c of the instance with rlo2 < d2 <= rhi2 (FP32 squared
-// distance from the seed) that pass the power filter F, into L.cand as
+// distance from the seed) that pass the power filter F (if on), into L.cand as
 // (d2 bits << 32 | q), unsorted.  Depth-first walk of the tree: a stack in
 // the wave's LDS, up to 64 nodes popped per step (one per lane), passing
 // leaves scanned at once, so the stack stays short however many nodes pass.
@@ -793,13 +797,25 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
         LF cy = vmaxf(vmaxf(B.ly - F.cy, F.cy - B.hy), 0.0f);
         LF cz = vmaxf(vmaxf(B.lz - F.cz, F.cz - B.hz), 0.0f);
         LF dc2 = cx * cx + cy * cy + cz * cz;
-        LB hit = in & (dmin <= lim) & (dmax * 1.00001f > rlo2) &
-                 pf_node_may_cut(F, dc2, B.wm, vsqrtf(dmin), vsqrtf(dmax), fx, fy, fz);
+        LB hit = in & (dmin <= lim) & (dmax * 1.00001f > rlo2);
+        if (F.on) hit = hit & pf_node_may_cut(F, dc2, B.wm, vsqrtf(dmin), vsqrtf(dmax), fx, fy, fz);
         LB internal = node < Lv;
         LB push = hit & internal;
-        LB leaf = hit & !internal;
-        uint64_t mp = ballot(push), ml = ballot(leaf);
-        if (top + 2 * popc64(mp) > kStack) return -1;
+        uint64_t mp = ballot(push);
+        // no room for every child: drop the farthest subtrees, and bound the
+        // ring below the nearest distance of what was dropped
+        while (top + 2 * popc64(mp) > kStack) {
+            const float far = wave_maxf(select(push, dmin, LF(-1.0f)));
+            LB drop = push & (dmin >= far);
+            push = push & !drop;
+            hit = hit & !drop;
+            mp = ballot(push);
+            const float nl = nextafterf(far, 0.0f);
+            if (!(nl > rlo2)) return -1;
+            lim = nl < lim ? nl : lim;
+        }
+        LB leaf = hit & !internal & (dmin <= lim);
+        uint64_t ml = ballot(leaf);
         LI rp = top + 2 * mbcnt(mp);
         lds_st(stk, select(push, rp, LI(0)), 2 * node, push);
         lds_st(stk, select(push, rp + 1, LI(0)), 2 * node + 1, push);
@@ -819,8 +835,8 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
             LF d2 = ex * ex + ey * ey + ez * ez;
             LF gx = q.x - F.cx, gy = q.y - F.cy, gz = q.z - F.cz;
             LF gc2 = gx * gx + gy * gy + gz * gz;
-            LB take = ok & (d2 > rlo2) & (d2 <= lim) &
-                      pf_may_cut(F, gc2, q.w, vsqrtf(d2), vabsf(ex), vabsf(ey), vabsf(ez));
+            LB take = ok & (d2 > rlo2) & (d2 <= lim);
+            if (F.on) take = take & pf_may_cut(F, gc2, q.w, vsqrtf(d2), vabsf(ex), vabsf(ey), vabsf(ez));
             if (cnt + popc64(ballot(take)) > kCandCap) {
                 // keep the seeds strictly nearer than the 64th nearest so far
                 sort_cand<S>(L, cnt);
@@ -844,6 +860,17 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
             cnt += popc64(mt);
         }
         wave_sync();
+    }
+    if (cnt > 0 && lim < rhi2) {
+        // candidates taken before the bound came down
+        sort_cand<S>(L, cnt);
+        const uint64_t kmax = uint64_t(__builtin_bit_cast(uint32_t, lim)) << 32 | 0xFFFFFFFFull;
+        int kept = 0;
+        for (int i0 = 0; i0 < cnt; i0 += kWave) {
+            LI i = i0 + lane;
+            kept += popc64(ballot((i < cnt) & (lds_ld(L.cand, select(i < cnt, i, LI(0))) <= LU64(kmax))));
+        }
+        cnt = kept;
     }
     rhi2 = lim;
     return cnt;
@@ -999,15 +1026,18 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         PowerFilter F{};
         int nfilt = 0;
         while (true) {
-            int cnt = (W && F.on)
+            // first ring: level-synchronous walk; later rings: depth first,
+            // bounded to the nearest seeds when the buffer fills
+            int cnt = rlo2 >= 0.0f
                           ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
             if (W && F.on && ++nfilt > kMaxFilteredRings) {
                 // the list tiers take the cell, with the same filter and no
                 // capacity limit
-                R.flags |= CELL_NEED_K;
+                R.flags |= CELL_NEED_K | CELL_WHY_RINGS;
                 break;
             }
+            if (cnt < 0) R.flags |= CELL_WHY_WALK;
             if (cnt < 0 || cnt > kCandCap) {
                 // too many seeds (or tree nodes) in the ring: narrow it, same
                 // shell-volume fraction in r^3, and gather again
@@ -1018,7 +1048,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 float nh = cbrtf(r3);
                 nh = nh * nh;
                 if (!(nh > lo * 1.000001f) || !(nh < rhi2)) {
-                    R.flags |= CELL_NEED_K;  // seeds at one distance: list tiers
+                    R.flags |= CELL_NEED_K | CELL_WHY_NARROW;  // seeds at one distance
                     break;
                 }
                 rhi2 = nh;

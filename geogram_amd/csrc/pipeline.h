@@ -47,6 +47,7 @@ struct EngineStats {
     double t_clip_a_gpu_ms = 0, t_knn_gpu_ms = 0;
     unsigned long long a_live = 0, a_new = 0, a_tets = 0, a_facets = 0, a_tests = 0, a_cells = 0;
     unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
+    unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
 };
 
 struct Problem {
@@ -297,6 +298,11 @@ public:
             else if (f & CELL_NEED_K)
                 need.push_back(c);
             if (f & CELL_EMPTY) ++nempty;
+            if (f & CELL_NEED_K) {
+                st.why_rings += (f & CELL_WHY_RINGS) != 0;
+                st.why_walk += (f & CELL_WHY_WALK) != 0;
+                st.why_narrow += (f & CELL_WHY_NARROW) != 0;
+            }
         }
         if (pb.abort_on_empty && nempty > 0) {
             st.nb_empty += nempty;

@@ -307,6 +307,11 @@ inline double wave_max(const LD& a) {
     for (int m = 32; m >= 1; m >>= 1) x = vmax(x, shfl_xor(x, m));
     return x.v[0];
 }
+inline float wave_maxf(const LF& a) {
+    LF x = a;
+    for (int m = 32; m >= 1; m >>= 1) x = vmaxf(x, shfl_xor(x, m));
+    return x.v[0];
+}
 
 // LDS access: gather, masked scatter (lane order: higher lane wins on a
 // collision, like the hardware's unspecified order; callers never rely on it)
@@ -427,6 +432,12 @@ VGC_HD double wave_max(double a) {
     double x = a;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) x = __builtin_fmax(x, __shfl_xor(x, m));
+    return x;
+}
+VGC_HD float wave_maxf(float a) {
+    float x = a;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = __builtin_fmaxf(x, __shfl_xor(x, m));
     return x;
 }
 

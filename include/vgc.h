@@ -63,6 +63,10 @@ typedef struct vgc_stats {
     uint64_t a_cells, a_tests, a_live, a_new, a_tets, a_facets;
     uint64_t nb_fail_capacity; /* failed cells that exceed 256 corners / planes or
                                   the facet staging stride                          */
+    /* first pass cells handed to the list tiers, by cause: ring limit of the
+       weighted search, a tree walk beyond the wave's buffers, a ring that
+       could not be narrowed */
+    uint64_t why_rings, why_walk, why_narrow;
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in
