@@ -260,7 +260,7 @@ struct EmuBackend {
     // ----------------------------------------------------------------- OT
     // reductions in the order of the device kernels (512 blocks x 256
     // threads, grid-stride, halving-stride block trees)
-    static constexpr int kRedBlocks = 512;
+    static constexpr int kRedBlocks = 2048;
     static double tree_sum(double* v) {
         for (int off = 128; off > 0; off >>= 1)
             for (int t = 0; t < off; ++t) v[t] += v[t + off];
@@ -354,6 +354,11 @@ struct EmuBackend {
         cg[5] = tol2 * bb;
         cg[6] = bb > 0.0 ? 0.0 : 1.0;
         cg[7] = 0.0;
+    }
+    void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
+                  const double* diag, double* x, double* r, double* z, double* p, double* q,
+                  double* cg) {
+        for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
     }
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
                  const double* diag, double* x, double* r, double* z, double* p, double* q,

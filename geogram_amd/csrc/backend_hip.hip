@@ -227,7 +227,7 @@ __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restr
 // Deterministic reductions: kRedBlocks blocks of 256 threads; thread t of
 // block b accumulates elements b*256+t + k*kRedBlocks*256 in order, the block
 // reduces by halving strides, and one block reduces the partials the same way.
-constexpr int kRedBlocks = 512;
+constexpr int kRedBlocks = 2048;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -621,6 +621,7 @@ struct HipBackend {
                 if (x) (void)hipEventDestroy(x);
         if (temp_) (void)hipFree(temp_);
         if (red_) (void)hipFree(red_);
+        if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     hipStream_t stream() const { return stream_; }
@@ -877,6 +878,29 @@ struct HipBackend {
         k_cg_start_final<<<1, 256, 0, stream_>>>(red_part(), cg, tol2);
         check();
     }
+    // k iterations as one hipGraph, captured once per argument set and
+    // replayed until the solve ends (the kernels read their scalars from cg)
+    void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
+                  const double* diag, double* x, double* r, double* z, double* p, double* q,
+                  double* cg) {
+        const void* key[12] = {(const void*)(long long)k, (const void*)(long long)n, ptr, col, val,
+                               diag, x, r, z, p, q, cg};
+        bool same = cg_exec_ != nullptr;
+        for (int i = 0; i < 12 && same; ++i) same = cg_key_[i] == key[i];
+        if (!same) {
+            if (cg_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cg_exec_));
+            cg_exec_ = nullptr;
+            red_part();  // allocated outside the capture
+            hipGraph_t g = nullptr;
+            VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
+            VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+            VGC_HIP_CHECK(hipGraphInstantiate(&cg_exec_, g, nullptr, nullptr, 0));
+            VGC_HIP_CHECK(hipGraphDestroy(g));
+            for (int i = 0; i < 12; ++i) cg_key_[i] = key[i];
+        }
+        VGC_HIP_CHECK(hipGraphLaunch(cg_exec_, stream_));
+    }
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
                  const double* diag, double* x, double* r, double* z, double* p, double* q,
                  double* cg) {
@@ -928,6 +952,8 @@ private:
     void* temp_ = nullptr;
     size_t temp_bytes_ = 0;
     double* red_ = nullptr;
+    hipGraphExec_t cg_exec_ = nullptr;
+    const void* cg_key_[12] = {};
 };
 
 typedef Ctx<HipBackend> HipCtx;

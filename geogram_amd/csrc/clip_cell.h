@@ -72,7 +72,7 @@ struct CornerBits {
 constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 320;  // tree nodes per level of the walk (two levels: the depth-first stack)
 constexpr int kBucketCap = 128; // buckets per ring
-constexpr int kMaxFilteredRings = 48;  // power-filtered rings before the list tiers
+constexpr int kMaxFilteredRings = 160;  // power-filtered rings before the list tiers
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 

@@ -174,8 +174,8 @@ public:
 
 private:
     // Jacobi-preconditioned CG from x = 0; returns the iterations done.  The
-    // iterations run on the device (scalars in device memory); the host reads
-    // the state every kCgCheck launches.
+    // iterations run on the device (scalars in device memory), kCgCheck at a
+    // time as one graph launch; the host reads the state between them.
     static constexpr int kCgCheck = 16;
     int pcg(int n, const long long* ptr, const int* col, const double* val, const double* diag,
             const double* b, double* x, double* r, double* z, double* p, double* q, double* scal,
@@ -186,7 +186,7 @@ private:
         be_.d2h(h, scal, sizeof(double) * 8);
         while (h[6] == 0.0 && issued < maxit) {
             const int k = std::min(kCgCheck, maxit - issued);
-            for (int i = 0; i < k; ++i) be_.cg_iter(n, ptr, col, val, diag, x, r, z, p, q, scal);
+            be_.cg_iters(k, n, ptr, col, val, diag, x, r, z, p, q, scal);
             issued += k;
             be_.d2h(h, scal, sizeof(double) * 8);
         }
