@@ -77,6 +77,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->why_rings = e.why_rings;
     s->why_walk = e.why_walk;
     s->why_narrow = e.why_narrow;
+    s->nb_regather = e.nb_regather;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }

@@ -72,7 +72,7 @@ struct CornerBits {
 constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 320;  // tree nodes per level of the walk (two levels: the depth-first stack)
 constexpr int kBucketCap = 128; // buckets per ring
-constexpr int kMaxFilteredRings = 160;  // power-filtered rings before the list tiers
+constexpr int kMaxFilteredRings = 48;  // power-filtered rings of the first pass
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -94,6 +94,7 @@ struct ClipParams {
     const int* list_len;        // per item length (nullptr: list_stride)
     int list_stride;
     int list_complete;      // lists hold every seed the cell can need
+    int ring_limit;         // gather mode: power-filtered rings before the list tiers
     int weighted;
     // outputs, indexed by the caller's seed index
     double* vol;
@@ -1031,7 +1032,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             int cnt = rlo2 >= 0.0f
                           ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
-            if (W && F.on && ++nfilt > kMaxFilteredRings) {
+            if (W && F.on && ++nfilt > P.ring_limit) {
                 // the list tiers take the cell, with the same filter and no
                 // capacity limit
                 R.flags |= CELL_NEED_K | CELL_WHY_RINGS;

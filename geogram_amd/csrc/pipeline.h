@@ -48,6 +48,7 @@ struct EngineStats {
     unsigned long long a_live = 0, a_new = 0, a_tets = 0, a_facets = 0, a_tests = 0, a_cells = 0;
     unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
     unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
+    unsigned long long nb_regather = 0;  // weighted cells gathered again without the ring limit
 };
 
 struct Problem {
@@ -270,6 +271,7 @@ public:
         P.list_len = nullptr;
         P.list_stride = K0;
         P.list_complete = 0;
+        P.ring_limit = kMaxFilteredRings;
         be_.event_begin(1);
         be_.clip(1, false, gather, P);
         be_.event_end(1);
@@ -308,6 +310,30 @@ public:
             st.nb_empty += nempty;
             st.t_total_ms += ms_since(tstart);
             return 1;
+        }
+        if (weighted_ && gather) {
+            // weighted cells that used up the first pass's ring limit: gather
+            // again without the limit (a wave per cell) before any list tier
+            std::vector<int> again, rest;
+            for (int c : need) (h_flags[c] & CELL_WHY_RINGS ? again : rest).push_back(c);
+            if (!again.empty()) {
+                st.nb_regather += again.size();
+                int* d_items = (int*)slot(S_ITEMS, sizeof(int) * again.size());
+                be_.h2d(d_items, again.data(), sizeof(int) * again.size());
+                ClipParams Q = P;
+                Q.items = d_items;
+                Q.n_items = (int)again.size();
+                Q.ring_limit = 1 << 30;
+                be_.clip(1, false, true, Q);
+                be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+                for (int c : again) {
+                    if (h_flags[c] & CELL_OVERFLOW)
+                        over.push_back(c);
+                    else if (h_flags[c] & CELL_NEED_K)
+                        rest.push_back(c);
+                }
+                need.swap(rest);
+            }
         }
         if (tiers(need, over, false, P, st)) return -1;
         // cells with signs inside their error bound: exact re-clip

@@ -67,6 +67,7 @@ typedef struct vgc_stats {
        weighted search, a tree walk beyond the wave's buffers, a ring that
        could not be narrowed */
     uint64_t why_rings, why_walk, why_narrow;
+    uint64_t nb_regather; /* weighted cells gathered again without the ring limit */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in
