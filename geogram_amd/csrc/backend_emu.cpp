@@ -260,7 +260,7 @@ struct EmuBackend {
     // ----------------------------------------------------------------- OT
     // reductions in the order of the device kernels (512 blocks x 256
     // threads, grid-stride, halving-stride block trees)
-    static constexpr int kRedBlocks = 2048;
+    static constexpr int kRedBlocks = 512;
     static double tree_sum(double* v) {
         for (int off = 128; off > 0; off >>= 1)
             for (int t = 0; t < off; ++t) v[t] += v[t + off];
@@ -364,7 +364,16 @@ struct EmuBackend {
                  const double* diag, double* x, double* r, double* z, double* p, double* q,
                  double* cg) {
         if (cg[6] != 0.0) return;
-        ot_spmv(n, ptr, col, val, diag, p, q);
+        for (int i = 0; i < n; ++i) {
+            double s[kSpmvLanes];
+            for (int l = 0; l < kSpmvLanes; ++l) s[l] = ot_spmv_lane(i, l, ptr, col, val, p);
+            for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                double t[kSpmvLanes];
+                for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
+                for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
+            }
+            q[i] = s[0] + diag[i] * p[i];
+        }
         const double pq = red_sum(n, [&](int i) { return p[i] * q[i]; });
         cg[2] = pq;
         if (!(pq > 0.0)) {

@@ -227,7 +227,7 @@ __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restr
 // Deterministic reductions: kRedBlocks blocks of 256 threads; thread t of
 // block b accumulates elements b*256+t + k*kRedBlocks*256 in order, the block
 // reduces by halving strides, and one block reduces the partials the same way.
-constexpr int kRedBlocks = 2048;
+constexpr int kRedBlocks = 512;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -511,22 +511,31 @@ __global__ __launch_bounds__(256) void k_cg_start_final(const double* __restrict
     }
 }
 
-__global__ __launch_bounds__(256) void k_cg_spmv_dot(int n, const long long* __restrict__ ptr,
-                                                     const int* __restrict__ col,
-                                                     const double* __restrict__ val,
-                                                     const double* __restrict__ diag,
-                                                     const double* __restrict__ p, double* q,
-                                                     const double* __restrict__ cg, double* part) {
-    __shared__ double sh[256];
+// q = A p, kSpmvLanes lanes per row
+__global__ __launch_bounds__(256) void k_cg_spmv(int n, const long long* __restrict__ ptr,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ diag,
+                                                 const double* __restrict__ p, double* q,
+                                                 const double* __restrict__ cg) {
     if (cg[6] != 0.0) return;  // uniform over the grid
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
+    double s = i < n ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+#pragma unroll
+    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    if (i < n && l == 0) q[i] = s + diag[i] * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_cg_dot_pq(int n, const double* __restrict__ p,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ cg, double* part) {
+    __shared__ double sh[256];
+    if (cg[6] != 0.0) return;
     double s = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
-        const double qi = ot_spmv_row(i, ptr, col, val, diag, p);
-        q[i] = qi;
-        s += p[i] * qi;
-    }
-    double t = block_sum(s, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += p[i] * q[i];
+    double tt = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tt;
 }
 
 __global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ part, double* cg) {
@@ -905,7 +914,9 @@ struct HipBackend {
                  const double* diag, double* x, double* r, double* z, double* p, double* q,
                  double* cg) {
         double* part = red_part();
-        k_cg_spmv_dot<<<kRedBlocks, 256, 0, stream_>>>(n, ptr, col, val, diag, p, q, cg, part);
+        k_cg_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(n, ptr, col, val,
+                                                                                   diag, p, q, cg);
+        k_cg_dot_pq<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, cg, part);
         k_cg_alpha<<<1, 256, 0, stream_>>>(part, cg);
         k_cg_update_dots<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part);
         k_cg_beta<<<1, 256, 0, stream_>>>(part, cg);
