@@ -277,19 +277,31 @@ class Context:
         self._check(rc, allow_partial=False)
 
     # ---------------------------------------------------------------- Lloyd
-    def lloyd(self, points, iters, box=UNIT_BOX, k0=0):
-        """iters Lloyd iterations; returns the new points (same kind of array)."""
+    def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None):
+        """iters Lloyd iterations; returns the new points (same kind of array).
+        inst_n / boxes: several independent instances at once (consecutive
+        seeds, one box each)."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
             x = points.detach().clone().to(torch.float64).contiguous().reshape(-1, 3)
         else:
             x = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
-        boxa = np.ascontiguousarray(box, dtype=np.float64)
         st = Stats()
-        rc = self.lib.fn("lloyd")(self.ctx, _ptr(x), x.shape[0], boxa.ctypes.data_as(ctypes.c_void_p),
-                                  int(iters), int(k0), VGC_DEVICE_PTRS if dev else 0,
-                                  ctypes.byref(st))
+        flags = VGC_DEVICE_PTRS if dev else 0
+        if inst_n is None:
+            boxa = np.ascontiguousarray(box, dtype=np.float64)
+            rc = self.lib.fn("lloyd")(self.ctx, _ptr(x), x.shape[0],
+                                      boxa.ctypes.data_as(ctypes.c_void_p), int(iters), int(k0),
+                                      flags, ctypes.byref(st))
+        else:
+            ia = np.ascontiguousarray(inst_n, dtype=np.uint32)
+            ba = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+            if ia.sum() != x.shape[0] or len(ba) != len(ia):
+                raise ValueError("inst_n must add up to the seed count, one box per instance")
+            rc = self.lib.fn("lloyd_batch")(self.ctx, _ptr(x), ia.ctypes.data_as(ctypes.c_void_p),
+                                            len(ia), ba.ctypes.data_as(ctypes.c_void_p),
+                                            int(iters), int(k0), flags, ctypes.byref(st))
         self._check(rc)
         return x, st.as_dict()
 

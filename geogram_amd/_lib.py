@@ -85,6 +85,9 @@ def _declare(L, prefix):
     f = L[prefix + "lloyd"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
+    f = L[prefix + "lloyd_batch"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
     f = L[prefix + "ot_solve"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
@@ -115,7 +118,8 @@ class _Lib:
         self.cdll = ctypes.CDLL(path)
         self.prefix = prefix
         self._f = {}
-        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "ot_solve", "ot_features",
+        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch", "ot_solve",
+                 "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:
             self._f[prefix + nm] = getattr(self.cdll, prefix + nm)

@@ -250,11 +250,12 @@ int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t fla
 }
 
 template <class B>
-int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
-              uint32_t k0, uint32_t flags, vgc_stats* stats) {
+int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
+                    const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
+                    vgc_stats* stats) {
     if (!ctx) return VGC_ERR_ARG;
     try {
-        if (!xyz || !box || n < 1) {
+        if (!xyz || !boxes || !inst_n || n_inst < 1) {
             ctx->err = "vgc_lloyd: bad argument";
             return VGC_ERR_ARG;
         }
@@ -265,10 +266,23 @@ int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t 
         B& be = ctx->be;
         const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
         Problem pb;
+        pb.n_inst = (int)n_inst;
+        pb.inst_n.resize(n_inst);
+        long long n = 0;
+        for (uint32_t i = 0; i < n_inst; ++i) {
+            if (inst_n[i] == 0) {
+                ctx->err = "vgc_lloyd: empty instance";
+                return VGC_ERR_ARG;
+            }
+            pb.inst_n[i] = (int)inst_n[i];
+            n += inst_n[i];
+        }
+        if (n > 0x7FFFFFFFll) {
+            ctx->err = "vgc_lloyd: more than 2^31 - 1 seeds";
+            return VGC_ERR_ARG;
+        }
         pb.n = (int)n;
-        pb.n_inst = 1;
-        pb.inst_n = {(int)n};
-        pb.box.assign(box, box + 6);
+        pb.box.assign(boxes, boxes + 6 * (size_t)n_inst);
         pb.k0 = (int)k0;
         double* dx = xyz;
         if (!dev) {
@@ -292,6 +306,12 @@ int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t 
         ctx->err = e.what();
         return VGC_ERR_DEVICE;
     }
+}
+
+template <class B>
+int api_lloyd(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+              uint32_t k0, uint32_t flags, vgc_stats* stats) {
+    return api_lloyd_batch(ctx, xyz, &n, 1u, box, iters, k0, flags, stats);
 }
 
 template <class B>

@@ -499,6 +499,11 @@ void vgc_emu_set_list_chunk(void* ctx, long long n) { ((EmuCtx*)ctx)->eng.list_c
 int vgc_emu_det4_sign(const double* rows) {
     return vgc::det4_sign_exact(rows, rows + 4, rows + 8, rows + 12);
 }
+int vgc_emu_lloyd_batch(void* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
+                        const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
+                        vgc_stats* st) {
+    return vgc::api_lloyd_batch((EmuCtx*)ctx, xyz, inst_n, n_inst, boxes, iters, k0, flags, st);
+}
 int vgc_emu_lloyd(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
                   uint32_t k0, uint32_t flags, vgc_stats* st) {
     return vgc::api_lloyd((EmuCtx*)ctx, xyz, n, box, iters, k0, flags, st);

@@ -1036,6 +1036,13 @@ int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t
     if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
     return vgc::api_lloyd(ctx->impl, xyz, n, box, iters, k0, flags, stats);
 }
+int vgc_lloyd_batch(vgc_ctx* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
+                    const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
+                    vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_lloyd_batch(ctx->impl, xyz, inst_n, n_inst, boxes, iters, k0, flags, stats);
+}
 
 int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                  const double* box, double tol, uint32_t max_newton, uint32_t flags,

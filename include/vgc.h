@@ -120,6 +120,14 @@ int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w,
 int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
               uint32_t k0, uint32_t flags, vgc_stats* stats);
 
+/* Lloyd iterations of n_inst independent CVT problems at once (instance
+   layout of vgc_cells_batch): each iteration clips every instance's cells in
+   the same launches.  No counterpart in geogram, which runs one
+   CentroidalVoronoiTesselation at a time. */
+int vgc_lloyd_batch(vgc_ctx* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
+                    const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
+                    vgc_stats* stats);
+
 typedef struct vgc_ot_stats {
     uint32_t newton_iters;        /* accepted Newton steps                       */
     uint32_t cg_iters;            /* conjugate-gradient iterations, all steps    */

@@ -260,3 +260,17 @@ def test_cells_beyond_256_corners(emu_ctx):
     r = emu_ctx.cells(x, weights=w)
     assert r.status == ga.VGC_PARTIAL
     assert r.stats["nb_failed"] == 1 and r.stats["nb_fail_capacity"] == 1
+
+
+def test_lloyd_batch_equals_separate_runs(emu_ctx):
+    sizes = [500, 3000, 64, 1500]
+    x = np.vstack([ga.sample_points(i % 3, 60 + i, s) for i, s in enumerate(sizes)])
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    boxes[1] = [-1.0, 0.0, 0.0, 2.0, 1.0, 1.5]
+    x[500:3500] = boxes[1, :3] + x[500:3500] * (boxes[1, 3:] - boxes[1, :3])
+    xb, st = emu_ctx.lloyd(x, 3, inst_n=sizes, boxes=boxes)
+    o = 0
+    for s, b in zip(sizes, boxes):
+        xi, _ = emu_ctx.lloyd(x[o:o + s], 3, box=b)
+        assert np.array_equal(xi, xb[o:o + s])
+        o += s

@@ -135,6 +135,9 @@ def test_gpu_batch_and_lloyd(gpu_ctx, emu_ctx):
           emu_ctx.cells(x, k0=32, inst_n=sizes, boxes=boxes))
     _same(gpu_ctx.cells(x, k0=0, inst_n=sizes, boxes=boxes),
           emu_ctx.cells(x, k0=0, inst_n=sizes, boxes=boxes))
+    gb, _ = gpu_ctx.lloyd(x, 3, inst_n=sizes, boxes=boxes)
+    eb, _ = emu_ctx.lloyd(x, 3, inst_n=sizes, boxes=boxes)
+    assert np.array_equal(gb, eb)
     y = ga.sample_points(0, 3, 4000)
     gy, _ = gpu_ctx.lloyd(y, 3, k0=0)
     ey, _ = emu_ctx.lloyd(y, 3, k0=0)
