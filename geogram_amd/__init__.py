@@ -214,6 +214,35 @@ class Context:
         self._check(rc)
         return w, st.as_dict()
 
+    def ot_solve_batch(self, points, nu, inst_n, boxes, w0=None, tol=1e-4, max_newton=100,
+                       verbose=False):
+        """Independent transport problems at once (consecutive seeds, one box
+        each; each instance's nu sums to its box volume).  Returns (w, list of
+        per-instance stats)."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            nu_ = nu.contiguous().to(torch.float64)
+            w = (torch.zeros(pts.shape[0], dtype=torch.float64, device=pts.device) if w0 is None
+                 else w0.detach().clone().to(torch.float64).contiguous())
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            nu_ = np.ascontiguousarray(nu, dtype=np.float64)
+            w = np.zeros(pts.shape[0]) if w0 is None else np.array(w0, dtype=np.float64, copy=True)
+        ia = np.ascontiguousarray(inst_n, dtype=np.uint32)
+        ba = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        if ia.sum() != pts.shape[0] or len(ba) != len(ia):
+            raise ValueError("inst_n must add up to the seed count, one box per instance")
+        st = (OtStats * len(ia))()
+        flags = (VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_OT_VERBOSE if verbose else 0)
+        rc = self.lib.fn("ot_solve_batch")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w),
+                                           ia.ctypes.data_as(ctypes.c_void_p), len(ia),
+                                           ba.ctypes.data_as(ctypes.c_void_p), float(tol),
+                                           int(max_newton), flags, st)
+        self._check(rc)
+        return w, [s.as_dict() for s in st]
+
     # ------------------------------------------------- learned warm start
     def ot_features(self, points, nu, box=UNIT_BOX):
         """Per-seed MLP inputs (n x 6, float32) of the learned OT warm start."""

@@ -92,6 +92,10 @@ def _declare(L, prefix):
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
                   ctypes.POINTER(OtStats)]
+    f = L[prefix + "ot_solve_batch"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
+                  ctypes.POINTER(OtStats)]
     f = L[prefix + "ot_features"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _vp]
@@ -119,7 +123,7 @@ class _Lib:
         self.prefix = prefix
         self._f = {}
         names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch", "ot_solve",
-                 "ot_features",
+                 "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:
             self._f[prefix + nm] = getattr(self.cdll, prefix + nm)

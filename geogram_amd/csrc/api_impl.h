@@ -396,6 +396,108 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
     }
 }
 
+inline void copy_ot_stats(const OtStats& os, vgc_ot_stats* s) {
+    s->newton_iters = (uint32_t)os.newton_iters;
+    s->cg_iters = (uint32_t)os.cg_iters;
+    s->cell_passes = (uint32_t)os.cell_passes;
+    s->halvings = (uint32_t)os.halvings;
+    s->used_fallback_start = (uint32_t)os.used_fallback_start;
+    s->converged = (uint32_t)os.converged;
+    s->err0 = os.err0;
+    s->final_err = os.final_err;
+    s->t_total_ms = os.t_total_ms;
+    s->t_cells_ms = os.t_cells_ms;
+    s->t_cg_ms = os.t_cg_ms;
+}
+
+template <class B>
+int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double* w,
+                       const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
+                       uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !nu || !w || !inst_n || !boxes || n_inst < 1 || !(tol > 0.0)) {
+            ctx->err = "vgc_ot_solve_batch: bad argument";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        Problem pb;
+        pb.n_inst = (int)n_inst;
+        pb.inst_n.resize(n_inst);
+        long long n = 0;
+        for (uint32_t k = 0; k < n_inst; ++k) {
+            const double* b = boxes + 6 * (size_t)k;
+            if (inst_n[k] == 0 || !(b[3] > b[0] && b[4] > b[1] && b[5] > b[2])) {
+                ctx->err = "vgc_ot_solve_batch: empty instance or degenerate box";
+                return VGC_ERR_ARG;
+            }
+            pb.inst_n[k] = (int)inst_n[k];
+            n += inst_n[k];
+        }
+        if (n > 0x7FFFFFFFll) {
+            ctx->err = "vgc_ot_solve_batch: more than 2^31 - 1 seeds";
+            return VGC_ERR_ARG;
+        }
+        pb.n = (int)n;
+        pb.box.assign(boxes, boxes + 6 * (size_t)n_inst);
+        pb.k0 = 0;
+        const double *dx = xyz, *dnu = nu;
+        double* dw = w;
+        if (!dev) {
+            double* sx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            double* sn = (double*)ctx->staging(5, sizeof(double) * (size_t)n);
+            double* sw = (double*)ctx->staging(6, sizeof(double) * (size_t)n);
+            be.h2d(sx, xyz, sizeof(double) * 3 * (size_t)n);
+            be.h2d(sn, nu, sizeof(double) * (size_t)n);
+            be.h2d(sw, w, sizeof(double) * (size_t)n);
+            dx = sx;
+            dnu = sn;
+            dw = sw;
+        }
+        // each instance's masses must add up to its box volume
+        std::vector<double> hnu(n);
+        be.d2h(hnu.data(), dnu, sizeof(double) * (size_t)n);
+        for (uint32_t k = 0, o = 0; k < n_inst; o += inst_n[k], ++k) {
+            const double* b = boxes + 6 * (size_t)k;
+            const double V = (b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+            double tot = 0.0, mn = 1e300;
+            for (uint32_t i = o; i < o + inst_n[k]; ++i) {
+                tot += hnu[i];
+                mn = std::min(mn, hnu[i]);
+            }
+            if (!(mn > 0.0) || std::fabs(tot - V) > 1e-9 * V) {
+                ctx->err = "vgc_ot_solve_batch: target masses of instance " + std::to_string(k) +
+                           " must be positive and sum to its box volume";
+                return VGC_ERR_ARG;
+            }
+        }
+        pb.d_xyz = dx;
+        std::vector<OtStats> os;
+        const uint32_t maxn = max_newton ? max_newton : 100;
+        if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
+                                (flags & VGC_OT_VERBOSE) != 0)) {
+            ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;
+            return VGC_ERR_INTERNAL;
+        }
+        if (!dev) be.d2h(w, dw, sizeof(double) * (size_t)n);
+        be.sync();
+        bool all = true;
+        for (uint32_t k = 0; k < n_inst; ++k) {
+            if (stats) copy_ot_stats(os[k], stats + k);
+            all = all && os[k].converged;
+        }
+        if (!all) {
+            ctx->err = "vgc_ot_solve_batch: not every instance converged";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
 // ------------------------------------------------------- OT warm start
 template <class B>
 int api_ot_features(Ctx<B>* ctx, const double* xyz, const double* nu, uint32_t n,

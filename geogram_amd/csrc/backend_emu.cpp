@@ -397,6 +397,173 @@ struct EmuBackend {
         ot_pcg_dir(n, cg[4], z, p);
     }
 
+    // --------------------------------------------------------- batched OT
+    // same block table and reduction orders as the gfx950 kernels
+    std::vector<double> seg_part_;
+    template <class F>
+    static void seg_block_sums(const SegTable& T, int blk, F f, double& t1, double& t2) {
+        const int k = T.blk_inst[blk];
+        const long long beg0 = T.off[k] + (long long)T.blk_idx[blk] * 256, end = T.off[k + 1];
+        const long long stride = (long long)T.cnt[k] * 256;
+        double a[256], b[256];
+        for (int t = 0; t < 256; ++t) {
+            double s1 = 0.0, s2 = 0.0;
+            for (long long i = beg0 + t; i < end; i += stride) f(i, s1, s2);
+            a[t] = s1;
+            b[t] = s2;
+        }
+        t1 = tree_sum(a);
+        t2 = tree_sum(b);
+    }
+    static void seg_final(const SegTable& T, int k, const double* part, double& t1, double& t2) {
+        double a[256], b[256];
+        for (int t = 0; t < 256; ++t) {
+            a[t] = t < T.cnt[k] ? part[T.first[k] + t] : 0.0;
+            b[t] = t < T.cnt[k] ? part[T.nblk + T.first[k] + t] : 0.0;
+        }
+        t1 = tree_sum(a);
+        t2 = tree_sum(b);
+    }
+    void seg_mass_stats(const SegTable& T, const double* m, const double* nu, double* out) {
+        for (int k = 0; k < T.n_inst; ++k) {
+            double a[256], mx = 0.0, mn = INFINITY;
+            for (int t = 0; t < 256; ++t) {
+                double s2 = 0.0;
+                for (long long i = T.off[k] + t; i < T.off[k + 1]; i += 256) {
+                    const double d = m[i] - nu[i];
+                    mx = std::fmax(mx, std::fabs(d) / nu[i]);
+                    s2 += d * d;
+                    mn = std::fmin(mn, m[i]);
+                }
+                a[t] = s2;
+            }
+            out[3 * k + 0] = mx;
+            out[3 * k + 1] = tree_sum(a);
+            out[3 * k + 2] = mn;
+        }
+    }
+    void seg_rhs(int n, const int* inst_of, const int* active, const double* nu, const double* m,
+                 double* b) {
+        for (int i = 0; i < n; ++i) b[i] = active[inst_of[i]] ? nu[i] - m[i] : 0.0;
+    }
+    void seg_axpy(int n, const int* inst_of, const double* w, const double* t, const double* d,
+                  double* out) {
+        for (int i = 0; i < n; ++i) out[i] = w[i] + t[inst_of[i]] * d[i];
+    }
+    void seg_copy(int n, const int* inst_of, const int* sel, const double* src, double* dst) {
+        for (int i = 0; i < n; ++i)
+            if (sel[inst_of[i]]) dst[i] = src[i];
+    }
+    void cgb_start(const SegTable& T, const double* b, const double* diag, double* x, double* r,
+                   double* z, double* p, double* cg, double tol2) {
+        seg_part_.assign(2 * (size_t)T.nblk, 0.0);
+        for (int blk = 0; blk < T.nblk; ++blk) {
+            double t1, t2;
+            seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
+                const double ri = b[i], zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+                x[i] = 0.0;
+                r[i] = ri;
+                z[i] = zi;
+                p[i] = zi;
+                s1 += ri * zi;
+                s2 += ri * ri;
+            }, t1, t2);
+            seg_part_[blk] = t1;
+            seg_part_[T.nblk + blk] = t2;
+        }
+        for (int k = 0; k < T.n_inst; ++k) {
+            double t1, t2;
+            seg_final(T, k, seg_part_.data(), t1, t2);
+            double* c = cg + 8 * k;
+            c[0] = t1;
+            c[1] = t2;
+            c[5] = tol2 * t2;
+            c[6] = t2 > 0.0 ? 0.0 : 1.0;
+            c[7] = 0.0;
+        }
+    }
+    void cgb_iters(int kk, const SegTable& T, int n, const long long* ptr, const int* col,
+                   const double* val, const double* diag, double* x, double* r, double* z,
+                   double* p, double* q, double* cg) {
+        seg_part_.assign(2 * (size_t)T.nblk, 0.0);
+        for (int it = 0; it < kk; ++it) {
+            for (int i = 0; i < n; ++i) {
+                if (cg[8 * T.inst_of[i] + 6] != 0.0) continue;
+                double s[kSpmvLanes];
+                for (int l = 0; l < kSpmvLanes; ++l) s[l] = ot_spmv_lane(i, l, ptr, col, val, p);
+                for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                    double t[kSpmvLanes];
+                    for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
+                    for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
+                }
+                q[i] = s[0] + diag[i] * p[i];
+            }
+            for (int blk = 0; blk < T.nblk; ++blk) {
+                const int k = T.blk_inst[blk];
+                double t1 = 0.0, t2 = 0.0;
+                if (cg[8 * k + 6] == 0.0)
+                    seg_block_sums(T, blk, [&](long long i, double& s1, double&) { s1 += p[i] * q[i]; },
+                                   t1, t2);
+                else {
+                    double a[256] = {};
+                    t1 = tree_sum(a);
+                }
+                seg_part_[blk] = t1;
+            }
+            for (int k = 0; k < T.n_inst; ++k) {
+                double* c = cg + 8 * k;
+                if (c[6] != 0.0) continue;
+                double pq, dummy;
+                seg_final(T, k, seg_part_.data(), pq, dummy);
+                c[2] = pq;
+                if (!(pq > 0.0)) {
+                    c[3] = 0.0;
+                    c[6] = 2.0;
+                } else {
+                    c[3] = c[0] / pq;
+                }
+            }
+            for (int blk = 0; blk < T.nblk; ++blk) {
+                const int k = T.blk_inst[blk];
+                double t1 = 0.0, t2 = 0.0;
+                if (cg[8 * k + 6] == 0.0) {
+                    const double alpha = cg[8 * k + 3];
+                    seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
+                        x[i] += alpha * p[i];
+                        const double ri = r[i] - alpha * q[i];
+                        r[i] = ri;
+                        const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+                        z[i] = zi;
+                        s1 += ri * zi;
+                        s2 += ri * ri;
+                    }, t1, t2);
+                }
+                seg_part_[blk] = t1;
+                seg_part_[T.nblk + blk] = t2;
+            }
+            for (int k = 0; k < T.n_inst; ++k) {
+                double* c = cg + 8 * k;
+                if (c[6] != 0.0) continue;
+                double rz, rr;
+                seg_final(T, k, seg_part_.data(), rz, rr);
+                c[1] = rr;
+                c[7] += 1.0;
+                if (rr <= c[5]) {
+                    c[4] = 0.0;
+                    c[6] = 1.0;
+                } else {
+                    c[4] = rz / c[0];
+                    c[0] = rz;
+                }
+            }
+            for (int i = 0; i < n; ++i) {
+                const double* c = cg + 8 * T.inst_of[i];
+                if (c[6] != 0.0) continue;
+                p[i] = z[i] + c[4] * p[i];
+            }
+        }
+    }
+
     // ---------------------------------------------------------------- MLP
     static constexpr int kRowsFwd = 64, kRowsGrad = 32;
     void mlp_forward(const float* P, const float* X, int n, float* out) {
@@ -477,6 +644,12 @@ int vgc_emu_ot_solve(void* ctx, const double* xyz, const double* nu, double* w, 
                      const double* box, double tol, uint32_t max_newton, uint32_t flags,
                      vgc_ot_stats* stats) {
     return vgc::api_ot_solve((EmuCtx*)ctx, xyz, nu, w, n, box, tol, max_newton, flags, stats);
+}
+int vgc_emu_ot_solve_batch(void* ctx, const double* xyz, const double* nu, double* w,
+                           const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
+                           uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    return vgc::api_ot_solve_batch((EmuCtx*)ctx, xyz, nu, w, inst_n, n_inst, boxes, tol, max_newton,
+                                   flags, stats);
 }
 int vgc_emu_ot_features(void* ctx, const double* xyz, const double* nu, uint32_t n,
                         const double* box, uint32_t flags, float* feats) {

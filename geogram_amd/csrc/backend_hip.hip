@@ -613,6 +613,231 @@ __global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
     if (i < n) p[i] = z[i] + beta * p[i];
 }
 
+
+// ------------------------------------------------------ batched OT kernels
+__device__ __forceinline__ void seg_range(const SegTable& T, int blk, int& k, long long& beg,
+                                          long long& end, long long& stride) {
+    k = T.blk_inst[blk];
+    beg = T.off[k] + (long long)T.blk_idx[blk] * 256 + threadIdx.x;
+    end = T.off[k + 1];
+    stride = (long long)T.cnt[k] * 256;
+}
+
+// per instance: [0] max |m-nu|/nu, [1] sum (m-nu)^2, [2] min m; one block each
+__global__ __launch_bounds__(256) void k_seg_mass(SegTable T, const double* __restrict__ m,
+                                                  const double* __restrict__ nu, double* out) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    double mx = 0.0, s2 = 0.0, mn = INFINITY;
+    for (long long i = T.off[k] + threadIdx.x; i < T.off[k + 1]; i += 256) {
+        const double d = m[i] - nu[i];
+        mx = fmax(mx, fabs(d) / nu[i]);
+        s2 += d * d;
+        mn = fmin(mn, m[i]);
+    }
+    double t2, dummy;
+    block_sum2(s2, 0.0, sh, sh2, t2, dummy);
+    __syncthreads();
+    sh[threadIdx.x] = mx;
+    sh2[threadIdx.x] = mn;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+            sh2[threadIdx.x] = fmin(sh2[threadIdx.x], sh2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * k + 0] = sh[0];
+        out[3 * k + 1] = t2;
+        out[3 * k + 2] = sh2[0];
+    }
+}
+
+// b = nu - m for active instances, 0 for the others
+__global__ void k_seg_rhs(int n, const int* __restrict__ inst_of, const int* __restrict__ active,
+                          const double* __restrict__ nu, const double* __restrict__ m, double* b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = active[inst_of[i]] ? nu[i] - m[i] : 0.0;
+}
+
+// out = w + t[instance] d
+__global__ void k_seg_axpy(int n, const int* __restrict__ inst_of, const double* __restrict__ w,
+                           const double* __restrict__ t, const double* __restrict__ d,
+                           double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = w[i] + t[inst_of[i]] * d[i];
+}
+
+// dst = src on the instances with sel != 0
+__global__ void k_seg_copy(int n, const int* __restrict__ inst_of, const int* __restrict__ sel,
+                           const double* __restrict__ src, double* dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && sel[inst_of[i]]) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_cgb_start_partial(SegTable T, const double* __restrict__ b,
+                                                           const double* __restrict__ diag,
+                                                           double* x, double* r, double* z,
+                                                           double* p, double* part) {
+    __shared__ double sh[256], sh2[256];
+    int k;
+    long long beg, end, stride;
+    seg_range(T, blockIdx.x, k, beg, end, stride);
+    double s1 = 0.0, s2 = 0.0;
+    for (long long i = beg; i < end; i += stride) {
+        const double ri = b[i], zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        s1 += ri * zi;
+        s2 += ri * ri;
+    }
+    double t1, t2;
+    block_sum2(s1, s2, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t1;
+        part[T.nblk + blockIdx.x] = t2;
+    }
+}
+
+__device__ __forceinline__ void seg_final2(const SegTable& T, int k, const double* part, double* sh,
+                                           double* sh2, double& t1, double& t2) {
+    const int c = T.cnt[k], f = T.first[k];
+    double a = (int)threadIdx.x < c ? part[f + threadIdx.x] : 0.0;
+    double b = (int)threadIdx.x < c ? part[T.nblk + f + threadIdx.x] : 0.0;
+    block_sum2(a, b, sh, sh2, t1, t2);
+}
+
+__global__ __launch_bounds__(256) void k_cgb_start_final(SegTable T, const double* __restrict__ part,
+                                                         double* cg, double tol2) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    double t1, t2;
+    seg_final2(T, k, part, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        double* c = cg + 8 * k;
+        c[0] = t1;
+        c[1] = t2;
+        c[5] = tol2 * t2;
+        c[6] = t2 > 0.0 ? 0.0 : 1.0;
+        c[7] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__ inst_of,
+                                                  const long long* __restrict__ ptr,
+                                                  const int* __restrict__ col,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ diag,
+                                                  const double* __restrict__ p, double* q,
+                                                  const double* __restrict__ cg) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
+    const bool live = i < n && cg[8 * inst_of[i < n ? i : 0] + 6] == 0.0;
+    double s = live ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+#pragma unroll
+    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    if (live && l == 0) q[i] = s + diag[i] * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_cgb_dot_pq(SegTable T, const double* __restrict__ p,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ cg, double* part) {
+    __shared__ double sh[256], sh2[256];
+    int k;
+    long long beg, end, stride;
+    seg_range(T, blockIdx.x, k, beg, end, stride);
+    double s = 0.0;
+    if (cg[8 * k + 6] == 0.0)
+        for (long long i = beg; i < end; i += stride) s += p[i] * q[i];
+    double t1, t2;
+    block_sum2(s, 0.0, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) part[blockIdx.x] = t1;
+}
+
+__global__ __launch_bounds__(256) void k_cgb_alpha(SegTable T, const double* __restrict__ part,
+                                                   double* cg) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    double* c = cg + 8 * k;
+    if (c[6] != 0.0) return;  // uniform over the block
+    double pq, dummy;
+    seg_final2(T, k, part, sh, sh2, pq, dummy);
+    if (threadIdx.x == 0) {
+        c[2] = pq;
+        if (!(pq > 0.0)) {
+            c[3] = 0.0;
+            c[6] = 2.0;
+        } else {
+            c[3] = c[0] / pq;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cgb_update_dots(SegTable T, const double* __restrict__ p,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ diag,
+                                                         double* x, double* r, double* z,
+                                                         const double* __restrict__ cg,
+                                                         double* part) {
+    __shared__ double sh[256], sh2[256];
+    int k;
+    long long beg, end, stride;
+    seg_range(T, blockIdx.x, k, beg, end, stride);
+    double s1 = 0.0, s2 = 0.0;
+    if (cg[8 * k + 6] == 0.0) {
+        const double alpha = cg[8 * k + 3];
+        for (long long i = beg; i < end; i += stride) {
+            x[i] += alpha * p[i];
+            const double ri = r[i] - alpha * q[i];
+            r[i] = ri;
+            const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+            z[i] = zi;
+            s1 += ri * zi;
+            s2 += ri * ri;
+        }
+    }
+    double t1, t2;
+    block_sum2(s1, s2, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t1;
+        part[T.nblk + blockIdx.x] = t2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cgb_beta(SegTable T, const double* __restrict__ part,
+                                                  double* cg) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    double* c = cg + 8 * k;
+    if (c[6] != 0.0) return;
+    double rz, rr;
+    seg_final2(T, k, part, sh, sh2, rz, rr);
+    if (threadIdx.x == 0) {
+        c[1] = rr;
+        c[7] += 1.0;
+        if (rr <= c[5]) {
+            c[4] = 0.0;
+            c[6] = 1.0;
+        } else {
+            c[4] = rz / c[0];
+            c[0] = rz;
+        }
+    }
+}
+
+__global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* __restrict__ z,
+                          double* p, const double* __restrict__ cg) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* c = cg + 8 * inst_of[i];
+    if (c[6] != 0.0) return;
+    p[i] = z[i] + c[4] * p[i];
+}
+
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
 // ----------------------------------------------------------------- backend
@@ -631,6 +856,7 @@ struct HipBackend {
         if (temp_) (void)hipFree(temp_);
         if (red_) (void)hipFree(red_);
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
+        if (seg_part_) (void)hipFree(seg_part_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     hipStream_t stream() const { return stream_; }
@@ -924,6 +1150,55 @@ struct HipBackend {
         check();
     }
 
+    // --------------------------------------------------------- batched OT
+    double* seg_part(int nblk) {
+        if (seg_part_cap_ < nblk) {
+            if (seg_part_) VGC_HIP_CHECK(hipFree(seg_part_));
+            VGC_HIP_CHECK(hipMalloc(&seg_part_, sizeof(double) * 2 * (size_t)nblk));
+            seg_part_cap_ = nblk;
+        }
+        return seg_part_;
+    }
+    void seg_mass_stats(const SegTable& T, const double* m, const double* nu, double* out) {
+        k_seg_mass<<<T.n_inst, 256, 0, stream_>>>(T, m, nu, out);
+        check();
+    }
+    void seg_rhs(int n, const int* inst_of, const int* active, const double* nu, const double* m,
+                 double* b) {
+        k_seg_rhs<<<blocks_for(n, 256), 256, 0, stream_>>>(n, inst_of, active, nu, m, b);
+        check();
+    }
+    void seg_axpy(int n, const int* inst_of, const double* w, const double* t, const double* d,
+                  double* out) {
+        k_seg_axpy<<<blocks_for(n, 256), 256, 0, stream_>>>(n, inst_of, w, t, d, out);
+        check();
+    }
+    void seg_copy(int n, const int* inst_of, const int* sel, const double* src, double* dst) {
+        k_seg_copy<<<blocks_for(n, 256), 256, 0, stream_>>>(n, inst_of, sel, src, dst);
+        check();
+    }
+    void cgb_start(const SegTable& T, const double* b, const double* diag, double* x, double* r,
+                   double* z, double* p, double* cg, double tol2) {
+        double* part = seg_part(T.nblk);
+        k_cgb_start_partial<<<T.nblk, 256, 0, stream_>>>(T, b, diag, x, r, z, p, part);
+        k_cgb_start_final<<<T.n_inst, 256, 0, stream_>>>(T, part, cg, tol2);
+        check();
+    }
+    void cgb_iters(int k, const SegTable& T, int n, const long long* ptr, const int* col,
+                   const double* val, const double* diag, double* x, double* r, double* z,
+                   double* p, double* q, double* cg) {
+        double* part = seg_part(T.nblk);
+        for (int it = 0; it < k; ++it) {
+            k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
+                n, T.inst_of, ptr, col, val, diag, p, q, cg);
+            k_cgb_dot_pq<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part);
+            k_cgb_alpha<<<T.n_inst, 256, 0, stream_>>>(T, part, cg);
+            k_cgb_update_dots<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part);
+            k_cgb_beta<<<T.n_inst, 256, 0, stream_>>>(T, part, cg);
+            k_cgb_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, T.inst_of, z, p, cg);
+        }
+        check();
+    }
     // ---------------------------------------------------------------- MLP
     void mlp_forward(const float* P, const float* X, int n, float* out) {
         int waves = (n + kMlpRowsFwd - 1) / kMlpRowsFwd;
@@ -965,6 +1240,8 @@ private:
     double* red_ = nullptr;
     hipGraphExec_t cg_exec_ = nullptr;
     const void* cg_key_[12] = {};
+    double* seg_part_ = nullptr;
+    int seg_part_cap_ = 0;
 };
 
 typedef Ctx<HipBackend> HipCtx;
@@ -1052,6 +1329,14 @@ int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, u
     return vgc::api_ot_solve(ctx->impl, xyz, nu, w, n, box, tol, max_newton, flags, stats);
 }
 
+int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double* w,
+                       const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
+                       uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_ot_solve_batch(ctx->impl, xyz, nu, w, inst_n, n_inst, boxes, tol, max_newton,
+                                   flags, stats);
+}
 int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t n,
                     const double* box, uint32_t flags, float* feats) {
     if (!ctx) return VGC_ERR_ARG;

@@ -59,4 +59,24 @@ VGC_SD double ot_spmv_lane(int i, int l, const long long* __restrict__ ptr,
     return s;
 }
 
+
+// ------------------------------------------------- batched (segmented) solves
+// Instance k holds the caller-order range [off[k], off[k+1]).  Reductions per
+// instance run over a fixed block table: instance k owns cnt[k] blocks
+// (first[k] ... first[k] + cnt[k] - 1); thread t of its block b takes the
+// elements off[k] + b * 256 + t + j * cnt[k] * 256 in order, the block sums
+// by a halving tree, and one block per instance adds its blocks' partials by
+// the same tree: fixed order, bit-reproducible.
+struct SegTable {
+    int n_inst;
+    int nblk;
+    const long long* off;   // n_inst + 1
+    const int* blk_inst;    // nblk
+    const int* blk_idx;     // nblk
+    const int* cnt;         // n_inst
+    const int* first;       // n_inst
+    const int* inst_of;     // caller-order element -> instance
+};
+constexpr int kSegMaxBlocks = 64;  // blocks per instance
+
 }  // namespace vgc

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "ot_kernels.h"
 #include "pipeline.h"
 
 namespace vgc {
@@ -169,6 +170,199 @@ public:
         st.final_err = err;
         st.converged = err <= tol;
         st.t_total_ms = ms_since(t0);
+        return 0;
+    }
+
+    // n_inst independent problems (pb.inst_n, pb.box) solved together: one
+    // cell pass covers every instance, the Laguerre system is block diagonal
+    // and its PCG keeps per-instance scalars (segmented reductions), each
+    // instance has its own damped-Newton state and step.  st[k] per instance.
+    int solve_batch(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
+                    double cg_tol, int cg_max, std::vector<OtStats>& st, bool verbose = false) {
+        auto t0 = std::chrono::steady_clock::now();
+        const int n = pb.n, K = pb.n_inst;
+        st.assign(K, OtStats());
+        pb.want_neighbors = true;
+        pb.abort_on_empty = false;
+        // instance ranges (caller order) and the reduction block table
+        std::vector<long long> off(K + 1, 0);
+        for (int k = 0; k < K; ++k) off[k + 1] = off[k] + pb.inst_n[k];
+        std::vector<int> blk_inst, blk_idx, cnt(K), first(K), p0(K);
+        for (int k = 0; k < K; ++k) {
+            p0[k] = (int)off[k];
+            cnt[k] = (int)std::min<long long>(kSegMaxBlocks, std::max<long long>(1, (pb.inst_n[k] + 2047) / 2048));
+            first[k] = (int)blk_inst.size();
+            for (int b = 0; b < cnt[k]; ++b) {
+                blk_inst.push_back(k);
+                blk_idx.push_back(b);
+            }
+        }
+        const int nblk = (int)blk_inst.size();
+        long long* d_off = (long long*)slot(20, sizeof(long long) * (K + 1));
+        int* d_tab = (int*)slot(21, sizeof(int) * (2 * (size_t)nblk + 3 * (size_t)K));
+        int* d_inst = (int*)slot(22, sizeof(int) * (size_t)n);
+        be_.h2d(d_off, off.data(), sizeof(long long) * (K + 1));
+        std::vector<int> tab;
+        tab.insert(tab.end(), blk_inst.begin(), blk_inst.end());
+        tab.insert(tab.end(), blk_idx.begin(), blk_idx.end());
+        tab.insert(tab.end(), cnt.begin(), cnt.end());
+        tab.insert(tab.end(), first.begin(), first.end());
+        tab.insert(tab.end(), p0.begin(), p0.end());
+        be_.h2d(d_tab, tab.data(), sizeof(int) * tab.size());
+        be_.fill_inst(d_inst, n, d_tab + 2 * nblk + 2 * K, K);
+        SegTable T{K, nblk, d_off, d_tab, d_tab + nblk, d_tab + 2 * nblk, d_tab + 2 * nblk + K, d_inst};
+
+        double* d_b = (double*)slot(0, sizeof(double) * n);
+        double* d_x = (double*)slot(1, sizeof(double) * n);
+        double* d_r = (double*)slot(2, sizeof(double) * n);
+        double* d_z = (double*)slot(3, sizeof(double) * n);
+        double* d_p = (double*)slot(4, sizeof(double) * n);
+        double* d_q = (double*)slot(5, sizeof(double) * n);
+        double* d_diag = (double*)slot(6, sizeof(double) * n);
+        double* d_wt = (double*)slot(7, sizeof(double) * n);
+        double* d_ms = (double*)slot(23, sizeof(double) * 3 * (size_t)K);
+        double* d_cg = (double*)slot(24, sizeof(double) * 8 * (size_t)K);
+        double* d_t = (double*)slot(25, sizeof(double) * (size_t)K);
+        int* d_sel = (int*)slot(26, sizeof(int) * (size_t)K);
+        EngineStats es;
+        std::vector<double> ms(3 * (size_t)K), tk(K), err(K), g2(K), eps0(K);
+        std::vector<int> sel(K), bad;
+        int passes = 0;
+        auto cells = [&](const double* w) -> int {
+            auto tc = std::chrono::steady_clock::now();
+            pb.d_w = w;
+            int rc = E_.cells(pb, es);
+            ++passes;
+            bad = E_.inst_bad;
+            be_.seg_mass_stats(T, E_.d_vol, d_nu, d_ms);
+            be_.d2h(ms.data(), d_ms, sizeof(double) * 3 * (size_t)K);
+            const double dt = ms_since(tc);
+            for (auto& s : st) s.t_cells_ms += dt;
+            return rc < 0 ? -1 : 0;
+        };
+        // start: instances whose start leaves a cell empty or failed restart from 0
+        if (cells(d_w)) return -1;
+        bool any0 = false;
+        for (int k = 0; k < K; ++k) {
+            sel[k] = bad[k] > 0 || !(ms[3 * k + 2] > 0.0);
+            if (sel[k]) {
+                st[k].used_fallback_start = 1;
+                any0 = true;
+            }
+        }
+        if (any0) {
+            be_.dzero(d_wt, sizeof(double) * n);
+            be_.h2d(d_sel, sel.data(), sizeof(int) * K);
+            be_.seg_copy(n, d_inst, d_sel, d_wt, d_w);
+            if (cells(d_w)) return -1;
+            for (int k = 0; k < K; ++k)
+                if (bad[k] > 0) return fail("the cells of the zero start failed");
+        }
+        std::vector<double> numin(3 * (size_t)K);
+        be_.seg_mass_stats(T, d_nu, d_nu, d_ms);  // [2] = min nu
+        be_.d2h(numin.data(), d_ms, sizeof(double) * 3 * (size_t)K);
+        be_.seg_mass_stats(T, E_.d_vol, d_nu, d_ms);
+        std::vector<int> active(K);
+        for (int k = 0; k < K; ++k) {
+            eps0[k] = 0.5 * std::min(numin[3 * k + 2], ms[3 * k + 2]);
+            err[k] = ms[3 * k + 0];
+            g2[k] = ms[3 * k + 1];
+            st[k].err0 = err[k];
+            active[k] = err[k] > tol;
+        }
+        int* d_act = (int*)slot(27, sizeof(int) * (size_t)K);
+        for (int it = 0; it < max_newton; ++it) {
+            bool any = false;
+            for (int k = 0; k < K; ++k) any = any || active[k];
+            if (!any) break;
+            auto tg = std::chrono::steady_clock::now();
+            const long long nnz = E_.total_neighbors;
+            double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
+            be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
+            be_.h2d(d_act, active.data(), sizeof(int) * K);
+            be_.seg_rhs(n, d_inst, d_act, d_nu, E_.d_vol, d_b);
+            be_.cgb_start(T, d_b, d_diag, d_x, d_r, d_z, d_p, d_cg, cg_tol * cg_tol);
+            std::vector<double> h(8 * (size_t)K);
+            be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
+            int issued = 0;
+            auto all_done = [&]() {
+                for (int k = 0; k < K; ++k)
+                    if (h[8 * k + 6] == 0.0) return false;
+                return true;
+            };
+            while (!all_done() && issued < cg_max) {
+                const int kk = std::min(kCgCheck, cg_max - issued);
+                be_.cgb_iters(kk, T, n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_x, d_r, d_z, d_p, d_q,
+                              d_cg);
+                issued += kk;
+                be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
+            }
+            const double tcg = ms_since(tg);
+            for (int k = 0; k < K; ++k) {
+                st[k].t_cg_ms += tcg;
+                if (active[k]) st[k].cg_iters += (int)h[8 * k + 7];
+            }
+            // per-instance damped steps
+            std::vector<int> pending(active), accepted(K, 0);
+            for (int k = 0; k < K; ++k) tk[k] = active[k] ? 1.0 : 0.0;
+            be_.d2d(d_wt, d_w, sizeof(double) * n);
+            for (int hv = 0; hv < 40; ++hv) {
+                bool anyp = false;
+                for (int k = 0; k < K; ++k) anyp = anyp || pending[k];
+                if (!anyp) break;
+                be_.h2d(d_t, tk.data(), sizeof(double) * K);
+                be_.seg_axpy(n, d_inst, d_w, d_t, d_x, d_wt);
+                if (cells(d_wt)) return -1;
+                for (int k = 0; k < K; ++k) {
+                    if (!pending[k]) continue;
+                    const bool okk = bad[k] == 0 && ms[3 * k + 2] >= eps0[k] &&
+                                     std::sqrt(ms[3 * k + 1]) <= (1.0 - 0.5 * tk[k]) * std::sqrt(g2[k]);
+                    if (okk) {
+                        pending[k] = 0;
+                        accepted[k] = 1;
+                        err[k] = ms[3 * k + 0];
+                        g2[k] = ms[3 * k + 1];
+                    } else {
+                        tk[k] *= 0.5;
+                        st[k].halvings++;
+                    }
+                }
+                // accepted instances keep their new weights from here on
+                for (int k = 0; k < K; ++k) sel[k] = accepted[k] && tk[k] != 0.0;
+                be_.h2d(d_sel, sel.data(), sizeof(int) * K);
+                be_.seg_copy(n, d_inst, d_sel, d_wt, d_w);
+                for (int k = 0; k < K; ++k)
+                    if (sel[k]) tk[k] = 0.0;
+            }
+            for (int k = 0; k < K; ++k) {
+                if (!active[k]) continue;
+                if (accepted[k]) {
+                    st[k].newton_iters++;
+                    active[k] = err[k] > tol;
+                } else {
+                    active[k] = 0;  // line search exhausted
+                }
+            }
+            if (verbose) {
+                int na = 0;
+                double emax = 0.0;
+                for (int k = 0; k < K; ++k) {
+                    na += active[k];
+                    emax = std::max(emax, err[k]);
+                }
+                std::fprintf(stderr, "[vgc ot batch] newton %d: %d of %d instances active, max err %.3e\n",
+                             it + 1, na, K, emax);
+                std::fflush(stderr);
+            }
+        }
+        be_.sync();
+        const double tt = ms_since(t0);
+        for (int k = 0; k < K; ++k) {
+            st[k].final_err = err[k];
+            st[k].converged = err[k] <= tol;
+            st[k].cell_passes = passes;
+            st[k].t_total_ms = tt;
+        }
         return 0;
     }
 

@@ -85,6 +85,7 @@ public:
     double* d_area = nullptr;
     long long total_neighbors = 0;
     long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
+    std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
     std::vector<uint32_t> h_flags;  // by Morton position
 
     // layout of the last build
@@ -368,13 +369,17 @@ public:
         st.nb_clips += ctr[0];
         st.nb_tests += ctr[1];
         st.nb_cells += n_;
-        for (int c = 0; c < n_; ++c) {
-            uint32_t f = h_flags[c];
-            if (!(f & CELL_DONE)) st.nb_failed++;
-            if (!(f & CELL_DONE) && (f & CELL_OVERFLOW)) st.nb_fail_capacity++;
-            if (f & CELL_UNCERTAIN) st.nb_uncertain++;
-            if (f & CELL_EMPTY) st.nb_empty++;
-            if (f & CELL_DISCONNECTED) st.nb_disconnected++;
+        inst_bad.assign(ni_, 0);
+        for (int k = 0, c = 0; k < ni_; ++k) {
+            for (const int e = h_p0_[k] + h_n_[k]; c < e; ++c) {
+                uint32_t f = h_flags[c];
+                if (!(f & CELL_DONE)) st.nb_failed++;
+                if (!(f & CELL_DONE) && (f & CELL_OVERFLOW)) st.nb_fail_capacity++;
+                if (f & CELL_UNCERTAIN) st.nb_uncertain++;
+                if (f & CELL_EMPTY) st.nb_empty++;
+                if (f & CELL_DISCONNECTED) st.nb_disconnected++;
+                if (!(f & CELL_DONE) || (f & CELL_EMPTY)) inst_bad[k]++;
+            }
         }
 
         // CSR

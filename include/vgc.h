@@ -154,6 +154,16 @@ int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, u
                  const double* box, double tol, uint32_t max_newton, uint32_t flags,
                  vgc_ot_stats* stats);
 
+/* n_inst independent transport problems at once (instance layout of
+   vgc_cells_batch; nu of each instance sums to its box volume; w in: starts,
+   out: solutions).  One cell pass per step covers every instance; each
+   instance has its own damped-Newton state and PCG scalars.  stats: n_inst
+   entries or NULL.  No counterpart in geogram (one OptimalTransportMap3d at a
+   time). */
+int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double* w,
+                       const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
+                       uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats);
+
 /* ---- learned warm start of vgc_ot_solve (no counterpart in geogram) ----
    MLP 6 -> 64 -> 64 -> 1, leaky ReLU, FP32; VGC_MLP_PARAMS floats laid out
    W1[6][64], b1[64], W2[64][64], b2[64], W3[64], b3.  Features of a seed:

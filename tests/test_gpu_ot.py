@@ -36,3 +36,20 @@ def test_gpu_ot_one_million_uniform(gpu_ctx):
     err = ((r.vol - nu).abs() / nu).max().item()
     assert err <= 1e-4
     assert abs(r.vol.sum().item() - 1.0) < 1e-12
+
+
+def test_gpu_ot_batch_matches_emulation(gpu_ctx, emu_ctx):
+    from geogram_amd import warmstart
+    sizes = [1200, 600, 2000, 300]
+    xs, nus = [], []
+    for i, s in enumerate(sizes):
+        x, nu = warmstart.make_instance(70 + i, s)
+        xs.append(x)
+        nus.append(nu)
+    x, nu = np.vstack(xs), np.concatenate(nus)
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    wg, sg = gpu_ctx.ot_solve_batch(x, nu, sizes, boxes, tol=1e-5)
+    we, se = emu_ctx.ot_solve_batch(x, nu, sizes, boxes, tol=1e-5)
+    assert [s["newton_iters"] for s in sg] == [s["newton_iters"] for s in se]
+    assert [s["cg_iters"] for s in sg] == [s["cg_iters"] for s in se]
+    assert np.array_equal(wg, we)

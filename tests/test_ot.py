@@ -120,3 +120,29 @@ def test_ot_fallback_start_and_argument_checks(emu_ctx):
     assert st["used_fallback_start"] == 1 and st["converged"] == 1
     with pytest.raises(ga.VgcError):
         emu_ctx.ot_solve(x, nu * 2, tol=1e-5)  # masses do not sum to the volume
+
+
+def test_ot_batch_equals_single_solves(emu_ctx):
+    from geogram_amd import warmstart
+    sizes = [1500, 800, 2500]
+    xs, nus = [], []
+    for i, s in enumerate(sizes):
+        x, nu = warmstart.make_instance(50 + i, s)
+        xs.append(x)
+        nus.append(nu)
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * 3, dtype=float)
+    # the second instance in another box (masses scaled to its volume)
+    boxes[1] = [1.0, -1.0, 0.0, 3.0, 0.0, 0.5]
+    xs[1] = boxes[1, :3] + xs[1] * (boxes[1, 3:] - boxes[1, :3])
+    nus[1] = nus[1] * 1.0
+    x, nu = np.vstack(xs), np.concatenate(nus)
+    w, sts = emu_ctx.ot_solve_batch(x, nu, sizes, boxes, tol=1e-5)
+    o = 0
+    for k, s in enumerate(sizes):
+        wk, sk = emu_ctx.ot_solve(x[o:o + s], nu[o:o + s], box=boxes[k], tol=1e-5)
+        assert sts[k]["converged"] == 1 and sts[k]["newton_iters"] == sk["newton_iters"]
+        a, b = w[o:o + s] - w[o:o + s].mean(), wk - wk.mean()
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+        o += s
+    with pytest.raises(ga.VgcError):
+        emu_ctx.ot_solve_batch(x, nu * 2, sizes, boxes, tol=1e-5)
