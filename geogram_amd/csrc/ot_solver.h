@@ -228,9 +228,11 @@ public:
         std::vector<double> ms(3 * (size_t)K), tk(K), err(K), g2(K), eps0(K);
         std::vector<int> sel(K), bad;
         int passes = 0;
+        std::vector<int> compute;  // instances whose cells a pass computes
         auto cells = [&](const double* w) -> int {
             auto tc = std::chrono::steady_clock::now();
             pb.d_w = w;
+            pb.active = compute.empty() ? nullptr : &compute;
             int rc = E_.cells(pb, es);
             ++passes;
             bad = E_.inst_bad;
@@ -310,6 +312,7 @@ public:
                 bool anyp = false;
                 for (int k = 0; k < K; ++k) anyp = anyp || pending[k];
                 if (!anyp) break;
+                compute = pending;  // the other instances keep their cells
                 be_.h2d(d_t, tk.data(), sizeof(double) * K);
                 be_.seg_axpy(n, d_inst, d_w, d_t, d_x, d_wt);
                 if (cells(d_wt)) return -1;

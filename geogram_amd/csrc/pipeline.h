@@ -64,6 +64,9 @@ struct Problem {
     bool count_work = false;        // algorithmic work counters (global atomics)
     bool abort_on_empty = false;    // stop after the first pass if a cell is empty
                                     // (cells() then returns 1; outputs unset)
+    // per instance, nonzero = compute its cells; the others keep the outputs
+    // (volumes, centroids, neighbours) of the last pass that computed them
+    const std::vector<int>* active = nullptr;
 };
 
 template <class B>
@@ -239,7 +242,15 @@ public:
         unsigned long long* d_ctr = (unsigned long long*)slot(S_CTR, sizeof(unsigned long long) * 8);
         be_.dzero(d_ctr, sizeof(unsigned long long) * 8);
         be_.dzero(d_flags, sizeof(uint32_t) * (size_t)n_);
-        be_.dzero(d_cnt, sizeof(int) * (size_t)n_);
+        // a subset of instances: the others keep their neighbour counts
+        std::vector<int> aitems;
+        if (pb.active) {
+            for (int k = 0; k < ni_; ++k)
+                if ((*pb.active)[k])
+                    for (int c = h_p0_[k]; c < h_p0_[k] + h_n_[k]; ++c) aitems.push_back(c);
+        } else {
+            be_.dzero(d_cnt, sizeof(int) * (size_t)n_);
+        }
 
         ClipParams P{};
         P.pts = d_pts;
@@ -267,6 +278,12 @@ public:
         auto t2 = now();
         P.items = nullptr;
         P.n_items = n_;
+        if (pb.active) {
+            int* d_ai = (int*)slot(S_AITEMS, sizeof(int) * std::max<size_t>(1, aitems.size()));
+            if (!aitems.empty()) be_.h2d(d_ai, aitems.data(), sizeof(int) * aitems.size());
+            P.items = d_ai;
+            P.n_items = (int)aitems.size();
+        }
         P.list = d_list;
         P.list_ptr = nullptr;
         P.list_len = nullptr;
@@ -274,7 +291,7 @@ public:
         P.list_complete = 0;
         P.ring_limit = kMaxFilteredRings;
         be_.event_begin(1);
-        be_.clip(1, false, gather, P);
+        if (P.n_items > 0) be_.clip(1, false, gather, P);
         be_.event_end(1);
         be_.sync();
         st.t_clip_ms += ms_since(t2);
@@ -287,7 +304,7 @@ public:
             st.a_new += c[3];
             st.a_tets += c[4];
             st.a_facets += c[5];
-            st.a_cells += n_;
+            st.a_cells += P.n_items;
         }
         auto t3 = now();
         h_flags.resize(n_);
@@ -368,9 +385,13 @@ public:
         be_.d2h(ctr, d_ctr, sizeof(ctr));
         st.nb_clips += ctr[0];
         st.nb_tests += ctr[1];
-        st.nb_cells += n_;
+        st.nb_cells += P.n_items;
         inst_bad.assign(ni_, 0);
         for (int k = 0, c = 0; k < ni_; ++k) {
+            if (pb.active && !(*pb.active)[k]) {
+                c = h_p0_[k] + h_n_[k];
+                continue;
+            }
             for (const int e = h_p0_[k] + h_n_[k]; c < e; ++c) {
                 uint32_t f = h_flags[c];
                 if (!(f & CELL_DONE)) st.nb_failed++;
@@ -419,7 +440,7 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
