@@ -215,7 +215,7 @@ class Context:
         return w, st.as_dict()
 
     def ot_solve_batch(self, points, nu, inst_n, boxes, w0=None, tol=1e-4, max_newton=100,
-                       verbose=False):
+                       verbose=False, autotune=False):
         """Independent transport problems at once (consecutive seeds, one box
         each; each instance's nu sums to its box volume).  Returns (w, list of
         per-instance stats)."""
@@ -235,7 +235,8 @@ class Context:
         if ia.sum() != pts.shape[0] or len(ba) != len(ia):
             raise ValueError("inst_n must add up to the seed count, one box per instance")
         st = (OtStats * len(ia))()
-        flags = (VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_OT_VERBOSE if verbose else 0)
+        flags = ((VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_OT_VERBOSE if verbose else 0)
+                 | (_lib.VGC_AUTOTUNE if autotune else 0))
         rc = self.lib.fn("ot_solve_batch")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w),
                                            ia.ctypes.data_as(ctypes.c_void_p), len(ia),
                                            ba.ctypes.data_as(ctypes.c_void_p), float(tol),
@@ -306,10 +307,11 @@ class Context:
         self._check(rc, allow_partial=False)
 
     # ---------------------------------------------------------------- Lloyd
-    def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None):
+    def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None, autotune=False):
         """iters Lloyd iterations; returns the new points (same kind of array).
         inst_n / boxes: several independent instances at once (consecutive
-        seeds, one box each)."""
+        seeds, one box each).  autotune: pick the first pass's work-group size
+        by timing (results do not depend on it)."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
@@ -317,7 +319,7 @@ class Context:
         else:
             x = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
         st = Stats()
-        flags = VGC_DEVICE_PTRS if dev else 0
+        flags = (VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_AUTOTUNE if autotune else 0)
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)
             rc = self.lib.fn("lloyd")(self.ctx, _ptr(x), x.shape[0],

@@ -78,6 +78,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->why_walk = e.why_walk;
     s->why_narrow = e.why_narrow;
     s->nb_regather = e.nb_regather;
+    s->tuned_waves = (uint64_t)e.tuned_waves;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
@@ -139,6 +140,7 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
             }
         }
         EngineStats es;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->eng.cells(pb, es)) {
             ctx->err = "vgc_cells: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -290,6 +292,7 @@ int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n
             be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
         }
         EngineStats es;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->eng.lloyd(pb, dx, (int)iters, es)) {
             ctx->err = "vgc_lloyd: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -365,6 +368,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         pb.d_xyz = dx;
         OtStats os;
         const uint32_t maxn = max_newton ? max_newton : 100;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                           (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0)) {
             ctx->err = "vgc_ot_solve: " + ctx->eng.error;
@@ -475,6 +479,7 @@ int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double*
         pb.d_xyz = dx;
         std::vector<OtStats> os;
         const uint32_t maxn = max_newton ? max_newton : 100;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                                 (flags & VGC_OT_VERBOSE) != 0)) {
             ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;

@@ -960,7 +960,10 @@ struct HipBackend {
     void clip_egw(int S, const ClipParams& P) {
         const unsigned g4 = blocks_for(P.n_items, 4), g1 = blocks_for(P.n_items, 1);
         if (S == 1) {
-            k_clip<1, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
+            if (waves_ == 2)
+                k_clip<1, 2, E, G, W><<<blocks_for(P.n_items, 2), 128, 0, stream_>>>(P);
+            else
+                k_clip<1, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
         } else if (S == 2) {
             k_clip<2, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
         } else if (S == 4) {
@@ -980,6 +983,10 @@ struct HipBackend {
         else
             clip_egw<E, G, false>(S, P);
     }
+    // wavefronts per work-group of the S = 1 kernels (2 or 4); the cells'
+    // results do not depend on it
+    void set_clip_waves(int w) { waves_ = w == 2 ? 2 : 4; }
+    int clip_waves() const { return waves_; }
     void clip(int S, bool exact, bool gather, const ClipParams& P) {
         if (P.n_items <= 0) return;
         if (exact) {
@@ -1242,6 +1249,7 @@ private:
     const void* cg_key_[12] = {};
     double* seg_part_ = nullptr;
     int seg_part_cap_ = 0;
+    int waves_ = 4;
 };
 
 typedef Ctx<HipBackend> HipCtx;

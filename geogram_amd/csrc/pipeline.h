@@ -49,6 +49,7 @@ struct EngineStats {
     unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
     unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
     unsigned long long nb_regather = 0;  // weighted cells gathered again without the ring limit
+    int tuned_waves = 0;                 // autotuner's choice (0: not tuned by this call)
 };
 
 struct Problem {
@@ -88,6 +89,7 @@ public:
     double* d_area = nullptr;
     long long total_neighbors = 0;
     long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
+    bool autotune = false;             // pick the first pass's work-group size by timing
     std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
     std::vector<uint32_t> h_flags;  // by Morton position
 
@@ -290,6 +292,27 @@ public:
         P.list_stride = K0;
         P.list_complete = 0;
         P.ring_limit = kMaxFilteredRings;
+        if (autotune && P.n_items > 0 && tuned_n_ != n_) {
+            // autotuner: time the first pass with 2 and 4 wavefronts per
+            // work-group (same results) and keep the faster for this size
+            double best = 1e300;
+            int best_w = 4;
+            for (int wv : {2, 4}) {
+                be_.set_clip_waves(wv);
+                be_.event_begin(1);
+                be_.clip(1, false, gather, P);
+                be_.event_end(1);
+                be_.sync();
+                const double ms = be_.event_ms(1);
+                if (ms < best) {
+                    best = ms;
+                    best_w = wv;
+                }
+            }
+            be_.set_clip_waves(best_w);
+            tuned_n_ = n_;
+            st.tuned_waves = best_w;
+        }
         be_.event_begin(1);
         if (P.n_items > 0) be_.clip(1, false, gather, P);
         be_.event_end(1);
@@ -554,6 +577,7 @@ private:
     int n_ = 0, ni_ = 1, max_h_ = 0;
     long long n_nodes_ = 0;
     bool weighted_ = false;
+    int tuned_n_ = -1;
     double* d_box_ = nullptr;
     double* d_wmax_ = nullptr;
     std::vector<int> h_p0_, h_n_, h_b0_, h_nb_, h_L_;

@@ -38,6 +38,9 @@ extern "C" {
                                  the start with the smaller mass residual */
 #define VGC_OT_VERBOSE 16u    /* vgc_ot_solve: one progress line per Newton step
                                  on stderr */
+#define VGC_AUTOTUNE 32u      /* time the first clipping pass with 2 and 4 wavefronts
+                                 per work-group on the first call for a problem size
+                                 and keep the faster (results do not depend on it) */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -68,6 +71,7 @@ typedef struct vgc_stats {
        could not be narrowed */
     uint64_t why_rings, why_walk, why_narrow;
     uint64_t nb_regather; /* weighted cells gathered again without the ring limit */
+    uint64_t tuned_waves; /* VGC_AUTOTUNE: wavefronts per work-group chosen (0: none) */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in
