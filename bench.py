@@ -18,7 +18,13 @@ The JSON line also carries:
                 stream, against the FP64 vector peak of MI355X (78.6 TFLOP/s);
   cpu_baseline  the oracle (a plain-C restatement of geogram's VBW cell +
                 security-radius loop with exact predicates) on the host cores,
-                on a bounded sample of the same workload.
+                on a bounded sample of the same workload;
+  ot            semi-discrete transport, 1M uniform seeds, equal masses, to
+                1e-4 max relative mass error from w = 0;
+  ot_warm_start 1M 16-Gaussian seeds with log-normal masses: plain start
+                against the learned MLP start (features + forward timed);
+  batched       256 independent instances (N log-uniform in [2k, 200k]):
+                20 batched Lloyd steps and batched transport to 1e-4.
 """
 from __future__ import annotations
 
@@ -47,6 +53,9 @@ def main():
                     help="seeds of the semi-discrete OT leg (0: skip)")
     ap.add_argument("--ot4-n", type=int, default=1_000_000,
                     help="seeds of the learned-warm-start OT leg (0: skip)")
+    ap.add_argument("--c5-instances", type=int, default=256,
+                    help="batched leg: independent instances, N log-uniform in [2k, 200k] (0: skip)")
+    ap.add_argument("--c5-ot", type=int, default=1, help="batched leg: also the transport solves")
     ap.add_argument("--cpu-sample", type=int, default=400_000,
                     help="cells of the workload timed on the host with the oracle (0: skip)")
     args = ap.parse_args()
@@ -154,6 +163,47 @@ def main():
         else:
             ws["learned"] = None
 
+    # batched uneven instances: Lloyd (20 steps) and transport to 1e-4 for
+    # every instance at once; even instances uniform, odd ones 16-Gaussian;
+    # masses equal for k % 4 < 2, log-normal otherwise
+    c5 = None
+    if args.c5_instances > 0:
+        K = args.c5_instances
+        rng = np.random.default_rng(5)
+        sizes = np.exp(rng.uniform(np.log(2000), np.log(200000), K)).astype(np.int64)
+        xs, nus = [], []
+        for k, sz in enumerate(sizes):
+            xs.append(ga.sample_points(0 if k % 2 == 0 else 2, 5000 + k, int(sz)))
+            if k % 4 < 2:
+                nuk = np.full(int(sz), 1.0 / sz)
+            else:
+                nuk = np.exp(np.random.default_rng(k).normal(0.0, 0.5, int(sz)))
+                nuk /= nuk.sum()
+            nus.append(nuk)
+        x5 = torch.from_numpy(np.vstack(xs)).cuda()
+        nu5 = torch.from_numpy(np.concatenate(nus)).cuda()
+        boxes5 = np.array([[0, 0, 0, 1, 1, 1]] * K, dtype=float)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        _, l5 = ctx.lloyd(x5, 20, inst_n=sizes, boxes=boxes5, autotune=True)
+        torch.cuda.synchronize()
+        tl5 = time.perf_counter() - t2
+        c5 = {"instances": K, "seeds": int(sizes.sum()),
+              "lloyd": {"steps": 20, "time_s": round(tl5, 3),
+                        "cells_per_s": round(20 * float(sizes.sum()) / tl5, 1),
+                        "failed": int(l5["nb_failed"]), "tuned_waves": int(l5["tuned_waves"])}}
+        if args.c5_ot:
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            _, s5 = ctx.ot_solve_batch(x5, nu5, sizes, boxes5, tol=1e-4, verbose=True,
+                                       autotune=True)
+            torch.cuda.synchronize()
+            to5 = time.perf_counter() - t2
+            its = [s["newton_iters"] for s in s5]
+            c5["ot"] = {"time_s": round(to5, 3), "converged": int(sum(s["converged"] for s in s5)),
+                        "newton_max": int(max(its)), "newton_mean": round(float(np.mean(its)), 2),
+                        "cell_passes": int(s5[0]["cell_passes"])}
+
     cpu = None
     if args.cpu_sample > 0:
         import oracle
@@ -196,6 +246,7 @@ def main():
         "cpu_baseline": cpu,
         "ot": ot,
         "ot_warm_start": ws,
+        "batched": c5,
         "checks": {"sum_vol_minus_1": sumvol_err, "failed": int(s["nb_failed"]),
                    "uncertain": int(s["nb_uncertain"])},
         "phases_ms": {k: round(s[k], 3) for k in ("t_build_ms", "t_knn_ms", "t_clip_ms",
