@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--c5-instances", type=int, default=256,
                     help="batched leg: independent instances, N log-uniform in [2k, 200k] (0: skip)")
     ap.add_argument("--c5-ot", type=int, default=1, help="batched leg: also the transport solves")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="one 256 MiB device-to-device copy before the steps, the known-size "
+                         "kernel that scales HBM counters under rocprofv3 --pmc")
     ap.add_argument("--cpu-sample", type=int, default=400_000,
                     help="cells of the workload timed on the host with the oracle (0: skip)")
     args = ap.parse_args()
@@ -78,6 +81,12 @@ def main():
     # algorithmic work of the first clipping pass (deterministic for the
     # workload), counted once outside the timed steps
     work = ctx.cells(xd, k0=args.k0, count_work=True).stats
+    if args.copy_calib:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         r = ctx.cells(xd, k0=args.k0)
     torch.cuda.synchronize()

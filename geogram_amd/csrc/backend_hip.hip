@@ -186,15 +186,21 @@ __global__ void k_rebase(const long long* __restrict__ off, int n, long long bas
     if (t < n) out[t] = off[t] - base;
 }
 
+// 16 lanes per cell: each cell's row of the per-cell staging (stride
+// entries) is read and its CSR range written in contiguous 64-byte runs
+constexpr int kCompactLanes = 16;
 __global__ void k_compact(int n, const int* __restrict__ cnt, const long long* __restrict__ ptr,
                           const int* __restrict__ nbr_tmp, const double* __restrict__ area_tmp,
                           int stride, int* nbr, double* area) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = t / kCompactLanes;
     if (i >= n) return;
-    long long o = ptr[i];
-    for (int k = 0; k < cnt[i]; ++k) {
-        nbr[o + k] = nbr_tmp[(size_t)i * stride + k];
-        area[o + k] = area_tmp[(size_t)i * stride + k];
+    const long long o = ptr[i];
+    const int m = cnt[i];
+    const size_t row = (size_t)i * stride;
+    for (int k = int(t % kCompactLanes); k < m; k += kCompactLanes) {
+        nbr[o + k] = nbr_tmp[row + k];
+        area[o + k] = area_tmp[row + k];
     }
 }
 
@@ -1053,8 +1059,8 @@ struct HipBackend {
     }
     void compact(int n, const int* cnt, const long long* ptr, const int* nbr_tmp,
                  const double* area_tmp, int stride, int* nbr, double* area) {
-        k_compact<<<blocks_for(n, 256), 256, 0, stream_>>>(n, cnt, ptr, nbr_tmp, area_tmp, stride,
-                                                           nbr, area);
+        k_compact<<<blocks_for((long long)n * kCompactLanes, 256), 256, 0, stream_>>>(
+            n, cnt, ptr, nbr_tmp, area_tmp, stride, nbr, area);
         check();
     }
     void lloyd_update(double* xyz, const double* vol, const double* mg, int n) {

@@ -877,8 +877,8 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
     return cnt;
 }
 
-// Ball around the partial cell of a weighted seed for the power-distance
-// filter (types.h): centre = middle of the corners' bounding box, radius =
+// Ball around the partial cell of a seed for the power-distance filter
+// (types.h; all weights zero for Voronoi cells): centre = middle of the corners' bounding box, radius =
 // farthest corner plus the corner's certified position error
 // (|dN| <= g nmax, |dD| <= g dabs).  Returns false (no filter) when a corner
 // is too close to degenerate for that bound.
@@ -1033,8 +1033,9 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                           ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
             if (W && F.on && ++nfilt > P.ring_limit) {
-                // the list tiers take the cell, with the same filter and no
-                // capacity limit
+                // weighted cells: the list tiers take the cell, with the same
+                // filter and no capacity limit (Voronoi cells keep gathering:
+                // deferring them measured slower)
                 R.flags |= CELL_NEED_K | CELL_WHY_RINGS;
                 break;
             }
@@ -1077,7 +1078,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 break;
             }
             rlo2 = rhi2;
-            if (W && make_filter<S>(R, F)) {
+            if (make_filter<S>(R, F)) {
                 // every seed beyond the ring, within the max-weight radius,
                 // that passes the power filter of the cell as it is now
                 // (narrowed above if too many)
@@ -1112,7 +1113,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         // the partial cell contains the cell: its ball filters the list tiers
         PowerFilter G{};
-        const bool okf = W && P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
+        const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
         if (lane0()) {
             P.flags[c] = flags;
             P.r2[c] = r2max;

@@ -270,7 +270,7 @@ public:
         P.face_stride = kFaceStride;
         P.flags = d_flags;
         P.r2 = d_r2;
-        P.ball = weighted_ ? (fball*)slot(S_BALL, sizeof(fball) * (size_t)n_) : nullptr;
+        P.ball = (fball*)slot(S_BALL, sizeof(fball) * (size_t)n_);
         P.counters = pb.count_work ? d_ctr : nullptr;
         P.pf = d_pf;
         P.nodes = d_nodes;
@@ -292,16 +292,39 @@ public:
         P.list_stride = K0;
         P.list_complete = 0;
         P.ring_limit = kMaxFilteredRings;
+        if (!autotune) {
+            // without the tuner every call runs the default work-group size
+            be_.set_clip_waves(4);
+            tuned_n_ = -1;
+        }
         if (autotune && P.n_items > 0 && tuned_n_ != n_) {
             // autotuner: time the first pass with 2 and 4 wavefronts per
             // work-group (same results) and keep the faster for this size
             // (interleaved, twice each, minimum kept: the first launch of a
-            // problem runs on cold caches)
+            // problem runs on cold caches).  Large problems are timed on a
+            // sample of 128 evenly spaced runs of 2048 consecutive cells, so
+            // that tuning costs about half of one full pass.
+            ClipParams T = P;
+            T.counters = nullptr;
+            const int kRun = 2048, kRuns = 128;
+            if (P.n_items > 2 * kRun * kRuns) {
+                std::vector<int> titems;
+                titems.reserve((size_t)kRun * kRuns);
+                for (int j = 0; j < kRuns; ++j) {
+                    const long long b = (long long)P.n_items * j / kRuns;
+                    for (int i = 0; i < kRun; ++i)
+                        titems.push_back(pb.active ? aitems[b + i] : (int)(b + i));
+                }
+                int* d_t = (int*)slot(S_TITEMS, sizeof(int) * titems.size());
+                be_.h2d(d_t, titems.data(), sizeof(int) * titems.size());
+                T.items = d_t;
+                T.n_items = (int)titems.size();
+            }
             double t2w = 1e300, t4w = 1e300;
             for (int wv : {4, 2, 4, 2}) {
                 be_.set_clip_waves(wv);
                 be_.event_begin(1);
-                be_.clip(1, false, gather, P);
+                be_.clip(1, false, gather, T);
                 be_.event_end(1);
                 be_.sync();
                 const double ms = be_.event_ms(1);
@@ -352,9 +375,9 @@ public:
             st.t_total_ms += ms_since(tstart);
             return 1;
         }
-        if (weighted_ && gather) {
-            // weighted cells that used up the first pass's ring limit: gather
-            // again without the limit (a wave per cell) before any list tier
+        if (gather) {
+            // cells that used up the first pass's ring limit: gather again
+            // without the limit (a wave per cell) before any list tier
             std::vector<int> again, rest;
             for (int c : need) (h_flags[c] & CELL_WHY_RINGS ? again : rest).push_back(c);
             if (!again.empty()) {
@@ -463,7 +486,7 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_AITEMS, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
@@ -517,8 +540,8 @@ private:
     }
 
     // Complete lists for `items`: every seed within the distance that can
-    // still cut the partial cell of the previous pass (weighted: that also
-    // passes the power filter of its ball), in chunks of at most list_chunk
+    // still cut the partial cell of the previous pass and that passes the
+    // (power) filter of its ball, in chunks of at most list_chunk
     // entries (or one item), each sorted by distance and clipped in list mode.
     int radius_pass(int S, bool exact, const std::vector<int>& items, ClipParams P,
                     EngineStats& st) {
@@ -526,8 +549,7 @@ private:
         int* d_items = (int*)slot(S_ITEMS, sizeof(int) * ni);
         be_.h2d(d_items, items.data(), sizeof(int) * ni);
         float* d_rho2 = (float*)slot(S_RHO2, sizeof(float) * ni);
-        PowerFilter* d_filt =
-            weighted_ ? (PowerFilter*)slot(S_FILT, sizeof(PowerFilter) * ni) : nullptr;
+        PowerFilter* d_filt = P.ball ? (PowerFilter*)slot(S_FILT, sizeof(PowerFilter) * ni) : nullptr;
         be_.radius_r2(d_items, ni, d_pts, P.r2, d_inst_of, d_wmax_, weighted_ ? 1 : 0, P.ball,
                       d_box_, d_rho2, d_filt);
         int* d_rcnt = (int*)slot(S_RCNT, sizeof(int) * ni);

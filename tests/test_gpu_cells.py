@@ -63,10 +63,13 @@ def test_voronoi_vs_oracle(gpu_ctx, kind, n, k0):
     assert abs(r.vol.sum() - 1.0) < 1e-12
 
 
-def test_gpu_matches_emulation_bitwise(gpu_ctx, emu_ctx):
+@pytest.mark.parametrize("kind,n", [(1, 3000), (2, 2000)])
+def test_gpu_matches_emulation_bitwise(gpu_ctx, emu_ctx, kind, n):
+    # kind 2 (16 tight clusters, few seeds): large cells beside the clusters,
+    # many filtered rings per cell
     import geogram_amd as ga
 
-    x = ga.sample_points(1, 5, 3000)
+    x = ga.sample_points(kind, 5, n)
     g = gpu_ctx.cells(x, k0=0)
     e = emu_ctx.cells(x, k0=0)
     assert np.array_equal(g.vol, e.vol)
