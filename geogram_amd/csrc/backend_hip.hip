@@ -138,11 +138,22 @@ __global__ __launch_bounds__(256) void k_knn(const pf4* __restrict__ pf,
 
 // S = 1 (every cell's first pass): registers capped for 3 waves per SIMD
 // (latency-bound kernel; occupancy hides the dependent LDS round trips)
+// Work-groups go to the 8 XCDs of the chip in turn; renumber them so that
+// each XCD takes one contiguous range of work-groups (cells in Morton order:
+// neighbouring cells read the same seeds and tree nodes, from that XCD's L2).
+// A bijection of [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 template <int S, int WAVES, bool EXACT, bool GATHER, bool W>
 __global__ __launch_bounds__(64 * WAVES, S == 1 ? 3 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
-    const int item = blockIdx.x * WAVES + wave;
+    const int item = xcd_block() * WAVES + wave;
     if (item >= P.n_items) return;  // whole wavefront
     process_cell<S, EXACT, GATHER, W>(P, item, lds[wave]);
 }

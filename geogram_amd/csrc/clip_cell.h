@@ -121,14 +121,26 @@ struct CellLds {
     uint32_t job[64 * S];  // new corner jobs: a | b << 8 | across << 16
     uint32_t cpl[64 * S];  // corner planes (integration walk)
     uint32_t cadj[64 * S]; // corner adjacency (integration walk)
-    double q[64 * S][3];   // corner points (integration)
     typename CornerBits<S>::id_t st_tab[kPMax];  // new corner whose first plane is p
     typename CornerBits<S>::id_t en_tab[kPMax];  // new corner whose second plane is p
-    // gather mode: candidates of the current distance ring, sorted by
-    // (FP32 squared distance, Morton position), and the tree walk buffers
-    uint64_t cand[kCandCap];
-    int front[2][kFrontCap];
-    int bucket[kBucketCap];
+    union {
+        // corner points (integration, after the last ring)
+        double q[64 * S][3];
+        // gather mode: candidates of the current distance ring, sorted by
+        // (FP32 squared distance, Morton position), and the tree walk buffers
+        struct {
+            uint64_t cand[kCandCap];
+            union {
+                struct {
+                    int front[2][kFrontCap];
+                    int bucket[kBucketCap];
+                };
+                // positions of the next 64 candidates, loaded together
+                // before they are clipped one by one
+                pt4 stage[kWave];
+            };
+        };
+    };
 };
 
 // ---------------------------------------------------------------- helpers
@@ -509,9 +521,8 @@ struct CellRun {
 // coincident-seed rule, the security-radius stop, then the clip.  Returns
 // true when the cell is finished (or failed) and the candidate loop must end.
 template <int S, bool EXACT, bool W>
-VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j) {
+VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j, const pt4& pj) {
     if (j == R.c) return false;
-    const pt4 pj = P.pts[j];
     const double ex = pj.x - R.pc.x, ey = pj.y - R.pc.y, ez = pj.z - R.pc.z;
     const double d2 = ex * ex + ey * ey + ez * ez;
     if (d2 == 0.0) {
@@ -945,6 +956,21 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
     return true;
 }
 
+// Candidates base .. base+63 of the sorted ring (those below cnt): one load
+// per lane of their FP64 positions into L.stage, so that the clipping loop
+// reads them from the LDS instead of waiting on a global load per candidate.
+template <int S>
+VGC_HD void stage_cand(const ClipParams& P, CellLds<S>& L, int base, int cnt) {
+    const LI k = base + lane_id();
+    const LB ok = k < cnt;
+    const LU64 key = lds_ld(L.cand, select(ok, k, LI(0)));
+    const LI j = cvt<int>(cvt<uint32_t>(key));
+    const Lane<pt4> q = gld(P.pts, select(ok, j, LI(0)), ok);
+    wave_sync();  // the previous batch is consumed
+    lds_st(L.stage, lane_id(), q, ok);
+    wave_sync();
+}
+
 // Security-radius loop + integration for work item `item`.  GATHER: the
 // wave finds the neighbours itself, ring after ring of distance; otherwise
 // it reads them from P.list.
@@ -991,7 +1017,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 complete = true;  // short list: the instance has fewer seeds
                 break;
             }
-            if (consider<S, EXACT, W>(P, L, R, j)) break;
+            if (consider<S, EXACT, W>(P, L, R, j, P.pts[j])) break;
         }
     } else {
         const pf4 pq = P.pf[c];
@@ -1059,8 +1085,9 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             sort_cand<S>(L, cnt);
             bool done = false;
             for (int jj = 0; jj < cnt; ++jj) {
+                if ((jj & (kWave - 1)) == 0) stage_cand<S>(P, L, jj, cnt);
                 const int j = int(uint32_t(L.cand[jj]));
-                if (consider<S, EXACT, W>(P, L, R, j)) {
+                if (consider<S, EXACT, W>(P, L, R, j, L.stage[jj & (kWave - 1)])) {
                     done = true;
                     break;
                 }
