@@ -5,7 +5,8 @@ Workload (N=1): 2,000,000 seeds in [0,1]^3, 1M uniform + 1M from an
 rejection to the cube), seed 2.  One step = one full pass of the cell path on
 seeds already resident in HBM: Morton sort, tree, kNN, clipping with the
 security radius (all tiers), FP64 volume / centroid / facet-area integration
-and the neighbour CSR.  Steps are timed with torch.cuda.synchronize() on both
+and the neighbour CSR (the first pass gathers its neighbours itself: no kNN
+pass at k0 = 0).  Steps are timed with torch.cuda.synchronize() on both
 sides.
 
 The JSON line also carries:
@@ -16,6 +17,10 @@ The JSON line also carries:
                 integration tetrahedron, 20 per facet), divided by its
                 duration from HIP events recorded on the library's launch
                 stream, against the FP64 vector peak of MI355X (78.6 TFLOP/s);
+                traffic (HBM bytes per launch) and valu_busy_frac (share of
+                the VALU issue slots in use, the kernel's actual bound) are
+                read from the committed rocprofv3 counter summary
+                profiles/k_clip_counters_2M_mixture.md (default workload only);
   cpu_baseline  the oracle (a plain-C restatement of geogram's VBW cell +
                 security-radius loop with exact predicates) on the host cores,
                 on a bounded sample of the same workload;
@@ -39,6 +44,21 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+COUNTERS = os.path.join(ROOT, "profiles", "k_clip_counters_2M_mixture.md")
+
+
+def profile_figure(key):
+    """A `key: value` line of the rocprofv3 counter summary of the clip
+    kernel (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE scaled by a
+    known-size copy, VALU issue fraction from SQ counters), or None."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -113,7 +133,9 @@ def main():
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-        "traffic": None,
+        "traffic": profile_figure("traffic_bytes_per_launch") if args.n == 2_000_000 and args.k0 == 0 else None,
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "valu_busy_frac": profile_figure("valu_busy_frac") if args.n == 2_000_000 and args.k0 == 0 else None,
         "alg_flops_per_launch": int(flops_a),
         "flops_per_cell": round(flops_a / n, 1),
         "launch_ms_hip_events": round(clip_ms, 3),
