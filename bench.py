@@ -3,10 +3,10 @@
 Workload (N=1): 2,000,000 seeds in [0,1]^3, 1M uniform + 1M from an
 8-component isotropic Gaussian mixture (means U[0.2,0.8]^3, sigma U[0.01,0.1],
 rejection to the cube), seed 2.  One step = one full pass of the cell path on
-seeds already resident in HBM: Morton sort, tree, kNN, clipping with the
-security radius (all tiers), FP64 volume / centroid / facet-area integration
-and the neighbour CSR (the first pass gathers its neighbours itself: no kNN
-pass at k0 = 0).  Steps are timed with torch.cuda.synchronize() on both
+seeds already resident in HBM: Morton sort, tree, clipping with the security
+radius (all tiers; the first pass gathers each cell's neighbours by its own
+tree walk, so there is no separate kNN pass at k0 = 0), FP64 volume /
+centroid / facet-area integration and the neighbour CSR.  Steps are timed with torch.cuda.synchronize() on both
 sides.
 
 The JSON line also carries:
