@@ -233,7 +233,8 @@ def main():
             its = [s["newton_iters"] for s in s5]
             c5["ot"] = {"time_s": round(to5, 3), "converged": int(sum(s["converged"] for s in s5)),
                         "newton_max": int(max(its)), "newton_mean": round(float(np.mean(its)), 2),
-                        "cell_passes": int(s5[0]["cell_passes"])}
+                        "cell_passes": int(s5[0]["cell_passes"]),
+                        "t_cells_ms": round(s5[0]["t_cells_ms"], 1), "t_cg_ms": round(s5[0]["t_cg_ms"], 1)}
 
     cpu = None
     if args.cpu_sample > 0:

@@ -148,6 +148,8 @@ struct EmuBackend {
             for (int it = 0; it < P.n_items; ++it) {
                 if (P.weighted)
                     process_cell<S, E, G, true>(P, it, *L);
+                else if (G && P.corner)
+                    process_cell<S, E, G, false, true>(P, it, *L);
                 else
                     process_cell<S, E, G, false>(P, it, *L);
             }

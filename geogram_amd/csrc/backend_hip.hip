@@ -149,13 +149,13 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < r ? x : r) + b / kXcds;
 }
 
-template <int S, int WAVES, bool EXACT, bool GATHER, bool W>
+template <int S, int WAVES, bool EXACT, bool GATHER, bool W, bool CORNER = W>
 __global__ __launch_bounds__(64 * WAVES, S == 1 ? 3 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = xcd_block() * WAVES + wave;
     if (item >= P.n_items) return;  // whole wavefront
-    process_cell<S, EXACT, GATHER, W>(P, item, lds[wave]);
+    process_cell<S, EXACT, GATHER, W, CORNER>(P, item, lds[wave]);
 }
 
 __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
@@ -977,6 +977,13 @@ struct HipBackend {
     void clip_egw(int S, const ClipParams& P) {
         const unsigned g4 = blocks_for(P.n_items, 4), g1 = blocks_for(P.n_items, 1);
         if (S == 1) {
+            if constexpr (G && !W) {
+                if (P.corner) {
+                    // second gather of Voronoi cells, with the corner test
+                    k_clip<1, 4, E, G, W, true><<<g4, 256, 0, stream_>>>(P);
+                    return;
+                }
+            }
             if (waves_ == 2)
                 k_clip<1, 2, E, G, W><<<blocks_for(P.n_items, 2), 128, 0, stream_>>>(P);
             else

@@ -73,6 +73,10 @@ constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 320;  // tree nodes per level of the walk (two levels: the depth-first stack)
 constexpr int kBucketCap = 128; // buckets per ring
 constexpr int kMaxFilteredRings = 48;  // power-filtered rings of the first pass
+// Voronoi cells: filtered rings of the first pass; cells that need more go to
+// a second gather with the corner test (clustered seeds: large cells beside
+// dense clusters, whose balls hold thousands of seeds that cannot cut)
+constexpr int kMaxVoronoiRings = 16;
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
 
@@ -94,8 +98,9 @@ struct ClipParams {
     const int* list_len;        // per item length (nullptr: list_stride)
     int list_stride;
     int list_complete;      // lists hold every seed the cell can need
-    int ring_limit;         // gather mode: power-filtered rings before the list tiers
+    int ring_limit;         // gather mode: filtered rings before the cell is handed on
     int weighted;
+    int corner;             // gather mode, Voronoi cells: the corner-test kernel
     // outputs, indexed by the caller's seed index
     double* vol;
     double* mg;             // 3 per seed: integral of x over the cell
@@ -769,8 +774,84 @@ VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
     }
 }
 
+// FP64 lanes to FP32, rounded up.
+VGC_HD LF f_up_lanes(const LD& x) {
+#if defined(VGC_EMU)
+    LF r;
+    for (int l = 0; l < kWave; ++l) r[l] = f_up(x[l]);
+    return r;
+#else
+    return f_up(x);
+#endif
+}
+
+// Corners of the partial cell for the seed test of the depth-first rings
+// (S = 1: one corner per lane), relative to the seed in FP32: position and
+// r = |v|^2 - wi plus its share of the error margin, rounded up; delta bounds
+// the position error of a corner (FP64 rounding) and of a seed's FP32 copy.
+struct CornerTab {
+    LF x, y, z, r;
+    uint64_t live;
+    float delta;
+};
+
+// Can seed q (FP32, relative to the seed: e, squared length d2, weight wq)
+// cut the partial cell?  It can iff some corner v has
+// |v - q|^2 - wq < |v|^2 - wi; evaluated in FP32 against every live corner
+// with the margin 1e-5 (|v - q|^2 + |v|^2 + |q|^2 + |r| + |wq|)
+// + 4 (|v| + |q|) delta + 4 delta^2, which covers the rounding of the copies
+// and of this evaluation, so no seed that can cut is dropped.  Lanes outside
+// `want` may come back either way.
+VGC_HD LB corner_may_cut(const CornerTab& CT, const LF& ex, const LF& ey, const LF& ez,
+                         const LF& d2, const LF& wq, const LB& want) {
+    const LF qs = wq + 1e-5f * (vabsf(wq) + d2) + (4.0f * CT.delta) * vsqrtf(d2);
+    LB hit = LB(false);
+    uint64_t m = CT.live;
+    int k = 0;
+    while (m) {
+        const int v = ffs64(m);
+        m &= m - 1;
+        const float vx = readlane(CT.x, v), vy = readlane(CT.y, v), vz = readlane(CT.z, v),
+                    rv = readlane(CT.r, v);
+        const LF dx = ex - vx, dy = ey - vy, dz = ez - vz;
+        const LF dd = dx * dx + dy * dy + dz * dz;
+        hit = hit | (dd * (1.0f - 1e-5f) < qs + rv);
+        if ((++k & 7) == 0 && ballot(want & !hit) == 0) break;
+    }
+    return hit;
+}
+
+// The candidates L.cand[0..cnt) that pass the corner test, compacted in
+// place (order kept); returns their count.
+template <int S>
+VGC_HD int corner_filter_cand(const ClipParams& P, CellLds<S>& L, int cnt, float px, float py,
+                              float pz, const CornerTab& CT) {
+    static_assert(kCandCap == 2 * kWave, "two candidates per lane");
+    const LI lane = lane_id();
+    const LB in0 = lane < cnt, in1 = lane + kWave < cnt;
+    const LU64 k0 = lds_ld(L.cand, select(in0, lane, LI(0)));
+    const LU64 k1 = lds_ld(L.cand, select(in1, lane + kWave, LI(0)));
+    LB keep[2] = {in0, in1};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!any(keep[h])) continue;
+        const LI m = cvt<int>(cvt<uint32_t>(h ? k1 : k0));
+        const SeedLanes q = lane_seed(P.pf, select(keep[h], m, LI(0)), keep[h]);
+        const LF ex = q.x - px, ey = q.y - py, ez = q.z - pz;
+        const LF d2 = ex * ex + ey * ey + ez * ez;
+        keep[h] = keep[h] & corner_may_cut(CT, ex, ey, ez, d2, q.w, keep[h]);
+    }
+    wave_sync();  // every key read before the first write
+    const uint64_t b0 = ballot(keep[0]), b1 = ballot(keep[1]);
+    lds_st(L.cand, select(keep[0], mbcnt(b0), LI(0)), k0, keep[0]);
+    lds_st(L.cand, select(keep[1], popc64(b0) + mbcnt(b1), LI(0)), k1, keep[1]);
+    wave_sync();
+    return popc64(b0) + popc64(b1);
+}
+
 // Seeds q != c of the instance with rlo2 < d2 <= rhi2 (FP32 squared
-// distance from the seed) that pass the power filter F (if on), into L.cand as
+// distance from the seed) that pass the power filter F (if on) and, once a
+// ring of the cell has overflowed (ct_on), the corner test CT, into L.cand as
 // (d2 bits << 32 | q), unsorted.  Depth-first walk of the tree: a stack in
 // the wave's LDS, up to 64 nodes popped per step (one per lane), passing
 // leaves scanned at once, so the stack stays short however many nodes pass.
@@ -779,9 +860,10 @@ VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
 // below that distance, so the ring (rlo2, rhi2] it returns is complete.
 // Returns the count (<= kCandCap), or -1 when the stack overflows or the
 // nearest 64 are all at one distance.
-template <int S>
+template <int S, bool CORNER>
 VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float px, float py,
-                      float pz, float rlo2, float& rhi2, const PowerFilter& F) {
+                      float pz, float rlo2, float& rhi2, const PowerFilter& F,
+                      const CornerTab& CT, bool& ct_on) {
     constexpr int kStack = 2 * kFrontCap;
     const LI lane = lane_id();
     const int p0 = P.IL.p0[inst], n = P.IL.n[inst], nb = P.IL.nb[inst], Lv = P.IL.L[inst];
@@ -849,6 +931,18 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
             LF gc2 = gx * gx + gy * gy + gz * gz;
             LB take = ok & (d2 > rlo2) & (d2 <= lim);
             if (F.on) take = take & pf_may_cut(F, gc2, q.w, vsqrtf(d2), vabsf(ex), vabsf(ey), vabsf(ez));
+            if constexpr (CORNER) {
+                if (ct_on && CT.live && any(take))
+                    take = take & corner_may_cut(CT, ex, ey, ez, d2, q.w, take);
+                if (cnt + popc64(ballot(take)) > kCandCap && !ct_on && CT.live) {
+                    // the ring overflows: from here on, and in the cell's
+                    // later rings, seeds must also pass the corner test (the
+                    // buffer and this batch first)
+                    ct_on = true;
+                    cnt = corner_filter_cand<S>(P, L, cnt, px, py, pz, CT);
+                    if (any(take)) take = take & corner_may_cut(CT, ex, ey, ez, d2, q.w, take);
+                }
+            }
             if (cnt + popc64(ballot(take)) > kCandCap) {
                 // keep the seeds strictly nearer than the 64th nearest so far
                 sort_cand<S>(L, cnt);
@@ -893,9 +987,10 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
 // farthest corner plus the corner's certified position error
 // (|dN| <= g nmax, |dD| <= g dabs).  Returns false (no filter) when a corner
 // is too close to degenerate for that bound.
-template <int S>
-VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
+template <int S, bool CORNER>
+VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F, CornerTab& CT) {
     const CellState<S>& st = R.st;
+    CT.live = 0;
     LD lox = LD(INFINITY), loy = LD(INFINITY), loz = LD(INFINITY);
     LD hix = LD(-INFINITY), hiy = LD(-INFINITY), hiz = LD(-INFINITY);
     bool bad = false;
@@ -923,7 +1018,8 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
     // the FP32 centre, relative to the seed
     const double fx = double(F.cx) - R.pc.x, fy = double(F.cy) - R.pc.y,
                  fz = double(F.cz) - R.pc.z;
-    LD rmax = LD(0.0), hmx = LD(0.0), hmy = LD(0.0), hmz = LD(0.0);
+    LD rmax = LD(0.0), hmx = LD(0.0), hmy = LD(0.0), hmz = LD(0.0), emax = LD(0.0);
+    LD cvx = LD(0.0), cvy = LD(0.0), cvz = LD(0.0), cvn = LD(0.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LB vs = mask_bit(st.valid[s]);
@@ -939,6 +1035,13 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
         hmx = vmax(hmx, select(vs, vabs(gx) + err, LD(0.0)));
         hmy = vmax(hmy, select(vs, vabs(gy) + err, LD(0.0)));
         hmz = vmax(hmz, select(vs, vabs(gz) + err, LD(0.0)));
+        if (CORNER && S == 1) {
+            emax = vmax(emax, select(vs, err, LD(0.0)));
+            cvx = qx;
+            cvy = qy;
+            cvz = qz;
+            cvn = qn;
+        }
     }
     const double rc = wave_max(rmax) * (1.0 + 1e-9);
     F.hx = f_up(wave_max(hmx) * (1.0 + 1e-9));
@@ -953,6 +1056,19 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F) {
     for (int k = 0; k < 6; ++k) cs = fmax(cs, fabs(R.box[k]));
     F.s = float(1e-6 * cs);
     F.on = 1;
+    if (CORNER && S == 1) {
+        // corner table of the seed test (corner_may_cut)
+        const double delta = (wave_max(emax) + double(F.s)) * (1.0 + 1e-9);
+        const LD vv = vfma(cvx, cvx, vfma(cvy, cvy, cvz * cvz));
+        const LD r = vv - R.wi;
+        const LD rv = r + 1e-5 * (vabs(r) + vv) + 4.0 * delta * cvn + 4.0 * delta * delta;
+        CT.x = cvt<float>(cvx);
+        CT.y = cvt<float>(cvy);
+        CT.z = cvt<float>(cvz);
+        CT.r = f_up_lanes(rv);
+        CT.delta = f_up(delta);
+        CT.live = st.valid[0];
+    }
     return true;
 }
 
@@ -974,7 +1090,7 @@ VGC_HD void stage_cand(const ClipParams& P, CellLds<S>& L, int base, int cnt) {
 // Security-radius loop + integration for work item `item`.  GATHER: the
 // wave finds the neighbours itself, ring after ring of distance; otherwise
 // it reads them from P.list.
-template <int S, bool EXACT, bool GATHER, bool W>
+template <int S, bool EXACT, bool GATHER, bool W, bool CORNER = W>
 VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
@@ -1051,17 +1167,18 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         if (!(rhi2 > 0.0f)) rhi2 = all2 * 1e-6f;
         float rlo2 = -1.0f;
         PowerFilter F{};
+        CornerTab CT;
+        CT.live = 0;
+        bool ct_on = false;
         int nfilt = 0;
         while (true) {
             // first ring: level-synchronous walk; later rings: depth first,
             // bounded to the nearest seeds when the buffer fills
             int cnt = rlo2 >= 0.0f
-                          ? gather_dfs<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F)
+                          ? gather_dfs<S, CORNER>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F, CT, ct_on)
                           : gather_ring<S>(P, L, c, inst, pq.x, pq.y, pq.z, rlo2, rhi2, F);
-            if (W && F.on && ++nfilt > P.ring_limit) {
-                // weighted cells: the list tiers take the cell, with the same
-                // filter and no capacity limit (Voronoi cells keep gathering:
-                // deferring them measured slower)
+            if (F.on && ++nfilt > P.ring_limit) {
+                // a second gather (then the list tiers) takes the cell
                 R.flags |= CELL_NEED_K | CELL_WHY_RINGS;
                 break;
             }
@@ -1105,13 +1222,14 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 break;
             }
             rlo2 = rhi2;
-            if (make_filter<S>(R, F)) {
+            if (make_filter<S, CORNER>(R, F, CT)) {
                 // every seed beyond the ring, within the max-weight radius,
                 // that passes the power filter of the cell as it is now
                 // (narrowed above if too many)
                 rhi2 = need2 < all2 ? need2 : all2;
             } else {
                 F.on = 0;
+                CT.live = 0;
                 rhi2 = need2 < all2 ? need2 : all2;
             }
             wave_sync();
@@ -1140,7 +1258,8 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         // the partial cell contains the cell: its ball filters the list tiers
         PowerFilter G{};
-        const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S>(R, G);
+        CornerTab GT;
+        const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S, false>(R, G, GT);
         if (lane0()) {
             P.flags[c] = flags;
             P.r2[c] = r2max;

@@ -291,7 +291,8 @@ public:
         P.list_len = nullptr;
         P.list_stride = K0;
         P.list_complete = 0;
-        P.ring_limit = kMaxFilteredRings;
+        P.ring_limit = weighted_ ? kMaxFilteredRings : kMaxVoronoiRings;
+        P.corner = 0;
         if (!autotune) {
             // without the tuner every call runs the default work-group size
             be_.set_clip_waves(4);
@@ -388,6 +389,7 @@ public:
                 Q.items = d_items;
                 Q.n_items = (int)again.size();
                 Q.ring_limit = 1 << 30;
+                Q.corner = 1;
                 be_.clip(1, false, true, Q);
                 be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
                 for (int c : again) {

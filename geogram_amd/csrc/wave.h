@@ -411,7 +411,16 @@ VGC_HD bool mask_test(uint64_t m, int idx) { return (m >> (idx & 63)) & 1; }
 
 template <class T>
 VGC_HD T readlane(T a, int lane) {
-    return __shfl(a, lane);
+    // lane is wave-uniform at every call: v_readlane into a scalar register
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "readlane of 4- or 8-byte values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), lane));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, a);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+        return __builtin_bit_cast(T, (uint64_t(hi) << 32) | lo);
+    }
 }
 template <class T>
 VGC_HD T shfl(T a, int src) {
