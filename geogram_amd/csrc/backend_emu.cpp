@@ -223,6 +223,24 @@ struct EmuBackend {
         for (int t = 0; t < n; ++t) out[t] = off[t] - base;
     }
 
+    unsigned census(const int* items, int n_items, const uint32_t* flags, uint32_t mask,
+                    int* out_cell, uint32_t* out_flag, unsigned* d_count) {
+        unsigned k = 0;
+        for (int t = 0; t < n_items; ++t) {
+            const int c = items ? items[t] : t;
+            const uint32_t f = flags[c];
+            if ((f & mask) || !(f & CELL_DONE)) {
+                out_cell[k] = c;
+                out_flag[k] = f;
+                ++k;
+            }
+        }
+        *d_count = k;
+        return k;
+    }
+    void gather_u32(const int* items, int n, const uint32_t* src, uint32_t* out) {
+        for (int t = 0; t < n; ++t) out[t] = src[items[t]];
+    }
     void scan_i32(const int* in, long long* out, int n) {
         out[0] = 0;
         for (int i = 0; i < n; ++i) out[i + 1] = out[i] + in[i];

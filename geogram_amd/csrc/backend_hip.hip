@@ -857,6 +857,29 @@ __global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* 
 
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
+// Cells of `items` (all n_items cells if null) that are not plainly done —
+// not CELL_DONE, or carrying a flag of `mask` — appended with their flags
+// (unordered; the host sorts them).
+__global__ void k_census(const int* __restrict__ items, int n_items,
+                         const uint32_t* __restrict__ flags, uint32_t mask, int* out_cell,
+                         uint32_t* out_flag, unsigned* count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_items) return;
+    const int c = items ? items[t] : t;
+    const uint32_t f = flags[c];
+    if ((f & mask) || !(f & CELL_DONE)) {
+        const unsigned k = atomicAdd(count, 1u);
+        out_cell[k] = c;
+        out_flag[k] = f;
+    }
+}
+
+__global__ void k_gather_u32(const int* __restrict__ items, int n, const uint32_t* __restrict__ src,
+                             uint32_t* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = src[items[t]];
+}
+
 // ----------------------------------------------------------------- backend
 
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -1048,6 +1071,22 @@ struct HipBackend {
     }
     void rebase(const long long* off, int n, long long base, long long* out) {
         k_rebase<<<blocks_for(n, 256), 256, 0, stream_>>>(off, n, base, out);
+        check();
+    }
+    // k_census into out_cell / out_flag (n_items capacity); returns the count
+    unsigned census(const int* items, int n_items, const uint32_t* flags, uint32_t mask,
+                    int* out_cell, uint32_t* out_flag, unsigned* d_count) {
+        VGC_HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned), stream_));
+        if (n_items > 0)
+            k_census<<<blocks_for(n_items, 256), 256, 0, stream_>>>(items, n_items, flags, mask,
+                                                                    out_cell, out_flag, d_count);
+        check();
+        unsigned cnt = 0;
+        d2h(&cnt, d_count, sizeof(unsigned));
+        return cnt;
+    }
+    void gather_u32(const int* items, int n, const uint32_t* src, uint32_t* out) {
+        if (n > 0) k_gather_u32<<<blocks_for(n, 256), 256, 0, stream_>>>(items, n, src, out);
         check();
     }
     void scan_i32(const int* in, long long* out, int n) {

@@ -52,6 +52,9 @@ enum CellFlag : uint32_t {
     CELL_WHY_WALK = 256u,    // a tree walk outgrew the wave's buffers
     CELL_WHY_NARROW = 512u,  // a ring could not be narrowed further
 };
+// a cell that is done and carries none of these needs no further pass
+constexpr uint32_t kCensusMask = CELL_NEED_K | CELL_OVERFLOW | CELL_UNCERTAIN | CELL_EMPTY |
+                                 CELL_DISCONNECTED | CELL_BAD_TOPO;
 
 // plane slots per cell (8-bit ids): 128 with 64 or 128 corner slots, 256 with
 // 256 or 512 corner slots

@@ -91,7 +91,8 @@ public:
     long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
     bool autotune = false;             // pick the first pass's work-group size by timing
     std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
-    std::vector<uint32_t> h_flags;  // by Morton position
+    std::vector<uint32_t> h_flags;  // by Morton position, current for census_ cells
+    std::vector<int> census_;       // cells of the last pass that are not plainly done
 
     // layout of the last build
     InstLayout IL{};
@@ -354,11 +355,35 @@ public:
             st.a_cells += P.n_items;
         }
         auto t3 = now();
+        // the cells that are not plainly done, with their flags, sorted by
+        // Morton position: the later passes only touch these, and h_flags is
+        // kept current for them alone
         h_flags.resize(n_);
-        be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+        census_.clear();
+        {
+            const size_t cap = std::max(1, P.n_items);
+            int* d_cc = (int*)slot(S_CENS_C, sizeof(int) * cap);
+            uint32_t* d_cf = (uint32_t*)slot(S_CENS_F, sizeof(uint32_t) * cap);
+            unsigned* d_cn = (unsigned*)slot(S_CENS_N, sizeof(unsigned));
+            const unsigned m = be_.census(P.items, P.n_items, d_flags, kCensusMask, d_cc, d_cf, d_cn);
+            std::vector<int> cc(m);
+            std::vector<uint32_t> cf(m);
+            if (m) {
+                be_.d2h(cc.data(), d_cc, sizeof(int) * m);
+                be_.d2h(cf.data(), d_cf, sizeof(uint32_t) * m);
+            }
+            std::vector<unsigned> ord(m);
+            for (unsigned i = 0; i < m; ++i) ord[i] = i;
+            std::sort(ord.begin(), ord.end(), [&](unsigned a, unsigned b) { return cc[a] < cc[b]; });
+            census_.resize(m);
+            for (unsigned i = 0; i < m; ++i) {
+                census_[i] = cc[ord[i]];
+                h_flags[cc[ord[i]]] = cf[ord[i]];
+            }
+        }
         std::vector<int> need, over;
         long long nempty = 0;
-        for (int c = 0; c < n_; ++c) {
+        for (int c : census_) {
             uint32_t f = h_flags[c];
             if (f & CELL_OVERFLOW)
                 over.push_back(c);
@@ -391,7 +416,7 @@ public:
                 Q.ring_limit = 1 << 30;
                 Q.corner = 1;
                 be_.clip(1, false, true, Q);
-                be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+                refresh_flags(again, d_flags);
                 for (int c : again) {
                     if (h_flags[c] & CELL_OVERFLOW)
                         over.push_back(c);
@@ -404,7 +429,7 @@ public:
         if (tiers(need, over, false, P, st)) return -1;
         // cells with signs inside their error bound: exact re-clip
         std::vector<int> unsure;
-        for (int c = 0; c < n_; ++c)
+        for (int c : census_)
             if (h_flags[c] & (CELL_UNCERTAIN | CELL_BAD_TOPO)) unsure.push_back(c);
         st.nb_pass_e += unsure.size();
         if (!unsure.empty()) {
@@ -415,7 +440,7 @@ public:
             Q.items = d_items;
             Q.n_items = ni;
             be_.clip(1, true, gather, Q);
-            be_.d2h(h_flags.data(), d_flags, sizeof(uint32_t) * (size_t)n_);
+            refresh_flags(unsure, d_flags);
             std::vector<int> need2, over2;
             for (int c : unsure) {
                 uint32_t f = h_flags[c];
@@ -434,20 +459,19 @@ public:
         st.nb_clips += ctr[0];
         st.nb_tests += ctr[1];
         st.nb_cells += P.n_items;
+        // the other computed cells are done with no flag
         inst_bad.assign(ni_, 0);
-        for (int k = 0, c = 0; k < ni_; ++k) {
-            if (pb.active && !(*pb.active)[k]) {
-                c = h_p0_[k] + h_n_[k];
-                continue;
-            }
-            for (const int e = h_p0_[k] + h_n_[k]; c < e; ++c) {
-                uint32_t f = h_flags[c];
-                if (!(f & CELL_DONE)) st.nb_failed++;
-                if (!(f & CELL_DONE) && (f & CELL_OVERFLOW)) st.nb_fail_capacity++;
-                if (f & CELL_UNCERTAIN) st.nb_uncertain++;
-                if (f & CELL_EMPTY) st.nb_empty++;
-                if (f & CELL_DISCONNECTED) st.nb_disconnected++;
-                if (!(f & CELL_DONE) || (f & CELL_EMPTY)) inst_bad[k]++;
+        for (int c : census_) {
+            const uint32_t f = h_flags[c];
+            if (!(f & CELL_DONE)) st.nb_failed++;
+            if (!(f & CELL_DONE) && (f & CELL_OVERFLOW)) st.nb_fail_capacity++;
+            if (f & CELL_UNCERTAIN) st.nb_uncertain++;
+            if (f & CELL_EMPTY) st.nb_empty++;
+            if (f & CELL_DISCONNECTED) st.nb_disconnected++;
+            if (!(f & CELL_DONE) || (f & CELL_EMPTY)) {
+                const int k = int(std::upper_bound(h_p0_.begin(), h_p0_.begin() + ni_, c) -
+                                  h_p0_.begin()) - 1;
+                inst_bad[k]++;
             }
         }
 
@@ -488,12 +512,25 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
         size_t bytes = 0;
     };
+
+    // h_flags[c] = the device flag of each cell c of `cells`
+    void refresh_flags(const std::vector<int>& cells, const uint32_t* d_flags) {
+        if (cells.empty()) return;
+        const size_t m = cells.size();
+        int* d_c = (int*)slot(S_CENS_C, sizeof(int) * m);
+        uint32_t* d_f = (uint32_t*)slot(S_CENS_F, sizeof(uint32_t) * m);
+        be_.h2d(d_c, cells.data(), sizeof(int) * m);
+        be_.gather_u32(d_c, (int)m, d_flags, d_f);
+        std::vector<uint32_t> f(m);
+        be_.d2h(f.data(), d_f, sizeof(uint32_t) * m);
+        for (size_t i = 0; i < m; ++i) h_flags[cells[i]] = f[i];
+    }
 
     void* slot(int id, size_t bytes) {
         if (slots_.size() < S_COUNT) slots_.resize(S_COUNT);
@@ -515,26 +552,26 @@ private:
         st.nb_pass_b += need.size();
         if (!need.empty()) {
             if (radius_pass(1, exact, need, P, st)) return -1;
-            be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+            refresh_flags(need, P.flags);
             for (int c : need)
                 if (h_flags[c] & CELL_OVERFLOW) over.push_back(c);
         }
         st.nb_pass_c += over.size();
         if (!over.empty()) {
             if (radius_pass(2, exact, over, P, st)) return -1;
-            be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+            refresh_flags(over, P.flags);
             std::vector<int> over4;
             for (int c : over)
                 if (h_flags[c] & CELL_OVERFLOW) over4.push_back(c);
             if (!over4.empty()) {
                 if (radius_pass(4, exact, over4, P, st)) return -1;
-                be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+                refresh_flags(over4, P.flags);
                 std::vector<int> over8;
                 for (int c : over4)
                     if (h_flags[c] & CELL_OVERFLOW) over8.push_back(c);
                 if (!over8.empty()) {
                     if (radius_pass(8, exact, over8, P, st)) return -1;
-                    be_.d2h(h_flags.data(), P.flags, sizeof(uint32_t) * (size_t)n_);
+                    refresh_flags(over8, P.flags);
                 }
             }
         }
