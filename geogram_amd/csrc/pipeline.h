@@ -294,6 +294,13 @@ public:
         P.list_complete = 0;
         P.ring_limit = weighted_ ? kMaxFilteredRings : kMaxVoronoiRings;
         P.corner = 0;
+        if (pb.abort_on_empty) {
+            // the caller only needs to know whether a cell is empty: the
+            // first empty cell stops the pass
+            unsigned* d_ab = (unsigned*)slot(S_ABORT, sizeof(unsigned));
+            be_.dzero(d_ab, sizeof(unsigned));
+            P.abort_flag = d_ab;
+        }
         if (!autotune) {
             // without the tuner every call runs the default work-group size
             be_.set_clip_waves(4);
@@ -308,6 +315,7 @@ public:
             // that tuning costs about half of one full pass.
             ClipParams T = P;
             T.counters = nullptr;
+            T.abort_flag = nullptr;
             const int kRun = 2048, kRuns = 128;
             if (P.n_items > 2 * kRun * kRuns) {
                 std::vector<int> titems;
@@ -355,6 +363,16 @@ public:
             st.a_cells += P.n_items;
         }
         auto t3 = now();
+        if (P.abort_flag) {
+            unsigned raised = 0;
+            be_.d2h(&raised, P.abort_flag, sizeof(unsigned));
+            if (raised) {
+                // an empty cell (the cells after it were skipped)
+                st.nb_empty += 1;
+                st.t_total_ms += ms_since(tstart);
+                return 1;
+            }
+        }
         // the cells that are not plainly done, with their flags, sorted by
         // Morton position: the later passes only touch these, and h_flags is
         // kept current for them alone
@@ -401,6 +419,7 @@ public:
             st.t_total_ms += ms_since(tstart);
             return 1;
         }
+        P.abort_flag = nullptr;
         if (gather) {
             // cells that used up the first pass's ring limit: gather again
             // without the limit (a wave per cell) before any list tier
@@ -512,7 +531,7 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_COUNT
     };
     struct Buf {
         void* p = nullptr;

@@ -340,6 +340,9 @@ inline bool lane0() { return true; }
 inline void atomic_add_u64(unsigned long long* p, unsigned long long v) {
     __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 }
+// a flag shared by the cells of a pass: read, and raised once set
+inline unsigned flag_load(const unsigned* p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+inline void flag_raise(unsigned* p) { __atomic_fetch_or(p, 1u, __ATOMIC_RELAXED); }
 
 }  // namespace vgc
 
@@ -477,6 +480,12 @@ VGC_HD bool any(bool b) { return __ballot(b) != 0; }
 VGC_HD bool lane0() { return __lane_id() == 0; }
 
 VGC_HD void atomic_add_u64(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
+// a flag shared by the cells of a pass: read (device-coherent), and raised
+// with a vector atomic
+VGC_HD unsigned flag_load(const unsigned* p) {
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+VGC_HD void flag_raise(unsigned* p) { atomicOr(p, 1u); }
 
 }  // namespace vgc
 
