@@ -224,10 +224,12 @@ struct EmuBackend {
     }
 
     unsigned census(const int* items, int n_items, const uint32_t* flags, uint32_t mask,
-                    int* out_cell, uint32_t* out_flag, unsigned* d_count) {
+                    const unsigned* skip, const int* inst_of, int* out_cell, uint32_t* out_flag,
+                    unsigned* d_count) {
         unsigned k = 0;
         for (int t = 0; t < n_items; ++t) {
             const int c = items ? items[t] : t;
+            if (skip && skip[inst_of ? inst_of[c] : 0]) continue;
             const uint32_t f = flags[c];
             if ((f & mask) || !(f & CELL_DONE)) {
                 out_cell[k] = c;

@@ -859,13 +859,16 @@ __global__ void k_set0_i64(long long* p) { *p = 0; }
 
 // Cells of `items` (all n_items cells if null) that are not plainly done —
 // not CELL_DONE, or carrying a flag of `mask` — appended with their flags
-// (unordered; the host sorts them).
+// (unordered; the host sorts them).  Cells of the instances marked in
+// `skip` (by inst_of; instance 0 if null) are left out.
 __global__ void k_census(const int* __restrict__ items, int n_items,
-                         const uint32_t* __restrict__ flags, uint32_t mask, int* out_cell,
-                         uint32_t* out_flag, unsigned* count) {
+                         const uint32_t* __restrict__ flags, uint32_t mask,
+                         const unsigned* __restrict__ skip, const int* __restrict__ inst_of,
+                         int* out_cell, uint32_t* out_flag, unsigned* count) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_items) return;
     const int c = items ? items[t] : t;
+    if (skip && skip[inst_of ? inst_of[c] : 0]) return;
     const uint32_t f = flags[c];
     if ((f & mask) || !(f & CELL_DONE)) {
         const unsigned k = atomicAdd(count, 1u);
@@ -1075,11 +1078,12 @@ struct HipBackend {
     }
     // k_census into out_cell / out_flag (n_items capacity); returns the count
     unsigned census(const int* items, int n_items, const uint32_t* flags, uint32_t mask,
-                    int* out_cell, uint32_t* out_flag, unsigned* d_count) {
+                    const unsigned* skip, const int* inst_of, int* out_cell, uint32_t* out_flag,
+                    unsigned* d_count) {
         VGC_HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned), stream_));
         if (n_items > 0)
-            k_census<<<blocks_for(n_items, 256), 256, 0, stream_>>>(items, n_items, flags, mask,
-                                                                    out_cell, out_flag, d_count);
+            k_census<<<blocks_for(n_items, 256), 256, 0, stream_>>>(
+                items, n_items, flags, mask, skip, inst_of, out_cell, out_flag, d_count);
         check();
         unsigned cnt = 0;
         d2h(&cnt, d_count, sizeof(unsigned));

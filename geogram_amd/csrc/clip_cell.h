@@ -104,7 +104,8 @@ struct ClipParams {
     int ring_limit;         // gather mode: filtered rings before the cell is handed on
     int weighted;
     int corner;             // gather mode, Voronoi cells: the corner-test kernel
-    unsigned* abort_flag;   // if set: raised by an empty cell; the pass's later cells skip
+    unsigned* abort_flag;   // if set, per instance: raised by an empty cell; the
+                            // instance's later cells skip
     // outputs, indexed by the caller's seed index
     double* vol;
     double* mg;             // 3 per seed: integral of x over the cell
@@ -1099,11 +1100,12 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
     wave_sync();  // the previous cell of this wave is done with the LDS
-    // a pass that only asks whether any cell is empty stops at the first one
-    if (P.abort_flag && flag_load(P.abort_flag)) return;
     const int c = P.items ? P.items[item] : item;
-    const pt4 pc = P.pts[c];
     const int inst = P.inst_of ? P.inst_of[c] : 0;
+    // a pass that only asks whether an instance has an empty cell stops that
+    // instance at its first one
+    if (P.abort_flag && flag_load(P.abort_flag + inst)) return;
+    const pt4 pc = P.pts[c];
     const double* box = P.box + 6 * inst;
 
     CellRun<S> R;
@@ -1287,7 +1289,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
 
     // ------------------------------------------------------ integration
     if (flags & CELL_EMPTY) {
-        if (P.abort_flag && lane0()) flag_raise(P.abort_flag);
+        if (P.abort_flag && lane0()) flag_raise(P.abort_flag + inst);
         if (lane0()) {
             P.vol[orig_c] = 0.0;
             P.mg[3 * (long long)orig_c + 0] = 0.0;

@@ -229,9 +229,12 @@ public:
         std::vector<int> sel(K), bad;
         int passes = 0;
         std::vector<int> compute;  // instances whose cells a pass computes
-        auto cells = [&](const double* w) -> int {
+        // trial passes: an instance with an empty cell fails its mass test
+        // whatever its other cells, so its evaluation stops there
+        auto cells = [&](const double* w, bool trial = false) -> int {
             auto tc = std::chrono::steady_clock::now();
             pb.d_w = w;
+            pb.abort_on_empty = trial;
             pb.active = compute.empty() ? nullptr : &compute;
             int rc = E_.cells(pb, es);
             ++passes;
@@ -315,7 +318,7 @@ public:
                 compute = pending;  // the other instances keep their cells
                 be_.h2d(d_t, tk.data(), sizeof(double) * K);
                 be_.seg_axpy(n, d_inst, d_w, d_t, d_x, d_wt);
-                if (cells(d_wt)) return -1;
+                if (cells(d_wt, true)) return -1;
                 for (int k = 0; k < K; ++k) {
                     if (!pending[k]) continue;
                     const bool okk = bad[k] == 0 && ms[3 * k + 2] >= eps0[k] &&

@@ -63,7 +63,7 @@ struct Problem {
                                     // 16..64: kNN lists of k0 for the first pass
     bool want_neighbors = true;
     bool count_work = false;        // algorithmic work counters (global atomics)
-    bool abort_on_empty = false;    // stop after the first pass if a cell is empty
+    bool abort_on_empty = false;    // an instance with an empty cell stops (and is bad)
                                     // (cells() then returns 1; outputs unset)
     // per instance, nonzero = compute its cells; the others keep the outputs
     // (volumes, centroids, neighbours) of the last pass that computed them
@@ -295,10 +295,10 @@ public:
         P.ring_limit = weighted_ ? kMaxFilteredRings : kMaxVoronoiRings;
         P.corner = 0;
         if (pb.abort_on_empty) {
-            // the caller only needs to know whether a cell is empty: the
-            // first empty cell stops the pass
-            unsigned* d_ab = (unsigned*)slot(S_ABORT, sizeof(unsigned));
-            be_.dzero(d_ab, sizeof(unsigned));
+            // the caller only needs to know whether an instance has an empty
+            // cell: its first empty cell stops it
+            unsigned* d_ab = (unsigned*)slot(S_ABORT, sizeof(unsigned) * (size_t)ni_);
+            be_.dzero(d_ab, sizeof(unsigned) * (size_t)ni_);
             P.abort_flag = d_ab;
         }
         if (!autotune) {
@@ -363,12 +363,20 @@ public:
             st.a_cells += P.n_items;
         }
         auto t3 = now();
+        // instances stopped by an empty cell: bad; no further pass for them
+        std::vector<unsigned> stopped;
         if (P.abort_flag) {
-            unsigned raised = 0;
-            be_.d2h(&raised, P.abort_flag, sizeof(unsigned));
-            if (raised) {
-                // an empty cell (the cells after it were skipped)
-                st.nb_empty += 1;
+            stopped.resize(ni_);
+            be_.d2h(stopped.data(), P.abort_flag, sizeof(unsigned) * (size_t)ni_);
+            int nstop = 0, nlive = 0;
+            for (int k = 0; k < ni_; ++k) {
+                nstop += stopped[k] != 0;
+                nlive += !pb.active || (*pb.active)[k];
+            }
+            st.nb_empty += nstop;
+            if (nstop == nlive) {
+                inst_bad.assign(ni_, 0);
+                for (int k = 0; k < ni_; ++k) inst_bad[k] = stopped[k] ? 1 : 0;
                 st.t_total_ms += ms_since(tstart);
                 return 1;
             }
@@ -383,7 +391,8 @@ public:
             int* d_cc = (int*)slot(S_CENS_C, sizeof(int) * cap);
             uint32_t* d_cf = (uint32_t*)slot(S_CENS_F, sizeof(uint32_t) * cap);
             unsigned* d_cn = (unsigned*)slot(S_CENS_N, sizeof(unsigned));
-            const unsigned m = be_.census(P.items, P.n_items, d_flags, kCensusMask, d_cc, d_cf, d_cn);
+            const unsigned m = be_.census(P.items, P.n_items, d_flags, kCensusMask, P.abort_flag,
+                                          P.inst_of, d_cc, d_cf, d_cn);
             std::vector<int> cc(m);
             std::vector<uint32_t> cf(m);
             if (m) {
@@ -414,7 +423,8 @@ public:
                 st.why_narrow += (f & CELL_WHY_NARROW) != 0;
             }
         }
-        if (pb.abort_on_empty && nempty > 0) {
+        if (pb.abort_on_empty && ni_ == 1 && nempty > 0) {
+            inst_bad.assign(1, 1);
             st.nb_empty += nempty;
             st.t_total_ms += ms_since(tstart);
             return 1;
@@ -480,6 +490,8 @@ public:
         st.nb_cells += P.n_items;
         // the other computed cells are done with no flag
         inst_bad.assign(ni_, 0);
+        for (size_t k = 0; k < stopped.size(); ++k)
+            if (stopped[k]) inst_bad[k]++;
         for (int c : census_) {
             const uint32_t f = h_flags[c];
             if (!(f & CELL_DONE)) st.nb_failed++;
