@@ -899,6 +899,7 @@ struct HipBackend {
         if (temp_) (void)hipFree(temp_);
         if (red_) (void)hipFree(red_);
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
+        if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -1258,10 +1259,34 @@ struct HipBackend {
         k_cgb_start_final<<<T.n_inst, 256, 0, stream_>>>(T, part, cg, tol2);
         check();
     }
+    // k batched CG iterations, captured once as a graph (for these buffers and
+    // this table) and replayed
     void cgb_iters(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
                    double* p, double* q, double* cg) {
-        double* part = seg_part(T.nblk);
+        double* part = seg_part(T.nblk);  // allocated outside the capture
+        const void* key[21] = {(const void*)(long long)k, (const void*)(long long)n,
+                               (const void*)(long long)T.n_inst, (const void*)(long long)T.nblk,
+                               T.off, T.blk_inst, T.blk_idx, T.cnt, T.first, T.inst_of, ptr, col,
+                               val, diag, x, r, z, p, q, cg, part};
+        bool same = cgb_exec_ != nullptr;
+        for (int i = 0; i < 21 && same; ++i) same = cgb_key_[i] == key[i];
+        if (!same) {
+            if (cgb_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cgb_exec_));
+            cgb_exec_ = nullptr;
+            hipGraph_t g = nullptr;
+            VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, q, cg, part);
+            VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+            VGC_HIP_CHECK(hipGraphInstantiate(&cgb_exec_, g, nullptr, nullptr, 0));
+            VGC_HIP_CHECK(hipGraphDestroy(g));
+            for (int i = 0; i < 21; ++i) cgb_key_[i] = key[i];
+        }
+        VGC_HIP_CHECK(hipGraphLaunch(cgb_exec_, stream_));
+    }
+    void cgb_launch(int k, const SegTable& T, int n, const long long* ptr, const int* col,
+                    const double* val, const double* diag, double* x, double* r, double* z,
+                    double* p, double* q, double* cg, double* part) {
         for (int it = 0; it < k; ++it) {
             k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
                 n, T.inst_of, ptr, col, val, diag, p, q, cg);
@@ -1314,6 +1339,8 @@ private:
     double* red_ = nullptr;
     hipGraphExec_t cg_exec_ = nullptr;
     const void* cg_key_[12] = {};
+    hipGraphExec_t cgb_exec_ = nullptr;
+    const void* cgb_key_[21] = {};
     double* seg_part_ = nullptr;
     int seg_part_cap_ = 0;
     int waves_ = 4;
