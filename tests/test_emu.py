@@ -274,3 +274,26 @@ def test_lloyd_batch_equals_separate_runs(emu_ctx):
         xi, _ = emu_ctx.lloyd(x[o:o + s], 3, box=b)
         assert np.array_equal(xi, xb[o:o + s])
         o += s
+
+
+def _clusters_and_outliers(seed=3, sig=0.005):
+    """Two tight clusters of 1495 seeds and 10 uniform outliers: the outliers'
+    cells reach far past the clusters, whose seeds fill their filtered rings."""
+    rng = np.random.default_rng(seed)
+    c = np.array([[0.3, 0.3, 0.3], [0.7, 0.6, 0.5]])
+    x = np.vstack([c[0] + sig * rng.standard_normal((1495, 3)),
+                   c[1] + sig * rng.standard_normal((1495, 3)), rng.random((10, 3))])
+    return np.clip(x, 1e-3, 1 - 1e-3)
+
+
+def test_second_gather_with_corner_test(emu_ctx):
+    """Voronoi cells that use up the first pass's filtered rings are gathered
+    again with the corner test (seeds that cannot cut any corner dropped)."""
+    x = _clusters_and_outliers()
+    n = x.shape[0]
+    r = emu_ctx.cells(x)
+    assert r.stats["nb_regather"] > 0
+    assert r.stats["nb_failed"] == 0
+    o = oracle.cells(x, k0=20)
+    compare(r, o, n, x=x)
+    assert abs(r.vol.sum() - 1.0) < 1e-12

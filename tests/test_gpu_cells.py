@@ -193,3 +193,17 @@ def test_c2_two_million_invariants(gpu_ctx):
     for t, i in enumerate(sample):
         assert set(r.neighbors(i).tolist()) == set(o["nbr"][t, : o["cnt"][t]].tolist())
         assert set(r.neighbors(i).tolist()) == set(oc["nbr"][t, : oc["cnt"][t]].tolist())
+
+
+def test_gpu_second_gather_matches_emulation(gpu_ctx, emu_ctx):
+    """Voronoi cells handed to the second gather (corner test) and weighted
+    cells (corner test in the first pass): GPU equals emulation bit for bit."""
+    from tests.test_emu import _clusters_and_outliers
+    x = _clusters_and_outliers()
+    g = gpu_ctx.cells(x)
+    e = emu_ctx.cells(x)
+    assert g.stats["nb_regather"] == e.stats["nb_regather"] > 0
+    _same(g, e)
+    rng = np.random.default_rng(8)
+    w = rng.random(x.shape[0]) * 1e-4
+    _same(gpu_ctx.cells(x, weights=w), emu_ctx.cells(x, weights=w))
