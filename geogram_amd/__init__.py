@@ -307,6 +307,29 @@ class Context:
         self._check(rc, allow_partial=False)
 
     # ---------------------------------------------------------------- Lloyd
+    def cvt_func_grad(self, points, box=UNIT_BOX):
+        """CVT energy f = sum_i integral over cell i of |x - p_i|^2 and its
+        gradient g_i = 2 (m_i p_i - mg_i) (compute_CVT_func_grad_in_volume,
+        RVD.cpp:792-899).  Returns (f, g); g is an (n, 3) array of the same
+        kind as points."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            x = points.detach().to(torch.float64).contiguous().reshape(-1, 3)
+            g = torch.zeros_like(x)
+        else:
+            x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            g = np.zeros_like(x)
+        f = ctypes.c_double(0.0)
+        st = Stats()
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        rc = self.lib.fn("cvt_func_grad")(self.ctx, _ptr(x), x.shape[0],
+                                          boxa.ctypes.data_as(ctypes.c_void_p),
+                                          VGC_DEVICE_PTRS if dev else 0, ctypes.byref(f), _ptr(g),
+                                          ctypes.byref(st))
+        self._check(rc)
+        return f.value, g
+
     def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None, autotune=False):
         """iters Lloyd iterations; returns the new points (same kind of array).
         inst_n / boxes: several independent instances at once (consecutive
@@ -424,6 +447,14 @@ class RestrictedVoronoiDiagram:
         mg += r.mg.reshape(mg.shape)
         m += r.vol
         return r
+
+    def compute_CVT_func_grad(self, g):
+        """RestrictedVoronoiDiagram::compute_CVT_func_grad_in_volume
+        (RVD.h:318): returns the energy and adds its gradient to g (n x 3)."""
+        ctx = self.ctx or default_context()
+        f, gg = ctx.cvt_func_grad(self.points, self.box)
+        g += gg.reshape(g.shape)
+        return f
 
 
 class CentroidalVoronoiTesselation:

@@ -86,6 +86,9 @@ def _declare(L, prefix):
     f = L[prefix + "lloyd"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
+    f = L[prefix + "cvt_func_grad"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _vp, _vp, ctypes.POINTER(Stats)]
     f = L[prefix + "lloyd_batch"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
@@ -123,7 +126,8 @@ class _Lib:
         self.cdll = ctypes.CDLL(path)
         self.prefix = prefix
         self._f = {}
-        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch", "ot_solve",
+        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch",
+                 "cvt_func_grad", "ot_solve",
                  "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:

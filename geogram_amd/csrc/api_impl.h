@@ -178,6 +178,74 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
     }
 }
 
+// CVT energy and gradient of the Voronoi cells of xyz in the box:
+// f += sum_i integral over cell i of |x - p_i|^2, g_i += 2 (m_i p_i - mg_i).
+template <class B>
+int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* box,
+                      uint32_t flags, double* f, double* g, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !box || !f || !g || n == 0) {
+            ctx->err = "vgc_cvt_func_grad: null argument";
+            return VGC_ERR_ARG;
+        }
+        if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+            ctx->err = "vgc_cvt_func_grad: degenerate box";
+            return VGC_ERR_ARG;
+        }
+        if (n > (1u << 30)) {
+            ctx->err = "vgc_cvt_func_grad: too many seeds";
+            return VGC_ERR_ARG;
+        }
+        Problem pb;
+        pb.n_inst = 1;
+        pb.inst_n.assign(1, (int)n);
+        pb.box.assign(box, box + 6);
+        pb.n = (int)n;
+        pb.k0 = 0;
+        pb.want_neighbors = false;
+        pb.want_e2 = true;
+        pb.count_work = (flags & VGC_COUNT_WORK) != 0;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        B& be = ctx->be;
+        double* dg = g;
+        if (dev) {
+            pb.d_xyz = xyz;
+        } else {
+            double* dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+            pb.d_xyz = dx;
+            dg = (double*)ctx->staging(1, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dg, g, sizeof(double) * 3 * (size_t)n);
+        }
+        EngineStats es;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        Engine<B>& E = ctx->eng;
+        if (E.cells(pb, es)) {
+            ctx->err = "vgc_cvt_func_grad: " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        double* df = (double*)ctx->staging(2, sizeof(double));
+        be.red_sum((int)n, E.d_e2, df);
+        double fs = 0.0;
+        be.d2h(&fs, df, sizeof(double));
+        *f += fs;
+        be.cvt_grad((int)n, pb.d_xyz, E.d_vol, E.d_mg, dg);
+        if (!dev) be.d2h(g, dg, sizeof(double) * 3 * (size_t)n);
+        be.sync();
+        fill_stats(es, stats);
+        if (es.nb_failed || es.nb_uncertain) {
+            ctx->err = "vgc_cvt_func_grad: " + std::to_string(es.nb_failed) + " cells failed, " +
+                       std::to_string(es.nb_uncertain) + " uncertain";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
 template <class B>
 int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
             float* d2) {

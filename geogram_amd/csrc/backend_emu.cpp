@@ -311,6 +311,16 @@ struct EmuBackend {
         }
         return tree_sum(sh);
     }
+    void red_sum(int n, const double* a, double* out) {
+        *out = red_sum(n, [&](int i) { return a[i]; });
+    }
+    void cvt_grad(int n, const double* xyz, const double* vol, const double* mg, double* g) {
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const size_t k = 3 * (size_t)i + c;
+                g[k] += 2.0 * (vol[i] * xyz[k] - mg[k]);
+            }
+    }
     void ot_dot(int n, const double* a, const double* b, double* out) {
         *out = red_sum(n, [&](int i) { return a[i] * b[i]; });
     }
@@ -701,6 +711,10 @@ int vgc_emu_lloyd_batch(void* ctx, double* xyz, const uint32_t* inst_n, uint32_t
                         const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
                         vgc_stats* st) {
     return vgc::api_lloyd_batch((EmuCtx*)ctx, xyz, inst_n, n_inst, boxes, iters, k0, flags, st);
+}
+int vgc_emu_cvt_func_grad(void* ctx, const double* xyz, uint32_t n, const double* box,
+                          uint32_t flags, double* f, double* g, vgc_stats* st) {
+    return vgc::api_cvt_func_grad((EmuCtx*)ctx, xyz, n, box, flags, f, g, st);
 }
 int vgc_emu_lloyd(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
                   uint32_t k0, uint32_t flags, vgc_stats* st) {

@@ -265,6 +265,26 @@ __global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __rest
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(256) void k_sum_partial(int n, const double* __restrict__ a,
+                                                     double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += a[i];
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// CVT gradient: g_i += 2 (m_i x_i - mg_i), caller order
+__global__ void k_cvt_grad(int n, const double* __restrict__ xyz, const double* __restrict__ vol,
+                           const double* __restrict__ mg, double* g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c) {
+        const size_t k = 3 * (size_t)i + c;
+        g[k] += 2.0 * (vol[i] * xyz[k] - mg[k]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, double* out) {
     __shared__ double sh[256];
     double s = 0.0;
@@ -1139,6 +1159,15 @@ struct HipBackend {
         k_sum_final<<<1, 256, 0, stream_>>>(red_part(), out);
         check();
     }
+    void red_sum(int n, const double* a, double* out) {
+        k_sum_partial<<<kRedBlocks, 256, 0, stream_>>>(n, a, red_part());
+        k_sum_final<<<1, 256, 0, stream_>>>(red_part(), out);
+        check();
+    }
+    void cvt_grad(int n, const double* xyz, const double* vol, const double* mg, double* g) {
+        k_cvt_grad<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, vol, mg, g);
+        check();
+    }
     void ot_mass_stats(int n, const double* m, const double* nu, double* out) {
         k_mass_partial<<<kRedBlocks, 256, 0, stream_>>>(n, m, nu, red_part());
         k_mass_final<<<1, 256, 0, stream_>>>(red_part(), out);
@@ -1407,6 +1436,13 @@ int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w, const uint
     if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
     return vgc::api_cells(ctx->impl, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr, nbr,
                           farea, cap, stats);
+}
+
+int vgc_cvt_func_grad(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* box,
+                      uint32_t flags, double* f, double* g, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_cvt_func_grad(ctx->impl, xyz, n, box, flags, f, g, stats);
 }
 
 int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,

@@ -109,6 +109,7 @@ struct ClipParams {
     // outputs, indexed by the caller's seed index
     double* vol;
     double* mg;             // 3 per seed: integral of x over the cell
+    double* e2;             // if set, per seed: integral of |x - p|^2 over the cell
     int* nbr_cnt;
     int* nbr_tmp;           // face_stride per seed: caller's index of neighbour
     double* area_tmp;       // face_stride per seed
@@ -1296,6 +1297,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             P.mg[3 * (long long)orig_c + 1] = 0.0;
             P.mg[3 * (long long)orig_c + 2] = 0.0;
             P.nbr_cnt[orig_c] = 0;
+            if (P.e2) P.e2[orig_c] = 0.0;
             P.flags[c] = flags | CELL_DONE;
             P.r2[c] = 0.0;
         }
@@ -1335,7 +1337,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         if (st.valid[s]) cid0 = 64 * s + ffs64(st.valid[s]);
     const double ox = L.q[cid0][0], oy = L.q[cid0][1], oz = L.q[cid0][2];
 
-    LD vacc = LD(0.0), mxa = LD(0.0), mya = LD(0.0), mza = LD(0.0);
+    LD vacc = LD(0.0), mxa = LD(0.0), mya = LD(0.0), mza = LD(0.0), e2a = LD(0.0);
     LD ntet = LD(0.0);
     int nfacets = 0;
     int out_base = 0;
@@ -1363,7 +1365,10 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         LI t = nxt(t1);
         LD q1x = lds_ld(&L.q[0][0], t1 * 3 + 0), q1y = lds_ld(&L.q[0][0], t1 * 3 + 1),
            q1z = lds_ld(&L.q[0][0], t1 * 3 + 2);
-        LD area = LD(0.0), fv = LD(0.0), fmx = LD(0.0), fmy = LD(0.0), fmz = LD(0.0);
+        LD area = LD(0.0), fv = LD(0.0), fmx = LD(0.0), fmy = LD(0.0), fmz = LD(0.0),
+           fe = LD(0.0);
+        // |o|^2 + |q0|^2, constant along the face (second moments)
+        const LD oq0 = (ox * ox + oy * oy + oz * oz) + (q0x * q0x + q0y * q0y + q0z * q0z);
         LB run = act & (t != t0);
         for (int guard = 0; guard < 64 * S && any(run); ++guard) {
             LD qx = lds_ld(&L.q[0][0], t * 3 + 0), qy = lds_ld(&L.q[0][0], t * 3 + 1),
@@ -1381,9 +1386,17 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LD ta = 0.5 * vsqrt(wx2 * wx2 + wy2 * wy2 + wz2 * wz2);
             ntet = select(run, ntet + 1.0, ntet);
             fv = select(run, fv + tv, fv);
-            fmx = select(run, fmx + tv * (ox + q0x + q1x + qx) / 4.0, fmx);
-            fmy = select(run, fmy + tv * (oy + q0y + q1y + qy) / 4.0, fmy);
-            fmz = select(run, fmz + tv * (oz + q0z + q1z + qz) / 4.0, fmz);
+            const LD sx = ox + q0x + q1x + qx, sy = oy + q0y + q1y + qy, sz = oz + q0z + q1z + qz;
+            fmx = select(run, fmx + tv * sx / 4.0, fmx);
+            fmy = select(run, fmy + tv * sy / 4.0, fmy);
+            fmz = select(run, fmz + tv * sz / 4.0, fmz);
+            if (P.e2) {
+                // second moment about the seed (seed-relative corners a..d):
+                // V/20 (sum |a|^2 + |a + b + c + d|^2)
+                const LD nn = oq0 + (q1x * q1x + q1y * q1y + q1z * q1z) + (qx * qx + qy * qy + qz * qz) +
+                              (sx * sx + sy * sy + sz * sz);
+                fe = select(run, fe + tv * nn / 20.0, fe);
+            }
             area = select(run, area + ta, area);
             q1x = select(run, qx, q1x);
             q1y = select(run, qy, q1y);
@@ -1393,6 +1406,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             run = run & (t != t0);
         }
         vacc = vacc + fv;
+        e2a = e2a + fe;
         mxa = mxa + fmx;
         mya = mya + fmy;
         mza = mza + fmz;
@@ -1451,6 +1465,10 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             }
         }
         P.r2[c] = r2max;
+    }
+    if (P.e2) {
+        const double E2 = wave_sum(e2a);
+        if (lane0() && !face_overflow) P.e2[orig_c] = E2;
     }
 }
 

@@ -62,6 +62,7 @@ struct Problem {
     int k0 = 0;                     // 0: neighbours gathered inside the clipping kernel;
                                     // 16..64: kNN lists of k0 for the first pass
     bool want_neighbors = true;
+    bool want_e2 = false;           // integral of |x - p|^2 per cell (CVT energy)
     bool count_work = false;        // algorithmic work counters (global atomics)
     bool abort_on_empty = false;    // an instance with an empty cell stops (and is bad)
                                     // (cells() then returns 1; outputs unset)
@@ -84,6 +85,7 @@ public:
     // outputs of the last cells() call (backend memory)
     double* d_vol = nullptr;
     double* d_mg = nullptr;
+    double* d_e2 = nullptr;  // if want_e2
     long long* d_ptr = nullptr;  // n + 1
     int* d_nbr = nullptr;
     double* d_area = nullptr;
@@ -265,6 +267,8 @@ public:
         P.weighted = weighted_ ? 1 : 0;
         P.vol = d_vol;
         P.mg = d_mg;
+        d_e2 = pb.want_e2 ? (double*)slot(S_E2, sizeof(double) * (size_t)n_) : nullptr;
+        P.e2 = d_e2;
         P.nbr_cnt = d_cnt;
         P.nbr_tmp = d_nbr_tmp;
         P.area_tmp = d_area_tmp;
@@ -543,7 +547,7 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2, S_COUNT
     };
     struct Buf {
         void* p = nullptr;

@@ -117,6 +117,15 @@ int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w,
                     uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
                     vgc_stats* stats);
 
+/* CVT energy and its gradient over the Voronoi cells of xyz in the box,
+   accumulated like RestrictedVoronoiDiagram::compute_CVT_func_grad_in_volume
+   (voronoi/RVD.h:318, RVD.cpp:792-899; the caller zeroes f and g):
+   f += sum_i integral over cell i of |x - p_i|^2, g[3i..3i+2] += 2 (m_i p_i -
+   mg_i).  VGC_DEVICE_PTRS: xyz and g are device pointers (f stays on the
+   host). */
+int vgc_cvt_func_grad(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* box,
+                      uint32_t flags, double* f, double* g, vgc_stats* stats);
+
 /* iters Lloyd iterations in place with the security radius on: each seed
    moves to the centroid of its cell when the cell mass exceeds 1e-30.
    Replaces CentroidalVoronoiTesselation::Lloyd_iterations

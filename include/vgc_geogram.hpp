@@ -81,6 +81,12 @@ public:
             for (int c = 0; c < 3; ++c) mg[3 * (size_t)i + c] += g[3 * (size_t)i + c];
         }
     }
+    // compute_CVT_func_grad_in_volume (voronoi/RVD.h:318): f and g[3n]
+    // caller-zeroed; the energy and its gradient are added
+    void compute_CVT_func_grad(double& f, double* g) {
+        vgc_stats st;
+        ctx_.check(vgc_cvt_func_grad(ctx_.get(), pts_, n_, box_, 0, &f, g, &st));
+    }
     // Laguerre variant (power distance |x - p|^2 - w)
     void compute_centroids(const double* weights, double* mg, double* m) {
         std::vector<double> vol(n_), g(3 * (size_t)n_);

@@ -63,6 +63,7 @@ def lib():
         L.orc_exact_fallbacks.restype = ctypes.c_longlong
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_local_planes.argtypes = [ctypes.c_int]
+        L.orc_set_e2_output.argtypes = [ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -103,10 +104,12 @@ def knn(xyz, k):
 
 
 def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None,
-          centred=False):
+          centred=False, e2=False):
     """Per-cell volume, mg (= integral of x), neighbour lists and facet areas.
 
     centred=True forms the planes in the seed-centred frame (same algorithm).
+    e2=True adds "e2": the integral of |x - p_i|^2 over each cell (the CVT
+    energy terms, RVD.cpp:792-866).
     Returns dict(vol, mg, cnt, nbr, farea, stats); nbr/farea are (n, stride)."""
     lib().orc_set_local_planes(1 if centred else 0)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
@@ -127,6 +130,8 @@ def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None
     nbr = np.full((nc, stride), -1, dtype=np.int32)
     fa = np.zeros((nc, stride))
     st = Stats()
+    e2a = np.zeros(nc) if e2 else None
+    lib().orc_set_e2_output(_dp(e2a) if e2 else None)
     if subset is None:
         err = lib().orc_cells(_dp(xyz), wp, n, _dp(box), int(k0), int(mode), _dp(vol),
                               _dp(mg), _ip(cnt), _ip(nbr), _dp(fa), stride, ctypes.byref(st))
@@ -134,9 +139,13 @@ def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None
         err = lib().orc_cells_subset(_dp(xyz), wp, n, _dp(box), int(k0), int(mode),
                                      _ip(subset), nc, _dp(vol), _dp(mg), _ip(cnt), _ip(nbr),
                                      _dp(fa), stride, ctypes.byref(st))
+    lib().orc_set_e2_output(None)
     if err:
         raise RuntimeError(f"orc_cells failed ({err})")
-    return dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, stats=st.as_dict())
+    out = dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, stats=st.as_dict())
+    if e2:
+        out["e2"] = e2a
+    return out
 
 
 def lloyd(xyz, iters, box=UNIT_BOX, k0=20):

@@ -18,6 +18,10 @@
 static int g_threads = 0;
 static int g_local = 0;
 void orc_set_local_planes(int on) { g_local = on; }
+/* optional per-cell output of the next orc_cells* calls: integral of
+   |x - p_i|^2 over cell i (NULL: off) */
+static double* g_e2 = NULL;
+void orc_set_e2_output(double* e2) { g_e2 = e2; }
 void orc_set_threads(int nt) { g_threads = nt; }
 int orc_get_threads(void) {
 #ifdef _OPENMP
@@ -600,6 +604,35 @@ static void cell_volume_mg(const cell* C, double* m, double* mg) {
     }
 }
 
+/* Integral of |x - p0|^2 over the cell, as ComputeCVTFuncGradVolumetric
+   (RVD.cpp:792-866) integrates it: tetrahedra (p0, p1, p2, p3) with apex at
+   the seed over the triangles of each facet's fan, each contributing
+   (m/10)(U.U + V.V + W.W + U.V + V.W + W.U), U, V, W = p1, p2, p3 - p0. */
+static double cell_second_moment(const cell* C, const double* p0) {
+    double f = 0.0;
+    for (int v = 0; v < C->nv; ++v) {
+        if (C->v2t[v] < 0) continue;
+        int t0 = C->v2t[v];
+        int t1 = walk_next(C, t0, v);
+        int t = walk_next(C, t1, v);
+        int guard = 0;
+        while (t != t0 && guard++ < 100000) {
+            const double *p1 = C->pt[t0], *p2 = C->pt[t1], *p3 = C->pt[t];
+            double mi = tet_vol(p0, p1, p2, p3);
+            double fi = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double Uc = p1[c] - p0[c], Vc = p2[c] - p0[c], Wc = p3[c] - p0[c];
+                fi += Uc * Uc + Vc * Vc + Wc * Wc;
+                fi += Uc * Vc + Vc * Wc + Wc * Uc;
+            }
+            f += fi * (mi / 10.0);
+            t1 = t;
+            t = walk_next(C, t, v);
+        }
+    }
+    return f;
+}
+
 /* facet_area(v) (convex_cell.cpp:1160-1191) */
 static double cell_facet_area(const cell* C, int v) {
     int t0 = C->v2t[v];
@@ -697,7 +730,7 @@ typedef struct {
 
 static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
                        double* vol, double* mg, int* cnt, int* nbr,
-                       double* farea, int stride, orc_stats* st) {
+                       double* farea, int stride, orc_stats* st, double* e2) {
     const double* pi_abs = J->xyz + 3 * (size_t)i;
     /* seed-centred frame (optional): the same planes and box translated by
        -pi before they are formed, so the offsets carry no |pi|^2 rounding */
@@ -769,10 +802,12 @@ static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
         st->nb_empty++;
         *vol = 0.0;
         mg[0] = mg[1] = mg[2] = 0.0;
+        if (e2) *e2 = 0.0;
         return 0;
     }
     cell_geometry(C);
     cell_volume_mg(C, vol, mg);
+    if (e2) *e2 = cell_second_moment(C, pi);
     if (g_local)
         for (int c = 0; c < 3; ++c) mg[c] += *vol * pi_abs[c];
     for (int v = 1; v < C->nv; ++v) {
@@ -829,7 +864,8 @@ int orc_cells_subset(const double* xyz, const double* w, int n,
             int i = cells ? cells[c] : c;
             int e = compute_one(&J, i, &C, &S, vol + c, mg + 3 * (size_t)c,
                                 nbr_cnt + c, nbr + (size_t)c * stride,
-                                farea + (size_t)c * stride, stride, &loc);
+                                farea + (size_t)c * stride, stride, &loc,
+                                g_e2 ? g_e2 + c : NULL);
             if (e) lerr = e;
         }
 #pragma omp critical

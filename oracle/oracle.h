@@ -76,6 +76,8 @@ int orc_lloyd(double* xyz, int n, const double* box, int iters, int k0,
    copy_Laguerre_cell_facet_from_Delaunay does.  Same algorithm; the centred
    frame avoids the |pi|^2 rounding of the plane offsets on tiny cells. */
 void orc_set_local_planes(int on);
+/* per-cell integral of |x - p_i|^2 of the next orc_cells* calls (NULL: off) */
+void orc_set_e2_output(double* e2);
 
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int nt);

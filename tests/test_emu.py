@@ -297,3 +297,27 @@ def test_second_gather_with_corner_test(emu_ctx):
     o = oracle.cells(x, k0=20)
     compare(r, o, n, x=x)
     assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_cvt_energy_and_gradient(emu_ctx, kind):
+    """vgc_cvt_func_grad: energy = sum of the oracle's per-cell second moments
+    (a different tetrahedral decomposition: apex at the seed), gradient =
+    2 (m p - mg); and the gradient is the derivative of the energy (central
+    differences)."""
+    n = 1200
+    x = ga.sample_points(kind, 60 + kind, n)
+    f, g = emu_ctx.cvt_func_grad(x)
+    o = oracle.cells(x, e2=True, centred=True)
+    assert abs(f - o["e2"].sum()) <= 1e-12 * o["e2"].sum()
+    go = 2.0 * (o["vol"][:, None] * x - o["mg"])
+    assert np.abs(g - go).max() <= 1e-10 * np.abs(go).max()
+    rng = np.random.default_rng(kind)
+    for i in rng.choice(n, 4, replace=False):
+        for c in range(3):
+            h = 1e-6
+            xp, xm = x.copy(), x.copy()
+            xp[i, c] += h
+            xm[i, c] -= h
+            fd = (emu_ctx.cvt_func_grad(xp)[0] - emu_ctx.cvt_func_grad(xm)[0]) / (2 * h)
+            assert abs(fd - g[i, c]) <= 1e-6 * np.abs(g).max()

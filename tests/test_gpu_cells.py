@@ -207,3 +207,14 @@ def test_gpu_second_gather_matches_emulation(gpu_ctx, emu_ctx):
     rng = np.random.default_rng(8)
     w = rng.random(x.shape[0]) * 1e-4
     _same(gpu_ctx.cells(x, weights=w), emu_ctx.cells(x, weights=w))
+
+
+def test_gpu_cvt_energy_matches_emulation(gpu_ctx, emu_ctx):
+    import geogram_amd as ga
+    import torch
+    x = ga.sample_points(1, 14, 20000)
+    fe, ge = emu_ctx.cvt_func_grad(x)
+    fg, gg = gpu_ctx.cvt_func_grad(x)
+    assert fg == fe and np.array_equal(gg, ge)
+    fd, gd = gpu_ctx.cvt_func_grad(torch.from_numpy(x).cuda())
+    assert fd == fe and np.array_equal(gd.cpu().numpy(), ge)

@@ -178,3 +178,22 @@ def test_lloyd_moves_to_centroids_and_lowers_energy():
     assert np.allclose(x1, r0["mg"] / r0["vol"][:, None], atol=1e-15)
     x5, _ = oracle.lloyd(x, 5)
     assert energy(x5) < energy(x1) < energy(x)
+
+
+def test_second_moments_closed_forms_and_sampling():
+    """Oracle cell integrals of |x - p_i|^2 (the CVT energy terms): one seed
+    owns the box (sum over axes of ((1-p)^3 + p^3)/3); the centred 8^3
+    lattice gives h^5/4 per cell; random seeds agree with a Monte Carlo
+    estimate of the energy."""
+    p = np.array([[0.3, 0.6, 0.8]])
+    o = oracle.cells(p, k0=1, e2=True)
+    assert abs(o["e2"][0] - sum(((1 - a) ** 3 + a ** 3) / 3 for a in p[0])) < 1e-15
+    g = (np.indices((8, 8, 8)).reshape(3, -1).T + 0.5) / 8
+    o = oracle.cells(g, e2=True, centred=True)
+    assert np.allclose(o["e2"], (1 / 8) ** 5 / 4, rtol=1e-12, atol=0)
+    x = np.random.default_rng(21).random((400, 3))
+    o = oracle.cells(x, e2=True, centred=True)
+    s = np.random.default_rng(5).random((400000, 3))
+    d, _ = cKDTree(x).query(s)
+    mc = np.mean(d ** 2)
+    assert abs(o["e2"].sum() - mc) < 4 * np.std(d ** 2) / np.sqrt(len(s))
