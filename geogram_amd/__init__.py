@@ -214,6 +214,33 @@ class Context:
         self._check(rc)
         return w, st.as_dict()
 
+    def ot_solve_levels(self, points, nu, levels, box=UNIT_BOX, tol=1e-4, max_newton=100,
+                        verbose=False):
+        """Multilevel transport (optimize_levels): points in level order,
+        levels = [0, ..., n] (see brio_levels).  Returns (w, per-level stats)."""
+        dev = _is_torch_cuda(points)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            nu_ = nu.contiguous().to(torch.float64)
+            w = torch.zeros(pts.shape[0], dtype=torch.float64, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            nu_ = np.ascontiguousarray(nu, dtype=np.float64)
+            w = np.zeros(pts.shape[0])
+        lv = np.ascontiguousarray(levels, dtype=np.uint32)
+        nl = len(lv) - 1
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        sts = (OtStats * max(nl, 1))()
+        rc = self.lib.fn("ot_solve_levels")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w), pts.shape[0],
+                                            boxa.ctypes.data_as(ctypes.c_void_p),
+                                            lv.ctypes.data_as(ctypes.c_void_p), nl, float(tol),
+                                            int(max_newton),
+                                            (VGC_DEVICE_PTRS if dev else 0)
+                                            | (_lib.VGC_OT_VERBOSE if verbose else 0), sts)
+        self._check(rc)
+        return w, [sts[k].as_dict() for k in range(nl)]
+
     def ot_solve_batch(self, points, nu, inst_n, boxes, w0=None, tol=1e-4, max_newton=100,
                        verbose=False, autotune=False):
         """Independent transport problems at once (consecutive seeds, one box
@@ -385,6 +412,18 @@ def ot_solve(points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100, device
     return default_context(device).ot_solve(points, nu, w0, box, tol, max_newton)
 
 
+def brio_levels(n: int, ratio: float = 0.125, min_level: int = 2000, seed: int = 0):
+    """A random order of n seeds and its levels for ot_solve_levels, as a
+    biased randomized insertion order cuts it: the last level holds all n, each
+    level before it a fraction `ratio` of the next, down to min_level seeds.
+    Returns (permutation, levels) with levels[0] = 0 and levels[-1] = n."""
+    perm = np.random.default_rng(seed).permutation(n)
+    sizes = [n]
+    while int(sizes[-1] * ratio) >= min_level:
+        sizes.append(int(sizes[-1] * ratio))
+    return perm, np.array([0] + sizes[::-1], dtype=np.uint32)
+
+
 def random_float64(seed: int, count: int) -> np.ndarray:
     """std::mt19937_64(seed) + uniform_real_distribution<double>(0,1) — the
     stream of Numeric::random_reset / random_float64 (basic/numeric.cpp:76-103)."""
@@ -541,4 +580,12 @@ class OptimalTransportMap3d:
             judge = True
         self.weights, self.last_stats = ctx.ot_solve(self.points_, self.nu, w0, self.box,
                                                      self.epsilon, max_iter, judge_start=judge)
+        return self.weights
+
+    def optimize_levels(self, levels, max_iter: int = 100):
+        """optimize_levels (compute_OTM/main.cpp:294): the points set with
+        set_points are in level order, levels = [0, ..., n]."""
+        ctx = self.ctx or default_context()
+        self.weights, self.last_stats = ctx.ot_solve_levels(self.points_, self.nu, levels,
+                                                            self.box, self.epsilon, max_iter)
         return self.weights

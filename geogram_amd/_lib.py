@@ -96,6 +96,9 @@ def _declare(L, prefix):
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
                   ctypes.POINTER(OtStats)]
+    f = L[prefix + "ot_solve_levels"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _vp, _u32, _vp, _vp, _u32, ctypes.c_double, _u32, _u32, _vp]
     f = L[prefix + "ot_solve_batch"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _u32,
@@ -127,7 +130,7 @@ class _Lib:
         self.prefix = prefix
         self._f = {}
         names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch",
-                 "cvt_func_grad", "ot_solve",
+                 "cvt_func_grad", "ot_solve", "ot_solve_levels",
                  "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:

@@ -777,4 +777,121 @@ inline int sample_points(uint32_t kind, uint64_t seed, uint32_t n, double* xyz) 
     return VGC_OK;
 }
 
+// Multilevel transport (OptimalTransportMap3d::optimize_levels): the seeds
+// come in levels, levels[0] = 0 < levels[1] < ... < levels[n_levels] = n;
+// level k solves the transport of the first levels[k+1] seeds, their masses
+// rescaled to the box volume, starting from the previous level's weights and,
+// for the new seeds, the weight of their power-nearest seed of the previous
+// level.  stats: one entry per level (may be null).
+template <class B>
+int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                        const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
+                        uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !nu || !w || !box || !levels || n < 1 || n_levels < 1 || !(tol > 0.0)) {
+            ctx->err = "vgc_ot_solve_levels: bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+            ctx->err = "vgc_ot_solve_levels: degenerate box";
+            return VGC_ERR_ARG;
+        }
+        if (levels[0] != 0 || levels[n_levels] != n) {
+            ctx->err = "vgc_ot_solve_levels: levels must start at 0 and end at n";
+            return VGC_ERR_ARG;
+        }
+        for (uint32_t k = 0; k < n_levels; ++k)
+            if (!(levels[k + 1] > levels[k])) {
+                ctx->err = "vgc_ot_solve_levels: levels must increase";
+                return VGC_ERR_ARG;
+            }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        const double *dx = xyz, *dnu = nu;
+        double* dw = w;
+        if (!dev) {
+            double* sx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            double* sn = (double*)ctx->staging(5, sizeof(double) * (size_t)n);
+            double* sw = (double*)ctx->staging(6, sizeof(double) * (size_t)n);
+            be.h2d(sx, xyz, sizeof(double) * 3 * (size_t)n);
+            be.h2d(sn, nu, sizeof(double) * (size_t)n);
+            dx = sx;
+            dnu = sn;
+            dw = sw;
+        }
+        std::vector<double> hnu(n);
+        be.d2h(hnu.data(), dnu, sizeof(double) * (size_t)n);
+        const double V = (box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
+        double tot = 0.0, mn = 1e300;
+        for (double v : hnu) {
+            tot += v;
+            mn = std::min(mn, v);
+        }
+        if (!(mn > 0.0) || std::fabs(tot - V) > 1e-9 * V) {
+            ctx->err =
+                "vgc_ot_solve_levels: target masses must be positive and sum to the box volume";
+            return VGC_ERR_ARG;
+        }
+        double* dnl = (double*)ctx->staging(7, sizeof(double) * (size_t)n);
+        const uint32_t maxn = max_newton ? max_newton : 100;
+        ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        bool all_conv = true;
+        std::vector<double> hl;
+        for (uint32_t k = 0; k < n_levels; ++k) {
+            const uint32_t m = levels[k + 1], m0 = levels[k];
+            // this level's masses: the first m, rescaled to the box volume
+            double sm = 0.0;
+            for (uint32_t i = 0; i < m; ++i) sm += hnu[i];
+            hl.resize(m);
+            for (uint32_t i = 0; i < m; ++i) hl[i] = hnu[i] * (V / sm);
+            be.h2d(dnl, hl.data(), sizeof(double) * (size_t)m);
+            // start: zero on the first level; then the previous solution and,
+            // for the new seeds, their power-nearest previous seed's weight
+            if (k == 0) {
+                be.dzero(dw, sizeof(double) * (size_t)m);
+            } else {
+                Problem pc;
+                pc.n = (int)m0;
+                pc.n_inst = 1;
+                pc.inst_n = {(int)m0};
+                pc.box.assign(box, box + 6);
+                pc.d_xyz = dx;
+                pc.d_w = dw;
+                EngineStats es;
+                if (ctx->eng.power_nearest_weights(pc, dx + 3 * (size_t)m0, (int)(m - m0), dw + m0,
+                                                   es)) {
+                    ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
+                    return VGC_ERR_INTERNAL;
+                }
+            }
+            Problem pb;
+            pb.n = (int)m;
+            pb.n_inst = 1;
+            pb.inst_n = {(int)m};
+            pb.box.assign(box, box + 6);
+            pb.k0 = 0;
+            pb.d_xyz = dx;
+            OtStats os;
+            if (ctx->ot.solve(pb, dnl, dw, tol, (int)maxn, 1e-7, 4000, os, false,
+                              (flags & VGC_OT_VERBOSE) != 0)) {
+                ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
+                return VGC_ERR_INTERNAL;
+            }
+            all_conv = all_conv && os.converged;
+            if (stats) copy_ot_stats(os, stats + k);
+        }
+        if (!dev) be.d2h(w, dw, sizeof(double) * (size_t)n);
+        be.sync();
+        if (!all_conv) {
+            ctx->err = "vgc_ot_solve_levels: not converged";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
 }  // namespace vgc

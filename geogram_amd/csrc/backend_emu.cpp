@@ -240,6 +240,15 @@ struct EmuBackend {
         *d_count = k;
         return k;
     }
+    void power_nearest(const pt4* pts, const bnode* nodes, const InstLayout& IL, const double* q,
+                       int nq, float cs2, double* wout) {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int t = 0; t < nq; ++t) {
+            const int j = power_nearest_query(pts, nodes, IL, q[3 * (size_t)t + 0],
+                                              q[3 * (size_t)t + 1], q[3 * (size_t)t + 2], cs2);
+            wout[t] = pts[j].w;
+        }
+    }
     void gather_u32(const int* items, int n, const uint32_t* src, uint32_t* out) {
         for (int t = 0; t < n; ++t) out[t] = src[items[t]];
     }
@@ -350,8 +359,10 @@ struct EmuBackend {
                  const double* x, double* y) {
         for (int i = 0; i < n; ++i) y[i] = ot_spmv_row(i, ptr, col, val, diag, x);
     }
-    void ot_rhs(int n, const double* nu, const double* m, double* b) {
+    void ot_rhs(int n, const double* nu, const double* m, double* b, double* tmp) {
         for (int i = 0; i < n; ++i) b[i] = nu[i] - m[i];
+        red_sum(n, b, tmp);
+        for (int i = 0; i < n; ++i) b[i] -= *tmp / double(n);
     }
     void ot_axpy(int n, const double* w, double t, const double* d, double* out) {
         for (int i = 0; i < n; ++i) out[i] = w[i] + t * d[i];
@@ -679,6 +690,26 @@ int vgc_emu_ot_solve(void* ctx, const double* xyz, const double* nu, double* w, 
                      const double* box, double tol, uint32_t max_newton, uint32_t flags,
                      vgc_ot_stats* stats) {
     return vgc::api_ot_solve((EmuCtx*)ctx, xyz, nu, w, n, box, tol, max_newton, flags, stats);
+}
+int vgc_emu_ot_solve_levels(void* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                            const double* box, const uint32_t* levels, uint32_t n_levels,
+                            double tol, uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    return vgc::api_ot_solve_levels((EmuCtx*)ctx, xyz, nu, w, n, box, levels, n_levels, tol,
+                                    max_newton, flags, stats);
+}
+// test hook: weights of the power-nearest seeds (xyz, w: m seeds) of nq points
+int vgc_emu_power_nearest(void* ctx, const double* xyz, const double* w, uint32_t m,
+                          const double* box, const double* q, uint32_t nq, double* wout) {
+    EmuCtx* c = (EmuCtx*)ctx;
+    vgc::Problem pb;
+    pb.n = (int)m;
+    pb.n_inst = 1;
+    pb.inst_n = {(int)m};
+    pb.box.assign(box, box + 6);
+    pb.d_xyz = xyz;
+    pb.d_w = w;
+    vgc::EngineStats es;
+    return c->eng.power_nearest_weights(pb, q, (int)nq, wout, es) ? VGC_ERR_INTERNAL : VGC_OK;
 }
 int vgc_emu_ot_solve_batch(void* ctx, const double* xyz, const double* nu, double* w,
                            const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,

@@ -378,6 +378,13 @@ __global__ void k_ot_rhs(int n, const double* __restrict__ nu, const double* __r
     if (i < n) b[i] = nu[i] - m[i];
 }
 
+// b -= sum(b) / n (sum from device memory): the Newton right-hand side
+// projected on the range of the Laplacian (whose kernel is the constants)
+__global__ void k_sub_mean(int n, double* b, const double* __restrict__ sum) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] -= *sum / double(n);
+}
+
 __global__ void k_ot_axpy(int n, const double* __restrict__ w, double t,
                           const double* __restrict__ d, double* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -877,6 +884,17 @@ __global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* 
 
 __global__ void k_set0_i64(long long* p) { *p = 0; }
 
+// weight of the power-nearest seed (instance 0 of the tree) of each query
+__global__ void k_power_nearest(const pt4* __restrict__ pts, const bnode* __restrict__ nodes,
+                                InstLayout IL, const double* __restrict__ q, int nq, float cs2,
+                                double* wout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int j = power_nearest_query(pts, nodes, IL, q[3 * (size_t)t + 0], q[3 * (size_t)t + 1],
+                                      q[3 * (size_t)t + 2], cs2);
+    wout[t] = pts[j].w;
+}
+
 // Cells of `items` (all n_items cells if null) that are not plainly done —
 // not CELL_DONE, or carrying a flag of `mask` — appended with their flags
 // (unordered; the host sorts them).  Cells of the instances marked in
@@ -1110,6 +1128,13 @@ struct HipBackend {
         d2h(&cnt, d_count, sizeof(unsigned));
         return cnt;
     }
+    void power_nearest(const pt4* pts, const bnode* nodes, const InstLayout& IL, const double* q,
+                       int nq, float cs2, double* wout) {
+        if (nq > 0)
+            k_power_nearest<<<blocks_for(nq, 256), 256, 0, stream_>>>(pts, nodes, IL, q, nq, cs2,
+                                                                      wout);
+        check();
+    }
     void gather_u32(const int* items, int n, const uint32_t* src, uint32_t* out) {
         if (n > 0) k_gather_u32<<<blocks_for(n, 256), 256, 0, stream_>>>(items, n, src, out);
         check();
@@ -1188,8 +1213,10 @@ struct HipBackend {
         k_ot_spmv<<<blocks_for(n, 256), 256, 0, stream_>>>(n, ptr, col, val, diag, x, y);
         check();
     }
-    void ot_rhs(int n, const double* nu, const double* m, double* b) {
+    void ot_rhs(int n, const double* nu, const double* m, double* b, double* tmp) {
         k_ot_rhs<<<blocks_for(n, 256), 256, 0, stream_>>>(n, nu, m, b);
+        red_sum(n, b, tmp);
+        k_sub_mean<<<blocks_for(n, 256), 256, 0, stream_>>>(n, b, tmp);
         check();
     }
     void ot_axpy(int n, const double* w, double t, const double* d, double* out) {
@@ -1465,6 +1492,15 @@ int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, u
     if (!ctx) return VGC_ERR_ARG;
     if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
     return vgc::api_ot_solve(ctx->impl, xyz, nu, w, n, box, tol, max_newton, flags, stats);
+}
+
+int vgc_ot_solve_levels(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                        const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
+                        uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    return vgc::api_ot_solve_levels(ctx->impl, xyz, nu, w, n, box, levels, n_levels, tol,
+                                    max_newton, flags, stats);
 }
 
 int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double* w,

@@ -136,6 +136,52 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
     }
 }
 
+// ------------------------------------------------- power-nearest queries
+// Seed of instance 0 nearest to x (FP64) in power distance |x - p|^2 - w
+// (ties: the smaller Morton position); returns its Morton position.  Tree
+// nodes are skipped when the FP32 lower bound box_dist2 - wmax exceeds the
+// best FP64 value by more than a margin covering the FP32 roundings (cs2 =
+// squared coordinate scale); seeds are compared in FP64.
+VGC_SD int power_nearest_query(const pt4* pts, const bnode* nodes, const InstLayout& IL, double x,
+                               double y, double z, float cs2) {
+    const int p0 = IL.p0[0], n = IL.n[0], nb = IL.nb[0], L = IL.L[0];
+    const bnode* T = nodes + IL.t0[0];
+    const float px = float(x), py = float(y), pz = float(z);
+    double best = INFINITY;
+    int bi = -1;
+    int node = 1;
+    while (true) {
+        const bnode& B = T[node];
+        const float lb = box_dist2(B, px, py, pz) - B.wmax;
+        const float bf = float(best);
+        const bool inside =
+            bi < 0 || lb <= bf + 1e-5f * (fabsf(lb) + fabsf(bf)) + 1e-6f * cs2;
+        if (inside && node < L) {
+            node = 2 * node;
+            continue;
+        }
+        if (inside) {
+            const int b = node - L;
+            if (b < nb) {
+                const int e = imin(p0 + n, p0 + (b + 1) * kBucket);
+                for (int m = p0 + b * kBucket; m < e; ++m) {
+                    const pt4 q = pts[m];
+                    const double dx = x - q.x, dy = y - q.y, dz = z - q.z;
+                    const double d = dx * dx + dy * dy + dz * dz - q.w;
+                    if (d < best || (d == best && m < bi)) {
+                        best = d;
+                        bi = m;
+                    }
+                }
+            }
+        }
+        while (node & 1) node >>= 1;
+        if (node == 0) break;
+        node += 1;
+    }
+    return bi;
+}
+
 // ------------------------------------------------------ radius queries
 // Seeds (other than q) with FP32 squared distance <= r2 and, with a filter
 // F, that pass the power-distance test (types.h), in traversal order.  Count

@@ -124,7 +124,10 @@ public:
             const long long nnz = E_.total_neighbors;
             double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
             be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
-            be_.ot_rhs(n, d_nu, E_.d_vol, d_b);
+            // Newton system H d = nu - m, projected on the range of H (the
+            // masses and nu can differ in sum by rounding; the constants are
+            // H's kernel, and CG on an inconsistent system stalls)
+            be_.ot_rhs(n, d_nu, E_.d_vol, d_b, d_scal + 4);
             int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_cg,
                          cg_tol, cg_max);
             st.cg_iters += it;

@@ -212,6 +212,18 @@ public:
         return 0;
     }
 
+    // Weights of the power-nearest seeds of pb (one instance, weighted) for
+    // the nq points d_q (FP64 xyz): the start of the new seeds of a finer
+    // level in the multilevel transport solver.
+    int power_nearest_weights(const Problem& pb, const double* d_q, int nq, double* d_wout,
+                              EngineStats& st) {
+        if (build(pb, st)) return -1;
+        double cs = 0.0;
+        for (int k = 0; k < 6; ++k) cs = std::max(cs, std::fabs(pb.box[k]));
+        be_.power_nearest(d_pts, d_nodes, IL, d_q, nq, float(cs * cs), d_wout);
+        return 0;
+    }
+
     // ------------------------------------------------------------ kNN
     // K nearest of every seed (Morton positions), n x K.
     int knn_lists(int K, int*& d_list, float* d_d2 = nullptr) {

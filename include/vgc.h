@@ -177,6 +177,18 @@ int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double
                        const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
                        uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats);
 
+/* Multilevel transport, OptimalTransportMap3d::optimize_levels
+   (compute_OTM/main.cpp:294): the seeds come in levels (a random order, such
+   as geogram's BRIO order, cut at levels[0] = 0 < levels[1] < ... <
+   levels[n_levels] = n).  Level k solves the transport of the first
+   levels[k+1] seeds (their masses rescaled to the box volume) from the
+   previous level's weights, each new seed starting with the weight of its
+   power-nearest seed of the previous level.  stats: n_levels entries (may be
+   null).  Other arguments as vgc_ot_solve; w is output only. */
+int vgc_ot_solve_levels(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
+                        const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
+                        uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats);
+
 /* ---- learned warm start of vgc_ot_solve (no counterpart in geogram) ----
    MLP 6 -> 64 -> 64 -> 1, leaky ReLU, FP32; VGC_MLP_PARAMS floats laid out
    W1[6][64], b1[64], W2[64][64], b2[64], W3[64], b3.  Features of a seed:

@@ -53,3 +53,14 @@ def test_gpu_ot_batch_matches_emulation(gpu_ctx, emu_ctx):
     assert [s["newton_iters"] for s in sg] == [s["newton_iters"] for s in se]
     assert [s["cg_iters"] for s in sg] == [s["cg_iters"] for s in se]
     assert np.array_equal(wg, we)
+
+
+def test_gpu_ot_levels_matches_emulation(gpu_ctx, emu_ctx):
+    n = 4000
+    x, nu = ga.sample_points(0, 41, n), np.full(n, 1.0 / n)
+    perm, lv = ga.brio_levels(n, min_level=400, seed=3)
+    x, nu = x[perm], nu[perm]
+    wg, sg = gpu_ctx.ot_solve_levels(x, nu, lv, tol=1e-5)
+    we, se = emu_ctx.ot_solve_levels(x, nu, lv, tol=1e-5)
+    assert [s["newton_iters"] for s in sg] == [s["newton_iters"] for s in se]
+    assert np.array_equal(wg, we)

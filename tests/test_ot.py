@@ -146,3 +146,40 @@ def test_ot_batch_equals_single_solves(emu_ctx):
         o += s
     with pytest.raises(ga.VgcError):
         emu_ctx.ot_solve_batch(x, nu * 2, sizes, boxes, tol=1e-5)
+
+
+def test_power_nearest_start_weights(emu_ctx):
+    """The start of a finer level's new seeds: the weight of the seed of the
+    previous level nearest in power distance (brute force check)."""
+    import ctypes
+    rng = np.random.default_rng(1)
+    m, nq = 3000, 2000
+    x = rng.random((m, 3))
+    w = rng.random(m) * 0.01
+    q = rng.random((nq, 3))
+    out = np.zeros(nq)
+    box = np.array([0, 0, 0, 1, 1, 1.0])
+    vp = ctypes.c_void_p
+    rc = emu_ctx.lib.cdll.vgc_emu_power_nearest(
+        emu_ctx.ctx, x.ctypes.data_as(vp), w.ctypes.data_as(vp), m, box.ctypes.data_as(vp),
+        q.ctypes.data_as(vp), nq, out.ctypes.data_as(vp))
+    assert rc == 0
+    d = ((q[:, None, :] - x[None]) ** 2).sum(-1) - w[None]
+    assert np.array_equal(out, w[d.argmin(1)])
+
+
+def test_ot_solve_levels(emu_ctx):
+    """optimize_levels: each level converges; the masses of the final weights,
+    re-measured by the oracle, meet the tolerance; bad levels are refused."""
+    n = 4000
+    x, nu = ga.sample_points(0, 41, n), np.full(n, 1.0 / n)
+    perm, lv = ga.brio_levels(n, min_level=400, seed=3)
+    x, nu = x[perm], nu[perm]
+    assert lv.tolist() == [0, 500, 4000]
+    w, sts = emu_ctx.ot_solve_levels(x, nu, lv, tol=1e-5)
+    assert len(sts) == 2 and all(s["converged"] == 1 for s in sts)
+    o = oracle.cells(x, w=w, centred=True, stride=256)
+    assert np.max(np.abs(o["vol"] - nu) / nu) <= 1.01e-5
+    for bad in ([0, 500], [0, 500, 500, 4000], [1, 4000]):
+        with pytest.raises(ga.VgcError):
+            emu_ctx.ot_solve_levels(x, nu, np.array(bad), tol=1e-5)
