@@ -186,11 +186,12 @@ class Context:
 
     # ------------------------------------------------------------------- OT
     def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                 judge_start=False, verbose=False):
+                 judge_start=False, verbose=False, inexact=False):
         """Laguerre weights w with cell masses nu (uniform density on box).
         Returns (w, stats); w0 is the start (None: zeros).  judge_start: keep
         w0 only if its mass residual beats the zero start's.  verbose: one
-        line per Newton step on stderr."""
+        line per Newton step on stderr.  inexact: CG tolerance of each Newton
+        step from its mass error (VGC_OT_INEXACT)."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
@@ -209,7 +210,8 @@ class Context:
                                      int(max_newton),
                                      (VGC_DEVICE_PTRS if dev else 0)
                                      | (_lib.VGC_OT_JUDGE_START if judge_start else 0)
-                                     | (_lib.VGC_OT_VERBOSE if verbose else 0),
+                                     | (_lib.VGC_OT_VERBOSE if verbose else 0)
+                                     | (_lib.VGC_OT_INEXACT if inexact else 0),
                                      ctypes.byref(st))
         self._check(rc)
         return w, st.as_dict()
@@ -242,7 +244,7 @@ class Context:
         return w, [sts[k].as_dict() for k in range(nl)]
 
     def ot_solve_batch(self, points, nu, inst_n, boxes, w0=None, tol=1e-4, max_newton=100,
-                       verbose=False, autotune=False):
+                       verbose=False, autotune=False, inexact=False):
         """Independent transport problems at once (consecutive seeds, one box
         each; each instance's nu sums to its box volume).  Returns (w, list of
         per-instance stats)."""
@@ -263,7 +265,7 @@ class Context:
             raise ValueError("inst_n must add up to the seed count, one box per instance")
         st = (OtStats * len(ia))()
         flags = ((VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_OT_VERBOSE if verbose else 0)
-                 | (_lib.VGC_AUTOTUNE if autotune else 0))
+                 | (_lib.VGC_AUTOTUNE if autotune else 0) | (_lib.VGC_OT_INEXACT if inexact else 0))
         rc = self.lib.fn("ot_solve_batch")(self.ctx, _ptr(pts), _ptr(nu_), _ptr(w),
                                            ia.ctypes.data_as(ctypes.c_void_p), len(ia),
                                            ba.ctypes.data_as(ctypes.c_void_p), float(tol),

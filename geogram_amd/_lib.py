@@ -27,6 +27,7 @@ VGC_COUNT_WORK = 4
 VGC_OT_JUDGE_START = 8
 VGC_OT_VERBOSE = 16
 VGC_AUTOTUNE = 32
+VGC_OT_INEXACT = 64
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6

@@ -438,7 +438,8 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
-                          (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0)) {
+                          (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0,
+                          (flags & VGC_OT_INEXACT) != 0)) {
             ctx->err = "vgc_ot_solve: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
         }
@@ -549,7 +550,7 @@ int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double*
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
-                                (flags & VGC_OT_VERBOSE) != 0)) {
+                                (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
             ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
         }
@@ -874,7 +875,7 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
             pb.d_xyz = dx;
             OtStats os;
             if (ctx->ot.solve(pb, dnl, dw, tol, (int)maxn, 1e-7, 4000, os, false,
-                              (flags & VGC_OT_VERBOSE) != 0)) {
+                              (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
                 ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
                 return VGC_ERR_INTERNAL;
             }

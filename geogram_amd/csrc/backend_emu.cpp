@@ -501,7 +501,7 @@ struct EmuBackend {
             if (sel[inst_of[i]]) dst[i] = src[i];
     }
     void cgb_start(const SegTable& T, const double* b, const double* diag, double* x, double* r,
-                   double* z, double* p, double* cg, double tol2) {
+                   double* z, double* p, double* cg, const double* tol2) {
         seg_part_.assign(2 * (size_t)T.nblk, 0.0);
         for (int blk = 0; blk < T.nblk; ++blk) {
             double t1, t2;
@@ -523,7 +523,7 @@ struct EmuBackend {
             double* c = cg + 8 * k;
             c[0] = t1;
             c[1] = t2;
-            c[5] = tol2 * t2;
+            c[5] = tol2[k] * t2;
             c[6] = t2 > 0.0 ? 0.0 : 1.0;
             c[7] = 0.0;
         }

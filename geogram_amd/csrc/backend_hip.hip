@@ -756,7 +756,8 @@ __device__ __forceinline__ void seg_final2(const SegTable& T, int k, const doubl
 }
 
 __global__ __launch_bounds__(256) void k_cgb_start_final(SegTable T, const double* __restrict__ part,
-                                                         double* cg, double tol2) {
+                                                         double* cg,
+                                                         const double* __restrict__ tol2) {
     __shared__ double sh[256], sh2[256];
     const int k = blockIdx.x;
     double t1, t2;
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(256) void k_cgb_start_final(SegTable T, const doubl
         double* c = cg + 8 * k;
         c[0] = t1;
         c[1] = t2;
-        c[5] = tol2 * t2;
+        c[5] = tol2[k] * t2;
         c[6] = t2 > 0.0 ? 0.0 : 1.0;
         c[7] = 0.0;
     }
@@ -1309,7 +1310,7 @@ struct HipBackend {
         check();
     }
     void cgb_start(const SegTable& T, const double* b, const double* diag, double* x, double* r,
-                   double* z, double* p, double* cg, double tol2) {
+                   double* z, double* p, double* cg, const double* tol2) {
         double* part = seg_part(T.nblk);
         k_cgb_start_partial<<<T.nblk, 256, 0, stream_>>>(T, b, diag, x, r, z, p, part);
         k_cgb_start_final<<<T.n_inst, 256, 0, stream_>>>(T, part, cg, tol2);

@@ -47,8 +47,17 @@ public:
     // caller order.  cg_tol: relative residual of the Newton systems.  judge:
     // also measure the zero start and keep whichever start has the smaller
     // |m - nu|_2 (one more cell pass).
+    // forcing term of the inexact Newton variant: CG relative residual for a
+    // step from max relative mass error err, once the previous step was a
+    // full one (the damped phase needs accurate directions: measured, loose
+    // directions there cost more step halvings than they save)
+    static double forcing(double err, double cg_tol, bool full_step) {
+        return full_step ? std::max(cg_tol, std::min(0.01, 0.01 * err)) : cg_tol;
+    }
+
     int solve(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
-              double cg_tol, int cg_max, OtStats& st, bool judge = false, bool verbose = false) {
+              double cg_tol, int cg_max, OtStats& st, bool judge = false, bool verbose = false,
+              bool inexact = false) {
         auto t0 = std::chrono::steady_clock::now();
         const int n = pb.n;
         pb.want_neighbors = true;
@@ -118,6 +127,7 @@ public:
         const double eps0 = 0.5 * std::min(numin, ms[2]);
         st.err0 = ms[0];
         double err = ms[0], g2 = ms[1];
+        double last_t = 0.0;  // step of the previous Newton iteration
         while (err > tol && st.newton_iters < max_newton) {
             // Newton system H d = nu - m
             auto tg = std::chrono::steady_clock::now();
@@ -129,7 +139,7 @@ public:
             // H's kernel, and CG on an inconsistent system stalls)
             be_.ot_rhs(n, d_nu, E_.d_vol, d_b, d_scal + 4);
             int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_cg,
-                         cg_tol, cg_max);
+                         inexact ? forcing(err, cg_tol, last_t == 1.0) : cg_tol, cg_max);
             st.cg_iters += it;
             st.t_cg_ms += ms_since(tg);
             // damped step
@@ -166,6 +176,7 @@ public:
                 std::fflush(stderr);
             }
             if (!ok) break;
+            last_t = t;
             be_.d2d(d_w, d_wt, sizeof(double) * n);
             st.newton_iters++;
         }
@@ -181,7 +192,8 @@ public:
     // and its PCG keeps per-instance scalars (segmented reductions), each
     // instance has its own damped-Newton state and step.  st[k] per instance.
     int solve_batch(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
-                    double cg_tol, int cg_max, std::vector<OtStats>& st, bool verbose = false) {
+                    double cg_tol, int cg_max, std::vector<OtStats>& st, bool verbose = false,
+                    bool inexact = false) {
         auto t0 = std::chrono::steady_clock::now();
         const int n = pb.n, K = pb.n_inst;
         st.assign(K, OtStats());
@@ -229,6 +241,7 @@ public:
         int* d_sel = (int*)slot(26, sizeof(int) * (size_t)K);
         EngineStats es;
         std::vector<double> ms(3 * (size_t)K), tk(K), err(K), g2(K), eps0(K);
+        std::vector<double> last_t(K, 0.0);  // each instance's previous step
         std::vector<int> sel(K), bad;
         int passes = 0;
         std::vector<int> compute;  // instances whose cells a pass computes
@@ -289,7 +302,14 @@ public:
             be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
             be_.h2d(d_act, active.data(), sizeof(int) * K);
             be_.seg_rhs(n, d_inst, d_act, d_nu, E_.d_vol, d_b);
-            be_.cgb_start(T, d_b, d_diag, d_x, d_r, d_z, d_p, d_cg, cg_tol * cg_tol);
+            std::vector<double> tol2(K);
+            for (int k = 0; k < K; ++k) {
+                const double tk2 = inexact ? forcing(err[k], cg_tol, last_t[k] == 1.0) : cg_tol;
+                tol2[k] = tk2 * tk2;
+            }
+            double* d_tol2 = (double*)slot(28, sizeof(double) * (size_t)K);
+            be_.h2d(d_tol2, tol2.data(), sizeof(double) * (size_t)K);
+            be_.cgb_start(T, d_b, d_diag, d_x, d_r, d_z, d_p, d_cg, d_tol2);
             std::vector<double> h(8 * (size_t)K);
             be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
             int issued = 0;
@@ -329,6 +349,7 @@ public:
                     if (okk) {
                         pending[k] = 0;
                         accepted[k] = 1;
+                        last_t[k] = tk[k];
                         err[k] = ms[3 * k + 0];
                         g2[k] = ms[3 * k + 1];
                     } else {

@@ -41,6 +41,10 @@ extern "C" {
 #define VGC_AUTOTUNE 32u      /* time the first clipping pass with 2 and 4 wavefronts
                                  per work-group on the first call for a problem size
                                  and keep the faster (results do not depend on it) */
+#define VGC_OT_INEXACT 64u    /* transport: inexact Newton — each step's CG stops at
+                                 relative residual min(0.1, 0.1 x the current max
+                                 relative mass error), not below the default 1e-7
+                                 (Eisenstat-Walker forcing) */
 
 typedef struct vgc_ctx vgc_ctx;
 
