@@ -25,7 +25,9 @@ The JSON line also carries:
                 security-radius loop with exact predicates) on the host cores,
                 on a bounded sample of the same workload;
   ot            semi-discrete transport, 1M uniform seeds, equal masses, to
-                1e-4 max relative mass error from w = 0;
+                1e-4 max relative mass error from w = 0 (every transport leg:
+                damped Newton, KMT step rule, CG tolerance from the mass error
+                once the steps are full, VGC_OT_INEXACT);
   ot_warm_start 1M 16-Gaussian seeds with log-normal masses: plain start
                 against the learned MLP start (features + forward timed);
   batched       256 independent instances (N log-uniform in [2k, 200k]):
@@ -150,7 +152,7 @@ def main():
         nu = torch.full((m,), 1.0 / m, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _, ost = ctx.ot_solve(xo, nu, tol=1e-4, verbose=True)
+        _, ost = ctx.ot_solve(xo, nu, tol=1e-4, verbose=True, inexact=True)
         torch.cuda.synchronize()
         tot = time.perf_counter() - t2
         ot = {"n": m, "start": "plain (w = 0)", "time_s": round(tot, 3),
@@ -171,7 +173,7 @@ def main():
         x4d, nu4d = torch.from_numpy(x4).cuda(), torch.from_numpy(nu4).cuda()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _, sp = ctx.ot_solve(x4d, nu4d, tol=1e-4, verbose=True)
+        _, sp = ctx.ot_solve(x4d, nu4d, tol=1e-4, verbose=True, inexact=True)
         torch.cuda.synchronize()
         tp = time.perf_counter() - t2
         ws = {"n": m, "plain": {"time_s": round(tp, 3), "newton_iters": int(sp["newton_iters"]),
@@ -182,7 +184,8 @@ def main():
             torch.cuda.synchronize()
             t2 = time.perf_counter()
             w0 = warmstart.predict_weights(ctx, x4d, nu4d, params=params)
-            _, sl = ctx.ot_solve(x4d, nu4d, w0=w0, tol=1e-4, judge_start=True, verbose=True)
+            _, sl = ctx.ot_solve(x4d, nu4d, w0=w0, tol=1e-4, judge_start=True, verbose=True,
+                                 inexact=True)
             torch.cuda.synchronize()
             tl = time.perf_counter() - t2
             ws["learned"] = {"time_s": round(tl, 3), "newton_iters": int(sl["newton_iters"]),
@@ -226,7 +229,7 @@ def main():
         if args.c5_ot:
             torch.cuda.synchronize()
             t2 = time.perf_counter()
-            _, s5 = ctx.ot_solve_batch(x5, nu5, sizes, boxes5, tol=1e-4, verbose=True,
+            _, s5 = ctx.ot_solve_batch(x5, nu5, sizes, boxes5, tol=1e-4, verbose=True, inexact=True,
                                        autotune=True)
             torch.cuda.synchronize()
             to5 = time.perf_counter() - t2
