@@ -488,9 +488,19 @@ struct EmuBackend {
             out[3 * k + 2] = mn;
         }
     }
-    void seg_rhs(int n, const int* inst_of, const int* active, const double* nu, const double* m,
-                 double* b) {
+    void seg_rhs(const SegTable& T, int n, const int* inst_of, const int* active, const double* nu,
+                 const double* m, double* b) {
         for (int i = 0; i < n; ++i) b[i] = active[inst_of[i]] ? nu[i] - m[i] : 0.0;
+        for (int k = 0; k < T.n_inst; ++k) {
+            double a[256];
+            for (int t = 0; t < 256; ++t) {
+                double s = 0.0;
+                for (long long i = T.off[k] + t; i < T.off[k + 1]; i += 256) s += b[i];
+                a[t] = s;
+            }
+            const double mean = tree_sum(a) / double(T.off[k + 1] - T.off[k]);
+            for (long long i = T.off[k]; i < T.off[k + 1]; ++i) b[i] -= mean;
+        }
     }
     void seg_axpy(int n, const int* inst_of, const double* w, const double* t, const double* d,
                   double* out) {

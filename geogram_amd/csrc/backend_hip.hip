@@ -874,6 +874,20 @@ __global__ __launch_bounds__(256) void k_cgb_beta(SegTable T, const double* __re
     }
 }
 
+// per instance (one block each): b -= mean of b over the instance's rows —
+// the Newton right-hand side projected on the range of its Laplacian
+__global__ __launch_bounds__(256) void k_seg_center(SegTable T, double* b) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    const long long beg = T.off[k], end = T.off[k + 1];
+    double s = 0.0;
+    for (long long i = beg + threadIdx.x; i < end; i += 256) s += b[i];
+    double t, dummy;
+    block_sum2(s, 0.0, sh, sh2, t, dummy);
+    const double mean = t / double(end - beg);
+    for (long long i = beg + threadIdx.x; i < end; i += 256) b[i] -= mean;
+}
+
 __global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* __restrict__ z,
                           double* p, const double* __restrict__ cg) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1295,9 +1309,10 @@ struct HipBackend {
         k_seg_mass<<<T.n_inst, 256, 0, stream_>>>(T, m, nu, out);
         check();
     }
-    void seg_rhs(int n, const int* inst_of, const int* active, const double* nu, const double* m,
-                 double* b) {
+    void seg_rhs(const SegTable& T, int n, const int* inst_of, const int* active, const double* nu,
+                 const double* m, double* b) {
         k_seg_rhs<<<blocks_for(n, 256), 256, 0, stream_>>>(n, inst_of, active, nu, m, b);
+        k_seg_center<<<T.n_inst, 256, 0, stream_>>>(T, b);
         check();
     }
     void seg_axpy(int n, const int* inst_of, const double* w, const double* t, const double* d,

@@ -301,7 +301,8 @@ public:
             double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
             be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
             be_.h2d(d_act, active.data(), sizeof(int) * K);
-            be_.seg_rhs(n, d_inst, d_act, d_nu, E_.d_vol, d_b);
+            // per instance projected on the range of its Laplacian (as solve)
+            be_.seg_rhs(T, n, d_inst, d_act, d_nu, E_.d_vol, d_b);
             std::vector<double> tol2(K);
             for (int k = 0; k < K; ++k) {
                 const double tk2 = inexact ? forcing(err[k], cg_tol, last_t[k] == 1.0) : cg_tol;
