@@ -701,15 +701,23 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
             LB internal = node < Lv;
             LB push = hit & internal;
             LB leaf = hit & !internal;
-            uint64_t mp = ballot(push), ml = ballot(leaf);
-            LI rp = nn + 2 * mbcnt(mp);
-            LB okp = push & (rp + 1 < kFrontCap);
-            lds_st(&L.front[cur ^ 1][0], select(okp, rp, LI(0)), 2 * node, okp);
-            lds_st(&L.front[cur ^ 1][0], select(okp, rp + 1, LI(0)), 2 * node + 1, okp);
+            // a node whose children are internal too pushes its four
+            // grandchildren (the children's boxes would only prune): half the
+            // dependent rounds down the tree
+            LB deep = push & (2 * node < Lv);
+            uint64_t mp = ballot(push), md = ballot(deep), ml = ballot(leaf);
+            LI rp = nn + 2 * mbcnt(mp) + 2 * mbcnt(md);
+            LI first = select(deep, 4 * node, 2 * node);
+            LB okp = push & (rp + select(deep, LI(3), LI(1)) < kFrontCap);
+            lds_st(&L.front[cur ^ 1][0], select(okp, rp, LI(0)), first, okp);
+            lds_st(&L.front[cur ^ 1][0], select(okp, rp + 1, LI(0)), first + 1, okp);
+            LB okd = okp & deep;
+            lds_st(&L.front[cur ^ 1][0], select(okd, rp + 2, LI(0)), first + 2, okd);
+            lds_st(&L.front[cur ^ 1][0], select(okd, rp + 3, LI(0)), first + 3, okd);
             LI rl = nbk + mbcnt(ml);
             LB okl = leaf & (rl < kBucketCap);
             lds_st(L.bucket, select(okl, rl, LI(0)), node - Lv, okl);
-            nn += 2 * popc64(mp);
+            nn += 2 * popc64(mp) + 2 * popc64(md);
             nbk += popc64(ml);
         }
         wave_sync();
@@ -901,25 +909,32 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
         if (F.on) hit = hit & pf_node_may_cut(F, dc2, B.wm, vsqrtf(dmin), vsqrtf(dmax), fx, fy, fz);
         LB internal = node < Lv;
         LB push = hit & internal;
-        uint64_t mp = ballot(push);
+        // nodes whose children are internal too push their four grandchildren
+        LB deep = push & (2 * node < Lv);
+        uint64_t mp = ballot(push), md = ballot(deep);
         // no room for every child: drop the farthest subtrees, and bound the
         // ring below the nearest distance of what was dropped
-        while (top + 2 * popc64(mp) > kStack) {
+        while (top + 2 * popc64(mp) + 2 * popc64(md) > kStack) {
             const float far = wave_maxf(select(push, dmin, LF(-1.0f)));
             LB drop = push & (dmin >= far);
             push = push & !drop;
+            deep = deep & !drop;
             hit = hit & !drop;
             mp = ballot(push);
+            md = ballot(deep);
             const float nl = nextafterf(far, 0.0f);
             if (!(nl > rlo2)) return -1;
             lim = nl < lim ? nl : lim;
         }
         LB leaf = hit & !internal & (dmin <= lim);
         uint64_t ml = ballot(leaf);
-        LI rp = top + 2 * mbcnt(mp);
-        lds_st(stk, select(push, rp, LI(0)), 2 * node, push);
-        lds_st(stk, select(push, rp + 1, LI(0)), 2 * node + 1, push);
-        top += 2 * popc64(mp);
+        LI rp = top + 2 * mbcnt(mp) + 2 * mbcnt(md);
+        LI first = select(deep, 4 * node, 2 * node);
+        lds_st(stk, select(push, rp, LI(0)), first, push);
+        lds_st(stk, select(push, rp + 1, LI(0)), first + 1, push);
+        lds_st(stk, select(deep, rp + 2, LI(0)), first + 2, deep);
+        lds_st(stk, select(deep, rp + 3, LI(0)), first + 3, deep);
+        top += 2 * popc64(mp) + 2 * popc64(md);
         lds_st(L.bucket, select(leaf, mbcnt(ml), LI(0)), node - Lv, leaf);
         const int nl = popc64(ml);
         wave_sync();
