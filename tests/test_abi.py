@@ -22,7 +22,7 @@ def test_exports_every_declared_entry_point():
     assert {"vgc_create", "vgc_destroy", "vgc_last_error", "vgc_knn", "vgc_cells",
             "vgc_cells_batch", "vgc_lloyd", "vgc_ot_solve", "vgc_random_float64",
             "vgc_sample_points", "vgc_cvt_func_grad", "vgc_lloyd_batch",
-            "vgc_ot_solve_batch"} <= set(names)
+            "vgc_ot_solve_batch", "vgc_ot_solve_levels"} <= set(names)
     lib = ctypes.CDLL(LIB)
     for nm in names:
         assert hasattr(lib, nm), nm
