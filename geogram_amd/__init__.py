@@ -106,11 +106,12 @@ class Context:
 
     # ---------------------------------------------------------------- cells
     def cells(self, points, weights=None, box=UNIT_BOX, k0=0, neighbors=True,
-              inst_n=None, boxes=None, count_work=False):
+              inst_n=None, boxes=None, count_work=False, autotune=False):
         """Voronoi (weights None) or Laguerre cells clipped to box.
 
         For a batch of instances give inst_n (seed counts, consecutive in
-        points) and boxes (6 per instance)."""
+        points) and boxes (6 per instance).  autotune: pick the first pass's
+        work-group size by timing (VGC_AUTOTUNE; results do not depend on it)."""
         dev = _is_torch_cuda(points)
         if dev:
             import torch
@@ -127,7 +128,8 @@ class Context:
             vol = np.empty(n)
             mg = np.empty((n, 3))
             ptr = np.empty(n + 1, dtype=np.uint64)
-        flags = (VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
+        flags = ((VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
+                 | (_lib.VGC_AUTOTUNE if autotune else 0))
         st = Stats()
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)

@@ -174,7 +174,7 @@ struct EmuBackend {
 
     void set_clip_waves(int w) { waves_ = w == 2 ? 2 : 4; }
     int clip_waves() const { return waves_; }
-    int waves_ = 4;
+    int waves_ = kDefaultClipWaves;
     void clip(int S, bool exact, bool gather, const ClipParams& P) {
         if (exact) {
             if (gather)

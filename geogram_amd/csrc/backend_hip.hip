@@ -1415,7 +1415,7 @@ private:
     const void* cgb_key_[21] = {};
     double* seg_part_ = nullptr;
     int seg_part_cap_ = 0;
-    int waves_ = 4;
+    int waves_ = kDefaultClipWaves;
 };
 
 typedef Ctx<HipBackend> HipCtx;

@@ -319,7 +319,7 @@ public:
         }
         if (!autotune) {
             // without the tuner every call runs the default work-group size
-            be_.set_clip_waves(4);
+            be_.set_clip_waves(kDefaultClipWaves);
             tuned_n_ = -1;
         }
         if (autotune && P.n_items > 0 && tuned_n_ != n_) {
@@ -357,7 +357,9 @@ public:
                 double& t = wv == 2 ? t2w : t4w;
                 t = ms < t ? ms : t;
             }
-            const int best_w = t2w < 0.97 * t4w ? 2 : 4;  // 4 unless 2 is clearly faster
+            // the default unless the other is clearly faster
+            const int best_w = kDefaultClipWaves == 2 ? (t4w < 0.97 * t2w ? 4 : 2)
+                                                      : (t2w < 0.97 * t4w ? 2 : 4);
             be_.set_clip_waves(best_w);
             tuned_n_ = n_;
             st.tuned_waves = best_w;
