@@ -129,7 +129,7 @@ def main():
     clip_ms = float(np.mean([t["t_clip_a_gpu_ms"] for t in stats]))
     achieved = flops_a / (clip_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "k_clip<1,2,false,true,false,false> (first clipping pass, every cell)",
+        "kernel": "k_clip<1,1,false,true,false,false> (first clipping pass, every cell)",
         "bound": "valu",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,

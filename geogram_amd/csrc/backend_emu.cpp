@@ -172,7 +172,7 @@ struct EmuBackend {
         }
     }
 
-    void set_clip_waves(int w) { waves_ = w == 2 ? 2 : 4; }
+    void set_clip_waves(int w) { waves_ = w == 1 ? 1 : 4; }
     int clip_waves() const { return waves_; }
     int waves_ = kDefaultClipWaves;
     void clip(int S, bool exact, bool gather, const ClipParams& P) {

@@ -1060,12 +1060,12 @@ struct HipBackend {
             if constexpr (G && !W) {
                 if (P.corner) {
                     // second gather of Voronoi cells, with the corner test
-                    k_clip<1, 4, E, G, W, true><<<g4, 256, 0, stream_>>>(P);
+                    k_clip<1, 1, E, G, W, true><<<g1, 64, 0, stream_>>>(P);
                     return;
                 }
             }
-            if (waves_ == 2)
-                k_clip<1, 2, E, G, W><<<blocks_for(P.n_items, 2), 128, 0, stream_>>>(P);
+            if (waves_ == 1)
+                k_clip<1, 1, E, G, W><<<g1, 64, 0, stream_>>>(P);
             else
                 k_clip<1, 4, E, G, W><<<g4, 256, 0, stream_>>>(P);
         } else if (S == 2) {
@@ -1087,9 +1087,9 @@ struct HipBackend {
         else
             clip_egw<E, G, false>(S, P);
     }
-    // wavefronts per work-group of the S = 1 kernels (2 or 4); the cells'
+    // wavefronts per work-group of the S = 1 kernels (1 or 4); the cells'
     // results do not depend on it
-    void set_clip_waves(int w) { waves_ = w == 2 ? 2 : 4; }
+    void set_clip_waves(int w) { waves_ = w == 1 ? 1 : 4; }
     int clip_waves() const { return waves_; }
     void clip(int S, bool exact, bool gather, const ClipParams& P) {
         if (P.n_items <= 0) return;

@@ -75,10 +75,11 @@ struct CornerBits {
 constexpr int kCandCap = 128;   // candidates per distance ring
 constexpr int kFrontCap = 320;  // tree nodes per level of the walk (two levels: the depth-first stack)
 constexpr int kBucketCap = 128; // buckets per ring
-// wavefronts per work-group of the first pass (2 or 4; same results): 2
-// measured 9 % faster on the 2M-seed mixture (finer-grained scheduling of the
-// 3 waves per SIMD)
-constexpr int kDefaultClipWaves = 2;
+// wavefronts per work-group of the first pass (1 or 4; same results): a
+// work-group's LDS is released when its last wave ends, so one long cell
+// holds its group's slots; 1 measured 13 % faster than 4 on the 2M-seed
+// mixture (2: 9 %)
+constexpr int kDefaultClipWaves = 1;
 constexpr int kMaxFilteredRings = 48;  // power-filtered rings of the first pass
 // Voronoi cells: filtered rings of the first pass; cells that need more go to
 // a second gather with the corner test (clustered seeds: large cells beside

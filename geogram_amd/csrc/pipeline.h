@@ -323,7 +323,7 @@ public:
             tuned_n_ = -1;
         }
         if (autotune && P.n_items > 0 && tuned_n_ != n_) {
-            // autotuner: time the first pass with 2 and 4 wavefronts per
+            // autotuner: time the first pass with 1 and 4 wavefronts per
             // work-group (same results) and keep the faster for this size
             // (interleaved, twice each, minimum kept: the first launch of a
             // problem runs on cold caches).  Large problems are timed on a
@@ -346,20 +346,20 @@ public:
                 T.items = d_t;
                 T.n_items = (int)titems.size();
             }
-            double t2w = 1e300, t4w = 1e300;
-            for (int wv : {4, 2, 4, 2}) {
+            double t1w = 1e300, t4w = 1e300;
+            for (int wv : {4, 1, 4, 1}) {
                 be_.set_clip_waves(wv);
                 be_.event_begin(1);
                 be_.clip(1, false, gather, T);
                 be_.event_end(1);
                 be_.sync();
                 const double ms = be_.event_ms(1);
-                double& t = wv == 2 ? t2w : t4w;
+                double& t = wv == 1 ? t1w : t4w;
                 t = ms < t ? ms : t;
             }
             // the default unless the other is clearly faster
-            const int best_w = kDefaultClipWaves == 2 ? (t4w < 0.97 * t2w ? 4 : 2)
-                                                      : (t2w < 0.97 * t4w ? 2 : 4);
+            const int best_w = kDefaultClipWaves == 1 ? (t4w < 0.97 * t1w ? 4 : 1)
+                                                      : (t1w < 0.97 * t4w ? 1 : 4);
             be_.set_clip_waves(best_w);
             tuned_n_ = n_;
             st.tuned_waves = best_w;

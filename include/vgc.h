@@ -38,7 +38,7 @@ extern "C" {
                                  the start with the smaller mass residual */
 #define VGC_OT_VERBOSE 16u    /* vgc_ot_solve: one progress line per Newton step
                                  on stderr */
-#define VGC_AUTOTUNE 32u      /* time the first clipping pass with 2 and 4 wavefronts
+#define VGC_AUTOTUNE 32u      /* time the first clipping pass with 1 and 4 wavefronts
                                  per work-group on the first call for a problem size
                                  and keep the faster (results do not depend on it) */
 #define VGC_OT_INEXACT 64u    /* transport: inexact Newton — each step's CG stops at
