@@ -136,8 +136,6 @@ __global__ __launch_bounds__(256) void k_knn(const pf4* __restrict__ pf,
     knn_query<K>(pf, nodes, IL, q, out + (size_t)t * K, out_d2 ? out_d2 + (size_t)t * K : nullptr);
 }
 
-// S = 1 (every cell's first pass): registers capped for 3 waves per SIMD
-// (latency-bound kernel; occupancy hides the dependent LDS round trips)
 // Work-groups go to the 8 XCDs of the chip in turn; renumber them so that
 // each XCD takes one contiguous range of work-groups (cells in Morton order:
 // neighbouring cells read the same seeds and tree nodes, from that XCD's L2).
@@ -149,8 +147,12 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < r ? x : r) + b / kXcds;
 }
 
+// S = 1 (every cell's first pass): registers capped at 128 for 4 waves per
+// SIMD (some spill to scratch); with one-wave work-groups that measured 7 %
+// faster than 168 registers and 3 waves — the wait for dependent LDS and
+// memory round trips is hidden by occupancy
 template <int S, int WAVES, bool EXACT, bool GATHER, bool W, bool CORNER = W>
-__global__ __launch_bounds__(64 * WAVES, S == 1 ? 3 : 1) void k_clip(ClipParams P) {
+__global__ __launch_bounds__(64 * WAVES, S == 1 ? 4 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = xcd_block() * WAVES + wave;
