@@ -32,6 +32,7 @@ UNIT_BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
 __all__ = [
     "Context", "CellResult", "NearestNeighborSearch", "RestrictedVoronoiDiagram",
     "CentroidalVoronoiTesselation", "OptimalTransportMap3d", "default_context", "cells", "knn",
+    "knn_search",
     "lloyd", "ot_solve",
     "random_float64", "sample_points", "Stats", "VgcError", "UNIT_BOX",
 ]
@@ -99,6 +100,14 @@ class Context:
         except Exception:
             pass
 
+    def _stream(self, dev):
+        """Device arrays: the library waits for the work queued before the
+        call on torch's current stream (vgc_set_stream)."""
+        if dev and self.lib.prefix == "vgc_":
+            import torch
+            self.lib.cdll.vgc_set_stream(self.ctx,
+                                         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
     def _check(self, rc, allow_partial=True):
         if rc == VGC_OK or (rc == VGC_PARTIAL and allow_partial):
             return rc
@@ -106,13 +115,15 @@ class Context:
 
     # ---------------------------------------------------------------- cells
     def cells(self, points, weights=None, box=UNIT_BOX, k0=0, neighbors=True,
-              inst_n=None, boxes=None, count_work=False, autotune=False):
+              inst_n=None, boxes=None, count_work=False, autotune=False, waves4=False):
         """Voronoi (weights None) or Laguerre cells clipped to box.
 
         For a batch of instances give inst_n (seed counts, consecutive in
         points) and boxes (6 per instance).  autotune: pick the first pass's
-        work-group size by timing (VGC_AUTOTUNE; results do not depend on it)."""
+        work-group size by timing (VGC_AUTOTUNE; results do not depend on it).
+        waves4: run the first pass with 4-wavefront work-groups (VGC_WAVES4)."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
@@ -129,7 +140,7 @@ class Context:
             mg = np.empty((n, 3))
             ptr = np.empty(n + 1, dtype=np.uint64)
         flags = ((VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
-                 | (_lib.VGC_AUTOTUNE if autotune else 0))
+                 | (_lib.VGC_AUTOTUNE if autotune else 0) | (_lib.VGC_WAVES4 if waves4 else 0))
         st = Stats()
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)
@@ -169,20 +180,50 @@ class Context:
 
     # ------------------------------------------------------------------ kNN
     def knn(self, points, k):
+        """k nearest other points of every point, exact (FP64 squared
+        distances, ties by index), with the coincident-point rule of
+        Delaunay_NearestNeighbors (vgc_knn).  Returns (idx int32 (n, k),
+        d2 float64 (n, k)); rows that end early are padded with -1."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
             n = pts.shape[0]
             idx = torch.empty((n, k), dtype=torch.int32, device=pts.device)
-            d2 = torch.empty((n, k), dtype=torch.float32, device=pts.device)
+            d2 = torch.empty((n, k), dtype=torch.float64, device=pts.device)
         else:
             pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
             n = pts.shape[0]
             idx = np.empty((n, k), dtype=np.int32)
-            d2 = np.empty((n, k), dtype=np.float32)
+            d2 = np.empty((n, k), dtype=np.float64)
         rc = self.lib.fn("knn")(self.ctx, _ptr(pts), n, int(k), VGC_DEVICE_PTRS if dev else 0,
                                 _ptr(idx), _ptr(d2))
+        self._check(rc, allow_partial=False)
+        return idx, d2
+
+    def knn_search(self, points, queries, k):
+        """k nearest points of each query point (NearestNeighborSearch::
+        get_nearest_neighbors by coordinates, vgc_knn_search): exact FP64
+        squared distances, ascending, ties by index.  Returns (idx, d2),
+        each (nq, k)."""
+        dev = _is_torch_cuda(points)
+        self._stream(dev)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            q = queries.contiguous().to(torch.float64).reshape(-1, 3)
+            nq = q.shape[0]
+            idx = torch.empty((nq, k), dtype=torch.int32, device=pts.device)
+            d2 = torch.empty((nq, k), dtype=torch.float64, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            q = np.ascontiguousarray(queries, dtype=np.float64).reshape(-1, 3)
+            nq = q.shape[0]
+            idx = np.empty((nq, k), dtype=np.int32)
+            d2 = np.empty((nq, k), dtype=np.float64)
+        rc = self.lib.fn("knn_search")(self.ctx, _ptr(pts), pts.shape[0], _ptr(q), nq, int(k),
+                                       VGC_DEVICE_PTRS if dev else 0, _ptr(idx), _ptr(d2))
         self._check(rc, allow_partial=False)
         return idx, d2
 
@@ -195,6 +236,7 @@ class Context:
         line per Newton step on stderr.  inexact: CG tolerance of each Newton
         step from its mass error (VGC_OT_INEXACT)."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
@@ -223,6 +265,7 @@ class Context:
         """Multilevel transport (optimize_levels): points in level order,
         levels = [0, ..., n] (see brio_levels).  Returns (w, per-level stats)."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
@@ -251,6 +294,7 @@ class Context:
         each; each instance's nu sums to its box volume).  Returns (w, list of
         per-instance stats)."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
@@ -279,6 +323,7 @@ class Context:
     def ot_features(self, points, nu, box=UNIT_BOX):
         """Per-seed MLP inputs (n x 6, float32) of the learned OT warm start."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             pts = points.contiguous().to(torch.float64).reshape(-1, 3)
@@ -298,6 +343,7 @@ class Context:
 
     def mlp_forward(self, params, feats):
         dev = _is_torch_cuda(feats)
+        self._stream(dev)
         n = feats.shape[0]
         if dev:
             import torch
@@ -315,6 +361,7 @@ class Context:
         """Gradient of the mean squared error of the MLP over the rows;
         returns (grad, loss)."""
         dev = _is_torch_cuda(feats)
+        self._stream(dev)
         n = feats.shape[0]
         if dev:
             import torch
@@ -333,6 +380,7 @@ class Context:
 
     def mlp_adam(self, params, grad, m, v, step, lr):
         """One in-place Adam step on device tensors (params, m, v)."""
+        self._stream(True)
         rc = self.lib.fn("mlp_adam")(self.ctx, _ptr(params), _ptr(grad), _ptr(m), _ptr(v),
                                      int(step), float(lr), VGC_DEVICE_PTRS)
         self._check(rc, allow_partial=False)
@@ -344,6 +392,7 @@ class Context:
         RVD.cpp:792-899).  Returns (f, g); g is an (n, 3) array of the same
         kind as points."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             x = points.detach().to(torch.float64).contiguous().reshape(-1, 3)
@@ -367,6 +416,7 @@ class Context:
         seeds, one box each).  autotune: pick the first pass's work-group size
         by timing (results do not depend on it)."""
         dev = _is_torch_cuda(points)
+        self._stream(dev)
         if dev:
             import torch
             x = points.detach().clone().to(torch.float64).contiguous().reshape(-1, 3)
@@ -406,6 +456,10 @@ def cells(points, weights=None, box=UNIT_BOX, k0=0, neighbors=True, device=0):
 
 def knn(points, k, device=0):
     return default_context(device).knn(points, k)
+
+
+def knn_search(points, queries, k, device=0):
+    return default_context(device).knn_search(points, queries, k)
 
 
 def lloyd(points, iters, box=UNIT_BOX, k0=0, device=0):
@@ -451,8 +505,11 @@ def sample_points(kind: int, seed: int, n: int) -> np.ndarray:
 # ------------------------------------------------------- reference mirrors
 
 class NearestNeighborSearch:
-    """points/nn_search.h: set_points then nearest neighbours of every point
-    (self excluded, ascending distance, FP32 ranking), k in {16,24,32,48,64}."""
+    """points/nn_search.h: set_points, then exact nearest neighbours (FP64
+    squared distances, ascending, ties by index) of query points
+    (get_nearest_neighbors by coordinates, nn_search.h:129-134, batched over
+    the rows of the query array) or of stored points by index (the point
+    itself included, nn_search.h:180-185)."""
 
     def __init__(self, dimension: int = 3, ctx: Context | None = None):
         if dimension != 3:
@@ -463,9 +520,27 @@ class NearestNeighborSearch:
     def set_points(self, points):
         self.points = points
 
-    def get_nearest_neighbors(self, k: int):
+    def nb_points(self):
+        return self.points.shape[0]
+
+    def get_nearest_neighbors(self, nb_neighbors: int, query_points):
+        """query_points: (nq, 3) coordinates, or an int / int array of stored
+        point indices.  Returns (neighbors, sq_dist), each (nq, nb_neighbors)."""
         ctx = self.ctx or default_context()
-        return ctx.knn(self.points, k)
+        q = query_points
+        if isinstance(q, (int, np.integer)) or (isinstance(q, np.ndarray) and q.dtype.kind in "iu"):
+            q = self.points[np.atleast_1d(q)]
+        return ctx.knn_search(self.points, q, nb_neighbors)
+
+    def get_nearest_neighbor(self, query_point):
+        idx, _ = self.get_nearest_neighbors(1, np.asarray(query_point, dtype=np.float64).reshape(1, 3))
+        return int(idx[0, 0])
+
+    def get_neighbors_of_all(self, nb_neighbors: int):
+        """Every stored point's nearest others with the coincident-point rule
+        of Delaunay_NearestNeighbors (delaunay/delaunay_nn.cpp:105-143)."""
+        ctx = self.ctx or default_context()
+        return ctx.knn(self.points, nb_neighbors)
 
 
 class RestrictedVoronoiDiagram:

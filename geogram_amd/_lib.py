@@ -28,6 +28,7 @@ VGC_OT_JUDGE_START = 8
 VGC_OT_VERBOSE = 16
 VGC_AUTOTUNE = 32
 VGC_OT_INEXACT = 64
+VGC_WAVES4 = 128
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6
@@ -76,6 +77,9 @@ def _declare(L, prefix):
     f = L[prefix + "knn"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _u32, _u32, _vp, _vp]
+    f = L[prefix + "knn_search"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, _vp, _vp]
     f = L[prefix + "cells"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u64,
@@ -130,7 +134,7 @@ class _Lib:
         self.cdll = ctypes.CDLL(path)
         self.prefix = prefix
         self._f = {}
-        names = ["last_error", "knn", "cells", "cells_batch", "lloyd", "lloyd_batch",
+        names = ["last_error", "knn", "knn_search", "cells", "cells_batch", "lloyd", "lloyd_batch",
                  "cvt_func_grad", "ot_solve", "ot_solve_levels",
                  "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
@@ -141,6 +145,7 @@ class _Lib:
             self._f["vgc_create"].argtypes = [ctypes.POINTER(_vp), ctypes.c_int]
             self.cdll.vgc_random_float64.argtypes = [_u64, _vp, _u64]
             self.cdll.vgc_sample_points.argtypes = [_u32, _u64, _u32, _vp]
+            self.cdll.vgc_set_stream.argtypes = [_vp, _vp]
         else:
             self._f["vgc_emu_create"].argtypes = [ctypes.POINTER(_vp)]
         self._f[prefix + "destroy"].argtypes = [_vp]

@@ -141,6 +141,7 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
         }
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         if (ctx->eng.cells(pb, es)) {
             ctx->err = "vgc_cells: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -220,6 +221,7 @@ int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* 
         }
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         Engine<B>& E = ctx->eng;
         if (E.cells(pb, es)) {
             ctx->err = "vgc_cvt_func_grad: " + E.error;
@@ -246,21 +248,48 @@ int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* 
     }
 }
 
+// Bounding box of n FP64 points (device memory copied to the host when
+// dev) for the Morton quantisation, and their largest coordinate magnitude.
 template <class B>
-int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
-            float* d2) {
+void host_extent(B& be, const double* xyz, uint32_t n, bool dev, double lo[3], double hi[3],
+                 double& S) {
+    std::vector<double> hx;
+    const double* hp = xyz;
+    if (dev) {
+        hx.resize(3 * (size_t)n);
+        be.d2h(hx.data(), xyz, sizeof(double) * 3 * (size_t)n);
+        hp = hx.data();
+    }
+    for (int c = 0; c < 3; ++c) lo[c] = hi[c] = hp[c];
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], hp[3 * (size_t)i + c]);
+            hi[c] = std::max(hi[c], hp[3 * (size_t)i + c]);
+        }
+    S = 0.0;
+    for (int c = 0; c < 3; ++c) S = std::max(S, std::max(std::fabs(lo[c]), std::fabs(hi[c])));
+}
+
+// Exact kNN (FP64 squared distances, ties by caller's index) of the n points
+// xyz: of every point itself (q == nullptr: k nearest others, with the
+// coincident-point rule) or of nq query points q (k nearest points, none
+// excluded).
+template <class B>
+int knn_common(Ctx<B>* ctx, const char* name, const double* xyz, uint32_t n, const double* q,
+               uint32_t nq, uint32_t k, uint32_t flags, int32_t* nbr, double* d2) {
     if (!ctx) return VGC_ERR_ARG;
     try {
-        if (!xyz || !nbr || n < 2) {
-            ctx->err = "vgc_knn: bad argument";
+        const bool self = q == nullptr;
+        if (!xyz || !nbr || n < 1 || k < 1 || (!self && nq < 1)) {
+            ctx->err = std::string(name) + ": bad argument";
             return VGC_ERR_ARG;
         }
-        if (!valid_k0(k)) {
-            ctx->err = "vgc_knn: k must be 16, 24, 32, 48 or 64";
+        if ((self && k > n - 1) || (!self && k > n)) {
+            ctx->err = std::string(name) + (self ? ": k exceeds n - 1" : ": k exceeds n");
             return VGC_ERR_ARG;
         }
-        if (k > n - 1) {
-            ctx->err = "vgc_knn: k exceeds n-1";
+        if (n > (1u << 30) || (uint64_t)(self ? n : nq) * k > (1ull << 34)) {
+            ctx->err = std::string(name) + ": problem too large";
             return VGC_ERR_ARG;
         }
         B& be = ctx->be;
@@ -269,47 +298,52 @@ int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t fla
         pb.n = (int)n;
         pb.n_inst = 1;
         pb.inst_n = {(int)n};
-        // bounding box of the points for the Morton quantisation
-        std::vector<double> hx;
-        const double* hp = xyz;
+        double lo[3], hi[3], S;
+        host_extent(be, xyz, n, dev, lo, hi, S);
+        if (!self) {
+            double qlo[3], qhi[3], qS;
+            host_extent(be, q, nq, dev, qlo, qhi, qS);
+            S = std::max(S, qS);
+        }
+        if (!(S < 1e300)) {
+            ctx->err = std::string(name) + ": coordinates must be finite";
+            return VGC_ERR_ARG;
+        }
+        // Morton box of the stored points (queries outside it are clamped)
+        pb.box = {lo[0], lo[1], lo[2], hi[0] + 1e-300 + (hi[0] - lo[0]) * 1e-12,
+                  hi[1] + 1e-300 + (hi[1] - lo[1]) * 1e-12, hi[2] + 1e-300 + (hi[2] - lo[2]) * 1e-12};
+        for (int c = 0; c < 3; ++c)
+            if (!(pb.box[3 + c] > pb.box[c])) pb.box[3 + c] = pb.box[c] + 1.0;
+        const double* dq = q;
         if (dev) {
-            hx.resize(3 * (size_t)n);
-            be.d2h(hx.data(), xyz, sizeof(double) * 3 * (size_t)n);
-            hp = hx.data();
             pb.d_xyz = xyz;
         } else {
             double* dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
             be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
             pb.d_xyz = dx;
-        }
-        double lo[3] = {hp[0], hp[1], hp[2]}, hi[3] = {hp[0], hp[1], hp[2]};
-        for (uint32_t i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::min(lo[c], hp[3 * (size_t)i + c]);
-                hi[c] = std::max(hi[c], hp[3 * (size_t)i + c]);
+            if (!self) {
+                double* sq = (double*)ctx->staging(5, sizeof(double) * 3 * (size_t)nq);
+                be.h2d(sq, q, sizeof(double) * 3 * (size_t)nq);
+                dq = sq;
             }
-        pb.box = {lo[0], lo[1], lo[2], hi[0] + 1e-300 + (hi[0] - lo[0]) * 1e-12,
-                  hi[1] + 1e-300 + (hi[1] - lo[1]) * 1e-12, hi[2] + 1e-300 + (hi[2] - lo[2]) * 1e-12};
-        for (int c = 0; c < 3; ++c)
-            if (!(pb.box[3 + c] > pb.box[c])) pb.box[3 + c] = pb.box[c] + 1.0;
+        }
         EngineStats es;
         Engine<B>& E = ctx->eng;
         if (E.build(pb, es)) {
-            ctx->err = "vgc_knn: " + E.error;
+            ctx->err = std::string(name) + ": " + E.error;
             return VGC_ERR_INTERNAL;
         }
-        int* d_list = nullptr;
-        float* d_d2 = (float*)ctx->staging(2, sizeof(float) * (size_t)n * k);
-        E.knn_lists((int)k, d_list, d_d2);
-        int* d_out = (int*)ctx->staging(3, sizeof(int) * (size_t)n * k);
-        float* d_outd = (float*)ctx->staging(4, sizeof(float) * (size_t)n * k);
-        be.knn_to_caller(E.d_orig, d_list, d_d2, (int)n, (int)k, d_out, d_outd);
-        if (dev) {
-            be.d2d(nbr, d_out, sizeof(int) * (size_t)n * k);
-            if (d2) be.d2d(d2, d_outd, sizeof(float) * (size_t)n * k);
-        } else {
-            be.d2h(nbr, d_out, sizeof(int) * (size_t)n * k);
-            if (d2) be.d2h(d2, d_outd, sizeof(float) * (size_t)n * k);
+        const size_t rows = self ? n : nq;
+        int* d_out = dev ? nbr : (int*)ctx->staging(3, sizeof(int) * rows * k);
+        double* d_outd = (dev && d2) ? d2 : (double*)ctx->staging(4, sizeof(double) * rows * k);
+        if (E.knn_exact(self ? nullptr : dq, (int)rows, (int)(self ? k + 1 : k), self, S, d_out,
+                        d_outd, es)) {
+            ctx->err = std::string(name) + ": " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        if (!dev) {
+            be.d2h(nbr, d_out, sizeof(int) * rows * k);
+            if (d2) be.d2h(d2, d_outd, sizeof(double) * rows * k);
         }
         be.sync();
         return VGC_OK;
@@ -317,6 +351,22 @@ int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t fla
         ctx->err = e.what();
         return VGC_ERR_DEVICE;
     }
+}
+
+template <class B>
+int api_knn(Ctx<B>* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
+            double* d2) {
+    return knn_common(ctx, "vgc_knn", xyz, n, nullptr, 0u, k, flags, nbr, d2);
+}
+
+template <class B>
+int api_knn_search(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* q, uint32_t nq,
+                   uint32_t k, uint32_t flags, int32_t* nbr, double* d2) {
+    if (!q) {
+        if (ctx) ctx->err = "vgc_knn_search: null query points";
+        return VGC_ERR_ARG;
+    }
+    return knn_common(ctx, "vgc_knn_search", xyz, n, q, nq, k, flags, nbr, d2);
 }
 
 template <class B>
@@ -361,6 +411,7 @@ int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n
         }
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         if (ctx->eng.lloyd(pb, dx, (int)iters, es)) {
             ctx->err = "vgc_lloyd: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -437,6 +488,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         OtStats os;
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                           (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0,
                           (flags & VGC_OT_INEXACT) != 0)) {
@@ -549,6 +601,7 @@ int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double*
         std::vector<OtStats> os;
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                                 (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
             ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;
@@ -837,6 +890,7 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
         double* dnl = (double*)ctx->staging(7, sizeof(double) * (size_t)n);
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         bool all_conv = true;
         std::vector<double> hl;
         for (uint32_t k = 0; k < n_levels; ++k) {

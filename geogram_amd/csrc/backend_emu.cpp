@@ -649,6 +649,86 @@ struct EmuBackend {
         for (int i = 0; i < n; ++i) ot_features_row(i, xyz, nu, d2, box, inv_cell, h, f + (long long)i * kMlpIn);
     }
 
+    // ------------------------------------------------------- exact kNN
+    void iota(int* d, int n) {
+        for (int i = 0; i < n; ++i) d[i] = i;
+    }
+    void lower_bound_u64(const uint64_t* keys, int n, const uint64_t* q, int nq, int* out) {
+        for (int t = 0; t < nq; ++t) {
+            const int lo = int(std::lower_bound(keys, keys + n, q[t]) - keys);
+            out[t] = lo < n ? lo : n - 1;
+        }
+    }
+    template <int K>
+    void knn_seed_k(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                    const int* qpos, int nq, int kk, double e, float* rr) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int t = 0; t < nq; ++t) rr[t] = knn_seed_radius_row<K>(pf, nodes, IL, qf, qpos, t, kk, e);
+    }
+    void knn_seed_radius(int K, const pf4* pf, const bnode* nodes, const InstLayout& IL,
+                         const pf4* qf, const int* qpos, int nq, int kk, double e, float* rr) {
+        switch (K) {
+            case 16: knn_seed_k<16>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 24: knn_seed_k<24>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 32: knn_seed_k<32>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 48: knn_seed_k<48>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 64: knn_seed_k<64>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            default: throw std::runtime_error("unsupported K");
+        }
+    }
+    void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                   const int* items, int ni, const float* rr, int* cnt) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < ni; ++i) {
+            const pf4 q = qf[items[i]];
+            cnt[i] = radius_query_pt(pf, nodes, IL, 0, q.x, q.y, q.z, -1, rr[items[i]], nullptr,
+                                     nullptr, nullptr);
+        }
+    }
+    void knn_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                  const int* items, int ni, const float* rr, const long long* off, int* lm,
+                  float* lf) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < ni; ++i) {
+            const pf4 q = qf[items[i]];
+            radius_query_pt(pf, nodes, IL, 0, q.x, q.y, q.z, -1, rr[items[i]], nullptr, lm + off[i],
+                            lf + off[i]);
+        }
+    }
+    void knn_d64(const pt4* qp, const pt4* pts, const int* orig, const int* items, int ni,
+                 const long long* off, const int* lm, int* li, double* ld) {
+        for (int i = 0; i < ni; ++i) knn_d64_row(qp, pts, orig, items[i], off[i], off[i + 1], lm, li, ld);
+    }
+    // stable sort of each segment by key (as a radix sort is)
+    template <class K, class V>
+    void seg_sort_kv(const K* kin, const V* vin, K* kout, V* vout, long long, int nseg,
+                     const long long* off) {
+        for (int sg = 0; sg < nseg; ++sg) {
+            const long long b = off[sg], e = off[sg + 1];
+            std::vector<long long> idx(e - b);
+            std::iota(idx.begin(), idx.end(), b);
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](long long x, long long y) { return kin[x] < kin[y]; });
+            for (long long k = b; k < e; ++k) {
+                kout[k] = kin[idx[k - b]];
+                vout[k] = vin[idx[k - b]];
+            }
+        }
+    }
+    void knn_select(const int* items, int ni, const long long* off, const int* cnt,
+                    const double* ld, const int* li, float* rr, double e, int kk, int n,
+                    int self_rule, const int* qrow, int* out_idx, double* out_d2, int* redo,
+                    unsigned* nredo) {
+        for (int i = 0; i < ni; ++i) {
+            const int t = items[i];
+            if (!knn_select_row(t, off[i], cnt[i], ld, li, rr[t], e, kk, n, self_rule, qrow, out_idx,
+                                out_d2)) {
+                rr[t] = knn_grow(rr[t]);
+                redo[(*nredo)++] = t;
+            }
+        }
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         for (int q = 0; q < n; ++q) {
@@ -680,8 +760,12 @@ const char* vgc_emu_last_error(const void* ctx) {
     return ctx ? ((const EmuCtx*)ctx)->err.c_str() : "null context";
 }
 int vgc_emu_knn(void* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
-                float* d2) {
+                double* d2) {
     return vgc::api_knn((EmuCtx*)ctx, xyz, n, k, flags, nbr, d2);
+}
+int vgc_emu_knn_search(void* ctx, const double* xyz, uint32_t n, const double* q, uint32_t nq,
+                       uint32_t k, uint32_t flags, int32_t* nbr, double* d2) {
+    return vgc::api_knn_search((EmuCtx*)ctx, xyz, n, q, nq, k, flags, nbr, d2);
 }
 int vgc_emu_cells(void* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
                   uint32_t k0, uint32_t flags, double* vol, double* mg, uint64_t* nbr_ptr,

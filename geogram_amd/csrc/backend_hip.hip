@@ -147,12 +147,13 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < r ? x : r) + b / kXcds;
 }
 
-// S = 1 (every cell's first pass): registers capped at 128 for 4 waves per
-// SIMD (some spill to scratch); with one-wave work-groups that measured 7 %
-// faster than 168 registers and 3 waves — the wait for dependent LDS and
-// memory round trips is hidden by occupancy
+// S = 1 with FP64 signs (every cell's first pass, and the second gather):
+// registers capped at 128 for 4 waves per SIMD (some spill to scratch); with
+// one-wave work-groups that measured 7 % faster than 168 registers and 3
+// waves — the wait for dependent LDS and memory round trips is hidden by
+// occupancy.  The exact re-clip (a few cells) keeps its registers.
 template <int S, int WAVES, bool EXACT, bool GATHER, bool W, bool CORNER = W>
-__global__ __launch_bounds__(64 * WAVES, S == 1 ? 4 : 1) void k_clip(ClipParams P) {
+__global__ __launch_bounds__(64 * WAVES, (S == 1 && !EXACT) ? 4 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = xcd_block() * WAVES + wave;
@@ -239,6 +240,83 @@ __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restr
         int j = list[(size_t)q * K + k];
         out[(size_t)i * K + k] = j >= 0 ? orig[j] : -1;
         outd[(size_t)i * K + k] = d2[(size_t)q * K + k];
+    }
+}
+
+// ------------------------------------------------------- exact kNN kernels
+__global__ void k_iota(int* d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = i;
+}
+
+// first position of each query key in the sorted keys (clamped to n - 1)
+__global__ void k_lower_bound_u64(const uint64_t* __restrict__ keys, int n,
+                                  const uint64_t* __restrict__ q, int nq, int* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const uint64_t k = q[t];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[t] = lo < n ? lo : n - 1;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_knn_seed_radius(const pf4* __restrict__ pf,
+                                                         const bnode* __restrict__ nodes,
+                                                         InstLayout IL, const pf4* __restrict__ qf,
+                                                         const int* __restrict__ qpos, int nq,
+                                                         int kk, double e, float* rr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nq) rr[t] = knn_seed_radius_row<K>(pf, nodes, IL, qf, qpos, t, kk, e);
+}
+
+__global__ void k_knn_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
+                            InstLayout IL, const pf4* __restrict__ qf, const int* __restrict__ items,
+                            int ni, const float* __restrict__ rr, int* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int t = items[i];
+    const pf4 q = qf[t];
+    cnt[i] = radius_query_pt(pf, nodes, IL, 0, q.x, q.y, q.z, -1, rr[t], nullptr, nullptr, nullptr);
+}
+
+__global__ void k_knn_fill(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
+                           InstLayout IL, const pf4* __restrict__ qf, const int* __restrict__ items,
+                           int ni, const float* __restrict__ rr, const long long* __restrict__ off,
+                           int* lm, float* lf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int t = items[i];
+    const pf4 q = qf[t];
+    radius_query_pt(pf, nodes, IL, 0, q.x, q.y, q.z, -1, rr[t], nullptr, lm + off[i], lf + off[i]);
+}
+
+__global__ void k_knn_d64(const pt4* __restrict__ qp, const pt4* __restrict__ pts,
+                          const int* __restrict__ orig, const int* __restrict__ items, int ni,
+                          const long long* __restrict__ off, const int* __restrict__ lm, int* li,
+                          double* ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ni) knn_d64_row(qp, pts, orig, items[i], off[i], off[i + 1], lm, li, ld);
+}
+
+__global__ void k_knn_select(const int* __restrict__ items, int ni,
+                             const long long* __restrict__ off, const int* __restrict__ cnt,
+                             const double* __restrict__ ld, const int* __restrict__ li, float* rr,
+                             double e, int kk, int n, int self_rule, const int* __restrict__ qrow,
+                             int* out_idx, double* out_d2, int* redo, unsigned* nredo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int t = items[i];
+    if (!knn_select_row(t, off[i], cnt[i], ld, li, rr[t], e, kk, n, self_rule, qrow, out_idx,
+                        out_d2)) {
+        rr[t] = knn_grow(rr[t]);
+        redo[atomicAdd(nredo, 1u)] = t;
     }
 }
 
@@ -956,9 +1034,20 @@ struct HipBackend {
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
+        if (enter_ev_) (void)hipEventDestroy(enter_ev_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     hipStream_t stream() const { return stream_; }
+
+    // The caller's stream (the null stream unless set): each call first
+    // waits, on the context's own stream, for the work queued there before
+    // it — device inputs written by the caller's asynchronous kernels.
+    void set_caller_stream(hipStream_t s) { caller_ = s; }
+    void enter() {
+        if (!enter_ev_) VGC_HIP_CHECK(hipEventCreateWithFlags(&enter_ev_, hipEventDisableTiming));
+        VGC_HIP_CHECK(hipEventRecord(enter_ev_, caller_));
+        VGC_HIP_CHECK(hipStreamWaitEvent(stream_, enter_ev_, 0));
+    }
 
     void* dalloc(size_t b) {
         void* p = nullptr;
@@ -1398,6 +1487,72 @@ struct HipBackend {
         check();
     }
 
+    // ------------------------------------------------------- exact kNN
+    void iota(int* d, int n) {
+        if (n > 0) k_iota<<<blocks_for(n, 256), 256, 0, stream_>>>(d, n);
+        check();
+    }
+    void lower_bound_u64(const uint64_t* keys, int n, const uint64_t* q, int nq, int* out) {
+        if (nq > 0) k_lower_bound_u64<<<blocks_for(nq, 256), 256, 0, stream_>>>(keys, n, q, nq, out);
+        check();
+    }
+    void knn_seed_radius(int K, const pf4* pf, const bnode* nodes, const InstLayout& IL,
+                         const pf4* qf, const int* qpos, int nq, int kk, double e, float* rr) {
+        const unsigned g = blocks_for(nq, 256);
+        switch (K) {
+            case 16: k_knn_seed_radius<16><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 24: k_knn_seed_radius<24><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 32: k_knn_seed_radius<32><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 48: k_knn_seed_radius<48><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            case 64: k_knn_seed_radius<64><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
+            default: throw std::runtime_error("unsupported K");
+        }
+        check();
+    }
+    void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                   const int* items, int ni, const float* rr, int* cnt) {
+        if (ni > 0) k_knn_count<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, qf, items, ni, rr, cnt);
+        check();
+    }
+    void knn_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                  const int* items, int ni, const float* rr, const long long* off, int* lm,
+                  float* lf) {
+        if (ni > 0)
+            k_knn_fill<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, qf, items, ni, rr, off, lm, lf);
+        check();
+    }
+    void knn_d64(const pt4* qp, const pt4* pts, const int* orig, const int* items, int ni,
+                 const long long* off, const int* lm, int* li, double* ld) {
+        if (ni > 0)
+            k_knn_d64<<<blocks_for(ni, 64), 64, 0, stream_>>>(qp, pts, orig, items, ni, off, lm, li, ld);
+        check();
+    }
+    template <class K, class V>
+    void seg_sort_kv(const K* kin, const V* vin, K* kout, V* vout, long long total, int nseg,
+                     const long long* off) {
+        if (total <= 0) return;
+        if (total > 0x7FFFFFFFll || nseg < 0)
+            throw std::runtime_error("seg_sort: list chunk exceeds 2^31 entries");
+        size_t tb = 0;
+        VGC_HIP_CHECK(rocprim::segmented_radix_sort_pairs(nullptr, tb, kin, kout, vin, vout,
+                                                          (unsigned)total, (unsigned)nseg, off,
+                                                          off + 1, 0, int(8 * sizeof(K)), stream_));
+        void* t = temp(tb);
+        VGC_HIP_CHECK(rocprim::segmented_radix_sort_pairs(t, tb, kin, kout, vin, vout,
+                                                          (unsigned)total, (unsigned)nseg, off,
+                                                          off + 1, 0, int(8 * sizeof(K)), stream_));
+    }
+    void knn_select(const int* items, int ni, const long long* off, const int* cnt,
+                    const double* ld, const int* li, float* rr, double e, int kk, int n,
+                    int self_rule, const int* qrow, int* out_idx, double* out_d2, int* redo,
+                    unsigned* nredo) {
+        if (ni > 0)
+            k_knn_select<<<blocks_for(ni, 256), 256, 0, stream_>>>(items, ni, off, cnt, ld, li, rr, e,
+                                                                   kk, n, self_rule, qrow, out_idx,
+                                                                   out_d2, redo, nredo);
+        check();
+    }
+
     void knn_to_caller(const int* orig, const int* list, const float* d2, int n, int K, int* out,
                        float* outd) {
         k_knn_to_caller<<<blocks_for(n, 256), 256, 0, stream_>>>(orig, list, d2, n, K, out, outd);
@@ -1407,6 +1562,8 @@ struct HipBackend {
 private:
     int dev_;
     hipStream_t stream_ = nullptr;
+    hipStream_t caller_ = nullptr;
+    hipEvent_t enter_ev_ = nullptr;
     hipEvent_t ev_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     void* temp_ = nullptr;
     size_t temp_bytes_ = 0;
@@ -1427,6 +1584,20 @@ typedef Ctx<HipBackend> HipCtx;
 struct vgc_ctx {
     vgc::HipCtx* impl;
 };
+
+// Every call makes the context's device current and orders the context's
+// stream after the work queued on the caller's stream before the call.
+static int vgc_enter(vgc_ctx* ctx) {
+    if (!ctx || !ctx->impl) return VGC_ERR_ARG;
+    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    try {
+        ctx->impl->be.enter();
+    } catch (const std::exception& e) {
+        ctx->impl->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+    return VGC_OK;
+}
 
 extern "C" {
 
@@ -1452,23 +1623,33 @@ void vgc_destroy(vgc_ctx* ctx) {
     delete ctx;
 }
 
+int vgc_set_stream(vgc_ctx* ctx, void* stream) {
+    if (!ctx || !ctx->impl) return VGC_ERR_ARG;
+    ctx->impl->be.set_caller_stream((hipStream_t)stream);
+    return VGC_OK;
+}
+
 const char* vgc_last_error(const vgc_ctx* ctx) {
     if (!ctx || !ctx->impl) return "null context";
     return ctx->impl->err.c_str();
 }
 
 int vgc_knn(vgc_ctx* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags, int32_t* nbr,
-            float* d2) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+            double* d2) {
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_knn(ctx->impl, xyz, n, k, flags, nbr, d2);
+}
+
+int vgc_knn_search(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* q, uint32_t nq,
+                   uint32_t k, uint32_t flags, int32_t* nbr, double* d2) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_knn_search(ctx->impl, xyz, n, q, nq, k, flags, nbr, d2);
 }
 
 int vgc_cells(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
               uint32_t k0, uint32_t flags, double* vol, double* mg, uint64_t* nbr_ptr,
               int32_t* nbr, double* farea, uint64_t cap, vgc_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_cells(ctx->impl, xyz, w, &n, 1u, box, k0, flags, vol, mg, nbr_ptr, nbr, farea,
                           cap, stats);
 }
@@ -1477,46 +1658,40 @@ int vgc_cells_batch(vgc_ctx* ctx, const double* xyz, const double* w, const uint
                     uint32_t n_inst, const double* boxes, uint32_t k0, uint32_t flags, double* vol,
                     double* mg, uint64_t* nbr_ptr, int32_t* nbr, double* farea, uint64_t cap,
                     vgc_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_cells(ctx->impl, xyz, w, inst_n, n_inst, boxes, k0, flags, vol, mg, nbr_ptr, nbr,
                           farea, cap, stats);
 }
 
 int vgc_cvt_func_grad(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* box,
                       uint32_t flags, double* f, double* g, vgc_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_cvt_func_grad(ctx->impl, xyz, n, box, flags, f, g, stats);
 }
 
 int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
               uint32_t k0, uint32_t flags, vgc_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_lloyd(ctx->impl, xyz, n, box, iters, k0, flags, stats);
 }
 int vgc_lloyd_batch(vgc_ctx* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
                     const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
                     vgc_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_lloyd_batch(ctx->impl, xyz, inst_n, n_inst, boxes, iters, k0, flags, stats);
 }
 
 int vgc_ot_solve(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                  const double* box, double tol, uint32_t max_newton, uint32_t flags,
                  vgc_ot_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_ot_solve(ctx->impl, xyz, nu, w, n, box, tol, max_newton, flags, stats);
 }
 
 int vgc_ot_solve_levels(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                         const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
                         uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_ot_solve_levels(ctx->impl, xyz, nu, w, n, box, levels, n_levels, tol,
                                     max_newton, flags, stats);
 }
@@ -1524,33 +1699,28 @@ int vgc_ot_solve_levels(vgc_ctx* ctx, const double* xyz, const double* nu, doubl
 int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double* w,
                        const uint32_t* inst_n, uint32_t n_inst, const double* boxes, double tol,
                        uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_ot_solve_batch(ctx->impl, xyz, nu, w, inst_n, n_inst, boxes, tol, max_newton,
                                    flags, stats);
 }
 int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t n,
                     const double* box, uint32_t flags, float* feats) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_ot_features(ctx->impl, xyz, nu, n, box, flags, feats);
 }
 int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint32_t n,
                     uint32_t flags, float* out) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_mlp_forward(ctx->impl, params, feats, n, flags, out);
 }
 int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const float* target,
                  uint32_t n, uint32_t flags, float* grad, double* loss) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_mlp_grad(ctx->impl, params, feats, target, n, flags, grad, loss);
 }
 int vgc_mlp_adam(vgc_ctx* ctx, float* params, const float* grad, float* m, float* v,
                  uint32_t step, float lr, uint32_t flags) {
-    if (!ctx) return VGC_ERR_ARG;
-    if (hipSetDevice(ctx->impl->device) != hipSuccess) return VGC_ERR_DEVICE;
+    if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_mlp_adam(ctx->impl, params, grad, m, v, step, lr, flags);
 }
 

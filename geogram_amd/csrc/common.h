@@ -79,16 +79,15 @@ VGC_SD void knn_scan_bucket(const pf4* pf, int p0, int n, int b, int self, float
     }
 }
 
+// K nearest seeds of instance `inst` to the FP32 point (px, py, pz), seed
+// `self` (a Morton position, or -1) excluded; the walk starts with the
+// buckets around Morton position qpos (the query's place in the order).
 template <int K>
-VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
-                      int* out, float* out_d2) {
-    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+VGC_SD void knn_query_pt(const pf4* pf, const bnode* nodes, const InstLayout& IL, int inst,
+                         int qpos, float px, float py, float pz, int self, float (&bd)[K],
+                         int (&bi)[K]) {
     const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
     const bnode* T = nodes + IL.t0[inst];
-    const pf4 pq = pf[q];
-    const float px = pq.x, py = pq.y, pz = pq.z;
-    float bd[K];
-    int bi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         bd[k] = INFINITY;
@@ -96,19 +95,19 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
     }
     // Morton neighbours first: buckets bq, bq-1, bq+1, bq-2, ... until K
     // other seeds have been seen, so the tree walk starts with a finite radius
-    const int bq = (q - p0) / kBucket;
+    const int bq = (qpos - p0) / kBucket;
     int blo = bq, bhi = bq;
-    knn_scan_bucket<K>(pf, p0, n, bq, q, px, py, pz, bd, bi);
-    int seen = imin(p0 + n, p0 + (bq + 1) * kBucket) - (p0 + bq * kBucket) - 1;
+    knn_scan_bucket<K>(pf, p0, n, bq, self, px, py, pz, bd, bi);
+    int seen = imin(p0 + n, p0 + (bq + 1) * kBucket) - (p0 + bq * kBucket) - (self >= 0 ? 1 : 0);
     while (seen < K && (blo > 0 || bhi < nb - 1)) {
         if (blo > 0) {
             --blo;
-            knn_scan_bucket<K>(pf, p0, n, blo, q, px, py, pz, bd, bi);
+            knn_scan_bucket<K>(pf, p0, n, blo, self, px, py, pz, bd, bi);
             seen += kBucket;
         }
         if (bhi < nb - 1) {
             ++bhi;
-            knn_scan_bucket<K>(pf, p0, n, bhi, q, px, py, pz, bd, bi);
+            knn_scan_bucket<K>(pf, p0, n, bhi, self, px, py, pz, bd, bi);
             seen += imin(p0 + n, p0 + (bhi + 1) * kBucket) - (p0 + bhi * kBucket);
         }
     }
@@ -123,17 +122,133 @@ VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, i
         if (inside) {
             int b = node - L;
             if (b < nb && (b < blo || b > bhi))
-                knn_scan_bucket<K>(pf, p0, n, b, q, px, py, pz, bd, bi);
+                knn_scan_bucket<K>(pf, p0, n, b, self, px, py, pz, bd, bi);
         }
         while (node & 1) node >>= 1;
         if (node == 0) break;
         node += 1;
     }
+}
+
+template <int K>
+VGC_SD void knn_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
+                      int* out, float* out_d2) {
+    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+    const pf4 pq = pf[q];
+    float bd[K];
+    int bi[K];
+    knn_query_pt<K>(pf, nodes, IL, inst, q, pq.x, pq.y, pq.z, q, bd, bi);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         out[k] = bi[k];
         if (out_d2) out_d2[k] = bd[k];
     }
+}
+
+// ------------------------------------------------------- exact kNN
+// Exact k nearest neighbours (FP64 distances) by FP32 radius lists: an FP32
+// radius that holds at least kk points, every point within it listed, the
+// list sorted by (FP64 squared distance, caller's index), and the kk-th
+// entry certified nearer than any point left out.
+//
+// e: bound of the error of an FP32 difference vector against the exact one,
+// from the FP32 rounding of coordinates of magnitude <= S:
+// per axis 2 u S (two roundings) + u 2 S (the subtraction), u = 2^-24, so
+// |v32 - v| <= sqrt(3) 4 u S (rounded up below).
+VGC_SD double knn_vec_err(double S) { return 7.0 * 5.9604644775390625e-8 * S * (1.0 + 1e-6); }
+
+// FP32 squared radius that holds every point whose FP64 squared distance is
+// at most that of a point found at FP32 squared distance d32.
+VGC_SD float knn_list_radius(float d32, double e) {
+    const double g = 1.0 + 4.0 * 5.9604644775390625e-8;  // 1 + gamma_3 (FP32), rounded up
+    const double a = sqrt(double(d32) * g) * (1.0 + 1e-6) + 2.5 * e;
+    return f_up(a * a * g * (1.0 + 1e-6));
+}
+
+// Lower bound of the FP64 squared distance (as computed in FP64) of a point
+// whose FP32 squared distance exceeds r32.
+VGC_SD double knn_outside_bound(float r32, double e) {
+    const double g = 1.0 + 4.0 * 5.9604644775390625e-8;
+    double a = sqrt(double(r32) / g) - e;
+    if (!(a > 0.0)) return 0.0;
+    return a * a * (1.0 - 4e-15);
+}
+
+// FP32 squared radius of query t of the exact kNN: from the K nearest points
+// found in FP32 (the kk-th, or beyond K the K-th scaled by the volume ratio,
+// checked later by the count).
+template <int K>
+VGC_SD float knn_seed_radius_row(const pf4* pf, const bnode* nodes, const InstLayout& IL,
+                                 const pf4* qf, const int* qpos, int t, int kk, double e) {
+    float bd[K];
+    int bi[K];
+    const pf4 q = qf[t];
+    knn_query_pt<K>(pf, nodes, IL, 0, qpos ? qpos[t] : t, q.x, q.y, q.z, -1, bd, bi);
+    float d = bd[(kk < K ? kk : K) - 1];
+    if (kk > K) d *= 1.5f * powf(float(kk) / float(K), 2.0f / 3.0f);
+    if (!(d < INFINITY)) return INFINITY;
+    return knn_list_radius(d, e);
+}
+
+// Listed entries of query t: caller's index and FP64 squared distance (as
+// computed by Geom::distance2, a sum of three squared differences).
+VGC_SD void knn_d64_row(const pt4* qp, const pt4* pts, const int* orig, int t, long long b,
+                        long long e, const int* lm, int* li, double* ld) {
+    const pt4 q = qp[t];
+    for (long long k = b; k < e; ++k) {
+        const int m = lm[k];
+        const pt4 p = pts[m];
+        const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        li[k] = orig[m];
+        ld[k] = dx * dx + dy * dy + dz * dz;
+    }
+}
+
+// Query t's sorted list [b, b + c): 1 when its kk-th entry is certified (or
+// every point is listed) and the row is written, 0 when the radius must grow.
+VGC_SD int knn_select_row(int t, long long b, int c, const double* ld, const int* li, float r32,
+                          double e, int kk, int n, int self_rule, const int* qrow, int* out_idx,
+                          double* out_d2) {
+    if (c < kk && c < n) return 0;
+    if (c < n && !(ld[b + kk - 1] < knn_outside_bound(r32, e))) return 0;
+    const int kout = self_rule ? kk - 1 : kk;
+    const long long row = (long long)qrow[t] * kout;
+    int m = 0;
+    if (!self_rule) {
+        for (; m < kk; ++m) {
+            out_idx[row + m] = li[b + m];
+            out_d2[row + m] = ld[b + m];
+        }
+    } else {
+        const int own = qrow[t];
+        bool disconnected = false;
+        for (int j = 0; j < kk && m < kout; ++j) {
+            const int id = li[b + j];
+            if (id == own) continue;
+            if (ld[b + j] == 0.0) {
+                if (id < own) {
+                    disconnected = true;
+                    break;
+                }
+                continue;
+            }
+            out_idx[row + m] = id;
+            out_d2[row + m] = ld[b + j];
+            ++m;
+        }
+        if (disconnected) m = 0;
+    }
+    for (; m < kout; ++m) {
+        out_idx[row + m] = -1;
+        out_d2[row + m] = -1.0;
+    }
+    return 1;
+}
+
+// radius of a query whose list must grow
+VGC_SD float knn_grow(float r32) {
+    const float g = 4.0f * r32;
+    return g > 1e-30f ? g : 1e-30f;
 }
 
 // ------------------------------------------------- power-nearest queries
@@ -186,13 +301,11 @@ VGC_SD int power_nearest_query(const pt4* pts, const bnode* nodes, const InstLay
 // Seeds (other than q) with FP32 squared distance <= r2 and, with a filter
 // F, that pass the power-distance test (types.h), in traversal order.  Count
 // mode when out == nullptr.
-VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
-                        float r2, const PowerFilter* F, int* out, float* out_d2) {
-    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+VGC_SD int radius_query_pt(const pf4* pf, const bnode* nodes, const InstLayout& IL, int inst,
+                           float px, float py, float pz, int self, float r2, const PowerFilter* F,
+                           int* out, float* out_d2) {
     const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
     const bnode* T = nodes + IL.t0[inst];
-    const pf4 pq = pf[q];
-    const float px = pq.x, py = pq.y, pz = pq.z;
     const bool filt = F && F->on;
     int cnt = 0;
     int node = 1;
@@ -218,7 +331,7 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
                 int e = s + kBucket;
                 if (e > p0 + n) e = p0 + n;
                 for (int m = s; m < e; ++m) {
-                    if (m == q) continue;
+                    if (m == self) continue;
                     const pf4 a = pf[m];
                     float d = dist2f(a, px, py, pz);
                     bool take = d <= r2;
@@ -240,6 +353,13 @@ VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL,
         node += 1;
     }
     return cnt;
+}
+
+VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
+                        float r2, const PowerFilter* F, int* out, float* out_d2) {
+    const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+    const pf4 pq = pf[q];
+    return radius_query_pt(pf, nodes, IL, inst, pq.x, pq.y, pq.z, q, r2, F, out, out_d2);
 }
 
 // ---------------------------------------------------------- tree build

@@ -92,6 +92,7 @@ public:
     long long total_neighbors = 0;
     long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
     bool autotune = false;             // pick the first pass's work-group size by timing
+    bool waves4 = false;               // first pass with 4-wavefront work-groups (no tuner)
     std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
     std::vector<uint32_t> h_flags;  // by Morton position, current for census_ cells
     std::vector<int> census_;       // cells of the last pass that are not plainly done
@@ -232,6 +233,112 @@ public:
         return 0;
     }
 
+    // ------------------------------------------------------- exact kNN
+    // Exact nearest neighbours among the points of the last build (one
+    // instance) of nq query points: per query, the kk points of smallest
+    // (FP64 squared distance, caller's index) — FP32 radius lists (see
+    // common.h), grown until the kk-th entry is certified.  d_q: FP64 query
+    // points in the caller's order (device); nullptr: the built points
+    // themselves.  S: largest coordinate magnitude of points and queries.
+    // Output rows (query order, kout = kk, or kk - 1 with self_rule) of caller
+    // indices / FP64 squared distances, -1 / -1.0 where there is none.
+    // self_rule: each point's own entry is dropped and coincident points
+    // follow Delaunay_NearestNeighbors::get_neighbors_internal
+    // (delaunay/delaunay_nn.cpp:105-143): a coincident point of smaller index
+    // leaves the row empty, one of larger index is skipped.
+    int knn_exact(const double* d_q, int nq, int kk, bool self_rule, double S, int* d_idx,
+                  double* d_d2, EngineStats& st) {
+        auto t0 = now();
+        if (ni_ != 1 || kk < 1 || kk > n_) {
+            error = "knn: one instance and 1 <= k <= number of points";
+            return -1;
+        }
+        const pt4* qp = d_pts;
+        const pf4* qf = d_pf;
+        const int* qrow = d_orig;
+        const int* qpos = nullptr;  // Morton place of each query (built points: itself)
+        if (d_q) {
+            uint64_t* k1 = (uint64_t*)slot(S_QKEYS, sizeof(uint64_t) * (size_t)nq);
+            uint64_t* k2 = (uint64_t*)slot(S_QKEYS2, sizeof(uint64_t) * (size_t)nq);
+            int* v1 = (int*)slot(S_QVALS, sizeof(int) * (size_t)nq);
+            int* v2 = (int*)slot(S_QVALS2, sizeof(int) * (size_t)nq);
+            be_.morton(d_q, nullptr, d_box_, nq, k1, v1);
+            be_.sort_pairs(k1, v1, k2, v2, nq, 48);
+            pt4* qp_ = (pt4*)slot(S_QPTS, sizeof(pt4) * (size_t)nq);
+            pf4* qf_ = (pf4*)slot(S_QPF, sizeof(pf4) * (size_t)nq);
+            int* qrow_ = (int*)slot(S_QROW, sizeof(int) * (size_t)nq);
+            int* qpos_ = (int*)slot(S_QPOS, sizeof(int) * (size_t)nq);
+            be_.gather(d_q, nullptr, nullptr, v2, nq, qp_, qf_, qrow_, nullptr);
+            be_.lower_bound_u64((const uint64_t*)slot(S_KEYS2, 0), n_, k2, nq, qpos_);
+            qp = qp_;
+            qf = qf_;
+            qrow = qrow_;
+            qpos = qpos_;
+        }
+        const double e = knn_vec_err(S);
+        float* rr = (float*)slot(S_QR2, sizeof(float) * (size_t)nq);
+        const int Kp = kk <= 16 ? 16 : kk <= 24 ? 24 : kk <= 32 ? 32 : kk <= 48 ? 48 : 64;
+        be_.knn_seed_radius(Kp, d_pf, d_nodes, IL, qf, qpos, nq, kk, e, rr);
+        std::vector<int> items;
+        bool all = true;
+        int* d_redo = (int*)slot(S_QREDO, sizeof(int) * (size_t)nq);
+        unsigned* d_nredo = (unsigned*)slot(S_CENS_N, sizeof(unsigned));
+        for (int round = 0;; ++round) {
+            const int ni = all ? nq : (int)items.size();
+            if (ni == 0) break;
+            if (round >= 96) {
+                error = "knn: radius did not converge";
+                return -1;
+            }
+            int* d_items = (int*)slot(S_ITEMS, sizeof(int) * (size_t)ni);
+            if (all)
+                be_.iota(d_items, ni);
+            else
+                be_.h2d(d_items, items.data(), sizeof(int) * (size_t)ni);
+            int* d_cnt = (int*)slot(S_RCNT, sizeof(int) * (size_t)ni);
+            be_.knn_count(d_pf, d_nodes, IL, qf, d_items, ni, rr, d_cnt);
+            long long* d_off = (long long*)slot(S_ROFF, sizeof(long long) * ((size_t)ni + 1));
+            be_.scan_i32(d_cnt, d_off, ni);
+            std::vector<long long> off(ni + 1);
+            be_.d2h(off.data(), d_off, sizeof(long long) * ((size_t)ni + 1));
+            st.radius_entries += off[ni];
+            be_.dzero(d_nredo, sizeof(unsigned));
+            for (int a = 0; a < ni;) {
+                int b = a + 1;
+                while (b < ni && off[b + 1] - off[a] <= list_chunk) ++b;
+                const long long base = off[a], total = off[b] - base;
+                const int nc = b - a;
+                const size_t cap = (size_t)std::max<long long>(total, 1);
+                int* l_m = (int*)slot(S_RLIST, sizeof(int) * cap);
+                float* l_f = (float*)slot(S_RD2, sizeof(float) * cap);
+                int* l_i = (int*)slot(S_RLIST2, sizeof(int) * cap);
+                int* l_i2 = (int*)slot(S_QLI2, sizeof(int) * cap);
+                double* l_d = (double*)slot(S_QLD, sizeof(double) * cap);
+                double* l_d2 = (double*)slot(S_QLD2, sizeof(double) * cap);
+                long long* d_coff = (long long*)slot(S_COFF, sizeof(long long) * ((size_t)nc + 1));
+                be_.rebase(d_off + a, nc + 1, base, d_coff);
+                be_.knn_fill(d_pf, d_nodes, IL, qf, d_items + a, nc, rr, d_coff, l_m, l_f);
+                be_.knn_d64(qp, d_pts, d_orig, d_items + a, nc, d_coff, l_m, l_i, l_d);
+                // (FP64 squared distance, caller's index): index first, then a
+                // stable sort by distance
+                be_.seg_sort_kv(l_i, l_d, l_i2, l_d2, total, nc, d_coff);
+                be_.seg_sort_kv(l_d2, l_i2, l_d, l_i, total, nc, d_coff);
+                be_.knn_select(d_items + a, nc, d_coff, d_cnt + a, l_d, l_i, rr, e, kk, n_,
+                               self_rule ? 1 : 0, qrow, d_idx, d_d2, d_redo, d_nredo);
+                a = b;
+            }
+            unsigned nr = 0;
+            be_.d2h(&nr, d_nredo, sizeof(unsigned));
+            items.resize(nr);
+            if (nr) be_.d2h(items.data(), d_redo, sizeof(int) * nr);
+            std::sort(items.begin(), items.end());
+            all = false;
+        }
+        be_.sync();
+        st.t_knn_ms += ms_since(t0);
+        return 0;
+    }
+
     // ------------------------------------------------------------ cells
     int cells(const Problem& pb, EngineStats& st) {
         auto tstart = now();
@@ -319,7 +426,7 @@ public:
         }
         if (!autotune) {
             // without the tuner every call runs the default work-group size
-            be_.set_clip_waves(kDefaultClipWaves);
+            be_.set_clip_waves(waves4 ? 4 : kDefaultClipWaves);
             tuned_n_ = -1;
         }
         if (autotune && P.n_items > 0 && tuned_n_ != n_) {
@@ -561,7 +668,9 @@ private:
         S_VALS, S_VALS2, S_PTS, S_PF, S_ORIG, S_INSTOF, S_NODES, S_LVLOFF, S_WMAX, S_LIST,
         S_VOL, S_MG, S_CNT, S_NBRTMP, S_AREATMP, S_FLAGS, S_R2, S_CTR, S_PTR, S_NBR, S_AREA,
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
-        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2, S_COUNT
+        S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2,
+        S_QKEYS, S_QKEYS2, S_QVALS, S_QVALS2, S_QPTS, S_QPF, S_QROW, S_QPOS, S_QR2, S_QREDO,
+        S_QLI2, S_QLD, S_QLD2, S_COUNT
     };
     struct Buf {
         void* p = nullptr;

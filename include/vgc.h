@@ -10,6 +10,13 @@
  * voronoi/CVT.cpp:146-152).  With VGC_DEVICE_PTRS the pointers are device
  * memory of the context's GPU and nothing crosses PCIe.
  *
+ * Streams: a context runs its kernels on a stream of its own.  Every call
+ * first orders that stream after the work queued before the call on the
+ * caller's stream (vgc_set_stream; default: the null stream), so device
+ * inputs written by the caller's asynchronous kernels are complete, and
+ * returns only when its own work is done, so device outputs are ready for
+ * any stream.  A context is used by one host thread at a time.
+ *
  * Numbering: seeds are indexed in the caller's order; neighbour ids too.
  * Coordinates are FP64 x,y,z triples (the layout of
  * CentroidalVoronoiTesselation::embedding, voronoi/CVT.h:333).
@@ -41,10 +48,15 @@ extern "C" {
 #define VGC_AUTOTUNE 32u      /* time the first clipping pass with 1 and 4 wavefronts
                                  per work-group on the first call for a problem size
                                  and keep the faster (results do not depend on it) */
-#define VGC_OT_INEXACT 64u    /* transport: inexact Newton — each step's CG stops at
-                                 relative residual min(0.1, 0.1 x the current max
-                                 relative mass error), not below the default 1e-7
-                                 (Eisenstat-Walker forcing) */
+#define VGC_OT_INEXACT 64u    /* transport: inexact Newton — once the previous step
+                                 was a full step (t = 1), the next step's CG stops
+                                 at relative residual min(0.01, 0.01 x the current
+                                 max relative mass error), never below the default
+                                 1e-7; damped steps keep 1e-7 (Eisenstat-Walker
+                                 forcing) */
+#define VGC_WAVES4 128u       /* run the first clipping pass with 4-wavefront
+                                 work-groups instead of the default 1 (identical
+                                 results; for tests and tuning) */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -83,15 +95,38 @@ typedef struct vgc_stats {
 int vgc_create(vgc_ctx** ctx, int device);
 void vgc_destroy(vgc_ctx* ctx);
 const char* vgc_last_error(const vgc_ctx* ctx);
+/* The stream (a hipStream_t) whose earlier work each later call waits for;
+   NULL: the null stream.  Replaces nothing in geogram (its calls are
+   synchronous CPU code). */
+int vgc_set_stream(vgc_ctx* ctx, void* stream);
 
-/* k nearest neighbours of every seed, self excluded, ascending distance
-   (FP32 ranking), caller's indices; nbr and d2 are n*k (d2 may be NULL).
-   k <= 64.  Replaces NearestNeighborSearch::get_nearest_neighbors
-   (points/nn_search.h, points/kd_tree.cpp:156-241) called for every point,
-   as Delaunay_NearestNeighbors::store_neighbors_CB does
-   (delaunay/delaunay_nn.cpp:73-84, 105-143). */
+/* k nearest neighbours of every point (k <= n - 1), exact: FP64 squared
+   distances, ascending, equal distances by ascending index; caller's
+   indices.  The neighbours of point i are those of
+   Delaunay_NearestNeighbors::get_neighbors_internal
+   (delaunay/delaunay_nn.cpp:105-143): the k + 1 nearest points of i (i
+   itself included) without i; if one of them coincides with i and has a
+   smaller index, i gets no neighbours; coincident points of larger index
+   are skipped.  Rows that end early are padded with nbr = -1, d2 = -1.
+   nbr and d2 are n*k (d2 may be NULL).  Replaces
+   NearestNeighborSearch::get_nearest_neighbors(nb, index_t, ...)
+   (points/nn_search.h:180-185, points/kd_tree.cpp:156-241) called for every
+   point, as Delaunay_NearestNeighbors::store_neighbors_CB does
+   (delaunay/delaunay_nn.cpp:73-84).  Ties: geogram's kd-tree puts a point
+   met later before an equally distant one (points/kd_tree.h:176-190), an
+   order that depends on its tree; here the smaller index comes first. */
 int vgc_knn(vgc_ctx* ctx, const double* xyz, uint32_t n, uint32_t k, uint32_t flags,
-            int32_t* nbr, float* d2);
+            int32_t* nbr, double* d2);
+
+/* k nearest of the n points xyz (k <= n) to each of nq query points q
+   (FP64 x,y,z triples; no point excluded, a point at the query itself comes
+   first at distance 0): nbr / d2 are nq*k, ascending (FP64 squared
+   distance, index).  Replaces NearestNeighborSearch::set_points +
+   get_nearest_neighbors(nb_neighbors, query_point, neighbors,
+   neighbors_sq_dist) (points/nn_search.h:96,129-134) called for each query
+   point. */
+int vgc_knn_search(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* q, uint32_t nq,
+                   uint32_t k, uint32_t flags, int32_t* nbr, double* d2);
 
 /* Voronoi (w == NULL) or Laguerre cells (power distance |x-p|^2 - w) of n
    seeds clipped to box = {xmin,ymin,zmin,xmax,ymax,zmax}.

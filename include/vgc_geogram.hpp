@@ -1,7 +1,7 @@
 // C++ mirror of geogram's entry points for the meshless cell path, header-only
 // over the C ABI of vgc.h (link with libvgc.so).  Class and method names follow
 // geogram so a caller switches by changing the type:
-//   NearestNeighborSearch          points/nn_search.h:69-180
+//   NearestNeighborSearch          points/nn_search.h:69-200
 //   RestrictedVoronoiDiagram       voronoi/RVD.h:267-290 (volumetric, box domain)
 //   CentroidalVoronoiTesselation   voronoi/CVT.h:139-164
 //   OptimalTransportMap3d          examples/exploragram/compute_OTM/main.cpp:283-297
@@ -39,18 +39,48 @@ private:
     vgc_ctx* ctx_ = nullptr;
 };
 
-// All-points k nearest neighbours (k in 16, 24, 32, 48, 64), self excluded.
+// NearestNeighborSearch (points/nn_search.h:69-200): exact FP64 squared
+// distances, ascending; equal distances by ascending index.
 class NearestNeighborSearch {
 public:
     explicit NearestNeighborSearch(Context& ctx) : ctx_(ctx) {}
+    // nn_search.h:96
     void set_points(index_t nb_points, const double* points) {
         n_ = nb_points;
         pts_ = points;
     }
     index_t nb_points() const { return n_; }
-    // neighbours[i*k + m], sq_dist[i*k + m] for every point i
-    void get_nearest_neighbors_all(index_t k, int* neighbours, float* sq_dist) const {
-        ctx_.check(vgc_knn(ctx_.get(), pts_, n_, k, 0, neighbours, sq_dist));
+    // nn_search.h:129-134: nb_neighbors nearest points of query_point
+    void get_nearest_neighbors(index_t nb_neighbors, const double* query_point,
+                               index_t* neighbors, double* neighbors_sq_dist) const {
+        get_nearest_neighbors(nb_neighbors, 1, query_point, neighbors, neighbors_sq_dist);
+    }
+    // nn_search.h:180-185: by the index of a stored point (itself included,
+    // at distance 0)
+    void get_nearest_neighbors(index_t nb_neighbors, index_t query_point, index_t* neighbors,
+                               double* neighbors_sq_dist) const {
+        get_nearest_neighbors(nb_neighbors, 1, pts_ + 3 * (size_t)query_point, neighbors,
+                              neighbors_sq_dist);
+    }
+    // nn_search.h:192-198
+    index_t get_nearest_neighbor(const double* query_point) const {
+        index_t result;
+        double sq_dist;
+        get_nearest_neighbors(1, query_point, &result, &sq_dist);
+        return result;
+    }
+    // the same for nq query points in one call: neighbors / sq_dist are
+    // nq * nb_neighbors (the form a GPU caller should use)
+    void get_nearest_neighbors(index_t nb_neighbors, index_t nq, const double* query_points,
+                               index_t* neighbors, double* sq_dist) const {
+        ctx_.check(vgc_knn_search(ctx_.get(), pts_, n_, query_points, nq, nb_neighbors, 0,
+                                  reinterpret_cast<int32_t*>(neighbors), sq_dist));
+    }
+    // every stored point's nb_neighbors nearest others, with the coincident-
+    // point rule of Delaunay_NearestNeighbors::get_neighbors_internal
+    // (delaunay/delaunay_nn.cpp:105-143); -1 pads rows that end early
+    void get_neighbors_of_all(index_t nb_neighbors, int32_t* neighbors, double* sq_dist) const {
+        ctx_.check(vgc_knn(ctx_.get(), pts_, n_, nb_neighbors, 0, neighbors, sq_dist));
     }
 
 private:

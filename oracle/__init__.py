@@ -103,6 +103,19 @@ def knn(xyz, k):
     return idx, d2
 
 
+def knn_query(xyz, q, k):
+    """Exact k nearest points of xyz to each query point (none excluded),
+    sorted by (squared distance, index)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    idx = np.zeros((q.shape[0], k), dtype=np.int32)
+    d2 = np.zeros((q.shape[0], k), dtype=np.float64)
+    err = lib().orc_knn_query(_dp(xyz), xyz.shape[0], _dp(q), q.shape[0], k, _ip(idx), _dp(d2))
+    if err:
+        raise RuntimeError(f"orc_knn_query failed ({err})")
+    return idx, d2
+
+
 def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None,
           centred=False, e2=False):
     """Per-cell volume, mg (= integral of x), neighbour lists and facet areas.

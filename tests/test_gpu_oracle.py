@@ -1,0 +1,293 @@
+"""GPU results against the oracle (not against the emulation) for every part of
+the path: Laguerre cells, the exact re-clip of lattice ties, the CVT energy and
+gradient, the exact kNN, batched cells / Lloyd / transport per instance, the
+MLP of the learned warm start, and the configurations at full size.
+
+Tolerances: volumes, centroids and facet areas 1e-10 relative against the
+oracle in the seed-centred frame (same algorithm, planes formed relative to
+the seed); neighbour sets and kNN lists exactly; transport masses re-measured
+by the oracle within the solver's tolerance (plus 1e-9 of the oracle's own
+relative rounding)."""
+import numpy as np
+import pytest
+
+import geogram_amd as ga
+import oracle
+from tests.test_gpu_cells import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _lattice(m):
+    g = (np.arange(m) + 0.5) / m
+    return np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+
+
+# ------------------------------------------------------------- Laguerre cells
+
+@pytest.mark.parametrize("k0", [0, 32])
+def test_laguerre_random_weights_vs_oracle(gpu_ctx, k0):
+    rng = np.random.default_rng(3)
+    n = 4000
+    x = rng.random((n, 3))
+    w = rng.random(n) * 4e-4
+    r = gpu_ctx.cells(x, weights=w, k0=k0)
+    assert r.stats["nb_failed"] == 0 and r.stats["nb_uncertain"] == 0
+    o = oracle.cells(x, w=w, k0=20)
+    compare(r, o, n, x=x, w=w)
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+def test_laguerre_transport_iterate_vs_oracle(gpu_ctx):
+    # weights of a damped Newton iterate of transport onto a 16-Gaussian
+    # mixture with log-normal masses: cells far from their seeds
+    from geogram_amd import warmstart
+    x, nu = warmstart.make_instance(77, 4000)
+    w, _ = gpu_ctx.ot_solve(x, nu, tol=1e-9, max_newton=4)
+    r = gpu_ctx.cells(x, weights=w)
+    assert r.stats["nb_failed"] == 0
+    o = oracle.cells(x, w=w, k0=20, stride=256)
+    compare(r, o, x.shape[0], x=x, w=w)
+
+
+def test_laguerre_512_corner_tier_vs_oracle(gpu_ctx):
+    # one heavy seed: a cell of 185 facets (beyond 256 corners)
+    x = ga.sample_points(0, 41, 20000)
+    x[0] = [0.5, 0.5, 0.5]
+    w = np.zeros(len(x))
+    w[0] = 0.012
+    r = gpu_ctx.cells(x, weights=w)
+    assert r.stats["nb_failed"] == 0 and len(r.neighbors(0)) == 185
+    assert r.stats["nb_pass_c"] > 0
+    sub = np.r_[0, r.neighbors(0)[:40], np.arange(1, 200)].astype(np.int32)
+    oc = oracle.cells(x, w=w, k0=20, subset=sub, centred=True, stride=512)
+    assert (np.abs(r.vol[sub] - oc["vol"]) / oc["vol"]).max() < 1e-10
+    for t, i in enumerate(sub):
+        assert set(r.neighbors(i).tolist()) == set(oc["nbr"][t, : oc["cnt"][t]].tolist())
+
+
+def test_laguerre_empty_cells_vs_oracle(gpu_ctx):
+    rng = np.random.default_rng(4)
+    n = 800
+    x = rng.random((n, 3))
+    w = np.zeros(n)
+    w[:10] = 0.05
+    r = gpu_ctx.cells(x, weights=w, k0=32)
+    o = oracle.cells(x, w=w, k0=20, stride=512, centred=True)
+    assert r.stats["nb_empty"] == o["stats"]["nb_empty"] > 0
+    nz = o["vol"] > 0
+    assert np.array_equal(r.vol == 0, ~nz)
+    assert (np.abs(r.vol[nz] - o["vol"][nz]) / o["vol"][nz]).max() < 1e-10
+
+
+# ------------------------------------------------- exact pass on lattice ties
+
+@pytest.mark.parametrize("k0", [0, 32])
+def test_exact_pass_dyadic_lattice_vs_oracle(gpu_ctx, k0):
+    # every corner of a lattice is shared by 8 cells: the certified FP64 signs
+    # fail and the cells are re-clipped exactly.  Dyadic coordinates make the
+    # seed-relative and absolute planes both exact, so the tie resolution —
+    # neighbour sets with their zero-area facets — matches the oracle's.
+    x = _lattice(8)
+    r = gpu_ctx.cells(x, k0=k0)
+    assert r.stats["nb_pass_e"] > 0
+    assert r.stats["nb_uncertain"] == 0 and r.stats["nb_failed"] == 0
+    o = oracle.cells(x, k0=20)
+    assert np.allclose(r.vol, 1.0 / 512, rtol=1e-14)
+    assert oracle.neighbour_sets(o) == [frozenset(r.neighbors(i).tolist())
+                                        for i in range(x.shape[0])]
+    for i in range(x.shape[0]):
+        da = dict(zip(r.neighbors(i).tolist(), r.facet_areas(i).tolist()))
+        for j, a in zip(o["nbr"][i, : o["cnt"][i]].tolist(), o["farea"][i, : o["cnt"][i]].tolist()):
+            assert abs(da[j] - a) <= 1e-14
+
+
+# ----------------------------------------------------- CVT energy / gradient
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_cvt_energy_gradient_vs_oracle(gpu_ctx, kind):
+    x = ga.sample_points(kind, 14 + kind, 20000)
+    f, g = gpu_ctx.cvt_func_grad(x)
+    o = oracle.cells(x, k0=20, centred=True, e2=True)
+    fo = o["e2"].sum()
+    assert abs(f - fo) <= 1e-12 * fo
+    go = 2.0 * (o["vol"][:, None] * x - o["mg"])
+    assert np.abs(g - go).max() <= 1e-10 * np.abs(go).max()
+
+
+# ------------------------------------------------------------------ exact kNN
+
+@pytest.mark.parametrize("kind,n,k", [(0, 20000, 20), (1, 20000, 64), (2, 8000, 100)])
+def test_knn_vs_oracle(gpu_ctx, kind, n, k):
+    x = ga.sample_points(kind, 8 + kind, n)
+    idx, d2 = gpu_ctx.knn(x, k)
+    oi, od = oracle.knn(x, k)
+    assert np.array_equal(idx, oi)
+    assert np.array_equal(d2, od)
+
+
+def test_knn_search_vs_oracle(gpu_ctx):
+    x = ga.sample_points(1, 21, 30000)
+    q = np.random.default_rng(2).random((5000, 3)) * 1.4 - 0.2  # some outside the box
+    for k in (1, 20, 70):
+        idx, d2 = gpu_ctx.knn_search(x, q, k)
+        oi, od = oracle.knn_query(x, q, k)
+        assert np.array_equal(idx, oi) and np.array_equal(d2, od)
+
+
+def test_knn_ties_and_coincident_points(gpu_ctx):
+    # lattice: many exactly equal distances, ordered by index
+    x = _lattice(6)
+    idx, d2 = gpu_ctx.knn(x, 26)
+    oi, od = oracle.knn(x, 26)
+    assert np.array_equal(idx, oi) and np.array_equal(d2, od)
+    # coincident points (delaunay_nn.cpp:105-143): the smallest index keeps
+    # its neighbours (the copies skipped), the later copies get none
+    x = ga.sample_points(0, 9, 300)
+    x[200] = x[17]
+    x[250] = x[17]
+    idx, d2 = gpu_ctx.knn(x, 10)
+    assert np.all(idx[200] == -1) and np.all(idx[250] == -1)
+    assert 200 not in idx[17] and 250 not in idx[17]
+    assert np.all(idx[17][:8] >= 0) and np.all(d2[17][:8] > 0)
+
+
+# ------------------------------------------------------ batched instances
+
+def test_batched_cells_and_lloyd_vs_oracle_per_instance(gpu_ctx):
+    sizes = [500, 3000, 64, 1500, 2200]
+    xs = [ga.sample_points(i % 3, 60 + i, s) for i, s in enumerate(sizes)]
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    boxes[2] = [-1, -1, -1, 2, 2, 2]
+    x = np.vstack(xs)
+    off = np.r_[0, np.cumsum(sizes)]
+    r = gpu_ctx.cells(x, k0=0, inst_n=sizes, boxes=boxes)
+    for i, s in enumerate(sizes):
+        oc = oracle.cells(xs[i], box=boxes[i], k0=20, centred=True)
+        sl = slice(off[i], off[i + 1])
+        assert (np.abs(r.vol[sl] - oc["vol"]) / oc["vol"]).max() < 1e-10
+        for c in range(s):
+            assert set((r.neighbors(off[i] + c) - off[i]).tolist()) == \
+                set(oc["nbr"][c, : oc["cnt"][c]].tolist())
+    gb, _ = gpu_ctx.lloyd(x, 3, inst_n=sizes, boxes=boxes)
+    for i in range(len(sizes)):
+        xo, _ = oracle.lloyd(xs[i], 3, box=boxes[i])
+        assert np.abs(gb[off[i]:off[i + 1]] - xo).max() < 1e-12
+
+
+def test_batched_transport_vs_oracle_masses(gpu_ctx):
+    from geogram_amd import warmstart
+    sizes = [1200, 600, 2000, 300, 900]
+    xs, nus = [], []
+    for i, s in enumerate(sizes):
+        xi, nui = warmstart.make_instance(70 + i, s)
+        xs.append(xi)
+        nus.append(nui)
+    x, nu = np.vstack(xs), np.concatenate(nus)
+    boxes = np.array([[0, 0, 0, 1, 1, 1]] * len(sizes), dtype=float)
+    w, st = gpu_ctx.ot_solve_batch(x, nu, sizes, boxes, tol=1e-6)
+    off = np.r_[0, np.cumsum(sizes)]
+    for i in range(len(sizes)):
+        assert st[i]["converged"] == 1
+        sl = slice(off[i], off[i + 1])
+        o = oracle.cells(xs[i], w=w[sl], centred=True, stride=256)
+        assert np.max(np.abs(o["vol"] - nus[i]) / nus[i]) <= 1e-6 + 1e-9
+
+
+# ----------------------------------------------------------- MLP (warm start)
+
+def test_mlp_forward_and_gradient_vs_oracle(gpu_ctx):
+    # oracle evaluated in FP64 on the FP32-rounded inputs: the gap is the
+    # kernels' FP32 arithmetic
+    from oracle import mlp as omlp
+    from tests.test_warmstart import _rand_problem
+    p, X, T = _rand_problem(5000, seed=4)
+    p32, X32, T32 = p.astype(np.float32), X.astype(np.float32), T.astype(np.float32)
+    pr, Xr, Tr = p32.astype(np.float64), X32.astype(np.float64), T32.astype(np.float64)
+    y = gpu_ctx.mlp_forward(p32, X32)
+    yr = omlp.forward(pr, Xr)
+    assert np.max(np.abs(y - yr)) <= 2e-6 * (1 + np.abs(yr).max())
+    g, loss = gpu_ctx.mlp_grad(p32, X32, T32)
+    lr_, gr = omlp.loss_and_grad(pr, Xr, Tr)
+    assert abs(loss - lr_) <= 1e-5 * lr_
+    assert np.max(np.abs(g - gr)) <= 2e-5 * np.abs(gr).max()
+    for lo, hi in [(0, omlp.OFF_B1), (omlp.OFF_W2, omlp.OFF_B2), (omlp.OFF_W3, omlp.OFF_B3)]:
+        assert np.linalg.norm(g[lo:hi] - gr[lo:hi]) <= 1e-5 * np.linalg.norm(gr[lo:hi])
+
+
+def test_ot_features_vs_oracle(gpu_ctx):
+    from oracle import mlp as omlp
+    from geogram_amd import warmstart
+    x, nu = warmstart.make_instance(5, 6000)
+    f = gpu_ctx.ot_features(x, nu)
+    _, d2 = oracle.knn(x, 16)
+    fr = omlp.features(x, nu, (0, 0, 0, 1, 1, 1), d2)
+    assert np.max(np.abs(f - fr)) <= 1e-5
+
+
+# -------------------------------------------- first pass with 4-wave groups
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_four_wave_first_pass_bitwise_and_oracle(gpu_ctx, weighted):
+    x = ga.sample_points(1, 31, 20000)
+    w = np.random.default_rng(5).random(len(x)) * 2e-5 if weighted else None
+    r1 = gpu_ctx.cells(x, weights=w)
+    r4 = gpu_ctx.cells(x, weights=w, waves4=True)
+    for a in ("vol", "mg", "nbr_ptr", "nbr", "farea"):
+        assert np.array_equal(getattr(r1, a), getattr(r4, a)), a
+    o = oracle.cells(x, w=w, k0=20, centred=True)
+    assert (np.abs(r4.vol - o["vol"]) / o["vol"]).max() < 1e-10
+
+
+# --------------------------------------------------- configurations, full size
+
+def test_c1_cvt_lloyd_100k_ten_iterations_vs_oracle(gpu_ctx):
+    """100k uniform seeds (RNG seed 1), 10 Lloyd iterations with the security
+    radius: the oracle runs the same iterations on the CPU."""
+    from geogram_amd import configs
+    x = configs.cvt_points()
+    xg, st = gpu_ctx.lloyd(x, 10)
+    xo, _ = oracle.lloyd(x, 10)
+    assert np.abs(xg - xo).max() < 1e-12
+    r = gpu_ctx.cells(xg)
+    assert abs(r.vol.sum() - 1.0) < 1e-12
+
+
+def test_c4_one_million_gaussian_transport_vs_oracle_sample(gpu_ctx):
+    """1M seeds from the 16-Gaussian mixture with log-normal masses solved to
+    1e-4 max relative mass error; the oracle re-measures the masses of 2000
+    sampled seeds from the solution's weights."""
+    import torch
+    from geogram_amd import configs
+    x, nu = configs.gaussian_transport_instance(1_000_000)
+    xt = torch.from_numpy(x).cuda()
+    nut = torch.from_numpy(nu).cuda()
+    w, st = gpu_ctx.ot_solve(xt, nut, tol=1e-4, inexact=True)
+    print(st)
+    assert st["converged"] == 1 and st["final_err"] <= 1e-4
+    wh = w.cpu().numpy()
+    sample = np.random.default_rng(0).choice(len(x), 2000, replace=False).astype(np.int32)
+    o = oracle.cells(x, w=wh, k0=20, subset=sample, centred=True, stride=256)
+    err = np.abs(o["vol"] - nu[sample]) / nu[sample]
+    assert err.max() <= 1e-4 * (1 + 1e-6)
+
+
+def test_c5_batched_instances_lloyd_and_transport(gpu_ctx):
+    """256 independent instances, seed counts log-uniform in [2k, 200k]
+    (about 11M seeds): 20 Lloyd steps and transport to 1e-4 in one batch;
+    every instance converges and the oracle re-measures the masses of two
+    sampled instances."""
+    from geogram_amd import configs
+    x, nu, sizes, boxes = configs.batched_instances(256)
+    xl, st = gpu_ctx.lloyd(x, 20, inst_n=sizes, boxes=boxes)
+    assert st["nb_failed"] == 0 and np.isfinite(xl).all()
+    w, sts = gpu_ctx.ot_solve_batch(x, nu, sizes, boxes, tol=1e-4, inexact=True)
+    assert all(s["converged"] == 1 for s in sts)
+    off = np.r_[0, np.cumsum(sizes)]
+    for i in (int(np.argmin(sizes)), 17):
+        sl = slice(off[i], off[i + 1])
+        sample = np.random.default_rng(i).choice(sizes[i], min(1500, sizes[i]),
+                                                 replace=False).astype(np.int32)
+        o = oracle.cells(x[sl], w=w[sl], k0=20, subset=sample, centred=True, stride=256)
+        err = np.abs(o["vol"] - nu[sl][sample]) / nu[sl][sample]
+        assert err.max() <= 1e-4 * (1 + 1e-6)
