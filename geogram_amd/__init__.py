@@ -178,6 +178,60 @@ class Context:
             return CellResult(vol, mg, ptr, nbr[:nnz], fa[:nnz], st.as_dict(), rc)
         raise VgcError(rc, self.lib.last_error(self.ctx))
 
+    # ------------------------------------------------------- meshes / RDT
+    def cell_mesh(self, points, weights=None, box=UNIT_BOX, shrink=0.0, tets=False):
+        """The cells as polyhedra (ConvexCell::append_to_mesh; vgc_cell_mesh).
+        Returns a dict: cell_vptr, cell_fptr (n + 1 each), vertices (V, 3),
+        facet_ptr (F + 1), facet_vertices, facet_nbr (-1: box), and with
+        tets=True the primal tetrahedra (T, 4).  Host arrays."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        n = pts.shape[0]
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        need = _lib.MeshSizes()
+        st = Stats()
+        f = self.lib.fn("cell_mesh")
+        rc = f(self.ctx, _ptr(pts), _ptr(w), n, boxa.ctypes.data_as(ctypes.c_void_p), float(shrink),
+               0, None, None, None, None, None, None, None, None, ctypes.byref(need),
+               ctypes.byref(st))
+        self._check(rc, allow_partial=False)
+        cap = _lib.MeshSizes(need.nb_vertices, need.nb_facets, need.nb_facet_vertices,
+                             need.nb_tets)
+        out = {"cell_vptr": np.empty(n + 1, np.uint64), "cell_fptr": np.empty(n + 1, np.uint64),
+               "vertices": np.empty((need.nb_vertices, 3)),
+               "facet_ptr": np.empty(need.nb_facets + 1, np.uint64),
+               "facet_vertices": np.empty(need.nb_facet_vertices, np.int32),
+               "facet_nbr": np.empty(need.nb_facets, np.int32)}
+        if tets:
+            out["tets"] = np.empty((need.nb_tets, 4), np.int32)
+        rc = f(self.ctx, _ptr(pts), _ptr(w), n, boxa.ctypes.data_as(ctypes.c_void_p), float(shrink),
+               0, _ptr(out["cell_vptr"]), _ptr(out["cell_fptr"]), _ptr(out["vertices"]),
+               _ptr(out["facet_ptr"]), _ptr(out["facet_vertices"]), _ptr(out["facet_nbr"]),
+               _ptr(out["tets"]) if tets else None, ctypes.byref(cap), ctypes.byref(need),
+               ctypes.byref(st))
+        self._check(rc)
+        out["stats"] = st.as_dict()
+        return out
+
+    def rdt(self, points, weights=None, box=UNIT_BOX):
+        """Primal tetrahedra of the cells (compute_RDT, volumetric; vgc_rdt):
+        a (T, 4) int32 array of seed indices."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        nt = _lib._u64(0)
+        st = Stats()
+        f = self.lib.fn("rdt")
+        rc = f(self.ctx, _ptr(pts), _ptr(w), pts.shape[0], boxa.ctypes.data_as(ctypes.c_void_p), 0,
+               None, 0, ctypes.byref(nt), ctypes.byref(st))
+        if rc != _lib.VGC_ERR_CAPACITY:
+            self._check(rc, allow_partial=False)
+        t = np.empty((nt.value, 4), np.int32)
+        rc = f(self.ctx, _ptr(pts), _ptr(w), pts.shape[0], boxa.ctypes.data_as(ctypes.c_void_p), 0,
+               _ptr(t), nt.value, ctypes.byref(nt), ctypes.byref(st))
+        self._check(rc)
+        return t
+
     # ------------------------------------------------------------------ kNN
     def knn(self, points, k):
         """k nearest other points of every point, exact (FP64 squared
@@ -410,6 +464,25 @@ class Context:
         self._check(rc)
         return f.value, g
 
+    def cvt_newton(self, points, iters, m=7, box=UNIT_BOX):
+        """Quasi-Newton CVT (Newton_iterations, CVT.cpp:272-302): L-BFGS with m
+        pairs and a More-Thuente line search on the CVT energy; returns the
+        new points (same kind of array) and the optimiser's statistics."""
+        dev = _is_torch_cuda(points)
+        self._stream(dev)
+        if dev:
+            import torch
+            x = points.detach().clone().to(torch.float64).contiguous().reshape(-1, 3)
+        else:
+            x = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
+        st = _lib.OptStats()
+        boxa = np.ascontiguousarray(box, dtype=np.float64)
+        rc = self.lib.fn("cvt_newton")(self.ctx, _ptr(x), x.shape[0],
+                                       boxa.ctypes.data_as(ctypes.c_void_p), int(iters), int(m),
+                                       VGC_DEVICE_PTRS if dev else 0, ctypes.byref(st))
+        self._check(rc)
+        return x, st.as_dict()
+
     def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None, autotune=False):
         """iters Lloyd iterations; returns the new points (same kind of array).
         inst_n / boxes: several independent instances at once (consecutive
@@ -566,6 +639,18 @@ class RestrictedVoronoiDiagram:
         m += r.vol
         return r
 
+    def compute_RDT(self):
+        """RVD.h:504-510 (volumetric): the primal tetrahedra, (T, 4) seed
+        indices, oriented by orient_3d."""
+        ctx = self.ctx or default_context()
+        return ctx.rdt(self.points, None, self.box)
+
+    def compute_RVD_cells(self, shrink: float = 0.0):
+        """The cells as polyhedra (the cube mode of compute_RVD, ConvexCell::
+        append_to_mesh); see Context.cell_mesh."""
+        ctx = self.ctx or default_context()
+        return ctx.cell_mesh(self.points, None, self.box, shrink)
+
     def compute_CVT_func_grad(self, g):
         """RestrictedVoronoiDiagram::compute_CVT_func_grad_in_volume
         (RVD.h:318): returns the energy and adds its gradient to g (n x 3)."""
@@ -598,6 +683,11 @@ class CentroidalVoronoiTesselation:
     def Lloyd_iterations(self, nb_iter: int):
         ctx = self.ctx or default_context()
         self.points_, self.last_stats = ctx.lloyd(self.points_, nb_iter, self.box, self.k0)
+
+    def Newton_iterations(self, nb_iter: int, m: int = 7):
+        """voronoi/CVT.h:164: quasi-Newton (L-BFGS, m pairs) on the CVT energy."""
+        ctx = self.ctx or default_context()
+        self.points_, self.last_stats = ctx.cvt_newton(self.points_, nb_iter, m, self.box)
 
 
 class OptimalTransportMap3d:

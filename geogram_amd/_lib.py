@@ -59,6 +59,20 @@ class OtStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MeshSizes(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("nb_vertices", "nb_facets", "nb_facet_vertices",
+                                                 "nb_tets")]
+
+
+class OptStats(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_uint32) for k in ("iterations", "evaluations")]
+                + [("line_search_info", ctypes.c_int32), ("failed_cells", ctypes.c_uint32)]
+                + [(k, ctypes.c_double) for k in ("f0", "f", "gnorm0", "gnorm", "t_total_ms")])
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VgcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"vgc error {code}: {msg}")
@@ -94,6 +108,17 @@ def _declare(L, prefix):
     f = L[prefix + "cvt_func_grad"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _vp, _vp, ctypes.POINTER(Stats)]
+    f = L[prefix + "cell_mesh"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, ctypes.c_double, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
+                  _vp, ctypes.POINTER(MeshSizes), ctypes.POINTER(MeshSizes), ctypes.POINTER(Stats)]
+    f = L[prefix + "rdt"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64),
+                  ctypes.POINTER(Stats)]
+    f = L[prefix + "cvt_newton"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(OptStats)]
     f = L[prefix + "lloyd_batch"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(Stats)]
@@ -135,7 +160,7 @@ class _Lib:
         self.prefix = prefix
         self._f = {}
         names = ["last_error", "knn", "knn_search", "cells", "cells_batch", "lloyd", "lloyd_batch",
-                 "cvt_func_grad", "ot_solve", "ot_solve_levels",
+                 "cvt_func_grad", "cvt_newton", "cell_mesh", "rdt", "ot_solve", "ot_solve_levels",
                  "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:

@@ -3,6 +3,7 @@
 // used by the tests instantiates it with the emulation backend.
 #pragma once
 
+#include <chrono>
 #include <exception>
 #include <random>
 #include <stdexcept>
@@ -10,6 +11,7 @@
 #include <vector>
 
 #include "../../include/vgc.h"
+#include "lbfgs.h"
 #include "mlp.h"
 #include "ot_solver.h"
 #include "pipeline.h"
@@ -367,6 +369,244 @@ int api_knn_search(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* q, 
         return VGC_ERR_ARG;
     }
     return knn_common(ctx, "vgc_knn_search", xyz, n, q, nq, k, flags, nbr, d2);
+}
+
+// Cell meshes (ConvexCell::append_to_mesh / save) and primal tetrahedra
+// (RestrictedVoronoiDiagram::compute_RDT, volumetric) of the cells of xyz
+// (weights w or NULL) in the box.
+template <class B>
+int api_cell_mesh(Ctx<B>* ctx, const char* name, const double* xyz, const double* w, uint32_t n,
+                  const double* box, double shrink, uint32_t flags, uint64_t* cell_vptr,
+                  uint64_t* cell_fptr, double* vertices, uint64_t* facet_ptr,
+                  int32_t* facet_vertices, int32_t* facet_nbr, int32_t* tets,
+                  const vgc_mesh_sizes* cap, vgc_mesh_sizes* need, vgc_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !box || n == 0 || !(shrink >= 0.0 && shrink <= 1.0)) {
+            ctx->err = std::string(name) + ": bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+            ctx->err = std::string(name) + ": degenerate box";
+            return VGC_ERR_ARG;
+        }
+        if (n > (1u << 28)) {
+            ctx->err = std::string(name) + ": too many seeds";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        pb.box.assign(box, box + 6);
+        pb.k0 = 0;
+        if (dev) {
+            pb.d_xyz = xyz;
+            pb.d_w = w;
+        } else {
+            double* dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+            pb.d_xyz = dx;
+            if (w) {
+                double* dw = (double*)ctx->staging(1, sizeof(double) * (size_t)n);
+                be.h2d(dw, w, sizeof(double) * (size_t)n);
+                pb.d_w = dw;
+            }
+        }
+        ctx->eng.autotune = false;
+        ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        const bool want_mesh = vertices && facet_ptr && facet_vertices && facet_nbr;
+        long long capv[4] = {0, 0, 0, 0};
+        if (cap) {
+            capv[0] = want_mesh ? (long long)cap->nb_vertices : 0;
+            capv[1] = want_mesh ? (long long)cap->nb_facets : 0;
+            capv[2] = want_mesh ? (long long)cap->nb_facet_vertices : 0;
+            capv[3] = tets ? (long long)cap->nb_tets : 0;
+        }
+        // the count pass decides; staging for host outputs is sized after it
+        MeshOut mo;
+        long long* d_coff = (long long*)ctx->staging(20, sizeof(long long) * 4 * ((size_t)n + 1));
+        mo.cell_off = d_coff;
+        long long tot[4] = {0, 0, 0, 0};
+        EngineStats es;
+        // first the counts only (capacities 0 unless device outputs are
+        // given and can hold any size): the totals size the host staging
+        long long zero[4] = {-1, -1, -1, -1};
+        int rc = ctx->eng.mesh(pb, shrink, zero, mo, tot, es);
+        if (rc < 0) {
+            ctx->err = std::string(name) + ": " + ctx->eng.error;
+            return VGC_ERR_INTERNAL;
+        }
+        if (need) {
+            need->nb_vertices = (uint64_t)tot[0];
+            need->nb_facets = (uint64_t)tot[1];
+            need->nb_facet_vertices = (uint64_t)tot[2];
+            need->nb_tets = (uint64_t)tot[3];
+        }
+        auto out = [&](void* dst, const void* src, size_t bytes) {
+            if (!dst || !bytes) return;
+            if (dev)
+                be.d2d(dst, src, bytes);
+            else
+                be.d2h(dst, src, bytes);
+        };
+        // per-cell offsets (uint64 CSR: the offsets are non-negative)
+        out(cell_vptr, d_coff, sizeof(long long) * ((size_t)n + 1));
+        out(cell_fptr, d_coff + (n + 1), sizeof(long long) * ((size_t)n + 1));
+        bool fits = true;
+        for (int k = 0; k < 4; ++k) {
+            const bool wanted = k < 3 ? want_mesh : tets != nullptr;
+            if (wanted && tot[k] > capv[k]) fits = false;
+        }
+        if (!want_mesh && !tets) {
+            be.sync();
+            return VGC_OK;
+        }
+        if (!fits) {
+            be.sync();
+            ctx->err = std::string(name) + ": output capacity too small (sizes in `need`)";
+            return VGC_ERR_CAPACITY;
+        }
+        if (dev) {
+            mo.v = want_mesh ? vertices : nullptr;
+            mo.fptr = want_mesh ? (long long*)facet_ptr : nullptr;
+            mo.fv = want_mesh ? facet_vertices : nullptr;
+            mo.fnbr = want_mesh ? facet_nbr : nullptr;
+            mo.tet = tets;
+        } else {
+            auto st = [&](int id, size_t b) { return ctx->staging(id, b ? b : 1); };
+            if (want_mesh) {
+                mo.v = (double*)st(21, sizeof(double) * 3 * (size_t)tot[0]);
+                mo.fptr = (long long*)st(22, sizeof(long long) * ((size_t)tot[1] + 1));
+                mo.fv = (int*)st(23, sizeof(int) * (size_t)tot[2]);
+                mo.fnbr = (int*)st(24, sizeof(int) * (size_t)tot[1]);
+            }
+            if (tets) mo.tet = (int*)st(25, sizeof(int) * 4 * (size_t)tot[3]);
+        }
+        mo.cell_off = nullptr;
+        long long tot2[4];
+        long long big[4] = {tot[0], tot[1], tot[2], tot[3]};
+        rc = ctx->eng.mesh(pb, shrink, big, mo, tot2, es);
+        if (rc != 0) {
+            ctx->err = std::string(name) + ": " +
+                       (rc < 0 ? ctx->eng.error : std::string("sizes changed between passes"));
+            return VGC_ERR_INTERNAL;
+        }
+        if (!dev) {
+            if (want_mesh) {
+                be.d2h(vertices, mo.v, sizeof(double) * 3 * (size_t)tot[0]);
+                be.d2h(facet_ptr, mo.fptr, sizeof(long long) * ((size_t)tot[1] + 1));
+                be.d2h(facet_vertices, mo.fv, sizeof(int) * (size_t)tot[2]);
+                be.d2h(facet_nbr, mo.fnbr, sizeof(int) * (size_t)tot[1]);
+            }
+            if (tets) be.d2h(tets, mo.tet, sizeof(int) * 4 * (size_t)tot[3]);
+        }
+        be.sync();
+        fill_stats(es, stats);
+        if (es.nb_failed || es.nb_uncertain) {
+            ctx->err = std::string(name) + ": flagged cells";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
+int api_rdt(Ctx<B>* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+            uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats) {
+    vgc_mesh_sizes c{0, 0, 0, cap}, need{0, 0, 0, 0};
+    const int rc = api_cell_mesh(ctx, "vgc_rdt", xyz, w, n, box, 0.0, flags, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, tets, &c, &need, stats);
+    if (nb_tets) *nb_tets = need.nb_tets;
+    if (!tets && rc == VGC_OK) return need.nb_tets ? VGC_ERR_CAPACITY : VGC_OK;
+    return rc;
+}
+
+// Quasi-Newton CVT (CentroidalVoronoiTesselation::Newton_iterations,
+// voronoi/CVT.cpp:272-302): L-BFGS with m pairs on the CVT energy of the
+// Voronoi cells of xyz in the box, security radius on, xyz updated in place.
+template <class B>
+int api_cvt_newton(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+                   uint32_t m, uint32_t flags, vgc_opt_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !box || n == 0 || m == 0 || m > 64) {
+            ctx->err = "vgc_cvt_newton: bad argument (1 <= m <= 64)";
+            return VGC_ERR_ARG;
+        }
+        if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+            ctx->err = "vgc_cvt_newton: degenerate box";
+            return VGC_ERR_ARG;
+        }
+        if (n > (1u << 28)) {
+            ctx->err = "vgc_cvt_newton: too many seeds";
+            return VGC_ERR_ARG;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        double* dx = xyz;
+        if (!dev) {
+            dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            be.h2d(dx, xyz, sizeof(double) * 3 * (size_t)n);
+        }
+        Engine<B>& E = ctx->eng;
+        E.autotune = (flags & VGC_AUTOTUNE) != 0;
+        E.waves4 = (flags & VGC_WAVES4) != 0;
+        double* d_f = (double*)ctx->staging(2, sizeof(double));
+        uint64_t failed = 0;
+        // energy and gradient: the caller's funcgrad of CVT.cpp:324-339
+        auto fg = [&](const double* x, double* g) -> double {
+            Problem pb;
+            pb.n_inst = 1;
+            pb.inst_n.assign(1, (int)n);
+            pb.box.assign(box, box + 6);
+            pb.n = (int)n;
+            pb.k0 = 0;
+            pb.want_neighbors = false;
+            pb.want_e2 = true;
+            pb.d_xyz = x;
+            EngineStats es;
+            if (E.cells(pb, es)) throw std::runtime_error("vgc_cvt_newton: " + E.error);
+            failed += es.nb_failed + es.nb_uncertain;
+            be.red_sum((int)n, E.d_e2, d_f);
+            double f = 0.0;
+            be.d2h(&f, d_f, sizeof(double));
+            be.dzero(g, sizeof(double) * 3 * (size_t)n);
+            be.cvt_grad((int)n, x, E.d_vol, E.d_mg, g);
+            return f;
+        };
+        LbfgsStats ls;
+        lbfgs_minimize(be, dx, 3ll * n, (int)m, (int)iters, fg, ls);
+        if (!dev) be.d2h(xyz, dx, sizeof(double) * 3 * (size_t)n);
+        be.sync();
+        if (stats) {
+            stats->iterations = (uint32_t)ls.iterations;
+            stats->evaluations = (uint32_t)ls.evaluations;
+            stats->line_search_info = ls.line_search_info;
+            stats->failed_cells = (uint32_t)std::min<uint64_t>(failed, 0xFFFFFFFFu);
+            stats->f0 = ls.f0;
+            stats->f = ls.f;
+            stats->gnorm0 = ls.gnorm0;
+            stats->gnorm = ls.gnorm;
+            stats->t_total_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+        }
+        if (failed) {
+            ctx->err = "vgc_cvt_newton: flagged cells";
+            return VGC_PARTIAL;
+        }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
 }
 
 template <class B>

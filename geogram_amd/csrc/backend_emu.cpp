@@ -330,6 +330,15 @@ struct EmuBackend {
                 g[k] += 2.0 * (vol[i] * xyz[k] - mg[k]);
             }
     }
+    double vdot(int n, const double* a, const double* b) {
+        return red_sum(n, [&](int i) { return a[i] * b[i]; });
+    }
+    void vscale(int n, double a, const double* x, double* out) {
+        for (int i = 0; i < n; ++i) out[i] = a * x[i];
+    }
+    void vaxpy(int n, double a, const double* x, double* y) {
+        for (int i = 0; i < n; ++i) y[i] += a * x[i];
+    }
     void ot_dot(int n, const double* a, const double* b, double* out) {
         *out = red_sum(n, [&](int i) { return a[i] * b[i]; });
     }
@@ -840,6 +849,23 @@ int vgc_emu_lloyd_batch(void* ctx, double* xyz, const uint32_t* inst_n, uint32_t
 int vgc_emu_cvt_func_grad(void* ctx, const double* xyz, uint32_t n, const double* box,
                           uint32_t flags, double* f, double* g, vgc_stats* st) {
     return vgc::api_cvt_func_grad((EmuCtx*)ctx, xyz, n, box, flags, f, g, st);
+}
+int vgc_emu_cell_mesh(void* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+                  double shrink, uint32_t flags, uint64_t* cell_vptr, uint64_t* cell_fptr,
+                  double* vertices, uint64_t* facet_ptr, int32_t* facet_vertices,
+                  int32_t* facet_nbr, int32_t* tets, const vgc_mesh_sizes* cap,
+                  vgc_mesh_sizes* need, vgc_stats* stats) {
+    return vgc::api_cell_mesh((EmuCtx*)ctx, "vgc_cell_mesh", xyz, w, n, box, shrink, flags, cell_vptr,
+                              cell_fptr, vertices, facet_ptr, facet_vertices, facet_nbr, tets, cap,
+                              need, stats);
+}
+int vgc_emu_rdt(void* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+            uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats) {
+    return vgc::api_rdt((EmuCtx*)ctx, xyz, w, n, box, flags, tets, cap, nb_tets, stats);
+}
+int vgc_emu_cvt_newton(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+                       uint32_t m, uint32_t flags, vgc_opt_stats* st) {
+    return vgc::api_cvt_newton((EmuCtx*)ctx, xyz, n, box, iters, m, flags, st);
 }
 int vgc_emu_lloyd(void* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
                   uint32_t k0, uint32_t flags, vgc_stats* st) {

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -494,6 +495,16 @@ __global__ void k_ot_pcg_update(int n, double alpha, const double* __restrict__ 
     z[i] = diag[i] > 0.0 ? ri / diag[i] : 0.0;
 }
 
+__global__ void k_vscale(int n, double a, const double* __restrict__ x, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i];
+}
+
+__global__ void k_vaxpy(int n, double a, const double* __restrict__ x, double* y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
 __global__ void k_ot_pcg_dir(int n, double beta, const double* __restrict__ z, double* p) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = z[i] + beta * p[i];
@@ -557,6 +568,23 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, int nw, float* grad
     for (int w = 0; w < nw; ++w) s += part[(long long)w * kMlpParams + i];
     grad[i] = s;
 }
+
+// loss = (sum of the per-wave loss sums, in wave order, in FP64) / n
+__global__ void k_mlp_loss(const float* __restrict__ lpart, int nw, int n, double* loss) {
+    if (threadIdx.x != 0) return;
+    double l = 0.0;
+    for (int w = 0; w < nw; ++w) l += lpart[w];
+    *loss = l / n;
+}
+
+// Scratch of vgc_mlp_grad_noalloc: the per-wave gradient partials and loss
+// sums of up to kNoAllocRows rows, in static device memory (each GPU holds
+// its own copy for the life of the process).  Host code reaches it through
+// the address hipGetSymbolAddress returns for the device (scratch_ptr), never
+// through the array's name.
+constexpr int kNoAllocRows = 32768;
+constexpr int kNoAllocWaves = kNoAllocRows / kMlpRowsGrad;
+static __device__ __align__(256) float scratch[(size_t)kNoAllocWaves * (kMlpParams + 1)];
 
 __global__ void k_mlp_adam(float* p, const float* __restrict__ g, float* m, float* v, float lr,
                            float b1, float b2, float c1, float c2, float eps) {
@@ -1024,6 +1052,7 @@ struct HipBackend {
     explicit HipBackend(int device) : dev_(device) {
         VGC_HIP_CHECK(hipSetDevice(dev_));
         VGC_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        VGC_HIP_CHECK(hipEventCreateWithFlags(&enter_ev_, hipEventDisableTiming));
     }
     ~HipBackend() {
         for (auto& e : ev_)
@@ -1031,6 +1060,7 @@ struct HipBackend {
                 if (x) (void)hipEventDestroy(x);
         if (temp_) (void)hipFree(temp_);
         if (red_) (void)hipFree(red_);
+        if (vdot_) (void)hipFree(vdot_);
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
@@ -1044,7 +1074,6 @@ struct HipBackend {
     // it — device inputs written by the caller's asynchronous kernels.
     void set_caller_stream(hipStream_t s) { caller_ = s; }
     void enter() {
-        if (!enter_ev_) VGC_HIP_CHECK(hipEventCreateWithFlags(&enter_ev_, hipEventDisableTiming));
         VGC_HIP_CHECK(hipEventRecord(enter_ev_, caller_));
         VGC_HIP_CHECK(hipStreamWaitEvent(stream_, enter_ev_, 0));
     }
@@ -1290,6 +1319,23 @@ struct HipBackend {
         k_sum_final<<<1, 256, 0, stream_>>>(red_part(), out);
         check();
     }
+    // vector helpers of the host-driven optimisers: a . b (fixed-order
+    // reduction, read back), out = a x, y += a x
+    double vdot(int n, const double* a, const double* b) {
+        if (!vdot_) VGC_HIP_CHECK(hipMalloc(&vdot_, sizeof(double)));
+        ot_dot(n, a, b, vdot_);
+        double r = 0.0;
+        d2h(&r, vdot_, sizeof(double));
+        return r;
+    }
+    void vscale(int n, double a, const double* x, double* out) {
+        k_vscale<<<blocks_for(n, 256), 256, 0, stream_>>>(n, a, x, out);
+        check();
+    }
+    void vaxpy(int n, double a, const double* x, double* y) {
+        k_vaxpy<<<blocks_for(n, 256), 256, 0, stream_>>>(n, a, x, y);
+        check();
+    }
     void red_sum(int n, const double* a, double* out) {
         k_sum_partial<<<kRedBlocks, 256, 0, stream_>>>(n, a, red_part());
         k_sum_final<<<1, 256, 0, stream_>>>(red_part(), out);
@@ -1481,6 +1527,13 @@ struct HipBackend {
                                                                       eps);
         check();
     }
+    // gradient and loss with caller-provided scratch (no allocation)
+    void mlp_grad_scratch(const float* P, const float* X, const float* T, int n, float* part,
+                          float* lpart, float* grad, double* loss) {
+        mlp_grad(P, X, T, n, 1.0f / float(n), part, lpart, grad);
+        if (loss) k_mlp_loss<<<1, 64, 0, stream_>>>(lpart, mlp_grad_waves(n), n, loss);
+        check();
+    }
     void ot_features(int n, const double* xyz, const double* nu, const float* d2,
                      const double* box, double inv_cell, double h, float* f) {
         k_ot_features<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, nu, d2, box, inv_cell, h, f);
@@ -1568,6 +1621,7 @@ private:
     void* temp_ = nullptr;
     size_t temp_bytes_ = 0;
     double* red_ = nullptr;
+    double* vdot_ = nullptr;
     hipGraphExec_t cg_exec_ = nullptr;
     const void* cg_key_[12] = {};
     hipGraphExec_t cgb_exec_ = nullptr;
@@ -1584,6 +1638,24 @@ typedef Ctx<HipBackend> HipCtx;
 struct vgc_ctx {
     vgc::HipCtx* impl;
 };
+
+namespace {
+// address of `scratch` on each device, looked up once per device at context
+// creation with that device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+void lookup_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || g_scratch[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(vgc::scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch[device] = static_cast<float*>(p);
+}
+}  // namespace
 
 // Every call makes the context's device current and orders the context's
 // stream after the work queued on the caller's stream before the call.
@@ -1609,7 +1681,8 @@ int vgc_create(vgc_ctx** ctx, int device) {
         return VGC_ERR_DEVICE;
     try {
         vgc_ctx* c = new vgc_ctx;
-        c->impl = new vgc::HipCtx(device);
+        c->impl = new vgc::HipCtx(device);  // makes the device current
+        lookup_scratch(device);
         *ctx = c;
         return VGC_OK;
     } catch (const std::exception&) {
@@ -1674,6 +1747,26 @@ int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t
     if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_lloyd(ctx->impl, xyz, n, box, iters, k0, flags, stats);
 }
+int vgc_cell_mesh(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+                  double shrink, uint32_t flags, uint64_t* cell_vptr, uint64_t* cell_fptr,
+                  double* vertices, uint64_t* facet_ptr, int32_t* facet_vertices,
+                  int32_t* facet_nbr, int32_t* tets, const vgc_mesh_sizes* cap,
+                  vgc_mesh_sizes* need, vgc_stats* stats) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_cell_mesh(ctx->impl, "vgc_cell_mesh", xyz, w, n, box, shrink, flags, cell_vptr,
+                              cell_fptr, vertices, facet_ptr, facet_vertices, facet_nbr, tets, cap,
+                              need, stats);
+}
+int vgc_rdt(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+            uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_rdt(ctx->impl, xyz, w, n, box, flags, tets, cap, nb_tets, stats);
+}
+int vgc_cvt_newton(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+                   uint32_t m, uint32_t flags, vgc_opt_stats* stats) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_cvt_newton(ctx->impl, xyz, n, box, iters, m, flags, stats);
+}
 int vgc_lloyd_batch(vgc_ctx* ctx, double* xyz, const uint32_t* inst_n, uint32_t n_inst,
                     const double* boxes, uint32_t iters, uint32_t k0, uint32_t flags,
                     vgc_stats* stats) {
@@ -1718,6 +1811,40 @@ int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const fl
     if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_mlp_grad(ctx->impl, params, feats, target, n, flags, grad, loss);
 }
+int vgc_mlp_grad_noalloc(vgc_ctx* ctx, const float* params, const float* feats,
+                         const float* target, uint32_t n, float* grad, double* loss) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    vgc::HipCtx* c = ctx->impl;
+    if (!params || !feats || !target || !grad || n == 0) {
+        c->err = "vgc_mlp_grad_noalloc: bad argument";
+        return VGC_ERR_ARG;
+    }
+    if (n > (uint32_t)vgc::kNoAllocRows) {
+        c->err = "vgc_mlp_grad_noalloc: " + std::to_string(n) + " rows exceed the static scratch (" +
+                 std::to_string(vgc::kNoAllocRows) + " rows at most)";
+        return VGC_ERR_CAPACITY;
+    }
+    const int dev = c->device;
+    float* base = (dev >= 0 && dev < kMaxDevices) ? g_scratch[dev] : nullptr;
+    if (!base) {
+        c->err = std::string("vgc_mlp_grad_noalloc: no scratch address for this device (") +
+                 hipGetErrorString(dev >= 0 && dev < kMaxDevices ? g_scratch_err[dev]
+                                                                  : hipErrorInvalidDevice) +
+                 ")";
+        return VGC_ERR_DEVICE;
+    }
+    try {
+        float* part = base;
+        float* lpart = base + (size_t)vgc::kNoAllocWaves * vgc::kMlpParams;
+        c->be.mlp_grad_scratch(params, feats, target, (int)n, part, lpart, grad, loss);
+        c->be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
 int vgc_mlp_adam(vgc_ctx* ctx, float* params, const float* grad, float* m, float* v,
                  uint32_t step, float lr, uint32_t flags) {
     if (int rc = vgc_enter(ctx)) return rc;

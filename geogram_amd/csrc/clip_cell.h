@@ -126,6 +126,22 @@ struct ClipParams {
     // [0] planes that cut, [1] planes tested, [2] live corners at each test,
     // [3] corners created, [4] integration tetrahedra, [5] facets
     unsigned long long* counters;
+    // cell meshes and primal tetrahedra (ConvexCell::append_to_mesh,
+    // RestrictedVoronoiDiagram::compute_RDT): 0 off, 1 count, 2 write.
+    // Per cell (caller's index i), four counts / offsets at stride n (counts)
+    // and n + 1 (exclusive offsets): [0] vertices, [1] facets, [2] facet
+    // vertex entries, [3] tetrahedra.
+    int mesh_mode;
+    int mesh_n;
+    int* mesh_cnt;
+    const long long* mesh_off;
+    double mesh_shrink;     // vertices pulled towards the cell's centroid
+    double* mesh_v;         // 3 per vertex (absolute coordinates)
+    long long* mesh_fptr;   // per facet: first entry in mesh_fv
+    int* mesh_fv;           // global vertex indices, facets counter-clockwise
+                            // seen from outside
+    int* mesh_fnbr;         // per facet: neighbour (caller's index), -1 domain
+    int* mesh_tet;          // 4 per tetrahedron (caller's indices)
 };
 
 template <int S>
@@ -1113,6 +1129,236 @@ VGC_HD void stage_cand(const ClipParams& P, CellLds<S>& L, int base, int cnt) {
     wave_sync();
 }
 
+// Successor of corner t around face f (corner t has plane f): the corner
+// across t's edge (previous plane, f); counter-clockwise seen from outside,
+// since each corner's planes a, b, c have det(ea, eb, ec) > 0.
+template <int S>
+VGC_HD LI face_next(const CellLds<S>& L, const LI& t, const LI& f) {
+    const LU plt = lds_ld(L.cpl, t);
+    const LU adt = lds_ld(L.cadj, t);
+    const LU fu = cvt<uint32_t>(f);
+    constexpr int B = CornerBits<S>::value;
+    const LI sh = select(byte_of(plt, 0) == fu, LI(2 * B), select(byte_of(plt, 1) == fu, LI(0), LI(B)));
+    return cvt<int>((adt >> cvt<uint32_t>(sh)) & LU(CornerBits<S>::mask)) & (64 * S - 1);
+}
+
+// Wave-wide exclusive prefix sum of non-negative lane values.
+VGC_HD LI wave_exscan(const LI& v) {
+    const LI lane = lane_id();
+    LI s = v;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const LI o = shfl(s, select(lane >= d, lane - d, LI(0)));
+        s = select(lane >= d, s + o, s);
+    }
+    return s - v;
+}
+
+// Sign of orient_3d(p0, p1, p2, p3) = sign det(p1 - p0, p2 - p0, p3 - p0)
+// (numerics/predicates.cpp:1889-1899): FP64 with a static error bound, else
+// exact as -det4 of the rows (p, 1).
+VGC_SD int orient3d_sign(const pt4& p0, const pt4& p1, const pt4& p2, const pt4& p3) {
+    const double a11 = p1.x - p0.x, a12 = p1.y - p0.y, a13 = p1.z - p0.z;
+    const double a21 = p2.x - p0.x, a22 = p2.y - p0.y, a23 = p2.z - p0.z;
+    const double a31 = p3.x - p0.x, a32 = p3.y - p0.y, a33 = p3.z - p0.z;
+    const double m1 = a22 * a33 - a23 * a32, m2 = a12 * a33 - a13 * a32, m3 = a12 * a23 - a13 * a22;
+    const double det = a11 * m1 - a21 * m2 + a31 * m3;
+    const double mag = fabs(a11) * (fabs(a22 * a33) + fabs(a23 * a32)) +
+                       fabs(a21) * (fabs(a12 * a33) + fabs(a13 * a32)) +
+                       fabs(a31) * (fabs(a12 * a23) + fabs(a13 * a22));
+    // differences exact to 1 ulp, 3 products and 2 sums per term: gamma_8
+    // of the magnitude covers the evaluation
+    if (fabs(det) > 2e-15 * mag) return det > 0.0 ? 1 : -1;
+    const double r0[4] = {p0.x, p0.y, p0.z, 1.0}, r1[4] = {p1.x, p1.y, p1.z, 1.0};
+    const double r2[4] = {p2.x, p2.y, p2.z, 1.0}, r3[4] = {p3.x, p3.y, p3.z, 1.0};
+    return -det4_sign_exact(r0, r1, r2, r3);
+}
+
+// Cell mesh and primal tetrahedra of a finished cell (mesh_mode 1: counts,
+// 2: write at the cell's offsets).  Vertices: the live corners in slot
+// order; facets: the non-empty faces in plane order, each a cycle of
+// corners (ConvexCell::append_to_mesh, voronoi/convex_cell.cpp:371-444, and
+// ConvexCell::save, :282-358); tetrahedra: (i, j, k, l) for every corner
+// whose three planes are bisectors with seeds j, k, l of larger caller
+// index than i, oriented so that orient_3d > 0 or = 0
+// (RestrictedVoronoiDiagram::compute_RDT, voronoi/RVD.cpp:2302-2327, and
+// PrimalTetrahedronAction, generic_RVD.h:1035-1055).  (gx, gy, gz): the
+// cell's centroid relative to the seed.
+template <int S>
+VGC_HD void mesh_emit(const ClipParams& P, CellLds<S>& L, const CellState<S>& st, int np,
+                      int orig_c, const pt4& pc, double gx, double gy, double gz) {
+    constexpr int kPMax = PMax<S>::value;
+    const LI lane = lane_id();
+    const long long n = P.mesh_n, n1 = n + 1;
+    const bool write = P.mesh_mode == 2;
+    long long voff = 0, foff = 0, fvoff = 0, toff = 0;
+    if (write) {
+        voff = P.mesh_off[orig_c];
+        foff = P.mesh_off[n1 + orig_c];
+        fvoff = P.mesh_off[2 * n1 + orig_c];
+        toff = P.mesh_off[3 * n1 + orig_c];
+    }
+    // vertices: rank of each live corner (kept in L.job for the facet walk)
+    int nv = 0;
+    const double sh = P.mesh_shrink;
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const LB vs = mask_bit(st.valid[s]);
+        const LI cid = lane + 64 * s;
+        const LI r = nv + mbcnt(st.valid[s]);
+        lds_st(L.job, cid, cvt<uint32_t>(r), vs);
+        if (write && P.mesh_v) {
+            const LD qx = lds_ld(&L.q[0][0], cid * 3 + 0), qy = lds_ld(&L.q[0][0], cid * 3 + 1),
+                     qz = lds_ld(&L.q[0][0], cid * 3 + 2);
+            const LD vx = pc.x + (sh * gx + (1.0 - sh) * qx);
+            const LD vy = pc.y + (sh * gy + (1.0 - sh) * qy);
+            const LD vz = pc.z + (sh * gz + (1.0 - sh) * qz);
+            const LI gi = cvt<int>(r) * 3;
+#if defined(VGC_EMU)
+            for (int l = 0; l < kWave; ++l)
+                if (vs[l]) {
+                    P.mesh_v[3 * voff + gi[l] + 0] = vx[l];
+                    P.mesh_v[3 * voff + gi[l] + 1] = vy[l];
+                    P.mesh_v[3 * voff + gi[l] + 2] = vz[l];
+                }
+#else
+            if (vs) {
+                P.mesh_v[3 * voff + gi + 0] = vx;
+                P.mesh_v[3 * voff + gi + 1] = vy;
+                P.mesh_v[3 * voff + gi + 2] = vz;
+            }
+#endif
+        }
+        nv += popc64(st.valid[s]);
+    }
+    wave_sync();
+    // facets, lane f walks face f
+    int nf = 0, nfv = 0;
+    for (int f0 = 0; f0 < np; f0 += 64) {
+        const LI f = f0 + lane;
+        const LB inr = f < np;
+        const LI fs = select(inr, f, LI(0));
+        LI t0 = lds_ld(L.v2t, fs);
+        const LB act = inr & (t0 != LI(0x7FFFFFFF));
+        t0 = select(act, t0, LI(0)) & (64 * S - 1);
+        LI size = select(act, LI(1), LI(0));
+        {
+            LI t = face_next<S>(L, t0, fs);
+            LB run = act & (t != t0);
+            for (int guard = 0; guard < 64 * S && any(run); ++guard) {
+                size = select(run, size + 1, size);
+                t = select(run, face_next<S>(L, t, fs), t);
+                run = run & (t != t0);
+            }
+        }
+        const LB keep = act & (size >= 3);
+        const uint64_t km = ballot(keep);
+        const LI frank = nf + mbcnt(km);
+        const LI start = nfv + wave_exscan(select(keep, size, LI(0)));
+        if (write && P.mesh_fptr && km) {
+            const LI pgf = lds_ld(L.pg, fs);
+            LI nb = LI(-1);
+#if defined(VGC_EMU)
+            for (int l = 0; l < kWave; ++l)
+                if (keep[l]) {
+                    nb[l] = pgf[l] >= 0 ? P.orig[pgf[l]] : -1;
+                    P.mesh_fptr[foff + frank[l]] = fvoff + start[l];
+                    P.mesh_fnbr[foff + frank[l]] = nb[l];
+                }
+#else
+            if (keep) {
+                nb = pgf >= 0 ? P.orig[pgf] : -1;
+                P.mesh_fptr[foff + frank] = fvoff + start;
+                P.mesh_fnbr[foff + frank] = nb;
+            }
+#endif
+            LI t = t0;
+            LI k = LI(0);
+            LB run = keep;
+            for (int guard = 0; guard < 64 * S && any(run); ++guard) {
+                const LI vr = cvt<int>(lds_ld(L.job, t));
+#if defined(VGC_EMU)
+                for (int l = 0; l < kWave; ++l)
+                    if (run[l]) P.mesh_fv[fvoff + start[l] + k[l]] = int(voff) + vr[l];
+#else
+                if (run) P.mesh_fv[fvoff + start + k] = int(voff) + vr;
+#endif
+                k = k + 1;
+                t = select(run, face_next<S>(L, t, fs), t);
+                run = run & (t != t0);
+            }
+        }
+        nf += popc64(km);
+        nfv += popc64(km) ? (int)wave_sum(cvt<double>(select(keep, size, LI(0)))) : 0;
+    }
+    // primal tetrahedra
+    int nt = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const LB vs = mask_bit(st.valid[s]);
+        const LI a = cvt<int>(byte_of(st.pl[s], 0)) & (kPMax - 1);
+        const LI b = cvt<int>(byte_of(st.pl[s], 1)) & (kPMax - 1);
+        const LI c = cvt<int>(byte_of(st.pl[s], 2)) & (kPMax - 1);
+        const LI ga = lds_ld(L.pg, a), gb = lds_ld(L.pg, b), gc = lds_ld(L.pg, c);
+        LB cand = vs & (ga >= 0) & (gb >= 0) & (gc >= 0);
+        LI oa = LI(-1), ob = LI(-1), oc = LI(-1);
+#if defined(VGC_EMU)
+        for (int l = 0; l < kWave; ++l)
+            if (cand[l]) {
+                oa[l] = P.orig[ga[l]];
+                ob[l] = P.orig[gb[l]];
+                oc[l] = P.orig[gc[l]];
+            }
+#else
+        if (cand) {
+            oa = P.orig[ga];
+            ob = P.orig[gb];
+            oc = P.orig[gc];
+        }
+#endif
+        const LB mine = cand & (oa > orig_c) & (ob > orig_c) & (oc > orig_c);
+        const uint64_t mm = ballot(mine);
+        if (write && P.mesh_tet && mm) {
+            const LI tr = nt + mbcnt(mm);
+#if defined(VGC_EMU)
+            for (int l = 0; l < kWave; ++l)
+                if (mine[l]) {
+                    int v0 = orig_c, v1 = oa[l];
+                    if (orient3d_sign(pc, P.pts[ga[l]], P.pts[gb[l]], P.pts[gc[l]]) < 0) {
+                        v0 = oa[l];
+                        v1 = orig_c;
+                    }
+                    int* T = P.mesh_tet + 4 * (toff + tr[l]);
+                    T[0] = v0;
+                    T[1] = v1;
+                    T[2] = ob[l];
+                    T[3] = oc[l];
+                }
+#else
+            if (mine) {
+                int v0 = orig_c, v1 = oa;
+                if (orient3d_sign(pc, P.pts[ga], P.pts[gb], P.pts[gc]) < 0) {
+                    v0 = oa;
+                    v1 = orig_c;
+                }
+                int* T = P.mesh_tet + 4 * (toff + tr);
+                T[0] = v0;
+                T[1] = v1;
+                T[2] = ob;
+                T[3] = oc;
+            }
+#endif
+        }
+        nt += popc64(mm);
+    }
+    if (!write && lane0()) {
+        P.mesh_cnt[orig_c] = nv;
+        P.mesh_cnt[n + orig_c] = nf;
+        P.mesh_cnt[2 * n + orig_c] = nfv;
+        P.mesh_cnt[3 * n + orig_c] = nt;
+    }
+}
+
 // Security-radius loop + integration for work item `item`.  GATHER: the
 // wave finds the neighbours itself, ring after ring of distance; otherwise
 // it reads them from P.list.
@@ -1318,6 +1564,8 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             P.mg[3 * (long long)orig_c + 2] = 0.0;
             P.nbr_cnt[orig_c] = 0;
             if (P.e2) P.e2[orig_c] = 0.0;
+            if (P.mesh_mode == 1)
+                for (int k = 0; k < 4; ++k) P.mesh_cnt[(long long)k * P.mesh_n + orig_c] = 0;
             P.flags[c] = flags | CELL_DONE;
             P.r2[c] = 0.0;
         }
@@ -1371,14 +1619,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         LB act = inr & (t0 != LI(0x7FFFFFFF));
         t0 = select(act, t0, LI(0)) & (64 * S - 1);
         // successor around face f: across edge (p[lv+2] -> p[lv])
-        auto nxt = [&](const LI& t) -> LI {
-            LU plt = lds_ld(L.cpl, t);
-            LU adt = lds_ld(L.cadj, t);
-            LU fu = cvt<uint32_t>(fs);
-            constexpr int B = CornerBits<S>::value;
-            LI sh = select(byte_of(plt, 0) == fu, LI(2 * B), select(byte_of(plt, 1) == fu, LI(0), LI(B)));
-            return cvt<int>((adt >> cvt<uint32_t>(sh)) & LU(CornerBits<S>::mask)) & (64 * S - 1);
-        };
+        auto nxt = [&](const LI& t) -> LI { return face_next<S>(L, t, fs); };
         LD q0x = lds_ld(&L.q[0][0], t0 * 3 + 0), q0y = lds_ld(&L.q[0][0], t0 * 3 + 1),
            q0z = lds_ld(&L.q[0][0], t0 * 3 + 2);
         LI t1 = nxt(t0);
@@ -1461,6 +1702,11 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     }
     const double V = wave_sum(vacc);
     const double ntets = wave_sum(ntet);
+    if (P.mesh_mode && !face_overflow) {
+        const double iv = V > 0.0 ? 1.0 / V : 0.0;
+        mesh_emit<S>(P, L, st, np, orig_c, pc, wave_sum(mxa) * iv, wave_sum(mya) * iv,
+                     wave_sum(mza) * iv);
+    }
     if (P.counters && lane0()) {
         atomic_add_u64(P.counters + 4, (unsigned long long)ntets);
         atomic_add_u64(P.counters + 5, (unsigned long long)nfacets);

@@ -69,6 +69,27 @@ struct Problem {
     // per instance, nonzero = compute its cells; the others keep the outputs
     // (volumes, centroids, neighbours) of the last pass that computed them
     const std::vector<int>* active = nullptr;
+    // cell meshes / primal tetrahedra (ClipParams::mesh_*), one instance
+    int mesh_mode = 0;
+    double mesh_shrink = 0.0;
+    int* mesh_cnt = nullptr;
+    const long long* mesh_off = nullptr;
+    double* mesh_v = nullptr;
+    long long* mesh_fptr = nullptr;
+    int* mesh_fv = nullptr;
+    int* mesh_fnbr = nullptr;
+    int* mesh_tet = nullptr;
+};
+
+// Outputs of Engine::mesh (device memory; null pointers are not written).
+struct MeshOut {
+    double* v = nullptr;        // 3 per vertex
+    long long* fptr = nullptr;  // facets + 1 (the last entry = facet vertex entries)
+    int* fv = nullptr;
+    int* fnbr = nullptr;
+    int* tet = nullptr;         // 4 per tetrahedron
+    long long* cell_off = nullptr;  // 4 (n + 1): per cell first vertex, facet,
+                                    // facet entry, tetrahedron (and the totals)
 };
 
 template <class B>
@@ -399,6 +420,16 @@ public:
         P.pf = d_pf;
         P.nodes = d_nodes;
         P.IL = IL;
+        P.mesh_mode = pb.mesh_mode;
+        P.mesh_n = n_;
+        P.mesh_cnt = pb.mesh_cnt;
+        P.mesh_off = pb.mesh_off;
+        P.mesh_shrink = pb.mesh_shrink;
+        P.mesh_v = pb.mesh_v;
+        P.mesh_fptr = pb.mesh_fptr;
+        P.mesh_fv = pb.mesh_fv;
+        P.mesh_fnbr = pb.mesh_fnbr;
+        P.mesh_tet = pb.mesh_tet;
 
         // pass A over every cell, then the tiers for the flagged ones
         auto t2 = now();
@@ -648,6 +679,47 @@ public:
         return 0;
     }
 
+    // Cell meshes and primal tetrahedra of one instance: a counting pass of
+    // the cells (every tier), exclusive offsets, then, if `cap` holds the
+    // totals (vertices, facets, facet entries, tetrahedra), a writing pass
+    // into `out`.  tot receives the totals.  Returns 0, or 1 when the
+    // capacity is too small (nothing written but the totals), -1 on error.
+    int mesh(Problem pb, double shrink, const long long cap[4], MeshOut& out, long long tot[4],
+             EngineStats& st) {
+        if (pb.n_inst != 1) {
+            error = "mesh: one instance";
+            return -1;
+        }
+        const long long n = pb.n;
+        int* cnt = (int*)slot(S_MCNT, sizeof(int) * 4 * (size_t)n);
+        long long* off = (long long*)slot(S_MOFF, sizeof(long long) * 4 * (size_t)(n + 1));
+        pb.want_neighbors = false;
+        pb.mesh_mode = 1;
+        pb.mesh_cnt = cnt;
+        if (cells(pb, st)) return -1;
+        for (int k = 0; k < 4; ++k) be_.scan_i32(cnt + k * n, off + k * (n + 1), (int)n);
+        for (int k = 0; k < 4; ++k) be_.d2h(&tot[k], off + k * (n + 1) + n, sizeof(long long));
+        if (tot[0] > 0x7FFFFFFFll || tot[2] > 0x7FFFFFFFll) {
+            error = "mesh: more than 2^31 - 1 vertices or facet entries";
+            return -1;
+        }
+        if (out.cell_off) be_.d2d(out.cell_off, off, sizeof(long long) * 4 * (size_t)(n + 1));
+        for (int k = 0; k < 4; ++k)
+            if (tot[k] > cap[k]) return 1;
+        pb.mesh_mode = 2;
+        pb.mesh_off = off;
+        pb.mesh_shrink = shrink;
+        pb.mesh_v = out.v;
+        pb.mesh_fptr = out.fptr;
+        pb.mesh_fv = out.fv;
+        pb.mesh_fnbr = out.fnbr;
+        pb.mesh_tet = out.tet;
+        if (cells(pb, st)) return -1;
+        if (out.fptr) be_.d2d(out.fptr + tot[1], off + 2 * (n + 1) + n, sizeof(long long));
+        be_.sync();
+        return 0;
+    }
+
     // Lloyd relaxation in place on d_xyz (caller order), security radius on.
     int lloyd(Problem pb, double* d_xyz, int iters, EngineStats& st) {
         pb.d_xyz = d_xyz;
@@ -670,7 +742,7 @@ private:
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
         S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2,
         S_QKEYS, S_QKEYS2, S_QVALS, S_QVALS2, S_QPTS, S_QPF, S_QROW, S_QPOS, S_QR2, S_QREDO,
-        S_QLI2, S_QLD, S_QLD2, S_COUNT
+        S_QLI2, S_QLD, S_QLD2, S_MCNT, S_MOFF, S_COUNT
     };
     struct Buf {
         void* p = nullptr;

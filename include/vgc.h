@@ -172,6 +172,65 @@ int vgc_cvt_func_grad(vgc_ctx* ctx, const double* xyz, uint32_t n, const double*
 int vgc_lloyd(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
               uint32_t k0, uint32_t flags, vgc_stats* stats);
 
+typedef struct vgc_mesh_sizes {
+    uint64_t nb_vertices;        /* cell corners, all cells                     */
+    uint64_t nb_facets;          /* polygons, all cells                         */
+    uint64_t nb_facet_vertices;  /* entries of the polygons' vertex lists       */
+    uint64_t nb_tets;            /* primal tetrahedra                           */
+} vgc_mesh_sizes;
+
+/* The cells as polyhedra, one per seed: VBW::ConvexCell::append_to_mesh /
+   save (voronoi/convex_cell.cpp:282-358, 371-444), as compute_RVD's cube
+   mode writes them (examples/geogram/compute_RVD/main.cpp:450-477).
+   vertices[3 * nb_vertices]: the corners in absolute coordinates, pulled
+   towards the cell's centroid by `shrink` (0: none); cell i owns vertices
+   cell_vptr[i] .. cell_vptr[i+1]-1 and facets cell_fptr[i] ..
+   cell_fptr[i+1]-1 (both n+1 entries).  Facet f is the polygon
+   facet_vertices[facet_ptr[f] .. facet_ptr[f+1]-1] (global vertex numbers,
+   counter-clockwise seen from outside), cut by seed facet_nbr[f] (caller's
+   index) or by the box (-1).  Optionally tets[4 * nb_tets] (may be NULL):
+   the primal tetrahedra, as vgc_rdt.  Two-call protocol: with NULL
+   outputs, or capacities `cap` below the sizes, only the sizes are
+   returned in `need` (VGC_ERR_CAPACITY when outputs were given). */
+int vgc_cell_mesh(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+                  double shrink, uint32_t flags, uint64_t* cell_vptr, uint64_t* cell_fptr,
+                  double* vertices, uint64_t* facet_ptr, int32_t* facet_vertices,
+                  int32_t* facet_nbr, int32_t* tets, const vgc_mesh_sizes* cap,
+                  vgc_mesh_sizes* need, vgc_stats* stats);
+
+/* Restricted Delaunay triangulation of a volume (a box here): one
+   tetrahedron (i, j, k, l) per cell corner whose three planes are bisectors
+   (or power planes, w != NULL) with seeds j, k, l, emitted by the cell of
+   the smallest index and oriented so that orient_3d(i, j, k, l) >= 0.
+   Replaces RestrictedVoronoiDiagram::compute_RDT in volumetric mode
+   (voronoi/RVD.h:504-510, RVD.cpp:2302-2327, for_each_primal_tetrahedron,
+   generic_RVD.h:1011-1055).  tets: 4 * cap entries; the count is returned
+   in *nb_tets (VGC_ERR_CAPACITY if cap is too small). */
+int vgc_rdt(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
+            uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats);
+
+typedef struct vgc_opt_stats {
+    uint32_t iterations;      /* line searches completed                        */
+    uint32_t evaluations;     /* energy / gradient evaluations (cell passes)    */
+    int32_t line_search_info; /* last line search: 1 converged, 2 interval below
+                                 xtol, 3 20 evaluations, 4/5 step at its bound,
+                                 6 rounding, -1 not a descent direction         */
+    uint32_t failed_cells;    /* flagged cells over all evaluations            */
+    double f0, f;             /* CVT energy at the start and at the end        */
+    double gnorm0, gnorm;     /* gradient norms                                */
+    double t_total_ms;
+} vgc_opt_stats;
+
+/* Quasi-Newton CVT: minimises the CVT energy of the Voronoi cells of xyz in
+   the box by L-BFGS with m correction pairs (m = 7 in geogram) and a
+   More-Thuente line search, xyz updated in place; the iteration stops when
+   a line search fails or after iters + 1 line searches (HLBFGS counts
+   that way).  Replaces CentroidalVoronoiTesselation::Newton_iterations
+   (voronoi/CVT.cpp:272-302, HLBFGS of third_party/HLBFGS with epsg = epsf =
+   epsx = 0) with the energy and gradient of vgc_cvt_func_grad. */
+int vgc_cvt_newton(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
+                   uint32_t m, uint32_t flags, vgc_opt_stats* stats);
+
 /* Lloyd iterations of n_inst independent CVT problems at once (instance
    layout of vgc_cells_batch): each iteration clips every instance's cells in
    the same launches.  No counterpart in geogram, which runs one
@@ -243,6 +302,16 @@ int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint3
 /* gradient of the mean squared error over the n rows; loss = that mean */
 int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const float* target,
                  uint32_t n, uint32_t flags, float* grad, double* loss);
+/* vgc_mlp_grad without any allocation, for callers that cannot allocate
+   device memory (code under HIP graph capture, training loops that forbid
+   allocation).  No counterpart in geogram.  Device pointers only; loss is
+   one device double (may be NULL), grad VGC_MLP_PARAMS floats; results equal
+   vgc_mlp_grad's bit for bit.  n <= 32768 rows (VGC_ERR_CAPACITY beyond):
+   the per-wave partials live in a static device array of 19.2 MB that each
+   GPU holds for the life of the process and that every call on that GPU
+   shares — calls that overlap on one GPU (several contexts or threads) race. */
+int vgc_mlp_grad_noalloc(vgc_ctx* ctx, const float* params, const float* feats,
+                         const float* target, uint32_t n, float* grad, double* loss);
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); device pointers only */
 int vgc_mlp_adam(vgc_ctx* ctx, float* params, const float* grad, float* m, float* v,
                  uint32_t step, float lr, uint32_t flags);

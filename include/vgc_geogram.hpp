@@ -176,12 +176,19 @@ public:
         vgc_stats st;
         ctx_.check(vgc_lloyd(ctx_.get(), points_.data(), nb_points(), box_, nb_iter, 0, 0, &st));
     }
+    // voronoi/CVT.h:164 (HLBFGS with m pairs in geogram)
+    virtual void Newton_iterations(index_t nb_iter, index_t m = 7) {
+        ctx_.check(vgc_cvt_newton(ctx_.get(), points_.data(), nb_points(), box_, nb_iter, m, 0,
+                                  &opt_stats_));
+    }
+    const vgc_opt_stats& optimizer_stats() const { return opt_stats_; }
     virtual ~CentroidalVoronoiTesselation() {}
 
 private:
     Context& ctx_;
     double box_[6];
     std::vector<double> points_;
+    vgc_opt_stats opt_stats_{};
 };
 
 // Semi-discrete optimal transport, uniform density on a box -> seeds.

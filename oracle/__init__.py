@@ -64,6 +64,8 @@ def lib():
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_local_planes.argtypes = [ctypes.c_int]
         L.orc_set_e2_output.argtypes = [ctypes.c_void_p]
+        L.orc_set_mesh_output.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -158,6 +160,37 @@ def cells(xyz, w=None, box=UNIT_BOX, k0=20, mode=MODE_SR, stride=96, subset=None
     out = dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, stats=st.as_dict())
     if e2:
         out["e2"] = e2a
+    return out
+
+
+def cell_meshes(xyz, w=None, box=UNIT_BOX, subset=None, centred=True, istride=2048,
+                vstride=768):
+    """Polyhedron and primal tetrahedra of each cell (ConvexCell::save /
+    append_to_mesh, compute_RDT).  Returns a list, per cell, of dicts with
+    "v" (nv, 3) vertices, "faces" [(neighbour or -1, [vertex numbers])] and
+    "tets" [(i, j, k, l)]."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    nc = xyz.shape[0] if subset is None else len(subset)
+    ib = np.zeros((nc, istride), dtype=np.int32)
+    vb = np.zeros((nc, vstride))
+    lib().orc_set_mesh_output(ib.ctypes.data_as(ctypes.c_void_p), istride,
+                              vb.ctypes.data_as(ctypes.c_void_p), vstride)
+    try:
+        cells(xyz, w=w, box=box, subset=subset, centred=centred, stride=256)
+    finally:
+        lib().orc_set_mesh_output(None, 0, None, 0)
+    out = []
+    for c in range(nc):
+        nv, nf, nt = ib[c, :3]
+        if nv < 0:
+            raise RuntimeError("cell_meshes: stride too small")
+        pos, faces = 3, []
+        for _ in range(nf):
+            nb, sz = int(ib[c, pos]), int(ib[c, pos + 1])
+            faces.append((nb, ib[c, pos + 2:pos + 2 + sz].tolist()))
+            pos += 2 + sz
+        tets = [tuple(ib[c, pos + 4 * t:pos + 4 * t + 4].tolist()) for t in range(nt)]
+        out.append({"v": vb[c, :3 * nv].reshape(-1, 3).copy(), "faces": faces, "tets": tets})
     return out
 
 

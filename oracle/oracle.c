@@ -22,6 +22,17 @@ void orc_set_local_planes(int on) { g_local = on; }
    |x - p_i|^2 over cell i (NULL: off) */
 static double* g_e2 = NULL;
 void orc_set_e2_output(double* e2) { g_e2 = e2; }
+/* cell meshes of the next orc_cells* calls (NULL: off), see oracle.h */
+static int* g_mi = NULL;
+static int g_mi_stride = 0;
+static double* g_mv = NULL;
+static int g_mv_stride = 0;
+void orc_set_mesh_output(int* ibuf, int istride, double* vbuf, int vstride) {
+    g_mi = ibuf;
+    g_mi_stride = istride;
+    g_mv = vbuf;
+    g_mv_stride = vstride;
+}
 void orc_set_threads(int nt) { g_threads = nt; }
 int orc_get_threads(void) {
 #ifdef _OPENMP
@@ -728,9 +739,96 @@ typedef struct {
     const kd_tree* T;
 } cell_job;
 
+/* Polyhedron of a finished cell as ConvexCell::save / append_to_mesh
+   write it (voronoi/convex_cell.cpp:282-358, 371-444): vertices = alive
+   triangles in index order (absolute coordinates), one facet per plane v
+   with v2t[v] set and at least 3 corners, walked by triangle_adjacent(t,
+   (lv + 1) % 3), tagged with the plane's global index (-1 for the box); and
+   the primal tetrahedra (i, v1, v2, v3) of triangles whose three planes are
+   bisectors with v1, v2, v3 > i (generic_RVD.h:1035-1055), reoriented so
+   that orient3d >= 0 (RVD.cpp:2311-2322).  ib: [nv, nf, nt, faces (nbr,
+   size, vertex...), tets (4 each)]; -1 in ib[0] when a stride is too
+   small. */
+/* orient_3d (numerics/predicates.cpp:1889-1899): sign of
+   det(p1 - p0, p2 - p0, p3 - p0) = -det4 of the rows (p, 1), exact */
+static int orient3d_sign_exact(const double* p0, const double* p1, const double* p2,
+                               const double* p3) {
+    double r0[4] = {p0[0], p0[1], p0[2], 1.0}, r1[4] = {p1[0], p1[1], p1[2], 1.0};
+    double r2[4] = {p2[0], p2[1], p2[2], 1.0}, r3[4] = {p3[0], p3[1], p3[2], 1.0};
+    return -orc_det4_sign(r0, r1, r2, r3);
+}
+
+static void cell_mesh_out(const cell* C, int i, const double* shift, const double* xyz,
+                          int* ib, int istride, double* vb, int vstride) {
+    int* rank = (int*)malloc(sizeof(int) * (size_t)(C->nt > 0 ? C->nt : 1));
+    int nv = 0;
+    for (int t = 0; t < C->nt; ++t) {
+        rank[t] = -1;
+        if (!C->alive[t]) continue;
+        if (3 * (nv + 1) > vstride) {
+            ib[0] = -1;
+            free(rank);
+            return;
+        }
+        for (int c = 0; c < 3; ++c) vb[3 * nv + c] = C->pt[t][c] + shift[c];
+        rank[t] = nv++;
+    }
+    int pos = 3, nf = 0;
+    for (int v = 1; v < C->nv; ++v) {
+        int t0 = C->v2t[v];
+        if (t0 < 0) continue;
+        int sz = 0, t = t0, guard = 0;
+        do {
+            ++sz;
+            t = walk_next(C, t, v);
+        } while (t != t0 && guard++ < 100000);
+        if (sz < 3) continue;
+        if (pos + 2 + sz > istride) {
+            ib[0] = -1;
+            free(rank);
+            return;
+        }
+        ib[pos++] = C->vg[v];
+        ib[pos++] = sz;
+        t = t0;
+        do {
+            ib[pos++] = rank[t];
+            t = walk_next(C, t, v);
+        } while (t != t0);
+        ++nf;
+    }
+    int nt = 0;
+    for (int t = 0; t < C->nt; ++t) {
+        if (!C->alive[t]) continue;
+        int a = C->vg[C->T[t][0]], b = C->vg[C->T[t][1]], c = C->vg[C->T[t][2]];
+        if (a < 0 || b < 0 || c < 0 || !(a > i && b > i && c > i)) continue;
+        if (pos + 4 > istride) {
+            ib[0] = -1;
+            free(rank);
+            return;
+        }
+        int v0 = i, v1 = a;
+        if (orient3d_sign_exact(xyz + 3 * (size_t)i, xyz + 3 * (size_t)a, xyz + 3 * (size_t)b,
+                                xyz + 3 * (size_t)c) < 0) {
+            v0 = a;
+            v1 = i;
+        }
+        ib[pos++] = v0;
+        ib[pos++] = v1;
+        ib[pos++] = b;
+        ib[pos++] = c;
+        ++nt;
+    }
+    ib[0] = nv;
+    ib[1] = nf;
+    ib[2] = nt;
+    free(rank);
+}
+
 static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
                        double* vol, double* mg, int* cnt, int* nbr,
-                       double* farea, int stride, orc_stats* st, double* e2) {
+                       double* farea, int stride, orc_stats* st, double* e2,
+                       int* mib, double* mvb) {
     const double* pi_abs = J->xyz + 3 * (size_t)i;
     /* seed-centred frame (optional): the same planes and box translated by
        -pi before they are formed, so the offsets carry no |pi|^2 rounding */
@@ -798,6 +896,7 @@ static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
         st->nb_knn_grow++;
     }
     *cnt = 0;
+    if (mib) mib[0] = mib[1] = mib[2] = 0;
     if (C->empty) {
         st->nb_empty++;
         *vol = 0.0;
@@ -806,6 +905,10 @@ static int compute_one(const cell_job* J, int i, cell* C, nbr_scratch* S,
         return 0;
     }
     cell_geometry(C);
+    if (mib) {
+        const double zero3[3] = {0.0, 0.0, 0.0};
+        cell_mesh_out(C, i, g_local ? pi_abs : zero3, J->xyz, mib, g_mi_stride, mvb, g_mv_stride);
+    }
     cell_volume_mg(C, vol, mg);
     if (e2) *e2 = cell_second_moment(C, pi);
     if (g_local)
@@ -865,7 +968,9 @@ int orc_cells_subset(const double* xyz, const double* w, int n,
             int e = compute_one(&J, i, &C, &S, vol + c, mg + 3 * (size_t)c,
                                 nbr_cnt + c, nbr + (size_t)c * stride,
                                 farea + (size_t)c * stride, stride, &loc,
-                                g_e2 ? g_e2 + c : NULL);
+                                g_e2 ? g_e2 + c : NULL,
+                                g_mi ? g_mi + (size_t)c * g_mi_stride : NULL,
+                                g_mv ? g_mv + (size_t)c * g_mv_stride : NULL);
             if (e) lerr = e;
         }
 #pragma omp critical

@@ -79,6 +79,13 @@ void orc_set_local_planes(int on);
 /* per-cell integral of |x - p_i|^2 of the next orc_cells* calls (NULL: off) */
 void orc_set_e2_output(double* e2);
 
+/* Cell polyhedra of the next orc_cells* calls (NULL: off): per cell c,
+   ibuf[c * istride ...] = nv, nf, nt, then nf faces (neighbour or -1, size,
+   size vertex numbers of the cell), then nt tetrahedra (4 seed indices);
+   vbuf[c * vstride ...] = 3 nv vertex coordinates.  nv = -1 when a stride
+   is too small. */
+void orc_set_mesh_output(int* ibuf, int istride, double* vbuf, int vstride);
+
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int nt);
 int orc_get_threads(void);

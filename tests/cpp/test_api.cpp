@@ -31,6 +31,19 @@ int main() {
     CentroidalVoronoiTesselation cvt(*ctx, box);
     cvt.set_points(n, x.data());
     cvt.Lloyd_iterations(5);
+    // quasi-Newton from the Lloyd points: the energy decreases
+    CentroidalVoronoiTesselation cvt2(*ctx, box);
+    cvt2.set_points(n, cvt.embedding(0));
+    cvt2.Newton_iterations(5);
+    if (!(cvt2.optimizer_stats().f < cvt2.optimizer_stats().f0)) return 5;
+    // exact kNN through the NearestNeighborSearch mirror: the nearest point
+    // of a stored point is itself, at distance 0
+    NearestNeighborSearch nn(*ctx);
+    nn.set_points(n, x.data());
+    index_t nb[4];
+    double d2[4];
+    nn.get_nearest_neighbors(4, index_t(17), nb, d2);
+    if (nb[0] != 17 || d2[0] != 0.0 || !(d2[1] > 0.0) || !(d2[2] >= d2[1])) return 6;
     OptimalTransportMap3d otm(*ctx, box);
     otm.set_points(n, cvt.embedding(0));
     otm.set_epsilon(1e-4);

@@ -291,3 +291,39 @@ def test_c5_batched_instances_lloyd_and_transport(gpu_ctx):
         o = oracle.cells(x[sl], w=w[sl], k0=20, subset=sample, centred=True, stride=256)
         err = np.abs(o["vol"] - nu[sl][sample]) / nu[sl][sample]
         assert err.max() <= 1e-4 * (1 + 1e-6)
+
+
+# ----------------------------------------------------- quasi-Newton CVT
+
+def test_cvt_newton_vs_oracle_lbfgs(gpu_ctx):
+    """L-BFGS (m = 7) on the CVT energy of 20k seeds: the GPU driver follows
+    the oracle optimiser (oracle/lbfgs.py) fed with the oracle's energy and
+    gradient, iterate for iterate."""
+    from oracle import lbfgs as olb
+
+    def fg(xf):
+        x = xf.reshape(-1, 3)
+        o = oracle.cells(x, k0=20, centred=True, e2=True)
+        return float(o["e2"].sum()), (2.0 * (o["vol"][:, None] * x - o["mg"])).reshape(-1)
+
+    x0 = ga.sample_points(1, 35, 20000)
+    xg, st = gpu_ctx.cvt_newton(x0, 8)
+    xo, fo, go, hist, nev, info = olb.lbfgs(fg, x0.reshape(-1), 8)
+    assert st["iterations"] == len(hist) and st["evaluations"] == nev
+    assert abs(st["f"] - fo) <= 1e-11 * fo
+    assert np.abs(xg.reshape(-1) - xo).max() <= 1e-9
+    assert st["f"] < st["f0"] and st["failed_cells"] == 0
+
+
+# --------------------------------------------- cell meshes / primal tetrahedra
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_cell_mesh_and_rdt_vs_oracle(gpu_ctx, weighted):
+    from tests.test_mesh import compare_meshes, compare_tets
+    x = ga.sample_points(1, 52, 4000)
+    w = np.random.default_rng(3).random(len(x)) * 2e-4 if weighted else None
+    r = gpu_ctx.cell_mesh(x, weights=w, tets=True)
+    m = oracle.cell_meshes(x, w=w)
+    compare_meshes(r, m, x)
+    compare_tets(r["tets"], m, x)
+    assert np.array_equal(gpu_ctx.rdt(x, weights=w), r["tets"])
