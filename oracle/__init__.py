@@ -222,3 +222,62 @@ def exact_fallbacks() -> int:
 def neighbour_sets(res):
     """List of frozensets of neighbour ids per cell."""
     return [frozenset(res["nbr"][i, : res["cnt"][i]].tolist()) for i in range(len(res["cnt"]))]
+
+
+# ------------------------------------------------- the reference's own code
+_REF_PATH = os.path.join(_HERE, "_ref", "libgeoref.so")
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF_PATH)
+
+
+def ref_lib():
+    """oracle/_ref/libgeoref.so (built by oracle/build_ref.sh from geogram's
+    sources): geogram's Delaunay + ConvexCell cells and its kd-tree kNN."""
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(_REF_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.georef_delaunay_cells.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int, dp, dp, ip,
+                                            ip, dp, ctypes.c_int, dp]
+        L.georef_knn.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.georef_max_threads.restype = ctypes.c_int
+        _ref = L
+    return _ref
+
+
+def ref_cells(xyz, w=None, box=UNIT_BOX, threads=0, stride=128):
+    """geogram's fastest CPU path (PeriodicDelaunay3d + ConvexCell + box
+    clips): dict(vol, mg, cnt, nbr, farea, t_delaunay_s, t_cells_s)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    wp = None
+    if w is not None:
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        wp = _dp(w)
+    vol = np.zeros(n)
+    mg = np.zeros((n, 3))
+    cnt = np.zeros(n, dtype=np.int32)
+    nbr = np.full((n, stride), -1, dtype=np.int32)
+    fa = np.zeros((n, stride))
+    t = np.zeros(2)
+    err = ref_lib().georef_delaunay_cells(_dp(xyz), wp, n, _dp(box), int(threads), _dp(vol),
+                                          _dp(mg), _ip(cnt), _ip(nbr), _dp(fa), stride, _dp(t))
+    if err:
+        raise RuntimeError(f"georef_delaunay_cells failed ({err})")
+    return dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, t_delaunay_s=float(t[0]),
+                t_cells_s=float(t[1]))
+
+
+def ref_knn(xyz, k):
+    """geogram's kd-tree: the k nearest points of each point, itself first."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    idx = np.zeros((n, k), dtype=np.int32)
+    d2 = np.zeros((n, k))
+    ref_lib().georef_knn(_dp(xyz), n, k, _ip(idx), _dp(d2))
+    return idx, d2

@@ -1,0 +1,129 @@
+// ORACLE — test infrastructure only.
+//
+// C entry points over geogram's own code (the Delaunay pluggable module that
+// oracle/build_ref.sh assembles from the reference sources): the reference
+// path itself, used to pin the oracle and as the CPU baseline.
+//
+//  georef_delaunay_cells: geogram's fastest CPU path for Voronoi / Laguerre
+//    cells in a box — PeriodicDelaunay3d(false) with the seeds (and
+//    weights), copy_Laguerre_cell_from_Delaunay, six box clips, then
+//    compute_geometry / volume / compute_mg / facet_area / v_global_index of
+//    the contributing planes (delaunay/periodic_delaunay_3d.cpp:3634-3733,
+//    voronoi/convex_cell.cpp:1160-1307; compute_RVD's cube mode,
+//    examples/geogram/compute_RVD/main.cpp:450-477), cells in parallel;
+//  georef_knn: NearestNeighborSearch "BNN" (points/kd_tree.cpp),
+//    get_nearest_neighbors of each point by index (the point included).
+#include "Delaunay_psm.h"
+
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+using namespace GEO;
+
+namespace {
+
+std::once_flag g_init;
+
+void init(int nthreads) {
+    std::call_once(g_init, [] {
+        GEO::initialize(GEO::GEOGRAM_INSTALL_NONE);
+        Logger::instance()->set_quiet(true);
+    });
+    if (nthreads > 0) Process::set_max_threads(index_t(nthreads));
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+// Cells of the n seeds (w: weights or NULL) clipped to box: vol[n],
+// mg[3n] (integral of x), cnt[n] and nbr / farea [n * stride] (contributing
+// neighbours, NO_INDEX box planes left out).  t[0]: Delaunay seconds,
+// t[1]: cells seconds.  Returns 0, -1 if a cell has more than stride
+// neighbours.
+int georef_delaunay_cells(const double* xyz, const double* w, int n, const double* box,
+                          int nthreads, double* vol, double* mg, int* cnt, int* nbr,
+                          double* farea, int stride, double* t) {
+    init(nthreads);
+    auto t0 = std::chrono::steady_clock::now();
+    PeriodicDelaunay3d D(false);
+    D.set_keeps_infinite(true);
+    D.set_vertices(index_t(n), xyz);
+    if (w) D.set_weights(w);
+    D.compute();
+    t[0] = seconds_since(t0);
+    auto t1 = std::chrono::steady_clock::now();
+    std::atomic<int> err(0);
+    parallel_for_slice(0, index_t(n), [&](index_t b, index_t e) {
+        VBW::ConvexCell C(VBW::WithVGlobal);
+        PeriodicDelaunay3d::IncidentTetrahedra W;
+        for (index_t v = b; v < e; ++v) {
+            D.copy_Laguerre_cell_from_Delaunay(v, C, W);
+            // box planes with the "no seed" global index (clip_by_plane(P)
+            // alone would leave the slot's index of an earlier cell)
+            const VBW::global_index_t none = VBW::global_index_t(-1);
+            C.clip_by_plane(vec4(1.0, 0.0, 0.0, -box[0]), none);
+            C.clip_by_plane(vec4(-1.0, 0.0, 0.0, box[3]), none);
+            C.clip_by_plane(vec4(0.0, 1.0, 0.0, -box[1]), none);
+            C.clip_by_plane(vec4(0.0, -1.0, 0.0, box[4]), none);
+            C.clip_by_plane(vec4(0.0, 0.0, 1.0, -box[2]), none);
+            C.clip_by_plane(vec4(0.0, 0.0, -1.0, box[5]), none);
+            cnt[v] = 0;
+            if (C.empty()) {
+                vol[v] = 0.0;
+                mg[3 * v] = mg[3 * v + 1] = mg[3 * v + 2] = 0.0;
+                continue;
+            }
+            C.compute_geometry();
+            double m = 0.0;
+            vec3 g;
+            C.compute_mg(m, g);
+            vol[v] = m;
+            mg[3 * v] = g.x;
+            mg[3 * v + 1] = g.y;
+            mg[3 * v + 2] = g.z;
+            for (index_t lv = 1; lv < C.nb_v(); ++lv) {
+                if (!C.vertex_is_contributing(lv)) continue;
+                const VBW::global_index_t j = C.v_global_index(lv);
+                if (j == VBW::global_index_t(-1)) continue;
+                if (cnt[v] >= stride) {
+                    err = -1;
+                    break;
+                }
+                nbr[size_t(v) * stride + cnt[v]] = int(j);
+                farea[size_t(v) * stride + cnt[v]] = C.facet_area(lv);
+                ++cnt[v];
+            }
+        }
+    });
+    t[1] = seconds_since(t1);
+    return err;
+}
+
+// k nearest points of each point (the point itself first), by geogram's
+// kd-tree: idx / d2 are n * k.
+int georef_knn(const double* xyz, int n, int k, int* idx, double* d2) {
+    init(0);
+    NearestNeighborSearch_var nn = NearestNeighborSearch::create(3, "BNN");
+    nn->set_points(index_t(n), xyz);
+    parallel_for(0, index_t(n), [&](index_t i) {
+        std::vector<index_t> ix(k);
+        nn->get_nearest_neighbors(index_t(k), i, ix.data(), d2 + size_t(i) * k);
+        for (int j = 0; j < k; ++j) idx[size_t(i) * k + j] = int(ix[j]);
+    });
+    return 0;
+}
+
+int georef_max_threads(void) {
+    init(0);
+    return int(Process::maximum_concurrent_threads());
+}
+
+}  // extern "C"

@@ -66,3 +66,20 @@ def test_cpp_mirror_runs_on_gpu(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+REFERENCE_LIB = "/root/reference/src/lib"
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_LIB), reason="geogram sources not present")
+def test_geogram_shim_compiles_against_reference_headers():
+    """geogram_amd/shim/geogram_vgc.cpp — the translation unit a geogram
+    maintainer adds (NearestNeighborSearch "VGC", RestrictedVoronoiDiagram
+    and CentroidalVoronoiTesselation subclasses, the OptimalTransportMap3d
+    calls) — compiles against geogram's own headers with every pure virtual
+    implemented."""
+    shim = os.path.join(ROOT, "geogram_amd", "shim", "geogram_vgc.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I", REFERENCE_LIB, "-I", os.path.join(ROOT, "include"), shim],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

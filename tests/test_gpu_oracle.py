@@ -327,3 +327,27 @@ def test_cell_mesh_and_rdt_vs_oracle(gpu_ctx, weighted):
     compare_meshes(r, m, x)
     compare_tets(r["tets"], m, x)
     assert np.array_equal(gpu_ctx.rdt(x, weights=w), r["tets"])
+
+
+# ------------------------------------------------- against the reference itself
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("weighted", [False, True])
+def test_cells_vs_reference_build(gpu_ctx, weighted):
+    """GPU cells against geogram's own Delaunay + ConvexCell code (the
+    reference compiled from its sources, oracle/_ref): every neighbour set
+    equal, volumes and facet areas within 1e-10 relative plus the rounding of
+    the reference's absolute-frame planes."""
+    x = ga.sample_points(1, 2, 200_000)
+    w = np.random.default_rng(1).random(len(x)) * 1e-6 if weighted else None
+    r = gpu_ctx.cells(x, weights=w)
+    ref = oracle.ref_cells(x, w)
+    vol = ref["vol"]
+    assert np.all(np.abs(r.vol - vol) <= vol * (1e-10 + 4e-15 / vol ** (2 / 3)))
+    ptr = r.nbr_ptr.astype(np.int64)
+    bad = 0
+    for i in range(len(x)):
+        a = r.nbr[ptr[i]:ptr[i + 1]]
+        if set(a.tolist()) != set(ref["nbr"][i, : ref["cnt"][i]].tolist()):
+            bad += 1
+    assert bad == 0
