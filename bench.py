@@ -52,15 +52,89 @@ COUNTERS = os.path.join(ROOT, "profiles", "k_clip_counters_2M_mixture.md")
 def profile_figure(key):
     """A `key: value` line of the rocprofv3 counter summary of the clip
     kernel (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE scaled by a
-    known-size copy, VALU issue fraction from SQ counters), or None."""
+    known-size copy, VALU issue fraction from SQ counters) — None unless the
+    summary records the hash of the kernel sources this build compiles
+    (kernel_sources_sha1), so a stale profile is never reported."""
+    vals = {}
     try:
         with open(COUNTERS) as f:
             for line in f:
-                if line.startswith(key + ":"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and not line.startswith(("|", "#", " ")):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if vals.get("kernel_sources_sha1") != kernel_sources_sha1() or key not in vals:
+        return None
+    return float(vals[key])
+
+
+def cell_errors(np, g, o, rows):
+    """Per-output error of GPU cells g (host arrays: vol, mg, ptr, nbr,
+    farea, caller order) against a CPU result o (vol, mg, cnt, nbr, farea;
+    row t is cell rows[t]): max relative volume error, max centroid error
+    over the cell size, max relative facet-area error (facets above 1e-6 of
+    the cell's face scale), and the count of neighbour sets that differ."""
+    vo = o["vol"]
+    vg = g["vol"][rows]
+    nz = vo > 0
+    vrel = np.abs(vg[nz] - vo[nz]) / vo[nz]
+    cg = g["mg"][rows][nz] / vg[nz, None]
+    co = o["mg"][nz] / vo[nz, None]
+    cen = np.abs(cg - co).max(axis=1) / np.cbrt(vo[nz])
+    # neighbour sets and facet areas, row by row in sorted order
+    gc = (g["ptr"][rows + 1] - g["ptr"][rows])
+    oc = o["cnt"].astype(np.int64)
+    same_cnt = gc == oc
+    if len(rows) == len(g["vol"]):
+        # the whole workload: the rows are every cell in order
+        gn, ga_ = g["nbr"], g["farea"]
+    else:
+        first = np.cumsum(gc) - gc
+        idx = np.repeat(g["ptr"][rows], gc) + (np.arange(int(gc.sum())) - np.repeat(first, gc))
+        gn, ga_ = g["nbr"][idx], g["farea"][idx]
+    grow = np.repeat(np.arange(len(rows)), gc)
+    orow = np.repeat(np.arange(len(rows)), oc)
+    mask = np.arange(o["nbr"].shape[1])[None, :] < oc[:, None]
+    on, oa = o["nbr"][mask], o["farea"][mask]
+    gk = np.lexsort((gn, grow))
+    ok = np.lexsort((on, orow))
+    if len(gk) == len(ok):
+        rowdiff = (gn[gk] != on[ok]) | (grow[gk] != orow[ok])
+        bad_rows = np.unique(grow[gk][rowdiff])
+        mism = int(len(bad_rows) + np.count_nonzero(~same_cnt & ~np.isin(np.arange(len(rows)),
+                                                                           bad_rows)))
+        good = ~np.isin(grow[gk], bad_rows) & same_cnt[grow[gk]]
+        scale = vo[grow[gk]] ** (2.0 / 3.0)
+        fa_o, fa_g = oa[ok], ga_[gk]
+        big = good & (fa_o > 1e-6 * scale)
+        frel = float((np.abs(fa_g - fa_o)[big] / fa_o[big]).max()) if big.any() else 0.0
+    else:
+        mism = int(np.count_nonzero(~same_cnt))
+        frel = None
+    return {"cells": int(len(rows)), "vol_rel_max": float(vrel.max()),
+            "centroid_max_over_cell_size": float(cen.max()), "facet_area_rel_max": frel,
+            "neighbour_sets_differing": mism}
+
+
+def reference_mass_error(np, x, w, nu):
+    """max_i |m_i - nu_i| / nu_i with the masses m of geogram's own Laguerre
+    cells of (x, w) (the reference build, oracle/_ref), or None."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    m = oracle.ref_cells(x, w, threads=min(16, os.cpu_count() or 1))["vol"]
+    return float(np.max(np.abs(m - nu) / nu))
+
+
+def kernel_sources_sha1():
+    """Hash of the clipping kernel's sources (the profiled code)."""
+    import hashlib
+    h = hashlib.sha1()
+    for f in ("clip_cell.h", "wave.h", "types.h", "common.h", "exact_det.h", "backend_hip.hip"):
+        with open(os.path.join(ROOT, "geogram_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def main():
@@ -136,7 +210,9 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
         "traffic": profile_figure("traffic_bytes_per_launch") if args.n == 2_000_000 and args.k0 == 0 else None,
-        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                        "from profiles/k_clip_counters_2M_mixture.md when it was taken of "
+                        "these kernel sources, else null)",
         "valu_busy_frac": profile_figure("valu_busy_frac") if args.n == 2_000_000 and args.k0 == 0 else None,
         "alg_flops_per_launch": int(flops_a),
         "flops_per_cell": round(flops_a / n, 1),
@@ -152,14 +228,16 @@ def main():
         nu = torch.full((m,), 1.0 / m, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _, ost = ctx.ot_solve(xo, nu, tol=1e-4, verbose=True, inexact=True)
+        wo, ost = ctx.ot_solve(xo, nu, tol=1e-4, verbose=True, inexact=True)
         torch.cuda.synchronize()
         tot = time.perf_counter() - t2
         ot = {"n": m, "start": "plain (w = 0)", "time_s": round(tot, 3),
               "newton_iters": int(ost["newton_iters"]), "cg_iters": int(ost["cg_iters"]),
               "cell_passes": int(ost["cell_passes"]), "err0": ost["err0"],
               "final_err": ost["final_err"], "converged": bool(ost["converged"]),
-              "t_cells_ms": round(ost["t_cells_ms"], 1), "t_cg_ms": round(ost["t_cg_ms"], 1)}
+              "t_cells_ms": round(ost["t_cells_ms"], 1), "t_cg_ms": round(ost["t_cg_ms"], 1),
+              "mass_err_reference": reference_mass_error(np, xo.cpu().numpy(), wo.cpu().numpy(),
+                                                         nu.cpu().numpy())}
 
     # learned warm start: 1M seeds of a 16-Gaussian mixture with log-normal
     # masses (seed 4), plain start against the MLP start (features + forward
@@ -173,12 +251,14 @@ def main():
         x4d, nu4d = torch.from_numpy(x4).cuda(), torch.from_numpy(nu4).cuda()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _, sp = ctx.ot_solve(x4d, nu4d, tol=1e-4, verbose=True, inexact=True)
+        w4, sp = ctx.ot_solve(x4d, nu4d, tol=1e-4, verbose=True, inexact=True)
         torch.cuda.synchronize()
         tp = time.perf_counter() - t2
         ws = {"n": m, "plain": {"time_s": round(tp, 3), "newton_iters": int(sp["newton_iters"]),
                                 "cell_passes": int(sp["cell_passes"]),
-                                "converged": bool(sp["converged"]), "err0": sp["err0"]}}
+                                "converged": bool(sp["converged"]), "err0": sp["err0"],
+                                "mass_err_reference": reference_mass_error(np, x4, w4.cpu().numpy(),
+                                                                           nu4)}}
         if os.path.exists(warmstart.PARAMS_FILE):
             params = torch.from_numpy(warmstart.load_params()).cuda()
             torch.cuda.synchronize()
@@ -239,24 +319,48 @@ def main():
                         "cell_passes": int(s5[0]["cell_passes"]),
                         "t_cells_ms": round(s5[0]["t_cells_ms"], 1), "t_cg_ms": round(s5[0]["t_cg_ms"], 1)}
 
-    cpu = None
+    # CPU baseline and per-output errors.  The reference itself (geogram's
+    # Delaunay + ConvexCell path, oracle/_ref) runs the whole workload on the
+    # host cores when it is built; the oracle (plain-C restatement) runs a
+    # sample otherwise.  Errors of the GPU cells: against the reference on
+    # every cell (planes in absolute coordinates, as geogram forms them), and
+    # against the oracle in the seed-centred frame on the sample.
+    cpu, errors = None, {}
     if args.cpu_sample > 0:
         import oracle
 
+        rh = {"vol": r.vol.cpu().numpy(), "mg": r.mg.cpu().numpy(),
+              "ptr": r.nbr_ptr.cpu().numpy().astype(np.int64), "nbr": r.nbr.cpu().numpy(),
+              "farea": r.farea.cpu().numpy()}
+        threads = min(16, os.cpu_count() or 1)
+        if oracle.ref_available():
+            ref = oracle.ref_cells(x, threads=threads)
+            tc = ref["t_delaunay_s"] + ref["t_cells_s"]
+            cpu = {"value": round(n / tc, 1), "unit": "cells/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"the whole workload ({n} cells) through geogram's fastest CPU path "
+                             f"(PeriodicDelaunay3d + VBW::ConvexCell + 6 box clips, compute_RVD "
+                             f"cube mode; {ref['t_delaunay_s']:.2f} s Delaunay + "
+                             f"{ref['t_cells_s']:.2f} s cells on {threads} threads)"}
+            errors["vs_reference"] = cell_errors(np, rh, ref, np.arange(n))
         sample = np.random.default_rng(0).choice(n, min(args.cpu_sample, n), replace=False)
         sample.sort()
+        oracle.set_threads(threads)
         oracle.cells(x[:1000], k0=20)  # load
         t1 = time.perf_counter()
-        oracle.cells(x, k0=20, subset=sample)
-        tc = time.perf_counter() - t1
-        cpu = {
-            "value": round(len(sample) / tc, 1),
-            "unit": "cells/s",
-            "cores": oracle.get_threads(),
-            "kind": "port",
-            "sample": f"{len(sample)} random cells of the same 2M workload, kd-tree over all "
-                      f"2M seeds included ({tc:.2f} s)",
-        }
+        oabs = oracle.cells(x, k0=20, subset=sample.astype(np.int32))
+        to = time.perf_counter() - t1
+        port = {"value": round(len(sample) / to, 1), "unit": "cells/s", "cores": threads,
+                "kind": "port",
+                "sample": f"{len(sample)} random cells of the same 2M workload, kd-tree over all "
+                          f"2M seeds included ({to:.2f} s)"}
+        if cpu is None:
+            cpu = port
+        else:
+            cpu["port"] = port
+        ocen = oracle.cells(x, k0=20, subset=sample.astype(np.int32), centred=True)
+        errors["vs_oracle_absolute_frame"] = cell_errors(np, rh, oabs, sample)
+        errors["vs_oracle_centred_frame"] = cell_errors(np, rh, ocen, sample)
 
     out = {
         "metric": "Voronoi cells/sec; 1M-seed OT time to 1e-4 mass err (learned warm start vs plain)",
@@ -279,6 +383,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "errors": errors or None,
         "ot": ot,
         "ot_warm_start": ws,
         "batched": c5,

@@ -142,6 +142,11 @@ struct ClipParams {
                             // seen from outside
     int* mesh_fnbr;         // per facet: neighbour (caller's index), -1 domain
     int* mesh_tet;          // 4 per tetrahedron (caller's indices)
+    // periodic domains: seeds with a caller index >= n_real are images
+    // (ghosts) of seeds of the period, candidates only (no cell of their
+    // own); src maps every caller index to its seed (null: identity)
+    int n_real;
+    const int* src;
 };
 
 template <int S>
@@ -1372,6 +1377,11 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     // a pass that only asks whether an instance has an empty cell stops that
     // instance at its first one
     if (P.abort_flag && flag_load(P.abort_flag + inst)) return;
+    if (P.n_real > 0 && P.orig[c] >= P.n_real) {
+        // an image of a seed across the period: a candidate only
+        if (lane0()) P.flags[c] = CELL_DONE;
+        return;
+    }
     const pt4 pc = P.pts[c];
     const double* box = P.box + 6 * inst;
 
@@ -1684,7 +1694,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LI oj = LI(0);
 #if defined(VGC_EMU)
             for (int l = 0; l < kWave; ++l)
-                if (outm[l]) oj[l] = P.orig[gj[l]];
+                if (outm[l]) oj[l] = P.src ? P.src[P.orig[gj[l]]] : P.orig[gj[l]];
             for (int l = 0; l < kWave; ++l)
                 if (outm[l]) {
                     P.nbr_tmp[obase + rank[l]] = oj[l];
@@ -1692,7 +1702,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 }
 #else
             if (outm) {
-                oj = P.orig[gj];
+                oj = P.src ? P.src[P.orig[gj]] : P.orig[gj];
                 P.nbr_tmp[obase + rank] = oj;
                 P.area_tmp[obase + rank] = area;
             }

@@ -267,6 +267,9 @@ def ref_cells(xyz, w=None, box=UNIT_BOX, threads=0, stride=128):
     t = np.zeros(2)
     err = ref_lib().georef_delaunay_cells(_dp(xyz), wp, n, _dp(box), int(threads), _dp(vol),
                                           _dp(mg), _ip(cnt), _ip(nbr), _dp(fa), stride, _dp(t))
+    if err == -3:
+        raise ValueError("georef_delaunay_cells: a weighted seed has an empty cell "
+                         "(geogram's triangulation stops)")
     if err:
         raise RuntimeError(f"georef_delaunay_cells failed ({err})")
     return dict(vol=vol, mg=mg, cnt=cnt, nbr=nbr, farea=fa, t_delaunay_s=float(t[0]),

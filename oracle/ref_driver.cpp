@@ -15,6 +15,7 @@
 //    get_nearest_neighbors of each point by index (the point included).
 #include "Delaunay_psm.h"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -47,18 +48,48 @@ extern "C" {
 // mg[3n] (integral of x), cnt[n] and nbr / farea [n * stride] (contributing
 // neighbours, NO_INDEX box planes left out).  t[0]: Delaunay seconds,
 // t[1]: cells seconds.  Returns 0, -1 if a cell has more than stride
-// neighbours.
+// neighbours, -3 if a weighted seed has an empty cell (geogram's
+// triangulation stops there).
 int georef_delaunay_cells(const double* xyz, const double* w, int n, const double* box,
                           int nthreads, double* vol, double* mg, int* cnt, int* nbr,
                           double* farea, int stride, double* t) {
     init(nthreads);
     auto t0 = std::chrono::steady_clock::now();
     PeriodicDelaunay3d D(false);
-    D.set_keeps_infinite(true);
-    D.set_vertices(index_t(n), xyz);
-    if (w) D.set_weights(w);
+    // Unweighted: the vertex at infinity bounds the cells of hull seeds, as
+    // in compute_RVD's cube mode.  Weighted: update_v_to_cell with the
+    // vertex at infinity indexes its table with the marker of deleted
+    // tetrahedra (delaunay/periodic_delaunay_3d.cpp:3451-3458), so the
+    // hull cells are bounded instead by 8 guard seeds of weight 0 at the
+    // corners of a cube 10 times the box around it: their power planes stay
+    // 5 box sizes away from every seed, outside the box, and their cells
+    // are not reported.
+    std::vector<double> gx, gw;
+    index_t nv = index_t(n);
+    if (w) {
+        const double cx = 0.5 * (box[0] + box[3]), cy = 0.5 * (box[1] + box[4]),
+                     cz = 0.5 * (box[2] + box[5]);
+        const double R = 10.0 * std::max(box[3] - box[0], std::max(box[4] - box[1], box[5] - box[2]));
+        gx.assign(xyz, xyz + 3 * size_t(n));
+        gw.assign(w, w + n);
+        for (int k = 0; k < 8; ++k) {
+            gx.push_back(cx + ((k & 1) ? R : -R));
+            gx.push_back(cy + ((k & 2) ? R : -R));
+            gx.push_back(cz + ((k & 4) ? R : -R));
+            gw.push_back(0.0);
+        }
+        nv += 8;
+        D.set_vertices(nv, gx.data());
+        D.set_weights(gw.data());
+    } else {
+        D.set_keeps_infinite(true);
+        D.set_vertices(nv, xyz);
+    }
     D.compute();
     t[0] = seconds_since(t0);
+    // a weighted seed without a cell stops the triangulation
+    // (periodic_delaunay_3d.cpp:3165-3167): the reference has no cells then
+    if (D.has_empty_cells()) return -3;
     auto t1 = std::chrono::steady_clock::now();
     std::atomic<int> err(0);
     parallel_for_slice(0, index_t(n), [&](index_t b, index_t e) {
@@ -66,6 +97,13 @@ int georef_delaunay_cells(const double* xyz, const double* w, int n, const doubl
         PeriodicDelaunay3d::IncidentTetrahedra W;
         for (index_t v = b; v < e; ++v) {
             D.copy_Laguerre_cell_from_Delaunay(v, C, W);
+            cnt[v] = 0;
+            if (C.empty()) {
+                // a hidden vertex of the regular triangulation: empty cell
+                vol[v] = 0.0;
+                mg[3 * v] = mg[3 * v + 1] = mg[3 * v + 2] = 0.0;
+                continue;
+            }
             // box planes with the "no seed" global index (clip_by_plane(P)
             // alone would leave the slot's index of an earlier cell)
             const VBW::global_index_t none = VBW::global_index_t(-1);
@@ -92,7 +130,7 @@ int georef_delaunay_cells(const double* xyz, const double* w, int n, const doubl
             for (index_t lv = 1; lv < C.nb_v(); ++lv) {
                 if (!C.vertex_is_contributing(lv)) continue;
                 const VBW::global_index_t j = C.v_global_index(lv);
-                if (j == VBW::global_index_t(-1)) continue;
+                if (j == VBW::global_index_t(-1) || j >= VBW::global_index_t(n)) continue;
                 if (cnt[v] >= stride) {
                     err = -1;
                     break;
