@@ -115,13 +115,15 @@ class Context:
 
     # ---------------------------------------------------------------- cells
     def cells(self, points, weights=None, box=UNIT_BOX, k0=0, neighbors=True,
-              inst_n=None, boxes=None, count_work=False, autotune=False, waves4=False):
+              inst_n=None, boxes=None, count_work=False, autotune=False, waves4=False,
+              periodic=False):
         """Voronoi (weights None) or Laguerre cells clipped to box.
 
         For a batch of instances give inst_n (seed counts, consecutive in
         points) and boxes (6 per instance).  autotune: pick the first pass's
         work-group size by timing (VGC_AUTOTUNE; results do not depend on it).
-        waves4: run the first pass with 4-wavefront work-groups (VGC_WAVES4)."""
+        waves4: run the first pass with 4-wavefront work-groups (VGC_WAVES4).
+        periodic: the box is one period of a torus (VGC_PERIODIC)."""
         dev = _is_torch_cuda(points)
         self._stream(dev)
         if dev:
@@ -140,7 +142,8 @@ class Context:
             mg = np.empty((n, 3))
             ptr = np.empty(n + 1, dtype=np.uint64)
         flags = ((VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
-                 | (_lib.VGC_AUTOTUNE if autotune else 0) | (_lib.VGC_WAVES4 if waves4 else 0))
+                 | (_lib.VGC_AUTOTUNE if autotune else 0) | (_lib.VGC_WAVES4 if waves4 else 0)
+                 | (_lib.VGC_PERIODIC if periodic else 0))
         st = Stats()
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)
@@ -283,7 +286,7 @@ class Context:
 
     # ------------------------------------------------------------------- OT
     def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                 judge_start=False, verbose=False, inexact=False):
+                 judge_start=False, verbose=False, inexact=False, periodic=False):
         """Laguerre weights w with cell masses nu (uniform density on box).
         Returns (w, stats); w0 is the start (None: zeros).  judge_start: keep
         w0 only if its mass residual beats the zero start's.  verbose: one
@@ -309,7 +312,8 @@ class Context:
                                      (VGC_DEVICE_PTRS if dev else 0)
                                      | (_lib.VGC_OT_JUDGE_START if judge_start else 0)
                                      | (_lib.VGC_OT_VERBOSE if verbose else 0)
-                                     | (_lib.VGC_OT_INEXACT if inexact else 0),
+                                     | (_lib.VGC_OT_INEXACT if inexact else 0)
+                                     | (_lib.VGC_PERIODIC if periodic else 0),
                                      ctypes.byref(st))
         self._check(rc)
         return w, st.as_dict()
@@ -440,7 +444,7 @@ class Context:
         self._check(rc, allow_partial=False)
 
     # ---------------------------------------------------------------- Lloyd
-    def cvt_func_grad(self, points, box=UNIT_BOX):
+    def cvt_func_grad(self, points, box=UNIT_BOX, periodic=False):
         """CVT energy f = sum_i integral over cell i of |x - p_i|^2 and its
         gradient g_i = 2 (m_i p_i - mg_i) (compute_CVT_func_grad_in_volume,
         RVD.cpp:792-899).  Returns (f, g); g is an (n, 3) array of the same
@@ -459,12 +463,14 @@ class Context:
         boxa = np.ascontiguousarray(box, dtype=np.float64)
         rc = self.lib.fn("cvt_func_grad")(self.ctx, _ptr(x), x.shape[0],
                                           boxa.ctypes.data_as(ctypes.c_void_p),
-                                          VGC_DEVICE_PTRS if dev else 0, ctypes.byref(f), _ptr(g),
+                                          (VGC_DEVICE_PTRS if dev else 0)
+                                          | (_lib.VGC_PERIODIC if periodic else 0),
+                                          ctypes.byref(f), _ptr(g),
                                           ctypes.byref(st))
         self._check(rc)
         return f.value, g
 
-    def cvt_newton(self, points, iters, m=7, box=UNIT_BOX):
+    def cvt_newton(self, points, iters, m=7, box=UNIT_BOX, periodic=False):
         """Quasi-Newton CVT (Newton_iterations, CVT.cpp:272-302): L-BFGS with m
         pairs and a More-Thuente line search on the CVT energy; returns the
         new points (same kind of array) and the optimiser's statistics."""
@@ -479,11 +485,13 @@ class Context:
         boxa = np.ascontiguousarray(box, dtype=np.float64)
         rc = self.lib.fn("cvt_newton")(self.ctx, _ptr(x), x.shape[0],
                                        boxa.ctypes.data_as(ctypes.c_void_p), int(iters), int(m),
-                                       VGC_DEVICE_PTRS if dev else 0, ctypes.byref(st))
+                                       (VGC_DEVICE_PTRS if dev else 0)
+                                       | (_lib.VGC_PERIODIC if periodic else 0), ctypes.byref(st))
         self._check(rc)
         return x, st.as_dict()
 
-    def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None, autotune=False):
+    def lloyd(self, points, iters, box=UNIT_BOX, k0=0, inst_n=None, boxes=None, autotune=False,
+              periodic=False):
         """iters Lloyd iterations; returns the new points (same kind of array).
         inst_n / boxes: several independent instances at once (consecutive
         seeds, one box each).  autotune: pick the first pass's work-group size
@@ -496,7 +504,8 @@ class Context:
         else:
             x = np.array(points, dtype=np.float64, copy=True).reshape(-1, 3)
         st = Stats()
-        flags = (VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_AUTOTUNE if autotune else 0)
+        flags = ((VGC_DEVICE_PTRS if dev else 0) | (_lib.VGC_AUTOTUNE if autotune else 0)
+                 | (_lib.VGC_PERIODIC if periodic else 0))
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)
             rc = self.lib.fn("lloyd")(self.ctx, _ptr(x), x.shape[0],

@@ -29,6 +29,7 @@ VGC_OT_VERBOSE = 16
 VGC_AUTOTUNE = 32
 VGC_OT_INEXACT = 64
 VGC_WAVES4 = 128
+VGC_PERIODIC = 256
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6
@@ -38,7 +39,7 @@ _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain n
 _STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
 _STAT_F64B = "t_clip_a_gpu_ms t_knn_gpu_ms".split()
 _STAT_U64B = ("a_cells a_tests a_live a_new a_tets a_facets nb_fail_capacity why_rings why_walk "
-              "why_narrow nb_regather tuned_waves").split()
+              "why_narrow nb_regather tuned_waves nb_periodic_retries").split()
 
 
 class Stats(ctypes.Structure):

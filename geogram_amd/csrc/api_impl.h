@@ -81,6 +81,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->why_narrow = e.why_narrow;
     s->nb_regather = e.nb_regather;
     s->tuned_waves = (uint64_t)e.tuned_waves;
+    s->nb_periodic_retries = e.nb_periodic_retries;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
@@ -124,6 +125,11 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
         }
         pb.n = (int)n;
         pb.k0 = (int)k0;
+        pb.periodic = (flags & VGC_PERIODIC) != 0;
+        if (pb.periodic && n_inst != 1) {
+            ctx->err = "vgc_cells: a periodic domain takes one instance";
+            return VGC_ERR_ARG;
+        }
         pb.want_neighbors = !(flags & VGC_NO_NEIGHBORS) && (nbr || farea);
         pb.count_work = (flags & VGC_COUNT_WORK) != 0;
         const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
@@ -208,6 +214,7 @@ int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* 
         pb.k0 = 0;
         pb.want_neighbors = false;
         pb.want_e2 = true;
+        pb.periodic = (flags & VGC_PERIODIC) != 0;
         pb.count_work = (flags & VGC_COUNT_WORK) != 0;
         const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
         B& be = ctx->be;
@@ -570,6 +577,7 @@ int api_cvt_newton(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint
             pb.k0 = 0;
             pb.want_neighbors = false;
             pb.want_e2 = true;
+            pb.periodic = (flags & VGC_PERIODIC) != 0;
             pb.d_xyz = x;
             EngineStats es;
             if (E.cells(pb, es)) throw std::runtime_error("vgc_cvt_newton: " + E.error);
@@ -644,6 +652,11 @@ int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n
         pb.n = (int)n;
         pb.box.assign(boxes, boxes + 6 * (size_t)n_inst);
         pb.k0 = (int)k0;
+        pb.periodic = (flags & VGC_PERIODIC) != 0;
+        if (pb.periodic && n_inst != 1) {
+            ctx->err = "vgc_lloyd: a periodic domain takes one instance";
+            return VGC_ERR_ARG;
+        }
         double* dx = xyz;
         if (!dev) {
             dx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
@@ -698,6 +711,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         pb.inst_n = {(int)n};
         pb.box.assign(box, box + 6);
         pb.k0 = 0;
+        pb.periodic = (flags & VGC_PERIODIC) != 0;
         const double *dx = xyz, *dnu = nu;
         double* dw = w;
         if (!dev) {

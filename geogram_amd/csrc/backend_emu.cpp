@@ -280,15 +280,31 @@ struct EmuBackend {
             }
     }
 
-    void lloyd_update(double* xyz, const double* vol, const double* mg, int n) {
+    void lloyd_update(double* xyz, const double* vol, const double* mg, int n,
+                      const double* wrap = nullptr) {
+        for (int i = 0; i < n; ++i) lloyd_update_row(i, xyz, vol, mg, wrap);
+    }
+    void ghost_count(const double* xyz, int n, const double* box, double delta, int* cnt) {
+        for (int i = 0; i < n; ++i)
+            cnt[i] = periodic_images(xyz, nullptr, i, box, delta, nullptr, nullptr, nullptr);
+    }
+    void ghost_fill(const double* xyz, const double* w, int n, const double* box, double delta,
+                    const long long* off, double* xe, double* we, int* src) {
         for (int i = 0; i < n; ++i) {
-            if (vol[i] > 1e-30) {
-                double s = 1.0 / vol[i];
-                xyz[3 * (size_t)i + 0] = s * mg[3 * (size_t)i + 0];
-                xyz[3 * (size_t)i + 1] = s * mg[3 * (size_t)i + 1];
-                xyz[3 * (size_t)i + 2] = s * mg[3 * (size_t)i + 2];
-            }
+            for (int a = 0; a < 3; ++a) xe[3 * (long long)i + a] = xyz[3 * (long long)i + a];
+            if (we) we[i] = w[i];
+            src[i] = i;
+            const long long o = n + off[i];
+            periodic_images(xyz, w, i, box, delta, xe + 3 * o, we ? we + o : nullptr, src + o);
         }
+    }
+    void periodic_reach(const pt4* pts, const int* orig, int ne, int n_real, const double* r2,
+                        const double* wmax, float* out) {
+        float v = 0.0f;
+        for (int c = 0; c < ne; ++c)
+            if (orig[c] < n_real)
+                v = std::max(v, radius_for(r2[c], pts[c].w, wmax ? wmax[0] : 0.0, wmax ? 1 : 0));
+        *out = v;
     }
 
     // ----------------------------------------------------------------- OT
@@ -361,8 +377,8 @@ struct EmuBackend {
         *out = mn;
     }
     void ot_hessian(int n, const double* xyz, const long long* ptr, const int* col,
-                    const double* area, double* val, double* diag) {
-        for (int i = 0; i < n; ++i) ot_hessian_row(i, xyz, ptr, col, area, val, diag);
+                    const double* area, double* val, double* diag, const double* period = nullptr) {
+        for (int i = 0; i < n; ++i) ot_hessian_row(i, xyz, ptr, col, area, val, diag, period);
     }
     void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,
                  const double* x, double* y) {

@@ -220,16 +220,44 @@ __global__ void k_compact(int n, const int* __restrict__ cnt, const long long* _
 }
 
 __global__ void k_lloyd_update(double* xyz, const double* __restrict__ vol,
-                               const double* __restrict__ mg, int n) {
+                               const double* __restrict__ mg, int n,
+                               const double* __restrict__ wrap) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double m = vol[i];
-    if (m > 1e-30) {
-        double s = 1.0 / m;
-        xyz[3 * (size_t)i + 0] = s * mg[3 * (size_t)i + 0];
-        xyz[3 * (size_t)i + 1] = s * mg[3 * (size_t)i + 1];
-        xyz[3 * (size_t)i + 2] = s * mg[3 * (size_t)i + 2];
-    }
+    lloyd_update_row(i, xyz, vol, mg, wrap);
+}
+
+__global__ void k_ghost_count(const double* __restrict__ xyz, int n, const double* __restrict__ box,
+                              double delta, int* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = periodic_images(xyz, nullptr, i, box, delta, nullptr, nullptr, nullptr);
+}
+
+// seeds of the period first (caller order), then the images of seed i at
+// n + off[i]
+__global__ void k_ghost_fill(const double* __restrict__ xyz, const double* __restrict__ w, int n,
+                             const double* __restrict__ box, double delta,
+                             const long long* __restrict__ off, double* xe, double* we, int* src) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int a = 0; a < 3; ++a) xe[3 * (long long)i + a] = xyz[3 * (long long)i + a];
+    if (we) we[i] = w[i];
+    src[i] = i;
+    const long long o = n + off[i];
+    periodic_images(xyz, w, i, box, delta, xe + 3 * o, we ? we + o : nullptr, src + o);
+}
+
+// largest squared radius (FP32, rounded up) within which a seed could still
+// cut a cell of the period (radius_for of its squared radius r2)
+__global__ void k_periodic_reach(const pt4* __restrict__ pts, const int* __restrict__ orig, int ne,
+                                 int n_real, const double* __restrict__ r2,
+                                 const double* __restrict__ wmax, float* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    float v = 0.0f;
+    if (c < ne && orig[c] < n_real)
+        v = radius_for(r2[c], pts[c].w, wmax ? wmax[0] : 0.0, wmax ? 1 : 0);
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+    if (__lane_id() == 0 && v > 0.0f) atomicMax((unsigned*)out, __float_as_uint(v));
 }
 
 __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restrict__ list,
@@ -441,9 +469,9 @@ __global__ __launch_bounds__(256) void k_mass_final(const double* __restrict__ p
 
 __global__ void k_ot_hessian(int n, const double* __restrict__ xyz, const long long* __restrict__ ptr,
                              const int* __restrict__ col, const double* __restrict__ area,
-                             double* val, double* diag) {
+                             double* val, double* diag, const double* __restrict__ period) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) ot_hessian_row(i, xyz, ptr, col, area, val, diag);
+    if (i < n) ot_hessian_row(i, xyz, ptr, col, area, val, diag, period);
 }
 
 __global__ void k_ot_spmv(int n, const long long* __restrict__ ptr, const int* __restrict__ col,
@@ -1305,8 +1333,26 @@ struct HipBackend {
             n, cnt, ptr, nbr_tmp, area_tmp, stride, nbr, area);
         check();
     }
-    void lloyd_update(double* xyz, const double* vol, const double* mg, int n) {
-        k_lloyd_update<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, vol, mg, n);
+    void lloyd_update(double* xyz, const double* vol, const double* mg, int n,
+                      const double* wrap = nullptr) {
+        k_lloyd_update<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, vol, mg, n, wrap);
+        check();
+    }
+    void ghost_count(const double* xyz, int n, const double* box, double delta, int* cnt) {
+        k_ghost_count<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, n, box, delta, cnt);
+        check();
+    }
+    void ghost_fill(const double* xyz, const double* w, int n, const double* box, double delta,
+                    const long long* off, double* xe, double* we, int* src) {
+        k_ghost_fill<<<blocks_for(n, 256), 256, 0, stream_>>>(xyz, w, n, box, delta, off, xe, we,
+                                                              src);
+        check();
+    }
+    void periodic_reach(const pt4* pts, const int* orig, int ne, int n_real, const double* r2,
+                        const double* wmax, float* out) {
+        VGC_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(float), stream_));
+        k_periodic_reach<<<blocks_for(ne, 256), 256, 0, stream_>>>(pts, orig, ne, n_real, r2, wmax,
+                                                                   out);
         check();
     }
     // ----------------------------------------------------------------- OT
@@ -1356,8 +1402,9 @@ struct HipBackend {
         check();
     }
     void ot_hessian(int n, const double* xyz, const long long* ptr, const int* col,
-                    const double* area, double* val, double* diag) {
-        k_ot_hessian<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, ptr, col, area, val, diag);
+                    const double* area, double* val, double* diag, const double* period = nullptr) {
+        k_ot_hessian<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, ptr, col, area, val, diag,
+                                                              period);
         check();
     }
     void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,

@@ -251,6 +251,58 @@ VGC_SD float knn_grow(float r32) {
     return g > 1e-30f ? g : 1e-30f;
 }
 
+// Lloyd update of seed i (CentroidalVoronoiTesselation::Lloyd_iterations,
+// voronoi/CVT.cpp:154-161): the centroid mg / m when m > 1e-30.  wrap: a
+// periodic box; the centroid is brought back into [lo, hi).
+VGC_SD void lloyd_update_row(int i, double* xyz, const double* vol, const double* mg,
+                             const double* wrap) {
+    const double m = vol[i];
+    if (!(m > 1e-30)) return;
+    const double s = 1.0 / m;
+    for (int a = 0; a < 3; ++a) {
+        double c = s * mg[3 * (long long)i + a];
+        if (wrap) {
+            const double lo = wrap[a], hi = wrap[3 + a], L = hi - lo;
+            c -= L * floor((c - lo) / L);
+            if (c >= hi) c -= L;
+            if (c < lo) c = lo;
+        }
+        xyz[3 * (long long)i + a] = c;
+    }
+}
+
+// ------------------------------------------------------ periodic images
+// Images of seed i for a periodic box: every shift s in {-1, 0, 1}^3, s != 0,
+// whose image x + s L lies within delta of the box (per axis
+// lo - delta <= x_a + s_a L_a <= hi + delta), in the order sx, sy, sz
+// (outer to inner).  Count mode when out == nullptr.
+VGC_SD int periodic_images(const double* __restrict__ xyz, const double* __restrict__ w, int i,
+                           const double* __restrict__ box, double delta, double* out_xyz,
+                           double* out_w, int* out_src) {
+    const double x[3] = {xyz[3 * (long long)i], xyz[3 * (long long)i + 1], xyz[3 * (long long)i + 2]};
+    int cnt = 0;
+    for (int sx = -1; sx <= 1; ++sx)
+        for (int sy = -1; sy <= 1; ++sy)
+            for (int sz = -1; sz <= 1; ++sz) {
+                if (sx == 0 && sy == 0 && sz == 0) continue;
+                const int sh[3] = {sx, sy, sz};
+                double y[3];
+                bool in = true;
+                for (int a = 0; a < 3; ++a) {
+                    y[a] = x[a] + double(sh[a]) * (box[3 + a] - box[a]);
+                    in = in && y[a] >= box[a] - delta && y[a] <= box[3 + a] + delta;
+                }
+                if (!in) continue;
+                if (out_xyz) {
+                    for (int a = 0; a < 3; ++a) out_xyz[3 * (long long)cnt + a] = y[a];
+                    if (out_w) out_w[cnt] = w[i];
+                    out_src[cnt] = i;
+                }
+                ++cnt;
+            }
+    return cnt;
+}
+
 // ------------------------------------------------- power-nearest queries
 // Seed of instance 0 nearest to x (FP64) in power distance |x - p|^2 - w
 // (ties: the smaller Morton position); returns its Morton position.  Tree

@@ -20,16 +20,26 @@ namespace vgc {
 
 // Row i of the Hessian from the cells' neighbour CSR (caller numbering):
 // off-diagonal values into val[ptr[i]..ptr[i+1]) and the diagonal.
+// period: the periods of a periodic box (null otherwise); the distance to a
+// neighbour is then the one to its nearest image.
+VGC_SD double min_image(double e, double L) { return e - L * rint(e / L); }
+
 VGC_SD void ot_hessian_row(int i, const double* __restrict__ xyz, const long long* __restrict__ ptr,
                            const int* __restrict__ col, const double* __restrict__ area,
-                           double* __restrict__ val, double* __restrict__ diag) {
+                           double* __restrict__ val, double* __restrict__ diag,
+                           const double* __restrict__ period) {
     const double px = xyz[3 * (long long)i], py = xyz[3 * (long long)i + 1],
                  pz = xyz[3 * (long long)i + 2];
     double d = 0.0;
     for (long long k = ptr[i]; k < ptr[i + 1]; ++k) {
         const int j = col[k];
-        const double ex = xyz[3 * (long long)j] - px, ey = xyz[3 * (long long)j + 1] - py,
-                     ez = xyz[3 * (long long)j + 2] - pz;
+        double ex = xyz[3 * (long long)j] - px, ey = xyz[3 * (long long)j + 1] - py,
+               ez = xyz[3 * (long long)j + 2] - pz;
+        if (period) {
+            ex = min_image(ex, period[0]);
+            ey = min_image(ey, period[1]);
+            ez = min_image(ez, period[2]);
+        }
         const double h = -area[k] / (2.0 * sqrt(ex * ex + ey * ey + ez * ez));
         val[k] = h;
         d -= h;

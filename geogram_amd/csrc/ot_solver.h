@@ -133,7 +133,14 @@ public:
             auto tg = std::chrono::steady_clock::now();
             const long long nnz = E_.total_neighbors;
             double* d_val = (double*)slot(9, sizeof(double) * std::max<long long>(nnz, 1));
-            be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag);
+            double* d_per = nullptr;
+            if (pb.periodic) {
+                const double per[3] = {pb.box[3] - pb.box[0], pb.box[4] - pb.box[1],
+                                       pb.box[5] - pb.box[2]};
+                d_per = (double*)slot(29, sizeof(double) * 3);
+                be_.h2d(d_per, per, sizeof(double) * 3);
+            }
+            be_.ot_hessian(n, pb.d_xyz, E_.d_ptr, E_.d_nbr, E_.d_area, d_val, d_diag, d_per);
             // Newton system H d = nu - m, projected on the range of H (the
             // masses and nu can differ in sum by rounding; the constants are
             // H's kernel, and CG on an inconsistent system stalls)

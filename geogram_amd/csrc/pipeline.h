@@ -49,6 +49,7 @@ struct EngineStats {
     unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
     unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
     unsigned long long nb_regather = 0;  // weighted cells gathered again without the ring limit
+    unsigned long long nb_periodic_retries = 0;  // periodic passes redone with a wider margin
     int tuned_waves = 0;                 // autotuner's choice (0: not tuned by this call)
 };
 
@@ -69,6 +70,13 @@ struct Problem {
     // per instance, nonzero = compute its cells; the others keep the outputs
     // (volumes, centroids, neighbours) of the last pass that computed them
     const std::vector<int>* active = nullptr;
+    // periodic domain (one instance): the box is the period; cells are
+    // computed with images of the seeds near the faces as extra candidates
+    bool periodic = false;
+    // internal: the first n_real seeds are the period's, src maps every
+    // seed (images included) to its seed of the period
+    int n_real = 0;
+    const int* src = nullptr;
     // cell meshes / primal tetrahedra (ClipParams::mesh_*), one instance
     int mesh_mode = 0;
     double mesh_shrink = 0.0;
@@ -362,6 +370,80 @@ public:
 
     // ------------------------------------------------------------ cells
     int cells(const Problem& pb, EngineStats& st) {
+        if (pb.periodic) return cells_periodic(pb, st);
+        return cells_core(pb, st);
+    }
+
+    // Periodic cells (the torus of the box; PeriodicDelaunay3d with
+    // periodic = true, delaunay/periodic_delaunay_3d.cpp:4017-4320, uses 27
+    // translated copies): images of the seeds within a margin delta of the
+    // faces join as candidates, the cells are clipped in the box grown by
+    // delta, and a cell counts only if everything that could still cut it
+    // (the radius of its security test) lies within the margin; otherwise
+    // the margin doubles.  Outputs as cells(): the first n entries; neighbour
+    // ids are seeds of the period (an image's seed).
+    int cells_periodic(const Problem& pb, EngineStats& st) {
+        if (pb.n_inst != 1 || pb.mesh_mode) {
+            error = "periodic: one instance, no mesh export";
+            return -1;
+        }
+        const long long n = pb.n;
+        const double* b = pb.box.data();
+        const double Lx = b[3] - b[0], Ly = b[4] - b[1], Lz = b[5] - b[2];
+        const double Lmax = std::max(Lx, std::max(Ly, Lz));
+        double delta = std::min(Lmax, 3.0 * std::cbrt(Lx * Ly * Lz / double(n)));
+        for (int round = 0;; ++round) {
+            // delta = Lmax keeps all 26 images of every seed.  That set is
+            // enough for any cell: a cell lies within half a period of its
+            // seed on each axis (its own images bound it), and for a point
+            // v there the nearest image of any seed lies within half a
+            // period of v on each axis, i.e. among the images one period
+            // away; farther images cut nothing the nearest one does not.
+            const bool full = delta >= Lmax;
+            int* gcnt = (int*)slot(S_GCNT, sizeof(int) * (size_t)n);
+            long long* goff = (long long*)slot(S_GOFF, sizeof(long long) * ((size_t)n + 1));
+            double* d_b = (double*)slot(S_GBOX, sizeof(double) * 6);
+            be_.h2d(d_b, b, sizeof(double) * 6);
+            be_.ghost_count(pb.d_xyz, (int)n, d_b, delta, gcnt);
+            be_.scan_i32(gcnt, goff, (int)n);
+            long long ng = 0;
+            be_.d2h(&ng, goff + n, sizeof(long long));
+            const long long ne = n + ng;
+            if (ne > (1ll << 30)) {
+                error = "periodic: too many images";
+                return -1;
+            }
+            double* xe = (double*)slot(S_GXYZ, sizeof(double) * 3 * (size_t)ne);
+            double* we = pb.d_w ? (double*)slot(S_GW, sizeof(double) * (size_t)ne) : nullptr;
+            int* src = (int*)slot(S_GSRC, sizeof(int) * (size_t)ne);
+            be_.ghost_fill(pb.d_xyz, pb.d_w, (int)n, d_b, delta, goff, xe, we, src);
+            Problem q = pb;
+            q.periodic = false;
+            q.n = (int)ne;
+            q.inst_n = {(int)ne};
+            q.box = {b[0] - delta, b[1] - delta, b[2] - delta, b[3] + delta, b[4] + delta,
+                     b[5] + delta};
+            q.d_xyz = xe;
+            q.d_w = we;
+            q.n_real = (int)n;
+            q.src = src;
+            const int rc = cells_core(q, st);
+            if (rc) return rc;  // error, or a transport trial stopped by an empty cell
+            // the reach of the cells: the radius within which a seed could
+            // still cut one (its security radius)
+            float* d_reach = (float*)slot(S_GREACH, sizeof(float));
+            be_.periodic_reach(d_pts, d_orig, (int)ne, (int)n, d_r2_, weighted_ ? d_wmax_ : nullptr,
+                               d_reach);
+            float reach2 = 0.0f;
+            be_.d2h(&reach2, d_reach, sizeof(float));
+            if (full || double(reach2) < delta * delta) break;
+            delta = std::min(Lmax, 2.0 * std::max(delta, std::sqrt(double(reach2))));
+            st.nb_periodic_retries++;
+        }
+        return 0;
+    }
+
+    int cells_core(const Problem& pb, EngineStats& st) {
         auto tstart = now();
         if (build(pb, st)) return -1;
         const int K0 = pb.k0;
@@ -430,6 +512,9 @@ public:
         P.mesh_fv = pb.mesh_fv;
         P.mesh_fnbr = pb.mesh_fnbr;
         P.mesh_tet = pb.mesh_tet;
+        P.n_real = pb.n_real;
+        P.src = pb.src;
+        d_r2_ = d_r2;
 
         // pass A over every cell, then the tiers for the flagged ones
         auto t2 = now();
@@ -724,9 +809,14 @@ public:
     int lloyd(Problem pb, double* d_xyz, int iters, EngineStats& st) {
         pb.d_xyz = d_xyz;
         pb.want_neighbors = false;
+        double* wrap = nullptr;
+        if (pb.periodic) {
+            wrap = (double*)slot(S_GWRAP, sizeof(double) * 6);
+            be_.h2d(wrap, pb.box.data(), sizeof(double) * 6);
+        }
         for (int it = 0; it < iters; ++it) {
             if (cells(pb, st)) return -1;
-            be_.lloyd_update(d_xyz, d_vol, d_mg, n_);
+            be_.lloyd_update(d_xyz, d_vol, d_mg, pb.n, wrap);
         }
         be_.sync();
         return 0;
@@ -742,7 +832,8 @@ private:
         S_ITEMS, S_RHO2, S_RCNT, S_ROFF, S_RLIST, S_RD2, S_RLIST2, S_RD22, S_RLEN, S_BALL,
         S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2,
         S_QKEYS, S_QKEYS2, S_QVALS, S_QVALS2, S_QPTS, S_QPF, S_QROW, S_QPOS, S_QR2, S_QREDO,
-        S_QLI2, S_QLD, S_QLD2, S_MCNT, S_MOFF, S_COUNT
+        S_QLI2, S_QLD, S_QLD2, S_MCNT, S_MOFF, S_GCNT, S_GOFF, S_GBOX, S_GXYZ, S_GW, S_GSRC,
+        S_GREACH, S_GWRAP, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
@@ -871,6 +962,7 @@ private:
     int tuned_n_ = -1;
     double* d_box_ = nullptr;
     double* d_wmax_ = nullptr;
+    double* d_r2_ = nullptr;  // squared radius per cell of the last pass (Morton order)
     std::vector<int> h_p0_, h_n_, h_b0_, h_nb_, h_L_;
     std::vector<long long> h_t0_, h_leaf_off_;
 };

@@ -54,6 +54,17 @@ extern "C" {
                                  max relative mass error), never below the default
                                  1e-7; damped steps keep 1e-7 (Eisenstat-Walker
                                  forcing) */
+#define VGC_PERIODIC 256u     /* periodic domain: the box is one period of a
+                                 3-torus (seeds inside it); cells, neighbour ids
+                                 and facet areas are those of the periodic
+                                 Voronoi / Laguerre diagram, a neighbour being
+                                 named by its seed (an image's seed).  Replaces
+                                 PeriodicDelaunay3d(periodic = true)
+                                 (delaunay/periodic_delaunay_3d.cpp:4017-4320)
+                                 for vgc_cells, vgc_lloyd (centroids wrapped
+                                 into the box), vgc_cvt_func_grad,
+                                 vgc_cvt_newton and vgc_ot_solve; one
+                                 instance */
 #define VGC_WAVES4 128u       /* run the first clipping pass with 4-wavefront
                                  work-groups instead of the default 1 (identical
                                  results; for tests and tuning) */
@@ -88,6 +99,8 @@ typedef struct vgc_stats {
     uint64_t why_rings, why_walk, why_narrow;
     uint64_t nb_regather; /* weighted cells gathered again without the ring limit */
     uint64_t tuned_waves; /* VGC_AUTOTUNE: wavefronts per work-group chosen (0: none) */
+    uint64_t nb_periodic_retries; /* VGC_PERIODIC: passes redone with a wider image
+                                     margin */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in

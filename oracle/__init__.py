@@ -194,6 +194,26 @@ def cell_meshes(xyz, w=None, box=UNIT_BOX, subset=None, centred=True, istride=20
     return out
 
 
+def periodic_cells(xyz, w=None, box=UNIT_BOX, centred=True, stride=128, e2=False):
+    """Cells of the 3-torus of box (one period): the seeds and their 26
+    translated copies x + s L (s in {-1, 0, 1}^3), the security-radius loop of
+    every seed of the period over all 27 n, neighbour ids taken modulo n
+    (PeriodicDelaunay3d's periodic_vertex_real, periodic.h:85-88)."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    box = np.asarray(box, dtype=np.float64)
+    L = box[3:] - box[:3]
+    shifts = [(0, 0, 0)] + [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)
+                            if (a, b, c) != (0, 0, 0)]
+    X = np.vstack([x + np.array(s, dtype=np.float64) * L for s in shifts])
+    W = None if w is None else np.tile(np.asarray(w, dtype=np.float64), len(shifts))
+    big = np.concatenate([box[:3] - L, box[3:] + L])
+    o = cells(X, w=W, box=big, subset=np.arange(n, dtype=np.int32), centred=centred,
+              stride=stride, e2=e2)
+    o["nbr"] = np.where(o["nbr"] >= 0, o["nbr"] % n, -1)
+    return o
+
+
 def lloyd(xyz, iters, box=UNIT_BOX, k0=20):
     x = np.array(xyz, dtype=np.float64, copy=True).reshape(-1, 3)
     box = np.ascontiguousarray(box, dtype=np.float64)
@@ -244,14 +264,16 @@ def ref_lib():
         L.georef_delaunay_cells.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int, dp, dp, ip,
                                             ip, dp, ctypes.c_int, dp]
         L.georef_knn.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.georef_periodic_cells.argtypes = L.georef_delaunay_cells.argtypes
         L.georef_max_threads.restype = ctypes.c_int
         _ref = L
     return _ref
 
 
-def ref_cells(xyz, w=None, box=UNIT_BOX, threads=0, stride=128):
+def ref_cells(xyz, w=None, box=UNIT_BOX, threads=0, stride=128, periodic=False):
     """geogram's fastest CPU path (PeriodicDelaunay3d + ConvexCell + box
-    clips): dict(vol, mg, cnt, nbr, farea, t_delaunay_s, t_cells_s)."""
+    clips; periodic: PeriodicDelaunay3d on the torus of the box):
+    dict(vol, mg, cnt, nbr, farea, t_delaunay_s, t_cells_s)."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     n = xyz.shape[0]
     box = np.ascontiguousarray(box, dtype=np.float64)
@@ -265,8 +287,9 @@ def ref_cells(xyz, w=None, box=UNIT_BOX, threads=0, stride=128):
     nbr = np.full((n, stride), -1, dtype=np.int32)
     fa = np.zeros((n, stride))
     t = np.zeros(2)
-    err = ref_lib().georef_delaunay_cells(_dp(xyz), wp, n, _dp(box), int(threads), _dp(vol),
-                                          _dp(mg), _ip(cnt), _ip(nbr), _dp(fa), stride, _dp(t))
+    f = ref_lib().georef_periodic_cells if periodic else ref_lib().georef_delaunay_cells
+    err = f(_dp(xyz), wp, n, _dp(box), int(threads), _dp(vol), _dp(mg), _ip(cnt), _ip(nbr),
+            _dp(fa), stride, _dp(t))
     if err == -3:
         raise ValueError("georef_delaunay_cells: a weighted seed has an empty cell "
                          "(geogram's triangulation stops)")

@@ -11,6 +11,9 @@
 //    the contributing planes (delaunay/periodic_delaunay_3d.cpp:3634-3733,
 //    voronoi/convex_cell.cpp:1160-1307; compute_RVD's cube mode,
 //    examples/geogram/compute_RVD/main.cpp:450-477), cells in parallel;
+//  georef_periodic_cells: the same over the 3-torus of the box,
+//    PeriodicDelaunay3d(period, true) and periodic_vertex_real
+//    (delaunay/periodic_delaunay_3d.h, delaunay/periodic.h);
 //  georef_knn: NearestNeighborSearch "BNN" (points/kd_tree.cpp),
 //    get_nearest_neighbors of each point by index (the point included).
 #include "Delaunay_psm.h"
@@ -136,6 +139,64 @@ int georef_delaunay_cells(const double* xyz, const double* w, int n, const doubl
                     break;
                 }
                 nbr[size_t(v) * stride + cnt[v]] = int(j);
+                farea[size_t(v) * stride + cnt[v]] = C.facet_area(lv);
+                ++cnt[v];
+            }
+        }
+    });
+    t[1] = seconds_since(t1);
+    return err;
+}
+
+// Periodic cells of the n seeds in the box (one period of a 3-torus):
+// PeriodicDelaunay3d(period, true) over the seeds translated to the origin,
+// copy_Laguerre_cell_from_Delaunay of each seed (no clip), the neighbours'
+// real indices (periodic_vertex_real).  Outputs as georef_delaunay_cells
+// (mg in the box's frame).
+int georef_periodic_cells(const double* xyz, const double* w, int n, const double* box,
+                          int nthreads, double* vol, double* mg, int* cnt, int* nbr,
+                          double* farea, int stride, double* t) {
+    init(nthreads);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> x0(3 * size_t(n));
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) x0[3 * size_t(i) + a] = xyz[3 * size_t(i) + a] - box[a];
+    PeriodicDelaunay3d D(vec3(box[3] - box[0], box[4] - box[1], box[5] - box[2]), true);
+    D.set_vertices(index_t(n), x0.data());
+    if (w) D.set_weights(w);
+    D.compute();
+    t[0] = seconds_since(t0);
+    if (D.has_empty_cells()) return -3;
+    auto t1 = std::chrono::steady_clock::now();
+    std::atomic<int> err(0);
+    parallel_for_slice(0, index_t(n), [&](index_t b, index_t e) {
+        VBW::ConvexCell C(VBW::WithVGlobal);
+        PeriodicDelaunay3d::IncidentTetrahedra W;
+        for (index_t v = b; v < e; ++v) {
+            D.copy_Laguerre_cell_from_Delaunay(v, C, W);
+            cnt[v] = 0;
+            if (C.empty()) {
+                vol[v] = 0.0;
+                mg[3 * v] = mg[3 * v + 1] = mg[3 * v + 2] = 0.0;
+                continue;
+            }
+            C.compute_geometry();
+            double m = 0.0;
+            vec3 g;
+            C.compute_mg(m, g);
+            vol[v] = m;
+            mg[3 * v] = g.x + m * box[0];
+            mg[3 * v + 1] = g.y + m * box[1];
+            mg[3 * v + 2] = g.z + m * box[2];
+            for (index_t lv = 1; lv < C.nb_v(); ++lv) {
+                if (!C.vertex_is_contributing(lv)) continue;
+                const VBW::global_index_t j = C.v_global_index(lv);
+                if (j == VBW::global_index_t(-1)) continue;
+                if (cnt[v] >= stride) {
+                    err = -1;
+                    break;
+                }
+                nbr[size_t(v) * stride + cnt[v]] = int(D.periodic_vertex_real(index_t(j)));
                 farea[size_t(v) * stride + cnt[v]] = C.facet_area(lv);
                 ++cnt[v];
             }
