@@ -155,6 +155,15 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="one 256 MiB device-to-device copy before the steps, the known-size "
                          "kernel that scales HBM counters under rocprofv3 --pmc")
+    ap.add_argument("--surface-level", type=int, default=7,
+                    help="surface leg: icosphere subdivision level (7: 327680 triangles; 0: skip)")
+    ap.add_argument("--surface-seeds", type=int, default=100_000)
+    ap.add_argument("--periodic-n", type=int, default=500_000,
+                    help="periodic leg: uniform seeds on the unit 3-torus (0: skip)")
+    ap.add_argument("--newton-iters", type=int, default=10,
+                    help="quasi-Newton CVT leg on the 100k CVT workload (0: skip)")
+    ap.add_argument("--mesh-n", type=int, default=200_000,
+                    help="cell-mesh / RDT export leg (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=400_000,
                     help="cells of the workload timed on the host with the oracle (0: skip)")
     args = ap.parse_args()
@@ -319,6 +328,85 @@ def main():
                         "cell_passes": int(s5[0]["cell_passes"]),
                         "t_cells_ms": round(s5[0]["t_cells_ms"], 1), "t_cg_ms": round(s5[0]["t_cg_ms"], 1)}
 
+    # the rows beside the cell pass, each timed on the GPU with its error
+    # against the oracle (or the reference build) on the same input
+    import oracle
+
+    def gpu_time(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t2 = time.perf_counter()
+            res = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t2)
+        return res, float(np.median(ts))
+
+    surf = None
+    if args.surface_level > 0:
+        from geogram_amd.configs import icosphere, sphere_seeds
+
+        V, F = icosphere(args.surface_level)
+        xs_ = sphere_seeds(args.surface_seeds, 13)
+        Vd, Fd, xsd = (torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda(),
+                       torch.from_numpy(xs_).cuda())
+        (g, mm, sst), ts = gpu_time(lambda: ctx.surface_centroids(xsd, Vd, Fd))
+        t2 = time.perf_counter()
+        mgo, mo, npoly = oracle.surface_centroids(xs_, V, F)
+        to_ = time.perf_counter() - t2
+        gh, mh = g.cpu().numpy(), mm.cpu().numpy()
+        surf = {"triangles": int(len(F)), "seeds": int(args.surface_seeds),
+                "gpu_ms": round(ts * 1e3, 3), "triangles_per_s": round(len(F) / ts, 1),
+                "polygons": int(npoly), "items": int(sst["nb_items"]),
+                "splits": int(sst["nb_split"]), "redo": int(sst["nb_redo"]),
+                "cpu_oracle_s": round(to_, 3), "cpu_oracle_cores": 1,
+                "max_rel_mass_err": float(np.abs(mh - mo).max() / mo.max()),
+                "max_rel_moment_err": float(np.abs(gh - mgo).max() / np.abs(mgo).max())}
+
+    per = None
+    if args.periodic_n > 0:
+        xp = ga.sample_points(0, 14, args.periodic_n)
+        xpd = torch.from_numpy(xp).cuda()
+        rp, tp_ = gpu_time(lambda: ctx.cells(xpd, periodic=True))
+        per = {"n": args.periodic_n, "gpu_ms": round(tp_ * 1e3, 3),
+               "cells_per_s": round(args.periodic_n / tp_, 1),
+               "retries": int(rp.stats["nb_periodic_retries"]),
+               "sum_vol_minus_1": abs(float(rp.vol.sum().item()) - 1.0)}
+        if oracle.ref_available():
+            threads = min(16, os.cpu_count() or 1)
+            refp = oracle.ref_cells(xp, threads=threads, periodic=True)
+            tref = refp["t_delaunay_s"] + refp["t_cells_s"]
+            vg = rp.vol.cpu().numpy()
+            per["cpu_reference"] = {"cells_per_s": round(args.periodic_n / tref, 1),
+                                    "cores": threads,
+                                    "what": "PeriodicDelaunay3d(periodic) + Laguerre cells"}
+            per["max_rel_vol_err_vs_reference"] = float(
+                (np.abs(vg - refp["vol"]) / refp["vol"]).max())
+
+    newton = None
+    if args.newton_iters > 0:
+        from geogram_amd.configs import cvt_points
+
+        xc = torch.from_numpy(cvt_points()).cuda()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        _, nst = ctx.cvt_newton(xc, args.newton_iters)
+        torch.cuda.synchronize()
+        tn = time.perf_counter() - t2
+        newton = {"n": int(xc.shape[0]), "iterations": int(nst["iterations"]),
+                  "evaluations": int(nst["evaluations"]), "time_s": round(tn, 3),
+                  "f0": nst["f0"], "f": nst["f"], "gnorm0": nst["gnorm0"], "gnorm": nst["gnorm"]}
+
+    meshleg = None
+    if args.mesh_n > 0:
+        xm = ga.sample_points(0, 15, args.mesh_n)
+        (tets), tr = gpu_time(lambda: ctx.rdt(xm), reps=2)
+        (cm), tc_ = gpu_time(lambda: ctx.cell_mesh(xm), reps=2)
+        meshleg = {"n": args.mesh_n, "rdt_ms": round(tr * 1e3, 3), "tets": int(len(tets)),
+                   "cell_mesh_ms": round(tc_ * 1e3, 3),
+                   "note": "host arrays in and out (PCIe included)"}
+
     # CPU baseline and per-output errors.  The reference itself (geogram's
     # Delaunay + ConvexCell path, oracle/_ref) runs the whole workload on the
     # host cores when it is built; the oracle (plain-C restatement) runs a
@@ -327,8 +415,6 @@ def main():
     # against the oracle in the seed-centred frame on the sample.
     cpu, errors = None, {}
     if args.cpu_sample > 0:
-        import oracle
-
         rh = {"vol": r.vol.cpu().numpy(), "mg": r.mg.cpu().numpy(),
               "ptr": r.nbr_ptr.cpu().numpy().astype(np.int64), "nbr": r.nbr.cpu().numpy(),
               "farea": r.farea.cpu().numpy()}
@@ -387,6 +473,10 @@ def main():
         "ot": ot,
         "ot_warm_start": ws,
         "batched": c5,
+        "surface": surf,
+        "periodic": per,
+        "cvt_newton": newton,
+        "mesh_export": meshleg,
         "checks": {"sum_vol_minus_1": sumvol_err, "failed": int(s["nb_failed"]),
                    "uncertain": int(s["nb_uncertain"])},
         "phases_ms": {k: round(s[k], 3) for k in ("t_build_ms", "t_knn_ms", "t_clip_ms",

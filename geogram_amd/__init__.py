@@ -235,6 +235,41 @@ class Context:
         self._check(rc)
         return t
 
+    # --------------------------------------------------------- surface RVD
+    def surface_centroids(self, points, vertices, triangles, vertex_weights=None):
+        """Voronoi cells of the seeds restricted to a triangle mesh
+        (compute_centroids_on_surface, RVD.cpp:371-415; vgc_surface_centroids):
+        returns (mg (n, 3), m (n,), stats); mg / m is each restricted cell's
+        centroid.  vertex_weights: a density per vertex, linear on each
+        triangle.  Arrays on the GPU (torch) stay there."""
+        dev = _is_torch_cuda(points)
+        self._stream(dev)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            V = vertices.contiguous().to(torch.float64).reshape(-1, 3)
+            F = triangles.contiguous().to(torch.int32).reshape(-1, 3)
+            vw = None if vertex_weights is None else vertex_weights.contiguous().to(torch.float64)
+            n = pts.shape[0]
+            mg = torch.empty((n, 3), dtype=torch.float64, device=pts.device)
+            m = torch.empty(n, dtype=torch.float64, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            V = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+            F = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+            vw = (None if vertex_weights is None
+                  else np.ascontiguousarray(vertex_weights, dtype=np.float64))
+            n = pts.shape[0]
+            mg = np.empty((n, 3))
+            m = np.empty(n)
+        st = _lib.SurfaceStats()
+        rc = self.lib.fn("surface_centroids")(self.ctx, _ptr(pts), n, _ptr(V), V.shape[0], _ptr(F),
+                                              F.shape[0], _ptr(vw),
+                                              VGC_DEVICE_PTRS if dev else 0, _ptr(mg), _ptr(m),
+                                              ctypes.byref(st))
+        self._check(rc, allow_partial=False)
+        return mg, m, st.as_dict()
+
     # ------------------------------------------------------------------ kNN
     def knn(self, points, k):
         """k nearest other points of every point, exact (FP64 squared
@@ -626,22 +661,42 @@ class NearestNeighborSearch:
 
 
 class RestrictedVoronoiDiagram:
-    """Volumetric restricted Voronoi diagram of seeds in a box domain
-    (voronoi/RVD.h).  compute_centroids accumulates into caller-zeroed arrays
-    like RVD_Nd_Impl::compute_centroids_in_volume (RVD.cpp:500-528)."""
+    """Restricted Voronoi diagram of seeds (voronoi/RVD.h): in a box domain
+    (volumetric), or on a triangulated surface when surface=(vertices,
+    triangles[, vertex_weights]) is given (volumetric() false).
+    compute_centroids accumulates into caller-zeroed arrays like
+    compute_centroids_in_volume / _on_surface (RVD.cpp:500-528, 371-415)."""
 
-    def __init__(self, points, box=UNIT_BOX, ctx: Context | None = None, k0: int = 64):
+    def __init__(self, points, box=UNIT_BOX, ctx: Context | None = None, k0: int = 64,
+                 surface=None):
         self.points = points
         self.box = box
         self.ctx = ctx
         self.k0 = k0
         self.check_SR = True
+        self.surface = surface
+
+    def volumetric(self) -> bool:
+        return self.surface is None
+
+    def compute_centroids_on_surface(self, mg, m):
+        if self.surface is None:
+            raise ValueError("no surface: construct with surface=(vertices, triangles)")
+        ctx = self.ctx or default_context()
+        V, F = self.surface[0], self.surface[1]
+        vw = self.surface[2] if len(self.surface) > 2 else None
+        g, mm, st = ctx.surface_centroids(self.points, V, F, vw)
+        mg += g.reshape(mg.shape)
+        m += mm
+        return st
 
     def set_check_SR(self, x: bool):
         if not x:
             raise ValueError("the GPU path always applies the security radius")
 
     def compute_centroids(self, mg, m):
+        if not self.volumetric():
+            return self.compute_centroids_on_surface(mg, m)
         ctx = self.ctx or default_context()
         r = ctx.cells(self.points, None, self.box, self.k0, neighbors=False)
         mg += r.mg.reshape(mg.shape)

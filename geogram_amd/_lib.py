@@ -65,6 +65,14 @@ class MeshSizes(ctypes.Structure):
                                                  "nb_tets")]
 
 
+class SurfaceStats(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_uint64) for k in ("nb_items", "nb_split", "nb_redo", "nb_records")]
+                + [("k_final", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("t_ms", ctypes.c_double)])
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class OptStats(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_uint32) for k in ("iterations", "evaluations")]
                 + [("line_search_info", ctypes.c_int32), ("failed_cells", ctypes.c_uint32)]
@@ -117,6 +125,10 @@ def _declare(L, prefix):
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64),
                   ctypes.POINTER(Stats)]
+    f = L[prefix + "surface_centroids"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp,
+                  ctypes.POINTER(SurfaceStats)]
     f = L[prefix + "cvt_newton"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(OptStats)]
@@ -161,7 +173,7 @@ class _Lib:
         self.prefix = prefix
         self._f = {}
         names = ["last_error", "knn", "knn_search", "cells", "cells_batch", "lloyd", "lloyd_batch",
-                 "cvt_func_grad", "cvt_newton", "cell_mesh", "rdt", "ot_solve", "ot_solve_levels",
+                 "cvt_func_grad", "cvt_newton", "cell_mesh", "rdt", "surface_centroids", "ot_solve", "ot_solve_levels",
                  "ot_solve_batch", "ot_features",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:

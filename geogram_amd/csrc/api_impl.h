@@ -15,6 +15,7 @@
 #include "mlp.h"
 #include "ot_solver.h"
 #include "pipeline.h"
+#include "surface.h"
 
 namespace vgc {
 
@@ -1196,6 +1197,271 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
             ctx->err = "vgc_ot_solve_levels: not converged";
             return VGC_PARTIAL;
         }
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+// Voronoi cells restricted to a triangle mesh: per seed, the mass and first
+// moment of its part of the surface (compute_centroids_on_surface,
+// voronoi/RVD.cpp:371-415).  Passes over items (the triangles, then the
+// items split or redone): nearest seed of each item's first vertex, the
+// per-item walk, then the flags decide the next pass.
+template <class B>
+int api_surface_centroids(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* vtx,
+                          uint32_t nv, const int32_t* tri, uint32_t nt, const double* vw,
+                          uint32_t flags, double* mg, double* m, vgc_surface_stats* stats) {
+    if (!ctx) return VGC_ERR_ARG;
+    const char* name = "vgc_surface_centroids";
+    struct Owned {
+        B& be;
+        std::vector<void*> p;
+        ~Owned() {
+            for (void* q : p)
+                if (q) be.dfree(q);
+        }
+    };
+    try {
+        if (!xyz || !vtx || !tri || !mg || !m || n < 1 || nv < 3 || nt < 1) {
+            ctx->err = std::string(name) + ": bad argument";
+            return VGC_ERR_ARG;
+        }
+        if (n > (1u << 28) || nt > (1u << 26) || nv > (1u << 30)) {
+            ctx->err = std::string(name) + ": problem too large";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        auto t0 = std::chrono::steady_clock::now();
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        // triangle indices in range (read back when on the device)
+        {
+            std::vector<int32_t> ht;
+            const int32_t* hp = tri;
+            if (dev) {
+                ht.resize(3 * (size_t)nt);
+                be.d2h(ht.data(), tri, sizeof(int32_t) * 3 * (size_t)nt);
+                be.sync();
+                hp = ht.data();
+            }
+            for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+                if (hp[i] < 0 || (uint32_t)hp[i] >= nv) {
+                    ctx->err = std::string(name) + ": triangle vertex index out of range";
+                    return VGC_ERR_ARG;
+                }
+        }
+        Problem pb;
+        pb.n = (int)n;
+        pb.n_inst = 1;
+        pb.inst_n = {(int)n};
+        double lo[3], hi[3], S, vlo[3], vhi[3], vS;
+        host_extent(be, xyz, n, dev, lo, hi, S);
+        host_extent(be, vtx, nv, dev, vlo, vhi, vS);
+        S = std::max(S, vS);
+        if (!(S < 1e300)) {
+            ctx->err = std::string(name) + ": coordinates must be finite";
+            return VGC_ERR_ARG;
+        }
+        pb.box = {lo[0], lo[1], lo[2], hi[0] + 1e-300 + (hi[0] - lo[0]) * 1e-12,
+                  hi[1] + 1e-300 + (hi[1] - lo[1]) * 1e-12, hi[2] + 1e-300 + (hi[2] - lo[2]) * 1e-12};
+        for (int c = 0; c < 3; ++c)
+            if (!(pb.box[3 + c] > pb.box[c])) pb.box[3 + c] = pb.box[c] + 1.0;
+        const double* dx = xyz;
+        const double* dV = vtx;
+        const int32_t* dF = tri;
+        const double* dW = vw;
+        if (!dev) {
+            double* a = (double*)ctx->staging(30, sizeof(double) * 3 * (size_t)n);
+            be.h2d(a, xyz, sizeof(double) * 3 * (size_t)n);
+            dx = a;
+            double* b = (double*)ctx->staging(31, sizeof(double) * 3 * (size_t)nv);
+            be.h2d(b, vtx, sizeof(double) * 3 * (size_t)nv);
+            dV = b;
+            int32_t* c = (int32_t*)ctx->staging(32, sizeof(int32_t) * 3 * (size_t)nt);
+            be.h2d(c, tri, sizeof(int32_t) * 3 * (size_t)nt);
+            dF = c;
+            if (vw) {
+                double* d = (double*)ctx->staging(33, sizeof(double) * (size_t)nv);
+                be.h2d(d, vw, sizeof(double) * (size_t)nv);
+                dW = d;
+            }
+        }
+        pb.d_xyz = dx;
+        EngineStats es;
+        Engine<B>& E = ctx->eng;
+        if (E.build(pb, es)) {
+            ctx->err = std::string(name) + ": " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        Owned own{be, {}};
+        auto alloc = [&](size_t bytes) {
+            void* p = be.dalloc(bytes < 256 ? 256 : bytes);
+            own.p.push_back(p);
+            return p;
+        };
+        auto release = [&](void* p) {
+            for (auto& q : own.p)
+                if (q == p) {
+                    be.dfree(q);
+                    q = nullptr;
+                }
+        };
+        // neighbour lists: K nearest other seeds, doubled while items need more
+        int K = (int)std::min<uint32_t>(32u, n - 1);
+        int* d_knn = nullptr;
+        double* d_knd = nullptr;
+        auto build_lists = [&]() -> bool {
+            if (K == 0) return true;
+            if (d_knn) release(d_knn);
+            if (d_knd) release(d_knd);
+            d_knn = (int*)alloc(sizeof(int) * (size_t)n * K);
+            d_knd = (double*)alloc(sizeof(double) * (size_t)n * K);
+            return E.knn_exact(nullptr, (int)n, K + 1, true, S, d_knn, d_knd, es) == 0;
+        };
+        if (!build_lists()) {
+            ctx->err = std::string(name) + ": " + E.error;
+            return VGC_ERR_INTERNAL;
+        }
+        // items of the current pass
+        long long n_cur = nt;
+        double* tv = (double*)alloc(sizeof(double) * 9 * (size_t)n_cur);
+        double* tw = dW ? (double*)alloc(sizeof(double) * 3 * (size_t)n_cur) : nullptr;
+        be.surface_items(dV, dF, dW, (int)nt, tv, tw);
+        // records and the flags of every item ever processed
+        unsigned long long rec_cap = 4ull * nt + 2ull * n + 1024ull;
+        unsigned long long* rkey = (unsigned long long*)alloc(sizeof(unsigned long long) * rec_cap);
+        double* rval = (double*)alloc(sizeof(double) * 4 * rec_cap);
+        unsigned long long* d_count = (unsigned long long*)alloc(sizeof(unsigned long long));
+        be.dzero(d_count, sizeof(unsigned long long));
+        long long flag_cap = 2ll * nt;
+        int* gflag = (int*)alloc(sizeof(int) * (size_t)flag_cap);
+        long long base = 0;
+        unsigned long long count = 0;
+        vgc_surface_stats sst{};
+        std::vector<int> hflag;
+        for (int pass = 0; n_cur > 0; ++pass) {
+            if (pass >= 40 || base + n_cur > (1ll << 31)) {
+                ctx->err = std::string(name) + ": items keep splitting (degenerate input?)";
+                return VGC_ERR_INTERNAL;
+            }
+            if (base + n_cur > flag_cap) {
+                const long long cap2 = std::max(2 * flag_cap, base + n_cur);
+                int* g2 = (int*)alloc(sizeof(int) * (size_t)cap2);
+                be.d2d(g2, gflag, sizeof(int) * (size_t)base);
+                release(gflag);
+                gflag = g2;
+                flag_cap = cap2;
+            }
+            // nearest seed of each item's first vertex
+            double* q = (double*)alloc(sizeof(double) * 3 * (size_t)n_cur);
+            int* seed0 = (int*)alloc(sizeof(int) * (size_t)n_cur);
+            double* sd2 = (double*)alloc(sizeof(double) * (size_t)n_cur);
+            be.surface_v0(tv, (int)n_cur, q);
+            if (E.knn_exact(q, (int)n_cur, 1, false, S, seed0, sd2, es)) {
+                ctx->err = std::string(name) + ": " + E.error;
+                return VGC_ERR_INTERNAL;
+            }
+            release(q);
+            release(sd2);
+            SurfParams P;
+            P.xyz = dx;
+            P.n = (int)n;
+            P.knn = d_knn;
+            P.knn_d2 = d_knd;
+            P.K = K;
+            P.k_all = K >= (int)n - 1;
+            P.tv = tv;
+            P.tw = tw;
+            P.seed0 = seed0;
+            P.n_items = (int)n_cur;
+            P.item_base = (uint32_t)base;
+            P.rec_count = d_count;
+            P.flags = gflag + base;
+            for (;;) {
+                P.rec_key = rkey;
+                P.rec_val = rval;
+                P.rec_cap = rec_cap;
+                be.surface_pass(P);
+                unsigned long long c2 = 0;
+                be.d2h(&c2, d_count, sizeof(c2));
+                be.sync();
+                if (c2 <= rec_cap) {
+                    count = c2;
+                    break;
+                }
+                // records overflowed: a larger buffer, the pass again
+                const unsigned long long cap2 = c2 + c2 / 2 + 1024;
+                unsigned long long* k2 = (unsigned long long*)alloc(sizeof(unsigned long long) * cap2);
+                double* v2 = (double*)alloc(sizeof(double) * 4 * cap2);
+                be.d2d(k2, rkey, sizeof(unsigned long long) * count);
+                be.d2d(v2, rval, sizeof(double) * 4 * count);
+                release(rkey);
+                release(rval);
+                rkey = k2;
+                rval = v2;
+                rec_cap = cap2;
+                be.h2d(d_count, &count, sizeof(count));
+            }
+            release(seed0);
+            hflag.resize((size_t)n_cur);
+            be.d2h(hflag.data(), gflag + base, sizeof(int) * (size_t)n_cur);
+            be.sync();
+            std::vector<int> split, redo;
+            for (long long t = 0; t < n_cur; ++t) {
+                if (hflag[t] == SURF_OVERFLOW) split.push_back((int)t);
+                if (hflag[t] == SURF_NEED_K) redo.push_back((int)t);
+            }
+            sst.nb_items += (uint64_t)n_cur;
+            sst.nb_split += split.size();
+            sst.nb_redo += redo.size();
+            base += n_cur;
+            const long long n_next = 4ll * (long long)split.size() + (long long)redo.size();
+            if (n_next == 0) break;
+            double* tv2 = (double*)alloc(sizeof(double) * 9 * (size_t)n_next);
+            double* tw2 = tw ? (double*)alloc(sizeof(double) * 3 * (size_t)n_next) : nullptr;
+            std::vector<int> lst(split);
+            lst.insert(lst.end(), redo.begin(), redo.end());
+            int* d_lst = (int*)alloc(sizeof(int) * lst.size());
+            be.h2d(d_lst, lst.data(), sizeof(int) * lst.size());
+            be.surface_next(tv, tw, d_lst, (int)split.size(), (int)redo.size(), tv2, tw2);
+            be.sync();
+            release(d_lst);
+            release(tv);
+            if (tw) release(tw);
+            tv = tv2;
+            tw = tw2;
+            n_cur = n_next;
+            if (!redo.empty() && K < (int)n - 1) {
+                K = (int)std::min<long long>(2ll * K, (long long)n - 1);
+                if (!build_lists()) {
+                    ctx->err = std::string(name) + ": " + E.error;
+                    return VGC_ERR_INTERNAL;
+                }
+            }
+        }
+        // records in (seed, item) order, summed per seed
+        const int nrec = (int)count;
+        uint64_t* k2 = (uint64_t*)alloc(sizeof(uint64_t) * (size_t)std::max(nrec, 1));
+        int* i1 = (int*)alloc(sizeof(int) * (size_t)std::max(nrec, 1));
+        int* i2 = (int*)alloc(sizeof(int) * (size_t)std::max(nrec, 1));
+        if (nrec > 0) {
+            be.iota(i1, nrec);
+            be.sort_pairs((uint64_t*)rkey, i1, k2, i2, nrec, 64);
+        }
+        double* om = dev ? m : (double*)ctx->staging(34, sizeof(double) * (size_t)n);
+        double* omg = dev ? mg : (double*)ctx->staging(35, sizeof(double) * 3 * (size_t)n);
+        be.surface_reduce((int)n, (const unsigned long long*)k2, i2, nrec, rval, gflag, om, omg);
+        if (!dev) {
+            be.d2h(m, om, sizeof(double) * (size_t)n);
+            be.d2h(mg, omg, sizeof(double) * 3 * (size_t)n);
+        }
+        be.sync();
+        sst.nb_records = count;
+        sst.k_final = (uint32_t)K;
+        sst.t_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                       .count();
+        if (stats) *stats = sst;
         return VGC_OK;
     } catch (const std::exception& e) {
         ctx->err = e.what();

@@ -674,6 +674,35 @@ struct EmuBackend {
         for (int i = 0; i < n; ++i) ot_features_row(i, xyz, nu, d2, box, inv_cell, h, f + (long long)i * kMlpIn);
     }
 
+    // ------------------------------------------ surface restricted cells
+    void surface_items(const double* V, const int* F, const double* vw, int nt, double* ov,
+                       double* ow) {
+        for (int t = 0; t < nt; ++t) surface_items_row(V, F, vw, t, ov, ow);
+    }
+    void surface_v0(const double* tv, int n, double* q) {
+        for (int t = 0; t < n; ++t)
+            for (int a = 0; a < 3; ++a) q[3 * (size_t)t + a] = tv[9 * (size_t)t + a];
+    }
+    void surface_pass(const SurfParams& P) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int t = 0; t < P.n_items; ++t) surface_item(P, t);
+    }
+    void surface_next(const double* tv, const double* tw, const int* lst, int nsplit, int nredo,
+                      double* ov, double* ow) {
+        for (int c = 0; c < nsplit; ++c) surface_split_row(tv, tw, lst[c], c, ov, ow);
+        for (int c = nsplit; c < nsplit + nredo; ++c) {
+            const int t = lst[c];
+            const size_t o = 4 * (size_t)nsplit + (size_t)(c - nsplit);
+            for (int a = 0; a < 9; ++a) ov[9 * o + a] = tv[9 * (size_t)t + a];
+            if (ow)
+                for (int a = 0; a < 3; ++a) ow[3 * o + a] = tw[3 * (size_t)t + a];
+        }
+    }
+    void surface_reduce(int n, const unsigned long long* keys, const int* idx, long long nrec,
+                        const double* val, const int* flag, double* m, double* mg) {
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < n; ++s) surface_reduce_row(s, keys, idx, nrec, val, flag, m, mg);
+    }
     // ------------------------------------------------------- exact kNN
     void iota(int* d, int n) {
         for (int i = 0; i < n; ++i) d[i] = i;
@@ -874,6 +903,12 @@ int vgc_emu_cell_mesh(void* ctx, const double* xyz, const double* w, uint32_t n,
     return vgc::api_cell_mesh((EmuCtx*)ctx, "vgc_cell_mesh", xyz, w, n, box, shrink, flags, cell_vptr,
                               cell_fptr, vertices, facet_ptr, facet_vertices, facet_nbr, tets, cap,
                               need, stats);
+}
+int vgc_emu_surface_centroids(void* ctx, const double* xyz, uint32_t n, const double* vtx,
+                              uint32_t nv, const int32_t* tri, uint32_t nt, const double* vw,
+                              uint32_t flags, double* mg, double* m, vgc_surface_stats* stats) {
+    return vgc::api_surface_centroids((EmuCtx*)ctx, xyz, n, vtx, nv, tri, nt, vw, flags, mg, m,
+                                      stats);
 }
 int vgc_emu_rdt(void* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
             uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats) {

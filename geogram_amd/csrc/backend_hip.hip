@@ -260,6 +260,48 @@ __global__ void k_periodic_reach(const pt4* __restrict__ pts, const int* __restr
     if (__lane_id() == 0 && v > 0.0f) atomicMax((unsigned*)out, __float_as_uint(v));
 }
 
+// ------------------------------------------------- surface restricted cells
+__global__ void k_surface_items(const double* __restrict__ V, const int* __restrict__ F,
+                                const double* __restrict__ vw, int nt, double* ov, double* ow) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) surface_items_row(V, F, vw, t, ov, ow);
+}
+
+__global__ void k_surface_v0(const double* __restrict__ tv, int n, double* q) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        for (int a = 0; a < 3; ++a) q[3 * (size_t)t + a] = tv[9 * (size_t)t + a];
+}
+
+__global__ __launch_bounds__(256) void k_surface_pass(SurfParams P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < P.n_items) surface_item(P, t);
+}
+
+// the split items' children (4 each), then the redone items unchanged
+__global__ void k_surface_next(const double* __restrict__ tv, const double* __restrict__ tw,
+                               const int* __restrict__ lst, int nsplit, int nredo, double* ov,
+                               double* ow) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nsplit) {
+        surface_split_row(tv, tw, lst[c], c, ov, ow);
+    } else if (c < nsplit + nredo) {
+        const int t = lst[c];
+        const size_t o = 4 * (size_t)nsplit + (size_t)(c - nsplit);
+        for (int a = 0; a < 9; ++a) ov[9 * o + a] = tv[9 * (size_t)t + a];
+        if (ow)
+            for (int a = 0; a < 3; ++a) ow[3 * o + a] = tw[3 * (size_t)t + a];
+    }
+}
+
+__global__ void k_surface_reduce(int n, const unsigned long long* __restrict__ keys,
+                                 const int* __restrict__ idx, long long nrec,
+                                 const double* __restrict__ val, const int* __restrict__ flag,
+                                 double* m, double* mg) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) surface_reduce_row(s, keys, idx, nrec, val, flag, m, mg);
+}
+
 __global__ void k_knn_to_caller(const int* __restrict__ orig, const int* __restrict__ list,
                                 const float* __restrict__ d2, int n, int K, int* out, float* outd) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1588,6 +1630,31 @@ struct HipBackend {
     }
 
     // ------------------------------------------------------- exact kNN
+    void surface_items(const double* V, const int* F, const double* vw, int nt, double* ov,
+                       double* ow) {
+        k_surface_items<<<blocks_for(nt, 256), 256, 0, stream_>>>(V, F, vw, nt, ov, ow);
+        check();
+    }
+    void surface_v0(const double* tv, int n, double* q) {
+        k_surface_v0<<<blocks_for(n, 256), 256, 0, stream_>>>(tv, n, q);
+        check();
+    }
+    void surface_pass(const SurfParams& P) {
+        k_surface_pass<<<blocks_for(P.n_items, 256), 256, 0, stream_>>>(P);
+        check();
+    }
+    void surface_next(const double* tv, const double* tw, const int* lst, int nsplit, int nredo,
+                      double* ov, double* ow) {
+        k_surface_next<<<blocks_for(nsplit + nredo, 256), 256, 0, stream_>>>(tv, tw, lst, nsplit,
+                                                                              nredo, ov, ow);
+        check();
+    }
+    void surface_reduce(int n, const unsigned long long* keys, const int* idx, long long nrec,
+                        const double* val, const int* flag, double* m, double* mg) {
+        k_surface_reduce<<<blocks_for(n, 256), 256, 0, stream_>>>(n, keys, idx, nrec, val, flag, m,
+                                                                   mg);
+        check();
+    }
     void iota(int* d, int n) {
         if (n > 0) k_iota<<<blocks_for(n, 256), 256, 0, stream_>>>(d, n);
         check();
@@ -1803,6 +1870,12 @@ int vgc_cell_mesh(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, 
     return vgc::api_cell_mesh(ctx->impl, "vgc_cell_mesh", xyz, w, n, box, shrink, flags, cell_vptr,
                               cell_fptr, vertices, facet_ptr, facet_vertices, facet_nbr, tets, cap,
                               need, stats);
+}
+int vgc_surface_centroids(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* vtx,
+                          uint32_t nv, const int32_t* tri, uint32_t nt, const double* vw,
+                          uint32_t flags, double* mg, double* m, vgc_surface_stats* stats) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_surface_centroids(ctx->impl, xyz, n, vtx, nv, tri, nt, vw, flags, mg, m, stats);
 }
 int vgc_rdt(vgc_ctx* ctx, const double* xyz, const double* w, uint32_t n, const double* box,
             uint32_t flags, int32_t* tets, uint64_t cap, uint64_t* nb_tets, vgc_stats* stats) {

@@ -244,6 +244,32 @@ typedef struct vgc_opt_stats {
 int vgc_cvt_newton(vgc_ctx* ctx, double* xyz, uint32_t n, const double* box, uint32_t iters,
                    uint32_t m, uint32_t flags, vgc_opt_stats* stats);
 
+/* Voronoi cells of the n seeds restricted to a triangulated surface
+   (vertices vtx[3 nv], triangles tri[3 nt] of vertex indices, vw[nv] a
+   density per vertex, linear on each triangle, or NULL for density 1): m[n]
+   is the mass (area, or integral of the density) of each seed's part of the
+   surface, mg[3 n] the integral of x over it (mg / m is its centroid); both
+   are overwritten.  Replaces RestrictedVoronoiDiagram::
+   compute_centroids_on_surface (voronoi/RVD.cpp:371-415; polygons of
+   voronoi/generic_RVD.h:1319-1420 and 2133-2190, Geom::triangle_centroid of
+   basic/geometry_nd.h:178-197 with a vertex "weight" attribute), which
+   accumulates into caller-zeroed arrays.  FP64 throughout; each seed's
+   polygons are summed in (triangle, split) order, so a call is
+   reproducible bit for bit.  flags: VGC_DEVICE_PTRS. */
+typedef struct vgc_surface_stats {
+    uint64_t nb_items;   /* triangles processed, splits and retries included */
+    uint64_t nb_split;   /* items split in four (polygon or seed list full)  */
+    uint64_t nb_redo;    /* items redone with longer neighbour lists         */
+    uint64_t nb_records; /* restricted polygons integrated (retried items'
+                            first attempts included)                         */
+    uint32_t k_final;    /* neighbours per seed in the last lists            */
+    uint32_t pad;
+    double t_ms;         /* wall time of the call                            */
+} vgc_surface_stats;
+int vgc_surface_centroids(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* vtx,
+                          uint32_t nv, const int32_t* tri, uint32_t nt, const double* vw,
+                          uint32_t flags, double* mg, double* m, vgc_surface_stats* stats);
+
 /* Lloyd iterations of n_inst independent CVT problems at once (instance
    layout of vgc_cells_batch): each iteration clips every instance's cells in
    the same launches.  No counterpart in geogram, which runs one

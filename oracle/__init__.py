@@ -58,6 +58,9 @@ def lib():
         L.orc_lloyd.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(Stats)]
         L.orc_det4_sign.argtypes = [dp, dp, dp, dp]
+        L.orc_surface_centroids.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ip, ctypes.c_int,
+                                            dp, ctypes.c_int, dp, dp,
+                                            ctypes.POINTER(ctypes.c_longlong)]
         L.orc_det4_sign_exact.argtypes = [dp, dp, dp, dp]
         L.orc_det3_sign.argtypes = [dp, dp, dp]
         L.orc_exact_fallbacks.restype = ctypes.c_longlong
@@ -212,6 +215,28 @@ def periodic_cells(xyz, w=None, box=UNIT_BOX, centred=True, stride=128, e2=False
               stride=stride, e2=e2)
     o["nbr"] = np.where(o["nbr"] >= 0, o["nbr"] % n, -1)
     return o
+
+
+def surface_centroids(xyz, V, F, vw=None, brute=False):
+    """Voronoi cells of the seeds restricted to the triangle mesh (V, F):
+    (mg (n, 3), m (n,), polygons).  vw: per-vertex density (linear on each
+    triangle).  brute: every facet clipped by all bisectors of every seed."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+    n = x.shape[0]
+    mg = np.zeros((n, 3))
+    m = np.zeros(n)
+    npoly = ctypes.c_longlong(0)
+    wp = None
+    if vw is not None:
+        vw = np.ascontiguousarray(vw, dtype=np.float64)
+        wp = _dp(vw)
+    err = lib().orc_surface_centroids(_dp(x), n, _dp(V), V.shape[0], _ip(F), F.shape[0], wp,
+                                      1 if brute else 0, _dp(mg), _dp(m), ctypes.byref(npoly))
+    if err:
+        raise RuntimeError(f"orc_surface_centroids failed ({err})")
+    return mg, m, int(npoly.value)
 
 
 def lloyd(xyz, iters, box=UNIT_BOX, k0=20):

@@ -1031,3 +1031,308 @@ int orc_lloyd(double* xyz, int n, const double* box, int iters, int k0,
     free(fa);
     return err;
 }
+
+/* ======================================================================
+ * Voronoi cells restricted to a triangulated surface: masses and first
+ * moments (RestrictedVoronoiDiagram::compute_centroids_on_surface,
+ * voronoi/RVD.cpp:371-415), by the traversal of
+ * compute_surfacic_with_seeds_priority (voronoi/generic_RVD.h:1319-1420):
+ * facets in index order, each started from the seed nearest its first
+ * vertex (find_seed_near_facet, generic_RVD.h:2010-2068) or from the seed
+ * that reached it through a shared edge, and the Delaunay 1-skeleton walked
+ * through the bisector edges of each restricted polygon.  A polygon is the
+ * facet clipped by Sutherland-Hodgman against the bisectors of the seed's
+ * neighbours in distance order until the security radius
+ * (clip_by_cell_SR, generic_RVD.h:2133-2190; Polygon::clip_by_plane_fast,
+ * voronoi/generic_RVD_polygon.h:242-366), and is integrated as a fan from
+ * its first vertex (TriangleAction, generic_RVD.h:425-460) with Heron's
+ * area (basic/geometry_nd.h:143-156) and, with vertex weights, the linear
+ * density of Geom::triangle_centroid (basic/geometry_nd.h:178-197).
+ * ====================================================================== */
+
+typedef struct {
+    double p[3];
+    double w;
+    int adj_f; /* facet across the edge leaving this vertex (-1: none) */
+    int adj_s; /* seed across the edge arriving at this vertex (-1: none) */
+} spoly_v;
+
+typedef struct {
+    int n, cap;
+    spoly_v* v;
+} spoly;
+
+static int spoly_push(spoly* P, spoly_v x) {
+    if (P->n == P->cap) {
+        int cap = P->cap ? 2 * P->cap : 16;
+        spoly_v* nv = (spoly_v*)realloc(P->v, sizeof(spoly_v) * (size_t)cap);
+        if (!nv) return -2;
+        P->v = nv;
+        P->cap = cap;
+    }
+    P->v[P->n++] = x;
+    return 0;
+}
+
+static int sgn(double x) { return (x > 0.0) - (x < 0.0); }
+
+/* dst = src on pi's side of the bisector of [pi, pj] (seed j) */
+static int spoly_clip(const spoly* src, spoly* dst, const double* pi, const double* pj, int j) {
+    dst->n = 0;
+    if (src->n == 0) return 0;
+    double d = 0.0;
+    for (int c = 0; c < 3; ++c) d += (pi[c] + pj[c]) * (pi[c] - pj[c]);
+    const spoly_v* pv = &src->v[src->n - 1];
+    double pl = 0.0;
+    for (int c = 0; c < 3; ++c) pl += pv->p[c] * (pi[c] - pj[c]);
+    int ps = sgn(2.0 * pl - d);
+    for (int k = 0; k < src->n; ++k) {
+        const spoly_v* vk = &src->v[k];
+        double l = 0.0;
+        for (int c = 0; c < 3; ++c) l += vk->p[c] * (pi[c] - pj[c]);
+        const int s = sgn(2.0 * l - d);
+        if (s != ps && ps != 0) {
+            const double denom = 2.0 * (pl - l);
+            double l1 = 0.5, l2 = 0.5;
+            if (fabs(denom) >= 1e-20) {
+                l1 = (d - 2.0 * l) / denom;
+                l2 = 1.0 - l1;
+            }
+            spoly_v I;
+            for (int c = 0; c < 3; ++c) I.p[c] = l1 * pv->p[c] + l2 * vk->p[c];
+            I.w = l1 * pv->w + l2 * vk->w;
+            if (s > 0) {
+                I.adj_f = pv->adj_f;
+                I.adj_s = j;
+            } else {
+                I.adj_f = -1;
+                I.adj_s = vk->adj_s;
+            }
+            if (spoly_push(dst, I)) return -2;
+        }
+        if (s > 0 && spoly_push(dst, *vk)) return -2;
+        pv = vk;
+        pl = l;
+        ps = s;
+    }
+    return 0;
+}
+
+static double heron(const double* a, const double* b, const double* c) {
+    const double la = sqrt(dist2(a, b)), lb = sqrt(dist2(b, c)), lc = sqrt(dist2(c, a));
+    const double s = 0.5 * (la + lb + lc);
+    const double A2 = s * (s - la) * (s - lb) * (s - lc);
+    return sqrt(A2 > 0.0 ? A2 : 0.0);
+}
+
+static void spoly_integrate(const spoly* P, int weighted, double* m, double* mg) {
+    for (int i = 1; i + 1 < P->n; ++i) {
+        const spoly_v *a = &P->v[0], *b = &P->v[i], *c = &P->v[i + 1];
+        const double area = heron(a->p, b->p, c->p);
+        if (!weighted) {
+            const double s = area / 3.0;
+            *m += area;
+            for (int k = 0; k < 3; ++k) mg[k] += s * (a->p[k] + b->p[k] + c->p[k]);
+        } else {
+            const double abc = a->w + b->w + c->w;
+            const double wa = a->w + abc, wb = b->w + abc, wc = c->w + abc, s = area / 12.0;
+            *m += area / 3.0 * abc;
+            for (int k = 0; k < 3; ++k) mg[k] += s * (wa * a->p[k] + wb * b->p[k] + wc * c->p[k]);
+        }
+    }
+}
+
+static int facet_polygon(const double* V, const int* F, const int* adj, const double* vw, int f,
+                         spoly* P) {
+    P->n = 0;
+    for (int k = 0; k < 3; ++k) {
+        const int v = F[3 * f + k];
+        spoly_v x;
+        for (int c = 0; c < 3; ++c) x.p[c] = V[3 * (size_t)v + c];
+        x.w = vw ? vw[v] : 1.0;
+        x.adj_f = adj[3 * f + k];
+        x.adj_s = -1;
+        if (spoly_push(P, x)) return -2;
+    }
+    return 0;
+}
+
+/* facet across edge (k, k+1) of each facet, -1 on the border: the
+   facet_corners adjacency of a manifold triangle mesh */
+static int facet_adjacency(const int* F, int nf, int* adj) {
+    const int ne = 3 * nf;
+    long long* key = (long long*)malloc(sizeof(long long) * (size_t)ne);
+    int* ord = (int*)malloc(sizeof(int) * (size_t)ne);
+    if (!key || !ord) {
+        free(key);
+        free(ord);
+        return -2;
+    }
+    for (int e = 0; e < ne; ++e) {
+        const int f = e / 3, k = e % 3;
+        const long long a = F[3 * f + k], b = F[3 * f + (k + 1) % 3];
+        key[e] = a < b ? (a << 32) | b : (b << 32) | a;
+        ord[e] = e;
+        adj[e] = -1;
+    }
+    /* insertion-free: shell sort of edge ids by key */
+    for (int gap = ne / 2; gap > 0; gap /= 2)
+        for (int i = gap; i < ne; ++i) {
+            const int t = ord[i];
+            int j = i;
+            while (j >= gap && key[ord[j - gap]] > key[t]) {
+                ord[j] = ord[j - gap];
+                j -= gap;
+            }
+            ord[j] = t;
+        }
+    for (int i = 0; i + 1 < ne; ++i)
+        if (key[ord[i]] == key[ord[i + 1]] && (i + 2 >= ne || key[ord[i + 2]] != key[ord[i]])) {
+            adj[ord[i]] = ord[i + 1] / 3;
+            adj[ord[i + 1]] = ord[i] / 3;
+        }
+    free(key);
+    free(ord);
+    return 0;
+}
+
+/* polygon of seed s on facet F: the SR loop of clip_by_cell_SR */
+static int surface_clip_seed(const kd_tree* T, const double* xyz, int n, int s, const spoly* F,
+                             spoly* ping, spoly* pong, nbr_scratch* S, int k0, long long* nclip) {
+    ping->n = 0;
+    for (int k = 0; k < F->n; ++k)
+        if (spoly_push(ping, F->v[k])) return -2;
+    const double* pi = xyz + 3 * (size_t)s;
+    int k = k0 < n - 1 ? k0 : n - 1;
+    int jj = 0, prev = -1;
+    while (1) {
+        const int got = get_neighbors(T, s, k, S);
+        if (got < 0) return got;
+        if (got == 0 || got == prev) return 0;
+        for (; jj < got; ++jj) {
+            const int j = S->nb[jj];
+            double R2 = 0.0;
+            for (int v = 0; v < ping->n; ++v) {
+                const double dv = dist2(pi, ping->v[v].p);
+                if (dv > R2) R2 = dv;
+            }
+            if (dist2(pi, xyz + 3 * (size_t)j) > 4.1 * R2) return 0;
+            if (spoly_clip(ping, pong, pi, xyz + 3 * (size_t)j, j)) return -2;
+            ++*nclip;
+            spoly tmp = *ping;
+            *ping = *pong;
+            *pong = tmp;
+        }
+        prev = got;
+        if (got >= n - 1) return 0;
+        k = got > 8 ? got + got / 8 : got + 1;
+        if (k > n - 1) k = n - 1;
+    }
+}
+
+/* mode 0: the reference traversal above; mode 1: every (facet, seed) pair,
+   the facet clipped by all n - 1 bisectors of the seed (brute force).
+   mg[3n] / m[n] are overwritten.  Returns 0 or <0. */
+int orc_surface_centroids(const double* xyz, int n, const double* V, int nv, const int* F, int nf,
+                          const double* vw, int mode, double* mg, double* m, long long* npoly) {
+    (void)nv;
+    for (int i = 0; i < n; ++i) {
+        m[i] = 0.0;
+        mg[3 * i] = mg[3 * i + 1] = mg[3 * i + 2] = 0.0;
+    }
+    if (npoly) *npoly = 0;
+    if (n == 0 || nf == 0) return 0;
+    int err = 0;
+    int* adj = (int*)malloc(sizeof(int) * 3 * (size_t)nf);
+    if (!adj) return -2;
+    err = facet_adjacency(F, nf, adj);
+    spoly Fp = {0, 0, NULL}, ping = {0, 0, NULL}, pong = {0, 0, NULL};
+    long long np = 0, nclip = 0;
+    if (!err && mode == 1) {
+        for (int f = 0; f < nf && !err; ++f) {
+            err = facet_polygon(V, F, adj, vw, f, &Fp);
+            for (int s = 0; s < n && !err; ++s) {
+                ping.n = 0;
+                for (int k = 0; k < Fp.n; ++k) spoly_push(&ping, Fp.v[k]);
+                for (int j = 0; j < n && ping.n > 0 && !err; ++j) {
+                    if (j == s) continue;
+                    err = spoly_clip(&ping, &pong, xyz + 3 * (size_t)s, xyz + 3 * (size_t)j, j);
+                    spoly tmp = ping;
+                    ping = pong;
+                    pong = tmp;
+                }
+                if (ping.n >= 3) {
+                    spoly_integrate(&ping, vw != NULL, m + s, mg + 3 * (size_t)s);
+                    ++np;
+                }
+            }
+        }
+    } else if (!err) {
+        kd_tree T;
+        nbr_scratch S = {0, NULL, NULL, NULL};
+        char* fmark = (char*)calloc((size_t)nf, 1);
+        int* stamp = (int*)malloc(sizeof(int) * (size_t)n);
+        int* fstack_f = (int*)malloc(sizeof(int) * (size_t)nf);
+        int* fstack_s = (int*)malloc(sizeof(int) * (size_t)nf);
+        int* sstack = (int*)malloc(sizeof(int) * (size_t)n);
+        if (kd_build(&T, xyz, n) != 0 || !fmark || !stamp || !fstack_f || !fstack_s || !sstack)
+            err = -2;
+        for (int i = 0; i < n && !err; ++i) stamp[i] = -1;
+        for (int f0 = 0; f0 < nf && !err; ++f0) {
+            if (fmark[f0]) continue;
+            fmark[f0] = 1;
+            int nfs = 0;
+            {
+                int near;
+                double nd;
+                kd_query(&T, V + 3 * (size_t)F[3 * f0], 1, &near, &nd);
+                fstack_f[nfs] = f0;
+                fstack_s[nfs++] = near;
+            }
+            while (nfs > 0 && !err) {
+                --nfs;
+                const int cf = fstack_f[nfs];
+                int cs = fstack_s[nfs];
+                err = facet_polygon(V, F, adj, vw, cf, &Fp);
+                stamp[cs] = cf;
+                int nss = 0;
+                sstack[nss++] = cs;
+                while (nss > 0 && !err) {
+                    cs = sstack[--nss];
+                    err = surface_clip_seed(&T, xyz, n, cs, &Fp, &ping, &pong, &S, 20, &nclip);
+                    if (err) break;
+                    if (ping.n >= 3) ++np;
+                    spoly_integrate(&ping, vw != NULL, m + cs, mg + 3 * (size_t)cs);
+                    for (int v = 0; v < ping.n; ++v) {
+                        const int nf2 = ping.v[v].adj_f;
+                        if (nf2 >= 0 && nf2 != cf && !fmark[nf2]) {
+                            fmark[nf2] = 1;
+                            fstack_f[nfs] = nf2;
+                            fstack_s[nfs++] = cs;
+                        }
+                        const int ns = ping.v[v].adj_s;
+                        if (ns >= 0 && stamp[ns] != cf) {
+                            stamp[ns] = cf;
+                            sstack[nss++] = ns;
+                        }
+                    }
+                }
+            }
+        }
+        kd_free(&T);
+        free(S.qi);
+        free(S.qd);
+        free(S.nb);
+        free(fmark);
+        free(stamp);
+        free(fstack_f);
+        free(fstack_s);
+        free(sstack);
+    }
+    free(Fp.v);
+    free(ping.v);
+    free(pong.v);
+    free(adj);
+    if (npoly) *npoly = np;
+    return err;
+}

@@ -22,7 +22,12 @@
  *  - the Laguerre plane of PeriodicDelaunay3d::copy_Laguerre_cell_facet_from_
  *    Delaunay (delaunay/periodic_delaunay_3d.cpp:3719-3726);
  *  - the Lloyd update of CentroidalVoronoiTesselation::Lloyd_iterations
- *    (voronoi/CVT.cpp:148-162), with the security radius on.
+ *    (voronoi/CVT.cpp:148-162), with the security radius on;
+ *  - the Voronoi cells restricted to a triangulated surface and their
+ *    masses / first moments (RestrictedVoronoiDiagram::
+ *    compute_centroids_on_surface, voronoi/RVD.cpp:371-415; traversal of
+ *    voronoi/generic_RVD.h:1319-1420, polygon clipping of
+ *    voronoi/generic_RVD_polygon.h:242-366).
  */
 #ifndef VGC_ORACLE_H
 #define VGC_ORACLE_H
@@ -89,6 +94,16 @@ void orc_set_mesh_output(int* ibuf, int istride, double* vbuf, int vstride);
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int nt);
 int orc_get_threads(void);
+
+/* Voronoi cells of the n seeds restricted to the triangle mesh (V: nv
+   vertices, F: nf triangles, vw: vertex weights = a linear density, or NULL
+   for density 1): m[n] = mass (area) of each restricted cell, mg[3n] = its
+   integral of x.  mode 0: the reference's traversal with the security
+   radius; mode 1: every facet clipped by all bisectors of every seed (brute
+   force).  npoly: restricted polygons with 3 or more vertices.  Returns 0
+   or <0. */
+int orc_surface_centroids(const double* xyz, int n, const double* V, int nv, const int* F, int nf,
+                          const double* vw, int mode, double* mg, double* m, long long* npoly);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,168 @@
+"""Voronoi cells restricted to a triangulated surface (vgc_surface_centroids,
+compute_centroids_on_surface of voronoi/RVD.cpp:371-415).
+
+The oracle (oracle.surface_centroids, the reference's facet / seed traversal
+with the security radius) is pinned by: a flat square, where the restricted
+cells of seeds on the plane are their 2D Voronoi cells (Qhull half-space
+intersection); a brute force clipping every facet by all bisectors of every
+seed; the totals (the cells partition the surface: masses add up to its
+area, or to the integral of the linear density).  The emulated kernels and
+the GPU are then held to the oracle."""
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull, HalfspaceIntersection
+
+import geogram_amd as ga
+import oracle
+from geogram_amd.configs import icosphere, sphere_seeds
+
+SQUARE_V = np.array([[0, 0, .5], [1, 0, .5], [1, 1, .5], [0, 1, .5]], float)
+SQUARE_F = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+
+
+def _voronoi_2d_areas(p):
+    out = []
+    for i in range(len(p)):
+        hs = [[-1, 0, 0], [1, 0, -1], [0, -1, 0], [0, 1, -1]]
+        for j in range(len(p)):
+            if j != i:
+                e = p[j] - p[i]
+                hs.append([e[0], e[1], -(p[j] @ p[j] - p[i] @ p[i]) / 2])
+        H = HalfspaceIntersection(np.array(hs, float), p[i])
+        out.append(ConvexHull(H.intersections).volume)
+    return np.array(out)
+
+
+def _tri_areas(V, F):
+    return 0.5 * np.linalg.norm(np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]), axis=1)
+
+
+def _agree(g, mm, mg, m, rel=1e-10):
+    scale = m.max()
+    assert np.abs(mm - m).max() <= rel * scale
+    assert np.abs(g - mg).max() <= rel * np.abs(mg).max()
+    nz = m > 1e-9 * scale
+    c1 = g[nz] / mm[nz, None]
+    c0 = mg[nz] / m[nz, None]
+    assert np.abs(c1 - c0).max() <= 1e-9
+
+
+# ---------------------------------------------------------------- oracle pins
+def test_oracle_flat_square_is_2d_voronoi():
+    rng = np.random.default_rng(0)
+    p = rng.random((60, 2))
+    x = np.c_[p, np.full(60, .5)]
+    mg, m, npoly = oracle.surface_centroids(x, SQUARE_V, SQUARE_F)
+    assert np.abs(m - _voronoi_2d_areas(p)).max() < 1e-14
+    assert abs(m.sum() - 1.0) < 1e-14
+    assert np.allclose(mg[:, 2], 0.5 * m, rtol=0, atol=1e-15)
+
+
+def test_oracle_traversal_vs_brute_force_and_totals():
+    V, F = icosphere(3)
+    x = sphere_seeds(80, 1)
+    mg, m, _ = oracle.surface_centroids(x, V, F)
+    mgb, mb, _ = oracle.surface_centroids(x, V, F, brute=True)
+    assert np.abs(m - mb).max() < 1e-14 and np.abs(mg - mgb).max() < 1e-14
+    A = _tri_areas(V, F)
+    assert abs(m.sum() - A.sum()) < 1e-12
+    # first moment of the whole surface: sum of triangle areas x centroids
+    assert np.abs(mg.sum(0) - (A[:, None] * V[F].mean(1)).sum(0)).max() < 1e-12
+    # linear density: total = sum over triangles of area (a + b + c) / 3
+    w = np.random.default_rng(2).random(len(V)) + 0.5
+    mgw, mw, _ = oracle.surface_centroids(x, V, F, vw=w)
+    assert abs(mw.sum() - (A * w[F].sum(1) / 3).sum()) < 1e-12
+
+
+def test_oracle_seed_height_invariance():
+    # seeds off the surface still partition it: lifting all seeds off the
+    # plane by the same height adds the same constant to every squared
+    # distance and leaves the cells unchanged
+    rng = np.random.default_rng(4)
+    p = rng.random((40, 2))
+    m0 = oracle.surface_centroids(np.c_[p, np.full(40, .5)], SQUARE_V, SQUARE_F)[1]
+    m1 = oracle.surface_centroids(np.c_[p, np.full(40, .9)], SQUARE_V, SQUARE_F)[1]
+    assert np.abs(m0 - m1).max() < 1e-14
+
+
+# ------------------------------------------------------- emulated kernels
+def test_emu_surface_square(emu_ctx):
+    rng = np.random.default_rng(5)
+    x = np.c_[rng.random((300, 2)), 0.5 + 0.05 * rng.standard_normal(300)]
+    mg, m, _ = oracle.surface_centroids(x, SQUARE_V, SQUARE_F)
+    g, mm, st = emu_ctx.surface_centroids(x, SQUARE_V, SQUARE_F)
+    assert st["nb_split"] > 0  # 2 triangles, 300 cells: the seed lists overflow
+    _agree(g, mm, mg, m)
+    assert abs(mm.sum() - 1.0) < 1e-13
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_emu_surface_sphere(emu_ctx, weighted):
+    V, F = icosphere(4)
+    x = sphere_seeds(700, 6)
+    w = np.random.default_rng(7).random(len(V)) + 0.5 if weighted else None
+    mg, m, _ = oracle.surface_centroids(x, V, F, vw=w)
+    g, mm, st = emu_ctx.surface_centroids(x, V, F, w)
+    _agree(g, mm, mg, m)
+
+
+def test_emu_surface_far_seeds_need_longer_lists(emu_ctx):
+    # seeds in a small cluster far from the sphere: every cell is large
+    # where it meets the surface and the first lists are too short
+    rng = np.random.default_rng(8)
+    x = 3.0 + 0.1 * rng.random((200, 3))
+    V, F = icosphere(2)
+    mg, m, _ = oracle.surface_centroids(x, V, F)
+    g, mm, st = emu_ctx.surface_centroids(x, V, F)
+    assert st["nb_redo"] > 0 and st["k_final"] > 32
+    _agree(g, mm, mg, m)
+
+
+def test_emu_surface_rvd_mirror(emu_ctx):
+    V, F = icosphere(3)
+    x = sphere_seeds(100, 9)
+    rvd = ga.RestrictedVoronoiDiagram(x, ctx=emu_ctx, surface=(V, F))
+    assert not rvd.volumetric()
+    mg = np.zeros((100, 3))
+    m = np.zeros(100)
+    rvd.compute_centroids(mg, m)
+    rvd.compute_centroids(mg, m)  # accumulates
+    mgo, mo, _ = oracle.surface_centroids(x, V, F)
+    _agree(mg / 2, m / 2, mgo, mo)
+
+
+def test_surface_bad_triangle_index(emu_ctx):
+    with pytest.raises(ga.VgcError):
+        emu_ctx.surface_centroids(np.zeros((2, 3)) + [[0, 0, 0], [1, 0, 0]], SQUARE_V,
+                                  np.array([[0, 1, 4]], np.int32))
+
+
+# --------------------------------------------------------------------- GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("weighted", [False, True])
+def test_gpu_surface_sphere_vs_oracle(gpu_ctx, emu_ctx, weighted):
+    V, F = icosphere(6)
+    x = sphere_seeds(20000, 10)
+    w = np.random.default_rng(11).random(len(V)) + 0.5 if weighted else None
+    print(f"surface sphere {len(F)} triangles, 20000 seeds, weighted={weighted}", flush=True)
+    mg, m, _ = oracle.surface_centroids(x, V, F, vw=w)
+    g, mm, st = gpu_ctx.surface_centroids(x, V, F, w)
+    _agree(g, mm, mg, m)
+    ge, me, _ = emu_ctx.surface_centroids(x, V, F, w)
+    assert np.array_equal(ge, g) and np.array_equal(me, mm)
+
+
+@pytest.mark.gpu
+def test_gpu_surface_splits_and_retries_vs_oracle(gpu_ctx):
+    rng = np.random.default_rng(12)
+    x = np.c_[rng.random((3000, 2)), 0.5 + 0.05 * rng.standard_normal(3000)]
+    mg, m, _ = oracle.surface_centroids(x, SQUARE_V, SQUARE_F)
+    g, mm, st = gpu_ctx.surface_centroids(x, SQUARE_V, SQUARE_F)
+    assert st["nb_split"] > 0
+    _agree(g, mm, mg, m)
+    xf = 3.0 + 0.1 * rng.random((500, 3))
+    V, F = icosphere(3)
+    mg, m, _ = oracle.surface_centroids(xf, V, F)
+    g, mm, st = gpu_ctx.surface_centroids(xf, V, F)
+    assert st["nb_redo"] > 0
+    _agree(g, mm, mg, m)
