@@ -53,20 +53,10 @@ def profile_figure(key):
     """A `key: value` line of the rocprofv3 counter summary of the clip
     kernel (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE scaled by a
     known-size copy, VALU issue fraction from SQ counters) — None unless the
-    summary records the hash of the kernel sources this build compiles
-    (kernel_sources_sha1), so a stale profile is never reported."""
-    vals = {}
-    try:
-        with open(COUNTERS) as f:
-            for line in f:
-                if ":" in line and not line.startswith(("|", "#", " ")):
-                    k, v = line.split(":", 1)
-                    vals[k.strip()] = v.strip()
-    except OSError:
-        return None
-    if vals.get("kernel_sources_sha1") != kernel_sources_sha1() or key not in vals:
-        return None
-    return float(vals[key])
+    summary records the hash of the kernel sources this build compiles, so a
+    stale profile is never reported."""
+    from geogram_amd.profiling import read_figure
+    return read_figure(COUNTERS, key)
 
 
 def cell_errors(np, g, o, rows):
@@ -125,16 +115,6 @@ def reference_mass_error(np, x, w, nu):
         return None
     m = oracle.ref_cells(x, w, threads=min(16, os.cpu_count() or 1))["vol"]
     return float(np.max(np.abs(m - nu) / nu))
-
-
-def kernel_sources_sha1():
-    """Hash of the clipping kernel's sources (the profiled code)."""
-    import hashlib
-    h = hashlib.sha1()
-    for f in ("clip_cell.h", "wave.h", "types.h", "common.h", "exact_det.h", "backend_hip.hip"):
-        with open(os.path.join(ROOT, "geogram_amd", "csrc", f), "rb") as fh:
-            h.update(fh.read())
-    return h.hexdigest()
 
 
 def main():

@@ -422,30 +422,60 @@ struct EmuBackend {
         const double bb = red_sum(n, [&](int i) { return r[i] * r[i]; });
         cg[0] = rz;
         cg[1] = bb;
+        cg[4] = 0.0;
         cg[5] = tol2 * bb;
         cg[6] = bb > 0.0 ? 0.0 : 1.0;
         cg[7] = 0.0;
     }
     void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
-                  const double* diag, double* x, double* r, double* z, double* p, double* q,
-                  double* cg) {
-        for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
+                  const double* diag, double* x, double* r, double* z, double* p, double* p2,
+                  double* q, double* cg) {
+        for (int i = 0; i < k; ++i)
+            cg_iter(n, ptr, col, val, diag, x, r, z, i % 2 ? p2 : p, i % 2 ? p : p2, q, cg);
     }
+    // the two PCG kernels, blocks and reduction orders as on the device
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
-                 const double* diag, double* x, double* r, double* z, double* p, double* q,
-                 double* cg) {
+                 const double* diag, double* x, double* r, double* z, const double* po, double* pn,
+                 double* q, double* cg) {
         if (cg[6] != 0.0) return;
-        for (int i = 0; i < n; ++i) {
-            double s[kSpmvLanes];
-            for (int l = 0; l < kSpmvLanes; ++l) s[l] = ot_spmv_lane(i, l, ptr, col, val, p);
-            for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
-                double t[kSpmvLanes];
-                for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
-                for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
-            }
-            q[i] = s[0] + diag[i] * p[i];
+        const double beta = cg[4];
+        std::vector<double> part(2 * kCgBlocks);
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < kCgBlocks; ++b) {
+            double acc[256] = {};
+            for (long long r0 = (long long)b * kCgRows; r0 < n; r0 += (long long)kCgBlocks * kCgRows)
+                for (int g = 0; g < kCgRows; ++g) {
+                    const long long i = r0 + g;
+                    if (i >= n) break;
+                    const double pi = z[i] + beta * po[i];
+                    double s[kSpmvLanes];
+                    for (int l = 0; l < kSpmvLanes; ++l) {
+                        double a = 0.0;
+                        for (long long k = ptr[i] + l; k < ptr[i + 1]; k += kSpmvLanes) {
+                            const int j = col[k];
+                            a += val[k] * (z[j] + beta * po[j]);
+                        }
+                        s[l] = a;
+                    }
+                    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                        double t[kSpmvLanes];
+                        for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
+                        for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
+                    }
+                    const double qi = s[0] + diag[i] * pi;
+                    q[i] = qi;
+                    pn[i] = pi;
+                    acc[g * kSpmvLanes] += pi * qi;
+                }
+            part[b] = tree_sum(acc);
         }
-        const double pq = red_sum(n, [&](int i) { return p[i] * q[i]; });
+        double sh[256];
+        for (int t = 0; t < 256; ++t) {
+            double a = 0.0;
+            for (int i = t; i < kCgBlocks; i += 256) a += part[i];
+            sh[t] = a;
+        }
+        const double pq = tree_sum(sh);
         cg[2] = pq;
         if (!(pq > 0.0)) {
             cg[3] = 0.0;
@@ -453,9 +483,38 @@ struct EmuBackend {
             return;
         }
         cg[3] = cg[0] / pq;
-        ot_pcg_update(n, cg[3], p, q, diag, x, r, z);
-        const double rz = red_sum(n, [&](int i) { return r[i] * z[i]; });
-        const double rr = red_sum(n, [&](int i) { return r[i] * r[i]; });
+        const double alpha = cg[3];
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < kCgBlocks; ++b) {
+            double a1[256], a2[256];
+            for (int t = 0; t < 256; ++t) {
+                double s1 = 0.0, s2 = 0.0;
+                for (long long i = (long long)b * 256 + t; i < n; i += (long long)kCgBlocks * 256) {
+                    x[i] += alpha * pn[i];
+                    const double ri = r[i] - alpha * q[i];
+                    r[i] = ri;
+                    const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+                    z[i] = zi;
+                    s1 += ri * zi;
+                    s2 += ri * ri;
+                }
+                a1[t] = s1;
+                a2[t] = s2;
+            }
+            part[b] = tree_sum(a1);
+            part[kCgBlocks + b] = tree_sum(a2);
+        }
+        double sh2[256];
+        for (int t = 0; t < 256; ++t) {
+            double a = 0.0, c = 0.0;
+            for (int i = t; i < kCgBlocks; i += 256) {
+                a += part[i];
+                c += part[kCgBlocks + i];
+            }
+            sh[t] = a;
+            sh2[t] = c;
+        }
+        const double rz = tree_sum(sh), rr = tree_sum(sh2);
         cg[1] = rr;
         cg[7] += 1.0;
         if (rr <= cg[5]) {
@@ -465,7 +524,6 @@ struct EmuBackend {
         }
         cg[4] = rz / cg[0];
         cg[0] = rz;
-        ot_pcg_dir(n, cg[4], z, p);
     }
 
     // --------------------------------------------------------- batched OT
@@ -558,38 +616,55 @@ struct EmuBackend {
             double* c = cg + 8 * k;
             c[0] = t1;
             c[1] = t2;
+            c[4] = 0.0;
             c[5] = tol2[k] * t2;
             c[6] = t2 > 0.0 ? 0.0 : 1.0;
             c[7] = 0.0;
         }
     }
-    void cgb_iters(int kk, const SegTable& T, int n, const long long* ptr, const int* col,
+    // the two batched PCG kernels, block table and reduction orders as on
+    // the device
+    void cgb_iters(int kk, const SegTable& T, int, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
-                   double* p, double* q, double* cg) {
+                   double* p, double* p2, double* q, double* cg) {
         seg_part_.assign(2 * (size_t)T.nblk, 0.0);
         for (int it = 0; it < kk; ++it) {
-            for (int i = 0; i < n; ++i) {
-                if (cg[8 * T.inst_of[i] + 6] != 0.0) continue;
-                double s[kSpmvLanes];
-                for (int l = 0; l < kSpmvLanes; ++l) s[l] = ot_spmv_lane(i, l, ptr, col, val, p);
-                for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
-                    double t[kSpmvLanes];
-                    for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
-                    for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
-                }
-                q[i] = s[0] + diag[i] * p[i];
-            }
+            const double* po = it % 2 ? p2 : p;
+            double* pn = it % 2 ? p : p2;
+#pragma omp parallel for schedule(dynamic, 4)
             for (int blk = 0; blk < T.nblk; ++blk) {
                 const int k = T.blk_inst[blk];
-                double t1 = 0.0, t2 = 0.0;
-                if (cg[8 * k + 6] == 0.0)
-                    seg_block_sums(T, blk, [&](long long i, double& s1, double&) { s1 += p[i] * q[i]; },
-                                   t1, t2);
-                else {
-                    double a[256] = {};
-                    t1 = tree_sum(a);
-                }
-                seg_part_[blk] = t1;
+                const double* c = cg + 8 * k;
+                if (c[6] != 0.0) continue;
+                const double beta = c[4];
+                const long long end = T.off[k + 1], stride = (long long)T.cnt[k] * kCgRows;
+                double acc[256] = {};
+                for (long long r0 = T.off[k] + (long long)T.blk_idx[blk] * kCgRows; r0 < end;
+                     r0 += stride)
+                    for (int g = 0; g < kCgRows; ++g) {
+                        const long long i = r0 + g;
+                        if (i >= end) break;
+                        const double pi = z[i] + beta * po[i];
+                        double sl[kSpmvLanes];
+                        for (int l = 0; l < kSpmvLanes; ++l) {
+                            double a = 0.0;
+                            for (long long e = ptr[i] + l; e < ptr[i + 1]; e += kSpmvLanes) {
+                                const int j = col[e];
+                                a += val[e] * (z[j] + beta * po[j]);
+                            }
+                            sl[l] = a;
+                        }
+                        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                            double t[kSpmvLanes];
+                            for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
+                            for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
+                        }
+                        const double qi = sl[0] + diag[i] * pi;
+                        q[i] = qi;
+                        pn[i] = pi;
+                        acc[g * kSpmvLanes] += pi * qi;
+                    }
+                seg_part_[blk] = tree_sum(acc);
             }
             for (int k = 0; k < T.n_inst; ++k) {
                 double* c = cg + 8 * k;
@@ -604,21 +679,21 @@ struct EmuBackend {
                     c[3] = c[0] / pq;
                 }
             }
+#pragma omp parallel for schedule(dynamic, 4)
             for (int blk = 0; blk < T.nblk; ++blk) {
                 const int k = T.blk_inst[blk];
-                double t1 = 0.0, t2 = 0.0;
-                if (cg[8 * k + 6] == 0.0) {
-                    const double alpha = cg[8 * k + 3];
-                    seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
-                        x[i] += alpha * p[i];
-                        const double ri = r[i] - alpha * q[i];
-                        r[i] = ri;
-                        const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
-                        z[i] = zi;
-                        s1 += ri * zi;
-                        s2 += ri * ri;
-                    }, t1, t2);
-                }
+                if (cg[8 * k + 6] != 0.0) continue;
+                const double alpha = cg[8 * k + 3];
+                double t1, t2;
+                seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
+                    x[i] += alpha * pn[i];
+                    const double ri = r[i] - alpha * q[i];
+                    r[i] = ri;
+                    const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+                    z[i] = zi;
+                    s1 += ri * zi;
+                    s2 += ri * ri;
+                }, t1, t2);
                 seg_part_[blk] = t1;
                 seg_part_[T.nblk + blk] = t2;
             }
@@ -636,11 +711,6 @@ struct EmuBackend {
                     c[4] = rz / c[0];
                     c[0] = rz;
                 }
-            }
-            for (int i = 0; i < n; ++i) {
-                const double* c = cg + 8 * T.inst_of[i];
-                if (c[6] != 0.0) continue;
-                p[i] = z[i] + c[4] * p[i];
             }
         }
     }

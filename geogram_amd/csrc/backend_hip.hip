@@ -727,45 +727,76 @@ __global__ __launch_bounds__(256) void k_cg_start_final(const double* __restrict
     if (threadIdx.x == 0) {
         cg[0] = t1;
         cg[1] = t2;
+        cg[4] = 0.0;
         cg[5] = tol2 * t2;
         cg[6] = t2 > 0.0 ? 0.0 : 1.0;
         cg[7] = 0.0;
     }
 }
 
-// q = A p, kSpmvLanes lanes per row
-__global__ __launch_bounds__(256) void k_cg_spmv(int n, const long long* __restrict__ ptr,
-                                                 const int* __restrict__ col,
-                                                 const double* __restrict__ val,
-                                                 const double* __restrict__ diag,
-                                                 const double* __restrict__ p, double* q,
-                                                 const double* __restrict__ cg) {
+// The last of a grid's blocks to finish, after each block's thread 0 wrote
+// its partial: the block partials are then visible to it (fence, counter,
+// fence), and it resets the counter for the next launch.
+__device__ __forceinline__ bool cg_last_block(unsigned* ctr) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+__device__ __forceinline__ double cg_read_part(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// PCG iteration, first kernel: the direction pn = z + beta po of every row
+// (beta = cg[4]; the rows' neighbours' entries formed on the fly, so the
+// product needs no separate direction pass), q = A pn, and p.q; the last
+// block sets alpha = rz / p.q (cg[3]) or the breakdown flag.
+__global__ __launch_bounds__(256) void k_cgf_dir_spmv(int n, const long long* __restrict__ ptr,
+                                                      const int* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ diag,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ po, double* pn,
+                                                      double* q, double* cg, double* part,
+                                                      unsigned* ctr) {
+    __shared__ double sh[256];
     if (cg[6] != 0.0) return;  // uniform over the grid
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
-    double s = i < n ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+    const double beta = cg[4];
+    const int g = threadIdx.x / kSpmvLanes, l = threadIdx.x % kSpmvLanes;
+    double acc = 0.0;
+    for (long long r0 = (long long)blockIdx.x * kCgRows; r0 < n;
+         r0 += (long long)kCgBlocks * kCgRows) {
+        const long long i = r0 + g;
+        double s = 0.0, pi = 0.0;
+        if (i < n) {
+            pi = z[i] + beta * po[i];
+            for (long long k = ptr[i] + l; k < ptr[i + 1]; k += kSpmvLanes) {
+                const int j = col[k];
+                s += val[k] * (z[j] + beta * po[j]);
+            }
+        }
 #pragma unroll
-    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    if (i < n && l == 0) q[i] = s + diag[i] * p[i];
-}
-
-__global__ __launch_bounds__(256) void k_cg_dot_pq(int n, const double* __restrict__ p,
-                                                   const double* __restrict__ q,
-                                                   const double* __restrict__ cg, double* part) {
-    __shared__ double sh[256];
-    if (cg[6] != 0.0) return;
+        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        if (i < n && l == 0) {
+            const double qi = s + diag[i] * pi;
+            q[i] = qi;
+            pn[i] = pi;
+            acc += pi * qi;
+        }
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    if (!cg_last_block(ctr)) return;
+    __syncthreads();
     double s = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += p[i] * q[i];
-    double tt = block_sum(s, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = tt;
-}
-
-__global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ part, double* cg) {
-    __shared__ double sh[256];
-    if (cg[6] != 0.0) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < kRedBlocks; i += 256) s += part[i];
-    double pq = block_sum(s, sh);
+    for (int i = threadIdx.x; i < kCgBlocks; i += 256) s += cg_read_part(part + i);
+    const double pq = block_sum(s, sh);
     if (threadIdx.x == 0) {
         cg[2] = pq;
         if (!(pq > 0.0)) {
@@ -774,21 +805,25 @@ __global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ par
         } else {
             cg[3] = cg[0] / pq;
         }
+        *ctr = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_cg_update_dots(int n, const double* __restrict__ p,
-                                                        const double* __restrict__ q,
-                                                        const double* __restrict__ diag, double* x,
-                                                        double* r, double* z,
-                                                        const double* __restrict__ cg,
-                                                        double* part) {
+// PCG iteration, second kernel: x += alpha pn, r -= alpha q, z = r / diag,
+// r.z and r.r; the last block counts the iteration, stops on
+// r.r <= tol^2 b.b, else sets beta = r.z / previous r.z (cg[4]).
+__global__ __launch_bounds__(256) void k_cgf_update(int n, const double* __restrict__ pn,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ diag, double* x,
+                                                    double* r, double* z, double* cg,
+                                                    double* part, unsigned* ctr) {
     __shared__ double sh[256], sh2[256];
     if (cg[6] != 0.0) return;
     const double alpha = cg[3];
     double s1 = 0.0, s2 = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
-        x[i] += alpha * p[i];
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)kCgBlocks * 256) {
+        x[i] += alpha * pn[i];
         const double ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
@@ -800,17 +835,14 @@ __global__ __launch_bounds__(256) void k_cg_update_dots(int n, const double* __r
     block_sum2(s1, s2, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = t1;
-        part[kRedBlocks + blockIdx.x] = t2;
+        part[kCgBlocks + blockIdx.x] = t2;
     }
-}
-
-__global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ part, double* cg) {
-    __shared__ double sh[256], sh2[256];
-    if (cg[6] != 0.0) return;
-    double s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
-        s1 += part[i];
-        s2 += part[kRedBlocks + i];
+    if (!cg_last_block(ctr)) return;
+    __syncthreads();
+    s1 = s2 = 0.0;
+    for (int i = threadIdx.x; i < kCgBlocks; i += 256) {
+        s1 += cg_read_part(part + i);
+        s2 += cg_read_part(part + kCgBlocks + i);
     }
     double rz, rr;
     block_sum2(s1, s2, sh, sh2, rz, rr);
@@ -824,15 +856,8 @@ __global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ part
             cg[4] = rz / cg[0];
             cg[0] = rz;
         }
+        *ctr = 0u;
     }
-}
-
-__global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
-                         const double* __restrict__ cg) {
-    if (cg[6] != 0.0) return;
-    const double beta = cg[4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
 }
 
 
@@ -944,51 +969,75 @@ __global__ __launch_bounds__(256) void k_cgb_start_final(SegTable T, const doubl
         double* c = cg + 8 * k;
         c[0] = t1;
         c[1] = t2;
+        c[4] = 0.0;
         c[5] = tol2[k] * t2;
         c[6] = t2 > 0.0 ? 0.0 : 1.0;
         c[7] = 0.0;
     }
 }
 
-__global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__ inst_of,
-                                                  const long long* __restrict__ ptr,
-                                                  const int* __restrict__ col,
-                                                  const double* __restrict__ val,
-                                                  const double* __restrict__ diag,
-                                                  const double* __restrict__ p, double* q,
-                                                  const double* __restrict__ cg) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
-    const bool live = i < n && cg[8 * inst_of[i < n ? i : 0] + 6] == 0.0;
-    double s = live ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
-#pragma unroll
-    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    if (live && l == 0) q[i] = s + diag[i] * p[i];
+// The last of instance k's cnt[k] blocks to finish (per-instance counter;
+// reset by that block for the next launch).
+__device__ __forceinline__ bool cgb_last_block(unsigned* ctr, int cnt) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(ctr, 1u) == (unsigned)cnt - 1u;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
 }
 
-__global__ __launch_bounds__(256) void k_cgb_dot_pq(SegTable T, const double* __restrict__ p,
-                                                    const double* __restrict__ q,
-                                                    const double* __restrict__ cg, double* part) {
+// Batched PCG iteration, first kernel (one block table, each block in one
+// instance; instances already stopped skip): pn = z + beta_k po, q = A pn,
+// p.q per instance; the instance's last block sets its alpha.
+__global__ __launch_bounds__(256) void k_cgbf_dir_spmv(SegTable T, const long long* __restrict__ ptr,
+                                                       const int* __restrict__ col,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ diag,
+                                                       const double* __restrict__ z,
+                                                       const double* __restrict__ po, double* pn,
+                                                       double* q, double* cg, double* part,
+                                                       unsigned* ctr) {
     __shared__ double sh[256], sh2[256];
-    int k;
-    long long beg, end, stride;
-    seg_range(T, blockIdx.x, k, beg, end, stride);
-    double s = 0.0;
-    if (cg[8 * k + 6] == 0.0)
-        for (long long i = beg; i < end; i += stride) s += p[i] * q[i];
-    double t1, t2;
-    block_sum2(s, 0.0, sh, sh2, t1, t2);
-    if (threadIdx.x == 0) part[blockIdx.x] = t1;
-}
-
-__global__ __launch_bounds__(256) void k_cgb_alpha(SegTable T, const double* __restrict__ part,
-                                                   double* cg) {
-    __shared__ double sh[256], sh2[256];
-    const int k = blockIdx.x;
+    const int k = T.blk_inst[blockIdx.x];
     double* c = cg + 8 * k;
-    if (c[6] != 0.0) return;  // uniform over the block
+    if (c[6] != 0.0) return;  // uniform over the block and the instance
+    const double beta = c[4];
+    const int g = threadIdx.x / kSpmvLanes, l = threadIdx.x % kSpmvLanes;
+    const long long end = T.off[k + 1], stride = (long long)T.cnt[k] * kCgRows;
+    double acc = 0.0;
+    for (long long r0 = T.off[k] + (long long)T.blk_idx[blockIdx.x] * kCgRows; r0 < end;
+         r0 += stride) {
+        const long long i = r0 + g;
+        double s = 0.0, pi = 0.0;
+        if (i < end) {
+            pi = z[i] + beta * po[i];
+            for (long long e = ptr[i] + l; e < ptr[i + 1]; e += kSpmvLanes) {
+                const int j = col[e];
+                s += val[e] * (z[j] + beta * po[j]);
+            }
+        }
+#pragma unroll
+        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        if (i < end && l == 0) {
+            const double qi = s + diag[i] * pi;
+            q[i] = qi;
+            pn[i] = pi;
+            acc += pi * qi;
+        }
+    }
+    double t1, t2;
+    block_sum2(acc, 0.0, sh, sh2, t1, t2);
+    if (threadIdx.x == 0) part[blockIdx.x] = t1;
+    if (!cgb_last_block(ctr + k, T.cnt[k])) return;
+    __syncthreads();
+    const int cnt = T.cnt[k], f = T.first[k];
+    const double a = (int)threadIdx.x < cnt ? cg_read_part(part + f + threadIdx.x) : 0.0;
     double pq, dummy;
-    seg_final2(T, k, part, sh, sh2, pq, dummy);
+    block_sum2(a, 0.0, sh, sh2, pq, dummy);
     if (threadIdx.x == 0) {
         c[2] = pq;
         if (!(pq > 0.0)) {
@@ -997,31 +1046,34 @@ __global__ __launch_bounds__(256) void k_cgb_alpha(SegTable T, const double* __r
         } else {
             c[3] = c[0] / pq;
         }
+        ctr[k] = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_cgb_update_dots(SegTable T, const double* __restrict__ p,
-                                                         const double* __restrict__ q,
-                                                         const double* __restrict__ diag,
-                                                         double* x, double* r, double* z,
-                                                         const double* __restrict__ cg,
-                                                         double* part) {
+// Batched PCG iteration, second kernel: the update and the two dots per
+// instance; the instance's last block counts the iteration and sets beta
+// or stops it.
+__global__ __launch_bounds__(256) void k_cgbf_update(SegTable T, const double* __restrict__ pn,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ diag, double* x,
+                                                     double* r, double* z, double* cg,
+                                                     double* part, unsigned* ctr) {
     __shared__ double sh[256], sh2[256];
     int k;
     long long beg, end, stride;
     seg_range(T, blockIdx.x, k, beg, end, stride);
+    double* c = cg + 8 * k;
+    if (c[6] != 0.0) return;
+    const double alpha = c[3];
     double s1 = 0.0, s2 = 0.0;
-    if (cg[8 * k + 6] == 0.0) {
-        const double alpha = cg[8 * k + 3];
-        for (long long i = beg; i < end; i += stride) {
-            x[i] += alpha * p[i];
-            const double ri = r[i] - alpha * q[i];
-            r[i] = ri;
-            const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
-            z[i] = zi;
-            s1 += ri * zi;
-            s2 += ri * ri;
-        }
+    for (long long i = beg; i < end; i += stride) {
+        x[i] += alpha * pn[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+        z[i] = zi;
+        s1 += ri * zi;
+        s2 += ri * ri;
     }
     double t1, t2;
     block_sum2(s1, s2, sh, sh2, t1, t2);
@@ -1029,16 +1081,13 @@ __global__ __launch_bounds__(256) void k_cgb_update_dots(SegTable T, const doubl
         part[blockIdx.x] = t1;
         part[T.nblk + blockIdx.x] = t2;
     }
-}
-
-__global__ __launch_bounds__(256) void k_cgb_beta(SegTable T, const double* __restrict__ part,
-                                                  double* cg) {
-    __shared__ double sh[256], sh2[256];
-    const int k = blockIdx.x;
-    double* c = cg + 8 * k;
-    if (c[6] != 0.0) return;
+    if (!cgb_last_block(ctr + T.n_inst + k, T.cnt[k])) return;
+    __syncthreads();
+    const int cnt = T.cnt[k], f = T.first[k];
+    const double a = (int)threadIdx.x < cnt ? cg_read_part(part + f + threadIdx.x) : 0.0;
+    const double b = (int)threadIdx.x < cnt ? cg_read_part(part + T.nblk + f + threadIdx.x) : 0.0;
     double rz, rr;
-    seg_final2(T, k, part, sh, sh2, rz, rr);
+    block_sum2(a, b, sh, sh2, rz, rr);
     if (threadIdx.x == 0) {
         c[1] = rr;
         c[7] += 1.0;
@@ -1049,6 +1098,7 @@ __global__ __launch_bounds__(256) void k_cgb_beta(SegTable T, const double* __re
             c[4] = rz / c[0];
             c[0] = rz;
         }
+        ctr[T.n_inst + k] = 0u;
     }
 }
 
@@ -1064,15 +1114,6 @@ __global__ __launch_bounds__(256) void k_seg_center(SegTable T, double* b) {
     block_sum2(s, 0.0, sh, sh2, t, dummy);
     const double mean = t / double(end - beg);
     for (long long i = beg + threadIdx.x; i < end; i += 256) b[i] -= mean;
-}
-
-__global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* __restrict__ z,
-                          double* p, const double* __restrict__ cg) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* c = cg + 8 * inst_of[i];
-    if (c[6] != 0.0) return;
-    p[i] = z[i] + c[4] * p[i];
 }
 
 __global__ void k_set0_i64(long long* p) { *p = 0; }
@@ -1131,6 +1172,9 @@ struct HipBackend {
         if (temp_) (void)hipFree(temp_);
         if (red_) (void)hipFree(red_);
         if (vdot_) (void)hipFree(vdot_);
+        if (cg_part_) (void)hipFree(cg_part_);
+        if (cg_ctr_) (void)hipFree(cg_ctr_);
+        if (cgb_ctr_) (void)hipFree(cgb_ctr_);
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
@@ -1486,40 +1530,51 @@ struct HipBackend {
         check();
     }
     // k iterations as one hipGraph, captured once per argument set and
-    // replayed until the solve ends (the kernels read their scalars from cg)
+    // replayed until the solve ends (the kernels read their scalars from cg).
+    // Iteration i reads the direction from p (i even) or p2 and writes the
+    // other; cg_start leaves it in p, and k is even except for a solve's last
+    // chunk.
     void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
-                  const double* diag, double* x, double* r, double* z, double* p, double* q,
-                  double* cg) {
-        const void* key[12] = {(const void*)(long long)k, (const void*)(long long)n, ptr, col, val,
-                               diag, x, r, z, p, q, cg};
+                  const double* diag, double* x, double* r, double* z, double* p, double* p2,
+                  double* q, double* cg) {
+        const void* key[13] = {(const void*)(long long)k, (const void*)(long long)n, ptr, col, val,
+                               diag, x, r, z, p, p2, q, cg};
         bool same = cg_exec_ != nullptr;
-        for (int i = 0; i < 12 && same; ++i) same = cg_key_[i] == key[i];
+        for (int i = 0; i < 13 && same; ++i) same = cg_key_[i] == key[i];
         if (!same) {
             if (cg_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cg_exec_));
             cg_exec_ = nullptr;
-            red_part();  // allocated outside the capture
+            cg_part();  // allocated outside the capture
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
+            for (int i = 0; i < k; ++i)
+                cg_iter(n, ptr, col, val, diag, x, r, z, i % 2 ? p2 : p, i % 2 ? p : p2, q, cg);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cg_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
-            for (int i = 0; i < 12; ++i) cg_key_[i] = key[i];
+            for (int i = 0; i < 13; ++i) cg_key_[i] = key[i];
         }
         VGC_HIP_CHECK(hipGraphLaunch(cg_exec_, stream_));
     }
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
-                 const double* diag, double* x, double* r, double* z, double* p, double* q,
-                 double* cg) {
-        double* part = red_part();
-        k_cg_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(n, ptr, col, val,
-                                                                                   diag, p, q, cg);
-        k_cg_dot_pq<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, cg, part);
-        k_cg_alpha<<<1, 256, 0, stream_>>>(part, cg);
-        k_cg_update_dots<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part);
-        k_cg_beta<<<1, 256, 0, stream_>>>(part, cg);
-        k_cg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, z, p, cg);
+                 const double* diag, double* x, double* r, double* z, const double* po, double* pn,
+                 double* q, double* cg) {
+        double* part = cg_part();
+        k_cgf_dir_spmv<<<kCgBlocks, 256, 0, stream_>>>(n, ptr, col, val, diag, z, po, pn, q, cg,
+                                                       part, cg_ctr_);
+        k_cgf_update<<<kCgBlocks, 256, 0, stream_>>>(n, pn, q, diag, x, r, z, cg, part,
+                                                     cg_ctr_ + 1);
         check();
+    }
+    // block partials and the two last-block counters of the PCG kernels
+    // (zeroed once here; each last block resets its counter)
+    double* cg_part() {
+        if (!cg_part_) {
+            VGC_HIP_CHECK(hipMalloc(&cg_part_, sizeof(double) * 2 * kCgBlocks));
+            VGC_HIP_CHECK(hipMalloc(&cg_ctr_, sizeof(unsigned) * 2));
+            VGC_HIP_CHECK(hipMemsetAsync(cg_ctr_, 0, sizeof(unsigned) * 2, stream_));
+        }
+        return cg_part_;
     }
 
     // --------------------------------------------------------- batched OT
@@ -1558,43 +1613,55 @@ struct HipBackend {
         check();
     }
     // k batched CG iterations, captured once as a graph (for these buffers and
-    // this table) and replayed
+    // this table) and replayed; the direction alternates between p and p2 as
+    // in cg_iters
     void cgb_iters(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
-                   double* p, double* q, double* cg) {
+                   double* p, double* p2, double* q, double* cg) {
         double* part = seg_part(T.nblk);  // allocated outside the capture
-        const void* key[21] = {(const void*)(long long)k, (const void*)(long long)n,
+        unsigned* ctr = cgb_ctr(T.n_inst);
+        const void* key[23] = {(const void*)(long long)k, (const void*)(long long)n,
                                (const void*)(long long)T.n_inst, (const void*)(long long)T.nblk,
                                T.off, T.blk_inst, T.blk_idx, T.cnt, T.first, T.inst_of, ptr, col,
-                               val, diag, x, r, z, p, q, cg, part};
+                               val, diag, x, r, z, p, p2, q, cg, part, ctr};
         bool same = cgb_exec_ != nullptr;
-        for (int i = 0; i < 21 && same; ++i) same = cgb_key_[i] == key[i];
+        for (int i = 0; i < 23 && same; ++i) same = cgb_key_[i] == key[i];
         if (!same) {
             if (cgb_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cgb_exec_));
             cgb_exec_ = nullptr;
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, q, cg, part);
+            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, p2, q, cg, part, ctr);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cgb_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
-            for (int i = 0; i < 21; ++i) cgb_key_[i] = key[i];
+            for (int i = 0; i < 23; ++i) cgb_key_[i] = key[i];
         }
         VGC_HIP_CHECK(hipGraphLaunch(cgb_exec_, stream_));
     }
-    void cgb_launch(int k, const SegTable& T, int n, const long long* ptr, const int* col,
+    void cgb_launch(int k, const SegTable& T, int, const long long* ptr, const int* col,
                     const double* val, const double* diag, double* x, double* r, double* z,
-                    double* p, double* q, double* cg, double* part) {
+                    double* p, double* p2, double* q, double* cg, double* part, unsigned* ctr) {
         for (int it = 0; it < k; ++it) {
-            k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
-                n, T.inst_of, ptr, col, val, diag, p, q, cg);
-            k_cgb_dot_pq<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part);
-            k_cgb_alpha<<<T.n_inst, 256, 0, stream_>>>(T, part, cg);
-            k_cgb_update_dots<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part);
-            k_cgb_beta<<<T.n_inst, 256, 0, stream_>>>(T, part, cg);
-            k_cgb_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, T.inst_of, z, p, cg);
+            const double* po = it % 2 ? p2 : p;
+            double* pn = it % 2 ? p : p2;
+            k_cgbf_dir_spmv<<<T.nblk, 256, 0, stream_>>>(T, ptr, col, val, diag, z, po, pn, q, cg,
+                                                         part, ctr);
+            k_cgbf_update<<<T.nblk, 256, 0, stream_>>>(T, pn, q, diag, x, r, z, cg, part, ctr);
         }
         check();
+    }
+    // per-instance last-block counters of the batched PCG kernels (two per
+    // instance), zeroed when allocated; each last block resets its own
+    unsigned* cgb_ctr(int n_inst) {
+        if (cgb_ctr_cap_ < n_inst) {
+            if (cgb_ctr_) VGC_HIP_CHECK(hipFree(cgb_ctr_));
+            VGC_HIP_CHECK(hipMalloc(&cgb_ctr_, sizeof(unsigned) * 2 * (size_t)n_inst));
+            VGC_HIP_CHECK(hipMemsetAsync(cgb_ctr_, 0, sizeof(unsigned) * 2 * (size_t)n_inst,
+                                         stream_));
+            cgb_ctr_cap_ = n_inst;
+        }
+        return cgb_ctr_;
     }
     // ---------------------------------------------------------------- MLP
     void mlp_forward(const float* P, const float* X, int n, float* out) {
@@ -1737,9 +1804,13 @@ private:
     double* red_ = nullptr;
     double* vdot_ = nullptr;
     hipGraphExec_t cg_exec_ = nullptr;
-    const void* cg_key_[12] = {};
+    const void* cg_key_[13] = {};
+    double* cg_part_ = nullptr;
+    unsigned* cg_ctr_ = nullptr;
     hipGraphExec_t cgb_exec_ = nullptr;
-    const void* cgb_key_[21] = {};
+    const void* cgb_key_[23] = {};
+    unsigned* cgb_ctr_ = nullptr;
+    int cgb_ctr_cap_ = 0;
     double* seg_part_ = nullptr;
     int seg_part_cap_ = 0;
     int waves_ = kDefaultClipWaves;

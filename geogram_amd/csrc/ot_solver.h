@@ -320,6 +320,7 @@ public:
             be_.cgb_start(T, d_b, d_diag, d_x, d_r, d_z, d_p, d_cg, d_tol2);
             std::vector<double> h(8 * (size_t)K);
             be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
+            double* d_p2 = (double*)slot(32, sizeof(double) * (size_t)n);
             int issued = 0;
             auto all_done = [&]() {
                 for (int k = 0; k < K; ++k)
@@ -328,8 +329,8 @@ public:
             };
             while (!all_done() && issued < cg_max) {
                 const int kk = std::min(kCgCheck, cg_max - issued);
-                be_.cgb_iters(kk, T, n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_x, d_r, d_z, d_p, d_q,
-                              d_cg);
+                be_.cgb_iters(kk, T, n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_x, d_r, d_z, d_p, d_p2,
+                              d_q, d_cg);
                 issued += kk;
                 be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
             }
@@ -413,12 +414,14 @@ private:
             const double* b, double* x, double* r, double* z, double* p, double* q, double* scal,
             double tol, int maxit) {
         be_.cg_start(n, b, diag, x, r, z, p, scal, tol * tol);
+        // second direction buffer (the iterations alternate between p and it)
+        double* p2 = (double*)slot(31, sizeof(double) * (size_t)n);
         int issued = 0;
         double h[8];
         be_.d2h(h, scal, sizeof(double) * 8);
         while (h[6] == 0.0 && issued < maxit) {
             const int k = std::min(kCgCheck, maxit - issued);
-            be_.cg_iters(k, n, ptr, col, val, diag, x, r, z, p, q, scal);
+            be_.cg_iters(k, n, ptr, col, val, diag, x, r, z, p, p2, q, scal);
             issued += k;
             be_.d2h(h, scal, sizeof(double) * 8);
         }

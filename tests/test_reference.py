@@ -41,7 +41,10 @@ def test_voronoi_cells_oracle_vs_reference(kind):
 def test_laguerre_cells_oracle_vs_reference():
     rng = np.random.default_rng(12)
     x = rng.random((8000, 3))
-    w = rng.random(8000) * 3e-4
+    # weights small enough that no cell is empty: with hidden seeds geogram's
+    # parallel triangulation may stop on them (has_empty_cells) or not,
+    # depending on the thread count
+    w = rng.random(8000) * 1e-4
     r = oracle.ref_cells(x, w)
     o = oracle.cells(x, w=w, k0=20)
     _agree(r, o, len(x))

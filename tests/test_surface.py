@@ -37,10 +37,13 @@ def _tri_areas(V, F):
     return 0.5 * np.linalg.norm(np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]), axis=1)
 
 
-def _agree(g, mm, mg, m, rel=1e-10):
+def _agree(g, mm, mg, m, rel=1e-10, extent=1.0):
+    """Masses to rel of the largest, first moments to rel of the largest
+    mass times the mesh's extent (a seed owning a whole sphere has a first
+    moment near 0), centroids to 1e-9."""
     scale = m.max()
     assert np.abs(mm - m).max() <= rel * scale
-    assert np.abs(g - mg).max() <= rel * np.abs(mg).max()
+    assert np.abs(g - mg).max() <= rel * max(np.abs(mg).max(), scale * extent)
     nz = m > 1e-9 * scale
     c1 = g[nz] / mm[nz, None]
     c0 = mg[nz] / m[nz, None]
