@@ -321,7 +321,7 @@ class Context:
 
     # ------------------------------------------------------------------- OT
     def ot_solve(self, points, nu, w0=None, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                 judge_start=False, verbose=False, inexact=False, periodic=False):
+                 judge_start=False, verbose=False, inexact=False, periodic=False, repair=False):
         """Laguerre weights w with cell masses nu (uniform density on box).
         Returns (w, stats); w0 is the start (None: zeros).  judge_start: keep
         w0 only if its mass residual beats the zero start's.  verbose: one
@@ -346,6 +346,7 @@ class Context:
                                      int(max_newton),
                                      (VGC_DEVICE_PTRS if dev else 0)
                                      | (_lib.VGC_OT_JUDGE_START if judge_start else 0)
+                                     | (_lib.VGC_OT_REPAIR if repair else 0)
                                      | (_lib.VGC_OT_VERBOSE if verbose else 0)
                                      | (_lib.VGC_OT_INEXACT if inexact else 0)
                                      | (_lib.VGC_PERIODIC if periodic else 0),
@@ -354,7 +355,7 @@ class Context:
         return w, st.as_dict()
 
     def ot_solve_levels(self, points, nu, levels, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                        verbose=False):
+                        verbose=False, inexact=False, repair=False):
         """Multilevel transport (optimize_levels): points in level order,
         levels = [0, ..., n] (see brio_levels).  Returns (w, per-level stats)."""
         dev = _is_torch_cuda(points)
@@ -377,7 +378,9 @@ class Context:
                                             lv.ctypes.data_as(ctypes.c_void_p), nl, float(tol),
                                             int(max_newton),
                                             (VGC_DEVICE_PTRS if dev else 0)
-                                            | (_lib.VGC_OT_VERBOSE if verbose else 0), sts)
+                                            | (_lib.VGC_OT_VERBOSE if verbose else 0)
+                                            | (_lib.VGC_OT_INEXACT if inexact else 0)
+                                            | (_lib.VGC_OT_REPAIR if repair else 0), sts)
         self._check(rc)
         return w, [sts[k].as_dict() for k in range(nl)]
 

@@ -30,6 +30,7 @@ VGC_AUTOTUNE = 32
 VGC_OT_INEXACT = 64
 VGC_WAVES4 = 128
 VGC_PERIODIC = 256
+VGC_OT_REPAIR = 512
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6
@@ -39,7 +40,7 @@ _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain n
 _STAT_F64 = "t_build_ms t_knn_ms t_clip_ms t_pass_bc_ms t_compact_ms t_total_ms".split()
 _STAT_F64B = "t_clip_a_gpu_ms t_knn_gpu_ms".split()
 _STAT_U64B = ("a_cells a_tests a_live a_new a_tets a_facets nb_fail_capacity why_rings why_walk "
-              "why_narrow nb_regather tuned_waves nb_periodic_retries").split()
+              "why_narrow nb_regather tuned_waves nb_periodic_retries nb_pieces").split()
 
 
 class Stats(ctypes.Structure):
@@ -52,7 +53,8 @@ class Stats(ctypes.Structure):
 
 class OtStats(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_uint32) for k in ("newton_iters", "cg_iters", "cell_passes", "halvings",
-                                                 "used_fallback_start", "converged")]
+                                                 "used_fallback_start", "converged",
+                                                 "repair_rounds", "pad")]
                 + [(k, ctypes.c_double) for k in ("err0", "final_err", "t_total_ms", "t_cells_ms",
                                                    "t_cg_ms")])
 

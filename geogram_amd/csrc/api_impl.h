@@ -83,6 +83,7 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->nb_regather = e.nb_regather;
     s->tuned_waves = (uint64_t)e.tuned_waves;
     s->nb_periodic_retries = e.nb_periodic_retries;
+    s->nb_pieces = e.nb_pieces;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
@@ -746,7 +747,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                           (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0,
-                          (flags & VGC_OT_INEXACT) != 0)) {
+                          (flags & VGC_OT_INEXACT) != 0, (flags & VGC_OT_REPAIR) != 0)) {
             ctx->err = "vgc_ot_solve: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
         }
@@ -759,6 +760,8 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
             stats->halvings = (uint32_t)os.halvings;
             stats->used_fallback_start = (uint32_t)os.used_fallback_start;
             stats->converged = (uint32_t)os.converged;
+            stats->repair_rounds = (uint32_t)os.repair_rounds;
+            stats->pad = 0;
             stats->err0 = os.err0;
             stats->final_err = os.final_err;
             stats->t_total_ms = os.t_total_ms;
@@ -783,6 +786,8 @@ inline void copy_ot_stats(const OtStats& os, vgc_ot_stats* s) {
     s->halvings = (uint32_t)os.halvings;
     s->used_fallback_start = (uint32_t)os.used_fallback_start;
     s->converged = (uint32_t)os.converged;
+    s->repair_rounds = (uint32_t)os.repair_rounds;
+    s->pad = 0;
     s->err0 = os.err0;
     s->final_err = os.final_err;
     s->t_total_ms = os.t_total_ms;
@@ -1184,7 +1189,8 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
             pb.d_xyz = dx;
             OtStats os;
             if (ctx->ot.solve(pb, dnl, dw, tol, (int)maxn, 1e-7, 4000, os, false,
-                              (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
+                              (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0,
+                              (flags & VGC_OT_REPAIR) != 0)) {
                 ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
                 return VGC_ERR_INTERNAL;
             }

@@ -240,6 +240,33 @@ struct EmuBackend {
         *d_count = k;
         return k;
     }
+    void piece_setup(int nq, const int* cells, const pt4* pts, const int* inst_of,
+                     const double* box, double* p_box) {
+        for (int q = 0; q < nq; ++q) piece_setup_row(q, cells, pts, inst_of, box, p_box);
+    }
+    void piece_merge(int m, const int* cells, const int* orig, const long long* pptr,
+                     const int* pidx, const double* p_vol, const double* p_mg, const double* p_e2,
+                     const int* p_cnt, const int* p_nbr, const double* p_area,
+                     const uint32_t* p_flags, int stride, const long long* x_off, double* vol,
+                     double* mg, double* e2, int* nbr_cnt, uint32_t* flags, int* x_nbr,
+                     double* x_area) {
+        for (int t = 0; t < m; ++t)
+            piece_merge_row(t, cells, orig, pptr, pidx, p_vol, p_mg, p_e2, p_cnt, p_nbr, p_area,
+                            p_flags, stride, x_off, vol, mg, e2, nbr_cnt, flags, x_nbr, x_area);
+    }
+    void piece_copy(int m, const int* cells, const int* orig, const long long* x_off,
+                    const int* x_nbr, const double* x_area, const int* nbr_cnt,
+                    const long long* ptr, int* nbr, double* area) {
+        for (int t = 0; t < m; ++t)
+            piece_copy_row(t, cells, orig, x_off, x_nbr, x_area, nbr_cnt, ptr, nbr, area);
+    }
+    void ot_repair(int n, const double* xyz, const double* w, const double* vol, const pt4* pts,
+                   const bnode* nodes, const InstLayout& IL, float cs2, double delta,
+                   double* wout) {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int i = 0; i < n; ++i)
+            wout[i] = ot_repair_row(i, xyz, w, vol, pts, nodes, IL, cs2, delta);
+    }
     void power_nearest(const pt4* pts, const bnode* nodes, const InstLayout& IL, const double* q,
                        int nq, float cs2, double* wout) {
 #pragma omp parallel for schedule(dynamic, 256)
@@ -274,7 +301,7 @@ struct EmuBackend {
     void compact(int n, const int* cnt, const long long* ptr, const int* nbr_tmp,
                  const double* area_tmp, int stride, int* nbr, double* area) {
         for (int i = 0; i < n; ++i)
-            for (int k = 0; k < cnt[i]; ++k) {
+            for (int k = 0; k < std::min(cnt[i], stride); ++k) {
                 nbr[ptr[i] + k] = nbr_tmp[(size_t)i * stride + k];
                 area[ptr[i] + k] = area_tmp[(size_t)i * stride + k];
             }
@@ -428,51 +455,42 @@ struct EmuBackend {
         cg[7] = 0.0;
     }
     void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
-                  const double* diag, double* x, double* r, double* z, double* p, double* p2,
-                  double* q, double* cg) {
-        for (int i = 0; i < k; ++i)
-            cg_iter(n, ptr, col, val, diag, x, r, z, i % 2 ? p2 : p, i % 2 ? p : p2, q, cg);
+                  const double* diag, double* x, double* r, double* z, double* p, double* q,
+                  double* cg) {
+        for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
     }
-    // the two PCG kernels, blocks and reduction orders as on the device
+    // the three PCG kernels, blocks and reduction orders as on the device
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
-                 const double* diag, double* x, double* r, double* z, const double* po, double* pn,
-                 double* q, double* cg) {
+                 const double* diag, double* x, double* r, double* z, double* p, double* q,
+                 double* cg) {
         if (cg[6] != 0.0) return;
         const double beta = cg[4];
-        std::vector<double> part(2 * kCgBlocks);
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        const int nb1 = (n + kCgRows - 1) / kCgRows;
+        std::vector<double> part(std::max(nb1, 2 * ((n + 255) / 256)));
 #pragma omp parallel for schedule(static)
-        for (int b = 0; b < kCgBlocks; ++b) {
+        for (int b = 0; b < nb1; ++b) {
             double acc[256] = {};
-            for (long long r0 = (long long)b * kCgRows; r0 < n; r0 += (long long)kCgBlocks * kCgRows)
-                for (int g = 0; g < kCgRows; ++g) {
-                    const long long i = r0 + g;
-                    if (i >= n) break;
-                    const double pi = z[i] + beta * po[i];
-                    double s[kSpmvLanes];
-                    for (int l = 0; l < kSpmvLanes; ++l) {
-                        double a = 0.0;
-                        for (long long k = ptr[i] + l; k < ptr[i + 1]; k += kSpmvLanes) {
-                            const int j = col[k];
-                            a += val[k] * (z[j] + beta * po[j]);
-                        }
-                        s[l] = a;
-                    }
-                    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
-                        double t[kSpmvLanes];
-                        for (int l = 0; l < kSpmvLanes; ++l) t[l] = s[l] + s[l ^ m];
-                        for (int l = 0; l < kSpmvLanes; ++l) s[l] = t[l];
-                    }
-                    const double qi = s[0] + diag[i] * pi;
-                    q[i] = qi;
-                    pn[i] = pi;
-                    acc[g * kSpmvLanes] += pi * qi;
+            for (int g = 0; g < kCgRows; ++g) {
+                const int i = b * kCgRows + g;
+                if (i >= n) break;
+                double sl[kSpmvLanes];
+                for (int l = 0; l < kSpmvLanes; ++l) sl[l] = ot_spmv_lane(i, l, ptr, col, val, p);
+                for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                    double t[kSpmvLanes];
+                    for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
+                    for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
                 }
+                const double qi = sl[0] + diag[i] * p[i];
+                q[i] = qi;
+                acc[g * kSpmvLanes] = p[i] * qi;
+            }
             part[b] = tree_sum(acc);
         }
-        double sh[256];
+        double sh[256], sh2[256];
         for (int t = 0; t < 256; ++t) {
             double a = 0.0;
-            for (int i = t; i < kCgBlocks; i += 256) a += part[i];
+            for (int b = t; b < nb1; b += 256) a += part[b];
             sh[t] = a;
         }
         const double pq = tree_sum(sh);
@@ -484,32 +502,29 @@ struct EmuBackend {
         }
         cg[3] = cg[0] / pq;
         const double alpha = cg[3];
+        const int nb2 = (n + 255) / 256;
 #pragma omp parallel for schedule(static)
-        for (int b = 0; b < kCgBlocks; ++b) {
-            double a1[256], a2[256];
+        for (int b = 0; b < nb2; ++b) {
+            double a1[256] = {}, a2[256] = {};
             for (int t = 0; t < 256; ++t) {
-                double s1 = 0.0, s2 = 0.0;
-                for (long long i = (long long)b * 256 + t; i < n; i += (long long)kCgBlocks * 256) {
-                    x[i] += alpha * pn[i];
-                    const double ri = r[i] - alpha * q[i];
-                    r[i] = ri;
-                    const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
-                    z[i] = zi;
-                    s1 += ri * zi;
-                    s2 += ri * ri;
-                }
-                a1[t] = s1;
-                a2[t] = s2;
+                const int i = b * 256 + t;
+                if (i >= n) break;
+                x[i] += alpha * p[i];
+                const double ri = r[i] - alpha * q[i];
+                r[i] = ri;
+                const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
+                z[i] = zi;
+                a1[t] = ri * zi;
+                a2[t] = ri * ri;
             }
             part[b] = tree_sum(a1);
-            part[kCgBlocks + b] = tree_sum(a2);
+            part[nb2 + b] = tree_sum(a2);
         }
-        double sh2[256];
         for (int t = 0; t < 256; ++t) {
             double a = 0.0, c = 0.0;
-            for (int i = t; i < kCgBlocks; i += 256) {
-                a += part[i];
-                c += part[kCgBlocks + i];
+            for (int b = t; b < nb2; b += 256) {
+                a += part[b];
+                c += part[nb2 + b];
             }
             sh[t] = a;
             sh2[t] = c;
@@ -524,6 +539,31 @@ struct EmuBackend {
         }
         cg[4] = rz / cg[0];
         cg[0] = rz;
+    }
+    // Morton order of a Newton system
+    void inv_perm(int n, const int* orig, int* inv) {
+        for (int m = 0; m < n; ++m) inv[orig[m]] = m;
+    }
+    void perm_row_len(int n, const int* orig, const long long* ptr, int* len) {
+        for (int m = 0; m < n; ++m) len[m] = int(ptr[orig[m] + 1] - ptr[orig[m]]);
+    }
+    void perm_rows(int n, const int* orig, const int* inv, const long long* ptr, const int* col,
+                   const double* val, const long long* ptr_m, int* col_m, double* val_m) {
+#pragma omp parallel for schedule(static)
+        for (int m = 0; m < n; ++m) {
+            const int i = orig[m];
+            const long long b = ptr[i], e = ptr[i + 1], o = ptr_m[m] - b;
+            for (long long k = b; k < e; ++k) {
+                col_m[o + k] = inv[col[k]];
+                val_m[o + k] = val[k];
+            }
+        }
+    }
+    void gather_f64(int n, const int* orig, const double* src, double* dst) {
+        for (int m = 0; m < n; ++m) dst[m] = src[orig[m]];
+    }
+    void scatter_f64(int n, const int* orig, const double* src, double* dst) {
+        for (int m = 0; m < n; ++m) dst[orig[m]] = src[m];
     }
 
     // --------------------------------------------------------- batched OT
@@ -622,49 +662,36 @@ struct EmuBackend {
             c[7] = 0.0;
         }
     }
-    // the two batched PCG kernels, block table and reduction orders as on
-    // the device
-    void cgb_iters(int kk, const SegTable& T, int, const long long* ptr, const int* col,
+    // the batched PCG kernels, block table and reduction orders as on the
+    // device
+    void cgb_iters(int kk, const SegTable& T, int n, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
-                   double* p, double* p2, double* q, double* cg) {
+                   double* p, double* q, double* cg) {
         seg_part_.assign(2 * (size_t)T.nblk, 0.0);
         for (int it = 0; it < kk; ++it) {
-            const double* po = it % 2 ? p2 : p;
-            double* pn = it % 2 ? p : p2;
-#pragma omp parallel for schedule(dynamic, 4)
+            for (int i = 0; i < n; ++i) {
+                const double* c = cg + 8 * T.inst_of[i];
+                if (c[6] == 0.0) p[i] = z[i] + c[4] * p[i];
+            }
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n; ++i) {
+                if (cg[8 * T.inst_of[i] + 6] != 0.0) continue;
+                double sl[kSpmvLanes];
+                for (int l = 0; l < kSpmvLanes; ++l) sl[l] = ot_spmv_lane(i, l, ptr, col, val, p);
+                for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                    double t[kSpmvLanes];
+                    for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
+                    for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
+                }
+                q[i] = sl[0] + diag[i] * p[i];
+            }
             for (int blk = 0; blk < T.nblk; ++blk) {
                 const int k = T.blk_inst[blk];
-                const double* c = cg + 8 * k;
-                if (c[6] != 0.0) continue;
-                const double beta = c[4];
-                const long long end = T.off[k + 1], stride = (long long)T.cnt[k] * kCgRows;
-                double acc[256] = {};
-                for (long long r0 = T.off[k] + (long long)T.blk_idx[blk] * kCgRows; r0 < end;
-                     r0 += stride)
-                    for (int g = 0; g < kCgRows; ++g) {
-                        const long long i = r0 + g;
-                        if (i >= end) break;
-                        const double pi = z[i] + beta * po[i];
-                        double sl[kSpmvLanes];
-                        for (int l = 0; l < kSpmvLanes; ++l) {
-                            double a = 0.0;
-                            for (long long e = ptr[i] + l; e < ptr[i + 1]; e += kSpmvLanes) {
-                                const int j = col[e];
-                                a += val[e] * (z[j] + beta * po[j]);
-                            }
-                            sl[l] = a;
-                        }
-                        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
-                            double t[kSpmvLanes];
-                            for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
-                            for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
-                        }
-                        const double qi = sl[0] + diag[i] * pi;
-                        q[i] = qi;
-                        pn[i] = pi;
-                        acc[g * kSpmvLanes] += pi * qi;
-                    }
-                seg_part_[blk] = tree_sum(acc);
+                if (cg[8 * k + 6] != 0.0) continue;
+                double t1, t2;
+                seg_block_sums(T, blk, [&](long long i, double& s1, double&) { s1 += p[i] * q[i]; },
+                               t1, t2);
+                seg_part_[blk] = t1;
             }
             for (int k = 0; k < T.n_inst; ++k) {
                 double* c = cg + 8 * k;
@@ -686,7 +713,7 @@ struct EmuBackend {
                 const double alpha = cg[8 * k + 3];
                 double t1, t2;
                 seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
-                    x[i] += alpha * pn[i];
+                    x[i] += alpha * p[i];
                     const double ri = r[i] - alpha * q[i];
                     r[i] = ri;
                     const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;

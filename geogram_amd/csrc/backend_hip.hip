@@ -211,7 +211,7 @@ __global__ void k_compact(int n, const int* __restrict__ cnt, const long long* _
     const long long i = t / kCompactLanes;
     if (i >= n) return;
     const long long o = ptr[i];
-    const int m = cnt[i];
+    const int m = min(cnt[i], stride);  // longer lists (pieces) are copied on their own
     const size_t row = (size_t)i * stride;
     for (int k = int(t % kCompactLanes); k < m; k += kCompactLanes) {
         nbr[o + k] = nbr_tmp[row + k];
@@ -753,50 +753,45 @@ __device__ __forceinline__ double cg_read_part(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// PCG iteration, first kernel: the direction pn = z + beta po of every row
-// (beta = cg[4]; the rows' neighbours' entries formed on the fly, so the
-// product needs no separate direction pass), q = A pn, and p.q; the last
-// block sets alpha = rz / p.q (cg[3]) or the breakdown flag.
-__global__ __launch_bounds__(256) void k_cgf_dir_spmv(int n, const long long* __restrict__ ptr,
-                                                      const int* __restrict__ col,
-                                                      const double* __restrict__ val,
-                                                      const double* __restrict__ diag,
-                                                      const double* __restrict__ z,
-                                                      const double* __restrict__ po, double* pn,
-                                                      double* q, double* cg, double* part,
-                                                      unsigned* ctr) {
-    __shared__ double sh[256];
+// PCG iteration, three kernels.  First, the direction p = z + beta p
+// (beta = cg[4]: 0 after cg_start, whose p is z).
+__global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
+                         const double* __restrict__ cg) {
     if (cg[6] != 0.0) return;  // uniform over the grid
     const double beta = cg[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// Second: q = A p, one 16-lane group per row (kCgRows rows per block), and
+// p.q; the last block to finish sets alpha = rz / p.q (cg[3]) or the
+// breakdown flag.
+__global__ __launch_bounds__(256) void k_cg_spmv_pq(int n, const long long* __restrict__ ptr,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ diag,
+                                                    const double* __restrict__ p, double* q,
+                                                    double* cg, double* part, unsigned* ctr) {
+    __shared__ double sh[256];
+    if (cg[6] != 0.0) return;
     const int g = threadIdx.x / kSpmvLanes, l = threadIdx.x % kSpmvLanes;
-    double acc = 0.0;
-    for (long long r0 = (long long)blockIdx.x * kCgRows; r0 < n;
-         r0 += (long long)kCgBlocks * kCgRows) {
-        const long long i = r0 + g;
-        double s = 0.0, pi = 0.0;
-        if (i < n) {
-            pi = z[i] + beta * po[i];
-            for (long long k = ptr[i] + l; k < ptr[i + 1]; k += kSpmvLanes) {
-                const int j = col[k];
-                s += val[k] * (z[j] + beta * po[j]);
-            }
-        }
+    const int i = blockIdx.x * kCgRows + g;
+    double s = i < n ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
 #pragma unroll
-        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-        if (i < n && l == 0) {
-            const double qi = s + diag[i] * pi;
-            q[i] = qi;
-            pn[i] = pi;
-            acc += pi * qi;
-        }
+    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    double acc = 0.0;
+    if (i < n && l == 0) {
+        const double pi = p[i], qi = s + diag[i] * pi;
+        q[i] = qi;
+        acc = pi * qi;
     }
     const double t = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
     if (!cg_last_block(ctr)) return;
     __syncthreads();
-    double s = 0.0;
-    for (int i = threadIdx.x; i < kCgBlocks; i += 256) s += cg_read_part(part + i);
-    const double pq = block_sum(s, sh);
+    double a = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) a += cg_read_part(part + b);
+    const double pq = block_sum(a, sh);
     if (threadIdx.x == 0) {
         cg[2] = pq;
         if (!(pq > 0.0)) {
@@ -809,40 +804,41 @@ __global__ __launch_bounds__(256) void k_cgf_dir_spmv(int n, const long long* __
     }
 }
 
-// PCG iteration, second kernel: x += alpha pn, r -= alpha q, z = r / diag,
-// r.z and r.r; the last block counts the iteration, stops on
+// Third: x += alpha p, r -= alpha q, z = r / diag, r.z and r.r (256
+// elements per block); the last block counts the iteration, stops on
 // r.r <= tol^2 b.b, else sets beta = r.z / previous r.z (cg[4]).
-__global__ __launch_bounds__(256) void k_cgf_update(int n, const double* __restrict__ pn,
-                                                    const double* __restrict__ q,
-                                                    const double* __restrict__ diag, double* x,
-                                                    double* r, double* z, double* cg,
-                                                    double* part, unsigned* ctr) {
+__global__ __launch_bounds__(256) void k_cg_update_beta(int n, const double* __restrict__ p,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ diag, double* x,
+                                                        double* r, double* z, double* cg,
+                                                        double* part, unsigned* ctr) {
     __shared__ double sh[256], sh2[256];
     if (cg[6] != 0.0) return;
     const double alpha = cg[3];
+    const int nb = (int)gridDim.x;
+    const int i = blockIdx.x * 256 + threadIdx.x;
     double s1 = 0.0, s2 = 0.0;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
-         i += (long long)kCgBlocks * 256) {
-        x[i] += alpha * pn[i];
+    if (i < n) {
+        x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
         z[i] = zi;
-        s1 += ri * zi;
-        s2 += ri * ri;
+        s1 = ri * zi;
+        s2 = ri * ri;
     }
     double t1, t2;
     block_sum2(s1, s2, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = t1;
-        part[kCgBlocks + blockIdx.x] = t2;
+        part[nb + blockIdx.x] = t2;
     }
     if (!cg_last_block(ctr)) return;
     __syncthreads();
     s1 = s2 = 0.0;
-    for (int i = threadIdx.x; i < kCgBlocks; i += 256) {
-        s1 += cg_read_part(part + i);
-        s2 += cg_read_part(part + kCgBlocks + i);
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        s1 += cg_read_part(part + b);
+        s2 += cg_read_part(part + nb + b);
     }
     double rz, rr;
     block_sum2(s1, s2, sh, sh2, rz, rr);
@@ -858,6 +854,42 @@ __global__ __launch_bounds__(256) void k_cgf_update(int n, const double* __restr
         }
         *ctr = 0u;
     }
+}
+
+// Morton order of a Newton system: inv[orig[m]] = m; row lengths, rows and
+// vectors gathered / scattered through orig.
+__global__ void k_inv_perm(int n, const int* __restrict__ orig, int* inv) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) inv[orig[m]] = m;
+}
+__global__ void k_perm_row_len(int n, const int* __restrict__ orig,
+                               const long long* __restrict__ ptr, int* len) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) len[m] = int(ptr[orig[m] + 1] - ptr[orig[m]]);
+}
+__global__ void k_perm_rows(int n, const int* __restrict__ orig, const int* __restrict__ inv,
+                            const long long* __restrict__ ptr, const int* __restrict__ col,
+                            const double* __restrict__ val, const long long* __restrict__ ptr_m,
+                            int* col_m, double* val_m) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
+    if (m >= n) return;
+    const int i = orig[m];
+    const long long b = ptr[i], e = ptr[i + 1], o = ptr_m[m] - b;
+    for (long long k = b + l; k < e; k += kSpmvLanes) {
+        col_m[o + k] = inv[col[k]];
+        val_m[o + k] = val[k];
+    }
+}
+__global__ void k_gather_f64(int n, const int* __restrict__ orig, const double* __restrict__ src,
+                             double* dst) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) dst[m] = src[orig[m]];
+}
+__global__ void k_scatter_f64(int n, const int* __restrict__ orig, const double* __restrict__ src,
+                              double* dst) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) dst[orig[m]] = src[m];
 }
 
 
@@ -990,47 +1022,47 @@ __device__ __forceinline__ bool cgb_last_block(unsigned* ctr, int cnt) {
     return last;
 }
 
-// Batched PCG iteration, first kernel (one block table, each block in one
-// instance; instances already stopped skip): pn = z + beta_k po, q = A pn,
-// p.q per instance; the instance's last block sets its alpha.
-__global__ __launch_bounds__(256) void k_cgbf_dir_spmv(SegTable T, const long long* __restrict__ ptr,
-                                                       const int* __restrict__ col,
-                                                       const double* __restrict__ val,
-                                                       const double* __restrict__ diag,
-                                                       const double* __restrict__ z,
-                                                       const double* __restrict__ po, double* pn,
-                                                       double* q, double* cg, double* part,
-                                                       unsigned* ctr) {
+// Batched PCG iteration: the direction p = z + beta_k p of the active
+// instances' rows, q = A p (one 16-lane group per row), then per instance
+// (block table) p.q, whose last block sets the instance's alpha.
+__global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* __restrict__ z,
+                          double* p, const double* __restrict__ cg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* c = cg + 8 * inst_of[i];
+    if (c[6] != 0.0) return;
+    p[i] = z[i] + c[4] * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__ inst_of,
+                                                  const long long* __restrict__ ptr,
+                                                  const int* __restrict__ col,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ diag,
+                                                  const double* __restrict__ p, double* q,
+                                                  const double* __restrict__ cg) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
+    const bool live = i < n && cg[8 * inst_of[i < n ? i : 0] + 6] == 0.0;
+    double s = live ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+#pragma unroll
+    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    if (live && l == 0) q[i] = s + diag[i] * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_cgb_dot_alpha(SegTable T, const double* __restrict__ p,
+                                                       const double* __restrict__ q, double* cg,
+                                                       double* part, unsigned* ctr) {
     __shared__ double sh[256], sh2[256];
-    const int k = T.blk_inst[blockIdx.x];
+    int k;
+    long long beg, end, stride;
+    seg_range(T, blockIdx.x, k, beg, end, stride);
     double* c = cg + 8 * k;
     if (c[6] != 0.0) return;  // uniform over the block and the instance
-    const double beta = c[4];
-    const int g = threadIdx.x / kSpmvLanes, l = threadIdx.x % kSpmvLanes;
-    const long long end = T.off[k + 1], stride = (long long)T.cnt[k] * kCgRows;
-    double acc = 0.0;
-    for (long long r0 = T.off[k] + (long long)T.blk_idx[blockIdx.x] * kCgRows; r0 < end;
-         r0 += stride) {
-        const long long i = r0 + g;
-        double s = 0.0, pi = 0.0;
-        if (i < end) {
-            pi = z[i] + beta * po[i];
-            for (long long e = ptr[i] + l; e < ptr[i + 1]; e += kSpmvLanes) {
-                const int j = col[e];
-                s += val[e] * (z[j] + beta * po[j]);
-            }
-        }
-#pragma unroll
-        for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-        if (i < end && l == 0) {
-            const double qi = s + diag[i] * pi;
-            q[i] = qi;
-            pn[i] = pi;
-            acc += pi * qi;
-        }
-    }
+    double s = 0.0;
+    for (long long i = beg; i < end; i += stride) s += p[i] * q[i];
     double t1, t2;
-    block_sum2(acc, 0.0, sh, sh2, t1, t2);
+    block_sum2(s, 0.0, sh, sh2, t1, t2);
     if (threadIdx.x == 0) part[blockIdx.x] = t1;
     if (!cgb_last_block(ctr + k, T.cnt[k])) return;
     __syncthreads();
@@ -1050,10 +1082,10 @@ __global__ __launch_bounds__(256) void k_cgbf_dir_spmv(SegTable T, const long lo
     }
 }
 
-// Batched PCG iteration, second kernel: the update and the two dots per
+// Batched PCG iteration, last kernel: the update and the two dots per
 // instance; the instance's last block counts the iteration and sets beta
 // or stops it.
-__global__ __launch_bounds__(256) void k_cgbf_update(SegTable T, const double* __restrict__ pn,
+__global__ __launch_bounds__(256) void k_cgb_update_beta(SegTable T, const double* __restrict__ p,
                                                      const double* __restrict__ q,
                                                      const double* __restrict__ diag, double* x,
                                                      double* r, double* z, double* cg,
@@ -1067,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_cgbf_update(SegTable T, const double* _
     const double alpha = c[3];
     double s1 = 0.0, s2 = 0.0;
     for (long long i = beg; i < end; i += stride) {
-        x[i] += alpha * pn[i];
+        x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
@@ -1117,6 +1149,42 @@ __global__ __launch_bounds__(256) void k_seg_center(SegTable T, double* b) {
 }
 
 __global__ void k_set0_i64(long long* p) { *p = 0; }
+
+__global__ void k_piece_setup(int nq, const int* __restrict__ cells, const pt4* __restrict__ pts,
+                              const int* __restrict__ inst_of, const double* __restrict__ box,
+                              double* p_box) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) piece_setup_row(q, cells, pts, inst_of, box, p_box);
+}
+__global__ void k_piece_merge(int m, const int* __restrict__ cells, const int* __restrict__ orig,
+                              const long long* __restrict__ pptr, const int* __restrict__ pidx,
+                              const double* __restrict__ p_vol, const double* __restrict__ p_mg,
+                              const double* __restrict__ p_e2, const int* __restrict__ p_cnt,
+                              const int* __restrict__ p_nbr, const double* __restrict__ p_area,
+                              const uint32_t* __restrict__ p_flags, int stride,
+                              const long long* __restrict__ x_off, double* vol, double* mg,
+                              double* e2, int* nbr_cnt, uint32_t* flags, int* x_nbr,
+                              double* x_area) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m)
+        piece_merge_row(t, cells, orig, pptr, pidx, p_vol, p_mg, p_e2, p_cnt, p_nbr, p_area,
+                        p_flags, stride, x_off, vol, mg, e2, nbr_cnt, flags, x_nbr, x_area);
+}
+__global__ void k_piece_copy(int m, const int* __restrict__ cells, const int* __restrict__ orig,
+                             const long long* __restrict__ x_off, const int* __restrict__ x_nbr,
+                             const double* __restrict__ x_area, const int* __restrict__ nbr_cnt,
+                             const long long* __restrict__ ptr, int* nbr, double* area) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) piece_copy_row(t, cells, orig, x_off, x_nbr, x_area, nbr_cnt, ptr, nbr, area);
+}
+
+__global__ void k_ot_repair(int n, const double* __restrict__ xyz, const double* __restrict__ w,
+                            const double* __restrict__ vol, const pt4* __restrict__ pts,
+                            const bnode* __restrict__ nodes, InstLayout IL, float cs2, double delta,
+                            double* wout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wout[i] = ot_repair_row(i, xyz, w, vol, pts, nodes, IL, cs2, delta);
+}
 
 // weight of the power-nearest seed (instance 0 of the tree) of each query
 __global__ void k_power_nearest(const pt4* __restrict__ pts, const bnode* __restrict__ nodes,
@@ -1530,25 +1598,21 @@ struct HipBackend {
         check();
     }
     // k iterations as one hipGraph, captured once per argument set and
-    // replayed until the solve ends (the kernels read their scalars from cg).
-    // Iteration i reads the direction from p (i even) or p2 and writes the
-    // other; cg_start leaves it in p, and k is even except for a solve's last
-    // chunk.
+    // replayed until the solve ends (the kernels read their scalars from cg)
     void cg_iters(int k, int n, const long long* ptr, const int* col, const double* val,
-                  const double* diag, double* x, double* r, double* z, double* p, double* p2,
-                  double* q, double* cg) {
+                  const double* diag, double* x, double* r, double* z, double* p, double* q,
+                  double* cg) {
+        double* part = cg_part(n);  // allocated outside the capture
         const void* key[13] = {(const void*)(long long)k, (const void*)(long long)n, ptr, col, val,
-                               diag, x, r, z, p, p2, q, cg};
+                               diag, x, r, z, p, q, cg, part};
         bool same = cg_exec_ != nullptr;
         for (int i = 0; i < 13 && same; ++i) same = cg_key_[i] == key[i];
         if (!same) {
             if (cg_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cg_exec_));
             cg_exec_ = nullptr;
-            cg_part();  // allocated outside the capture
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < k; ++i)
-                cg_iter(n, ptr, col, val, diag, x, r, z, i % 2 ? p2 : p, i % 2 ? p : p2, q, cg);
+            for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cg_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
@@ -1557,24 +1621,54 @@ struct HipBackend {
         VGC_HIP_CHECK(hipGraphLaunch(cg_exec_, stream_));
     }
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
-                 const double* diag, double* x, double* r, double* z, const double* po, double* pn,
-                 double* q, double* cg) {
-        double* part = cg_part();
-        k_cgf_dir_spmv<<<kCgBlocks, 256, 0, stream_>>>(n, ptr, col, val, diag, z, po, pn, q, cg,
-                                                       part, cg_ctr_);
-        k_cgf_update<<<kCgBlocks, 256, 0, stream_>>>(n, pn, q, diag, x, r, z, cg, part,
-                                                     cg_ctr_ + 1);
+                 const double* diag, double* x, double* r, double* z, double* p, double* q,
+                 double* cg) {
+        double* part = cg_part(n);
+        k_cg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, z, p, cg);
+        k_cg_spmv_pq<<<blocks_for(n, kCgRows), 256, 0, stream_>>>(n, ptr, col, val, diag, p, q, cg,
+                                                                   part, cg_ctr_);
+        k_cg_update_beta<<<blocks_for(n, 256), 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part,
+                                                                   cg_ctr_ + 1);
         check();
     }
-    // block partials and the two last-block counters of the PCG kernels
-    // (zeroed once here; each last block resets its counter)
-    double* cg_part() {
-        if (!cg_part_) {
-            VGC_HIP_CHECK(hipMalloc(&cg_part_, sizeof(double) * 2 * kCgBlocks));
+    // block partials (one per block of the product, two per block of the
+    // update) and the two last-block counters of the PCG kernels (zeroed
+    // when allocated; each last block resets its counter)
+    double* cg_part(int n) {
+        const long long need = std::max<long long>(blocks_for(n, kCgRows), 2ll * blocks_for(n, 256));
+        if (cg_part_cap_ < need) {
+            if (cg_part_) VGC_HIP_CHECK(hipFree(cg_part_));
+            VGC_HIP_CHECK(hipMalloc(&cg_part_, sizeof(double) * (size_t)need));
+            cg_part_cap_ = need;
+        }
+        if (!cg_ctr_) {
             VGC_HIP_CHECK(hipMalloc(&cg_ctr_, sizeof(unsigned) * 2));
             VGC_HIP_CHECK(hipMemsetAsync(cg_ctr_, 0, sizeof(unsigned) * 2, stream_));
         }
         return cg_part_;
+    }
+    // Morton order of a Newton system (orig: Morton position -> caller index)
+    void inv_perm(int n, const int* orig, int* inv) {
+        k_inv_perm<<<blocks_for(n, 256), 256, 0, stream_>>>(n, orig, inv);
+        check();
+    }
+    void perm_row_len(int n, const int* orig, const long long* ptr, int* len) {
+        k_perm_row_len<<<blocks_for(n, 256), 256, 0, stream_>>>(n, orig, ptr, len);
+        check();
+    }
+    void perm_rows(int n, const int* orig, const int* inv, const long long* ptr, const int* col,
+                   const double* val, const long long* ptr_m, int* col_m, double* val_m) {
+        k_perm_rows<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
+            n, orig, inv, ptr, col, val, ptr_m, col_m, val_m);
+        check();
+    }
+    void gather_f64(int n, const int* orig, const double* src, double* dst) {
+        k_gather_f64<<<blocks_for(n, 256), 256, 0, stream_>>>(n, orig, src, dst);
+        check();
+    }
+    void scatter_f64(int n, const int* orig, const double* src, double* dst) {
+        k_scatter_f64<<<blocks_for(n, 256), 256, 0, stream_>>>(n, orig, src, dst);
+        check();
     }
 
     // --------------------------------------------------------- batched OT
@@ -1613,41 +1707,40 @@ struct HipBackend {
         check();
     }
     // k batched CG iterations, captured once as a graph (for these buffers and
-    // this table) and replayed; the direction alternates between p and p2 as
-    // in cg_iters
+    // this table) and replayed
     void cgb_iters(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
-                   double* p, double* p2, double* q, double* cg) {
+                   double* p, double* q, double* cg) {
         double* part = seg_part(T.nblk);  // allocated outside the capture
         unsigned* ctr = cgb_ctr(T.n_inst);
-        const void* key[23] = {(const void*)(long long)k, (const void*)(long long)n,
+        const void* key[22] = {(const void*)(long long)k, (const void*)(long long)n,
                                (const void*)(long long)T.n_inst, (const void*)(long long)T.nblk,
                                T.off, T.blk_inst, T.blk_idx, T.cnt, T.first, T.inst_of, ptr, col,
-                               val, diag, x, r, z, p, p2, q, cg, part, ctr};
+                               val, diag, x, r, z, p, q, cg, part, ctr};
         bool same = cgb_exec_ != nullptr;
-        for (int i = 0; i < 23 && same; ++i) same = cgb_key_[i] == key[i];
+        for (int i = 0; i < 22 && same; ++i) same = cgb_key_[i] == key[i];
         if (!same) {
             if (cgb_exec_) VGC_HIP_CHECK(hipGraphExecDestroy(cgb_exec_));
             cgb_exec_ = nullptr;
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, p2, q, cg, part, ctr);
+            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, q, cg, part, ctr);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cgb_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
-            for (int i = 0; i < 23; ++i) cgb_key_[i] = key[i];
+            for (int i = 0; i < 22; ++i) cgb_key_[i] = key[i];
         }
         VGC_HIP_CHECK(hipGraphLaunch(cgb_exec_, stream_));
     }
-    void cgb_launch(int k, const SegTable& T, int, const long long* ptr, const int* col,
+    void cgb_launch(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                     const double* val, const double* diag, double* x, double* r, double* z,
-                    double* p, double* p2, double* q, double* cg, double* part, unsigned* ctr) {
+                    double* p, double* q, double* cg, double* part, unsigned* ctr) {
         for (int it = 0; it < k; ++it) {
-            const double* po = it % 2 ? p2 : p;
-            double* pn = it % 2 ? p : p2;
-            k_cgbf_dir_spmv<<<T.nblk, 256, 0, stream_>>>(T, ptr, col, val, diag, z, po, pn, q, cg,
-                                                         part, ctr);
-            k_cgbf_update<<<T.nblk, 256, 0, stream_>>>(T, pn, q, diag, x, r, z, cg, part, ctr);
+            k_cgb_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, T.inst_of, z, p, cg);
+            k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
+                n, T.inst_of, ptr, col, val, diag, p, q, cg);
+            k_cgb_dot_alpha<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part, ctr);
+            k_cgb_update_beta<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part, ctr);
         }
         check();
     }
@@ -1720,6 +1813,37 @@ struct HipBackend {
                         const double* val, const int* flag, double* m, double* mg) {
         k_surface_reduce<<<blocks_for(n, 256), 256, 0, stream_>>>(n, keys, idx, nrec, val, flag, m,
                                                                    mg);
+        check();
+    }
+    void piece_setup(int nq, const int* cells, const pt4* pts, const int* inst_of,
+                     const double* box, double* p_box) {
+        k_piece_setup<<<blocks_for(nq, 256), 256, 0, stream_>>>(nq, cells, pts, inst_of, box, p_box);
+        check();
+    }
+    void piece_merge(int m, const int* cells, const int* orig, const long long* pptr,
+                     const int* pidx, const double* p_vol, const double* p_mg, const double* p_e2,
+                     const int* p_cnt, const int* p_nbr, const double* p_area,
+                     const uint32_t* p_flags, int stride, const long long* x_off, double* vol,
+                     double* mg, double* e2, int* nbr_cnt, uint32_t* flags, int* x_nbr,
+                     double* x_area) {
+        k_piece_merge<<<blocks_for(m, 64), 64, 0, stream_>>>(m, cells, orig, pptr, pidx, p_vol, p_mg,
+                                                             p_e2, p_cnt, p_nbr, p_area, p_flags,
+                                                             stride, x_off, vol, mg, e2, nbr_cnt,
+                                                             flags, x_nbr, x_area);
+        check();
+    }
+    void piece_copy(int m, const int* cells, const int* orig, const long long* x_off,
+                    const int* x_nbr, const double* x_area, const int* nbr_cnt,
+                    const long long* ptr, int* nbr, double* area) {
+        k_piece_copy<<<blocks_for(m, 64), 64, 0, stream_>>>(m, cells, orig, x_off, x_nbr, x_area,
+                                                            nbr_cnt, ptr, nbr, area);
+        check();
+    }
+    void ot_repair(int n, const double* xyz, const double* w, const double* vol, const pt4* pts,
+                   const bnode* nodes, const InstLayout& IL, float cs2, double delta,
+                   double* wout) {
+        k_ot_repair<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, w, vol, pts, nodes, IL, cs2,
+                                                              delta, wout);
         check();
     }
     void iota(int* d, int n) {
@@ -1806,9 +1930,10 @@ private:
     hipGraphExec_t cg_exec_ = nullptr;
     const void* cg_key_[13] = {};
     double* cg_part_ = nullptr;
+    long long cg_part_cap_ = 0;
     unsigned* cg_ctr_ = nullptr;
     hipGraphExec_t cgb_exec_ = nullptr;
-    const void* cgb_key_[23] = {};
+    const void* cgb_key_[22] = {};
     unsigned* cgb_ctr_ = nullptr;
     int cgb_ctr_cap_ = 0;
     double* seg_part_ = nullptr;

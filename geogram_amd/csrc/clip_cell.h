@@ -147,6 +147,11 @@ struct ClipParams {
     // own); src maps every caller index to its seed (null: identity)
     int n_real;
     const int* src;
+    // pieces of cells over the plane limit: item k clips its cell inside
+    // item_box[6k .. 6k+5] (one octant of the domain around the seed) and,
+    // with piece set, writes its outputs and flags by item, not by seed
+    const double* item_box;
+    int piece;
 };
 
 template <int S>
@@ -1383,7 +1388,25 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         return;
     }
     const pt4 pc = P.pts[c];
-    const double* box = P.box + 6 * inst;
+    const double* box = P.item_box ? P.item_box + 6 * (long long)item : P.box + 6 * inst;
+    // outputs by item for pieces, else by seed (caller's index) and Morton
+    // position
+    const int oc = P.piece ? item : P.orig[c];
+    const int fc = P.piece ? item : c;
+    if (P.item_box && !(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) {
+        // a flat octant (the seed on the domain's boundary): nothing of the
+        // cell in it
+        if (lane0()) {
+            P.vol[oc] = 0.0;
+            P.mg[3 * (long long)oc + 0] = P.mg[3 * (long long)oc + 1] = 0.0;
+            P.mg[3 * (long long)oc + 2] = 0.0;
+            P.nbr_cnt[oc] = 0;
+            if (P.e2) P.e2[oc] = 0.0;
+            P.flags[fc] = CELL_DONE | CELL_EMPTY;
+            P.r2[fc] = 0.0;
+        }
+        return;
+    }
 
     CellRun<S> R;
     R.pc = pc;
@@ -1546,8 +1569,8 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         CornerTab GT;
         const bool okf = P.ball && !(flags & CELL_BAD_TOPO) && make_filter<S, false>(R, G, GT);
         if (lane0()) {
-            P.flags[c] = flags;
-            P.r2[c] = r2max;
+            P.flags[fc] = flags;
+            P.r2[fc] = r2max;
             if (P.ball) {
                 fball b;
                 b.cx = G.cx;
@@ -1558,7 +1581,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 b.hy = G.hy;
                 b.hz = G.hz;
                 b.pad = 0.0f;
-                P.ball[c] = b;
+                P.ball[fc] = b;
             }
         }
         return;
@@ -1568,16 +1591,16 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (flags & CELL_EMPTY) {
         if (P.abort_flag && lane0()) flag_raise(P.abort_flag + inst);
         if (lane0()) {
-            P.vol[orig_c] = 0.0;
-            P.mg[3 * (long long)orig_c + 0] = 0.0;
-            P.mg[3 * (long long)orig_c + 1] = 0.0;
-            P.mg[3 * (long long)orig_c + 2] = 0.0;
-            P.nbr_cnt[orig_c] = 0;
-            if (P.e2) P.e2[orig_c] = 0.0;
+            P.vol[oc] = 0.0;
+            P.mg[3 * (long long)oc + 0] = 0.0;
+            P.mg[3 * (long long)oc + 1] = 0.0;
+            P.mg[3 * (long long)oc + 2] = 0.0;
+            P.nbr_cnt[oc] = 0;
+            if (P.e2) P.e2[oc] = 0.0;
             if (P.mesh_mode == 1)
                 for (int k = 0; k < 4; ++k) P.mesh_cnt[(long long)k * P.mesh_n + orig_c] = 0;
-            P.flags[c] = flags | CELL_DONE;
-            P.r2[c] = 0.0;
+            P.flags[fc] = flags | CELL_DONE;
+            P.r2[fc] = 0.0;
         }
         return;
     }
@@ -1620,7 +1643,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     int nfacets = 0;
     int out_base = 0;
     bool face_overflow = false;
-    const long long obase = (long long)orig_c * P.face_stride;
+    const long long obase = (long long)oc * P.face_stride;
     for (int f0 = 0; f0 < np; f0 += 64) {
         LI f = f0 + lane;
         LB inr = f < np;
@@ -1725,26 +1748,26 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     if (face_overflow) flags |= CELL_OVERFLOW;
     if (lane0()) {
         if (!face_overflow) {
-            P.vol[orig_c] = V;
-            P.mg[3 * (long long)orig_c + 0] = Mx + V * pc.x;
-            P.mg[3 * (long long)orig_c + 1] = My + V * pc.y;
-            P.mg[3 * (long long)orig_c + 2] = Mz + V * pc.z;
-            P.nbr_cnt[orig_c] = out_base;
-            P.flags[c] = flags | CELL_DONE;
+            P.vol[oc] = V;
+            P.mg[3 * (long long)oc + 0] = Mx + V * pc.x;
+            P.mg[3 * (long long)oc + 1] = My + V * pc.y;
+            P.mg[3 * (long long)oc + 2] = Mz + V * pc.z;
+            P.nbr_cnt[oc] = out_base;
+            P.flags[fc] = flags | CELL_DONE;
         } else {
-            P.flags[c] = flags;
+            P.flags[fc] = flags;
             if (P.ball) {
                 fball b;
                 b.cx = b.cy = b.cz = b.hx = b.hy = b.hz = b.pad = 0.0f;
                 b.rc = -1.0f;
-                P.ball[c] = b;
+                P.ball[fc] = b;
             }
         }
-        P.r2[c] = r2max;
+        P.r2[fc] = r2max;
     }
     if (P.e2) {
         const double E2 = wave_sum(e2a);
-        if (lane0() && !face_overflow) P.e2[orig_c] = E2;
+        if (lane0() && !face_overflow) P.e2[oc] = E2;
     }
 }
 

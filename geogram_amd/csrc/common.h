@@ -349,6 +349,107 @@ VGC_SD int power_nearest_query(const pt4* pts, const bnode* nodes, const InstLay
     return bi;
 }
 
+// ------------------------------------------ cells over the plane limit
+// Such a cell is computed in pieces, its parts in boxes that tile its
+// instance's box: first the eight octants around the seed, then, for a piece
+// still over the limit, the eight halves-of-edges sub-boxes of its box, and
+// so on.  Each piece is clipped by the cell's complete neighbour list;
+// volumes, first and second moments add up over the pieces and so do the
+// facet areas of one neighbour.
+constexpr int kPieces = 8;
+
+// the octant boxes of cell t: items 8t .. 8t + 7
+VGC_SD void piece_setup_row(int q, const int* cells, const pt4* pts, const int* inst_of,
+                            const double* box, double* p_box) {
+    const int t = q / kPieces, k = q % kPieces;
+    const int c = cells[t];
+    const double* b = box + 6 * (inst_of ? inst_of[c] : 0);
+    const pt4 p = pts[c];
+    const double pc[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+        const double s = fmin(fmax(pc[a], b[a]), b[3 + a]);  // the seed, inside the box
+        const bool upper = (k >> a) & 1;
+        p_box[6 * (size_t)q + a] = upper ? s : b[a];
+        p_box[6 * (size_t)q + 3 + a] = upper ? b[3 + a] : s;
+    }
+}
+
+// cell t from its finished pieces pidx[pptr[t] .. pptr[t+1]) (in that
+// order): sums, and the union of the neighbour lists with the areas of a
+// neighbour added, into x_nbr / x_area from x_off[t]
+VGC_SD void piece_merge_row(int t, const int* cells, const int* orig, const long long* pptr,
+                            const int* pidx, const double* p_vol, const double* p_mg,
+                            const double* p_e2, const int* p_cnt, const int* p_nbr,
+                            const double* p_area, const uint32_t* p_flags, int stride,
+                            const long long* x_off, double* vol, double* mg, double* e2,
+                            int* nbr_cnt, uint32_t* flags, int* x_nbr, double* x_area) {
+    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, e = 0.0;
+    uint32_t any = 0;
+    int cnt = 0;
+    int* xn = x_nbr + x_off[t];
+    double* xa = x_area + x_off[t];
+    for (long long u = pptr[t]; u < pptr[t + 1]; ++u) {
+        const int q = pidx[u];
+        v += p_vol[q];
+        gx += p_mg[3 * (size_t)q];
+        gy += p_mg[3 * (size_t)q + 1];
+        gz += p_mg[3 * (size_t)q + 2];
+        if (p_e2) e += p_e2[q];
+        any |= p_flags[q];
+        for (int m = 0; m < p_cnt[q]; ++m) {
+            const int j = p_nbr[(size_t)q * stride + m];
+            const double a = p_area[(size_t)q * stride + m];
+            int k = 0;
+            while (k < cnt && xn[k] != j) ++k;
+            if (k == cnt) {
+                xn[cnt] = j;
+                xa[cnt] = a;
+                ++cnt;
+            } else {
+                xa[k] += a;
+            }
+        }
+    }
+    const int c = cells[t], i = orig[c];
+    vol[i] = v;
+    mg[3 * (size_t)i] = gx;
+    mg[3 * (size_t)i + 1] = gy;
+    mg[3 * (size_t)i + 2] = gz;
+    if (e2) e2[i] = e;
+    nbr_cnt[i] = cnt;
+    flags[c] = CELL_DONE | (v > 0.0 ? 0u : (uint32_t)CELL_EMPTY) |
+               (any & (uint32_t)(CELL_UNCERTAIN | CELL_DISCONNECTED));
+}
+
+// the merged neighbour list of piece-computed cell t into the CSR
+VGC_SD void piece_copy_row(int t, const int* cells, const int* orig, const long long* x_off,
+                           const int* x_nbr, const double* x_area, const int* nbr_cnt,
+                           const long long* ptr, int* nbr, double* area) {
+    const int i = orig[cells[t]];
+    const long long o = ptr[i];
+    for (int m = 0; m < nbr_cnt[i]; ++m) {
+        nbr[o + m] = x_nbr[x_off[t] + m];
+        area[o + m] = x_area[x_off[t] + m];
+    }
+}
+
+// Admissibility repair of a transport start: an empty cell i gets the
+// smallest weight that puts its seed inside its cell with margin delta,
+// w_i = max_j (w_j - |p_i - p_j|^2) + delta (j the power-nearest seed of
+// p_i, which is not i since p_i lies outside its cell); other weights are
+// kept.  pts / nodes: the tree built for the weights w.
+VGC_SD double ot_repair_row(int i, const double* xyz, const double* w, const double* vol,
+                            const pt4* pts, const bnode* nodes, const InstLayout& IL, float cs2,
+                            double delta) {
+    if (vol[i] > 0.0) return w[i];
+    const double x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+    const int j = power_nearest_query(pts, nodes, IL, x, y, z, cs2);
+    const pt4 q = pts[j];
+    const double dx = x - q.x, dy = y - q.y, dz = z - q.z;
+    const double v = q.w - (dx * dx + dy * dy + dz * dz) + delta;
+    return v > w[i] ? v : w[i];
+}
+
 // ------------------------------------------------------ radius queries
 // Seeds (other than q) with FP32 squared distance <= r2 and, with a filter
 // F, that pass the power-distance test (types.h), in traversal order.  Count

@@ -29,6 +29,7 @@ struct OtStats {
     int cell_passes = 0;
     int halvings = 0;
     int used_fallback_start = 0;
+    int repair_rounds = 0;
     double err0 = 0, final_err = 0;
     double t_total_ms = 0, t_cells_ms = 0, t_cg_ms = 0;
     int converged = 0;
@@ -57,7 +58,7 @@ public:
 
     int solve(Problem pb, const double* d_nu, double* d_w, double tol, int max_newton,
               double cg_tol, int cg_max, OtStats& st, bool judge = false, bool verbose = false,
-              bool inexact = false) {
+              bool inexact = false, bool repair = false) {
         auto t0 = std::chrono::steady_clock::now();
         const int n = pb.n;
         pb.want_neighbors = true;
@@ -110,8 +111,25 @@ public:
             if (cells(d_wt, false)) return fail("the cells of the zero start failed");
             masses(m0);
         }
-        int rc0 = cells(d_w, true);
+        // with repair, full passes (every empty cell found): the weights of
+        // empty cells are raised until each seed lies in its cell
+        int rc0 = cells(d_w, !repair);
         if (rc0 < 0) return -1;
+        if (repair) {
+            const double* b = pb.box.data();
+            const double h2 = std::pow((b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]) / n, 2.0 / 3.0);
+            for (int round = 0; round < kRepairRounds && rc0 > 0 && rej_failed == 0; ++round) {
+                E_.repair_empty(pb, kRepairMargin * h2, d_wt);
+                be_.d2d(d_w, d_wt, sizeof(double) * n);
+                st.repair_rounds++;
+                const long long e0 = es.nb_empty;
+                rc0 = cells(d_w, false);
+                if (rc0 < 0) return -1;
+                if (verbose)
+                    std::fprintf(stderr, "[vgc ot] start repair %d: %lld empty cells\n", round + 1,
+                                 es.nb_empty - e0);
+            }
+        }
         double ms[3] = {0, 0, 0};
         if (rc0 == 0) masses(ms);
         if (rc0 > 0 || !(ms[2] > 0.0) || (judge && m0[1] < ms[1])) {
@@ -145,8 +163,19 @@ public:
             // masses and nu can differ in sum by rounding; the constants are
             // H's kernel, and CG on an inconsistent system stalls)
             be_.ot_rhs(n, d_nu, E_.d_vol, d_b, d_scal + 4);
-            int it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_cg,
-                         inexact ? forcing(err, cg_tol, last_t == 1.0) : cg_tol, cg_max);
+            const double ctol = inexact ? forcing(err, cg_tol, last_t == 1.0) : cg_tol;
+            int it;
+            if (!pb.periodic) {
+                // the system in the cells' Morton order: neighbouring rows,
+                // neighbouring columns, so the product's gathers stay in cache
+                const MortonSystem M = to_morton(n, nnz, d_val, d_diag, d_b);
+                it = pcg(n, M.ptr, M.col, M.val, M.diag, M.b, M.x, d_r, d_z, d_p, d_q, d_cg, ctol,
+                         cg_max);
+                be_.scatter_f64(n, E_.d_orig, M.x, d_x);
+            } else {
+                it = pcg(n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_b, d_x, d_r, d_z, d_p, d_q, d_cg,
+                         ctol, cg_max);
+            }
             st.cg_iters += it;
             st.t_cg_ms += ms_since(tg);
             // damped step
@@ -317,10 +346,12 @@ public:
             }
             double* d_tol2 = (double*)slot(28, sizeof(double) * (size_t)K);
             be_.h2d(d_tol2, tol2.data(), sizeof(double) * (size_t)K);
-            be_.cgb_start(T, d_b, d_diag, d_x, d_r, d_z, d_p, d_cg, d_tol2);
+            // Morton order (instances keep their ranges: the order leads
+            // with the instance)
+            const MortonSystem M = to_morton(n, nnz, d_val, d_diag, d_b);
+            be_.cgb_start(T, M.b, M.diag, M.x, d_r, d_z, d_p, d_cg, d_tol2);
             std::vector<double> h(8 * (size_t)K);
             be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
-            double* d_p2 = (double*)slot(32, sizeof(double) * (size_t)n);
             int issued = 0;
             auto all_done = [&]() {
                 for (int k = 0; k < K; ++k)
@@ -329,11 +360,11 @@ public:
             };
             while (!all_done() && issued < cg_max) {
                 const int kk = std::min(kCgCheck, cg_max - issued);
-                be_.cgb_iters(kk, T, n, E_.d_ptr, E_.d_nbr, d_val, d_diag, d_x, d_r, d_z, d_p, d_p2,
-                              d_q, d_cg);
+                be_.cgb_iters(kk, T, n, M.ptr, M.col, M.val, M.diag, M.x, d_r, d_z, d_p, d_q, d_cg);
                 issued += kk;
                 be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
             }
+            be_.scatter_f64(n, E_.d_orig, M.x, d_x);
             const double tcg = ms_since(tg);
             for (int k = 0; k < K; ++k) {
                 st[k].t_cg_ms += tcg;
@@ -410,22 +441,53 @@ private:
     // iterations run on the device (scalars in device memory), kCgCheck at a
     // time as one graph launch; the host reads the state between them.
     static constexpr int kCgCheck = 16;
+    static constexpr int kRepairRounds = 16;
+    // margin of the repair, in units of h^2 (h = (box volume / n)^(1/3))
+    static constexpr double kRepairMargin = 0.01;
     int pcg(int n, const long long* ptr, const int* col, const double* val, const double* diag,
             const double* b, double* x, double* r, double* z, double* p, double* q, double* scal,
             double tol, int maxit) {
         be_.cg_start(n, b, diag, x, r, z, p, scal, tol * tol);
-        // second direction buffer (the iterations alternate between p and it)
-        double* p2 = (double*)slot(31, sizeof(double) * (size_t)n);
         int issued = 0;
         double h[8];
         be_.d2h(h, scal, sizeof(double) * 8);
         while (h[6] == 0.0 && issued < maxit) {
             const int k = std::min(kCgCheck, maxit - issued);
-            be_.cg_iters(k, n, ptr, col, val, diag, x, r, z, p, p2, q, scal);
+            be_.cg_iters(k, n, ptr, col, val, diag, x, r, z, p, q, scal);
             issued += k;
             be_.d2h(h, scal, sizeof(double) * 8);
         }
         return (int)h[7];
+    }
+
+    // The Newton system H d = b of the last cell pass (caller-order CSR of
+    // the engine, values val, diagonal diag) permuted to the pass's Morton
+    // order; x is its solution buffer, in that order.
+    struct MortonSystem {
+        const long long* ptr;
+        const int* col;
+        const double* val;
+        const double* diag;
+        const double* b;
+        double* x;
+    };
+    MortonSystem to_morton(int n, long long nnz, const double* val, const double* diag,
+                           const double* b) {
+        int* inv = (int*)slot(33, sizeof(int) * (size_t)n);
+        int* len = (int*)slot(34, sizeof(int) * (size_t)n);
+        long long* ptr = (long long*)slot(35, sizeof(long long) * ((size_t)n + 1));
+        int* col = (int*)slot(36, sizeof(int) * (size_t)std::max<long long>(nnz, 1));
+        double* v = (double*)slot(37, sizeof(double) * (size_t)std::max<long long>(nnz, 1));
+        double* dg = (double*)slot(38, sizeof(double) * (size_t)n);
+        double* bb = (double*)slot(39, sizeof(double) * (size_t)n);
+        double* x = (double*)slot(40, sizeof(double) * (size_t)n);
+        be_.inv_perm(n, E_.d_orig, inv);
+        be_.perm_row_len(n, E_.d_orig, E_.d_ptr, len);
+        be_.scan_i32(len, ptr, n);
+        be_.perm_rows(n, E_.d_orig, inv, E_.d_ptr, E_.d_nbr, val, ptr, col, v);
+        be_.gather_f64(n, E_.d_orig, diag, dg);
+        be_.gather_f64(n, E_.d_orig, b, bb);
+        return MortonSystem{ptr, col, v, dg, bb, x};
     }
 
     int fail(const char* msg) {

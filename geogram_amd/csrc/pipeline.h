@@ -47,6 +47,7 @@ struct EngineStats {
     double t_clip_a_gpu_ms = 0, t_knn_gpu_ms = 0;
     unsigned long long a_live = 0, a_new = 0, a_tets = 0, a_facets = 0, a_tests = 0, a_cells = 0;
     unsigned long long nb_fail_capacity = 0;  // failed cells over the corner/plane/facet limits
+    unsigned long long nb_pieces = 0;  // cells over the plane limit computed in octant pieces
     unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
     unsigned long long nb_regather = 0;  // weighted cells gathered again without the ring limit
     unsigned long long nb_periodic_retries = 0;  // periodic passes redone with a wider margin
@@ -254,6 +255,15 @@ public:
         return 0;
     }
 
+    // Weights of the empty cells of the last cell pass (built for pb.d_w)
+    // raised until each seed lies in its cell (ot_repair_row), into d_wout.
+    void repair_empty(const Problem& pb, double delta, double* d_wout) {
+        double cs = 0.0;
+        for (int k = 0; k < 6; ++k) cs = std::max(cs, std::fabs(pb.box[k]));
+        be_.ot_repair(pb.n, pb.d_xyz, pb.d_w, d_vol, d_pts, d_nodes, IL, float(cs * cs), delta,
+                      d_wout);
+    }
+
     // ------------------------------------------------------------ kNN
     // K nearest of every seed (Morton positions), n x K.
     int knn_lists(int K, int*& d_list, float* d_d2 = nullptr) {
@@ -446,6 +456,8 @@ public:
     int cells_core(const Problem& pb, EngineStats& st) {
         auto tstart = now();
         if (build(pb, st)) return -1;
+        piece_cells_.clear();
+        x_total_ = 0;
         const int K0 = pb.k0;
         const bool gather = K0 == 0;
         auto t1 = now();
@@ -757,6 +769,10 @@ public:
             d_nbr = (int*)slot(S_NBR, sizeof(int) * std::max<long long>(1, total_neighbors));
             d_area = (double*)slot(S_AREA, sizeof(double) * std::max<long long>(1, total_neighbors));
             be_.compact(n_, d_cnt, d_ptr, d_nbr_tmp, d_area_tmp, kFaceStride, d_nbr, d_area);
+            if (!piece_cells_.empty())
+                be_.piece_copy((int)piece_cells_.size(), (const int*)slots_[S_XCELLS].p, d_orig,
+                               (const long long*)slots_[S_XOFF].p, (const int*)slots_[S_XNBR].p,
+                               (const double*)slots_[S_XAREA].p, d_cnt, d_ptr, d_nbr, d_area);
         }
         be_.sync();
         st.t_compact_ms += ms_since(t4);
@@ -833,7 +849,9 @@ private:
         S_FILT, S_COFF, S_AITEMS, S_TITEMS, S_CENS_C, S_CENS_F, S_CENS_N, S_ABORT, S_E2,
         S_QKEYS, S_QKEYS2, S_QVALS, S_QVALS2, S_QPTS, S_QPF, S_QROW, S_QPOS, S_QR2, S_QREDO,
         S_QLI2, S_QLD, S_QLD2, S_MCNT, S_MOFF, S_GCNT, S_GOFF, S_GBOX, S_GXYZ, S_GW, S_GSRC,
-        S_GREACH, S_GWRAP, S_COUNT
+        S_GREACH, S_GWRAP, S_PCELLS, S_PITEMS, S_PLPTR, S_PLLEN, S_PBOX, S_PVOL, S_PMG, S_PE2,
+        S_PCNT, S_PNBR, S_PAREA, S_PFLAGS, S_PR2, S_PPTR, S_PIDX, S_XCELLS, S_XOFF, S_XNBR,
+        S_XAREA, S_COUNT
     };
     struct Buf {
         void* p = nullptr;
@@ -893,6 +911,13 @@ private:
                 if (!over8.empty()) {
                     if (radius_pass(8, exact, over8, P, st)) return -1;
                     refresh_flags(over8, P.flags);
+                    std::vector<int> over16;
+                    for (int c : over8)
+                        if (h_flags[c] & CELL_OVERFLOW) over16.push_back(c);
+                    if (!over16.empty() && !P.mesh_mode) {
+                        if (piece_pass(exact, over16, P, st)) return -1;
+                        refresh_flags(over16, P.flags);
+                    }
                 }
             }
         }
@@ -949,6 +974,198 @@ private:
         return 0;
     }
 
+    // Cells over the plane limit (more than 256 facets), each computed in
+    // pieces: its complete radius list (as radius_pass), one item per octant
+    // of the instance's box around the seed (512 corner slots), a piece still
+    // over the limit split into the eight sub-boxes of its box (at most
+    // kPieceLevels levels), the finished pieces merged per cell.  The merged
+    // neighbour lists are kept for the CSR.  Cells whose pieces fail stay
+    // flagged.
+    int piece_pass(bool exact, const std::vector<int>& cells, ClipParams P, EngineStats& st) {
+        const int m = (int)cells.size();
+        int* d_cells = (int*)slot(S_PCELLS, sizeof(int) * m);
+        be_.h2d(d_cells, cells.data(), sizeof(int) * m);
+        float* d_rho2 = (float*)slot(S_RHO2, sizeof(float) * m);
+        PowerFilter* d_filt = P.ball ? (PowerFilter*)slot(S_FILT, sizeof(PowerFilter) * m) : nullptr;
+        be_.radius_r2(d_cells, m, d_pts, P.r2, d_inst_of, d_wmax_, weighted_ ? 1 : 0, P.ball,
+                      d_box_, d_rho2, d_filt);
+        int* d_rcnt = (int*)slot(S_RCNT, sizeof(int) * m);
+        be_.radius_count(d_pf, d_nodes, IL, d_cells, m, d_rho2, d_filt, d_rcnt);
+        long long* d_roff = (long long*)slot(S_ROFF, sizeof(long long) * (m + 1));
+        be_.scan_i32(d_rcnt, d_roff, m);
+        std::vector<long long> roff(m + 1);
+        std::vector<int> rcnt(m);
+        be_.d2h(roff.data(), d_roff, sizeof(long long) * (m + 1));
+        be_.d2h(rcnt.data(), d_rcnt, sizeof(int) * m);
+        const long long total = roff[m];
+        if (total > list_chunk) return 0;  // the cells stay failed
+        st.radius_entries += total;
+        const size_t cap = (size_t)std::max<long long>(total, 1);
+        int* d_rl = (int*)slot(S_RLIST, sizeof(int) * cap);
+        float* d_rd = (float*)slot(S_RD2, sizeof(float) * cap);
+        int* d_rl2 = (int*)slot(S_RLIST2, sizeof(int) * cap);
+        float* d_rd2 = (float*)slot(S_RD22, sizeof(float) * cap);
+        be_.radius_fill(d_pf, d_nodes, IL, d_cells, m, d_rho2, d_filt, d_roff, 0, d_rl, d_rd);
+        be_.seg_sort(d_rd, d_rl, d_rd2, d_rl2, total, m, d_roff);
+        // the octants, then level after level
+        double* d_b0 = (double*)slot(S_PBOX, sizeof(double) * 6 * kPieces * m);
+        be_.piece_setup(kPieces * m, d_cells, d_pts, d_inst_of, d_box_, d_b0);
+        std::vector<double> box(6 * (size_t)kPieces * m);
+        be_.d2h(box.data(), d_b0, sizeof(double) * box.size());
+        std::vector<int> owner(kPieces * (size_t)m);
+        for (int q = 0; q < kPieces * m; ++q) owner[q] = q / kPieces;
+        std::vector<std::vector<int>> fin(m);
+        std::vector<char> bad(m, 0);
+        long long npieces = 0;
+        const uint32_t stop = CELL_NEED_K | CELL_BAD_TOPO;
+        for (int level = 0; !owner.empty(); ++level) {
+            const int nq = (int)owner.size();
+            if (level == kPieceLevels || npieces + nq > kMaxPieces) {
+                for (int t : owner) bad[t] = 1;
+                break;
+            }
+            const long long base = npieces;
+            npieces += nq;
+            std::vector<int> items(nq);
+            std::vector<long long> lptr(nq);
+            std::vector<int> llen(nq);
+            for (int q = 0; q < nq; ++q) {
+                items[q] = cells[owner[q]];
+                lptr[q] = roff[owner[q]];
+                llen[q] = rcnt[owner[q]];
+            }
+            int* p_items = (int*)slot(S_PITEMS, sizeof(int) * nq);
+            long long* p_lptr = (long long*)slot(S_PLPTR, sizeof(long long) * nq);
+            int* p_llen = (int*)slot(S_PLLEN, sizeof(int) * nq);
+            double* p_box = (double*)slot(S_PBOX, sizeof(double) * 6 * nq);
+            be_.h2d(p_items, items.data(), sizeof(int) * nq);
+            be_.h2d(p_lptr, lptr.data(), sizeof(long long) * nq);
+            be_.h2d(p_llen, llen.data(), sizeof(int) * nq);
+            be_.h2d(p_box, box.data(), sizeof(double) * 6 * nq);
+            const size_t kb = sizeof(double) * base, kn = sizeof(double) * npieces;
+            double* p_vol = (double*)grow_keep(S_PVOL, kb, kn);
+            double* p_mg = (double*)grow_keep(S_PMG, 3 * kb, 3 * kn);
+            double* p_e2 = P.e2 ? (double*)grow_keep(S_PE2, kb, kn) : nullptr;
+            int* p_cnt = (int*)grow_keep(S_PCNT, sizeof(int) * base, sizeof(int) * npieces);
+            int* p_nbr = (int*)grow_keep(S_PNBR, sizeof(int) * base * kFaceStride,
+                                         sizeof(int) * npieces * kFaceStride);
+            double* p_area = (double*)grow_keep(S_PAREA, kb * kFaceStride, kn * kFaceStride);
+            uint32_t* p_flags = (uint32_t*)grow_keep(S_PFLAGS, sizeof(uint32_t) * base,
+                                                     sizeof(uint32_t) * npieces);
+            double* p_r2 = (double*)grow_keep(S_PR2, kb, kn);
+            ClipParams Q = P;
+            Q.items = p_items;
+            Q.n_items = nq;
+            Q.list = d_rl2;
+            Q.list_ptr = p_lptr;
+            Q.list_len = p_llen;
+            Q.list_stride = 0;
+            Q.list_complete = 1;
+            Q.item_box = p_box;
+            Q.piece = 1;
+            Q.vol = p_vol + base;
+            Q.mg = p_mg + 3 * base;
+            Q.e2 = p_e2 ? p_e2 + base : nullptr;
+            Q.nbr_cnt = p_cnt + base;
+            Q.nbr_tmp = p_nbr + base * kFaceStride;
+            Q.area_tmp = p_area + base * kFaceStride;
+            Q.face_stride = kFaceStride;
+            Q.flags = p_flags + base;
+            Q.r2 = p_r2 + base;
+            Q.ball = nullptr;
+            be_.clip(8, exact, false, Q);
+            std::vector<uint32_t> f(nq);
+            be_.d2h(f.data(), p_flags + base, sizeof(uint32_t) * nq);
+            std::vector<int> next_owner;
+            std::vector<double> next_box;
+            for (int q = 0; q < nq; ++q) {
+                const int t = owner[q];
+                if ((f[q] & CELL_DONE) && !(f[q] & (stop | CELL_OVERFLOW))) {
+                    fin[t].push_back(int(base + q));
+                } else if ((f[q] & CELL_OVERFLOW) && !(f[q] & stop)) {
+                    const double* b = box.data() + 6 * (size_t)q;
+                    for (int k = 0; k < kPieces; ++k) {
+                        next_owner.push_back(t);
+                        for (int a = 0; a < 3; ++a) {
+                            const double mid = 0.5 * (b[a] + b[3 + a]);
+                            const bool upper = (k >> a) & 1;
+                            next_box.push_back(upper ? mid : b[a]);
+                        }
+                        for (int a = 0; a < 3; ++a) {
+                            const double mid = 0.5 * (b[a] + b[3 + a]);
+                            const bool upper = (k >> a) & 1;
+                            next_box.push_back(upper ? b[3 + a] : mid);
+                        }
+                    }
+                } else {
+                    bad[t] = 1;
+                }
+            }
+            owner.swap(next_owner);
+            box.swap(next_box);
+        }
+        // merge the cells whose pieces all finished
+        std::vector<int> cnt(npieces);
+        be_.d2h(cnt.data(), slots_[S_PCNT].p, sizeof(int) * npieces);
+        std::vector<int> mc;
+        std::vector<long long> pptr(1, 0), xoff;
+        std::vector<int> pidx;
+        long long xt = x_total_;
+        for (int t = 0; t < m; ++t) {
+            if (bad[t]) continue;
+            mc.push_back(cells[t]);
+            long long c = 0;
+            for (int q : fin[t]) {
+                pidx.push_back(q);
+                c += cnt[q];
+            }
+            pptr.push_back((long long)pidx.size());
+            xoff.push_back(xt);
+            xt += c;
+        }
+        const int mm = (int)mc.size();
+        st.nb_pieces += mm;
+        if (mm == 0) return 0;
+        const size_t b0 = piece_cells_.size();
+        piece_cells_.insert(piece_cells_.end(), mc.begin(), mc.end());
+        const size_t b1 = piece_cells_.size();
+        int* x_cells = (int*)grow_keep(S_XCELLS, sizeof(int) * b0, sizeof(int) * b1);
+        long long* x_off = (long long*)grow_keep(S_XOFF, sizeof(long long) * b0, sizeof(long long) * b1);
+        int* x_nbr = (int*)grow_keep(S_XNBR, sizeof(int) * x_total_,
+                                     sizeof(int) * std::max<long long>(xt, 1));
+        double* x_area = (double*)grow_keep(S_XAREA, sizeof(double) * x_total_,
+                                            sizeof(double) * std::max<long long>(xt, 1));
+        x_total_ = xt;
+        be_.h2d(x_cells + b0, mc.data(), sizeof(int) * mm);
+        be_.h2d(x_off + b0, xoff.data(), sizeof(long long) * mm);
+        long long* d_pptr = (long long*)slot(S_PPTR, sizeof(long long) * (mm + 1));
+        int* d_pidx = (int*)slot(S_PIDX, sizeof(int) * std::max<size_t>(pidx.size(), 1));
+        be_.h2d(d_pptr, pptr.data(), sizeof(long long) * (mm + 1));
+        if (!pidx.empty()) be_.h2d(d_pidx, pidx.data(), sizeof(int) * pidx.size());
+        be_.piece_merge(mm, x_cells + b0, d_orig, d_pptr, d_pidx, (const double*)slots_[S_PVOL].p,
+                        (const double*)slots_[S_PMG].p,
+                        P.e2 ? (const double*)slots_[S_PE2].p : nullptr,
+                        (const int*)slots_[S_PCNT].p, (const int*)slots_[S_PNBR].p,
+                        (const double*)slots_[S_PAREA].p, (const uint32_t*)slots_[S_PFLAGS].p,
+                        kFaceStride, x_off + b0, P.vol, P.mg, P.e2, P.nbr_cnt, P.flags, x_nbr,
+                        x_area);
+        be_.sync();
+        return 0;
+    }
+    // slot id grown to `bytes`, its first `keep` bytes kept
+    void* grow_keep(int id, size_t keep, size_t bytes) {
+        if (slots_.size() < S_COUNT) slots_.resize(S_COUNT);
+        Buf& b = slots_[id];
+        if (b.bytes >= bytes) return b.p;
+        size_t nb = std::max(bytes, b.bytes + b.bytes / 4);
+        void* p = be_.dalloc(nb);
+        if (keep) be_.d2d(p, b.p, keep);
+        if (b.p) be_.dfree(b.p);
+        b.p = p;
+        b.bytes = nb;
+        return p;
+    }
+
     static std::chrono::steady_clock::time_point now() { return std::chrono::steady_clock::now(); }
     static double ms_since(std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double, std::milli>(now() - t).count();
@@ -964,6 +1181,10 @@ private:
     double* d_wmax_ = nullptr;
     double* d_r2_ = nullptr;  // squared radius per cell of the last pass (Morton order)
     std::vector<int> h_p0_, h_n_, h_b0_, h_nb_, h_L_;
+    std::vector<int> piece_cells_;  // cells of this pass computed in pieces
+    long long x_total_ = 0;         // their merged neighbour entries
+    static constexpr int kPieceLevels = 4;
+    static constexpr long long kMaxPieces = 1 << 16;
     std::vector<long long> h_t0_, h_leaf_off_;
 };
 

@@ -54,6 +54,14 @@ extern "C" {
                                  max relative mass error), never below the default
                                  1e-7; damped steps keep 1e-7 (Eisenstat-Walker
                                  forcing) */
+#define VGC_OT_REPAIR 512u    /* transport: a start that leaves cells empty is
+                                 repaired instead of dropped — each empty cell's
+                                 weight is raised to the smallest value that puts
+                                 its seed inside its cell with a margin of
+                                 h^2 / 100 (h = (box volume / n)^(1/3)), for at most
+                                 16 rounds (a raised weight can empty a
+                                 neighbour); w = 0 only if cells stay empty.  No
+                                 reference counterpart (the learned start). */
 #define VGC_PERIODIC 256u     /* periodic domain: the box is one period of a
                                  3-torus (seeds inside it); cells, neighbour ids
                                  and facet areas are those of the periodic
@@ -101,6 +109,9 @@ typedef struct vgc_stats {
     uint64_t tuned_waves; /* VGC_AUTOTUNE: wavefronts per work-group chosen (0: none) */
     uint64_t nb_periodic_retries; /* VGC_PERIODIC: passes redone with a wider image
                                      margin */
+    uint64_t nb_pieces;   /* cells over 256 facets computed as eight octant pieces
+                             (ConvexCell grows without bound,
+                             voronoi/convex_cell.cpp:1086-1101) */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in
@@ -285,6 +296,9 @@ typedef struct vgc_ot_stats {
     uint32_t halvings;            /* step halvings of the damped Newton rule     */
     uint32_t used_fallback_start; /* the given start had an empty cell: w = 0    */
     uint32_t converged;
+    uint32_t repair_rounds;       /* VGC_OT_REPAIR: rounds that raised the weights
+                                     of empty cells of the start                 */
+    uint32_t pad;
     double err0, final_err;       /* max_i |m_i - nu_i| / nu_i, first and last   */
     double t_total_ms, t_cells_ms, t_cg_ms;
 } vgc_ot_stats;

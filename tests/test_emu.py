@@ -242,8 +242,7 @@ def test_transport_weights_power_filter(emu_ctx):
 
 def test_cells_beyond_256_corners(emu_ctx):
     # a heavy seed in a uniform cloud: a ball-like cell with 185 facets and
-    # ~370 corners runs in the 512-corner tier; 295 facets exceed the 256
-    # plane slots and are reported as a capacity failure
+    # ~370 corners runs in the 512-corner tier
     x = ga.sample_points(0, 41, 20000)
     x[0] = [0.5, 0.5, 0.5]
     w = np.zeros(len(x))
@@ -256,10 +255,23 @@ def test_cells_beyond_256_corners(emu_ctx):
     da = dict(zip(r.neighbors(0).tolist(), r.facet_areas(0).tolist()))
     for j, a in zip(o["nbr"][0, :o["cnt"][0]].tolist(), o["farea"][0, :o["cnt"][0]].tolist()):
         assert abs(da[j] - a) <= 1e-10 * a + 1e-14
-    w[0] = 0.02
-    r = emu_ctx.cells(x, weights=w)
-    assert r.status == ga.VGC_PARTIAL
-    assert r.stats["nb_failed"] == 1 and r.stats["nb_fail_capacity"] == 1
+    # 295 and 664 facets: over the 256 plane slots, computed in octant
+    # pieces (split again where a piece is still over the limit)
+    for w0, nf in ((0.02, 295), (0.05, 664)):
+        w[0] = w0
+        r = emu_ctx.cells(x, weights=w)
+        assert r.status == ga.VGC_OK and r.stats["nb_failed"] == 0 and r.stats["nb_pieces"] == 1
+        o = oracle.cells(x, w=w, centred=True, stride=1024)
+        assert o["cnt"][0] == nf and len(r.neighbors(0)) == nf
+        vo = np.maximum(o["vol"], 1e-300)
+        assert np.all(np.abs(r.vol - o["vol"]) <= 1e-10 * vo)
+        assert np.abs(r.mg[0] / r.vol[0] - o["mg"][0] / o["vol"][0]).max() < 1e-12
+        bad = [i for i in range(len(x))
+               if set(r.neighbors(i).tolist()) != set(o["nbr"][i, :o["cnt"][i]].tolist())]
+        assert not bad
+        da = dict(zip(r.neighbors(0).tolist(), r.facet_areas(0).tolist()))
+        for j, a in zip(o["nbr"][0, :o["cnt"][0]].tolist(), o["farea"][0, :o["cnt"][0]].tolist()):
+            assert abs(da[j] - a) <= 1e-9 * a + 1e-13
 
 
 def test_lloyd_batch_equals_separate_runs(emu_ctx):

@@ -366,3 +366,30 @@ def test_cells_vs_reference_build(gpu_ctx, weighted):
         if set(a.tolist()) != set(ref["nbr"][i, : ref["cnt"][i]].tolist()):
             bad += 1
     assert bad == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w0,nf", [(0.02, 295), (0.1, 1350)])
+def test_cells_over_plane_limit_in_pieces_vs_oracle(gpu_ctx, emu_ctx, w0, nf):
+    # one seed with a large weight: 295 / 1350 facets, over the 256 plane
+    # slots; computed in octant pieces (split again where still over)
+    x = ga.sample_points(0, 41, 20000)
+    x[0] = [0.5, 0.5, 0.5]
+    w = np.zeros(len(x))
+    w[0] = w0
+    print(f"pieces: w0 {w0}", flush=True)
+    r = gpu_ctx.cells(x, weights=w)
+    assert r.stats["nb_failed"] == 0 and r.stats["nb_pieces"] == 1
+    o = oracle.cells(x, w=w, centred=True, stride=2048)
+    assert o["cnt"][0] == nf and len(r.neighbors(0)) == nf
+    vo = np.maximum(o["vol"], 1e-300)
+    assert np.all(np.abs(r.vol - o["vol"]) <= 1e-10 * vo)
+    assert np.abs(r.mg[0] / r.vol[0] - o["mg"][0] / o["vol"][0]).max() < 1e-12
+    bad = [i for i in range(len(x))
+           if set(r.neighbors(i).tolist()) != set(o["nbr"][i, :o["cnt"][i]].tolist())]
+    assert not bad
+    da = dict(zip(r.neighbors(0).tolist(), r.facet_areas(0).tolist()))
+    for j, a in zip(o["nbr"][0, :o["cnt"][0]].tolist(), o["farea"][0, :o["cnt"][0]].tolist()):
+        assert abs(da[j] - a) <= 1e-9 * a + 1e-13
+    e = emu_ctx.cells(x, weights=w)
+    assert np.array_equal(e.vol, r.vol) and np.array_equal(e.farea, r.farea)

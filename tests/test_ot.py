@@ -183,3 +183,23 @@ def test_ot_solve_levels(emu_ctx):
     for bad in ([0, 500], [0, 500, 500, 4000], [1, 4000]):
         with pytest.raises(ga.VgcError):
             emu_ctx.ot_solve_levels(x, nu, np.array(bad), tol=1e-5)
+
+
+def test_repaired_start_keeps_a_start_with_a_few_empty_cells(emu_ctx):
+    # a converged solution with one weight lowered until its cell empties:
+    # the repair raises that weight back (one round), the solve keeps the
+    # start and needs fewer Newton steps than from w = 0
+    x = ga.sample_points(0, 31, 3000)
+    nu = np.full(3000, 1.0 / 3000)
+    ws, s0 = emu_ctx.ot_solve(x, nu, tol=1e-6)
+    w = ws.copy()
+    w[[10, 20]] -= 0.05
+    r = emu_ctx.cells(x, w)
+    assert (r.vol[[10, 20]] == 0).all()
+    _, plain = emu_ctx.ot_solve(x, nu, w0=w, tol=1e-6)
+    assert plain["used_fallback_start"] == 1
+    w2, st = emu_ctx.ot_solve(x, nu, w0=w, tol=1e-6, repair=True)
+    assert st["used_fallback_start"] == 0 and st["repair_rounds"] >= 1 and st["converged"]
+    assert st["newton_iters"] < plain["newton_iters"]
+    o = oracle.cells(x, w=w2, centred=True)
+    assert np.abs(o["vol"] - nu).max() <= 1.01e-6 * nu[0]
