@@ -459,41 +459,25 @@ struct EmuBackend {
                   double* cg) {
         for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
     }
-    // the three PCG kernels, blocks and reduction orders as on the device
+    // the four PCG kernels, reduction orders as on the device
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
                  const double* diag, double* x, double* r, double* z, double* p, double* q,
                  double* cg) {
         if (cg[6] != 0.0) return;
         const double beta = cg[4];
         for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
-        const int nb1 = (n + kCgRows - 1) / kCgRows;
-        std::vector<double> part(std::max(nb1, 2 * ((n + 255) / 256)));
 #pragma omp parallel for schedule(static)
-        for (int b = 0; b < nb1; ++b) {
-            double acc[256] = {};
-            for (int g = 0; g < kCgRows; ++g) {
-                const int i = b * kCgRows + g;
-                if (i >= n) break;
-                double sl[kSpmvLanes];
-                for (int l = 0; l < kSpmvLanes; ++l) sl[l] = ot_spmv_lane(i, l, ptr, col, val, p);
-                for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
-                    double t[kSpmvLanes];
-                    for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
-                    for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
-                }
-                const double qi = sl[0] + diag[i] * p[i];
-                q[i] = qi;
-                acc[g * kSpmvLanes] = p[i] * qi;
+        for (int i = 0; i < n; ++i) {
+            double sl[kSpmvLanes];
+            for (int l = 0; l < kSpmvLanes; ++l) sl[l] = ot_spmv_lane(i, l, ptr, col, val, p);
+            for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
+                double t[kSpmvLanes];
+                for (int l = 0; l < kSpmvLanes; ++l) t[l] = sl[l] + sl[l ^ m];
+                for (int l = 0; l < kSpmvLanes; ++l) sl[l] = t[l];
             }
-            part[b] = tree_sum(acc);
+            q[i] = sl[0] + diag[i] * p[i];
         }
-        double sh[256], sh2[256];
-        for (int t = 0; t < 256; ++t) {
-            double a = 0.0;
-            for (int b = t; b < nb1; b += 256) a += part[b];
-            sh[t] = a;
-        }
-        const double pq = tree_sum(sh);
+        const double pq = red_sum(n, [&](int i) { return p[i] * q[i]; });
         cg[2] = pq;
         if (!(pq > 0.0)) {
             cg[3] = 0.0;
@@ -501,35 +485,9 @@ struct EmuBackend {
             return;
         }
         cg[3] = cg[0] / pq;
-        const double alpha = cg[3];
-        const int nb2 = (n + 255) / 256;
-#pragma omp parallel for schedule(static)
-        for (int b = 0; b < nb2; ++b) {
-            double a1[256] = {}, a2[256] = {};
-            for (int t = 0; t < 256; ++t) {
-                const int i = b * 256 + t;
-                if (i >= n) break;
-                x[i] += alpha * p[i];
-                const double ri = r[i] - alpha * q[i];
-                r[i] = ri;
-                const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
-                z[i] = zi;
-                a1[t] = ri * zi;
-                a2[t] = ri * ri;
-            }
-            part[b] = tree_sum(a1);
-            part[nb2 + b] = tree_sum(a2);
-        }
-        for (int t = 0; t < 256; ++t) {
-            double a = 0.0, c = 0.0;
-            for (int b = t; b < nb2; b += 256) {
-                a += part[b];
-                c += part[nb2 + b];
-            }
-            sh[t] = a;
-            sh2[t] = c;
-        }
-        const double rz = tree_sum(sh), rr = tree_sum(sh2);
+        ot_pcg_update(n, cg[3], p, q, diag, x, r, z);
+        const double rz = red_sum(n, [&](int i) { return r[i] * z[i]; });
+        const double rr = red_sum(n, [&](int i) { return r[i] * r[i]; });
         cg[1] = rr;
         cg[7] += 1.0;
         if (rr <= cg[5]) {

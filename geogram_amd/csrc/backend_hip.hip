@@ -763,34 +763,37 @@ __global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
     if (i < n) p[i] = z[i] + beta * p[i];
 }
 
-// Second: q = A p, one 16-lane group per row (kCgRows rows per block), and
-// p.q; the last block to finish sets alpha = rz / p.q (cg[3]) or the
-// breakdown flag.
-__global__ __launch_bounds__(256) void k_cg_spmv_pq(int n, const long long* __restrict__ ptr,
-                                                    const int* __restrict__ col,
-                                                    const double* __restrict__ val,
-                                                    const double* __restrict__ diag,
-                                                    const double* __restrict__ p, double* q,
-                                                    double* cg, double* part, unsigned* ctr) {
-    __shared__ double sh[256];
+// Second: q = A p, one 16-lane group per row.
+__global__ __launch_bounds__(256) void k_cg_spmv(int n, const long long* __restrict__ ptr,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ diag,
+                                                 const double* __restrict__ p, double* q,
+                                                 const double* __restrict__ cg) {
     if (cg[6] != 0.0) return;
-    const int g = threadIdx.x / kSpmvLanes, l = threadIdx.x % kSpmvLanes;
-    const int i = blockIdx.x * kCgRows + g;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
     double s = i < n ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
 #pragma unroll
     for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    double acc = 0.0;
-    if (i < n && l == 0) {
-        const double pi = p[i], qi = s + diag[i] * pi;
-        q[i] = qi;
-        acc = pi * qi;
-    }
-    const double t = block_sum(acc, sh);
+    if (i < n && l == 0) q[i] = s + diag[i] * p[i];
+}
+
+// Third: p.q (kRedBlocks blocks, the fixed-order two-level reduction); the
+// last block to finish sets alpha = rz / p.q (cg[3]) or the breakdown flag.
+__global__ __launch_bounds__(256) void k_cg_dot_alpha(int n, const double* __restrict__ p,
+                                                      const double* __restrict__ q, double* cg,
+                                                      double* part, unsigned* ctr) {
+    __shared__ double sh[256];
+    if (cg[6] != 0.0) return;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += p[i] * q[i];
+    const double t = block_sum(s, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
     if (!cg_last_block(ctr)) return;
     __syncthreads();
     double a = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) a += cg_read_part(part + b);
+    for (int b = threadIdx.x; b < kRedBlocks; b += 256) a += cg_read_part(part + b);
     const double pq = block_sum(a, sh);
     if (threadIdx.x == 0) {
         cg[2] = pq;
@@ -804,8 +807,8 @@ __global__ __launch_bounds__(256) void k_cg_spmv_pq(int n, const long long* __re
     }
 }
 
-// Third: x += alpha p, r -= alpha q, z = r / diag, r.z and r.r (256
-// elements per block); the last block counts the iteration, stops on
+// Fourth: x += alpha p, r -= alpha q, z = r / diag, r.z and r.r
+// (kRedBlocks blocks); the last block counts the iteration, stops on
 // r.r <= tol^2 b.b, else sets beta = r.z / previous r.z (cg[4]).
 __global__ __launch_bounds__(256) void k_cg_update_beta(int n, const double* __restrict__ p,
                                                         const double* __restrict__ q,
@@ -815,30 +818,28 @@ __global__ __launch_bounds__(256) void k_cg_update_beta(int n, const double* __r
     __shared__ double sh[256], sh2[256];
     if (cg[6] != 0.0) return;
     const double alpha = cg[3];
-    const int nb = (int)gridDim.x;
-    const int i = blockIdx.x * 256 + threadIdx.x;
     double s1 = 0.0, s2 = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double zi = diag[i] > 0.0 ? ri / diag[i] : 0.0;
         z[i] = zi;
-        s1 = ri * zi;
-        s2 = ri * ri;
+        s1 += ri * zi;
+        s2 += ri * ri;
     }
     double t1, t2;
     block_sum2(s1, s2, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = t1;
-        part[nb + blockIdx.x] = t2;
+        part[kRedBlocks + blockIdx.x] = t1;
+        part[2 * kRedBlocks + blockIdx.x] = t2;
     }
     if (!cg_last_block(ctr)) return;
     __syncthreads();
     s1 = s2 = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256) {
-        s1 += cg_read_part(part + b);
-        s2 += cg_read_part(part + nb + b);
+    for (int b = threadIdx.x; b < kRedBlocks; b += 256) {
+        s1 += cg_read_part(part + kRedBlocks + b);
+        s2 += cg_read_part(part + 2 * kRedBlocks + b);
     }
     double rz, rr;
     block_sum2(s1, s2, sh, sh2, rz, rr);
@@ -1625,17 +1626,18 @@ struct HipBackend {
                  double* cg) {
         double* part = cg_part(n);
         k_cg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, z, p, cg);
-        k_cg_spmv_pq<<<blocks_for(n, kCgRows), 256, 0, stream_>>>(n, ptr, col, val, diag, p, q, cg,
-                                                                   part, cg_ctr_);
-        k_cg_update_beta<<<blocks_for(n, 256), 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part,
-                                                                   cg_ctr_ + 1);
+        k_cg_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(n, ptr, col, val,
+                                                                                   diag, p, q, cg);
+        k_cg_dot_alpha<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, cg, part, cg_ctr_);
+        k_cg_update_beta<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part,
+                                                          cg_ctr_ + 1);
         check();
     }
-    // block partials (one per block of the product, two per block of the
+    // block partials (one per block of the dot, two per block of the
     // update) and the two last-block counters of the PCG kernels (zeroed
     // when allocated; each last block resets its counter)
-    double* cg_part(int n) {
-        const long long need = std::max<long long>(blocks_for(n, kCgRows), 2ll * blocks_for(n, 256));
+    double* cg_part(int) {
+        const long long need = 3ll * kRedBlocks;
         if (cg_part_cap_ < need) {
             if (cg_part_) VGC_HIP_CHECK(hipFree(cg_part_));
             VGC_HIP_CHECK(hipMalloc(&cg_part_, sizeof(double) * (size_t)need));

@@ -61,12 +61,7 @@ VGC_SD double ot_spmv_row(int i, const long long* __restrict__ ptr, const int* _
 // the lanes are combined by a butterfly over offsets 8, 4, 2, 1, then the
 // diagonal term is added.  The CPU emulation follows the same order.
 constexpr int kSpmvLanes = 16;
-// the two-kernel PCG iteration: a fixed grid of kCgBlocks blocks of 256
-// threads (kCgRows rows of the product per block and stride step); its
-// reductions add per-thread partials by the halving-stride block tree, then
-// the block partials (thread t: partials t, t + 256, ...) by the same tree
-constexpr int kCgBlocks = 1024;
-constexpr int kCgRows = 256 / kSpmvLanes;
+
 VGC_SD double ot_spmv_lane(int i, int l, const long long* __restrict__ ptr,
                            const int* __restrict__ col, const double* __restrict__ val,
                            const double* __restrict__ x) {

@@ -104,12 +104,14 @@ public:
 
 // -------------------------------------------------- restricted Voronoi
 
-// Volumetric restricted Voronoi diagram of the Delaunay vertices in the box
-// of `mesh`.  The integrals compute_centroids_in_volume and
-// compute_CVT_func_grad_in_volume run on the GPU (accumulated into the
-// caller's zeroed arrays, RVD.cpp:468-478, 878-899); the primal
-// tetrahedra of compute_RDT too.  Surface and mesh-traversal methods are
-// not part of the meshless path.
+// Restricted Voronoi diagram of the Delaunay vertices: in the box of `mesh`
+// (volumetric, the default) or on the triangles of `mesh` (set_volumetric
+// (false)).  compute_centroids_in_volume, compute_CVT_func_grad_in_volume
+// and compute_centroids_on_surface run on the GPU (accumulated into the
+// caller's zeroed arrays, RVD.cpp:468-478, 878-899, 371-415; a vertex
+// attribute "weight" is the surface density, as in
+// ComputeCentroidsWeighted); the primal tetrahedra of compute_RDT too.
+// Mesh-traversal callbacks are not part of the meshless path.
 class VgcRestrictedVoronoiDiagram : public RestrictedVoronoiDiagram {
 public:
     VgcRestrictedVoronoiDiagram(Delaunay* delaunay, Mesh* mesh)
@@ -119,7 +121,7 @@ public:
         volumetric_ = true;
     }
 
-    void set_volumetric(bool x) override { geo_assert(x); }
+    void set_volumetric(bool x) override { volumetric_ = x; }
     void set_check_SR(bool x) override { geo_assert(x); }
     void set_exact_predicates(bool) override {}
     bool exact_predicates() const override { return true; }
@@ -188,7 +190,30 @@ public:
         }
     }
 
-    void compute_centroids_on_surface(double*, double*) override { geo_assert_not_reached; }
+    void compute_centroids_on_surface(double* mg, double* m) override {
+        const Mesh* S = mesh();
+        const index_t n = delaunay()->nb_vertices(), nv = S->vertices.nb(), nt = S->facets.nb();
+        std::vector<double> V(3 * size_t(nv)), w;
+        std::vector<int32_t> F(3 * size_t(nt));
+        for (index_t v = 0; v < nv; ++v)
+            for (coord_index_t c = 0; c < 3; ++c) V[3 * v + c] = S->vertices.point_ptr(v)[c];
+        for (index_t f = 0; f < nt; ++f) {
+            geo_assert(S->facets.nb_vertices(f) == 3);  // triangulated surfaces
+            for (index_t k = 0; k < 3; ++k) F[3 * f + k] = int32_t(S->facets.vertex(f, k));
+        }
+        Attribute<double> weight;
+        weight.bind_if_is_defined(S->vertices.attributes(), "weight");
+        if (weight.is_bound()) w.assign(weight.data(), weight.data() + nv);
+        std::vector<double> g(3 * size_t(n)), mm(n);
+        vgc_surface_stats st;
+        vgc_check(vgc_surface_centroids(vgc_context(), delaunay()->vertex_ptr(0), n, V.data(), nv,
+                                        F.data(), nt, w.empty() ? nullptr : w.data(), 0, g.data(),
+                                        mm.data(), &st));
+        for (index_t i = 0; i < n; ++i) {
+            m[i] += mm[i];
+            for (coord_index_t c = 0; c < 3; ++c) mg[3 * i + c] += g[3 * i + c];
+        }
+    }
     void compute_CVT_func_grad_on_surface(double&, double*) override { geo_assert_not_reached; }
     bool compute_initial_sampling_on_surface(double*, index_t, bool) override {
         geo_assert_not_reached;
