@@ -131,7 +131,7 @@ def _per_dispatch(rows, kernel):
 
 
 def counters_summary(fetch_csv, write_csv, sq_csvs, kernel, title, commands, skip_first,
-                     calib_bytes, notes=""):
+                     calib_bytes, notes="", sq_skip_first=None):
     out = [f"# {title}", "", notes, "", "Commands (one counter set per run):", "", "```"]
     out += commands + ["```", "", f"kernel_sources_sha1: {kernel_sources_sha1()}", ""]
     scale = {}
@@ -157,9 +157,11 @@ def counters_summary(fetch_csv, write_csv, sq_csvs, kernel, title, commands, ski
             for c, x in v.items():
                 sq.setdefault(c, []).append(x)
     if sq:
+        k = skip_first if sq_skip_first is None else sq_skip_first
+        sq = {c: (v[k:] or v) for c, v in sq.items()}
         out += ["| counter | mean per launch |", "|---|---:|"]
         for c in sorted(sq):
-            out.append(f"| {c} | {statistics.fmean(sq[c][skip_first:] or sq[c]):,.1f} |")
+            out.append(f"| {c} | {statistics.fmean(sq[c]):,.1f} |")
         if "SQ_ACTIVE_INST_VALU" in sq and "SQ_BUSY_CYCLES" in sq:
             v = statistics.fmean(sq["SQ_ACTIVE_INST_VALU"])
             busy = statistics.fmean(sq["SQ_BUSY_CYCLES"]) / N_SE
@@ -187,6 +189,8 @@ def main(argv=None):
     c.add_argument("--command", action="append", required=True)
     c.add_argument("--skip-first", type=int, default=0,
                    help="launches to leave out (work-counter pass, warmup)")
+    c.add_argument("--sq-skip-first", type=int, default=None,
+                   help="launches of the SQ runs to leave out (default: --skip-first)")
     c.add_argument("--calib-bytes", type=float, default=float(256 << 20))
     c.add_argument("--notes", default="")
     c.add_argument("--out", required=True)
@@ -195,7 +199,7 @@ def main(argv=None):
         text = trace_summary(a.csv_dir, a.prefix, a.kernel, a.title, a.command, a.notes)
     else:
         text = counters_summary(a.fetch, a.write, a.sq, a.kernel, a.title, a.command, a.skip_first,
-                                a.calib_bytes, a.notes)
+                                a.calib_bytes, a.notes, a.sq_skip_first)
     with open(a.out, "w") as f:
         f.write(text)
     print(text)
