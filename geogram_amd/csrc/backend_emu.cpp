@@ -517,6 +517,10 @@ struct EmuBackend {
             }
         }
     }
+    void range_map(int n, const int* inst, const long long* soff, const long long* off,
+                   const int* orig, int* out) {
+        for (int r = 0; r < n; ++r) out[r] = orig[off[inst[r]] + (r - soff[inst[r]])];
+    }
     void gather_f64(int n, const int* orig, const double* src, double* dst) {
         for (int m = 0; m < n; ++m) dst[m] = src[orig[m]];
     }

@@ -882,6 +882,14 @@ __global__ void k_perm_rows(int n, const int* __restrict__ orig, const int* __re
         val_m[o + k] = val[k];
     }
 }
+// the Morton position (= caller position range) of row r of a subset of
+// instances, through orig: out[r] = orig[off[k] + r - soff[k]], k = inst[r]
+__global__ void k_range_map(int n, const int* __restrict__ inst, const long long* __restrict__ soff,
+                            const long long* __restrict__ off, const int* __restrict__ orig,
+                            int* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = orig[off[inst[r]] + (r - soff[inst[r]])];
+}
 __global__ void k_gather_f64(int n, const int* __restrict__ orig, const double* __restrict__ src,
                              double* dst) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1662,6 +1670,11 @@ struct HipBackend {
                    const double* val, const long long* ptr_m, int* col_m, double* val_m) {
         k_perm_rows<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
             n, orig, inv, ptr, col, val, ptr_m, col_m, val_m);
+        check();
+    }
+    void range_map(int n, const int* inst, const long long* soff, const long long* off,
+                   const int* orig, int* out) {
+        if (n > 0) k_range_map<<<blocks_for(n, 256), 256, 0, stream_>>>(n, inst, soff, off, orig, out);
         check();
     }
     void gather_f64(int n, const int* orig, const double* src, double* dst) {

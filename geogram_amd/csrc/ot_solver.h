@@ -347,9 +347,48 @@ public:
             double* d_tol2 = (double*)slot(28, sizeof(double) * (size_t)K);
             be_.h2d(d_tol2, tol2.data(), sizeof(double) * (size_t)K);
             // Morton order (instances keep their ranges: the order leads
-            // with the instance)
-            const MortonSystem M = to_morton(n, nnz, d_val, d_diag, d_b);
-            be_.cgb_start(T, M.b, M.diag, M.x, d_r, d_z, d_p, d_cg, d_tol2);
+            // with the instance), restricted to the rows of the instances
+            // still active: the others' CG would only skip their rows
+            int n_sub = n;
+            const int* omap = E_.d_orig;
+            SegTable Ts = T;
+            int n_act = 0;
+            for (int k = 0; k < K; ++k) n_act += active[k] ? 1 : 0;
+            if (n_act < K) {
+                std::vector<long long> soff(K + 1, 0);
+                std::vector<int> sbi, sbx, scnt(K, 0), sfirst(K, 0), sp0(K);
+                for (int k = 0; k < K; ++k) {
+                    sp0[k] = (int)soff[k];
+                    soff[k + 1] = soff[k] + (active[k] ? pb.inst_n[k] : 0);
+                    scnt[k] = active[k] ? cnt[k] : 0;
+                    sfirst[k] = (int)sbi.size();
+                    for (int b = 0; b < scnt[k]; ++b) {
+                        sbi.push_back(k);
+                        sbx.push_back(b);
+                    }
+                }
+                n_sub = (int)soff[K];
+                const int nbs = (int)sbi.size();
+                long long* d_soff = (long long*)slot(41, sizeof(long long) * (K + 1));
+                int* d_stab = (int*)slot(42, sizeof(int) * (2 * (size_t)nbs + 3 * (size_t)K));
+                int* d_sinst = (int*)slot(43, sizeof(int) * std::max(n_sub, 1));
+                int* d_omap = (int*)slot(44, sizeof(int) * std::max(n_sub, 1));
+                be_.h2d(d_soff, soff.data(), sizeof(long long) * (K + 1));
+                std::vector<int> stab;
+                stab.insert(stab.end(), sbi.begin(), sbi.end());
+                stab.insert(stab.end(), sbx.begin(), sbx.end());
+                stab.insert(stab.end(), scnt.begin(), scnt.end());
+                stab.insert(stab.end(), sfirst.begin(), sfirst.end());
+                stab.insert(stab.end(), sp0.begin(), sp0.end());
+                be_.h2d(d_stab, stab.data(), sizeof(int) * stab.size());
+                be_.fill_inst(d_sinst, n_sub, d_stab + 2 * nbs + 2 * K, K);
+                be_.range_map(n_sub, d_sinst, d_soff, d_off, E_.d_orig, d_omap);
+                omap = d_omap;
+                Ts = SegTable{K, nbs, d_soff, d_stab, d_stab + nbs, d_stab + 2 * nbs,
+                              d_stab + 2 * nbs + K, d_sinst};
+            }
+            const MortonSystem M = to_morton(n_sub, nnz, d_val, d_diag, d_b, omap);
+            be_.cgb_start(Ts, M.b, M.diag, M.x, d_r, d_z, d_p, d_cg, d_tol2);
             std::vector<double> h(8 * (size_t)K);
             be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
             int issued = 0;
@@ -360,11 +399,13 @@ public:
             };
             while (!all_done() && issued < cg_max) {
                 const int kk = std::min(kCgCheck, cg_max - issued);
-                be_.cgb_iters(kk, T, n, M.ptr, M.col, M.val, M.diag, M.x, d_r, d_z, d_p, d_q, d_cg);
+                be_.cgb_iters(kk, Ts, n_sub, M.ptr, M.col, M.val, M.diag, M.x, d_r, d_z, d_p, d_q,
+                              d_cg);
                 issued += kk;
                 be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
             }
-            be_.scatter_f64(n, E_.d_orig, M.x, d_x);
+            if (n_sub < n) be_.dzero(d_x, sizeof(double) * n);
+            be_.scatter_f64(n_sub, omap, M.x, d_x);
             const double tcg = ms_since(tg);
             for (int k = 0; k < K; ++k) {
                 st[k].t_cg_ms += tcg;
@@ -472,8 +513,9 @@ private:
         double* x;
     };
     MortonSystem to_morton(int n, long long nnz, const double* val, const double* diag,
-                           const double* b) {
-        int* inv = (int*)slot(33, sizeof(int) * (size_t)n);
+                           const double* b, const int* orig = nullptr) {
+        if (!orig) orig = E_.d_orig;  // rows: orig[m] is the caller's row of row m
+        int* inv = (int*)slot(33, sizeof(int) * (size_t)E_.total_points());
         int* len = (int*)slot(34, sizeof(int) * (size_t)n);
         long long* ptr = (long long*)slot(35, sizeof(long long) * ((size_t)n + 1));
         int* col = (int*)slot(36, sizeof(int) * (size_t)std::max<long long>(nnz, 1));
@@ -481,12 +523,12 @@ private:
         double* dg = (double*)slot(38, sizeof(double) * (size_t)n);
         double* bb = (double*)slot(39, sizeof(double) * (size_t)n);
         double* x = (double*)slot(40, sizeof(double) * (size_t)n);
-        be_.inv_perm(n, E_.d_orig, inv);
-        be_.perm_row_len(n, E_.d_orig, E_.d_ptr, len);
+        be_.inv_perm(n, orig, inv);
+        be_.perm_row_len(n, orig, E_.d_ptr, len);
         be_.scan_i32(len, ptr, n);
-        be_.perm_rows(n, E_.d_orig, inv, E_.d_ptr, E_.d_nbr, val, ptr, col, v);
-        be_.gather_f64(n, E_.d_orig, diag, dg);
-        be_.gather_f64(n, E_.d_orig, b, bb);
+        be_.perm_rows(n, orig, inv, E_.d_ptr, E_.d_nbr, val, ptr, col, v);
+        be_.gather_f64(n, orig, diag, dg);
+        be_.gather_f64(n, orig, b, bb);
         return MortonSystem{ptr, col, v, dg, bb, x};
     }
 

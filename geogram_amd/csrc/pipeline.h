@@ -255,6 +255,9 @@ public:
         return 0;
     }
 
+    // seeds of the last build
+    int total_points() const { return n_; }
+
     // Weights of the empty cells of the last cell pass (built for pb.d_w)
     // raised until each seed lies in its cell (ot_repair_row), into d_wout.
     void repair_empty(const Problem& pb, double delta, double* d_wout) {
