@@ -772,22 +772,10 @@ struct EmuBackend {
             out[t] = lo < n ? lo : n - 1;
         }
     }
-    template <int K>
-    void knn_seed_k(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
-                    const int* qpos, int nq, int kk, double e, float* rr) {
-#pragma omp parallel for schedule(dynamic, 64)
-        for (int t = 0; t < nq; ++t) rr[t] = knn_seed_radius_row<K>(pf, nodes, IL, qf, qpos, t, kk, e);
-    }
-    void knn_seed_radius(int K, const pf4* pf, const bnode* nodes, const InstLayout& IL,
-                         const pf4* qf, const int* qpos, int nq, int kk, double e, float* rr) {
-        switch (K) {
-            case 16: knn_seed_k<16>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
-            case 24: knn_seed_k<24>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
-            case 32: knn_seed_k<32>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
-            case 48: knn_seed_k<48>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
-            case 64: knn_seed_k<64>(pf, nodes, IL, qf, qpos, nq, kk, e, rr); break;
-            default: throw std::runtime_error("unsupported K");
-        }
+    void knn_seed_radius(const pf4* pf, const InstLayout& IL, const pf4* qf, const int* qpos,
+                         int nq, int kk, double e, float* rr) {
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < nq; ++t) rr[t] = knn_seed_radius_row(pf, IL, qf, qpos, t, kk, e);
     }
     void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
                    const int* items, int ni, const float* rr, int* cnt) {

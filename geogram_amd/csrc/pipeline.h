@@ -319,8 +319,7 @@ public:
         }
         const double e = knn_vec_err(S);
         float* rr = (float*)slot(S_QR2, sizeof(float) * (size_t)nq);
-        const int Kp = kk <= 16 ? 16 : kk <= 24 ? 24 : kk <= 32 ? 32 : kk <= 48 ? 48 : 64;
-        be_.knn_seed_radius(Kp, d_pf, d_nodes, IL, qf, qpos, nq, kk, e, rr);
+        be_.knn_seed_radius(d_pf, IL, qf, qpos, nq, kk, e, rr);
         std::vector<int> items;
         bool all = true;
         int* d_redo = (int*)slot(S_QREDO, sizeof(int) * (size_t)nq);
