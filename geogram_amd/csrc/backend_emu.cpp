@@ -772,10 +772,10 @@ struct EmuBackend {
             out[t] = lo < n ? lo : n - 1;
         }
     }
-    void knn_seed_radius(const pf4* pf, const InstLayout& IL, const pf4* qf, const int* qpos,
-                         int nq, int kk, double e, float* rr) {
-#pragma omp parallel for schedule(static)
-        for (int t = 0; t < nq; ++t) rr[t] = knn_seed_radius_row(pf, IL, qf, qpos, t, kk, e);
+    void knn_seed_radius(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                         const int* qpos, int nq, int kk, double e, float* rr) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int t = 0; t < nq; ++t) rr[t] = knn_seed_radius_row(pf, nodes, IL, qf, qpos, t, kk, e);
     }
     void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
                    const int* items, int ni, const float* rr, int* cnt) {

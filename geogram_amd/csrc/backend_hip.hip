@@ -337,12 +337,13 @@ __global__ void k_lower_bound_u64(const uint64_t* __restrict__ keys, int n,
     out[t] = lo < n ? lo : n - 1;
 }
 
-__global__ __launch_bounds__(256) void k_knn_seed_radius(const pf4* __restrict__ pf, InstLayout IL,
-                                                         const pf4* __restrict__ qf,
+__global__ __launch_bounds__(256) void k_knn_seed_radius(const pf4* __restrict__ pf,
+                                                         const bnode* __restrict__ nodes,
+                                                         InstLayout IL, const pf4* __restrict__ qf,
                                                          const int* __restrict__ qpos, int nq,
                                                          int kk, double e, float* rr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nq) rr[t] = knn_seed_radius_row(pf, IL, qf, qpos, t, kk, e);
+    if (t < nq) rr[t] = knn_seed_radius_row(pf, nodes, IL, qf, qpos, t, kk, e);
 }
 
 __global__ void k_knn_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
@@ -1867,9 +1868,10 @@ struct HipBackend {
         if (nq > 0) k_lower_bound_u64<<<blocks_for(nq, 256), 256, 0, stream_>>>(keys, n, q, nq, out);
         check();
     }
-    void knn_seed_radius(const pf4* pf, const InstLayout& IL, const pf4* qf, const int* qpos,
-                         int nq, int kk, double e, float* rr) {
-        k_knn_seed_radius<<<blocks_for(nq, 256), 256, 0, stream_>>>(pf, IL, qf, qpos, nq, kk, e, rr);
+    void knn_seed_radius(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
+                         const int* qpos, int nq, int kk, double e, float* rr) {
+        k_knn_seed_radius<<<blocks_for(nq, 256), 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk,
+                                                                    e, rr);
         check();
     }
     void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,

@@ -176,41 +176,20 @@ VGC_SD double knn_outside_bound(float r32, double e) {
 
 // FP32 squared radius of query t of the exact kNN, a first guess checked
 // later by the count and the certification (which grow it when short): the
-// m-th smallest FP32 squared distance among the kKnnWindow points around
-// the query's Morton position (m = min(kk, window); those are points, so
-// the ball holds at least m of them), scaled by the volume ratio beyond the
-// window.
-constexpr int kKnnWindow = 32;
-VGC_SD float knn_seed_radius_row(const pf4* pf, const InstLayout& IL, const pf4* qf,
-                                 const int* qpos, int t, int kk, double e) {
+// kk-th FP32 squared distance of the 16 nearest points found by the tree
+// walk, or beyond 16 the 16th scaled by twice the volume ratio (the factor
+// two covers points spread on a surface, whose count grows as r^2).
+constexpr int kKnnSeedK = 16;
+VGC_SD float knn_seed_radius_row(const pf4* pf, const bnode* nodes, const InstLayout& IL,
+                                 const pf4* qf, const int* qpos, int t, int kk, double e) {
+    float bd[kKnnSeedK];
+    int bi[kKnnSeedK];
     const pf4 q = qf[t];
-    const int p0 = IL.p0[0], n = IL.n[0];
-    const int w = n < kKnnWindow ? n : kKnnWindow;
-    int b = (qpos ? qpos[t] : t) - kKnnWindow / 2;
-    if (b > p0 + n - w) b = p0 + n - w;
-    if (b < p0) b = p0;
-    float d[kKnnWindow];
-#pragma unroll
-    for (int i = 0; i < kKnnWindow; ++i) {
-        d[i] = INFINITY;
-        if (i < w) {
-            const pf4 p = pf[b + i];
-            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-            d[i] = dx * dx + dy * dy + dz * dz;
-        }
-    }
-    const int m = kk < w ? kk : w;
-    float r = INFINITY;
-#pragma unroll
-    for (int i = 0; i < kKnnWindow; ++i) {
-        int rank = 0;
-#pragma unroll
-        for (int j = 0; j < kKnnWindow; ++j) rank += (d[j] < d[i] || (d[j] == d[i] && j < i)) ? 1 : 0;
-        if (rank == m - 1) r = d[i];
-    }
-    if (kk > w) r *= 1.5f * powf(float(kk) / float(w), 2.0f / 3.0f);
-    if (!(r < INFINITY)) return INFINITY;
-    return knn_list_radius(r, e);
+    knn_query_pt<kKnnSeedK>(pf, nodes, IL, 0, qpos ? qpos[t] : t, q.x, q.y, q.z, -1, bd, bi);
+    float d = bd[(kk < kKnnSeedK ? kk : kKnnSeedK) - 1];
+    if (kk > kKnnSeedK) d *= 2.0f * powf(float(kk) / float(kKnnSeedK), 2.0f / 3.0f);
+    if (!(d < INFINITY)) return INFINITY;
+    return knn_list_radius(d, e);
 }
 
 // Listed entries of query t: caller's index and FP64 squared distance (as

@@ -319,7 +319,7 @@ public:
         }
         const double e = knn_vec_err(S);
         float* rr = (float*)slot(S_QR2, sizeof(float) * (size_t)nq);
-        be_.knn_seed_radius(d_pf, IL, qf, qpos, nq, kk, e, rr);
+        be_.knn_seed_radius(d_pf, d_nodes, IL, qf, qpos, nq, kk, e, rr);
         std::vector<int> items;
         bool all = true;
         int* d_redo = (int*)slot(S_QREDO, sizeof(int) * (size_t)nq);
