@@ -795,24 +795,7 @@ VGC_HD int gather_ring(const ClipParams& P, CellLds<S>& L, int c, int inst, floa
 template <int S>
 VGC_HD void sort_cand(CellLds<S>& L, int cnt) {
     const LI lane = lane_id();
-    if (cnt <= 64) {
-        // one key per lane, the bitonic network through lane shuffles
-        const LB in = lane < cnt;
-        LU64 v = select(in, lds_ld(L.cand, select(in, lane, LI(0))), LU64(~0ull));
-        for (int k = 2; k <= 64; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const LU64 o = shfl_xor(v, j);
-                const LB take_min = ((lane & j) == 0) == ((lane & k) == 0);
-                const LB lt = v < o;
-                v = select(take_min, select(lt, v, o), select(lt, o, v));
-            }
-        }
-        wave_sync();
-        lds_st(L.cand, lane, v, LB(true));
-        wave_sync();
-        return;
-    }
-    const int P2 = 128;
+    const int P2 = cnt <= 64 ? 64 : 128;
     for (int i0 = 0; i0 < P2; i0 += kWave) {
         LI i = i0 + lane;
         lds_st(L.cand, i, LU64(~0ull), i >= cnt);
