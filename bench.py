@@ -11,12 +11,17 @@ sides.
 
 The JSON line also carries:
   roofline      the first clipping pass (one launch over every cell, the
-                dominant kernel): algorithmic FLOPs of that launch
-                (sum over cells of 7 V + 66 C + 34 T + 20 F: 7 flops per
-                corner-plane test, 66 per corner from three planes, 34 per
-                integration tetrahedron, 20 per facet), divided by its
-                duration from HIP events recorded on the library's launch
-                stream, against the FP64 vector peak of MI355X (78.6 TFLOP/s);
+                dominant kernel; FP32 corner state, FP64 integration):
+                algorithmic FLOPs of that launch (sum over cells of
+                7 V + 66 C + 34 T + 20 F: 7 flops per corner-plane test, 66
+                per corner from three planes, 34 per integration tetrahedron,
+                20 per facet), divided by its duration from HIP events
+                recorded on the library's launch stream, against MI355X's
+                vector peak of 78.6 T operations/s (FP64, and FP32 without
+                packed instructions, which the build excludes);
+  fp32_pass     cells the FP32 first pass left to the FP64 first pass, that
+                launch's time, and the FP64 first pass timed alone
+                (VGC_FP64_FIRST) on the same seeds for comparison;
                 traffic (HBM bytes per launch) and valu_busy_frac (share of
                 the VALU issue slots in use, the kernel's actual bound) are
                 read from the committed rocprofv3 counter summary
@@ -192,7 +197,8 @@ def main():
     clip_ms = float(np.mean([t["t_clip_a_gpu_ms"] for t in stats]))
     achieved = flops_a / (clip_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "k_clip<1,1,false,true,false,false> (first clipping pass, every cell)",
+        "kernel": "k_clip<1,1,false,true,false,false,true> (first clipping pass, every cell, "
+                  "FP32 corner state)",
         "bound": "valu",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
@@ -207,6 +213,20 @@ def main():
         "flops_per_cell": round(flops_a / n, 1),
         "launch_ms_hip_events": round(clip_ms, 3),
     }
+
+    # the FP64 first pass alone on the same seeds (identical outputs)
+    fp64_ms = []
+    for _ in range(3):
+        r64 = ctx.cells(xd, k0=args.k0, fp64_first=True)
+        fp64_ms.append(r64.stats["t_clip_a_gpu_ms"])
+    same64 = bool(torch.equal(r64.vol, r.vol) and torch.equal(r64.nbr, r.nbr)
+                  and torch.equal(r64.farea, r.farea))
+    fp32_pass = {"redo_cells": int(s["nb_fp64_redo"]),
+                 "redo_gpu_ms": round(float(np.mean([t["t_redo_gpu_ms"] for t in stats])), 3),
+                 "fp32_first_pass_gpu_ms": round(clip_ms, 3),
+                 "fp64_first_pass_gpu_ms": round(float(min(fp64_ms)), 3),
+                 "outputs_identical_to_fp64_pass": same64}
+    del r64
 
     # semi-discrete OT leg: 1M uniform seeds, equal masses, to 1e-4 max
     # relative mass error, plain start (w = 0)
@@ -448,6 +468,7 @@ def main():
             "k0": args.k0,
         },
         "roofline": roofline,
+        "fp32_pass": fp32_pass,
         "cpu_baseline": cpu,
         "errors": errors or None,
         "ot": ot,

@@ -116,14 +116,16 @@ class Context:
     # ---------------------------------------------------------------- cells
     def cells(self, points, weights=None, box=UNIT_BOX, k0=0, neighbors=True,
               inst_n=None, boxes=None, count_work=False, autotune=False, waves4=False,
-              periodic=False):
+              periodic=False, fp64_first=False):
         """Voronoi (weights None) or Laguerre cells clipped to box.
 
         For a batch of instances give inst_n (seed counts, consecutive in
         points) and boxes (6 per instance).  autotune: pick the first pass's
         work-group size by timing (VGC_AUTOTUNE; results do not depend on it).
         waves4: run the first pass with 4-wavefront work-groups (VGC_WAVES4).
-        periodic: the box is one period of a torus (VGC_PERIODIC)."""
+        periodic: the box is one period of a torus (VGC_PERIODIC).
+        fp64_first: first pass with FP64 corner state (VGC_FP64_FIRST; the
+        default FP32 pass gives identical results)."""
         dev = _is_torch_cuda(points)
         self._stream(dev)
         if dev:
@@ -143,7 +145,7 @@ class Context:
             ptr = np.empty(n + 1, dtype=np.uint64)
         flags = ((VGC_DEVICE_PTRS if dev else 0) | (VGC_COUNT_WORK if count_work else 0)
                  | (_lib.VGC_AUTOTUNE if autotune else 0) | (_lib.VGC_WAVES4 if waves4 else 0)
-                 | (_lib.VGC_PERIODIC if periodic else 0))
+                 | (_lib.VGC_PERIODIC if periodic else 0) | (_lib.VGC_FP64_FIRST if fp64_first else 0))
         st = Stats()
         if inst_n is None:
             boxa = np.ascontiguousarray(box, dtype=np.float64)

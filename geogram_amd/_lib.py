@@ -31,6 +31,7 @@ VGC_OT_INEXACT = 64
 VGC_WAVES4 = 128
 VGC_PERIODIC = 256
 VGC_OT_REPAIR = 512
+VGC_FP64_FIRST = 1024
 
 VGC_MLP_PARAMS = 4673
 VGC_MLP_FEATURES = 6
@@ -45,7 +46,8 @@ _STAT_U64B = ("a_cells a_tests a_live a_new a_tets a_facets nb_fail_capacity why
 
 class Stats(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_uint64) for k in _STAT_U64] + [(k, ctypes.c_double) for k in _STAT_F64]
-                + [(k, ctypes.c_double) for k in _STAT_F64B] + [(k, ctypes.c_uint64) for k in _STAT_U64B])
+                + [(k, ctypes.c_double) for k in _STAT_F64B] + [(k, ctypes.c_uint64) for k in _STAT_U64B]
+                + [("nb_fp64_redo", ctypes.c_uint64), ("t_redo_gpu_ms", ctypes.c_double)])
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

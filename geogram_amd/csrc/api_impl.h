@@ -84,6 +84,8 @@ inline void fill_stats(const EngineStats& e, vgc_stats* s) {
     s->tuned_waves = (uint64_t)e.tuned_waves;
     s->nb_periodic_retries = e.nb_periodic_retries;
     s->nb_pieces = e.nb_pieces;
+    s->nb_fp64_redo = e.nb_fp64_redo;
+    s->t_redo_gpu_ms = e.t_redo_gpu_ms;
 }
 
 inline bool valid_k0(uint32_t k) { return k == 16 || k == 24 || k == 32 || k == 48 || k == 64; }
@@ -152,6 +154,7 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         if (ctx->eng.cells(pb, es)) {
             ctx->err = "vgc_cells: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -233,6 +236,7 @@ int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* 
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         Engine<B>& E = ctx->eng;
         if (E.cells(pb, es)) {
             ctx->err = "vgc_cvt_func_grad: " + E.error;
@@ -426,6 +430,7 @@ int api_cell_mesh(Ctx<B>* ctx, const char* name, const double* xyz, const double
         }
         ctx->eng.autotune = false;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         const bool want_mesh = vertices && facet_ptr && facet_vertices && facet_nbr;
         long long capv[4] = {0, 0, 0, 0};
         if (cap) {
@@ -567,6 +572,7 @@ int api_cvt_newton(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint
         Engine<B>& E = ctx->eng;
         E.autotune = (flags & VGC_AUTOTUNE) != 0;
         E.waves4 = (flags & VGC_WAVES4) != 0;
+        E.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         double* d_f = (double*)ctx->staging(2, sizeof(double));
         uint64_t failed = 0;
         // energy and gradient: the caller's funcgrad of CVT.cpp:324-339
@@ -667,6 +673,7 @@ int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n
         EngineStats es;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         if (ctx->eng.lloyd(pb, dx, (int)iters, es)) {
             ctx->err = "vgc_lloyd: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -745,6 +752,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                           (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0,
                           (flags & VGC_OT_INEXACT) != 0, (flags & VGC_OT_REPAIR) != 0)) {
@@ -862,6 +870,7 @@ int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double*
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                                 (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
             ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;
@@ -1151,6 +1160,7 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
         const uint32_t maxn = max_newton ? max_newton : 100;
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
+        ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
         bool all_conv = true;
         std::vector<double> hl;
         for (uint32_t k = 0; k < n_levels; ++k) {

@@ -146,6 +146,16 @@ struct EmuBackend {
             std::memset(L.get(), 0, sizeof(CellLds<S>));
 #pragma omp for schedule(dynamic, 16)
             for (int it = 0; it < P.n_items; ++it) {
+                if constexpr (S == 1 && G && !E) {
+                    if (P.fp32) {
+                        // the first pass with FP32 corner state
+                        if (P.weighted)
+                            process_cell<S, false, G, true, true, float>(P, it, *L);
+                        else
+                            process_cell<S, false, G, false, false, float>(P, it, *L);
+                        continue;
+                    }
+                }
                 if (P.weighted)
                     process_cell<S, E, G, true>(P, it, *L);
                 else if (G && P.corner)

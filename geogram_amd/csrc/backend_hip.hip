@@ -153,13 +153,16 @@ __device__ __forceinline__ int xcd_block() {
 // one-wave work-groups that measured 7 % faster than 168 registers and 3
 // waves — the wait for dependent LDS and memory round trips is hidden by
 // occupancy.  The exact re-clip (a few cells) keeps its registers.
-template <int S, int WAVES, bool EXACT, bool GATHER, bool W, bool CORNER = W>
+// F32: the first pass with FP32 corner state (the cells it cannot certify
+// are left to the FP64 kernel).
+template <int S, int WAVES, bool EXACT, bool GATHER, bool W, bool CORNER = W, bool F32 = false>
 __global__ __launch_bounds__(64 * WAVES, (S == 1 && !EXACT) ? 4 : 1) void k_clip(ClipParams P) {
     __shared__ CellLds<S> lds[WAVES];
     const int wave = threadIdx.x >> 6;
     const int item = xcd_block() * WAVES + wave;
     if (item >= P.n_items) return;  // whole wavefront
-    process_cell<S, EXACT, GATHER, W, CORNER>(P, item, lds[wave]);
+    process_cell<S, EXACT, GATHER, W, CORNER, typename std::conditional<F32, float, double>::type>(
+        P, item, lds[wave]);
 }
 
 __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __restrict__ pts,
@@ -1371,6 +1374,15 @@ struct HipBackend {
                 if (P.corner) {
                     // second gather of Voronoi cells, with the corner test
                     k_clip<1, 1, E, G, W, true><<<g1, 64, 0, stream_>>>(P);
+                    return;
+                }
+            }
+            if constexpr (G && !E) {
+                if (P.fp32) {
+                    if (waves_ == 1)
+                        k_clip<1, 1, false, true, W, W, true><<<g1, 64, 0, stream_>>>(P);
+                    else
+                        k_clip<1, 4, false, true, W, W, true><<<g4, 256, 0, stream_>>>(P);
                     return;
                 }
             }

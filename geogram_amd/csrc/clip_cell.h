@@ -29,6 +29,16 @@
 // certified when |t| > gamma_9 * (|e|_1 max|N| + |h| |D|); an uncertified one
 // follows the tie rule above and flags the cell.  Plane equations, new-corner
 // jobs and the two border-walk tables live in the wave's slice of LDS.
+//
+// Corner state in FP32 (the first pass over every cell).  Plane rows stay
+// FP64 in LDS; the corners' N, D and error magnitudes are FP32, built from
+// the rows rounded to FP32, and a sign is certified against the exact
+// determinant of the FP64 rows (the bound of Prec<float>).  A cell with a sign
+// or corner it cannot certify stops at once and is left to the FP64 pass.
+// Every other cell took the decisions the FP64 pass takes, in the same order
+// and into the same corner slots; its corners are rebuilt in FP64 from their
+// plane triples for the integration, exactly as the FP64 pass computed them,
+// so its outputs equal the FP64 pass's bit for bit.
 #pragma once
 
 #include <type_traits>
@@ -87,6 +97,32 @@ constexpr int kMaxFilteredRings = 48;  // power-filtered rings of the first pass
 constexpr int kMaxVoronoiRings = 16;
 constexpr double kGamma9 = 1.2e-15;  // > gamma_9 for the conflict determinant
 constexpr double kGamma5 = 0.8e-15;  // > gamma_5 for the corner determinant D
+
+// Error bounds of the corner state of scalar type T.  gamma9: relative bound
+// on the conflict determinant t = e.N - h D against the exact determinant of
+// the FP64 plane rows; gamma5: corners with D <= gamma5 |D|_abs are
+// degenerate (flagged).
+template <class T>
+struct Prec;
+template <>
+struct Prec<double> {
+    static constexpr double gamma9 = kGamma9;
+    static constexpr double gamma5 = kGamma5;
+    static constexpr bool fp32 = false;
+};
+// FP32: each of the four factors of a term of t is an FP64 row entry rounded
+// to FP32 (relative 2^-24) and the evaluation adds at most 9 roundings, so a
+// term is off by at most gamma_13 < 7.8e-7 of its magnitude; 1.2e-6 also
+// covers the rounding of the magnitude sum.  N and D are off by at most
+// gamma9 of their magnitudes, so with D > 1e-5 |D|_abs the position N / D is
+// known to 12 % of D.  Magnitudes outside [1e-30, 1e30] (FP32 underflow or
+// overflow) are uncertain.
+template <>
+struct Prec<float> {
+    static constexpr float gamma9 = 1.2e-6f;
+    static constexpr float gamma5 = 1.0e-5f;
+    static constexpr bool fp32 = true;
+};
 
 struct ClipParams {
     const pt4* pts;        // seeds in Morton order (x, y, z, weight)
@@ -152,6 +188,9 @@ struct ClipParams {
     // with piece set, writes its outputs and flags by item, not by seed
     const double* item_box;
     int piece;
+    // host dispatch: run the first pass (S = 1, gather, FP64 signs) with FP32
+    // corner state
+    int fp32;
 };
 
 template <int S>
@@ -204,17 +243,24 @@ VGC_HD LU adj_of(const LU& w, int k) {
     return (w >> LU(CornerBits<S>::value * k)) & LU(CornerBits<S>::mask);
 }
 
-struct Corner {
-    LD nx, ny, nz, d, nmax, dabs, r2;
+template <class T>
+struct CornerT {
+    Lane<T> nx, ny, nz, d, nmax, dabs, r2;
     LB degenerate;
 };
+typedef CornerT<double> Corner;
 
 // Corner of planes a, b, c (outward normals e, offsets h), oriented so that
 // D = det(ea, eb, ec) > 0.  N = ha (eb x ec) + hb (ec x ea) + hc (ea x eb).
-VGC_HD Corner make_corner(const LD& eax, const LD& eay, const LD& eaz, const LD& ha,
-                          const LD& ebx, const LD& eby, const LD& ebz, const LD& hb,
-                          const LD& ecx, const LD& ecy, const LD& ecz, const LD& hc) {
-    Corner C;
+// In FP32, r2 is an upper bound of the squared distance of the exact corner
+// of the FP64 rows (the security radius may then only stop later).
+template <class T>
+VGC_HD CornerT<T> make_corner(const Lane<T>& eax, const Lane<T>& eay, const Lane<T>& eaz,
+                              const Lane<T>& ha, const Lane<T>& ebx, const Lane<T>& eby,
+                              const Lane<T>& ebz, const Lane<T>& hb, const Lane<T>& ecx,
+                              const Lane<T>& ecy, const Lane<T>& ecz, const Lane<T>& hc) {
+    typedef Lane<T> LD;  // the lane type of the corner state
+    CornerT<T> C;
     // cross products and their magnitude sums
     LD bcx = vfma(eby, ecz, -(ebz * ecy)), bcy = vfma(ebz, ecx, -(ebx * ecz)),
        bcz = vfma(ebx, ecy, -(eby * ecx));
@@ -241,26 +287,49 @@ VGC_HD Corner make_corner(const LD& eax, const LD& eay, const LD& eaz, const LD&
     LD naz = vfma(Aha, abcz, vfma(Ahb, acaz, Ahc * aabz));
     C.nmax = vmax(nax, vmax(nay, naz));
     C.dabs = vfma(Aeax, abcx, vfma(Aeay, abcy, Aeaz * abcz));
-    C.degenerate = C.d <= kGamma5 * C.dabs;
-    LD dd = select(C.degenerate, LD(1.0), C.d);
-    LD inv = 1.0 / dd;
+    C.degenerate = C.d <= T(Prec<T>::gamma5) * C.dabs;
+    if constexpr (Prec<T>::fp32)
+        C.degenerate = C.degenerate | !(C.dabs >= T(1e-30f)) | !(C.dabs < T(1e30f)) |
+                       !(C.nmax < T(1e30f));
+    LD dd = select(C.degenerate, LD(T(1)), C.d);
+    LD inv = T(1) / dd;
     LD qx = C.nx * inv, qy = C.ny * inv, qz = C.nz * inv;
     C.r2 = vfma(qx, qx, vfma(qy, qy, qz * qz));
+    if constexpr (Prec<T>::fp32) {
+        // (|N| + sqrt(3) gamma9 max|N|) / (D - gamma9 |D|_abs), squared, plus
+        // 1e-5 for the rounding of this evaluation
+        const LD nn = vsqrt(vfma(C.nx, C.nx, vfma(C.ny, C.ny, C.nz * C.nz)));
+        const LD r = (nn + T(1.8f * Prec<T>::gamma9) * C.nmax) /
+                     select(C.degenerate, LD(T(1)), C.d - T(Prec<T>::gamma9) * C.dabs);
+        C.r2 = (r * r) * T(1.00001f);
+    }
     return C;
 }
 
 // ------------------------------------------------------------- the cell
 
-template <int S>
+template <int S, class T = double>
 struct CellState {
-    LD nx[S], ny[S], nz[S], d[S], nmax[S], dabs[S], r2[S];
+    typedef T scalar;
+    Lane<T> nx[S], ny[S], nz[S], d[S], nmax[S], dabs[S], r2[S];
     LU pl[S], adj[S];
     uint64_t valid[S];
 };
 
+// Plane row p of the wave's table (FP64 in LDS) in the corner state's type.
+template <class T, int S>
+VGC_HD void load_row(const CellLds<S>& L, const LI& p, Lane<T>& ex, Lane<T>& ey, Lane<T>& ez,
+                     Lane<T>& h) {
+    const double* pe = &L.pe[0][0];
+    ex = cvt<T>(lds_ld(pe, p * 4 + 0));
+    ey = cvt<T>(lds_ld(pe, p * 4 + 1));
+    ez = cvt<T>(lds_ld(pe, p * 4 + 2));
+    h = cvt<T>(lds_ld(pe, p * 4 + 3));
+}
+
 // Box planes 0..5 = +x -x +y -y +z -z (outward), corners in lanes 0..7.
-template <int S>
-VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const pt4& pc) {
+template <int S, class T>
+VGC_HD void init_box(CellState<S, T>& st, CellLds<S>& L, const double* box, const pt4& pc) {
     const LI lane = lane_id();
     {
         LB m = lane < 6;
@@ -294,20 +363,18 @@ VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const p
         LI a0 = l8 ^ 4;
         LI a1 = select(pos, l8 ^ 1, l8 ^ 2);
         LI a2 = select(pos, l8 ^ 2, l8 ^ 1);
-        LD eax = lds_ld(&L.pe[0][0], a * 4 + 0), eay = lds_ld(&L.pe[0][0], a * 4 + 1),
-           eaz = lds_ld(&L.pe[0][0], a * 4 + 2), ha = lds_ld(&L.pe[0][0], a * 4 + 3);
-        LD ebx = lds_ld(&L.pe[0][0], b * 4 + 0), eby = lds_ld(&L.pe[0][0], b * 4 + 1),
-           ebz = lds_ld(&L.pe[0][0], b * 4 + 2), hb = lds_ld(&L.pe[0][0], b * 4 + 3);
-        LD ecx = lds_ld(&L.pe[0][0], c * 4 + 0), ecy = lds_ld(&L.pe[0][0], c * 4 + 1),
-           ecz = lds_ld(&L.pe[0][0], c * 4 + 2), hc = lds_ld(&L.pe[0][0], c * 4 + 3);
-        Corner C = make_corner(eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc);
+        Lane<T> eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc;
+        load_row<T>(L, a, eax, eay, eaz, ha);
+        load_row<T>(L, b, ebx, eby, ebz, hb);
+        load_row<T>(L, c, ecx, ecy, ecz, hc);
+        CornerT<T> C = make_corner<T>(eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc);
         st.nx[0] = C.nx;
         st.ny[0] = C.ny;
         st.nz[0] = C.nz;
         st.d[0] = C.d;
         st.nmax[0] = C.nmax;
         st.dabs[0] = C.dabs;
-        st.r2[0] = select(m, C.r2, LD(0.0));
+        st.r2[0] = select(m, C.r2, Lane<T>(T(0)));
         st.pl[0] = cvt<uint32_t>(a) | (cvt<uint32_t>(b) << LU(8)) | (cvt<uint32_t>(c) << LU(16));
         st.adj[0] = cvt<uint32_t>(a0) | (cvt<uint32_t>(a1) << LU(CornerBits<S>::value)) |
                     (cvt<uint32_t>(a2) << LU(2 * CornerBits<S>::value));
@@ -315,9 +382,9 @@ VGC_HD void init_box(CellState<S>& st, CellLds<S>& L, const double* box, const p
 #pragma unroll
         for (int s = 1; s < S; ++s) {
             st.valid[s] = 0;
-            st.nx[s] = st.ny[s] = st.nz[s] = 0.0;
-            st.d[s] = 1.0;
-            st.nmax[s] = st.dabs[s] = st.r2[s] = 0.0;
+            st.nx[s] = st.ny[s] = st.nz[s] = T(0);
+            st.d[s] = T(1);
+            st.nmax[s] = st.dabs[s] = st.r2[s] = T(0);
             st.pl[s] = 0;
             st.adj[s] = 0;
         }
@@ -367,26 +434,34 @@ VGC_HD LI exact_sign_lanes(const CellLds<S>& L, const LU& pl, const LB& m, doubl
 
 // Clip by plane (e, h) stored at slot np with neighbour id j.  Returns the
 // updated flags; *cut tells whether the plane removed a corner.
-template <int S, bool EXACT>
-VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, double ey,
+template <int S, bool EXACT, class T>
+VGC_HD uint32_t clip_plane(CellState<S, T>& st, CellLds<S>& L, int& np, double ex, double ey,
                            double ez, double h, int j, uint32_t flags, bool& cut,
                            unsigned long long& nlive, unsigned long long& nnew) {
+    static_assert(!(EXACT && Prec<T>::fp32), "exact signs run with FP64 corners");
+    typedef Lane<T> LT;
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
-    const double e1 = vabs(ex) + vabs(ey) + vabs(ez);
-    const double ha = vabs(h);
+    // the plane in the corner state's type (FP64: itself)
+    const T ext = T(ex), eyt = T(ey), ezt = T(ez), ht = T(h);
+    const T e1 = vabs(ext) + vabs(eyt) + vabs(ezt);
+    const T ha = vabs(ht);
     uint64_t cm[S];
     bool anyc = false, allc = true, anyu = false;
 #pragma unroll
     for (int s = 0; s < S; ++s) nlive += (unsigned long long)popc64(st.valid[s]);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        LD t = vfma(LD(ex), st.nx[s], vfma(LD(ey), st.ny[s], vfma(LD(ez), st.nz[s], -(h * st.d[s]))));
-        LD bnd = kGamma9 * vfma(LD(e1), st.nmax[s], ha * st.dabs[s]);
+        LT t = vfma(LT(ext), st.nx[s], vfma(LT(eyt), st.ny[s], vfma(LT(ezt), st.nz[s], -(ht * st.d[s]))));
+        LT bnd = T(Prec<T>::gamma9) * vfma(LT(e1), st.nmax[s], ha * st.dabs[s]);
         LB vs = mask_bit(st.valid[s]);
-        LB conf = t >= 0.0;
-        LB unc = vs & (vabs(t) <= bnd);
-        if (EXACT) {
+        LB conf = t >= T(0);
+        LB unc;
+        if constexpr (Prec<T>::fp32)
+            unc = vs & !((vabs(t) > bnd) & (bnd >= T(1e-30f)) & (bnd < T(1e30f)));
+        else
+            unc = vs & (vabs(t) <= bnd);
+        if constexpr (EXACT) {
             if (any(unc)) {
                 LI sg = exact_sign_lanes<S>(L, st.pl[s], unc, ex, ey, ez, h);
                 LB known = unc & (sg != 2);
@@ -476,12 +551,11 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
                 LI a = cvt<int>(jb & LU(0xFF)) & (kPMax - 1);
                 LI b = cvt<int>((jb >> LU(8)) & LU(0xFF)) & (kPMax - 1);
                 LU y = (jb >> LU(16)) & LU(CornerBits<S>::mask);
-                LD eax = lds_ld(&L.pe[0][0], a * 4 + 0), eay = lds_ld(&L.pe[0][0], a * 4 + 1),
-                   eaz = lds_ld(&L.pe[0][0], a * 4 + 2), hav = lds_ld(&L.pe[0][0], a * 4 + 3);
-                LD ebx = lds_ld(&L.pe[0][0], b * 4 + 0), eby = lds_ld(&L.pe[0][0], b * 4 + 1),
-                   ebz = lds_ld(&L.pe[0][0], b * 4 + 2), hbv = lds_ld(&L.pe[0][0], b * 4 + 3);
-                Corner C = make_corner(eax, eay, eaz, hav, ebx, eby, ebz, hbv, LD(ex), LD(ey),
-                                       LD(ez), LD(hn));
+                LT eax, eay, eaz, hav, ebx, eby, ebz, hbv;
+                load_row<T>(L, a, eax, eay, eaz, hav);
+                load_row<T>(L, b, ebx, eby, ebz, hbv);
+                CornerT<T> C = make_corner<T>(eax, eay, eaz, hav, ebx, eby, ebz, hbv, LT(ext),
+                                              LT(eyt), LT(ezt), LT(T(hn)));
                 if (any(is_t & C.degenerate)) flags |= CELL_UNCERTAIN;
                 st.nx[s] = select(is_t, C.nx, st.nx[s]);
                 st.ny[s] = select(is_t, C.ny, st.ny[s]);
@@ -546,9 +620,9 @@ VGC_HD uint32_t clip_plane(CellState<S>& st, CellLds<S>& L, int& np, double ex, 
 }
 
 // State of one cell's security-radius loop.
-template <int S>
+template <int S, class T = double>
 struct CellRun {
-    CellState<S> st;
+    CellState<S, T> st;
     int np;
     uint32_t flags;
     bool stopped;
@@ -562,15 +636,15 @@ struct CellRun {
 // One candidate neighbour j (Morton position) in nearest-first order: the
 // coincident-seed rule, the security-radius stop, then the clip.  Returns
 // true when the cell is finished (or failed) and the candidate loop must end.
-template <int S, bool EXACT, bool W>
-VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j, const pt4& pj) {
+template <int S, bool EXACT, bool W, class T>
+VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S, T>& R, int j, const pt4& pj) {
     if (j == R.c) return false;
     const double ex = pj.x - R.pc.x, ey = pj.y - R.pc.y, ez = pj.z - R.pc.z;
     const double d2 = ex * ex + ey * ey + ez * ez;
     if (d2 == 0.0) {
         if (P.orig[j] < R.orig_c) {
             wave_sync();
-            init_box<S>(R.st, L, R.box, R.pc);
+            init_box<S, T>(R.st, L, R.box, R.pc);
             R.np = 6;
             R.flags = CELL_DISCONNECTED;
             R.stopped = true;
@@ -584,14 +658,14 @@ VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j, c
         bool anyin = false;
 #pragma unroll
         for (int s = 0; s < S; ++s)
-            anyin = anyin || any(mask_bit(R.st.valid[s]) & (4.1 * R.st.r2[s] >= d2));
+            anyin = anyin || any(mask_bit(R.st.valid[s]) & (T(4.1) * R.st.r2[s] >= T(d2)));
         reach = !anyin;
     } else {
         const double sd = d2 + R.cw;
         bool anyin = false;
 #pragma unroll
         for (int s = 0; s < S; ++s)
-            anyin = anyin || any(mask_bit(R.st.valid[s]) & ((4.1 * d2) * R.st.r2[s] >= sd * sd));
+            anyin = anyin || any(mask_bit(R.st.valid[s]) & (T(4.1 * d2) * R.st.r2[s] >= T(sd * sd)));
         reach = (sd > 0.0) && !anyin;
     }
     if (reach) {
@@ -611,10 +685,14 @@ VGC_HD bool consider(const ClipParams& P, CellLds<S>& L, CellRun<S>& R, int j, c
     const double h = W ? 0.5 * (d2 + (R.wi - pj.w)) : 0.5 * d2;
     bool cut = false;
     ++R.ntest;
-    R.flags = clip_plane<S, EXACT>(R.st, L, R.np, ex, ey, ez, h, j, R.flags, cut, R.nlive, R.nnew);
+    R.flags = clip_plane<S, EXACT, T>(R.st, L, R.np, ex, ey, ez, h, j, R.flags, cut, R.nlive, R.nnew);
     R.prev_slot = (cut && !(R.flags & (CELL_EMPTY | CELL_OVERFLOW))) ? R.np - 1 : -1;
     if (cut) ++R.nclip;
-    if (R.flags & (CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO)) {
+    // FP32 corners: the first sign or corner that is not certified ends the
+    // cell (the FP64 pass takes it over)
+    constexpr uint32_t kEnd =
+        CELL_EMPTY | CELL_OVERFLOW | CELL_BAD_TOPO | (Prec<T>::fp32 ? uint32_t(CELL_UNCERTAIN) : 0u);
+    if (R.flags & kEnd) {
         R.stopped = (R.flags & CELL_EMPTY) != 0;
         return true;
     }
@@ -1039,9 +1117,13 @@ VGC_HD int gather_dfs(const ClipParams& P, CellLds<S>& L, int c, int inst, float
 // farthest corner plus the corner's certified position error
 // (|dN| <= g nmax, |dD| <= g dabs).  Returns false (no filter) when a corner
 // is too close to degenerate for that bound.
-template <int S, bool CORNER>
-VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F, CornerTab& CT) {
-    const CellState<S>& st = R.st;
+template <int S, bool CORNER, class T>
+VGC_HD bool make_filter(const CellRun<S, T>& R, PowerFilter& F, CornerTab& CT) {
+    const CellState<S, T>& st = R.st;
+    // position error of a corner: (G max|N| + |q| G |D|_abs) / (D - G |D|_abs)
+    // per coordinate, G = gamma9 of the corner state; in FP32 also a bound of
+    // the error's length (sqrt(3) per-coordinate bounds)
+    const double G = Prec<T>::fp32 ? 1.8 * double(Prec<T>::gamma9) : kGamma9;
     CT.live = 0;
     LD lox = LD(INFINITY), loy = LD(INFINITY), loz = LD(INFINITY);
     LD hix = LD(-INFINITY), hiy = LD(-INFINITY), hiz = LD(-INFINITY);
@@ -1049,10 +1131,12 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F, CornerTab& CT) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LB vs = mask_bit(st.valid[s]);
-        LD den = st.d[s] - kGamma9 * st.dabs[s];
+        const LD sd = cvt<double>(st.d[s]), sdabs = cvt<double>(st.dabs[s]);
+        LD den = sd - G * sdabs;
         bad = bad || any(vs & !(den > 0.0));
-        LD inv = 1.0 / select(vs, st.d[s], LD(1.0));
-        LD qx = st.nx[s] * inv, qy = st.ny[s] * inv, qz = st.nz[s] * inv;
+        LD inv = 1.0 / select(vs, sd, LD(1.0));
+        LD qx = cvt<double>(st.nx[s]) * inv, qy = cvt<double>(st.ny[s]) * inv,
+           qz = cvt<double>(st.nz[s]) * inv;
         lox = select(vs, vmin(lox, qx), lox);
         loy = select(vs, vmin(loy, qy), loy);
         loz = select(vs, vmin(loz, qz), loz);
@@ -1075,14 +1159,15 @@ VGC_HD bool make_filter(const CellRun<S>& R, PowerFilter& F, CornerTab& CT) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LB vs = mask_bit(st.valid[s]);
-        LD dd = select(vs, st.d[s], LD(1.0));
+        const LD sdabs = cvt<double>(st.dabs[s]);
+        LD dd = select(vs, cvt<double>(st.d[s]), LD(1.0));
         LD inv = 1.0 / dd;
-        LD qx = st.nx[s] * inv, qy = st.ny[s] * inv, qz = st.nz[s] * inv;
+        LD qx = cvt<double>(st.nx[s]) * inv, qy = cvt<double>(st.ny[s]) * inv,
+           qz = cvt<double>(st.nz[s]) * inv;
         LD gx = qx - fx, gy = qy - fy, gz = qz - fz;
         LD dist = vsqrt(vfma(gx, gx, vfma(gy, gy, gz * gz)));
         LD qn = vsqrt(vfma(qx, qx, vfma(qy, qy, qz * qz)));
-        LD err = kGamma9 * (st.nmax[s] + qn * st.dabs[s]) /
-                 select(vs, dd - kGamma9 * st.dabs[s], LD(1.0));
+        LD err = G * (cvt<double>(st.nmax[s]) + qn * sdabs) / select(vs, dd - G * sdabs, LD(1.0));
         rmax = vmax(rmax, select(vs, dist + err, LD(0.0)));
         hmx = vmax(hmx, select(vs, vabs(gx) + err, LD(0.0)));
         hmy = vmax(hmy, select(vs, vabs(gy) + err, LD(0.0)));
@@ -1193,8 +1278,8 @@ VGC_SD int orient3d_sign(const pt4& p0, const pt4& p1, const pt4& p2, const pt4&
 // (RestrictedVoronoiDiagram::compute_RDT, voronoi/RVD.cpp:2302-2327, and
 // PrimalTetrahedronAction, generic_RVD.h:1035-1055).  (gx, gy, gz): the
 // cell's centroid relative to the seed.
-template <int S>
-VGC_HD void mesh_emit(const ClipParams& P, CellLds<S>& L, const CellState<S>& st, int np,
+template <int S, class CS>
+VGC_HD void mesh_emit(const ClipParams& P, CellLds<S>& L, const CS& st, int np,
                       int orig_c, const pt4& pc, double gx, double gy, double gz) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
@@ -1372,7 +1457,7 @@ VGC_HD void mesh_emit(const ClipParams& P, CellLds<S>& L, const CellState<S>& st
 // Security-radius loop + integration for work item `item`.  GATHER: the
 // wave finds the neighbours itself, ring after ring of distance; otherwise
 // it reads them from P.list.
-template <int S, bool EXACT, bool GATHER, bool W, bool CORNER = W>
+template <int S, bool EXACT, bool GATHER, bool W, bool CORNER = W, class T = double>
 VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     constexpr int kPMax = PMax<S>::value;
     const LI lane = lane_id();
@@ -1408,7 +1493,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         return;
     }
 
-    CellRun<S> R;
+    CellRun<S, T> R;
     R.pc = pc;
     R.c = c;
     R.orig_c = P.orig[c];
@@ -1421,7 +1506,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     R.prev.x = R.prev.y = R.prev.z = R.prev.w = NAN;
     R.prev_slot = -1;
     R.nclip = R.ntest = R.nlive = R.nnew = 0;
-    init_box<S>(R.st, L, box, pc);
+    init_box<S, T>(R.st, L, box, pc);
     bool complete = false;
 
     if (!GATHER) {
@@ -1441,7 +1526,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 complete = true;  // short list: the instance has fewer seeds
                 break;
             }
-            if (consider<S, EXACT, W>(P, L, R, j, P.pts[j])) break;
+            if (consider<S, EXACT, W, T>(P, L, R, j, P.pts[j])) break;
         }
     } else {
         const pf4 pq = P.pf[c];
@@ -1512,7 +1597,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             for (int jj = 0; jj < cnt; ++jj) {
                 if ((jj & (kWave - 1)) == 0) stage_cand<S>(P, L, jj, cnt);
                 const int j = int(uint32_t(L.cand[jj]));
-                if (consider<S, EXACT, W>(P, L, R, j, L.stage[jj & (kWave - 1)])) {
+                if (consider<S, EXACT, W, T>(P, L, R, j, L.stage[jj & (kWave - 1)])) {
                     done = true;
                     break;
                 }
@@ -1522,7 +1607,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LD r2m = LD(0.0);
 #pragma unroll
             for (int s = 0; s < S; ++s)
-                r2m = vmax(r2m, select(mask_bit(R.st.valid[s]), R.st.r2[s], LD(0.0)));
+                r2m = vmax(r2m, select(mask_bit(R.st.valid[s]), cvt<double>(R.st.r2[s]), LD(0.0)));
             const double r2now = wave_max(r2m);
             const float need2 = radius_for(r2now, R.wi, W ? P.wmax[inst] : 0.0, W);
             if (need2 <= rhi2 || rhi2 >= all2) {
@@ -1545,7 +1630,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     }
     const bool stopped = R.stopped;
     uint32_t flags = R.flags;
-    CellState<S>& st = R.st;
+    CellState<S, T>& st = R.st;
     int np = R.np;
     const unsigned long long nclip = R.nclip, ntest = R.ntest, nlive = R.nlive, nnew = R.nnew;
     const int orig_c = R.orig_c;
@@ -1554,14 +1639,24 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     // squared radius of the (possibly partial) cell
     LD r2m = LD(0.0);
 #pragma unroll
-    for (int s = 0; s < S; ++s) r2m = vmax(r2m, select(mask_bit(st.valid[s]), st.r2[s], LD(0.0)));
-    const double r2max = wave_max(r2m);
+    for (int s = 0; s < S; ++s)
+        r2m = vmax(r2m, select(mask_bit(st.valid[s]), cvt<double>(st.r2[s]), LD(0.0)));
+    double r2max = wave_max(r2m);
 
     if (P.counters && lane0()) {
         atomic_add_u64(P.counters + 0, nclip);
         atomic_add_u64(P.counters + 1, ntest);
         atomic_add_u64(P.counters + 2, nlive);
         atomic_add_u64(P.counters + 3, nnew);
+    }
+    if (Prec<T>::fp32 && (flags & CELL_UNCERTAIN)) {
+        // FP32 corners that could not certify a decision: only the flag (the
+        // FP64 pass redoes the cell and writes every output)
+        if (lane0()) {
+            P.flags[fc] = CELL_UNCERTAIN;
+            P.r2[fc] = 0.0;
+        }
+        return;
     }
     if (flags & (CELL_NEED_K | CELL_OVERFLOW | CELL_BAD_TOPO)) {
         // the partial cell contains the cell: its ball filters the list tiers
@@ -1605,17 +1700,39 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         return;
     }
     // corner points and topology to LDS
+    LD r2f = LD(0.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         LB vs = mask_bit(st.valid[s]);
         LI cid = lane + 64 * s;
-        LD inv = 1.0 / select(vs, st.d[s], LD(1.0));
-        lds_st(&L.q[0][0], cid * 3 + 0, st.nx[s] * inv, vs);
-        lds_st(&L.q[0][0], cid * 3 + 1, st.ny[s] * inv, vs);
-        lds_st(&L.q[0][0], cid * 3 + 2, st.nz[s] * inv, vs);
+        LD nx, ny, nz, dd;
+        if constexpr (Prec<T>::fp32) {
+            // the FP64 corner of the plane triple, as the FP64 pass builds it
+            LD eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc;
+            load_row<double>(L, cvt<int>(byte_of(st.pl[s], 0)) & (kPMax - 1), eax, eay, eaz, ha);
+            load_row<double>(L, cvt<int>(byte_of(st.pl[s], 1)) & (kPMax - 1), ebx, eby, ebz, hb);
+            load_row<double>(L, cvt<int>(byte_of(st.pl[s], 2)) & (kPMax - 1), ecx, ecy, ecz, hc);
+            const Corner C = make_corner<double>(eax, eay, eaz, ha, ebx, eby, ebz, hb, ecx, ecy, ecz, hc);
+            nx = C.nx;
+            ny = C.ny;
+            nz = C.nz;
+            dd = C.d;
+            r2f = vmax(r2f, select(vs, C.r2, LD(0.0)));
+        } else {
+            nx = st.nx[s];
+            ny = st.ny[s];
+            nz = st.nz[s];
+            dd = st.d[s];
+        }
+        LD inv = 1.0 / select(vs, dd, LD(1.0));
+        lds_st(&L.q[0][0], cid * 3 + 0, nx * inv, vs);
+        lds_st(&L.q[0][0], cid * 3 + 1, ny * inv, vs);
+        lds_st(&L.q[0][0], cid * 3 + 2, nz * inv, vs);
         lds_st(L.cpl, cid, st.pl[s], vs);
         lds_st(L.cadj, cid, st.adj[s], vs);
     }
+    // the FP64 cell's radius (the FP32 one bounds it from above)
+    if constexpr (Prec<T>::fp32) r2max = wave_max(r2f);
     for (int f0 = 0; f0 < np; f0 += 64) {
         LI f = f0 + lane;
         lds_st(L.v2t, select(f < np, f, LI(0)), LI(0x7FFFFFFF), f < np);
@@ -1735,7 +1852,9 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
     }
     const double V = wave_sum(vacc);
     const double ntets = wave_sum(ntet);
-    if (P.mesh_mode && !face_overflow) {
+    // (the FP32 pass runs without cell meshes: the host sends mesh exports to
+    // the FP64 pass)
+    if (!Prec<T>::fp32 && P.mesh_mode && !face_overflow) {
         const double iv = V > 0.0 ? 1.0 / V : 0.0;
         mesh_emit<S>(P, L, st, np, orig_c, pc, wave_sum(mxa) * iv, wave_sum(mya) * iv,
                      wave_sum(mza) * iv);

@@ -50,6 +50,10 @@ struct EngineStats {
     unsigned long long nb_pieces = 0;  // cells over the plane limit computed in octant pieces
     unsigned long long why_rings = 0, why_walk = 0, why_narrow = 0;  // pass A -> lists
     unsigned long long nb_regather = 0;  // weighted cells gathered again without the ring limit
+    // cells of the FP32 first pass it could not certify, redone by the FP64
+    // first pass, and that launch's GPU time
+    unsigned long long nb_fp64_redo = 0;
+    double t_redo_gpu_ms = 0;
     unsigned long long nb_periodic_retries = 0;  // periodic passes redone with a wider margin
     int tuned_waves = 0;                 // autotuner's choice (0: not tuned by this call)
 };
@@ -123,6 +127,7 @@ public:
     long long list_chunk = 1ll << 27;  // entries per chunk of the list tiers
     bool autotune = false;             // pick the first pass's work-group size by timing
     bool waves4 = false;               // first pass with 4-wavefront work-groups (no tuner)
+    bool fp64_first = false;           // first pass with FP64 corner state (default FP32)
     std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
     std::vector<uint32_t> h_flags;  // by Morton position, current for census_ cells
     std::vector<int> census_;       // cells of the last pass that are not plainly done
@@ -547,6 +552,9 @@ public:
         P.list_complete = 0;
         P.ring_limit = weighted_ ? kMaxFilteredRings : kMaxVoronoiRings;
         P.corner = 0;
+        // FP32 corner state for the first pass, except for cell meshes (those
+        // are emitted by the FP64 pass)
+        P.fp32 = (gather && !fp64_first && !pb.mesh_mode) ? 1 : 0;
         if (pb.abort_on_empty) {
             // the caller only needs to know whether an instance has an empty
             // cell: its first empty cell stops it
@@ -616,6 +624,40 @@ public:
             st.a_tets += c[4];
             st.a_facets += c[5];
             st.a_cells += P.n_items;
+        }
+        if (P.fp32) {
+            // cells with a sign or corner the FP32 corners could not certify
+            // (flagged uncertain and left without outputs): the FP64 first
+            // pass, which writes every output of them
+            P.fp32 = 0;
+            const size_t cap = std::max(1, P.n_items);
+            int* d_cc = (int*)slot(S_CENS_C, sizeof(int) * cap);
+            uint32_t* d_cf = (uint32_t*)slot(S_CENS_F, sizeof(uint32_t) * cap);
+            unsigned* d_cn = (unsigned*)slot(S_CENS_N, sizeof(unsigned));
+            const unsigned m = be_.census(P.items, P.n_items, d_flags, CELL_UNCERTAIN, P.abort_flag,
+                                          P.inst_of, d_cc, d_cf, d_cn);
+            std::vector<int> cc(m), redo;
+            std::vector<uint32_t> cf(m);
+            if (m) {
+                be_.d2h(cc.data(), d_cc, sizeof(int) * m);
+                be_.d2h(cf.data(), d_cf, sizeof(uint32_t) * m);
+            }
+            for (unsigned i = 0; i < m; ++i)
+                if (cf[i] & CELL_UNCERTAIN) redo.push_back(cc[i]);
+            std::sort(redo.begin(), redo.end());
+            st.nb_fp64_redo += redo.size();
+            if (!redo.empty()) {
+                int* d_items = (int*)slot(S_ITEMS, sizeof(int) * redo.size());
+                be_.h2d(d_items, redo.data(), sizeof(int) * redo.size());
+                ClipParams Q = P;
+                Q.items = d_items;
+                Q.n_items = (int)redo.size();
+                be_.event_begin(1);
+                be_.clip(1, false, gather, Q);
+                be_.event_end(1);
+                be_.sync();
+                st.t_redo_gpu_ms += be_.event_ms(1);
+            }
         }
         auto t3 = now();
         // instances stopped by an empty cell: bad; no further pass for them

@@ -204,6 +204,37 @@ inline double vabs(double a) { return std::fabs(a); }
 inline double vsqrt(double a) { return std::sqrt(a); }
 inline double vmax(double a, double b) { return std::fmax(a, b); }
 inline double vmin(double a, double b) { return std::fmin(a, b); }
+// FP32 lanes under the same names (corner state of the FP32 clipping pass)
+inline LF vfma(const LF& a, const LF& b, const LF& c) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fma(a.v[i], b.v[i], c.v[i]);
+    return r;
+}
+inline LF vabs(const LF& a) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fabs(a.v[i]);
+    return r;
+}
+inline LF vsqrt(const LF& a) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::sqrt(a.v[i]);
+    return r;
+}
+inline LF vmax(const LF& a, const LF& b) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmax(a.v[i], b.v[i]);
+    return r;
+}
+inline LF vmin(const LF& a, const LF& b) {
+    LF r;
+    for (int i = 0; i < kWave; ++i) r.v[i] = std::fmin(a.v[i], b.v[i]);
+    return r;
+}
+inline float vfma(float a, float b, float c) { return std::fma(a, b, c); }
+inline float vabs(float a) { return std::fabs(a); }
+inline float vsqrt(float a) { return std::sqrt(a); }
+inline float vmax(float a, float b) { return std::fmax(a, b); }
+inline float vmin(float a, float b) { return std::fmin(a, b); }
 
 inline LF vmaxf(const LF& a, const LF& b) {
     LF r;
@@ -386,6 +417,11 @@ VGC_HD double vabs(double a) { return __builtin_fabs(a); }
 VGC_HD double vsqrt(double a) { return __builtin_sqrt(a); }
 VGC_HD double vmax(double a, double b) { return __builtin_fmax(a, b); }
 VGC_HD double vmin(double a, double b) { return __builtin_fmin(a, b); }
+VGC_HD float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+VGC_HD float vabs(float a) { return __builtin_fabsf(a); }
+VGC_HD float vsqrt(float a) { return __builtin_sqrtf(a); }
+VGC_HD float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+VGC_HD float vmin(float a, float b) { return __builtin_fminf(a, b); }
 
 VGC_HD float vmaxf(float a, float b) { return __builtin_fmaxf(a, b); }
 VGC_HD float vsqrtf(float a) { return __builtin_sqrtf(a); }

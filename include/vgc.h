@@ -76,6 +76,11 @@ extern "C" {
 #define VGC_WAVES4 128u       /* run the first clipping pass with 4-wavefront
                                  work-groups instead of the default 1 (identical
                                  results; for tests and tuning) */
+#define VGC_FP64_FIRST 1024u  /* run the first clipping pass with FP64 corner
+                                 state.  Default: FP32 corner state, signs
+                                 certified against the FP64 planes, and the
+                                 cells it cannot certify redone in FP64
+                                 (vgc_stats.nb_fp64_redo); identical results */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -112,6 +117,9 @@ typedef struct vgc_stats {
     uint64_t nb_pieces;   /* cells over 256 facets computed as eight octant pieces
                              (ConvexCell grows without bound,
                              voronoi/convex_cell.cpp:1086-1101) */
+    uint64_t nb_fp64_redo; /* cells of the FP32 first pass redone by the FP64
+                              first pass (a sign or corner not certified) */
+    double t_redo_gpu_ms;  /* GPU time of that FP64 launch */
 } vgc_stats;
 
 /* Context on one GPU (device index as seen by HIP).  Replaces nothing in

@@ -333,3 +333,60 @@ def test_cvt_energy_and_gradient(emu_ctx, kind):
             xm[i, c] -= h
             fd = (emu_ctx.cvt_func_grad(xp)[0] - emu_ctx.cvt_func_grad(xm)[0]) / (2 * h)
             assert abs(fd - g[i, c]) <= 1e-6 * np.abs(g).max()
+
+
+# ------------------------------------ first pass with FP32 corner state
+
+def _same_cells(a, b):
+    """The same bits in every output (the CSR up to its end)."""
+    for f in ("vol", "mg", "nbr_ptr"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    m = int(a.nbr_ptr[-1])
+    assert np.array_equal(a.nbr[:m], b.nbr[:m]), "nbr"
+    assert np.array_equal(a.farea[:m], b.farea[:m]), "farea"
+
+
+@pytest.mark.parametrize("case", ["uniform", "mixture", "laguerre", "periodic", "coincident",
+                                  "batch"])
+def test_fp32_first_pass_equals_fp64_bitwise(emu_ctx, case):
+    # FP32 corners take the FP64 pass's decisions (signs certified against
+    # the FP64 planes) and rebuild the FP64 corners for the integration, so
+    # the outputs are the same bits; uncertain cells are redone in FP64
+    rng = np.random.default_rng(11)
+    kw = {}
+    w = None
+    if case == "mixture":
+        x = ga.sample_points(1, 12, 6000)
+    elif case == "laguerre":
+        x = rng.random((6000, 3))
+        w = rng.random(6000) * 2e-4
+    elif case == "periodic":
+        x = rng.random((3000, 3))
+        kw["periodic"] = True
+    elif case == "coincident":
+        x = rng.random((3000, 3))
+        x[2000:2010] = x[100:110]
+    elif case == "batch":
+        x = rng.random((5000, 3))
+        kw["inst_n"] = [1000, 2500, 1500]
+        kw["boxes"] = np.tile([0.0, 0, 0, 1, 1, 1], (3, 1))
+    else:
+        x = rng.random((6000, 3))
+    a = emu_ctx.cells(x, w, **kw)
+    b = emu_ctx.cells(x, w, fp64_first=True, **kw)
+    _same_cells(a, b)
+    assert a.stats["nb_failed"] == b.stats["nb_failed"] == 0
+    assert b.stats["nb_fp64_redo"] == 0
+    assert a.stats["nb_fp64_redo"] < 0.02 * len(x)
+
+
+def test_fp32_first_pass_lattice_all_redone(emu_ctx):
+    # on an exact lattice every cell has ties: FP32 certifies none of them,
+    # the FP64 pass redoes every cell and sends them to the exact pass
+    g = (np.arange(6) + 0.5) / 6
+    x = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+    a = emu_ctx.cells(x)
+    b = emu_ctx.cells(x, fp64_first=True)
+    _same_cells(a, b)
+    assert a.stats["nb_fp64_redo"] == len(x)
+    assert a.stats["nb_pass_e"] == b.stats["nb_pass_e"] == len(x)

@@ -227,18 +227,65 @@ def test_ot_features_vs_oracle(gpu_ctx):
 
 # -------------------------------------------- first pass with 4-wave groups
 
+def _same_cells(a, b):
+    """The same bits in every output (the CSR up to its end)."""
+    for f in ("vol", "mg", "nbr_ptr"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    m = int(a.nbr_ptr[-1])
+    assert np.array_equal(a.nbr[:m], b.nbr[:m]), "nbr"
+    assert np.array_equal(a.farea[:m], b.farea[:m]), "farea"
+
+
 @pytest.mark.parametrize("weighted", [False, True])
 def test_four_wave_first_pass_bitwise_and_oracle(gpu_ctx, weighted):
     x = ga.sample_points(1, 31, 20000)
     w = np.random.default_rng(5).random(len(x)) * 2e-5 if weighted else None
     r1 = gpu_ctx.cells(x, weights=w)
     r4 = gpu_ctx.cells(x, weights=w, waves4=True)
-    for a in ("vol", "mg", "nbr_ptr", "nbr", "farea"):
-        assert np.array_equal(getattr(r1, a), getattr(r4, a)), a
+    _same_cells(r1, r4)
     o = oracle.cells(x, w=w, k0=20, centred=True, stride=256)
     nz = o["vol"] > 0
     assert np.array_equal(r4.vol > 0, nz)
     assert (np.abs(r4.vol[nz] - o["vol"][nz]) / o["vol"][nz]).max() < 1e-10
+
+
+# ------------------------------------ first pass with FP32 corner state
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_fp32_first_pass_bitwise_and_oracle(gpu_ctx, weighted):
+    # the default FP32 first pass against the FP64 one (VGC_FP64_FIRST): the
+    # same bits, with the cells whose signs FP32 cannot certify redone in
+    # FP64; and the oracle on every 64th cell
+    x = ga.sample_points(1, 37, 200000)
+    w = np.random.default_rng(6).random(len(x)) * 2e-6 if weighted else None
+    a = gpu_ctx.cells(x, weights=w)
+    b = gpu_ctx.cells(x, weights=w, fp64_first=True)
+    _same_cells(a, b)
+    assert 0 < a.stats["nb_fp64_redo"] < 0.01 * len(x)
+    assert b.stats["nb_fp64_redo"] == 0
+    assert a.stats["nb_failed"] == 0
+    idx = np.arange(0, len(x), 64)
+    o = oracle.cells(x, w=w, k0=20, centred=True, stride=128, subset=idx)
+    nz = o["vol"] > 0
+    assert np.array_equal(a.vol[idx] > 0, nz)
+    assert (np.abs(a.vol[idx][nz] - o["vol"][nz]) / o["vol"][nz]).max() < 1e-10
+    for k, i in enumerate(idx):
+        assert set(a.neighbors(i).tolist()) == set(o["nbr"][k, : o["cnt"][k]].tolist())
+
+
+def test_fp32_first_pass_periodic_and_lattice(gpu_ctx):
+    rng = np.random.default_rng(8)
+    x = rng.random((20000, 3))
+    a = gpu_ctx.cells(x, periodic=True)
+    b = gpu_ctx.cells(x, periodic=True, fp64_first=True)
+    _same_cells(a, b)
+    g = (np.arange(8) + 0.5) / 8
+    x = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+    a = gpu_ctx.cells(x)
+    b = gpu_ctx.cells(x, fp64_first=True)
+    _same_cells(a, b)
+    assert a.stats["nb_fp64_redo"] == len(x)
 
 
 # --------------------------------------------------- configurations, full size
