@@ -296,12 +296,14 @@ VGC_HD CornerT<T> make_corner(const Lane<T>& eax, const Lane<T>& eay, const Lane
     LD qx = C.nx * inv, qy = C.ny * inv, qz = C.nz * inv;
     C.r2 = vfma(qx, qx, vfma(qy, qy, qz * qz));
     if constexpr (Prec<T>::fp32) {
-        // (|N| + sqrt(3) gamma9 max|N|) / (D - gamma9 |D|_abs), squared, plus
-        // 1e-5 for the rounding of this evaluation
-        const LD nn = vsqrt(vfma(C.nx, C.nx, vfma(C.ny, C.ny, C.nz * C.nz)));
-        const LD r = (nn + T(1.8f * Prec<T>::gamma9) * C.nmax) /
-                     select(C.degenerate, LD(T(1)), C.d - T(Prec<T>::gamma9) * C.dabs);
-        C.r2 = (r * r) * T(1.00001f);
+        // (|N| + a)^2 / (D - gamma9 |D|_abs)^2 with a = 1.8 gamma9 max|N|abs
+        // (the length of N's error), plus 1e-5 for the rounding of this
+        // evaluation; |N| <= sqrt(3) max_i |N_i| in the cross term
+        const LD a = T(1.8f * Prec<T>::gamma9) * C.nmax;
+        const LD m = vmax(vabs(C.nx), vmax(vabs(C.ny), vabs(C.nz)));
+        const LD num = vfma(C.nx, C.nx, vfma(C.ny, C.ny, C.nz * C.nz)) + a * vfma(T(3.47f), m, a);
+        const LD den = select(C.degenerate, LD(T(1)), C.d - T(Prec<T>::gamma9) * C.dabs);
+        C.r2 = (num / (den * den)) * T(1.00001f);
     }
     return C;
 }
@@ -1789,7 +1791,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
             LD vx = q1x - ox, vy = q1y - oy, vz = q1z - oz;
             LD wx = qx - ox, wy = qy - oy, wz = qz - oz;
             LD cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
-            LD tv = vabs(cx * wx + cy * wy + cz * wz) / 6.0;
+            LD tv = vabs(cx * wx + cy * wy + cz * wz) * (1.0 / 6.0);
             // triangle (q0, q1, q) as in triangle_area (convex_cell.cpp:1147-1158)
             LD ax = q1x - q0x, ay = q1y - q0y, az = q1z - q0z;
             LD bx = qx - q0x, by = qy - q0y, bz = qz - q0z;
@@ -1806,7 +1808,7 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
                 // V/20 (sum |a|^2 + |a + b + c + d|^2)
                 const LD nn = oq0 + (q1x * q1x + q1y * q1y + q1z * q1z) + (qx * qx + qy * qy + qz * qz) +
                               (sx * sx + sy * sy + sz * sz);
-                fe = select(run, fe + tv * nn / 20.0, fe);
+                fe = select(run, fe + tv * nn * (1.0 / 20.0), fe);
             }
             area = select(run, area + ta, area);
             q1x = select(run, qx, q1x);

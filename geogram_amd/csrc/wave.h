@@ -476,17 +476,46 @@ VGC_HD double wave_sum(double a) {
     for (int m = 32; m >= 1; m >>= 1) x = x + __shfl_xor(x, m);
     return x;
 }
+// Maxima over the wave through DPP lane moves (no LDS): quad swaps, the
+// half-row and row mirrors (max over 16 lanes), then lane 15 of rows 0 / 2
+// into rows 1 / 3 and lane 31 into rows 2 / 3, so lane 63 holds the maximum.
+// A DPP move leaves the rows outside its row mask at their own value.  The
+// maximum does not depend on the order of the reduction; every call has
+// the whole wave active.
+template <int CTRL, int ROWS>
+VGC_HD uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+VGC_HD double dpp_f64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = dpp_u32<CTRL, ROWS>((uint32_t)u);
+    const uint32_t hi = dpp_u32<CTRL, ROWS>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+template <int CTRL, int ROWS>
+VGC_HD float dpp_f32(float x) {
+    return __builtin_bit_cast(float, dpp_u32<CTRL, ROWS>(__builtin_bit_cast(uint32_t, x)));
+}
 VGC_HD double wave_max(double a) {
     double x = a;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x = __builtin_fmax(x, __shfl_xor(x, m));
-    return x;
+    x = __builtin_fmax(x, dpp_f64<0xB1, 0xF>(x));   // quad_perm [1, 0, 3, 2]
+    x = __builtin_fmax(x, dpp_f64<0x4E, 0xF>(x));   // quad_perm [2, 3, 0, 1]
+    x = __builtin_fmax(x, dpp_f64<0x141, 0xF>(x));  // row_half_mirror
+    x = __builtin_fmax(x, dpp_f64<0x140, 0xF>(x));  // row_mirror
+    x = __builtin_fmax(x, dpp_f64<0x142, 0xA>(x));  // row_bcast:15 into rows 1, 3
+    x = __builtin_fmax(x, dpp_f64<0x143, 0xC>(x));  // row_bcast:31 into rows 2, 3
+    return readlane(x, 63);
 }
 VGC_HD float wave_maxf(float a) {
     float x = a;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x = __builtin_fmaxf(x, __shfl_xor(x, m));
-    return x;
+    x = __builtin_fmaxf(x, dpp_f32<0xB1, 0xF>(x));
+    x = __builtin_fmaxf(x, dpp_f32<0x4E, 0xF>(x));
+    x = __builtin_fmaxf(x, dpp_f32<0x141, 0xF>(x));
+    x = __builtin_fmaxf(x, dpp_f32<0x140, 0xF>(x));
+    x = __builtin_fmaxf(x, dpp_f32<0x142, 0xA>(x));
+    x = __builtin_fmaxf(x, dpp_f32<0x143, 0xC>(x));
+    return readlane(x, 63);
 }
 
 template <class T>
