@@ -625,7 +625,7 @@ struct EmuBackend {
         for (int k = 0; k < T.n_inst; ++k) {
             double t1, t2;
             seg_final(T, k, seg_part_.data(), t1, t2);
-            double* c = cg + 8 * k;
+            double* c = cg + kCgStride * k;
             c[0] = t1;
             c[1] = t2;
             c[4] = 0.0;
@@ -642,12 +642,12 @@ struct EmuBackend {
         seg_part_.assign(2 * (size_t)T.nblk, 0.0);
         for (int it = 0; it < kk; ++it) {
             for (int i = 0; i < n; ++i) {
-                const double* c = cg + 8 * T.inst_of[i];
+                const double* c = cg + kCgStride * T.inst_of[i];
                 if (c[6] == 0.0) p[i] = z[i] + c[4] * p[i];
             }
 #pragma omp parallel for schedule(static)
             for (int i = 0; i < n; ++i) {
-                if (cg[8 * T.inst_of[i] + 6] != 0.0) continue;
+                if (cg[kCgStride * T.inst_of[i] + 6] != 0.0) continue;
                 double sl[kSpmvLanes];
                 for (int l = 0; l < kSpmvLanes; ++l) sl[l] = ot_spmv_lane(i, l, ptr, col, val, p);
                 for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) {
@@ -659,14 +659,14 @@ struct EmuBackend {
             }
             for (int blk = 0; blk < T.nblk; ++blk) {
                 const int k = T.blk_inst[blk];
-                if (cg[8 * k + 6] != 0.0) continue;
+                if (cg[kCgStride * k + 6] != 0.0) continue;
                 double t1, t2;
                 seg_block_sums(T, blk, [&](long long i, double& s1, double&) { s1 += p[i] * q[i]; },
                                t1, t2);
                 seg_part_[blk] = t1;
             }
             for (int k = 0; k < T.n_inst; ++k) {
-                double* c = cg + 8 * k;
+                double* c = cg + kCgStride * k;
                 if (c[6] != 0.0) continue;
                 double pq, dummy;
                 seg_final(T, k, seg_part_.data(), pq, dummy);
@@ -681,8 +681,8 @@ struct EmuBackend {
 #pragma omp parallel for schedule(dynamic, 4)
             for (int blk = 0; blk < T.nblk; ++blk) {
                 const int k = T.blk_inst[blk];
-                if (cg[8 * k + 6] != 0.0) continue;
-                const double alpha = cg[8 * k + 3];
+                if (cg[kCgStride * k + 6] != 0.0) continue;
+                const double alpha = cg[kCgStride * k + 3];
                 double t1, t2;
                 seg_block_sums(T, blk, [&](long long i, double& s1, double& s2) {
                     x[i] += alpha * p[i];
@@ -697,7 +697,7 @@ struct EmuBackend {
                 seg_part_[T.nblk + blk] = t2;
             }
             for (int k = 0; k < T.n_inst; ++k) {
-                double* c = cg + 8 * k;
+                double* c = cg + kCgStride * k;
                 if (c[6] != 0.0) continue;
                 double rz, rr;
                 seg_final(T, k, seg_part_.data(), rz, rr);

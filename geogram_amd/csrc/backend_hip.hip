@@ -733,93 +733,158 @@ __global__ __launch_bounds__(256) void k_cg_start_final(const double* __restrict
         cg[5] = tol2 * t2;
         cg[6] = t2 > 0.0 ? 0.0 : 1.0;
         cg[7] = 0.0;
+        cg[8] = t1;    // r.z of the first direction (iteration parity 0)
+        cg[10] = 1.0;  // beta (cg[4]) is current
     }
 }
 
-// The last of a grid's blocks to finish, after each block's thread 0 wrote
-// its partial: the block partials are then visible to it (fence, counter,
-// fence), and it resets the counter for the next launch.
-__device__ __forceinline__ bool cg_last_block(unsigned* ctr) {
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+// PCG iteration, four kernels without atomics.  Each reduction writes
+// kRedBlocks block partials; the kernel that needs the total has every one
+// of its blocks add the partials in the fixed order (thread t takes t,
+// t + 256, ..., then the halving tree), so every block holds the same value
+// and block 0 alone writes the scalars.  Scalars cg[16]: [0] r.z, [1] r.r,
+// [2] p.q, [3] alpha, [4] beta, [5] tol^2 b.b, [6] stop (1 converged,
+// 2 breakdown), [7] iterations, [8 + q] r.z of the direction of the
+// iterations of parity q (double-buffered: a kernel reads one slot while its
+// block 0 writes the other), [10] beta is current (set by cg_start and
+// k_cg_close).  Partials: [0, R) p.q, [R (1 + 2q), R (3 + 2q)) r.z and r.r of
+// the update of parity q.
+__device__ __forceinline__ void red_final2(const double* __restrict__ pa,
+                                           const double* __restrict__ pb, double* sh, double* sh2,
+                                           double& ta, double& tb) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < kRedBlocks; k += 256) {
+        a += pa[k];
+        b += pb[k];
     }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
+    block_sum2(a, b, sh, sh2, ta, tb);
 }
 
-__device__ __forceinline__ double cg_read_part(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Closes the update of parity par: counts it, stops on r.r <= tol^2 b.b,
+// else beta = r.z / previous r.z.  Block 0 of the caller writes the
+// scalars; the next direction uses parity slot next.
+__device__ __forceinline__ bool cg_close_update(double* cg, const double* __restrict__ part,
+                                                int par, int next, double* sh, double* sh2,
+                                                double& beta) {
+    double rz, rr;
+    red_final2(part + kRedBlocks * (1 + 2 * par), part + kRedBlocks * (2 + 2 * par), sh, sh2, rz,
+               rr);
+    const bool stop = rr <= cg[5];
+    beta = stop ? 0.0 : rz / cg[8 + par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cg[1] = rr;
+        cg[7] += 1.0;
+        cg[4] = beta;
+        if (stop) {
+            cg[6] = 1.0;
+        } else {
+            cg[0] = rz;
+            cg[8 + next] = rz;
+        }
+    }
+    return stop;
 }
 
-// PCG iteration, three kernels.  First, the direction p = z + beta p
-// (beta = cg[4]: 0 after cg_start, whose p is z).
-__global__ void k_cg_dir(int n, const double* __restrict__ z, double* p,
-                         const double* __restrict__ cg) {
+// First: complete the previous iteration (unless beta is current), then the
+// direction p = z + beta p.
+__global__ __launch_bounds__(256) void k_cg_dir(int n, const double* __restrict__ z,
+                                                double* __restrict__ p, double* cg,
+                                                const double* __restrict__ part, int par) {
+    __shared__ double sh[256], sh2[256];
     if (cg[6] != 0.0) return;  // uniform over the grid
-    const double beta = cg[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
+    double beta = cg[4];
+    if (cg[10] == 0.0 && cg_close_update(cg, part, par ^ 1, par, sh, sh2, beta)) return;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256)
+        p[i] = z[i] + beta * p[i];
 }
 
-// Second: q = A p, one 16-lane group per row.
+// Second: q = A p, one 16-lane group per row; a grid of at most
+// kSpmvBlocks resident work-groups strides over the rows (a work-group per
+// 16 rows would spend its short life being dispatched).
+constexpr int kSpmvBlocks = 2048;
 __global__ __launch_bounds__(256) void k_cg_spmv(int n, const long long* __restrict__ ptr,
                                                  const int* __restrict__ col,
                                                  const double* __restrict__ val,
                                                  const double* __restrict__ diag,
                                                  const double* __restrict__ p, double* q,
-                                                 const double* __restrict__ cg) {
+                                                 double* cg) {
     if (cg[6] != 0.0) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
-    double s = i < n ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cg[10] = 0.0;  // read by the next k_cg_dir only
+    // four row groups per step, their loads issued together (the first
+    // kSpmvLanes entries of a row as straight-line code); each row's sum is
+    // formed in the order of ot_spmv_lane
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * blockDim.x, N = (long long)n * kSpmvLanes;
+    for (long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t0 < N; t0 += U * stride) {
+        double s[U];
+        int row[U];
+        bool ok[U];
+        long long k[U], e[U];
 #pragma unroll
-    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    if (i < n && l == 0) q[i] = s + diag[i] * p[i];
+        for (int u = 0; u < U; ++u) {
+            const long long t = t0 + u * stride;
+            ok[u] = t < N;
+            row[u] = ok[u] ? int(t / kSpmvLanes) : 0;
+            k[u] = ptr[row[u]] + int(t % kSpmvLanes);
+            e[u] = ok[u] ? ptr[row[u] + 1] : 0;
+        }
+        double v[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = k[u] < e[u];
+            v[u] = in ? val[k[u]] : 0.0;
+            x[u] = in ? p[col[k[u]]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s[u] = 0.0;
+            if (k[u] < e[u]) s[u] += v[u] * x[u];
+            for (long long kk = k[u] + kSpmvLanes; kk < e[u]; kk += kSpmvLanes) s[u] += val[kk] * p[col[kk]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s[u] += __shfl_xor(s[u], m);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u] && (t0 % kSpmvLanes) == 0) q[row[u]] = s[u] + diag[row[u]] * p[row[u]];
+    }
 }
 
-// Third: p.q (kRedBlocks blocks, the fixed-order two-level reduction); the
-// last block to finish sets alpha = rz / p.q (cg[3]) or the breakdown flag.
-__global__ __launch_bounds__(256) void k_cg_dot_alpha(int n, const double* __restrict__ p,
-                                                      const double* __restrict__ q, double* cg,
-                                                      double* part, unsigned* ctr) {
+// Third: the block partials of p.q.
+__global__ __launch_bounds__(256) void k_cg_dot(int n, const double* __restrict__ p,
+                                                const double* __restrict__ q,
+                                                const double* __restrict__ cg, double* part) {
     __shared__ double sh[256];
     if (cg[6] != 0.0) return;
     double s = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) s += p[i] * q[i];
     const double t = block_sum(s, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
-    if (!cg_last_block(ctr)) return;
-    __syncthreads();
-    double a = 0.0;
-    for (int b = threadIdx.x; b < kRedBlocks; b += 256) a += cg_read_part(part + b);
-    const double pq = block_sum(a, sh);
-    if (threadIdx.x == 0) {
-        cg[2] = pq;
-        if (!(pq > 0.0)) {
-            cg[3] = 0.0;
-            cg[6] = 2.0;  // breakdown: stop, the iteration does not count
-        } else {
-            cg[3] = cg[0] / pq;
-        }
-        *ctr = 0u;
-    }
 }
 
-// Fourth: x += alpha p, r -= alpha q, z = r / diag, r.z and r.r
-// (kRedBlocks blocks); the last block counts the iteration, stops on
-// r.r <= tol^2 b.b, else sets beta = r.z / previous r.z (cg[4]).
-__global__ __launch_bounds__(256) void k_cg_update_beta(int n, const double* __restrict__ p,
-                                                        const double* __restrict__ q,
-                                                        const double* __restrict__ diag, double* x,
-                                                        double* r, double* z, double* cg,
-                                                        double* part, unsigned* ctr) {
+// Fourth: alpha = r.z / p.q (or the breakdown stop), x += alpha p,
+// r -= alpha q, z = r / diag, and the block partials of r.z and r.r.
+__global__ __launch_bounds__(256) void k_cg_update(int n, const double* __restrict__ p,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ diag,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ z, double* cg,
+                                                   double* part, int par) {
     __shared__ double sh[256], sh2[256];
     if (cg[6] != 0.0) return;
-    const double alpha = cg[3];
+    double a = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += 256) a += part[b];
+    const double pq = block_sum(a, sh);
+    const bool breakdown = !(pq > 0.0);
+    const double alpha = breakdown ? 0.0 : cg[8 + par] / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cg[2] = pq;
+        cg[3] = alpha;
+        if (breakdown) cg[6] = 2.0;  // stop, the iteration does not count
+    }
+    if (breakdown) return;
     double s1 = 0.0, s2 = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
         x[i] += alpha * p[i];
@@ -833,30 +898,20 @@ __global__ __launch_bounds__(256) void k_cg_update_beta(int n, const double* __r
     double t1, t2;
     block_sum2(s1, s2, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
-        part[kRedBlocks + blockIdx.x] = t1;
-        part[2 * kRedBlocks + blockIdx.x] = t2;
+        part[kRedBlocks * (1 + 2 * par) + blockIdx.x] = t1;
+        part[kRedBlocks * (2 + 2 * par) + blockIdx.x] = t2;
     }
-    if (!cg_last_block(ctr)) return;
-    __syncthreads();
-    s1 = s2 = 0.0;
-    for (int b = threadIdx.x; b < kRedBlocks; b += 256) {
-        s1 += cg_read_part(part + kRedBlocks + b);
-        s2 += cg_read_part(part + 2 * kRedBlocks + b);
-    }
-    double rz, rr;
-    block_sum2(s1, s2, sh, sh2, rz, rr);
-    if (threadIdx.x == 0) {
-        cg[1] = rr;
-        cg[7] += 1.0;
-        if (rr <= cg[5]) {
-            cg[4] = 0.0;
-            cg[6] = 1.0;
-        } else {
-            cg[4] = rz / cg[0];
-            cg[0] = rz;
-        }
-        *ctr = 0u;
-    }
+}
+
+// After the last iteration of a captured chunk (parity par): its update
+// closed by one block, so the host reads final scalars between chunks.
+__global__ __launch_bounds__(256) void k_cg_close(double* cg, const double* __restrict__ part,
+                                                  int par) {
+    __shared__ double sh[256], sh2[256];
+    if (cg[6] != 0.0 || cg[10] != 0.0) return;
+    double beta;
+    cg_close_update(cg, part, par, 0, sh, sh2, beta);
+    if (threadIdx.x == 0) cg[10] = 1.0;
 }
 
 // Morton order of a Newton system: inv[orig[m]] = m; row lengths, rows and
@@ -1009,40 +1064,65 @@ __global__ __launch_bounds__(256) void k_cgb_start_final(SegTable T, const doubl
     double t1, t2;
     seg_final2(T, k, part, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
-        double* c = cg + 8 * k;
+        double* c = cg + kCgStride * k;
         c[0] = t1;
         c[1] = t2;
         c[4] = 0.0;
         c[5] = tol2[k] * t2;
         c[6] = t2 > 0.0 ? 0.0 : 1.0;
         c[7] = 0.0;
+        c[8] = t1;    // r.z of the first direction (iteration parity 0)
+        c[10] = 1.0;  // beta (c[4]) is current
     }
 }
 
-// The last of instance k's cnt[k] blocks to finish (per-instance counter;
-// reset by that block for the next launch).
-__device__ __forceinline__ bool cgb_last_block(unsigned* ctr, int cnt) {
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(ctr, 1u) == (unsigned)cnt - 1u;
+// Batched PCG iteration: the kernels of the single-instance iteration
+// (k_cg_dir ... k_cg_update) per instance over the block table: every block
+// of instance k adds the instance's cnt[k] block partials in the fixed order
+// (thread t takes partial t, then the halving tree), so the instance's
+// blocks agree, and its block 0 (blk_idx 0) writes its scalars
+// (kCgStride per instance).  Partials: [0, nblk) p.q, [nblk (1 + 2q),
+// nblk (3 + 2q)) r.z and r.r of the update of parity q.
+__device__ __forceinline__ bool cgb_close_update(const SegTable& T, int k, double* c,
+                                                 const double* __restrict__ part, int par, int next,
+                                                 bool writer, double* sh, double* sh2,
+                                                 double& beta) {
+    const int cnt = T.cnt[k], f = T.first[k];
+    const double* pa = part + T.nblk * (1 + 2 * par);
+    const double a = (int)threadIdx.x < cnt ? pa[f + threadIdx.x] : 0.0;
+    const double b = (int)threadIdx.x < cnt ? pa[T.nblk + f + threadIdx.x] : 0.0;
+    double rz, rr;
+    block_sum2(a, b, sh, sh2, rz, rr);
+    const bool stop = rr <= c[5];
+    beta = stop ? 0.0 : rz / c[8 + par];
+    if (writer && threadIdx.x == 0) {
+        c[1] = rr;
+        c[7] += 1.0;
+        c[4] = beta;
+        if (stop) {
+            c[6] = 1.0;
+        } else {
+            c[0] = rz;
+            c[8 + next] = rz;
+        }
     }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
+    return stop;
 }
 
-// Batched PCG iteration: the direction p = z + beta_k p of the active
-// instances' rows, q = A p (one 16-lane group per row), then per instance
-// (block table) p.q, whose last block sets the instance's alpha.
-__global__ void k_cgb_dir(int n, const int* __restrict__ inst_of, const double* __restrict__ z,
-                          double* p, const double* __restrict__ cg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* c = cg + 8 * inst_of[i];
-    if (c[6] != 0.0) return;
-    p[i] = z[i] + c[4] * p[i];
+__global__ __launch_bounds__(256) void k_cgb_dir(SegTable T, const double* __restrict__ z,
+                                                 double* __restrict__ p, double* cg,
+                                                 const double* __restrict__ part, int par) {
+    __shared__ double sh[256], sh2[256];
+    int k;
+    long long beg, end, stride;
+    seg_range(T, blockIdx.x, k, beg, end, stride);
+    double* c = cg + kCgStride * k;
+    if (c[6] != 0.0) return;  // uniform over the block and the instance
+    double beta = c[4];
+    if (c[10] == 0.0 &&
+        cgb_close_update(T, k, c, part, par ^ 1, par, T.blk_idx[blockIdx.x] == 0, sh, sh2, beta))
+        return;
+    for (long long i = beg; i < end; i += stride) p[i] = z[i] + beta * p[i];
 }
 
 __global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__ inst_of,
@@ -1054,60 +1134,54 @@ __global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__
                                                   const double* __restrict__ cg) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
-    const bool live = i < n && cg[8 * inst_of[i < n ? i : 0] + 6] == 0.0;
+    const bool live = i < n && cg[kCgStride * inst_of[i < n ? i : 0] + 6] == 0.0;
     double s = live ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
 #pragma unroll
     for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
     if (live && l == 0) q[i] = s + diag[i] * p[i];
 }
 
-__global__ __launch_bounds__(256) void k_cgb_dot_alpha(SegTable T, const double* __restrict__ p,
-                                                       const double* __restrict__ q, double* cg,
-                                                       double* part, unsigned* ctr) {
+__global__ __launch_bounds__(256) void k_cgb_dot(SegTable T, const double* __restrict__ p,
+                                                 const double* __restrict__ q, double* cg,
+                                                 double* part) {
     __shared__ double sh[256], sh2[256];
     int k;
     long long beg, end, stride;
     seg_range(T, blockIdx.x, k, beg, end, stride);
-    double* c = cg + 8 * k;
-    if (c[6] != 0.0) return;  // uniform over the block and the instance
+    double* c = cg + kCgStride * k;
+    if (c[6] != 0.0) return;
+    if (T.blk_idx[blockIdx.x] == 0 && threadIdx.x == 0) c[10] = 0.0;  // read by the next dir only
     double s = 0.0;
     for (long long i = beg; i < end; i += stride) s += p[i] * q[i];
     double t1, t2;
     block_sum2(s, 0.0, sh, sh2, t1, t2);
     if (threadIdx.x == 0) part[blockIdx.x] = t1;
-    if (!cgb_last_block(ctr + k, T.cnt[k])) return;
-    __syncthreads();
-    const int cnt = T.cnt[k], f = T.first[k];
-    const double a = (int)threadIdx.x < cnt ? cg_read_part(part + f + threadIdx.x) : 0.0;
-    double pq, dummy;
-    block_sum2(a, 0.0, sh, sh2, pq, dummy);
-    if (threadIdx.x == 0) {
-        c[2] = pq;
-        if (!(pq > 0.0)) {
-            c[3] = 0.0;
-            c[6] = 2.0;
-        } else {
-            c[3] = c[0] / pq;
-        }
-        ctr[k] = 0u;
-    }
 }
 
-// Batched PCG iteration, last kernel: the update and the two dots per
-// instance; the instance's last block counts the iteration and sets beta
-// or stops it.
-__global__ __launch_bounds__(256) void k_cgb_update_beta(SegTable T, const double* __restrict__ p,
-                                                     const double* __restrict__ q,
-                                                     const double* __restrict__ diag, double* x,
-                                                     double* r, double* z, double* cg,
-                                                     double* part, unsigned* ctr) {
+__global__ __launch_bounds__(256) void k_cgb_update(SegTable T, const double* __restrict__ p,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ diag,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ z, double* cg,
+                                                    double* part, int par) {
     __shared__ double sh[256], sh2[256];
     int k;
     long long beg, end, stride;
     seg_range(T, blockIdx.x, k, beg, end, stride);
-    double* c = cg + 8 * k;
+    double* c = cg + kCgStride * k;
     if (c[6] != 0.0) return;
-    const double alpha = c[3];
+    const int cnt = T.cnt[k], f = T.first[k];
+    const double a = (int)threadIdx.x < cnt ? part[f + threadIdx.x] : 0.0;
+    double pq, dummy;
+    block_sum2(a, 0.0, sh, sh2, pq, dummy);
+    const bool breakdown = !(pq > 0.0);
+    const double alpha = breakdown ? 0.0 : c[8 + par] / pq;
+    if (T.blk_idx[blockIdx.x] == 0 && threadIdx.x == 0) {
+        c[2] = pq;
+        c[3] = alpha;
+        if (breakdown) c[6] = 2.0;
+    }
+    if (breakdown) return;
     double s1 = 0.0, s2 = 0.0;
     for (long long i = beg; i < end; i += stride) {
         x[i] += alpha * p[i];
@@ -1121,28 +1195,21 @@ __global__ __launch_bounds__(256) void k_cgb_update_beta(SegTable T, const doubl
     double t1, t2;
     block_sum2(s1, s2, sh, sh2, t1, t2);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = t1;
-        part[T.nblk + blockIdx.x] = t2;
+        part[T.nblk * (1 + 2 * par) + blockIdx.x] = t1;
+        part[T.nblk * (2 + 2 * par) + blockIdx.x] = t2;
     }
-    if (!cgb_last_block(ctr + T.n_inst + k, T.cnt[k])) return;
-    __syncthreads();
-    const int cnt = T.cnt[k], f = T.first[k];
-    const double a = (int)threadIdx.x < cnt ? cg_read_part(part + f + threadIdx.x) : 0.0;
-    const double b = (int)threadIdx.x < cnt ? cg_read_part(part + T.nblk + f + threadIdx.x) : 0.0;
-    double rz, rr;
-    block_sum2(a, b, sh, sh2, rz, rr);
-    if (threadIdx.x == 0) {
-        c[1] = rr;
-        c[7] += 1.0;
-        if (rr <= c[5]) {
-            c[4] = 0.0;
-            c[6] = 1.0;
-        } else {
-            c[4] = rz / c[0];
-            c[0] = rz;
-        }
-        ctr[T.n_inst + k] = 0u;
-    }
+}
+
+// one block per instance after a chunk's last iteration (parity par)
+__global__ __launch_bounds__(256) void k_cgb_close(SegTable T, double* cg,
+                                                   const double* __restrict__ part, int par) {
+    __shared__ double sh[256], sh2[256];
+    const int k = blockIdx.x;
+    double* c = cg + kCgStride * k;
+    if (c[6] != 0.0 || c[10] != 0.0) return;
+    double beta;
+    cgb_close_update(T, k, c, part, par, 0, true, sh, sh2, beta);
+    if (threadIdx.x == 0) c[10] = 1.0;
 }
 
 // per instance (one block each): b -= mean of b over the instance's rows —
@@ -1252,8 +1319,6 @@ struct HipBackend {
         if (red_) (void)hipFree(red_);
         if (vdot_) (void)hipFree(vdot_);
         if (cg_part_) (void)hipFree(cg_part_);
-        if (cg_ctr_) (void)hipFree(cg_ctr_);
-        if (cgb_ctr_) (void)hipFree(cgb_ctr_);
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
@@ -1632,7 +1697,8 @@ struct HipBackend {
             cg_exec_ = nullptr;
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg);
+            for (int i = 0; i < k; ++i) cg_iter(n, ptr, col, val, diag, x, r, z, p, q, cg, i & 1);
+            k_cg_close<<<1, 256, 0, stream_>>>(cg, part, (k - 1) & 1);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cg_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
@@ -1642,29 +1708,23 @@ struct HipBackend {
     }
     void cg_iter(int n, const long long* ptr, const int* col, const double* val,
                  const double* diag, double* x, double* r, double* z, double* p, double* q,
-                 double* cg) {
+                 double* cg, int par) {
         double* part = cg_part(n);
-        k_cg_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, z, p, cg);
-        k_cg_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(n, ptr, col, val,
-                                                                                   diag, p, q, cg);
-        k_cg_dot_alpha<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, cg, part, cg_ctr_);
-        k_cg_update_beta<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part,
-                                                          cg_ctr_ + 1);
+        k_cg_dir<<<kRedBlocks, 256, 0, stream_>>>(n, z, p, cg, part, par);
+        const unsigned gs = blocks_for((long long)n * kSpmvLanes, 256);
+        k_cg_spmv<<<gs < kSpmvBlocks ? gs : kSpmvBlocks, 256, 0, stream_>>>(n, ptr, col, val, diag,
+                                                                          p, q, cg);
+        k_cg_dot<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, cg, part);
+        k_cg_update<<<kRedBlocks, 256, 0, stream_>>>(n, p, q, diag, x, r, z, cg, part, par);
         check();
     }
-    // block partials (one per block of the dot, two per block of the
-    // update) and the two last-block counters of the PCG kernels (zeroed
-    // when allocated; each last block resets its counter)
+    // block partials (p.q, and r.z, r.r of the two update parities)
     double* cg_part(int) {
-        const long long need = 3ll * kRedBlocks;
+        const long long need = 5ll * kRedBlocks;
         if (cg_part_cap_ < need) {
             if (cg_part_) VGC_HIP_CHECK(hipFree(cg_part_));
             VGC_HIP_CHECK(hipMalloc(&cg_part_, sizeof(double) * (size_t)need));
             cg_part_cap_ = need;
-        }
-        if (!cg_ctr_) {
-            VGC_HIP_CHECK(hipMalloc(&cg_ctr_, sizeof(unsigned) * 2));
-            VGC_HIP_CHECK(hipMemsetAsync(cg_ctr_, 0, sizeof(unsigned) * 2, stream_));
         }
         return cg_part_;
     }
@@ -1698,11 +1758,12 @@ struct HipBackend {
     }
 
     // --------------------------------------------------------- batched OT
-    double* seg_part(int nblk) {
-        if (seg_part_cap_ < nblk) {
+    // block partials of the batched reductions: count doubles
+    double* seg_part(long long count) {
+        if (seg_part_cap_ < count) {
             if (seg_part_) VGC_HIP_CHECK(hipFree(seg_part_));
-            VGC_HIP_CHECK(hipMalloc(&seg_part_, sizeof(double) * 2 * (size_t)nblk));
-            seg_part_cap_ = nblk;
+            VGC_HIP_CHECK(hipMalloc(&seg_part_, sizeof(double) * (size_t)count));
+            seg_part_cap_ = count;
         }
         return seg_part_;
     }
@@ -1727,7 +1788,7 @@ struct HipBackend {
     }
     void cgb_start(const SegTable& T, const double* b, const double* diag, double* x, double* r,
                    double* z, double* p, double* cg, const double* tol2) {
-        double* part = seg_part(T.nblk);
+        double* part = seg_part(2 * (long long)T.nblk);
         k_cgb_start_partial<<<T.nblk, 256, 0, stream_>>>(T, b, diag, x, r, z, p, part);
         k_cgb_start_final<<<T.n_inst, 256, 0, stream_>>>(T, part, cg, tol2);
         check();
@@ -1737,12 +1798,11 @@ struct HipBackend {
     void cgb_iters(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                    const double* val, const double* diag, double* x, double* r, double* z,
                    double* p, double* q, double* cg) {
-        double* part = seg_part(T.nblk);  // allocated outside the capture
-        unsigned* ctr = cgb_ctr(T.n_inst);
+        double* part = seg_part(5 * (long long)T.nblk);  // allocated outside the capture
         const void* key[22] = {(const void*)(long long)k, (const void*)(long long)n,
                                (const void*)(long long)T.n_inst, (const void*)(long long)T.nblk,
                                T.off, T.blk_inst, T.blk_idx, T.cnt, T.first, T.inst_of, ptr, col,
-                               val, diag, x, r, z, p, q, cg, part, ctr};
+                               val, diag, x, r, z, p, q, cg, part, nullptr};
         bool same = cgb_exec_ != nullptr;
         for (int i = 0; i < 22 && same; ++i) same = cgb_key_[i] == key[i];
         if (!same) {
@@ -1750,7 +1810,7 @@ struct HipBackend {
             cgb_exec_ = nullptr;
             hipGraph_t g = nullptr;
             VGC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, q, cg, part, ctr);
+            cgb_launch(k, T, n, ptr, col, val, diag, x, r, z, p, q, cg, part);
             VGC_HIP_CHECK(hipStreamEndCapture(stream_, &g));
             VGC_HIP_CHECK(hipGraphInstantiate(&cgb_exec_, g, nullptr, nullptr, 0));
             VGC_HIP_CHECK(hipGraphDestroy(g));
@@ -1760,27 +1820,16 @@ struct HipBackend {
     }
     void cgb_launch(int k, const SegTable& T, int n, const long long* ptr, const int* col,
                     const double* val, const double* diag, double* x, double* r, double* z,
-                    double* p, double* q, double* cg, double* part, unsigned* ctr) {
+                    double* p, double* q, double* cg, double* part) {
         for (int it = 0; it < k; ++it) {
-            k_cgb_dir<<<blocks_for(n, 256), 256, 0, stream_>>>(n, T.inst_of, z, p, cg);
+            k_cgb_dir<<<T.nblk, 256, 0, stream_>>>(T, z, p, cg, part, it & 1);
             k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
                 n, T.inst_of, ptr, col, val, diag, p, q, cg);
-            k_cgb_dot_alpha<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part, ctr);
-            k_cgb_update_beta<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part, ctr);
+            k_cgb_dot<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part);
+            k_cgb_update<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part, it & 1);
         }
+        k_cgb_close<<<T.n_inst, 256, 0, stream_>>>(T, cg, part, (k - 1) & 1);
         check();
-    }
-    // per-instance last-block counters of the batched PCG kernels (two per
-    // instance), zeroed when allocated; each last block resets its own
-    unsigned* cgb_ctr(int n_inst) {
-        if (cgb_ctr_cap_ < n_inst) {
-            if (cgb_ctr_) VGC_HIP_CHECK(hipFree(cgb_ctr_));
-            VGC_HIP_CHECK(hipMalloc(&cgb_ctr_, sizeof(unsigned) * 2 * (size_t)n_inst));
-            VGC_HIP_CHECK(hipMemsetAsync(cgb_ctr_, 0, sizeof(unsigned) * 2 * (size_t)n_inst,
-                                         stream_));
-            cgb_ctr_cap_ = n_inst;
-        }
-        return cgb_ctr_;
     }
     // ---------------------------------------------------------------- MLP
     void mlp_forward(const float* P, const float* X, int n, float* out) {
@@ -1950,13 +1999,10 @@ private:
     const void* cg_key_[13] = {};
     double* cg_part_ = nullptr;
     long long cg_part_cap_ = 0;
-    unsigned* cg_ctr_ = nullptr;
     hipGraphExec_t cgb_exec_ = nullptr;
     const void* cgb_key_[22] = {};
-    unsigned* cgb_ctr_ = nullptr;
-    int cgb_ctr_cap_ = 0;
     double* seg_part_ = nullptr;
-    int seg_part_cap_ = 0;
+    long long seg_part_cap_ = 0;
     int waves_ = kDefaultClipWaves;
 };
 

@@ -62,6 +62,12 @@ VGC_SD double ot_spmv_row(int i, const long long* __restrict__ ptr, const int* _
 // diagonal term is added.  The CPU emulation follows the same order.
 constexpr int kSpmvLanes = 16;
 
+// Scalars per instance of the batched PCG (device resident): [0] r.z,
+// [1] r.r, [2] p.q, [3] alpha, [4] beta, [5] tol^2 b.b, [6] stop (1
+// converged, 2 breakdown), [7] iterations, [8 + q] r.z of the direction of
+// the iterations of parity q, [10] beta is current.
+constexpr int kCgStride = 16;
+
 VGC_SD double ot_spmv_lane(int i, int l, const long long* __restrict__ ptr,
                            const int* __restrict__ col, const double* __restrict__ val,
                            const double* __restrict__ x) {

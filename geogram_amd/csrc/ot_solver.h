@@ -72,7 +72,7 @@ public:
         double* d_diag = (double*)slot(6, sizeof(double) * n);
         double* d_wt = (double*)slot(7, sizeof(double) * n);
         double* d_scal = (double*)slot(8, sizeof(double) * 8);
-        double* d_cg = (double*)slot(10, sizeof(double) * 8);
+        double* d_cg = (double*)slot(10, sizeof(double) * 16);
         EngineStats es;
 
         // 0: cells of w computed; 1: some cell empty or failed (the masses
@@ -272,7 +272,7 @@ public:
         double* d_diag = (double*)slot(6, sizeof(double) * n);
         double* d_wt = (double*)slot(7, sizeof(double) * n);
         double* d_ms = (double*)slot(23, sizeof(double) * 3 * (size_t)K);
-        double* d_cg = (double*)slot(24, sizeof(double) * 8 * (size_t)K);
+        double* d_cg = (double*)slot(24, sizeof(double) * kCgStride * (size_t)K);
         double* d_t = (double*)slot(25, sizeof(double) * (size_t)K);
         int* d_sel = (int*)slot(26, sizeof(int) * (size_t)K);
         EngineStats es;
@@ -389,12 +389,12 @@ public:
             }
             const MortonSystem M = to_morton(n_sub, nnz, d_val, d_diag, d_b, omap);
             be_.cgb_start(Ts, M.b, M.diag, M.x, d_r, d_z, d_p, d_cg, d_tol2);
-            std::vector<double> h(8 * (size_t)K);
-            be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
+            std::vector<double> h(kCgStride * (size_t)K);
+            be_.d2h(h.data(), d_cg, sizeof(double) * kCgStride * K);
             int issued = 0;
             auto all_done = [&]() {
                 for (int k = 0; k < K; ++k)
-                    if (h[8 * k + 6] == 0.0) return false;
+                    if (h[kCgStride * k + 6] == 0.0) return false;
                 return true;
             };
             while (!all_done() && issued < cg_max) {
@@ -402,14 +402,14 @@ public:
                 be_.cgb_iters(kk, Ts, n_sub, M.ptr, M.col, M.val, M.diag, M.x, d_r, d_z, d_p, d_q,
                               d_cg);
                 issued += kk;
-                be_.d2h(h.data(), d_cg, sizeof(double) * 8 * K);
+                be_.d2h(h.data(), d_cg, sizeof(double) * kCgStride * K);
             }
             if (n_sub < n) be_.dzero(d_x, sizeof(double) * n);
             be_.scatter_f64(n_sub, omap, M.x, d_x);
             const double tcg = ms_since(tg);
             for (int k = 0; k < K; ++k) {
                 st[k].t_cg_ms += tcg;
-                if (active[k]) st[k].cg_iters += (int)h[8 * k + 7];
+                if (active[k]) st[k].cg_iters += (int)h[kCgStride * k + 7];
             }
             // per-instance damped steps
             std::vector<int> pending(active), accepted(K, 0);
