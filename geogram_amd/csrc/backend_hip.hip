@@ -1132,13 +1132,45 @@ __global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__
                                                   const double* __restrict__ diag,
                                                   const double* __restrict__ p, double* q,
                                                   const double* __restrict__ cg) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = int(t / kSpmvLanes), l = int(t % kSpmvLanes);
-    const bool live = i < n && cg[kCgStride * inst_of[i < n ? i : 0] + 6] == 0.0;
-    double s = live ? ot_spmv_lane(i, l, ptr, col, val, p) : 0.0;
+    // as k_cg_spmv (grid-stride, four row groups per step), rows of stopped
+    // instances skipped
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * blockDim.x, N = (long long)n * kSpmvLanes;
+    for (long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t0 < N; t0 += U * stride) {
+        double s[U];
+        int row[U];
+        bool ok[U];
+        long long k[U], e[U];
 #pragma unroll
-    for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    if (live && l == 0) q[i] = s + diag[i] * p[i];
+        for (int u = 0; u < U; ++u) {
+            const long long t = t0 + u * stride;
+            row[u] = t < N ? int(t / kSpmvLanes) : 0;
+            ok[u] = t < N && cg[kCgStride * inst_of[row[u]] + 6] == 0.0;
+            k[u] = ptr[row[u]] + int(t % kSpmvLanes);
+            e[u] = ok[u] ? ptr[row[u] + 1] : 0;
+        }
+        double v[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = k[u] < e[u];
+            v[u] = in ? val[k[u]] : 0.0;
+            x[u] = in ? p[col[k[u]]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s[u] = 0.0;
+            if (k[u] < e[u]) s[u] += v[u] * x[u];
+            for (long long kk = k[u] + kSpmvLanes; kk < e[u]; kk += kSpmvLanes) s[u] += val[kk] * p[col[kk]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int m = kSpmvLanes / 2; m >= 1; m >>= 1) s[u] += __shfl_xor(s[u], m);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u] && (t0 % kSpmvLanes) == 0) q[row[u]] = s[u] + diag[row[u]] * p[row[u]];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_cgb_dot(SegTable T, const double* __restrict__ p,
@@ -1823,7 +1855,8 @@ struct HipBackend {
                     double* p, double* q, double* cg, double* part) {
         for (int it = 0; it < k; ++it) {
             k_cgb_dir<<<T.nblk, 256, 0, stream_>>>(T, z, p, cg, part, it & 1);
-            k_cgb_spmv<<<blocks_for((long long)n * kSpmvLanes, 256), 256, 0, stream_>>>(
+            const unsigned gs = blocks_for((long long)n * kSpmvLanes, 256);
+            k_cgb_spmv<<<gs < kSpmvBlocks ? gs : kSpmvBlocks, 256, 0, stream_>>>(
                 n, T.inst_of, ptr, col, val, diag, p, q, cg);
             k_cgb_dot<<<T.nblk, 256, 0, stream_>>>(T, p, q, cg, part);
             k_cgb_update<<<T.nblk, 256, 0, stream_>>>(T, p, q, diag, x, r, z, cg, part, it & 1);
