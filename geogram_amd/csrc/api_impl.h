@@ -155,6 +155,7 @@ int api_cells(Ctx<B>* ctx, const double* xyz, const double* w, const uint32_t* i
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         if (ctx->eng.cells(pb, es)) {
             ctx->err = "vgc_cells: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -237,6 +238,7 @@ int api_cvt_func_grad(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* 
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         Engine<B>& E = ctx->eng;
         if (E.cells(pb, es)) {
             ctx->err = "vgc_cvt_func_grad: " + E.error;
@@ -431,6 +433,7 @@ int api_cell_mesh(Ctx<B>* ctx, const char* name, const double* xyz, const double
         ctx->eng.autotune = false;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         const bool want_mesh = vertices && facet_ptr && facet_vertices && facet_nbr;
         long long capv[4] = {0, 0, 0, 0};
         if (cap) {
@@ -573,6 +576,7 @@ int api_cvt_newton(Ctx<B>* ctx, double* xyz, uint32_t n, const double* box, uint
         E.autotune = (flags & VGC_AUTOTUNE) != 0;
         E.waves4 = (flags & VGC_WAVES4) != 0;
         E.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        E.begin_call();
         double* d_f = (double*)ctx->staging(2, sizeof(double));
         uint64_t failed = 0;
         // energy and gradient: the caller's funcgrad of CVT.cpp:324-339
@@ -674,6 +678,7 @@ int api_lloyd_batch(Ctx<B>* ctx, double* xyz, const uint32_t* inst_n, uint32_t n
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         if (ctx->eng.lloyd(pb, dx, (int)iters, es)) {
             ctx->err = "vgc_lloyd: " + ctx->eng.error;
             return VGC_ERR_INTERNAL;
@@ -753,6 +758,7 @@ int api_ot_solve(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, ui
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         if (ctx->ot.solve(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                           (flags & VGC_OT_JUDGE_START) != 0, (flags & VGC_OT_VERBOSE) != 0,
                           (flags & VGC_OT_INEXACT) != 0, (flags & VGC_OT_REPAIR) != 0)) {
@@ -871,6 +877,7 @@ int api_ot_solve_batch(Ctx<B>* ctx, const double* xyz, const double* nu, double*
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         if (ctx->ot.solve_batch(pb, dnu, dw, tol, (int)maxn, 1e-7, 4000, os,
                                 (flags & VGC_OT_VERBOSE) != 0, (flags & VGC_OT_INEXACT) != 0)) {
             ctx->err = "vgc_ot_solve_batch: " + ctx->eng.error;
@@ -1161,6 +1168,7 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
         ctx->eng.autotune = (flags & VGC_AUTOTUNE) != 0;
         ctx->eng.waves4 = (flags & VGC_WAVES4) != 0;
         ctx->eng.fp64_first = (flags & VGC_FP64_FIRST) != 0;
+        ctx->eng.begin_call();
         bool all_conv = true;
         std::vector<double> hl;
         for (uint32_t k = 0; k < n_levels; ++k) {

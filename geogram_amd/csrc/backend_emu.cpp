@@ -212,6 +212,7 @@ struct EmuBackend {
         }
     }
 
+    void set_tree_leaves(int) {}
     void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
                       int ni, const float* rho2, const PowerFilter* filt, int* cnt) {
 #pragma omp parallel for schedule(dynamic, 16)

@@ -178,25 +178,129 @@ __global__ void k_radius_r2(const int* __restrict__ items, int ni, const pt4* __
     if (filt) filt[t] = filter_from_ball(ball[c], pts[c], box + 6 * inst);
 }
 
+// Radius lists, one lane per query.  A walk over kRadiusBudget nodes (the
+// ball of a cell with a very large weight) stops and flags its query
+// (wide[t]) for k_radius_wide.
+constexpr int kRadiusBudget = 512;
+static_assert(kBucket == 16, "k_radius_wide takes four leaves of 16 seeds per step");
 __global__ void k_radius_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
                                InstLayout IL, const int* __restrict__ items, int ni,
                                const float* __restrict__ rho2, const PowerFilter* __restrict__ filt,
-                               int* cnt) {
+                               int* cnt, unsigned char* wide) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
-    cnt[t] = radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr, nullptr,
-                          nullptr);
+    const int c = radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
+                               nullptr, nullptr, kRadiusBudget);
+    wide[t] = c < 0 ? 1 : 0;
+    cnt[t] = c < 0 ? 0 : c;
 }
 
 __global__ void k_radius_fill(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
                               InstLayout IL, const int* __restrict__ items, int ni,
                               const float* __restrict__ rho2, const PowerFilter* __restrict__ filt,
                               const long long* __restrict__ off, long long base, int* list,
-                              float* d2) {
+                              float* d2, unsigned char* wide) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
-    radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
-                 list + (off[t] - base), d2 + (off[t] - base));
+    const int c = radius_query(pf, nodes, IL, items[t], rho2[t], filt ? filt + t : nullptr,
+                               list + (off[t] - base), d2 + (off[t] - base), kRadiusBudget);
+    wide[t] = c < 0 ? 1 : 0;
+}
+
+// The flagged radius queries, one wavefront each (a grid of waves strides
+// over the queries): the tree level by level, 64 nodes per step, each
+// level's passing nodes kept in order in the wave's scratch (two buffers of
+// fstride >= L nodes), then the seeds of the passing leaves, four leaves
+// (64 seeds) per step.  A node is expanded exactly when the depth-first
+// walk of radius_query_pt expands it, by the same tests, and the seeds come
+// out in its order (leaves in order, seeds by Morton position): count mode
+// (list == nullptr) gives its count, fill mode its list, entry for entry.
+__global__ __launch_bounds__(64) void k_radius_wide(const pf4* __restrict__ pf,
+                                                    const bnode* __restrict__ nodes, InstLayout IL,
+                                                    const int* __restrict__ items, int ni,
+                                                    const float* __restrict__ rho2,
+                                                    const PowerFilter* __restrict__ filt,
+                                                    const unsigned char* __restrict__ wide,
+                                                    int* front, long long fstride, int* cnt,
+                                                    const long long* __restrict__ off,
+                                                    long long base, int* list, float* d2) {
+    const int lane = threadIdx.x;
+    int* fa = front + (long long)blockIdx.x * 2 * fstride;
+    int* fb = fa + fstride;
+    for (int t = blockIdx.x; t < ni; t += gridDim.x) {
+        if (!wide[t]) continue;  // uniform over the wave
+        const int q = items[t];
+        const int inst = IL.inst_of ? IL.inst_of[q] : 0;
+        const pf4 pq = pf[q];
+        const float r2 = rho2[t];
+        const PowerFilter* F = filt ? filt + t : nullptr;
+        const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
+        const bnode* T = nodes + IL.t0[inst];
+        int* out = list ? list + (off[t] - base) : nullptr;
+        float* od = list ? d2 + (off[t] - base) : nullptr;
+        int* A = fa;
+        int* B = fb;
+        if (lane == 0) A[0] = 1;
+        __syncthreads();
+        int na = 1, c = 0;
+        for (int first = 1; na > 0; first *= 2) {
+            if (first < L) {
+                // internal level: the passing nodes' children, in order
+                int nn = 0;
+                for (int i0 = 0; i0 < na; i0 += 64) {
+                    const int i = i0 + lane;
+                    const int node = i < na ? A[i] : 0;
+                    const bool pass = i < na && radius_node_in(T[node], pq.x, pq.y, pq.z, r2, F);
+                    const uint64_t m = __ballot(pass);
+                    const int r = nn + 2 * mbcnt(m);
+                    if (pass) {
+                        B[r] = 2 * node;
+                        B[r + 1] = 2 * node + 1;
+                    }
+                    nn += 2 * __popcll(m);
+                }
+                __syncthreads();
+                int* tmp = A;
+                A = B;
+                B = tmp;
+                na = nn;
+                continue;
+            }
+            // leaf level
+            for (int i0 = 0; i0 < na; i0 += 64) {
+                const int i = i0 + lane;
+                const int node = i < na ? A[i] : 0;
+                const bool pass =
+                    i < na && radius_node_in(T[node], pq.x, pq.y, pq.z, r2, F) && node - L < nb;
+                uint64_t lm = __ballot(pass);
+                while (lm) {
+                    // the next four passing leaves, 16 lanes each
+                    int sel = -1;
+                    for (int g = 0; g < 4 && lm; ++g) {
+                        const int bit = __builtin_ctzll(lm);
+                        lm &= lm - 1;
+                        if ((lane >> 4) == g) sel = bit;
+                    }
+                    const int leaf0 = __shfl(node, sel >= 0 ? sel : 0);  // every lane takes part
+                    const int leaf = sel >= 0 ? leaf0 : 0;
+                    const int mseed = p0 + (leaf - L) * kBucket + (lane & (kBucket - 1));
+                    const bool ok = sel >= 0 && mseed < p0 + n && mseed != q;
+                    float d = 0.0f;
+                    const bool take = ok && radius_seed_take(pf[mseed], pq.x, pq.y, pq.z, r2, F, d);
+                    const uint64_t tm = __ballot(take);
+                    if (out && take) {
+                        const int r = c + mbcnt(tm);
+                        out[r] = mseed;
+                        od[r] = d;
+                    }
+                    c += __popcll(tm);
+                }
+            }
+            break;
+        }
+        if (!list && lane == 0) cnt[t] = c;
+        __syncthreads();  // the scratch is reused by the wave's next query
+    }
 }
 
 __global__ void k_rebase(const long long* __restrict__ off, int n, long long base, long long* out) {
@@ -1354,6 +1458,8 @@ struct HipBackend {
         if (cg_exec_) (void)hipGraphExecDestroy(cg_exec_);
         if (cgb_exec_) (void)hipGraphExecDestroy(cgb_exec_);
         if (seg_part_) (void)hipFree(seg_part_);
+        if (wide_) (void)hipFree(wide_);
+        if (wide_front_) (void)hipFree(wide_front_);
         if (enter_ev_) (void)hipEventDestroy(enter_ev_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -1532,18 +1638,52 @@ struct HipBackend {
                                                               weighted, ball, box, rho2, filt);
         check();
     }
+    // largest leaf count of the instances' trees (scratch of k_radius_wide)
+    void set_tree_leaves(int lmax) { tree_leaves_ = lmax; }
     void radius_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
                       int ni, const float* rho2, const PowerFilter* filt, int* cnt) {
+        if (ni <= 0) return;
+        unsigned char* wide = wide_flags(ni);
         k_radius_count<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, filt,
-                                                               cnt);
+                                                               cnt, wide);
+        radius_wide(pf, nodes, IL, items, ni, rho2, filt, wide, cnt, nullptr, 0, nullptr, nullptr);
         check();
     }
     void radius_fill(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
                      int ni, const float* rho2, const PowerFilter* filt, const long long* off,
                      long long base, int* list, float* d2) {
+        if (ni <= 0) return;
+        unsigned char* wide = wide_flags(ni);
         k_radius_fill<<<blocks_for(ni, 64), 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, filt,
-                                                              off, base, list, d2);
+                                                              off, base, list, d2, wide);
+        radius_wide(pf, nodes, IL, items, ni, rho2, filt, wide, nullptr, off, base, list, d2);
         check();
+    }
+    // the flagged queries by k_radius_wide: up to 64 waves, each with two
+    // frontier buffers of tree_leaves_ nodes (at most 256 MB in all)
+    void radius_wide(const pf4* pf, const bnode* nodes, const InstLayout& IL, const int* items,
+                     int ni, const float* rho2, const PowerFilter* filt, const unsigned char* wide,
+                     int* cnt, const long long* off, long long base, int* list, float* d2) {
+        const long long fstride = std::max(1, tree_leaves_);
+        long long g = (256ll << 20) / (2 * sizeof(int) * fstride);
+        g = std::max(1ll, std::min(g, std::min(64ll, (long long)ni)));
+        const long long need = g * 2 * fstride;
+        if (wide_front_cap_ < need) {
+            if (wide_front_) VGC_HIP_CHECK(hipFree(wide_front_));
+            VGC_HIP_CHECK(hipMalloc(&wide_front_, sizeof(int) * (size_t)need));
+            wide_front_cap_ = need;
+        }
+        k_radius_wide<<<(unsigned)g, 64, 0, stream_>>>(pf, nodes, IL, items, ni, rho2, filt, wide,
+                                                       wide_front_, fstride, cnt, off, base, list,
+                                                       d2);
+    }
+    unsigned char* wide_flags(int ni) {
+        if (wide_cap_ < ni) {
+            if (wide_) VGC_HIP_CHECK(hipFree(wide_));
+            VGC_HIP_CHECK(hipMalloc(&wide_, (size_t)ni));
+            wide_cap_ = ni;
+        }
+        return wide_;
     }
     void rebase(const long long* off, int n, long long base, long long* out) {
         k_rebase<<<blocks_for(n, 256), 256, 0, stream_>>>(off, n, base, out);
@@ -2036,6 +2176,11 @@ private:
     const void* cgb_key_[22] = {};
     double* seg_part_ = nullptr;
     long long seg_part_cap_ = 0;
+    int tree_leaves_ = 1;
+    unsigned char* wide_ = nullptr;
+    int wide_cap_ = 0;
+    int* wide_front_ = nullptr;
+    long long wide_front_cap_ = 0;
     int waves_ = kDefaultClipWaves;
 };
 

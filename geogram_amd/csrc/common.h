@@ -453,28 +453,49 @@ VGC_SD double ot_repair_row(int i, const double* xyz, const double* w, const dou
 }
 
 // ------------------------------------------------------ radius queries
+// Does the subtree of box B hold seeds within r2 of p (FP32 squared
+// distance) that may pass the power-distance filter F?
+VGC_SD bool radius_node_in(const bnode& B, float px, float py, float pz, float r2,
+                           const PowerFilter* F) {
+    bool inside = box_dist2(B, px, py, pz) <= r2;
+    if (inside && F && F->on) {
+        const float fx = fmaxf(px - B.lo[0], B.hi[0] - px);
+        const float fy = fmaxf(py - B.lo[1], B.hi[1] - py);
+        const float fz = fmaxf(pz - B.lo[2], B.hi[2] - pz);
+        const float dfar = sqrtf(fx * fx + fy * fy + fz * fz);
+        inside = pf_node_may_cut(*F, box_dist2(B, F->cx, F->cy, F->cz), B.wmax,
+                                 sqrtf(box_dist2(B, px, py, pz)), dfar, fx, fy, fz);
+    }
+    return inside;
+}
+
+// Is seed a within r2 of p, and does it pass F?  d: its squared distance.
+VGC_SD bool radius_seed_take(const pf4& a, float px, float py, float pz, float r2,
+                             const PowerFilter* F, float& d) {
+    d = dist2f(a, px, py, pz);
+    bool take = d <= r2;
+    if (take && F && F->on)
+        take = pf_may_cut(*F, dist2f(a, F->cx, F->cy, F->cz), a.w, sqrtf(d), fabsf(a.x - px),
+                          fabsf(a.y - py), fabsf(a.z - pz));
+    return take;
+}
+
 // Seeds (other than q) with FP32 squared distance <= r2 and, with a filter
-// F, that pass the power-distance test (types.h), in traversal order.  Count
-// mode when out == nullptr.
+// F, that pass the power-distance test (types.h), in traversal order
+// (depth first, children in order: increasing Morton position).  Count
+// mode when out == nullptr.  With budget > 0, a walk that visits more nodes
+// returns -1 (the caller hands the query to a wider walk).
 VGC_SD int radius_query_pt(const pf4* pf, const bnode* nodes, const InstLayout& IL, int inst,
                            float px, float py, float pz, int self, float r2, const PowerFilter* F,
-                           int* out, float* out_d2) {
+                           int* out, float* out_d2, int budget = 0) {
     const int p0 = IL.p0[inst], n = IL.n[inst], nb = IL.nb[inst], L = IL.L[inst];
     const bnode* T = nodes + IL.t0[inst];
-    const bool filt = F && F->on;
     int cnt = 0;
     int node = 1;
+    int visits = 0;
     while (true) {
-        const bnode B = T[node];
-        bool inside = box_dist2(B, px, py, pz) <= r2;
-        if (inside && filt) {
-            const float fx = fmaxf(px - B.lo[0], B.hi[0] - px);
-            const float fy = fmaxf(py - B.lo[1], B.hi[1] - py);
-            const float fz = fmaxf(pz - B.lo[2], B.hi[2] - pz);
-            const float dfar = sqrtf(fx * fx + fy * fy + fz * fz);
-            inside = pf_node_may_cut(*F, box_dist2(B, F->cx, F->cy, F->cz), B.wmax,
-                                     sqrtf(box_dist2(B, px, py, pz)), dfar, fx, fy, fz);
-        }
+        if (budget > 0 && ++visits > budget) return -1;
+        const bool inside = radius_node_in(T[node], px, py, pz, r2, F);
         if (inside && node < L) {
             node = 2 * node;
             continue;
@@ -485,15 +506,16 @@ VGC_SD int radius_query_pt(const pf4* pf, const bnode* nodes, const InstLayout& 
                 int s = p0 + b * kBucket;
                 int e = s + kBucket;
                 if (e > p0 + n) e = p0 + n;
-                for (int m = s; m < e; ++m) {
-                    if (m == self) continue;
-                    const pf4 a = pf[m];
-                    float d = dist2f(a, px, py, pz);
-                    bool take = d <= r2;
-                    if (take && filt)
-                        take = pf_may_cut(*F, dist2f(a, F->cx, F->cy, F->cz), a.w, sqrtf(d),
-                                          fabsf(a.x - px), fabsf(a.y - py), fabsf(a.z - pz));
-                    if (take) {
+                // the bucket's seeds loaded together, then tested in order
+                pf4 a[kBucket];
+#pragma unroll
+                for (int k = 0; k < kBucket; ++k) a[k] = s + k < e ? pf[s + k] : pf4{};
+#pragma unroll
+                for (int k = 0; k < kBucket; ++k) {
+                    const int m = s + k;
+                    if (m >= e || m == self) continue;
+                    float d;
+                    if (radius_seed_take(a[k], px, py, pz, r2, F, d)) {
                         if (out) {
                             out[cnt] = m;
                             out_d2[cnt] = d;
@@ -511,10 +533,10 @@ VGC_SD int radius_query_pt(const pf4* pf, const bnode* nodes, const InstLayout& 
 }
 
 VGC_SD int radius_query(const pf4* pf, const bnode* nodes, const InstLayout& IL, int q,
-                        float r2, const PowerFilter* F, int* out, float* out_d2) {
+                        float r2, const PowerFilter* F, int* out, float* out_d2, int budget = 0) {
     const int inst = IL.inst_of ? IL.inst_of[q] : 0;
     const pf4 pq = pf[q];
-    return radius_query_pt(pf, nodes, IL, inst, pq.x, pq.y, pq.z, q, r2, F, out, out_d2);
+    return radius_query_pt(pf, nodes, IL, inst, pq.x, pq.y, pq.z, q, r2, F, out, out_d2, budget);
 }
 
 // ---------------------------------------------------------- tree build

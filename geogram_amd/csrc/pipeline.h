@@ -128,6 +128,9 @@ public:
     bool autotune = false;             // pick the first pass's work-group size by timing
     bool waves4 = false;               // first pass with 4-wavefront work-groups (no tuner)
     bool fp64_first = false;           // first pass with FP64 corner state (default FP32)
+    // start of a library call: the FP32 / FP64 choice of the first pass
+    // follows the passes of this call only
+    void begin_call() { fp32_skip_ = 0; }
     std::vector<int> inst_bad;         // per instance: failed or empty cells of the last pass
     std::vector<uint32_t> h_flags;  // by Morton position, current for census_ cells
     std::vector<int> census_;       // cells of the last pass that are not plainly done
@@ -196,6 +199,7 @@ public:
         be_.h2d(d_b0, h_b0_.data(), sizeof(int) * ni_);
         be_.h2d(d_nb, h_nb_.data(), sizeof(int) * ni_);
         be_.h2d(d_L, h_L_.data(), sizeof(int) * ni_);
+        be_.set_tree_leaves(*std::max_element(h_L_.begin(), h_L_.begin() + ni_));
         be_.h2d(d_t0, h_t0_.data(), sizeof(long long) * ni_);
         be_.h2d(d_leaf_off, h_leaf_off_.data(), sizeof(long long) * (ni_ + 1));
         be_.h2d(d_box, pb.box.data(), sizeof(double) * 6 * ni_);
@@ -553,8 +557,12 @@ public:
         P.ring_limit = weighted_ ? kMaxFilteredRings : kMaxVoronoiRings;
         P.corner = 0;
         // FP32 corner state for the first pass, except for cell meshes (those
-        // are emitted by the FP64 pass)
-        P.fp32 = (gather && !fp64_first && !pb.mesh_mode) ? 1 : 0;
+        // are emitted by the FP64 pass) and for a few passes after one that
+        // had to redo more than 5 % of its cells in FP64 (large Laguerre
+        // weights: there the FP64 pass alone is faster; the outputs are the
+        // same either way)
+        P.fp32 = (gather && !fp64_first && !pb.mesh_mode && fp32_skip_ == 0) ? 1 : 0;
+        if (fp32_skip_ > 0) --fp32_skip_;
         if (pb.abort_on_empty) {
             // the caller only needs to know whether an instance has an empty
             // cell: its first empty cell stops it
@@ -646,6 +654,7 @@ public:
                 if (cf[i] & CELL_UNCERTAIN) redo.push_back(cc[i]);
             std::sort(redo.begin(), redo.end());
             st.nb_fp64_redo += redo.size();
+            if (redo.size() * 20 > (size_t)P.n_items) fp32_skip_ = kFp32Skip;
             if (!redo.empty()) {
                 int* d_items = (int*)slot(S_ITEMS, sizeof(int) * redo.size());
                 be_.h2d(d_items, redo.data(), sizeof(int) * redo.size());
@@ -1225,6 +1234,10 @@ private:
     double* d_wmax_ = nullptr;
     double* d_r2_ = nullptr;  // squared radius per cell of the last pass (Morton order)
     std::vector<int> h_p0_, h_n_, h_b0_, h_nb_, h_L_;
+    // first passes left to run with FP64 corners (after an FP32 pass that
+    // redid more than 5 % of its cells)
+    static constexpr int kFp32Skip = 8;
+    int fp32_skip_ = 0;
     std::vector<int> piece_cells_;  // cells of this pass computed in pieces
     long long x_total_ = 0;         // their merged neighbour entries
     static constexpr int kPieceLevels = 4;
