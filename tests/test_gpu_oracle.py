@@ -66,6 +66,26 @@ def test_laguerre_512_corner_tier_vs_oracle(gpu_ctx):
         assert set(r.neighbors(i).tolist()) == set(oc["nbr"][t, : oc["cnt"][t]].tolist())
 
 
+def test_wide_radius_walks_equal_lane_walks(gpu_ctx, emu_ctx):
+    # heavy seeds whose list balls cover most of the domain: their radius
+    # walks exceed the lane walk's node budget and run wave-wide on the GPU;
+    # the emulation walks them lane by lane.  Same lists, so the same bits,
+    # and the oracle agrees.
+    x = ga.sample_points(0, 43, 12000)
+    w = np.zeros(len(x))
+    w[:3] = [0.03, 0.01, 0.008]  # seed 0: 343 facets, in octant pieces
+    x[:3] = [[0.3, 0.3, 0.3], [0.7, 0.6, 0.5], [0.5, 0.7, 0.2]]
+    r = gpu_ctx.cells(x, weights=w)
+    e = emu_ctx.cells(x, weights=w)
+    _same_cells(r, e)
+    # lists of about 11k of the 12k seeds: walks of over 700 leaves
+    assert r.stats["radius_entries"] == e.stats["radius_entries"] > 8000
+    assert r.stats["nb_failed"] == 0 and r.stats["nb_pieces"] == 1
+    sub = np.r_[0, 1, 2, r.neighbors(0)[:20]].astype(np.int32)
+    oc = oracle.cells(x, w=w, k0=20, subset=sub, centred=True, stride=512)
+    assert (np.abs(r.vol[sub] - oc["vol"]) / oc["vol"]).max() < 1e-10
+
+
 def test_laguerre_empty_cells_vs_oracle(gpu_ctx):
     rng = np.random.default_rng(4)
     n = 800
