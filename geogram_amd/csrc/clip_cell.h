@@ -1226,6 +1226,23 @@ VGC_HD void stage_cand(const ClipParams& P, CellLds<S>& L, int base, int cnt) {
     wave_sync();
 }
 
+// List mode: entries lbeg+base .. +63 of the cell's list (those below len),
+// their ids into L.cand and FP64 positions into L.stage, one load per lane,
+// so that the clipping loop does not wait on two dependent loads per
+// candidate.  An id of -1 (end of a short list) is kept as -1.
+template <int S>
+VGC_HD void stage_list(const ClipParams& P, CellLds<S>& L, long long lbeg, int base, int len) {
+    const LI k = base + lane_id();
+    const LB ok = k < len;
+    const LI j = gld(P.list + lbeg, select(ok, k, LI(0)), ok);
+    const LB okj = ok & (j >= 0);
+    const Lane<pt4> q = gld(P.pts, select(okj, j, LI(0)), okj);
+    wave_sync();  // the previous batch is consumed
+    lds_st(L.cand, lane_id(), cvt<uint64_t>(cvt<uint32_t>(select(ok, j, LI(-1)))), LB(true));
+    lds_st(L.stage, lane_id(), q, okj);
+    wave_sync();
+}
+
 // Successor of corner t around face f (corner t has plane f): the corner
 // across t's edge (previous plane, f); counter-clockwise seen from outside,
 // since each corner's planes a, b, c have det(ea, eb, ec) > 0.
@@ -1523,12 +1540,13 @@ VGC_HD void process_cell(const ClipParams& P, int item, CellLds<S>& L) {
         }
         complete = P.list_complete != 0 || llen >= P.inst_n[inst] - 1;
         for (int jj = 0; jj < llen; ++jj) {
-            const int j = P.list[lbeg + jj];
+            if ((jj & (kWave - 1)) == 0) stage_list<S>(P, L, lbeg, jj, llen);
+            const int j = int(uint32_t(L.cand[jj & (kWave - 1)]));
             if (j < 0) {
                 complete = true;  // short list: the instance has fewer seeds
                 break;
             }
-            if (consider<S, EXACT, W, T>(P, L, R, j, P.pts[j])) break;
+            if (consider<S, EXACT, W, T>(P, L, R, j, L.stage[jj & (kWave - 1)])) break;
         }
     } else {
         const pf4 pq = P.pf[c];
