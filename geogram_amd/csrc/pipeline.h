@@ -401,7 +401,10 @@ public:
     // faces join as candidates, the cells are clipped in the box grown by
     // delta, and a cell counts only if everything that could still cut it
     // (the radius of its security test) lies within the margin; otherwise
-    // the margin doubles.  Outputs as cells(): the first n entries; neighbour
+    // the pass is redone with 1.1 times the largest such radius.  The first
+    // margin, 8 mean spacings, covers the security radius of Voronoi cells
+    // of spread-out seeds (about 2 cell radii), so one pass is the common
+    // case.  Outputs as cells(): the first n entries; neighbour
     // ids are seeds of the period (an image's seed).
     int cells_periodic(const Problem& pb, EngineStats& st) {
         if (pb.n_inst != 1 || pb.mesh_mode) {
@@ -412,7 +415,7 @@ public:
         const double* b = pb.box.data();
         const double Lx = b[3] - b[0], Ly = b[4] - b[1], Lz = b[5] - b[2];
         const double Lmax = std::max(Lx, std::max(Ly, Lz));
-        double delta = std::min(Lmax, 3.0 * std::cbrt(Lx * Ly * Lz / double(n)));
+        double delta = std::min(Lmax, 8.0 * std::cbrt(Lx * Ly * Lz / double(n)));
         for (int round = 0;; ++round) {
             // delta = Lmax keeps all 26 images of every seed.  That set is
             // enough for any cell: a cell lies within half a period of its
@@ -458,7 +461,8 @@ public:
             float reach2 = 0.0f;
             be_.d2h(&reach2, d_reach, sizeof(float));
             if (full || double(reach2) < delta * delta) break;
-            delta = std::min(Lmax, 2.0 * std::max(delta, std::sqrt(double(reach2))));
+            // reach2 >= delta^2 here, so the margin grows
+            delta = std::min(Lmax, 1.1 * std::sqrt(double(reach2)));
             st.nb_periodic_retries++;
         }
         return 0;
