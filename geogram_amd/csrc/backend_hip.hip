@@ -444,13 +444,14 @@ __global__ void k_lower_bound_u64(const uint64_t* __restrict__ keys, int n,
     out[t] = lo < n ? lo : n - 1;
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void k_knn_seed_radius(const pf4* __restrict__ pf,
                                                          const bnode* __restrict__ nodes,
                                                          InstLayout IL, const pf4* __restrict__ qf,
                                                          const int* __restrict__ qpos, int nq,
                                                          int kk, double e, float* rr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nq) rr[t] = knn_seed_radius_row(pf, nodes, IL, qf, qpos, t, kk, e);
+    if (t < nq) rr[t] = knn_seed_radius_row<W>(pf, nodes, IL, qf, qpos, t, kk, e);
 }
 
 __global__ void k_knn_count(const pf4* __restrict__ pf, const bnode* __restrict__ nodes,
@@ -2104,8 +2105,15 @@ struct HipBackend {
     }
     void knn_seed_radius(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,
                          const int* qpos, int nq, int kk, double e, float* rr) {
-        k_knn_seed_radius<<<blocks_for(nq, 256), 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk,
-                                                                    e, rr);
+        // the narrowest walk that holds kk (same radius as the 16-wide one)
+        const unsigned g = blocks_for(nq, 256);
+        if (kk == 1)
+            k_knn_seed_radius<1><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr);
+        else if (kk <= 4)
+            k_knn_seed_radius<4><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e, rr);
+        else
+            k_knn_seed_radius<kKnnSeedK><<<g, 256, 0, stream_>>>(pf, nodes, IL, qf, qpos, nq, kk, e,
+                                                                 rr);
         check();
     }
     void knn_count(const pf4* pf, const bnode* nodes, const InstLayout& IL, const pf4* qf,

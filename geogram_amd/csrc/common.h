@@ -72,9 +72,15 @@ VGC_SD void knn_scan_bucket(const pf4* pf, int p0, int n, int b, int self, float
     int s = p0 + b * kBucket;
     int e = s + kBucket;
     if (e > p0 + n) e = p0 + n;
-    for (int m = s; m < e; ++m) {
-        if (m == self) continue;
-        float d = dist2f(pf[m], px, py, pz);
+    // the bucket's points loaded together, then taken in order
+    pf4 a[kBucket];
+#pragma unroll
+    for (int k = 0; k < kBucket; ++k) a[k] = s + k < e ? pf[s + k] : pf4{};
+#pragma unroll
+    for (int k = 0; k < kBucket; ++k) {
+        const int m = s + k;
+        if (m >= e || m == self) continue;
+        float d = dist2f(a[k], px, py, pz);
         if (knn_less<K>(d, m, bd[K - 1], bi[K - 1])) knn_insert<K>(bd, bi, d, m);
     }
 }
@@ -179,15 +185,18 @@ VGC_SD double knn_outside_bound(float r32, double e) {
 // kk-th FP32 squared distance of the 16 nearest points found by the tree
 // walk, or beyond 16 the 16th scaled by twice the volume ratio (the factor
 // two covers points spread on a surface, whose count grows as r^2).
+// A walk that keeps the W <= 16 nearest (kk <= W) finds the same kk-th
+// distance as one that keeps 16: the GPU takes W = 1 or 4 when kk allows.
 constexpr int kKnnSeedK = 16;
+template <int W = kKnnSeedK>
 VGC_SD float knn_seed_radius_row(const pf4* pf, const bnode* nodes, const InstLayout& IL,
                                  const pf4* qf, const int* qpos, int t, int kk, double e) {
-    float bd[kKnnSeedK];
-    int bi[kKnnSeedK];
+    float bd[W];
+    int bi[W];
     const pf4 q = qf[t];
-    knn_query_pt<kKnnSeedK>(pf, nodes, IL, 0, qpos ? qpos[t] : t, q.x, q.y, q.z, -1, bd, bi);
-    float d = bd[(kk < kKnnSeedK ? kk : kKnnSeedK) - 1];
-    if (kk > kKnnSeedK) d *= 2.0f * powf(float(kk) / float(kKnnSeedK), 2.0f / 3.0f);
+    knn_query_pt<W>(pf, nodes, IL, 0, qpos ? qpos[t] : t, q.x, q.y, q.z, -1, bd, bi);
+    float d = bd[(kk < W ? kk : W) - 1];
+    if (kk > W) d *= 2.0f * powf(float(kk) / float(W), 2.0f / 3.0f);
     if (!(d < INFINITY)) return INFINITY;
     return knn_list_radius(d, e);
 }
