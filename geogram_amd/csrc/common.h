@@ -195,7 +195,12 @@ VGC_SD float knn_seed_radius_row(const pf4* pf, const bnode* nodes, const InstLa
     int bi[W];
     const pf4 q = qf[t];
     knn_query_pt<W>(pf, nodes, IL, 0, qpos ? qpos[t] : t, q.x, q.y, q.z, -1, bd, bi);
-    float d = bd[(kk < W ? kk : W) - 1];
+    // entry kk - 1 by constant indices (a variable index would put the
+    // arrays in scratch memory)
+    const int want = (kk < W ? kk : W) - 1;
+    float d = bd[0];
+#pragma unroll
+    for (int k = 1; k < W; ++k) d = k == want ? bd[k] : d;
     if (kk > W) d *= 2.0f * powf(float(kk) / float(W), 2.0f / 3.0f);
     if (!(d < INFINITY)) return INFINITY;
     return knn_list_radius(d, e);
