@@ -915,10 +915,10 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int n, const long long* __restr
                                                  double* cg) {
     if (cg[6] != 0.0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) cg[10] = 0.0;  // read by the next k_cg_dir only
-    // four row groups per step, their loads issued together (the first
+    // two row groups per step, their loads issued together (the first
     // kSpmvLanes entries of a row as straight-line code); each row's sum is
     // formed in the order of ot_spmv_lane
-    constexpr int U = 4;
+    constexpr int U = 2;  // measured: 66.6 us against 73.6 us at U = 4 (1M rows, 15.3M entries, 2048 blocks)
     const long long stride = (long long)gridDim.x * blockDim.x, N = (long long)n * kSpmvLanes;
     for (long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t0 < N; t0 += U * stride) {
         double s[U];
@@ -1237,8 +1237,10 @@ __global__ __launch_bounds__(256) void k_cgb_spmv(int n, const int* __restrict__
                                                   const double* __restrict__ diag,
                                                   const double* __restrict__ p, double* q,
                                                   const double* __restrict__ cg) {
-    // as k_cg_spmv (grid-stride, four row groups per step), rows of stopped
-    // instances skipped
+    // as k_cg_spmv (grid-stride), rows of stopped instances skipped; four
+    // row groups per step here (two: 5.92 s of CG in the 256-instance
+    // transport against 5.72 s; the stop-flag read lengthens each row's
+    // chain of dependent loads)
     constexpr int U = 4;
     const long long stride = (long long)gridDim.x * blockDim.x, N = (long long)n * kSpmvLanes;
     for (long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t0 < N; t0 += U * stride) {
