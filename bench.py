@@ -34,7 +34,11 @@ The JSON line also carries:
                 damped Newton, KMT step rule, CG tolerance from the mass error
                 once the steps are full, VGC_OT_INEXACT);
   ot_warm_start 1M 16-Gaussian seeds with log-normal masses: plain start
-                against the learned MLP start (features + forward timed);
+                against the learned MLP start (features + forward timed;
+                newton_cut, time_cut) and against the coarse-plan start
+                (every 128th seed solved to 1e-3, extended by a convex soft
+                maximum; coarse solve and start timed; coarse_newton_cut,
+                coarse_time_cut);
   batched       256 independent instances (N log-uniform in [2k, 200k]):
                 20 batched Lloyd steps and batched transport to 1e-4.
 """
@@ -285,6 +289,21 @@ def main():
             ws["time_cut"] = round(1.0 - tl / tp, 3)
         else:
             ws["learned"] = None
+        # start from a coarse plan (every 128th seed solved to 1e-3, the fine
+        # start its convex soft-max extension; coarse solve and start timed)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        w0c, cinfo = warmstart.coarse_start(ctx, x4d, nu4d)
+        _, sc = ctx.ot_solve(x4d, nu4d, w0=w0c, tol=1e-4, judge_start=True, verbose=True,
+                             inexact=True)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - t2
+        ws["coarse"] = {"time_s": round(tc, 3), "newton_iters": int(sc["newton_iters"]),
+                        "cell_passes": int(sc["cell_passes"]), "converged": bool(sc["converged"]),
+                        "err0": sc["err0"], "fell_back_to_zero": bool(sc["used_fallback_start"]),
+                        **cinfo}
+        ws["coarse_newton_cut"] = round(1.0 - sc["newton_iters"] / max(1, sp["newton_iters"]), 3)
+        ws["coarse_time_cut"] = round(1.0 - tc / tp, 3)
 
     # batched uneven instances: Lloyd (20 steps) and transport to 1e-4 for
     # every instance at once; even instances uniform, odd ones 16-Gaussian;

@@ -439,6 +439,26 @@ class Context:
         self._check(rc, allow_partial=False)
         return f
 
+    def ot_convex_start(self, points, pieces, s):
+        """Start weights w_i = |p_i|^2 - s log sum_j exp(piece_j(p_i)/s)
+        (vgc_ot_convex_start); pieces is (m, 9): a, g[3], mu, b[3], R."""
+        dev = _is_torch_cuda(points)
+        self._stream(dev)
+        if dev:
+            import torch
+            pts = points.contiguous().to(torch.float64).reshape(-1, 3)
+            pc = pieces.contiguous().to(device=pts.device, dtype=torch.float64).reshape(-1, 9)
+            w = torch.empty(pts.shape[0], dtype=torch.float64, device=pts.device)
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            pc = np.ascontiguousarray(pieces, dtype=np.float64).reshape(-1, 9)
+            w = np.empty(pts.shape[0])
+        rc = self.lib.fn("ot_convex_start")(self.ctx, _ptr(pts), pts.shape[0], _ptr(pc),
+                                            pc.shape[0], float(s), VGC_DEVICE_PTRS if dev else 0,
+                                            _ptr(w))
+        self._check(rc, allow_partial=False)
+        return w
+
     def mlp_forward(self, params, feats):
         dev = _is_torch_cuda(feats)
         self._stream(dev)
@@ -775,6 +795,7 @@ class OptimalTransportMap3d:
         self.weights = None
         self.last_stats = None
         self.learned_params = None
+        self.coarse_ratio = None
 
     def set_points(self, points, masses=None):
         """Seeds and their target masses (default: equal, summing to the box volume)."""
@@ -808,10 +829,21 @@ class OptimalTransportMap3d:
         from . import warmstart
         self.learned_params = warmstart.load_params() if params is None else params
 
+    def use_coarse_warm_start(self, ratio=128):
+        """Start Newton from a coarse transport plan (warmstart.coarse_start:
+        a convex start, no empty cell), judged against the zero start.
+        ratio: fine seeds per coarse seed; None = off."""
+        self.coarse_ratio = ratio
+
     def optimize(self, max_iter: int = 100):
         ctx = self.ctx or default_context()
         w0, judge = self.weights, False
-        if w0 is None and self.learned_params is not None:
+        if w0 is None and getattr(self, "coarse_ratio", None):
+            from . import warmstart
+            w0, _ = warmstart.coarse_start(ctx, self.points_, self.nu, self.box,
+                                           ratio=self.coarse_ratio)
+            judge = True
+        elif w0 is None and self.learned_params is not None:
             from . import warmstart
             w0 = warmstart.predict_weights(ctx, self.points_, self.nu, self.box,
                                            self.learned_params)

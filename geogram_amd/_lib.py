@@ -34,6 +34,7 @@ VGC_OT_REPAIR = 512
 VGC_FP64_FIRST = 1024
 
 VGC_MLP_PARAMS = 4673
+VGC_START_PIECE = 9
 VGC_MLP_FEATURES = 6
 
 _STAT_U64 = ("nb_cells nb_neighbors nb_pass_b nb_pass_c nb_pass_e nb_uncertain nb_failed nb_empty "
@@ -153,6 +154,9 @@ def _declare(L, prefix):
     f = L[prefix + "ot_features"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _vp, _u32, _vp]
+    f = L[prefix + "ot_convex_start"]
+    f.restype = ctypes.c_int
+    f.argtypes = [ctx_t, _vp, _u32, _vp, _u32, ctypes.c_double, _u32, _vp]
     f = L[prefix + "mlp_forward"]
     f.restype = ctypes.c_int
     f.argtypes = [ctx_t, _vp, _vp, _u32, _u32, _vp]
@@ -178,7 +182,7 @@ class _Lib:
         self._f = {}
         names = ["last_error", "knn", "knn_search", "cells", "cells_batch", "lloyd", "lloyd_batch",
                  "cvt_func_grad", "cvt_newton", "cell_mesh", "rdt", "surface_centroids", "ot_solve", "ot_solve_levels",
-                 "ot_solve_batch", "ot_features",
+                 "ot_solve_batch", "ot_features", "ot_convex_start",
                  "mlp_forward", "mlp_grad", "mlp_adam", "create", "destroy"]
         for nm in names:
             self._f[prefix + nm] = getattr(self.cdll, prefix + nm)

@@ -956,6 +956,39 @@ int api_ot_features(Ctx<B>* ctx, const double* xyz, const double* nu, uint32_t n
 }
 
 template <class B>
+int api_ot_convex_start(Ctx<B>* ctx, const double* xyz, uint32_t n, const double* pieces,
+                        uint32_t m, double s, uint32_t flags, double* w) {
+    if (!ctx) return VGC_ERR_ARG;
+    try {
+        if (!xyz || !pieces || !w || n == 0 || m == 0 || !(s > 0.0) || !std::isfinite(s) ||
+            n > (uint32_t)INT32_MAX || m > (uint32_t)INT32_MAX) {
+            ctx->err = "vgc_ot_convex_start: bad argument";
+            return VGC_ERR_ARG;
+        }
+        B& be = ctx->be;
+        const bool dev = (flags & VGC_DEVICE_PTRS) != 0;
+        const double *dx = xyz, *dp = pieces;
+        double* dw = w;
+        if (!dev) {
+            double* sx = (double*)ctx->staging(0, sizeof(double) * 3 * (size_t)n);
+            double* sp = (double*)ctx->staging(1, sizeof(double) * kStartPiece * (size_t)m);
+            dw = (double*)ctx->staging(2, sizeof(double) * (size_t)n);
+            be.h2d(sx, xyz, sizeof(double) * 3 * (size_t)n);
+            be.h2d(sp, pieces, sizeof(double) * kStartPiece * (size_t)m);
+            dx = sx;
+            dp = sp;
+        }
+        be.ot_convex_start((int)n, dx, dp, (int)m, s, dw);
+        if (!dev) be.d2h(w, dw, sizeof(double) * (size_t)n);
+        be.sync();
+        return VGC_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return VGC_ERR_DEVICE;
+    }
+}
+
+template <class B>
 int api_mlp_forward(Ctx<B>* ctx, const float* params, const float* feats, uint32_t n,
                     uint32_t flags, float* out) {
     if (!ctx) return VGC_ERR_ARG;

@@ -418,6 +418,11 @@ struct EmuBackend {
                     const double* area, double* val, double* diag, const double* period = nullptr) {
         for (int i = 0; i < n; ++i) ot_hessian_row(i, xyz, ptr, col, area, val, diag, period);
     }
+    void ot_convex_start(int n, const double* xyz, const double* piece, int m, double s,
+                         double* w) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) w[i] = ot_convex_start_row(xyz + 3 * (long long)i, piece, m, s);
+    }
     void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,
                  const double* x, double* y) {
         for (int i = 0; i < n; ++i) y[i] = ot_spmv_row(i, ptr, col, val, diag, x);
@@ -926,6 +931,10 @@ int vgc_emu_ot_solve_batch(void* ctx, const double* xyz, const double* nu, doubl
 int vgc_emu_ot_features(void* ctx, const double* xyz, const double* nu, uint32_t n,
                         const double* box, uint32_t flags, float* feats) {
     return vgc::api_ot_features((EmuCtx*)ctx, xyz, nu, n, box, flags, feats);
+}
+int vgc_emu_ot_convex_start(void* ctx, const double* xyz, uint32_t n, const double* pieces,
+                            uint32_t m, double s, uint32_t flags, double* w) {
+    return vgc::api_ot_convex_start((EmuCtx*)ctx, xyz, n, pieces, m, s, flags, w);
 }
 int vgc_emu_mlp_forward(void* ctx, const float* params, const float* feats, uint32_t n,
                         uint32_t flags, float* out) {

@@ -623,6 +623,15 @@ __global__ void k_ot_hessian(int n, const double* __restrict__ xyz, const long l
     if (i < n) ot_hessian_row(i, xyz, ptr, col, area, val, diag, period);
 }
 
+// one thread per seed; every lane reads the same piece at a time (one
+// broadcast load per piece and wave)
+__global__ __launch_bounds__(256) void k_ot_convex_start(int n, const double* __restrict__ xyz,
+                                                         const double* __restrict__ piece, int m,
+                                                         double s, double* w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = ot_convex_start_row(xyz + 3 * (long long)i, piece, m, s);
+}
+
 __global__ void k_ot_spmv(int n, const long long* __restrict__ ptr, const int* __restrict__ col,
                           const double* __restrict__ val, const double* __restrict__ diag,
                           const double* __restrict__ x, double* y) {
@@ -1821,6 +1830,11 @@ struct HipBackend {
                                                               period);
         check();
     }
+    void ot_convex_start(int n, const double* xyz, const double* piece, int m, double s,
+                         double* w) {
+        k_ot_convex_start<<<blocks_for(n, 256), 256, 0, stream_>>>(n, xyz, piece, m, s, w);
+        check();
+    }
     void ot_spmv(int n, const long long* ptr, const int* col, const double* val, const double* diag,
                  const double* x, double* y) {
         k_ot_spmv<<<blocks_for(n, 256), 256, 0, stream_>>>(n, ptr, col, val, diag, x, y);
@@ -2375,6 +2389,11 @@ int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t 
                     const double* box, uint32_t flags, float* feats) {
     if (int rc = vgc_enter(ctx)) return rc;
     return vgc::api_ot_features(ctx->impl, xyz, nu, n, box, flags, feats);
+}
+int vgc_ot_convex_start(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* pieces,
+                        uint32_t m, double s, uint32_t flags, double* w) {
+    if (int rc = vgc_enter(ctx)) return rc;
+    return vgc::api_ot_convex_start(ctx->impl, xyz, n, pieces, m, s, flags, w);
 }
 int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint32_t n,
                     uint32_t flags, float* out) {

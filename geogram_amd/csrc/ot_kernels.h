@@ -96,4 +96,43 @@ struct SegTable {
 };
 constexpr int kSegMaxBlocks = 64;  // blocks per instance
 
+// --------------------------------------------- convex start from a coarse plan
+// With phi_i = |p_i|^2 - w_i, the cell of seed i is {x : phi_i - 2 x.p_i
+// minimal}: it is non-empty exactly when (p_i, phi_i) is a vertex of the
+// lower convex hull of the lifted seeds.  Sampling a strictly convex function
+// phi therefore gives weights with no empty cell (cell i contains
+// grad phi(p_i) / 2).  The start is
+//     phi(p) = s log sum_j exp(piece_j(p) / s),
+//     piece_j(p) = a_j + g_j.p + mu_j H_Rj(|p - b_j|^2),
+//     H_R(r2) = r2 for r <= R, 2 R r - R^2 beyond (convex, linear far away),
+// a soft maximum of convex pieces, hence convex; each piece is the tangent
+// of a coarse transport plan's potential at a coarse seed plus a curvature
+// term that spreads the fine seeds near that coarse seed over its coarse
+// Laguerre cell.  kStartPiece doubles per piece: a, g[3], mu, b[3], R.
+// Terms more than 40 s below the running maximum are skipped: each would
+// add less than half an ulp to a sum that is at least 1, so the result is
+// the same.
+constexpr int kStartPiece = 9;
+
+VGC_SD double ot_convex_start_row(const double* __restrict__ p, const double* __restrict__ piece,
+                                  int m, double s) {
+    const double px = p[0], py = p[1], pz = p[2];
+    const double inv_s = 1.0 / s;
+    double mx = -1.0e300, sum = 0.0;
+    for (int j = 0; j < m; ++j) {
+        const double* c = piece + (long long)kStartPiece * j;
+        const double dx = px - c[5], dy = py - c[6], dz = pz - c[7];
+        const double r2 = dx * dx + dy * dy + dz * dz, R = c[8];
+        const double h = r2 <= R * R ? r2 : R * (2.0 * sqrt(r2) - R);
+        const double v = c[0] + c[1] * px + c[2] * py + c[3] * pz + c[4] * h;
+        if (v > mx) {
+            sum = sum * exp((mx - v) * inv_s) + 1.0;
+            mx = v;
+        } else if (v > mx - 40.0 * s) {
+            sum += exp((v - mx) * inv_s);
+        }
+    }
+    return px * px + py * py + pz * pz - (mx + s * log(sum));
+}
+
 }  // namespace vgc

@@ -94,6 +94,72 @@ def predict_weights(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), params=None):
     return ctx.mlp_forward(np.asarray(params, dtype=np.float32), feats).astype(np.float64)
 
 
+def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.0,
+                 coarse_tol=1e-3, min_coarse=64, seed=12345):
+    """Newton start from a coarse transport plan, with no empty cell.
+
+    A random subset of n // ratio seeds carries the masses of the fine seeds
+    nearest to it and is solved to coarse_tol.  Coarse seed j gives one convex
+    piece: the tangent of the coarse potential phi = |q|^2 - w at q_j, with
+    slope 2 c_j (c_j the centroid of its Laguerre cell), plus a curvature
+    mu_j = (laguerre volume / voronoi volume)^(1/3) about the centroid b_j of
+    its Voronoi cell among the coarse seeds, so that the fine seeds around q_j
+    spread over c_j's cell; the curvature is quadratic up to the Voronoi
+    cell's radius and linear beyond.  The fine start is the soft maximum of
+    all pieces (vgc_ot_convex_start), a convex function, so every fine cell
+    is non-empty.  s: s_factor x the median over coarse seeds of
+    4 x (Voronoi radius) x (Laguerre radius).  Returns (w, info)."""
+    dev = _is_torch_cuda(points)
+    n = int(points.shape[0])
+    lo = np.asarray(box[:3], dtype=np.float64)
+    hi = np.asarray(box[3:], dtype=np.float64)
+    m = max(min_coarse, n // ratio)
+    if m >= n:
+        raise ValueError("coarse_start: too few seeds for a coarse level")
+    sel = np.sort(np.random.default_rng(seed).choice(n, m, replace=False))
+    if dev:
+        import torch
+        q_dev = points[torch.from_numpy(sel).to(points.device)].contiguous()
+        nn, _ = ctx.knn_search(q_dev, points, 1)
+        nn = nn[:, 0].cpu().numpy()
+        nu_h = nu.cpu().numpy()
+        q = q_dev.cpu().numpy()
+    else:
+        q = np.ascontiguousarray(np.asarray(points)[sel])
+        nn, _ = ctx.knn_search(q, points, 1)
+        nn = np.asarray(nn)[:, 0]
+        nu_h = np.asarray(nu, dtype=np.float64)
+    nu_c = np.bincount(nn.astype(np.int64), weights=nu_h, minlength=m)
+    w_c, st = ctx.ot_solve(q, nu_c, box=box, tol=coarse_tol, inexact=True)
+    r_l = ctx.cells(q, weights=w_c, box=box, neighbors=False)
+    r_v = ctx.cells(q, box=box, neighbors=False)
+    v_l, v_v = np.asarray(r_l.vol), np.asarray(r_v.vol)
+    if not (v_l > 0).all():
+        raise RuntimeError("coarse_start: the coarse plan has an empty cell")
+    c = np.asarray(r_l.mg) / v_l[:, None]
+    b = np.asarray(r_v.mg) / v_v[:, None]
+    mu = np.cbrt(v_l / v_v)
+    r_vor = np.cbrt(3.0 * v_v / (4.0 * np.pi))
+    r_lag = np.cbrt(3.0 * v_l / (4.0 * np.pi))
+    s = s_factor * float(np.median(4.0 * r_vor * r_lag))
+    dwall = np.minimum(c - lo, hi - c).min(axis=1)
+    R = np.minimum(r_vor, 0.98 * dwall / mu)
+    g = 2.0 * c
+    r2 = ((q - b) ** 2).sum(axis=1)
+    hq = np.where(r2 <= R * R, r2, R * (2.0 * np.sqrt(r2) - R))
+    a = (q * q).sum(axis=1) - w_c - (g * q).sum(axis=1) - mu * hq
+    pieces = np.ascontiguousarray(np.column_stack([a, g, mu, b, R]))
+    if dev:
+        import torch
+        pieces = torch.from_numpy(pieces).to(points.device)
+    w = ctx.ot_convex_start(points, pieces, s)
+    w = w - w.mean()
+    info = {"coarse_seeds": m, "coarse_newton": int(st["newton_iters"]),
+            "coarse_cell_passes": int(st["cell_passes"]), "coarse_converged": bool(st["converged"]),
+            "s": s}
+    return w, info
+
+
 def _is_torch_cuda(a):
     try:
         import torch

@@ -360,6 +360,16 @@ int vgc_ot_features(vgc_ctx* ctx, const double* xyz, const double* nu, uint32_t 
                     const double* box, uint32_t flags, float* feats);
 int vgc_mlp_forward(vgc_ctx* ctx, const float* params, const float* feats, uint32_t n,
                     uint32_t flags, float* out);
+/* Newton start from a coarse transport plan (geogram's counterpart is the
+   multilevel start of OptimalTransportMap3d, compute_OTM/main.cpp:293-294):
+   w_i = |p_i|^2 - s log sum_j exp(piece_j(p_i) / s) with
+   piece_j(p) = a_j + g_j.p + mu_j H(|p - b_j|^2), H(r2) = r2 up to R_j^2 and
+   R_j (2 sqrt(r2) - R_j) beyond; VGC_START_PIECE doubles per piece
+   (a, g[3], mu, b[3], R), m pieces, s > 0.  The function is convex, so no
+   cell of the start is empty.  flags: VGC_DEVICE_PTRS. */
+#define VGC_START_PIECE 9
+int vgc_ot_convex_start(vgc_ctx* ctx, const double* xyz, uint32_t n, const double* pieces,
+                        uint32_t m, double s, uint32_t flags, double* w);
 /* gradient of the mean squared error over the n rows; loss = that mean */
 int vgc_mlp_grad(vgc_ctx* ctx, const float* params, const float* feats, const float* target,
                  uint32_t n, uint32_t flags, float* grad, double* loss);

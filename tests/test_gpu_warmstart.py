@@ -116,3 +116,54 @@ def test_learned_start_against_plain(gpu_ctx):
     if sl["used_fallback_start"]:
         assert sl["newton_iters"] == sp["newton_iters"]
         assert sl["cell_passes"] <= sp["cell_passes"] + 2
+
+
+def test_gpu_convex_start_matches_emulation(gpu_ctx, emu_ctx):
+    """The convex-start kernel on the GPU against its CPU emulation (the
+    emulation is checked against the definition in test_warmstart.py); exp
+    and log come from different math libraries, hence a tolerance."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    x = rng.random((50_000, 3))
+    m = 3000
+    pc = np.column_stack([rng.normal(size=m), rng.normal(size=(m, 3)), rng.random(m) * 2,
+                          rng.random((m, 3)), rng.random(m) * 0.3])
+    for s in (1e-3, 0.05):
+        we = emu_ctx.ot_convex_start(x, pc, s)
+        wd = gpu_ctx.ot_convex_start(torch.from_numpy(x).cuda(), torch.from_numpy(pc).cuda(), s)
+        wh = gpu_ctx.ot_convex_start(x, pc, s)
+        assert np.array_equal(wd.cpu().numpy(), wh)
+        assert np.abs(wh - we).max() <= 1e-13 * max(1.0, np.abs(we).max())
+
+
+def test_coarse_start_against_plain(gpu_ctx):
+    """The coarse-plan start on a held-out 200k-seed 16-Gaussian instance
+    (log-normal masses) solved to 1e-4: no empty cell, kept by the judge, at
+    least 30 % fewer Newton steps than the zero start and less time in all
+    (the coarse solve and the start included)."""
+    import time
+    import torch
+
+    x, nu = warmstart.make_instance(4242, 200_000)
+    xd, nud = torch.from_numpy(x).cuda(), torch.from_numpy(nu).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, sp = gpu_ctx.ot_solve(xd, nud, tol=1e-4, inexact=True)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    w0, info = warmstart.coarse_start(gpu_ctx, xd, nud)
+    vol = gpu_ctx.cells(xd, weights=w0, neighbors=False).vol
+    assert bool((vol > 0).all())
+    w, sc = gpu_ctx.ot_solve(xd, nud, w0=w0, tol=1e-4, inexact=True, judge_start=True)
+    torch.cuda.synchronize()
+    tc = time.perf_counter() - t0
+    print(f"plain: {sp['newton_iters']} Newton, {tp:.2f} s; coarse start ({info}): "
+          f"{sc['newton_iters']} Newton, {tc:.2f} s")
+    assert sp["converged"] == 1 and sc["converged"] == 1
+    assert sc["used_fallback_start"] == 0
+    assert sc["newton_iters"] <= 0.7 * sp["newton_iters"]
+    assert tc < tp
+    m = gpu_ctx.cells(xd, weights=w, neighbors=False).vol
+    assert float(((m - nud).abs() / nud).max()) <= 1e-4

@@ -154,3 +154,51 @@ def test_judged_start_falls_back_to_zero(emu_ctx):
     # the converged weights as start: kept, and nothing left to do
     w2, s2 = emu_ctx.ot_solve(x, nu, w0=w0, tol=1e-5, judge_start=True)
     assert s2["used_fallback_start"] == 0 and s2["newton_iters"] == 0
+
+
+# ------------------------------------------------ convex start (coarse plan)
+def _lse_reference(x, pc, s):
+    """w = |p|^2 - s log sum_j exp(piece_j(p) / s), straight from the
+    definition (all terms, FP64)."""
+    d = x[:, None, :] - pc[None, :, 5:8]
+    r2 = (d ** 2).sum(2)
+    R = pc[None, :, 8]
+    h = np.where(r2 <= R * R, r2, R * (2.0 * np.sqrt(r2) - R))
+    v = pc[None, :, 0] + x @ pc[:, 1:4].T + pc[None, :, 4] * h
+    mx = v.max(1)
+    return (x * x).sum(1) - (mx + s * np.log(np.exp((v - mx[:, None]) / s).sum(1)))
+
+
+@pytest.mark.parametrize("s", [1e-4, 0.05, 3.0])
+def test_convex_start_kernel_matches_definition(emu_ctx, s):
+    rng = np.random.default_rng(3)
+    x = rng.random((700, 3))
+    m = 150
+    pc = np.column_stack([rng.normal(size=m), rng.normal(size=(m, 3)), rng.random(m) * 2,
+                          rng.random((m, 3)), rng.random(m) * 0.3])
+    w = emu_ctx.ot_convex_start(x, pc, s)
+    ref = _lse_reference(x, pc, s)
+    assert np.abs(w - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    # one piece: the soft maximum is that piece
+    w1 = emu_ctx.ot_convex_start(x, pc[:1], s)
+    assert np.allclose(w1, _lse_reference(x, pc[:1], s), rtol=0, atol=1e-14)
+    with pytest.raises(ga.VgcError):
+        emu_ctx.ot_convex_start(x, pc, 0.0)
+
+
+def test_coarse_start_no_empty_cell_and_fewer_newton_steps(emu_ctx):
+    """On a 16-Gaussian instance with log-normal masses (the family the zero
+    start handles worst) the coarse-plan start leaves no cell empty, the
+    judge keeps it, and Newton needs at most 70 % of the zero start's steps."""
+    x, nu = warmstart.make_instance(4, 6000)
+    _, sp = emu_ctx.ot_solve(x, nu, tol=1e-4, inexact=True)
+    w0, info = warmstart.coarse_start(emu_ctx, x, nu, ratio=16)
+    assert info["coarse_seeds"] == 375
+    vol = np.asarray(emu_ctx.cells(x, weights=w0, neighbors=False).vol)
+    assert (vol > 0).all()
+    w, sl = emu_ctx.ot_solve(x, nu, w0=w0, tol=1e-4, inexact=True, judge_start=True)
+    assert sp["converged"] == 1 and sl["converged"] == 1
+    assert sl["used_fallback_start"] == 0
+    assert sl["newton_iters"] <= 0.7 * sp["newton_iters"]
+    m = np.asarray(emu_ctx.cells(x, weights=w, neighbors=False).vol)
+    assert np.max(np.abs(m - nu) / nu) <= 1e-4
