@@ -301,7 +301,7 @@ def main():
         ws["coarse"] = {"time_s": round(tc, 3), "newton_iters": int(sc["newton_iters"]),
                         "cell_passes": int(sc["cell_passes"]), "converged": bool(sc["converged"]),
                         "err0": sc["err0"], "fell_back_to_zero": bool(sc["used_fallback_start"]),
-                        **cinfo}
+                        **{k: v for k, v in cinfo.items() if k != "mu_fine"}}
         ws["coarse_newton_cut"] = round(1.0 - sc["newton_iters"] / max(1, sp["newton_iters"]), 3)
         ws["coarse_time_cut"] = round(1.0 - tc / tp, 3)
 

@@ -155,6 +155,7 @@ def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.
     w = ctx.ot_convex_start(points, pieces, s)
     w = w - w.mean()
     info = {"coarse_seeds": m, "coarse_newton": int(st["newton_iters"]),
+            "mu_fine": mu[nn.astype(np.int64)],
             "coarse_cell_passes": int(st["cell_passes"]), "coarse_converged": bool(st["converged"]),
             "s": s}
     return w, info
