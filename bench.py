@@ -38,7 +38,8 @@ The JSON line also carries:
                 newton_cut, time_cut) and against the coarse-plan start
                 (every 128th seed solved to 1e-3, extended by a convex soft
                 maximum; coarse solve and start timed; coarse_newton_cut,
-                coarse_time_cut);
+                coarse_time_cut), and optimize_levels with each level started
+                from the convex extension of the previous one (levels_convex);
   batched       256 independent instances (N log-uniform in [2k, 200k]):
                 20 batched Lloyd steps and batched transport to 1e-4.
 """
@@ -304,6 +305,23 @@ def main():
                         **{k: v for k, v in cinfo.items() if k != "mu_fine"}}
         ws["coarse_newton_cut"] = round(1.0 - sc["newton_iters"] / max(1, sp["newton_iters"]), 3)
         ws["coarse_time_cut"] = round(1.0 - tc / tp, 3)
+        # optimize_levels over BRIO-style levels (each 1/8 of the next, the
+        # coarsest at least 8000 seeds), each level from the convex extension
+        # of the previous plan (VGC_OT_CONVEX_LEVELS); the seeds permuted
+        # into level order before the timed region
+        perm, lv = ga.brio_levels(m, min_level=8000, seed=3)
+        pt = torch.from_numpy(np.asarray(perm, dtype=np.int64)).cuda()
+        x4p, nu4p = x4d[pt].contiguous(), nu4d[pt].contiguous()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        _, sv = ctx.ot_solve_levels(x4p, nu4p, lv, tol=1e-4, inexact=True, convex=True)
+        torch.cuda.synchronize()
+        tv = time.perf_counter() - t2
+        ws["levels_convex"] = {"time_s": round(tv, 3), "levels": [int(v) for v in lv],
+                               "newton_iters": [int(st["newton_iters"]) for st in sv],
+                               "cell_passes": [int(st["cell_passes"]) for st in sv],
+                               "converged": all(bool(st["converged"]) for st in sv),
+                               "fell_back_to_zero": [bool(st["used_fallback_start"]) for st in sv]}
 
     # batched uneven instances: Lloyd (20 steps) and transport to 1e-4 for
     # every instance at once; even instances uniform, odd ones 16-Gaussian;

@@ -357,9 +357,11 @@ class Context:
         return w, st.as_dict()
 
     def ot_solve_levels(self, points, nu, levels, box=UNIT_BOX, tol=1e-4, max_newton=100,
-                        verbose=False, inexact=False, repair=False):
+                        verbose=False, inexact=False, repair=False, convex=False):
         """Multilevel transport (optimize_levels): points in level order,
-        levels = [0, ..., n] (see brio_levels).  Returns (w, per-level stats)."""
+        levels = [0, ..., n] (see brio_levels).  convex: start each level from
+        the convex extension of the previous plan (VGC_OT_CONVEX_LEVELS)
+        instead of copied weights.  Returns (w, per-level stats)."""
         dev = _is_torch_cuda(points)
         self._stream(dev)
         if dev:
@@ -382,7 +384,8 @@ class Context:
                                             (VGC_DEVICE_PTRS if dev else 0)
                                             | (_lib.VGC_OT_VERBOSE if verbose else 0)
                                             | (_lib.VGC_OT_INEXACT if inexact else 0)
-                                            | (_lib.VGC_OT_REPAIR if repair else 0), sts)
+                                            | (_lib.VGC_OT_REPAIR if repair else 0)
+                                            | (_lib.VGC_OT_CONVEX_LEVELS if convex else 0), sts)
         self._check(rc)
         return w, [sts[k].as_dict() for k in range(nl)]
 
@@ -852,10 +855,12 @@ class OptimalTransportMap3d:
                                                      self.epsilon, max_iter, judge_start=judge)
         return self.weights
 
-    def optimize_levels(self, levels, max_iter: int = 100):
+    def optimize_levels(self, levels, max_iter: int = 100, convex: bool = False):
         """optimize_levels (compute_OTM/main.cpp:294): the points set with
-        set_points are in level order, levels = [0, ..., n]."""
+        set_points are in level order, levels = [0, ..., n].  convex: each
+        level starts from the convex extension of the previous plan."""
         ctx = self.ctx or default_context()
         self.weights, self.last_stats = ctx.ot_solve_levels(self.points_, self.nu, levels,
-                                                            self.box, self.epsilon, max_iter)
+                                                            self.box, self.epsilon, max_iter,
+                                                            convex=convex)
         return self.weights

@@ -31,6 +31,7 @@ VGC_OT_INEXACT = 64
 VGC_WAVES4 = 128
 VGC_PERIODIC = 256
 VGC_OT_REPAIR = 512
+VGC_OT_CONVEX_LEVELS = 2048
 VGC_FP64_FIRST = 1024
 
 VGC_MLP_PARAMS = 4673

@@ -1146,6 +1146,128 @@ inline int sample_points(uint32_t kind, uint64_t seed, uint32_t n, double* xyz) 
 // rescaled to the box volume, starting from the previous level's weights and,
 // for the new seeds, the weight of their power-nearest seed of the previous
 // level.  stats: one entry per level (may be null).
+// The start of a level from the previous level's plan (m0 seeds, weights
+// dw[0, m0)): one convex piece per previous seed j, the tangent of the
+// potential phi = |q|^2 - w at q_j with slope 2 c_j (c_j the centroid of its
+// Laguerre cell) plus mu_j H(|p - b_j|^2), mu_j = (Laguerre volume / Voronoi
+// volume)^(1/3), b_j the centroid of its Voronoi cell among the m0 seeds, H
+// quadratic up to that cell's radius (capped so that c_j + mu_j (p - b_j)
+// stays in the box) and linear beyond; the weights of all m seeds come from
+// the soft maximum of the pieces (ot_convex_start_row), s the median of
+// 4 x (Voronoi radius) x (Laguerre radius).  A convex function of the
+// seeds: no cell of the start is empty.
+template <class B>
+int convex_level_start(Ctx<B>* ctx, const double* dx, double* dw, uint32_t m0, uint32_t m,
+                       const double* box) {
+    B& be = ctx->be;
+    Engine<B>& E = ctx->eng;
+    Problem pc;
+    pc.n = (int)m0;
+    pc.n_inst = 1;
+    pc.inst_n = {(int)m0};
+    pc.box.assign(box, box + 6);
+    pc.d_xyz = dx;
+    pc.want_neighbors = false;
+    EngineStats es;
+    std::vector<double> vl(m0), ml(3 * (size_t)m0), vv(m0), mv(3 * (size_t)m0), q(3 * (size_t)m0),
+        w(m0);
+    pc.d_w = dw;
+    if (E.cells(pc, es)) return 1;
+    be.d2h(vl.data(), E.d_vol, sizeof(double) * m0);
+    be.d2h(ml.data(), E.d_mg, sizeof(double) * 3 * (size_t)m0);
+    pc.d_w = nullptr;
+    if (E.cells(pc, es)) return 1;
+    be.d2h(vv.data(), E.d_vol, sizeof(double) * m0);
+    be.d2h(mv.data(), E.d_mg, sizeof(double) * 3 * (size_t)m0);
+    be.d2h(q.data(), dx, sizeof(double) * 3 * (size_t)m0);
+    be.d2h(w.data(), dw, sizeof(double) * m0);
+    be.sync();
+    const double kPi = 3.14159265358979323846;
+    std::vector<double> pieces((size_t)kStartPiece * m0), prod(m0);
+    for (uint32_t j = 0; j < m0; ++j) {
+        if (!(vl[j] > 0.0) || !(vv[j] > 0.0)) {
+            E.error = "convex level start: an empty cell in the previous level's plan";
+            return 1;
+        }
+        prod[j] = 4.0 * std::cbrt(3.0 * vv[j] / (4.0 * kPi)) * std::cbrt(3.0 * vl[j] / (4.0 * kPi));
+    }
+    std::vector<double> srt(prod);
+    std::sort(srt.begin(), srt.end());
+    const double s =
+        (m0 % 2) ? srt[m0 / 2] : 0.5 * (srt[m0 / 2 - 1] + srt[m0 / 2]);
+    for (uint32_t j = 0; j < m0; ++j) {
+        double c[3], b[3], dwall = 1e300, r2 = 0.0, gq = 0.0, qq = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            c[a] = ml[3 * (size_t)j + a] / vl[j];
+            b[a] = mv[3 * (size_t)j + a] / vv[j];
+            dwall = std::min(dwall, std::min(c[a] - box[a], box[3 + a] - c[a]));
+            const double qa = q[3 * (size_t)j + a];
+            r2 += (qa - b[a]) * (qa - b[a]);
+            gq += 2.0 * c[a] * qa;
+            qq += qa * qa;
+        }
+        const double mu = std::cbrt(vl[j] / vv[j]);
+        const double R = std::min(std::cbrt(3.0 * vv[j] / (4.0 * kPi)), 0.98 * dwall / mu);
+        const double h = r2 <= R * R ? r2 : R * (2.0 * std::sqrt(r2) - R);
+        double* pj = pieces.data() + (size_t)kStartPiece * j;
+        pj[0] = qq - w[j] - gq - mu * h;
+        for (int a = 0; a < 3; ++a) {
+            pj[1 + a] = 2.0 * c[a];
+            pj[5 + a] = b[a];
+        }
+        pj[4] = mu;
+        pj[8] = R;
+    }
+    double* dp = (double*)ctx->staging(8, sizeof(double) * pieces.size());
+    be.h2d(dp, pieces.data(), sizeof(double) * pieces.size());
+    be.ot_convex_start((int)m, dx, dp, (int)m0, s, dw);
+    return 0;
+}
+
+// Masses of a coarse level with VGC_OT_CONVEX_LEVELS: each of the first m
+// seeds carries the masses of all seeds whose nearest seed among the first m
+// it is (exact FP64 nearest, ties by index), summed on the host in index
+// order.  The coarse plan then transports the fine plan's own measure, so its
+// convex extension starts the finer level close to its masses.
+template <class B>
+int nearest_masses(Ctx<B>* ctx, const double* dx, const std::vector<double>& hnu, uint32_t n,
+                   uint32_t m, std::vector<double>& out) {
+    B& be = ctx->be;
+    Engine<B>& E = ctx->eng;
+    double lo[3], hi[3], S;
+    host_extent(be, dx, n, true, lo, hi, S);
+    if (!(S < 1e300)) {
+        E.error = "coordinates must be finite";
+        return 1;
+    }
+    Problem pb;
+    pb.n = (int)m;
+    pb.n_inst = 1;
+    pb.inst_n = {(int)m};
+    pb.box = {lo[0], lo[1], lo[2], hi[0] + 1e-300 + (hi[0] - lo[0]) * 1e-12,
+              hi[1] + 1e-300 + (hi[1] - lo[1]) * 1e-12, hi[2] + 1e-300 + (hi[2] - lo[2]) * 1e-12};
+    for (int c = 0; c < 3; ++c)
+        if (!(pb.box[3 + c] > pb.box[c])) pb.box[3 + c] = pb.box[c] + 1.0;
+    pb.d_xyz = dx;
+    EngineStats es;
+    if (E.build(pb, es)) return 1;
+    int* d_idx = (int*)ctx->staging(9, sizeof(int) * (size_t)n);
+    double* d_d2 = (double*)ctx->staging(10, sizeof(double) * (size_t)n);
+    if (E.knn_exact(dx, (int)n, 1, false, S, d_idx, d_d2, es)) return 1;
+    std::vector<int> idx(n);
+    be.d2h(idx.data(), d_idx, sizeof(int) * (size_t)n);
+    be.sync();
+    out.assign(m, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (idx[i] < 0 || idx[i] >= (int)m) {
+            E.error = "nearest coarse seed missing";
+            return 1;
+        }
+        out[idx[i]] += hnu[i];
+    }
+    return 0;
+}
+
 template <class B>
 int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                         const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
@@ -1207,15 +1329,28 @@ int api_ot_solve_levels(Ctx<B>* ctx, const double* xyz, const double* nu, double
         for (uint32_t k = 0; k < n_levels; ++k) {
             const uint32_t m = levels[k + 1], m0 = levels[k];
             // this level's masses: the first m, rescaled to the box volume
-            double sm = 0.0;
-            for (uint32_t i = 0; i < m; ++i) sm += hnu[i];
-            hl.resize(m);
-            for (uint32_t i = 0; i < m; ++i) hl[i] = hnu[i] * (V / sm);
+            // (VGC_OT_CONVEX_LEVELS: the masses of their nearest seeds)
+            if ((flags & VGC_OT_CONVEX_LEVELS) && k + 1 < n_levels) {
+                if (nearest_masses(ctx, dx, hnu, n, m, hl)) {
+                    ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
+                    return VGC_ERR_INTERNAL;
+                }
+            } else {
+                double sm = 0.0;
+                for (uint32_t i = 0; i < m; ++i) sm += hnu[i];
+                hl.resize(m);
+                for (uint32_t i = 0; i < m; ++i) hl[i] = hnu[i] * (V / sm);
+            }
             be.h2d(dnl, hl.data(), sizeof(double) * (size_t)m);
             // start: zero on the first level; then the previous solution and,
             // for the new seeds, their power-nearest previous seed's weight
             if (k == 0) {
                 be.dzero(dw, sizeof(double) * (size_t)m);
+            } else if (flags & VGC_OT_CONVEX_LEVELS) {
+                if (convex_level_start(ctx, dx, dw, m0, m, box)) {
+                    ctx->err = "vgc_ot_solve_levels: " + ctx->eng.error;
+                    return VGC_ERR_INTERNAL;
+                }
             } else {
                 Problem pc;
                 pc.n = (int)m0;

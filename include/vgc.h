@@ -81,6 +81,13 @@ extern "C" {
                                  certified against the FP64 planes, and the
                                  cells it cannot certify redone in FP64
                                  (vgc_stats.nb_fp64_redo); identical results */
+#define VGC_OT_CONVEX_LEVELS 2048u /* vgc_ot_solve_levels: each coarse level's
+                                 seeds carry the masses of the seeds nearest
+                                 to them (not their own, rescaled), and each
+                                 level starts from the convex extension of the
+                                 previous level's plan (vgc_ot_convex_start's
+                                 soft maximum, no empty cell) instead of
+                                 copied weights */
 
 typedef struct vgc_ctx vgc_ctx;
 
@@ -342,8 +349,10 @@ int vgc_ot_solve_batch(vgc_ctx* ctx, const double* xyz, const double* nu, double
    levels[n_levels] = n).  Level k solves the transport of the first
    levels[k+1] seeds (their masses rescaled to the box volume) from the
    previous level's weights, each new seed starting with the weight of its
-   power-nearest seed of the previous level.  stats: n_levels entries (may be
-   null).  Other arguments as vgc_ot_solve; w is output only. */
+   power-nearest seed of the previous level (VGC_OT_CONVEX_LEVELS: every seed
+   of the level from the convex extension of the previous plan).  stats:
+   n_levels entries (may be null).  Other arguments as vgc_ot_solve; w is
+   output only. */
 int vgc_ot_solve_levels(vgc_ctx* ctx, const double* xyz, const double* nu, double* w, uint32_t n,
                         const double* box, const uint32_t* levels, uint32_t n_levels, double tol,
                         uint32_t max_newton, uint32_t flags, vgc_ot_stats* stats);

@@ -185,6 +185,28 @@ def test_ot_solve_levels(emu_ctx):
             emu_ctx.ot_solve_levels(x, nu, np.array(bad), tol=1e-5)
 
 
+def test_ot_solve_levels_convex_start(emu_ctx):
+    """optimize_levels with VGC_OT_CONVEX_LEVELS on a 16-Gaussian instance
+    with log-normal masses, where copied weights leave cells empty: the
+    finer level starts from the convex extension of the coarser plan, keeps
+    it (no fallback), needs at most 70 % of a zero-start solve's Newton steps,
+    and its masses, re-measured by the oracle, meet the tolerance."""
+    from geogram_amd import warmstart
+
+    n = 6000
+    x, nu = warmstart.make_instance(4, n)
+    perm, lv = ga.brio_levels(n, min_level=700, seed=3)
+    x, nu = np.ascontiguousarray(x[perm]), np.ascontiguousarray(nu[perm])
+    assert lv.tolist() == [0, 750, 6000]
+    _, sp = emu_ctx.ot_solve(x, nu, tol=1e-4, inexact=True)
+    w, sts = emu_ctx.ot_solve_levels(x, nu, lv, tol=1e-4, inexact=True, convex=True)
+    assert all(st["converged"] == 1 for st in sts)
+    assert sts[1]["used_fallback_start"] == 0
+    assert sts[1]["newton_iters"] <= 0.7 * sp["newton_iters"]
+    o = oracle.cells(x, w=w, centred=True, stride=256)
+    assert np.max(np.abs(o["vol"] - nu) / nu) <= 1.01e-4
+
+
 def test_repaired_start_keeps_a_start_with_a_few_empty_cells(emu_ctx):
     # a converged solution with one weight lowered until its cell empties:
     # the repair raises that weight back (one round), the solve keeps the
