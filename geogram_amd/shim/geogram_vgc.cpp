@@ -291,6 +291,10 @@ public:
     }
     void set_nu(index_t i, double nu) { nu_[i] = nu; }
     void set_epsilon(double eps) { eps_ = eps; }
+    // optimize_levels: start each level from the convex extension of the
+    // previous one, coarse levels carrying their nearest seeds' masses
+    // (VGC_OT_CONVEX_LEVELS) instead of copied weights
+    void set_convex_levels(bool x) { convex_levels_ = x; }
     void optimize(index_t max_iter) {
         vgc_check(vgc_ot_solve(vgc_context(), points_.data(), nu_.data(), weights_.data(),
                                index_t(nu_.size()), box_, eps_, max_iter, 0, &stats_));
@@ -300,7 +304,8 @@ public:
         std::vector<vgc_ot_stats> st(lv.size() > 1 ? lv.size() - 1 : 1);
         vgc_check(vgc_ot_solve_levels(vgc_context(), points_.data(), nu_.data(), weights_.data(),
                                       index_t(nu_.size()), box_, lv.data(),
-                                      index_t(lv.size() - 1), eps_, max_iter, 0, st.data()));
+                                      index_t(lv.size() - 1), eps_, max_iter,
+                                      convex_levels_ ? VGC_OT_CONVEX_LEVELS : 0u, st.data()));
         stats_ = st.back();
     }
     double weight(index_t i) const { return weights_[i]; }
@@ -309,6 +314,7 @@ public:
 private:
     double box_[6];
     double eps_ = 1e-3;
+    bool convex_levels_ = false;
     std::vector<double> points_, nu_, weights_;
     vgc_ot_stats stats_{};
 };

@@ -229,6 +229,19 @@ public:
                                 index_t(nu_.size()), box_, eps_, max_iter,
                                 judge_ ? VGC_OT_JUDGE_START : 0u, &stats_));
     }
+    // optimize_levels (compute_OTM/main.cpp:294): the seeds set with
+    // set_points are in level order, levels = {0, ..., n}; convex: coarse
+    // levels carry their nearest seeds' masses and each level starts from the
+    // convex extension of the previous one (VGC_OT_CONVEX_LEVELS)
+    void optimize_levels(const std::vector<uint32_t>& levels, index_t max_iter,
+                         bool convex = false) {
+        std::vector<vgc_ot_stats> st(levels.size() > 1 ? levels.size() - 1 : 1);
+        ctx_.check(vgc_ot_solve_levels(ctx_.get(), points_.data(), nu_.data(), w_.data(),
+                                       index_t(nu_.size()), box_, levels.data(),
+                                       index_t(levels.size() - 1), eps_, max_iter,
+                                       convex ? VGC_OT_CONVEX_LEVELS : 0u, st.data()));
+        stats_ = st.back();
+    }
     const std::vector<double>& weights() const { return w_; }
     const vgc_ot_stats& stats() const { return stats_; }
 

@@ -56,6 +56,13 @@ int main() {
     otm2.set_learned_initial_weights(params.data());
     otm2.optimize(50);
     if (!otm2.stats().converged || otm2.stats().newton_iters != otm.stats().newton_iters) return 4;
+    // optimize_levels with convex level starts: two levels (the seeds are in
+    // a random order already), each converges
+    OptimalTransportMap3d otm3(*ctx, box);
+    otm3.set_points(n, cvt.embedding(0));
+    otm3.set_epsilon(1e-4);
+    otm3.optimize_levels({0u, uint32_t(n / 8), uint32_t(n)}, 50, true);
+    if (!otm3.stats().converged) return 7;
     std::printf("ok: sum m = %.17g, OT newton %u\n", s, otm.stats().newton_iters);
     delete ctx;
     return 0;
