@@ -364,6 +364,23 @@ def main():
                         "newton_max": int(max(its)), "newton_mean": round(float(np.mean(its)), 2),
                         "cell_passes": int(s5[0]["cell_passes"]),
                         "t_cells_ms": round(s5[0]["t_cells_ms"], 1), "t_cg_ms": round(s5[0]["t_cg_ms"], 1)}
+            # the same batch from each instance's coarse-plan start (every
+            # 128th seed, at least 64, solved together to 1e-3; timed)
+            from geogram_amd import warmstart
+
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            w05, ci5 = warmstart.coarse_start_batch(ctx, x5, nu5, sizes, boxes5)
+            _, s5c = ctx.ot_solve_batch(x5, nu5, sizes, boxes5, w0=w05, tol=1e-4, verbose=True,
+                                        inexact=True, autotune=True)
+            torch.cuda.synchronize()
+            tc5 = time.perf_counter() - t2
+            itc = [s["newton_iters"] for s in s5c]
+            c5["ot_coarse_start"] = {
+                "time_s": round(tc5, 3), "converged": int(sum(s["converged"] for s in s5c)),
+                "newton_max": int(max(itc)), "newton_mean": round(float(np.mean(itc)), 2),
+                "cell_passes": int(s5c[0]["cell_passes"]),
+                "restarted_from_zero": int(sum(s["used_fallback_start"] for s in s5c)), **ci5}
 
     # the rows beside the cell pass, each timed on the GPU with its error
     # against the oracle (or the reference build) on the same input

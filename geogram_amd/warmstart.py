@@ -94,6 +94,27 @@ def predict_weights(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), params=None):
     return ctx.mlp_forward(np.asarray(params, dtype=np.float32), feats).astype(np.float64)
 
 
+def _pieces(q, w_c, v_l, mg_l, v_v, mg_v, lo, hi, s_factor):
+    """Convex pieces (m x 9: a, g, mu, b, R) and soft-max scale s of a coarse
+    plan: seeds q, weights w_c, Laguerre and Voronoi volumes / first moments
+    of the coarse cells in the box [lo, hi]."""
+    if not (v_l > 0).all():
+        raise RuntimeError("coarse_start: the coarse plan has an empty cell")
+    c = mg_l / v_l[:, None]
+    b = mg_v / v_v[:, None]
+    mu = np.cbrt(v_l / v_v)
+    r_vor = np.cbrt(3.0 * v_v / (4.0 * np.pi))
+    r_lag = np.cbrt(3.0 * v_l / (4.0 * np.pi))
+    s = s_factor * float(np.median(4.0 * r_vor * r_lag))
+    dwall = np.minimum(c - lo, hi - c).min(axis=1)
+    R = np.minimum(r_vor, 0.98 * dwall / mu)
+    g = 2.0 * c
+    r2 = ((q - b) ** 2).sum(axis=1)
+    hq = np.where(r2 <= R * R, r2, R * (2.0 * np.sqrt(r2) - R))
+    a = (q * q).sum(axis=1) - w_c - (g * q).sum(axis=1) - mu * hq
+    return np.ascontiguousarray(np.column_stack([a, g, mu, b, R])), s, mu
+
+
 def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.0,
                  coarse_tol=1e-3, min_coarse=64, seed=12345):
     """Newton start from a coarse transport plan, with no empty cell.
@@ -133,22 +154,8 @@ def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.
     w_c, st = ctx.ot_solve(q, nu_c, box=box, tol=coarse_tol, inexact=True)
     r_l = ctx.cells(q, weights=w_c, box=box, neighbors=False)
     r_v = ctx.cells(q, box=box, neighbors=False)
-    v_l, v_v = np.asarray(r_l.vol), np.asarray(r_v.vol)
-    if not (v_l > 0).all():
-        raise RuntimeError("coarse_start: the coarse plan has an empty cell")
-    c = np.asarray(r_l.mg) / v_l[:, None]
-    b = np.asarray(r_v.mg) / v_v[:, None]
-    mu = np.cbrt(v_l / v_v)
-    r_vor = np.cbrt(3.0 * v_v / (4.0 * np.pi))
-    r_lag = np.cbrt(3.0 * v_l / (4.0 * np.pi))
-    s = s_factor * float(np.median(4.0 * r_vor * r_lag))
-    dwall = np.minimum(c - lo, hi - c).min(axis=1)
-    R = np.minimum(r_vor, 0.98 * dwall / mu)
-    g = 2.0 * c
-    r2 = ((q - b) ** 2).sum(axis=1)
-    hq = np.where(r2 <= R * R, r2, R * (2.0 * np.sqrt(r2) - R))
-    a = (q * q).sum(axis=1) - w_c - (g * q).sum(axis=1) - mu * hq
-    pieces = np.ascontiguousarray(np.column_stack([a, g, mu, b, R]))
+    pieces, s, mu = _pieces(q, w_c, np.asarray(r_l.vol), np.asarray(r_l.mg), np.asarray(r_v.vol),
+                            np.asarray(r_v.mg), lo, hi, s_factor)
     if dev:
         import torch
         pieces = torch.from_numpy(pieces).to(points.device)
@@ -158,6 +165,67 @@ def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.
             "mu_fine": mu[nn.astype(np.int64)],
             "coarse_cell_passes": int(st["cell_passes"]), "coarse_converged": bool(st["converged"]),
             "s": s}
+    return w, info
+
+
+def coarse_start_batch(ctx, points, nu, inst_n, boxes, ratio=128, s_factor=1.0,
+                       coarse_tol=1e-3, min_coarse=64, seed=12345):
+    """coarse_start for independent instances (consecutive seed ranges, one
+    box each, as vgc_ot_solve_batch takes them): the coarse plans are solved
+    together by the batched solver, each instance's start is its own convex
+    soft maximum.  Returns (w, info)."""
+    dev = _is_torch_cuda(points)
+    inst_n = np.asarray(inst_n, dtype=np.int64)
+    boxes = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+    off = np.concatenate([[0], np.cumsum(inst_n)])
+    nu_h = nu.cpu().numpy() if dev else np.asarray(nu, dtype=np.float64)
+    qs, nus, sizes, nns = [], [], [], []
+    for k, nk in enumerate(inst_n):
+        m = max(min_coarse, int(nk) // ratio)
+        if m >= nk:
+            raise ValueError("coarse_start_batch: too few seeds for a coarse level")
+        sel = np.sort(np.random.default_rng(seed + k).choice(int(nk), m, replace=False))
+        xk = points[int(off[k]):int(off[k + 1])]
+        if dev:
+            import torch
+            qk_dev = xk[torch.from_numpy(sel).to(points.device)].contiguous()
+            nn, _ = ctx.knn_search(qk_dev, xk, 1)
+            nn = nn[:, 0].cpu().numpy()
+            qk = qk_dev.cpu().numpy()
+        else:
+            qk = np.ascontiguousarray(np.asarray(xk)[sel])
+            nn = np.asarray(ctx.knn_search(qk, xk, 1)[0])[:, 0]
+        qs.append(qk)
+        nus.append(np.bincount(nn.astype(np.int64), weights=nu_h[int(off[k]):int(off[k + 1])],
+                               minlength=m))
+        sizes.append(m)
+    q = np.ascontiguousarray(np.concatenate(qs))
+    nu_c = np.concatenate(nus)
+    w_c, sts = ctx.ot_solve_batch(q, nu_c, sizes, boxes, tol=coarse_tol, inexact=True)
+    r_l = ctx.cells(q, weights=w_c, inst_n=sizes, boxes=boxes, neighbors=False)
+    r_v = ctx.cells(q, inst_n=sizes, boxes=boxes, neighbors=False)
+    v_l, mg_l = np.asarray(r_l.vol), np.asarray(r_l.mg)
+    v_v, mg_v = np.asarray(r_v.vol), np.asarray(r_v.mg)
+    coff = np.concatenate([[0], np.cumsum(sizes)])
+    parts = []
+    for k in range(len(inst_n)):
+        a, b = int(coff[k]), int(coff[k + 1])
+        pieces, s, _ = _pieces(q[a:b], w_c[a:b], v_l[a:b], mg_l[a:b], v_v[a:b], mg_v[a:b],
+                               boxes[k, :3], boxes[k, 3:], s_factor)
+        xk = points[int(off[k]):int(off[k + 1])]
+        if dev:
+            import torch
+            pieces = torch.from_numpy(pieces).to(points.device)
+        wk = ctx.ot_convex_start(xk, pieces, s)
+        parts.append(wk - wk.mean())
+    if dev:
+        import torch
+        w = torch.cat(parts)
+    else:
+        w = np.concatenate(parts)
+    info = {"coarse_seeds": int(sum(sizes)),
+            "coarse_newton_max": int(max(st["newton_iters"] for st in sts)),
+            "coarse_converged": int(sum(bool(st["converged"]) for st in sts))}
     return w, info
 
 
