@@ -36,7 +36,7 @@ The JSON line also carries:
   ot_warm_start 1M 16-Gaussian seeds with log-normal masses: plain start
                 against the learned MLP start (features + forward timed;
                 newton_cut, time_cut) and against the coarse-plan start
-                (every 128th seed solved to 1e-3, extended by a convex soft
+                (every 256th seed solved to 1e-3, extended by a convex soft
                 maximum; coarse solve and start timed; coarse_newton_cut,
                 coarse_time_cut), and optimize_levels with each level started
                 from the convex extension of the previous one (levels_convex);
@@ -290,7 +290,7 @@ def main():
             ws["time_cut"] = round(1.0 - tl / tp, 3)
         else:
             ws["learned"] = None
-        # start from a coarse plan (every 128th seed solved to 1e-3, the fine
+        # start from a coarse plan (every 256th seed solved to 1e-3, the fine
         # start its convex soft-max extension; coarse solve and start timed)
         torch.cuda.synchronize()
         t2 = time.perf_counter()

@@ -832,7 +832,7 @@ class OptimalTransportMap3d:
         from . import warmstart
         self.learned_params = warmstart.load_params() if params is None else params
 
-    def use_coarse_warm_start(self, ratio=128):
+    def use_coarse_warm_start(self, ratio=256):
         """Start Newton from a coarse transport plan (warmstart.coarse_start:
         a convex start, no empty cell), judged against the zero start.
         ratio: fine seeds per coarse seed; None = off."""

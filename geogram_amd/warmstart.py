@@ -115,7 +115,7 @@ def _pieces(q, w_c, v_l, mg_l, v_v, mg_v, lo, hi, s_factor):
     return np.ascontiguousarray(np.column_stack([a, g, mu, b, R])), s, mu
 
 
-def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.0,
+def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=256, s_factor=1.0,
                  coarse_tol=1e-3, min_coarse=64, seed=12345):
     """Newton start from a coarse transport plan, with no empty cell.
 
@@ -129,7 +129,11 @@ def coarse_start(ctx, points, nu, box=(0, 0, 0, 1, 1, 1), ratio=128, s_factor=1.
     cell's radius and linear beyond.  The fine start is the soft maximum of
     all pieces (vgc_ot_convex_start), a convex function, so every fine cell
     is non-empty.  s: s_factor x the median over coarse seeds of
-    4 x (Voronoi radius) x (Laguerre radius).  Returns (w, info)."""
+    4 x (Voronoi radius) x (Laguerre radius).  Returns (w, info).  Default
+    ratio 256: measured on 1M 16-Gaussian seeds with log-normal masses, 10-12
+    Newton steps and 2.3-2.6 s to 1e-4 in three trials (s_factor 1, 1.5, 2;
+    two coarse subsets), against 10-13 steps and 2.2-4.4 s over three coarse
+    subsets at ratio 128."""
     dev = _is_torch_cuda(points)
     n = int(points.shape[0])
     lo = np.asarray(box[:3], dtype=np.float64)
@@ -173,7 +177,8 @@ def coarse_start_batch(ctx, points, nu, inst_n, boxes, ratio=128, s_factor=1.0,
     """coarse_start for independent instances (consecutive seed ranges, one
     box each, as vgc_ot_solve_batch takes them): the coarse plans are solved
     together by the batched solver, each instance's start is its own convex
-    soft maximum.  Returns (w, info)."""
+    soft maximum.  Returns (w, info).  Default ratio 128: measured on 256
+    instances of 2k to 200k seeds, 11.7 s to 1e-4 against 13.8 s at 256."""
     dev = _is_torch_cuda(points)
     inst_n = np.asarray(inst_n, dtype=np.int64)
     boxes = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)

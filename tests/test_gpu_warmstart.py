@@ -153,7 +153,7 @@ def test_coarse_start_against_plain(gpu_ctx):
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     t0 = time.perf_counter()
-    w0, info = warmstart.coarse_start(gpu_ctx, xd, nud)
+    w0, info = warmstart.coarse_start(gpu_ctx, xd, nud, ratio=128)
     vol = gpu_ctx.cells(xd, weights=w0, neighbors=False).vol
     assert bool((vol > 0).all())
     w, sc = gpu_ctx.ot_solve(xd, nud, w0=w0, tol=1e-4, inexact=True, judge_start=True)
