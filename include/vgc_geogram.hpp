@@ -10,6 +10,10 @@
 #ifndef VGC_GEOGRAM_HPP
 #define VGC_GEOGRAM_HPP
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -222,6 +226,77 @@ public:
                                    feats.data()));
         ctx_.check(vgc_mlp_forward(ctx_.get(), params, feats.data(), n, 0, y.data()));
         for (index_t i = 0; i < n; ++i) w_[i] = y[i];
+        judge_ = true;
+    }
+    // Start Newton from a coarse transport plan (no counterpart in geogram;
+    // the start of optimize_levels' finer levels is its analogue): every
+    // ratio-th seed of a fixed random subset carries the masses of the seeds
+    // nearest to it and is solved to 1e-3; vgc_ot_convex_start extends the
+    // plan to all seeds as a convex soft maximum (no empty cell).  The zero
+    // start stays the fallback (judged by the mass residual).
+    void set_coarse_initial_weights(index_t ratio = 256, uint64_t seed = 12345) {
+        const index_t n = index_t(nu_.size());
+        const index_t m = std::max<index_t>(64, n / ratio);
+        if (m >= n) return;  // too few seeds: the zero start
+        std::vector<index_t> idx(n);
+        for (index_t i = 0; i < n; ++i) idx[i] = i;
+        std::mt19937_64 rng(seed);
+        for (index_t i = 0; i < m; ++i) {  // partial Fisher-Yates
+            std::uniform_int_distribution<index_t> pick(i, n - 1);
+            std::swap(idx[i], idx[pick(rng)]);
+        }
+        std::sort(idx.begin(), idx.begin() + m);
+        std::vector<double> q(3 * (size_t)m), nuc(m, 0.0), wc(m, 0.0);
+        for (index_t j = 0; j < m; ++j)
+            for (int a = 0; a < 3; ++a) q[3 * (size_t)j + a] = points_[3 * (size_t)idx[j] + a];
+        std::vector<int32_t> nn(n);
+        std::vector<double> d2(n);
+        ctx_.check(vgc_knn_search(ctx_.get(), q.data(), m, points_.data(), n, 1, 0, nn.data(),
+                                  d2.data()));
+        for (index_t i = 0; i < n; ++i) nuc[nn[i]] += nu_[i];
+        vgc_ot_stats cst{};
+        ctx_.check(vgc_ot_solve(ctx_.get(), q.data(), nuc.data(), wc.data(), m, box_, 1e-3, 100,
+                                VGC_OT_INEXACT, &cst));
+        std::vector<double> vl(m), ml(3 * (size_t)m), vv(m), mv(3 * (size_t)m);
+        ctx_.check(vgc_cells(ctx_.get(), q.data(), wc.data(), m, box_, 0, VGC_NO_NEIGHBORS,
+                             vl.data(), ml.data(), nullptr, nullptr, nullptr, 0, nullptr));
+        ctx_.check(vgc_cells(ctx_.get(), q.data(), nullptr, m, box_, 0, VGC_NO_NEIGHBORS,
+                             vv.data(), mv.data(), nullptr, nullptr, nullptr, 0, nullptr));
+        const double kPi = 3.14159265358979323846;
+        std::vector<double> prod(m), pieces((size_t)VGC_START_PIECE * m);
+        for (index_t j = 0; j < m; ++j) {
+            if (!(vl[j] > 0.0)) return;  // an empty coarse cell: the zero start
+            prod[j] = 4.0 * std::cbrt(3.0 * vv[j] / (4.0 * kPi)) *
+                      std::cbrt(3.0 * vl[j] / (4.0 * kPi));
+        }
+        std::vector<double> srt(prod);
+        std::sort(srt.begin(), srt.end());
+        const double s = (m % 2) ? srt[m / 2] : 0.5 * (srt[m / 2 - 1] + srt[m / 2]);
+        for (index_t j = 0; j < m; ++j) {
+            double c[3], b[3], dwall = 1e300, r2 = 0.0, gq = 0.0, qq = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                c[a] = ml[3 * (size_t)j + a] / vl[j];
+                b[a] = mv[3 * (size_t)j + a] / vv[j];
+                dwall = std::min(dwall, std::min(c[a] - box_[a], box_[3 + a] - c[a]));
+                const double qa = q[3 * (size_t)j + a];
+                r2 += (qa - b[a]) * (qa - b[a]);
+                gq += 2.0 * c[a] * qa;
+                qq += qa * qa;
+            }
+            const double mu = std::cbrt(vl[j] / vv[j]);
+            const double R = std::min(std::cbrt(3.0 * vv[j] / (4.0 * kPi)), 0.98 * dwall / mu);
+            const double h = r2 <= R * R ? r2 : R * (2.0 * std::sqrt(r2) - R);
+            double* pj = pieces.data() + (size_t)VGC_START_PIECE * j;
+            pj[0] = qq - wc[j] - gq - mu * h;
+            for (int a = 0; a < 3; ++a) {
+                pj[1 + a] = 2.0 * c[a];
+                pj[5 + a] = b[a];
+            }
+            pj[4] = mu;
+            pj[8] = R;
+        }
+        ctx_.check(vgc_ot_convex_start(ctx_.get(), points_.data(), n, pieces.data(), m, s, 0,
+                                       w_.data()));
         judge_ = true;
     }
     void optimize(index_t max_iter) {

@@ -63,6 +63,15 @@ int main() {
     otm3.set_epsilon(1e-4);
     otm3.optimize_levels({0u, uint32_t(n / 8), uint32_t(n)}, 50, true);
     if (!otm3.stats().converged) return 7;
+    // coarse-plan start (every 16th seed), judged against the zero start
+    // (which a centroidal diagram with equal masses nearly solves already):
+    // converges in no more Newton steps than the zero start
+    OptimalTransportMap3d otm4(*ctx, box);
+    otm4.set_points(n, cvt.embedding(0));
+    otm4.set_epsilon(1e-4);
+    otm4.set_coarse_initial_weights(16);
+    otm4.optimize(50);
+    if (!otm4.stats().converged || otm4.stats().newton_iters > otm.stats().newton_iters) return 8;
     std::printf("ok: sum m = %.17g, OT newton %u\n", s, otm.stats().newton_iters);
     delete ctx;
     return 0;
